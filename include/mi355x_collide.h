@@ -1,0 +1,188 @@
+/* mi355x_collide.h — C ABI of the MI355X collision library.
+ *
+ * Drop-in for the CPU collision path of inMyRoom_vulkan:
+ *   entity transforms in -> colliding entity pairs with contact segments,
+ *   contact normals and vertex-outside flags out.
+ * Plain pointers and sizes only; every function returns an int status
+ * (CD_OK = 0, negative on error; cd_last_error() gives the message).  No
+ * C++ exception crosses this boundary.  One cd_ctx per host thread; a
+ * context is not internally synchronised.  All output buffers are owned by
+ * the caller; the library keeps no caller pointer after a call returns.
+ *
+ * Reference paths are relative to the inMyRoom_vulkan repository root.
+ */
+#ifndef MI355X_COLLIDE_H
+#define MI355X_COLLIDE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CD_OK 0
+#define CD_E_INVALID (-1)  /* bad argument (null pointer, unknown mesh, non-finite input, ...) */
+#define CD_E_CAPACITY (-2) /* output buffer too small; the required count is written back */
+#define CD_E_HIP (-3)      /* HIP runtime error */
+#define CD_E_STACK (-4)    /* a traversal stack would overflow (tree deeper than supported) */
+#define CD_E_NOMEM (-5)    /* device allocation failed */
+
+/* ---- modes --------------------------------------------------------------- */
+/* Plain: reference semantics bit for bit (Möller test with its 1e-6 snap,
+ * float SAT), one lane per query.  Batched: persistent load-balanced
+ * traversal, exact predicates (float filter, double fallback, exact
+ * expansion as last resort), conservative SAT.                              */
+#define CD_MODE_PLAIN 0u
+#define CD_MODE_BATCHED 1u
+
+/* ---- records --------------------------------------------------------------- */
+typedef struct cd_ctx cd_ctx;
+
+typedef struct cd_config {
+    uint32_t leaf_max;   /* triangles per OBB-tree leaf; 0 -> 4 (include/Geometry/OBBtree.h:49) */
+    uint32_t device;     /* HIP device ordinal */
+    uint32_t reserved[6];
+} cd_config;
+
+/* CollisionDetectionEntry (inMyRoom_vulkan/include/ECS/ECStypes.h:149-156):
+ * the OBBtree pointer becomes a registered mesh id.                         */
+typedef struct cd_entity {
+    float current[16];       /* currentGlobalMatrix, column-major (glm) */
+    float previous[16];      /* previousGlobalMatrix */
+    uint32_t mesh_id;        /* from cd_register_mesh */
+    uint16_t entity;         /* Entity (uint16_t, ECStypes.h:18) */
+    uint8_t should_callback; /* shouldCallback */
+    uint8_t pad;
+} cd_entity;
+
+/* One mesh-pair query: triangles of mesh_b are taken into mesh_a's space by
+ * rel (OBBtreesCollision.cpp:15 computes rel = inverse(A) * B).             */
+typedef struct cd_query {
+    uint32_t mesh_a, mesh_b;
+    float rel[16];
+} cd_query;
+
+/* One colliding triangle pair (intersecting and not coplanar,
+ * CreateUncollideRays.cpp:88), in the first entity's model space.  tri_a /
+ * tri_b are tree-order triangle slots (use cd_export_tree's permutation to
+ * map them to input triangle indices).  flags bit k: A's vertex k is
+ * outside B's plane; bit 3+k: B's vertex k is outside A's plane
+ * (CreateUncollideRays.cpp:102-112).  na / nb: unit face normals.          */
+typedef struct cd_contact {
+    uint32_t query, tri_a, tri_b, flags;
+    float s[3], t[3];
+    float na[3], nb[3];
+} cd_contact;
+
+/* An overlapping entity pair from the broad phase, oriented as the
+ * reference orients it (first = earlier on the sweep axis,
+ * SweepAndPrune.cpp:63), with its range in the contact array.               */
+typedef struct cd_pair {
+    uint32_t first, second;
+    uint32_t contact_begin, contact_count;
+} cd_pair;
+
+typedef struct cd_mesh_info {
+    uint32_t ntri, nnodes, depth, leaf_max;
+    float root[12];          /* root OBB: centre, u, v, w half-extent sides */
+    float slack;             /* max distance a vertex lies outside a box meant to hold it */
+    float radius;            /* max |vertex| */
+} cd_mesh_info;
+
+typedef struct cd_counters {
+    uint64_t sat_tests;       /* node-pair separating-axis tests */
+    uint64_t leaf_pairs;      /* candidate leaf pairs */
+    uint64_t tri_tests;       /* triangle-triangle tests */
+    uint64_t contacts;        /* colliding pairs written */
+    uint64_t fp64_fallbacks;  /* orientation predicates re-evaluated in double */
+    uint64_t exact_fallbacks; /* ... and then in exact expansion arithmetic */
+    uint64_t ub_cases;        /* Möller cases the reference leaves undefined */
+    uint64_t broad_pairs;     /* entity pairs out of the broad phase */
+} cd_counters;
+
+typedef struct cd_timings {   /* device time of the last run, from HIP events */
+    float total_ms;
+    float traverse_ms;        /* plain: traversal kernel; batched: persistent kernel */
+    float narrow_ms;          /* plain: narrow kernel */
+    float broad_ms;
+    uint32_t launches;        /* launches of the dominant kernel in the run */
+    uint32_t pad;
+} cd_timings;
+
+/* ---- context --------------------------------------------------------------- */
+/* Replaces CollisionDetection::CollisionDetection (src/CollisionDetection/CollisionDetection.cpp:5-26). */
+int cd_create(cd_ctx** out, const cd_config* cfg);
+int cd_destroy(cd_ctx* ctx);
+const char* cd_last_error(const cd_ctx* ctx);
+
+/* ---- meshes -------------------------------------------------------------------- */
+/* Builds the OBB tree of an indexed triangle list on the GPU.
+ * Replaces OBBtree::OBBtree (src/Geometry/OBBtree.cpp:321-358) fed by
+ * Triangle::CreateTriangleList in glTF "triangles" mode (src/Geometry/Triangle.cpp:259-280). */
+int cd_register_mesh(cd_ctx* ctx, const float* positions, uint32_t nvert, const uint32_t* indices,
+                     uint32_t ntri, uint32_t* mesh_id);
+/* Many triangle soups at once (9 floats per triangle, tri_counts[i] triangles
+ * for mesh i, concatenated); ids are first_id .. first_id + nmesh - 1.      */
+int cd_register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* tri_counts, uint32_t nmesh,
+                      uint32_t* first_id);
+int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out);
+/* Flattened tree as the reference stores it (include/Geometry/OBBtree.h:61-75,129-134):
+ * node_obbs: nnodes x 24 floats (left box, right box); node_links: nnodes x 4
+ * (left link, right link, left count, right count; count 0 = internal child);
+ * perm: ntri tree-order slot -> input triangle index.                     */
+int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, uint32_t* node_links,
+                   uint32_t* perm);
+
+/* ---- queries ------------------------------------------------------------------- */
+/* Triangle::IntersectTriangles (src/Geometry/Triangle.cpp:85-110) over n
+ * pairs.  flags: bit0 hit, bit1 coplanar, bit2 undefined-in-reference case
+ * (plain); bit0 hit, bit1 coplanar (batched = exact semantics).
+ * seg6 (may be NULL): source, target when hit and not coplanar.           */
+int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode,
+                    int32_t* flags, float* seg6);
+
+/* Mesh-pair queries: OBBtreesCollision::ExecuteOBBtreesCollision
+ * (src/CollisionDetection/OBBtreesCollision.cpp:7-18) followed by the
+ * triangle loop of CreateUncollideRays::ExecuteCreateUncollideRays
+ * (src/CollisionDetection/CreateUncollideRays.cpp:60-128).  Contacts of
+ * query q carry query = q.  CD_E_CAPACITY writes the needed count to n_out. */
+int cd_query_batch(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t mode, cd_contact* out,
+                   uint64_t cap, uint64_t* n_out);
+
+/* Candidate leaf pairs of the plain traversal (the mid-phase seam,
+ * OBBtreesIntersectInfo, include/Geometry/OBBtree.h:8-19), for a host that
+ * keeps its own narrow phase: 4 words each {query, offA, offB, cntA | cntB<<16}. */
+int cd_query_leafpairs(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t* out4, uint64_t cap,
+                       uint64_t* n_out);
+
+/* Phases 1-3 of CollisionDetection::ExecuteCollisionDetection
+ * (src/CollisionDetection/CollisionDetection.cpp:38-67): sweep-and-prune on
+ * the fixed U/V/W frame, relative matrix per pair, traversal, triangle tests.
+ * Pairs without contacts are dropped, as :52 and :63 drop them.            */
+int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs,
+             uint32_t pair_cap, uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap,
+             uint64_t* n_contacts);
+
+/* Broad phase alone (src/CollisionDetection/SweepAndPrune.cpp:15-88):
+ * oriented overlapping pairs, sorted by (first, second).                  */
+int cd_broad_phase(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t* pairs2, uint32_t cap,
+                   uint32_t* n_pairs);
+
+/* ---- device-resident runs (benchmarks, frame loops) ---------------------------- */
+/* Copy queries into device memory owned by the context ... */
+int cd_stage_queries(cd_ctx* ctx, const cd_query* q, uint32_t nq);
+/* ... run them with inputs already in HBM; results stay on the device ... */
+int cd_run_staged(cd_ctx* ctx, uint32_t mode, uint64_t* n_contacts);
+/* ... and fetch them. */
+int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_out);
+
+int cd_get_counters(cd_ctx* ctx, cd_counters* out);
+int cd_get_timings(cd_ctx* ctx, cd_timings* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI355X_COLLIDE_H */

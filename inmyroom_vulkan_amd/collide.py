@@ -1,0 +1,281 @@
+"""ctypes binding of the HIP collision library (include/mi355x_collide.h).
+
+The library is the product; this module only marshals numpy arrays across
+the C ABI.  Loading fails loudly when the HIP library is missing: there is
+no CPU fallback anywhere behind this interface.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmi355x_collide.so")
+
+CD_OK = 0
+CD_E_INVALID = -1
+CD_E_CAPACITY = -2
+CD_E_HIP = -3
+CD_E_STACK = -4
+CD_E_NOMEM = -5
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+
+# Symbols the header declares (checked by the CPU tests).
+EXPORTS = (
+    "cd_create", "cd_destroy", "cd_last_error", "cd_register_mesh", "cd_register_soups", "cd_mesh_info_get",
+    "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
+    "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
+)
+
+CONTACT_DTYPE = np.dtype([
+    ("query", "<u4"), ("tri_a", "<u4"), ("tri_b", "<u4"), ("flags", "<u4"),
+    ("s", "<f4", 3), ("t", "<f4", 3), ("na", "<f4", 3), ("nb", "<f4", 3),
+])
+QUERY_DTYPE = np.dtype([("mesh_a", "<u4"), ("mesh_b", "<u4"), ("rel", "<f4", 16)])
+ENTITY_DTYPE = np.dtype([
+    ("current", "<f4", 16), ("previous", "<f4", 16), ("mesh_id", "<u4"), ("entity", "<u2"),
+    ("should_callback", "u1"), ("pad", "u1"),
+])
+PAIR_DTYPE = np.dtype([("first", "<u4"), ("second", "<u4"), ("contact_begin", "<u4"), ("contact_count", "<u4")])
+assert CONTACT_DTYPE.itemsize == 64 and QUERY_DTYPE.itemsize == 72 and ENTITY_DTYPE.itemsize == 136
+
+
+class cd_config(C.Structure):
+    _fields_ = [("leaf_max", C.c_uint32), ("device", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+
+
+class cd_mesh_info(C.Structure):
+    _fields_ = [("ntri", C.c_uint32), ("nnodes", C.c_uint32), ("depth", C.c_uint32), ("leaf_max", C.c_uint32),
+                ("root", C.c_float * 12), ("slack", C.c_float), ("radius", C.c_float)]
+
+
+class cd_counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("sat_tests", "leaf_pairs", "tri_tests", "contacts", "fp64_fallbacks",
+                                            "exact_fallbacks", "ub_cases", "broad_pairs")]
+
+
+class cd_timings(C.Structure):
+    _fields_ = [("total_ms", C.c_float), ("traverse_ms", C.c_float), ("narrow_ms", C.c_float),
+                ("broad_ms", C.c_float), ("launches", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class CollideError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load the built HIP library; raise if it is not there."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"HIP collision library not built: {LIB_PATH} (run python -m inmyroom_vulkan_amd.build)")
+    lib = C.CDLL(LIB_PATH)
+    vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+    P = C.POINTER
+    lib.cd_create.argtypes = [P(vp), P(cd_config)]
+    lib.cd_destroy.argtypes = [vp]
+    lib.cd_last_error.argtypes = [vp]
+    lib.cd_last_error.restype = C.c_char_p
+    lib.cd_register_mesh.argtypes = [vp, vp, u32, vp, u32, P(u32)]
+    lib.cd_register_soups.argtypes = [vp, vp, vp, u32, P(u32)]
+    lib.cd_mesh_info_get.argtypes = [vp, u32, P(cd_mesh_info)]
+    lib.cd_export_tree.argtypes = [vp, u32, vp, vp, vp, vp]
+    lib.cd_tritri_batch.argtypes = [vp, vp, vp, u64, u32, vp, vp]
+    lib.cd_query_batch.argtypes = [vp, vp, u32, u32, vp, u64, P(u64)]
+    lib.cd_query_leafpairs.argtypes = [vp, vp, u32, vp, u64, P(u64)]
+    lib.cd_frame.argtypes = [vp, vp, u32, u32, vp, u32, P(u32), vp, u64, P(u64)]
+    lib.cd_broad_phase.argtypes = [vp, vp, u32, vp, u32, P(u32)]
+    lib.cd_stage_queries.argtypes = [vp, vp, u32]
+    lib.cd_run_staged.argtypes = [vp, u32, P(u64)]
+    lib.cd_fetch_contacts.argtypes = [vp, vp, u64, P(u64)]
+    lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
+    lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
+    for name in EXPORTS:
+        if name != "cd_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+class Collider:
+    """One library context (cd_ctx) on one GPU."""
+
+    def __init__(self, leaf_max: int = 4, device: int = 0):
+        self.lib = load_library()
+        cfg = cd_config(leaf_max=leaf_max, device=device)
+        h = C.c_void_p()
+        rc = self.lib.cd_create(C.byref(h), C.byref(cfg))
+        if rc != CD_OK:
+            raise CollideError(rc, "cd_create failed (no HIP device?)")
+        self.h = h
+        self.leaf_max = leaf_max
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != CD_OK:
+            raise CollideError(rc, f"{what}: {self.lib.cd_last_error(self.h).decode()}")
+
+    # -- meshes ---------------------------------------------------------------
+    def register_mesh(self, positions, indices) -> int:
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        mid = C.c_uint32()
+        self._check(self.lib.cd_register_mesh(self.h, _ptr(pos), len(pos), _ptr(idx), len(idx), C.byref(mid)),
+                    "cd_register_mesh")
+        return mid.value
+
+    def register_soups(self, soups) -> list[int]:
+        """soups: list of (T, 9) float32 triangle arrays.  Returns their mesh ids."""
+        counts = np.array([len(s) for s in soups], np.uint32)
+        tri = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float32).reshape(-1, 9) for s in soups])
+                                   if len(soups) else np.zeros((0, 9), np.float32))
+        first = C.c_uint32()
+        self._check(self.lib.cd_register_soups(self.h, _ptr(tri), _ptr(counts), len(counts), C.byref(first)),
+                    "cd_register_soups")
+        return list(range(first.value, first.value + len(soups)))
+
+    def mesh_info(self, mesh: int) -> dict:
+        info = cd_mesh_info()
+        self._check(self.lib.cd_mesh_info_get(self.h, mesh, C.byref(info)), "cd_mesh_info_get")
+        return {"ntri": info.ntri, "nnodes": info.nnodes, "depth": info.depth, "leaf_max": info.leaf_max,
+                "root": np.array(info.root, np.float32), "slack": info.slack, "radius": info.radius}
+
+    def export_tree(self, mesh: int) -> dict:
+        info = self.mesh_info(mesh)
+        n, t = info["nnodes"], info["ntri"]
+        root = np.zeros(12, np.float32)
+        obbs = np.zeros((max(n, 1), 24), np.float32)
+        links = np.zeros((max(n, 1), 4), np.uint32)
+        perm = np.zeros(max(t, 1), np.uint32)
+        self._check(self.lib.cd_export_tree(self.h, mesh, _ptr(root), _ptr(obbs), _ptr(links), _ptr(perm)),
+                    "cd_export_tree")
+        return {"root": root, "obbs": obbs[:n], "links": links[:n], "perm": perm[:t], "depth": info["depth"]}
+
+    # -- queries --------------------------------------------------------------
+    def tritri(self, a9, b9, mode: int = MODE_PLAIN):
+        a9 = np.ascontiguousarray(a9, np.float32).reshape(-1, 9)
+        b9 = np.ascontiguousarray(b9, np.float32).reshape(-1, 9)
+        n = len(a9)
+        flags = np.zeros(n, np.int32)
+        seg = np.zeros((n, 6), np.float32)
+        self._check(self.lib.cd_tritri_batch(self.h, _ptr(a9), _ptr(b9), n, mode, _ptr(flags), _ptr(seg)),
+                    "cd_tritri_batch")
+        return flags, seg
+
+    @staticmethod
+    def make_queries(mesh_a, mesh_b, rel) -> np.ndarray:
+        rel = np.asarray(rel, np.float32).reshape(-1, 16)
+        q = np.zeros(len(rel), QUERY_DTYPE)
+        q["mesh_a"] = mesh_a
+        q["mesh_b"] = mesh_b
+        q["rel"] = rel
+        return q
+
+    def query_batch(self, queries: np.ndarray, mode: int = MODE_PLAIN, cap: int | None = None) -> np.ndarray:
+        queries = np.ascontiguousarray(queries, QUERY_DTYPE)
+        cap = cap if cap is not None else 1 << 16
+        while True:
+            out = np.zeros(cap, CONTACT_DTYPE)
+            n = C.c_uint64()
+            rc = self.lib.cd_query_batch(self.h, _ptr(queries), len(queries), mode, _ptr(out), cap, C.byref(n))
+            if rc == CD_E_CAPACITY:
+                cap = int(n.value)
+                continue
+            self._check(rc, "cd_query_batch")
+            return out[: n.value]
+
+    def query_leafpairs(self, queries: np.ndarray) -> np.ndarray:
+        queries = np.ascontiguousarray(queries, QUERY_DTYPE)
+        cap = 1 << 16
+        while True:
+            out = np.zeros((cap, 4), np.uint32)
+            n = C.c_uint64()
+            rc = self.lib.cd_query_leafpairs(self.h, _ptr(queries), len(queries), _ptr(out), cap, C.byref(n))
+            if rc == CD_E_CAPACITY:
+                cap = int(n.value)
+                continue
+            self._check(rc, "cd_query_leafpairs")
+            return out[: n.value]
+
+    def stage(self, queries: np.ndarray):
+        queries = np.ascontiguousarray(queries, QUERY_DTYPE)
+        self._check(self.lib.cd_stage_queries(self.h, _ptr(queries), len(queries)), "cd_stage_queries")
+
+    def run_staged(self, mode: int = MODE_PLAIN) -> int:
+        n = C.c_uint64()
+        self._check(self.lib.cd_run_staged(self.h, mode, C.byref(n)), "cd_run_staged")
+        return int(n.value)
+
+    def fetch(self) -> np.ndarray:
+        n = C.c_uint64()
+        rc = self.lib.cd_fetch_contacts(self.h, None, 0, C.byref(n))
+        if rc not in (CD_OK, CD_E_CAPACITY):
+            self._check(rc, "cd_fetch_contacts")
+        out = np.zeros(int(n.value), CONTACT_DTYPE)
+        self._check(self.lib.cd_fetch_contacts(self.h, _ptr(out), len(out), C.byref(n)), "cd_fetch_contacts")
+        return out
+
+    def frame(self, entities: np.ndarray, mode: int = MODE_PLAIN):
+        ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
+        pcap, ccap = 1 << 12, 1 << 16
+        while True:
+            pairs = np.zeros(pcap, PAIR_DTYPE)
+            cons = np.zeros(ccap, CONTACT_DTYPE)
+            npairs, ncons = C.c_uint32(), C.c_uint64()
+            rc = self.lib.cd_frame(self.h, _ptr(ents), len(ents), mode, _ptr(pairs), pcap, C.byref(npairs),
+                                   _ptr(cons), ccap, C.byref(ncons))
+            if rc == CD_E_CAPACITY:
+                pcap = max(pcap, int(npairs.value))
+                ccap = max(ccap, int(ncons.value))
+                continue
+            self._check(rc, "cd_frame")
+            return pairs[: npairs.value], cons[: ncons.value]
+
+    def broad_phase(self, entities: np.ndarray) -> np.ndarray:
+        ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
+        cap = max(1024, 4 * len(ents))
+        while True:
+            out = np.zeros((cap, 2), np.uint32)
+            n = C.c_uint32()
+            rc = self.lib.cd_broad_phase(self.h, _ptr(ents), len(ents), _ptr(out), cap, C.byref(n))
+            if rc == CD_E_CAPACITY:
+                cap = int(n.value)
+                continue
+            self._check(rc, "cd_broad_phase")
+            return out[: n.value]
+
+    def counters(self) -> dict:
+        c = cd_counters()
+        self._check(self.lib.cd_get_counters(self.h, C.byref(c)), "cd_get_counters")
+        return {f: getattr(c, f) for f, _ in cd_counters._fields_}
+
+    def timings(self) -> dict:
+        t = cd_timings()
+        self._check(self.lib.cd_get_timings(self.h, C.byref(t)), "cd_get_timings")
+        return {f: getattr(t, f) for f, _ in cd_timings._fields_ if f != "pad"}

@@ -1,0 +1,620 @@
+// OBB-tree build on the GPU, bit-identical to the reference builder.
+//
+// Reference: inMyRoom_vulkan/src/Geometry/OBB.cpp:33-178 (PCA box),
+// inMyRoom_vulkan/src/Geometry/OBBtree.cpp:8-108 (recursive split) and
+// :179-233, :321-394 (DFS flatten).
+//
+// Level-synchronous: every node of one depth, across every mesh of the
+// batch, is one 64-lane workgroup of a single launch.  Within a node:
+//   * the distinct-point check is a wave ballot;
+//   * the mean and the six covariance sums are SEQUENTIAL double chains in
+//     point order (the reference's std::accumulate / std::inner_product),
+//     one lane per chain, fed from LDS-staged chunks of the node's points;
+//   * eig3 runs in FP64 on one lane;
+//   * the three min/max projections are wave reductions whose tie rule
+//     reproduces std::min/std::max visiting the points in order;
+//   * the split is a stable partition built from ballots, scattering the
+//     node's triangles into the next level's buffer.
+// Afterwards the tree is flattened to the reference's DFS layout with
+// per-level prefix passes (subtree counts bottom-up, preorder top-down).
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "eig3.hpp"
+#include "internal.hpp"
+
+namespace cdk {
+
+struct BNode {
+    uint32_t mesh, beg, end, level;
+    uint32_t kid[2];
+    uint32_t flags;  // bit0 = leaf
+    uint32_t icount, tcount, pre, toff;
+    float surf;
+    float box[12];
+};
+
+struct LevelArgs {
+    BNode* nodes;
+    const float* pos_in;
+    const uint32_t* id_in;
+    float* pos_out;
+    uint32_t* id_out;
+    float* pos_fin;
+    uint32_t* id_fin;
+    unsigned int* next_count;
+    unsigned int* mesh_depth;
+    uint32_t leaf_max;
+};
+
+constexpr int CHUNK = 256;  // points staged in LDS per step of the sequential chains
+
+__device__ inline void wave_min_last(double& v, int& i) {
+    // smaller value wins; equal values (== , so -0 and +0 too): larger index wins
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(i, off, 64);
+        if (ov < v || (ov == v && oi > i)) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+__device__ inline void wave_max_first(double& v, int& i) {
+    // larger value wins; equal values: smaller index wins
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(i, off, 64);
+        if (ov > v || (ov == v && oi < i)) {
+            v = ov;
+            i = oi;
+        }
+    }
+}
+
+struct AxisCand {
+    float len;
+    f3 dir;
+};
+
+// std::sort of three (length, axis) pairs, descending (OBBtree.cpp:45-51):
+// libstdc++ runs its insertion sort for three elements; reproduced branch
+// for branch.
+__device__ inline void sort3_desc(AxisCand c[3]) {
+#pragma unroll
+    for (int i = 1; i < 3; ++i) {
+        AxisCand val = c[i];
+        if (val.len > c[0].len) {
+            for (int k = i; k > 0; --k) c[k] = c[k - 1];
+            c[0] = val;
+        } else {
+            int j = i;
+            while (val.len > c[j - 1].len) {
+                c[j] = c[j - 1];
+                --j;
+            }
+            c[j] = val;
+        }
+    }
+}
+
+// Triangle::GetMinMaxProjectionToAxis (Triangle.cpp:113-138) then the
+// left/right rule of OBBtree.cpp:66-75.
+__device__ inline bool goes_left(const float* T, f3 axis, float cproj) {
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float q = dot(axis, mk(T[3 * k], T[3 * k + 1], T[3 * k + 2]));
+        if (q < lo) lo = q;
+        if (q > hi) hi = q;
+    }
+    float mid = (lo + hi) / 2.f;
+    return mid <= cproj;
+}
+
+__global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint32_t next_first) {
+    __shared__ float s_pts[CHUNK * 3];
+    __shared__ double s_mean[3];
+    __shared__ double s_cov[6];
+    __shared__ double s_ax[9];
+    __shared__ float s_box[12];
+
+    const int lane = threadIdx.x;
+    const uint32_t ni = first + blockIdx.x;
+    BNode* nd = &a.nodes[ni];
+    const uint32_t beg = nd->beg, end = nd->end;
+    const uint32_t n = end - beg, npts = 3 * n;
+    const float* P = a.pos_in + size_t(beg) * 9;
+
+    // ---- OBB::CreateOBBfromPoints (OBB.cpp:33-89) ----
+    int differs = 0;
+    for (uint32_t k = 1 + lane; k < npts; k += 64) {
+        if (!(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2])) differs = 1;
+    }
+    const bool multi = __ballot(differs) != 0;
+    Box box;
+    if (npts == 0 || !multi) {
+        const float e = FLT_EPSILON;  // OBB::EmptyOBB (OBB.cpp:168-178)
+        box.c = npts ? mk(P[0], P[1], P[2]) : mk(0.f, 0.f, 0.f);
+        box.u = mk(e, 0.f, 0.f);
+        box.v = mk(0.f, e, 0.f);
+        box.w = mk(0.f, 0.f, e);
+    } else {
+        const double N = double(npts);
+        double acc = 0.0;
+        for (uint32_t base = 0; base < npts; base += CHUNK) {
+            const uint32_t m = min(uint32_t(CHUNK), npts - base);
+            for (uint32_t t = lane; t < 3 * m; t += 64) s_pts[t] = P[3 * size_t(base) + t];
+            __syncthreads();
+            if (lane < 3) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < m; ++k) acc = acc + double(s_pts[3 * k + lane]);
+            }
+            __syncthreads();
+        }
+        if (lane < 3) s_mean[lane] = acc / N;
+        __syncthreads();
+        // covariance order xx, yy, zz, xy, xz, yz (OBB.cpp:56-73)
+        const int ca = lane == 3 || lane == 4 ? 0 : (lane == 5 ? 1 : (lane < 3 ? lane : 0));
+        const int cb = lane == 3 ? 1 : (lane == 4 || lane == 5 ? 2 : (lane < 3 ? lane : 0));
+        const double ma = s_mean[ca], mb = s_mean[cb];
+        acc = 0.0;
+        for (uint32_t base = 0; base < npts; base += CHUNK) {
+            const uint32_t m = min(uint32_t(CHUNK), npts - base);
+            for (uint32_t t = lane; t < 3 * m; t += 64) s_pts[t] = P[3 * size_t(base) + t];
+            __syncthreads();
+            if (lane < 6) {
+#pragma unroll 8
+                for (uint32_t k = 0; k < m; ++k) {
+                    double da = double(s_pts[3 * k + ca]) - ma;
+                    double db = double(s_pts[3 * k + cb]) - mb;
+                    acc = acc + da * db;
+                }
+            }
+            __syncthreads();
+        }
+        if (lane < 6) s_cov[lane] = acc / N;
+        __syncthreads();
+        if (lane == 0) {
+            const double xx = s_cov[0], yy = s_cov[1], zz = s_cov[2], xy = s_cov[3], xz = s_cov[4], yz = s_cov[5];
+            double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+            double V[3][3], d[3];
+            eig3(A, V, d);
+            // OBB.cpp:80-87: the box axes are the ROWS of eig3's V
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) s_ax[r * 3 + c] = V[r][c];
+        }
+        __syncthreads();
+        double ax[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) ax[t] = s_ax[t];
+        // OBB::CreateAABBfromPoints (OBB.cpp:105-166)
+        double lo[3], hi[3];
+        int loi[3], hii[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            lo[q] = INFINITY;
+            hi[q] = -INFINITY;
+            loi[q] = -1;
+            hii[q] = -1;
+        }
+        for (uint32_t k = lane; k < npts; k += 64) {
+            const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
+                if (!(lo[q] < pr)) { lo[q] = pr; loi[q] = int(k); }
+                if (hi[q] < pr) { hi[q] = pr; hii[q] = int(k); }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            wave_min_last(lo[q], loi[q]);
+            wave_max_first(hi[q], hii[q]);
+        }
+        if (lane == 0) {
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            float side[3][3];
+            for (int q = 0; q < 3; ++q) {
+                double delta = (hi[q] - lo[q]) + 2.0 * double(FLT_EPSILON);
+                double mid = (hi[q] + lo[q]) / 2.0;
+                cx = cx + mid * ax[3 * q];
+                cy = cy + mid * ax[3 * q + 1];
+                cz = cz + mid * ax[3 * q + 2];
+                double half = delta / 2.0;
+                side[q][0] = float(half * ax[3 * q]);
+                side[q][1] = float(half * ax[3 * q + 1]);
+                side[q][2] = float(half * ax[3 * q + 2]);
+            }
+            s_box[0] = float(cx);
+            s_box[1] = float(cy);
+            s_box[2] = float(cz);
+            for (int q = 0; q < 3; ++q)
+                for (int c = 0; c < 3; ++c) s_box[3 + 3 * q + c] = side[q][c];
+        }
+        __syncthreads();
+        box = box_from12(s_box);
+    }
+    if (lane == 0) {
+        box_to12(box, nd->box);
+        nd->surf = surface(box);
+    }
+
+    // ---- leaf: park the triangles in their final slots ----
+    if (n <= a.leaf_max) {
+        for (uint32_t t = lane; t < 9 * n; t += 64) a.pos_fin[size_t(beg) * 9 + t] = P[t];
+        for (uint32_t t = lane; t < n; t += 64) a.id_fin[beg + t] = a.id_in[beg + t];
+        if (lane == 0) {
+            nd->flags = 1u;
+            nd->icount = 0;
+            nd->tcount = n;
+        }
+        return;
+    }
+
+    // ---- OBBtreeSplitBuildNode::SplitOBBandCreateChildren (OBBtree.cpp:43-108) ----
+    AxisCand cand[3] = {{vlength(box.u), normalize(box.u)}, {vlength(box.v), normalize(box.v)},
+                        {vlength(box.w), normalize(box.w)}};
+    sort3_desc(cand);
+    bool ok = false;
+    f3 axis = cand[0].dir;
+    float cproj = 0.f;
+    uint32_t nl = 0;
+    for (int ch = 0; ch < 3 && !ok; ++ch) {
+        axis = cand[ch].dir;
+        cproj = dot(box.c, axis);
+        uint32_t cnt = 0;
+        for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            const bool left = i < n && goes_left(P + size_t(i) * 9, axis, cproj);
+            cnt += uint32_t(__popcll(__ballot(left)));
+        }
+        nl = cnt;
+        ok = nl != 0 && nl != n;
+    }
+    if (!ok) nl = n / 2;  // index halves (OBBtree.cpp:81-93)
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t runL = 0, runR = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        const uint32_t i = c0 + lane;
+        const bool valid = i < n;
+        bool left = false;
+        if (valid) left = ok ? goes_left(P + size_t(i) * 9, axis, cproj) : (i < n / 2);
+        const unsigned long long mL = __ballot(valid && left);
+        const unsigned long long mR = __ballot(valid && !left);
+        if (valid) {
+            const uint32_t dst = left ? beg + runL + uint32_t(__popcll(mL & below))
+                                      : beg + nl + runR + uint32_t(__popcll(mR & below));
+            const float* src = P + size_t(i) * 9;
+            float* o = a.pos_out + size_t(dst) * 9;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) o[t] = src[t];
+            a.id_out[dst] = a.id_in[beg + i];
+        }
+        runL += uint32_t(__popcll(mL));
+        runR += uint32_t(__popcll(mR));
+    }
+    if (lane == 0) {
+        const uint32_t slot = next_first + atomicAdd(a.next_count, 2u);
+        BNode* k0 = &a.nodes[slot];
+        BNode* k1 = &a.nodes[slot + 1];
+        k0->mesh = nd->mesh; k0->beg = beg;      k0->end = beg + nl; k0->level = nd->level + 1; k0->flags = 0;
+        k1->mesh = nd->mesh; k1->beg = beg + nl; k1->end = end;      k1->level = nd->level + 1; k1->flags = 0;
+        nd->kid[0] = slot;
+        nd->kid[1] = slot + 1;
+        nd->flags = 0;
+        atomicMax(&a.mesh_depth[nd->mesh], nd->level + 1);
+    }
+}
+
+// Left child = larger surface (OBBtree.cpp:101-107).
+__global__ void k_order_kids(BNode* nodes, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    BNode* nd = &nodes[i];
+    if (nd->flags & 1u) return;
+    const uint32_t l = nd->kid[0], r = nd->kid[1];
+    if (nodes[r].surf > nodes[l].surf) {
+        nd->kid[0] = r;
+        nd->kid[1] = l;
+    }
+}
+
+// Subtree sizes, one level at a time from the bottom.
+__global__ void k_counts(BNode* nodes, uint32_t first, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    BNode* nd = &nodes[first + i];
+    if (nd->flags & 1u) return;
+    const BNode& l = nodes[nd->kid[0]];
+    const BNode& r = nodes[nd->kid[1]];
+    nd->icount = 1 + l.icount + r.icount;
+    nd->tcount = l.tcount + r.tcount;
+}
+
+// Preorder index and DFS triangle offset, one level at a time from the top
+// (OBBtree.cpp:179-233, :385-394).
+__global__ void k_preorder(BNode* nodes, uint32_t first, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const BNode& nd = nodes[first + i];
+    if (nd.flags & 1u) return;
+    BNode& l = nodes[nd.kid[0]];
+    BNode& r = nodes[nd.kid[1]];
+    l.pre = nd.pre + 1;
+    l.toff = nd.toff;
+    r.pre = nd.pre + 1 + l.icount;
+    r.toff = nd.toff + l.tcount;
+}
+
+__global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t* node_base, NodeRec* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const BNode& nd = nodes[i];
+    if (nd.flags & 1u) return;
+    NodeRec rec;
+    uint32_t cnt = 0;
+    for (int s = 0; s < 2; ++s) {
+        const BNode& k = nodes[nd.kid[s]];
+        for (int t = 0; t < 12; ++t) rec.box[s][t] = k.box[t];
+        rec.surf[s] = k.surf;
+        if (k.flags & 1u) {
+            rec.link[s] = k.toff;
+            cnt |= (k.tcount & 0xFFFFu) << (16 * s);
+        } else {
+            rec.link[s] = k.pre;
+        }
+    }
+    rec.cnt = cnt;
+    rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
+    out[node_base[nd.mesh] + nd.pre] = rec;
+}
+
+__global__ void k_emit_tris(const BNode* nodes, uint32_t count, const uint32_t* tri_base, const float* pos_fin,
+                            const uint32_t* id_fin, float* tris, uint32_t* perm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const BNode& nd = nodes[i];
+    if (!(nd.flags & 1u)) return;
+    const size_t dst0 = size_t(tri_base[nd.mesh]) + nd.toff;
+    for (uint32_t t = 0; t < nd.tcount; ++t) {
+        const float* s = pos_fin + size_t(nd.beg + t) * 9;
+        float* o = tris + (dst0 + t) * 9;
+        for (int q = 0; q < 9; ++q) o[q] = s[q];
+        perm[dst0 + t] = id_fin[nd.beg + t];
+    }
+}
+
+// Containment slack of every box over its own triangles, and the mesh radius.
+__global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count, const float* pos_fin,
+                                              unsigned int* mesh_slack_bits, unsigned int* mesh_radius_bits) {
+    const uint32_t ni = blockIdx.x;
+    if (ni >= count) return;
+    const BNode& nd = nodes[ni];
+    const int lane = threadIdx.x;
+    const double u[3] = {nd.box[3], nd.box[4], nd.box[5]};
+    const double v[3] = {nd.box[6], nd.box[7], nd.box[8]};
+    const double w[3] = {nd.box[9], nd.box[10], nd.box[11]};
+    auto det3 = [](const double* a, const double* b, const double* c) {
+        return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+    };
+    const double D = det3(u, v, w);
+    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double worst = 0.0, rad = 0.0;
+    const uint32_t npts = 3 * (nd.end - nd.beg);
+    const float* P = pos_fin + size_t(nd.beg) * 9;
+    for (uint32_t k = lane; k < npts; k += 64) {
+        const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+        const double r[3] = {p[0] - nd.box[0], p[1] - nd.box[1], p[2] - nd.box[2]};
+        double ex;
+        if (fabs(D) > 0.0) {
+            const double al = det3(r, v, w) / D, be = det3(u, r, w) / D, ga = det3(u, v, r) / D;
+            ex = fmax(0.0, fabs(al) - 1.0) * lu + fmax(0.0, fabs(be) - 1.0) * lv + fmax(0.0, fabs(ga) - 1.0) * lw;
+        } else {
+            ex = INFINITY;
+        }
+        worst = fmax(worst, ex);
+        rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        worst = fmax(worst, __shfl_xor(worst, off, 64));
+        rad = fmax(rad, __shfl_xor(rad, off, 64));
+    }
+    if (lane == 0) {
+        // round up to float so the bound stays an upper bound
+        float wf = float(worst);
+        if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
+        float rf = float(rad);
+        if (double(rf) < rad) rf = nextafterf(rf, INFINITY);
+        atomicMax(&mesh_slack_bits[nd.mesh], __float_as_uint(wf));
+        if (nd.level == 0) atomicMax(&mesh_radius_bits[nd.mesh], __float_as_uint(rf));
+    }
+}
+
+#define CK(x)                                                        \
+    do {                                                             \
+        hipError_t e_ = (x);                                         \
+        if (e_ != hipSuccess) {                                      \
+            res.err = std::string(#x ": ") + hipGetErrorString(e_);  \
+            goto fail;                                               \
+        }                                                            \
+    } while (0)
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
+
+// Build every soup of the batch and append the trees to the context arenas.
+BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh) {
+    BuildResult res;
+    hipStream_t st = ctx->stream;
+    uint64_t T = 0, node_cap = 0;
+    std::vector<uint64_t> tri_begin(nmesh);
+    for (uint32_t m = 0; m < nmesh; ++m) {
+        tri_begin[m] = T;
+        T += counts[m];
+        node_cap += counts[m] ? 2ull * counts[m] - 1 : 1;
+    }
+    if (T >= 0xFFFFFFF0ull || node_cap >= 0xFFFFFFF0ull) {
+        res.err = "batch too large (triangle or node count exceeds 32-bit indexing)";
+        return res;
+    }
+    DevBuf pos[2], ids[2], pos_fin, id_fin, nodes, mesh_depth, next_count, base_nodes, base_tris, slack_bits, radius_bits;
+    std::vector<BNode> roots(nmesh);
+    std::vector<std::pair<uint32_t, uint32_t>> levels;
+    std::vector<uint32_t> host_ids;
+    std::vector<uint32_t> node_base(nmesh), tri_base(nmesh), depth(nmesh), slack(nmesh), radius(nmesh);
+    uint64_t total_internal = 0;
+    uint32_t first = 0, count = nmesh, cur = 0;
+    const size_t posb = std::max<uint64_t>(T, 1) * 9 * sizeof(float), idb = std::max<uint64_t>(T, 1) * sizeof(uint32_t);
+
+    CK(pos[0].ensure(posb));
+    CK(pos[1].ensure(posb));
+    CK(pos_fin.ensure(posb));
+    CK(ids[0].ensure(idb));
+    CK(ids[1].ensure(idb));
+    CK(id_fin.ensure(idb));
+    CK(nodes.ensure(node_cap * sizeof(BNode)));
+    CK(mesh_depth.ensure(nmesh * sizeof(uint32_t)));
+    CK(slack_bits.ensure(nmesh * sizeof(uint32_t)));
+    CK(radius_bits.ensure(nmesh * sizeof(uint32_t)));
+    CK(next_count.ensure(sizeof(uint32_t)));
+    CK(base_nodes.ensure(nmesh * sizeof(uint32_t)));
+    CK(base_tris.ensure(nmesh * sizeof(uint32_t)));
+
+    if (T) CK(hipMemcpyAsync(pos[0].p, tri9, T * 9 * sizeof(float), hipMemcpyHostToDevice, st));
+    host_ids.resize(T);
+    for (uint32_t m = 0; m < nmesh; ++m)
+        for (uint32_t k = 0; k < counts[m]; ++k) host_ids[tri_begin[m] + k] = k;
+    if (T) CK(hipMemcpyAsync(ids[0].p, host_ids.data(), T * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    for (uint32_t m = 0; m < nmesh; ++m) {
+        BNode r{};
+        r.mesh = m;
+        r.beg = uint32_t(tri_begin[m]);
+        r.end = uint32_t(tri_begin[m] + counts[m]);
+        r.level = 0;
+        r.pre = 0;
+        r.toff = 0;
+        roots[m] = r;
+    }
+    CK(hipMemcpyAsync(nodes.p, roots.data(), nmesh * sizeof(BNode), hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(mesh_depth.p, 0, nmesh * sizeof(uint32_t), st));
+    CK(hipMemsetAsync(slack_bits.p, 0, nmesh * sizeof(uint32_t), st));
+    CK(hipMemsetAsync(radius_bits.p, 0, nmesh * sizeof(uint32_t), st));
+
+    while (count > 0) {
+        LevelArgs a;
+        a.nodes = nodes.as<BNode>();
+        a.pos_in = pos[cur].as<float>();
+        a.id_in = ids[cur].as<uint32_t>();
+        a.pos_out = pos[cur ^ 1].as<float>();
+        a.id_out = ids[cur ^ 1].as<uint32_t>();
+        a.pos_fin = pos_fin.as<float>();
+        a.id_fin = id_fin.as<uint32_t>();
+        a.next_count = next_count.as<unsigned int>();
+        a.mesh_depth = mesh_depth.as<unsigned int>();
+        a.leaf_max = ctx->leaf_max;
+        CK(hipMemsetAsync(next_count.p, 0, sizeof(uint32_t), st));
+        const uint32_t next_first = first + count;
+        hipLaunchKernelGGL(k_level, dim3(count), dim3(64), 0, st, a, first, next_first);
+        CK(hipGetLastError());
+        uint32_t nxt = 0;
+        CK(hipMemcpyAsync(&nxt, next_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        levels.push_back({first, count});
+        first = next_first;
+        count = nxt;
+        cur ^= 1;
+    }
+    {
+        const uint32_t total = first;
+        hipLaunchKernelGGL(k_order_kids, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total);
+        CK(hipGetLastError());
+        for (size_t l = levels.size(); l-- > 0;) {
+            hipLaunchKernelGGL(k_counts, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st, nodes.as<BNode>(),
+                               levels[l].first, levels[l].second);
+            CK(hipGetLastError());
+        }
+        for (size_t l = 0; l < levels.size(); ++l) {
+            hipLaunchKernelGGL(k_preorder, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st, nodes.as<BNode>(),
+                               levels[l].first, levels[l].second);
+            CK(hipGetLastError());
+        }
+        CK(hipMemcpyAsync(roots.data(), nodes.p, nmesh * sizeof(BNode), hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(depth.data(), mesh_depth.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        for (uint32_t m = 0; m < nmesh; ++m) {
+            node_base[m] = uint32_t(ctx->nodes_used + total_internal);
+            total_internal += roots[m].icount;
+            tri_base[m] = uint32_t(ctx->tris_used + tri_begin[m]);
+        }
+        if (ctx->nodes_used + total_internal >= 0xFFFFFFF0ull || ctx->tris_used + T >= 0xFFFFFFF0ull) {
+            res.err = "arena exceeds 32-bit indexing";
+            goto fail;
+        }
+        // grow the arenas (keeping what is already registered)
+        auto grow = [&](DevBuf& b, size_t need) -> hipError_t {
+            if (need <= b.bytes) return hipSuccess;
+            size_t want = std::max(need, b.bytes * 2);
+            void* np = nullptr;
+            hipError_t e = hipMalloc(&np, want);
+            if (e != hipSuccess) return e;
+            if (b.p) {
+                e = hipMemcpyAsync(np, b.p, b.bytes, hipMemcpyDeviceToDevice, st);
+                if (e != hipSuccess) return e;
+                e = hipStreamSynchronize(st);
+                if (e != hipSuccess) return e;
+                (void)hipFree(b.p);
+            }
+            b.p = np;
+            b.bytes = want;
+            return hipSuccess;
+        };
+        CK(grow(ctx->d_nodes, (ctx->nodes_used + total_internal + 1) * sizeof(NodeRec)));
+        CK(grow(ctx->d_tris, (ctx->tris_used + T + 1) * 9 * sizeof(float)));
+        CK(grow(ctx->d_perm, (ctx->tris_used + T + 1) * sizeof(uint32_t)));
+        CK(hipMemcpyAsync(base_nodes.p, node_base.data(), nmesh * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(base_tris.p, tri_base.data(), nmesh * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_emit_nodes, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total,
+                           base_nodes.as<uint32_t>(), ctx->d_nodes.as<NodeRec>());
+        CK(hipGetLastError());
+        hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total,
+                           base_tris.as<uint32_t>(), pos_fin.as<float>(), id_fin.as<uint32_t>(), ctx->d_tris.as<float>(),
+                           ctx->d_perm.as<uint32_t>());
+        CK(hipGetLastError());
+        hipLaunchKernelGGL(k_slack, dim3(total), dim3(64), 0, st, nodes.as<BNode>(), total, pos_fin.as<float>(),
+                           slack_bits.as<unsigned int>(), radius_bits.as<unsigned int>());
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(slack.data(), slack_bits.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(radius.data(), radius_bits.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        res.meshes.resize(nmesh);
+        for (uint32_t m = 0; m < nmesh; ++m) {
+            MeshRec& r = res.meshes[m];
+            r.node_base = node_base[m];
+            r.tri_base = tri_base[m];
+            r.ntri = counts[m];
+            r.nnodes = roots[m].icount;
+            std::memcpy(r.root, roots[m].box, sizeof r.root);
+            r.root_surf = roots[m].surf;
+            r.depth = depth[m];
+            std::memcpy(&r.slack, &slack[m], sizeof(float));
+            std::memcpy(&r.radius, &radius[m], sizeof(float));
+        }
+        ctx->nodes_used += total_internal;
+        ctx->tris_used += T;
+    }
+fail:
+    for (auto* b : {&pos[0], &pos[1], &ids[0], &ids[1], &pos_fin, &id_fin, &nodes, &mesh_depth, &next_count, &base_nodes,
+                    &base_tris, &slack_bits, &radius_bits})
+        b->release();
+    return res;
+}
+
+}  // namespace cdk

@@ -1,0 +1,376 @@
+// C ABI of the collision library (declared in include/mi355x_collide.h).
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+using namespace cdk;
+
+namespace {
+
+int fail(cd_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+int hip_fail(cd_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, CD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HCK(ctx, x)                                                 \
+    do {                                                            \
+        hipError_t e_ = (x);                                        \
+        if (e_ != hipSuccess) return hip_fail((ctx), e_, #x);       \
+    } while (0)
+
+bool all_finite(const float* p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int upload_mesh_table(cd_ctx* ctx) {
+    const size_t bytes = ctx->meshes.size() * sizeof(MeshRec);
+    HCK(ctx, ctx->d_meshes.ensure(bytes + sizeof(MeshRec)));
+    if (bytes) HCK(ctx, hipMemcpyAsync(ctx->d_meshes.p, ctx->meshes.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+int register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh, uint32_t* first_id) {
+    uint64_t T = 0;
+    for (uint32_t m = 0; m < nmesh; ++m) T += counts[m];
+    if (T && !tri9) return fail(ctx, CD_E_INVALID, "null triangle array");
+    if (!all_finite(tri9, size_t(T) * 9)) return fail(ctx, CD_E_INVALID, "non-finite vertex coordinate");
+    for (uint32_t m = 0; m < nmesh; ++m)
+        if (counts[m] > 0x7FFFFFFFu) return fail(ctx, CD_E_INVALID, "mesh too large");
+    BuildResult r = build_soups(ctx, tri9, counts, nmesh);
+    if (!r.err.empty()) return fail(ctx, CD_E_HIP, "tree build: " + r.err);
+    if (first_id) *first_id = uint32_t(ctx->meshes.size());
+    ctx->meshes.insert(ctx->meshes.end(), r.meshes.begin(), r.meshes.end());
+    return upload_mesh_table(ctx);
+}
+
+int check_queries(cd_ctx* ctx, const QueryRec* q, uint32_t nq) {
+    const uint32_t nm = uint32_t(ctx->meshes.size());
+    for (uint32_t i = 0; i < nq; ++i) {
+        if (q[i].mesh_a >= nm || q[i].mesh_b >= nm) return fail(ctx, CD_E_INVALID, "query names an unknown mesh");
+        if (!all_finite(q[i].rel, 16)) return fail(ctx, CD_E_INVALID, "non-finite relative matrix");
+        if (ctx->meshes[q[i].mesh_a].depth + ctx->meshes[q[i].mesh_b].depth + 2 > uint32_t(PLAIN_STACK))
+            return fail(ctx, CD_E_STACK, "trees too deep for the plain traversal stack");
+    }
+    return CD_OK;
+}
+
+RunCounters* dev_counters(cd_ctx* ctx) { return ctx->d_counters.as<RunCounters>(); }
+
+int read_counters(cd_ctx* ctx, RunCounters& h) {
+    HCK(ctx, hipMemcpyAsync(&h, ctx->d_counters.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+// Plain run over the nq queries already in d_queries; contacts stay in d_contacts.
+int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
+    hipStream_t st = ctx->stream;
+    const QueryRec* dq = ctx->d_queries.as<QueryRec>();
+    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    const NodeRec* dn = ctx->d_nodes.as<NodeRec>();
+    if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
+    if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
+    RunCounters h{};
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
+        HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
+        HCK(ctx, launch_zero(dev_counters(ctx), st));
+        HCK(ctx, hipEventRecord(ctx->ev[0], st));
+        HCK(ctx, launch_traverse_plain(dq, nq, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, dev_counters(ctx), st));
+        HCK(ctx, hipEventRecord(ctx->ev[1], st));
+        int rc = read_counters(ctx, h);
+        if (rc) return rc;
+        if (h.err) return fail(ctx, CD_E_STACK, "plain traversal stack overflow");
+        if (h.leafpairs > ctx->lp_cap) {
+            ctx->lp_cap = h.leafpairs + h.leafpairs / 4;
+            continue;
+        }
+        HCK(ctx, hipEventRecord(ctx->ev[2], st));
+        HCK(ctx, launch_narrow_plain(ctx->d_leafpairs.as<uint4>(), h.leafpairs, ctx->leaf_max, dq, dm,
+                                     ctx->d_tris.as<float>(), ctx->d_contacts.as<cd_contact>(), ctx->contact_cap,
+                                     dev_counters(ctx), st));
+        HCK(ctx, hipEventRecord(ctx->ev[3], st));
+        rc = read_counters(ctx, h);
+        if (rc) return rc;
+        if (h.contacts > ctx->contact_cap) {
+            ctx->contact_cap = h.contacts + h.contacts / 4;
+            continue;
+        }
+        float t_trav = 0.f, t_narrow = 0.f;
+        HCK(ctx, hipEventElapsedTime(&t_trav, ctx->ev[0], ctx->ev[1]));
+        HCK(ctx, hipEventElapsedTime(&t_narrow, ctx->ev[2], ctx->ev[3]));
+        ctx->timings = cd_timings{};
+        ctx->timings.traverse_ms = t_trav;
+        ctx->timings.narrow_ms = t_narrow;
+        ctx->timings.total_ms = t_trav + t_narrow;
+        ctx->timings.launches = 1;
+        ctx->counters = cd_counters{};
+        ctx->counters.sat_tests = h.sat;
+        ctx->counters.leaf_pairs = h.leafpairs;
+        ctx->counters.tri_tests = h.tritests;
+        ctx->counters.contacts = h.contacts;
+        ctx->counters.ub_cases = h.ub;
+        *n_contacts = h.contacts;
+        return CD_OK;
+    }
+    return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
+}
+
+int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
+    if (mode == CD_MODE_PLAIN) return run_plain(ctx, nq, n_contacts);
+    return fail(ctx, CD_E_INVALID, "unknown mode");
+}
+
+int stage(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
+    if (nq && !q) return fail(ctx, CD_E_INVALID, "null query array");
+    const QueryRec* qr = reinterpret_cast<const QueryRec*>(q);
+    int rc = check_queries(ctx, qr, nq);
+    if (rc) return rc;
+    HCK(ctx, ctx->d_queries.ensure(size_t(std::max<uint32_t>(nq, 1)) * sizeof(QueryRec)));
+    if (nq) HCK(ctx, hipMemcpyAsync(ctx->d_queries.p, qr, size_t(nq) * sizeof(QueryRec), hipMemcpyHostToDevice, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->staged_nq = nq;
+    return CD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cd_create(cd_ctx** out, const cd_config* cfg) {
+    if (!out) return CD_E_INVALID;
+    *out = nullptr;
+    cd_ctx* ctx = new (std::nothrow) cd_ctx();
+    if (!ctx) return CD_E_NOMEM;
+    if (cfg) {
+        ctx->leaf_max = cfg->leaf_max ? cfg->leaf_max : 4;
+        ctx->device = int(cfg->device);
+    }
+    if (ctx->leaf_max > 64) {
+        delete ctx;
+        return CD_E_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ctx->device >= ndev) {
+        delete ctx;
+        return CD_E_HIP;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CD_E_HIP;
+    }
+    for (auto& e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return CD_E_HIP;
+        }
+    if (ctx->d_counters.ensure(sizeof(RunCounters)) != hipSuccess) {
+        delete ctx;
+        return CD_E_NOMEM;
+    }
+    *out = ctx;
+    return CD_OK;
+}
+
+int cd_destroy(cd_ctx* ctx) {
+    if (!ctx) return CD_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : {&ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_perm, &ctx->d_queries, &ctx->d_leafpairs,
+                      &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg})
+        b->release();
+    for (auto& b : ctx->d_work) b.release();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CD_OK;
+}
+
+const char* cd_last_error(const cd_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cd_register_mesh(cd_ctx* ctx, const float* positions, uint32_t nvert, const uint32_t* indices, uint32_t ntri,
+                     uint32_t* mesh_id) {
+    if (!ctx) return CD_E_INVALID;
+    if (ntri && (!positions || !indices)) return fail(ctx, CD_E_INVALID, "null mesh arrays");
+    std::vector<float> soup(size_t(ntri) * 9);
+    for (uint32_t t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = indices[size_t(t) * 3 + k];
+            if (vi >= nvert) return fail(ctx, CD_E_INVALID, "index out of range");
+            std::memcpy(&soup[size_t(t) * 9 + k * 3], positions + size_t(vi) * 3, 3 * sizeof(float));
+        }
+    return register_soups(ctx, soup.data(), &ntri, 1, mesh_id);
+}
+
+int cd_register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* tri_counts, uint32_t nmesh, uint32_t* first_id) {
+    if (!ctx) return CD_E_INVALID;
+    if (nmesh && !tri_counts) return fail(ctx, CD_E_INVALID, "null counts");
+    (void)hipSetDevice(ctx->device);
+    return register_soups(ctx, tri9, tri_counts, nmesh, first_id);
+}
+
+int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out) {
+    if (!ctx || !out) return CD_E_INVALID;
+    if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");
+    const MeshRec& m = ctx->meshes[mesh];
+    out->ntri = m.ntri;
+    out->nnodes = m.nnodes;
+    out->depth = m.depth;
+    out->leaf_max = ctx->leaf_max;
+    std::memcpy(out->root, m.root, sizeof out->root);
+    out->slack = m.slack;
+    out->radius = m.radius;
+    return CD_OK;
+}
+
+int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, uint32_t* node_links, uint32_t* perm) {
+    if (!ctx) return CD_E_INVALID;
+    if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");
+    const MeshRec& m = ctx->meshes[mesh];
+    if (root12) std::memcpy(root12, m.root, sizeof m.root);
+    if (m.nnodes && (node_obbs || node_links)) {
+        std::vector<NodeRec> recs(m.nnodes);
+        HCK(ctx, hipMemcpyAsync(recs.data(), ctx->d_nodes.as<NodeRec>() + m.node_base, m.nnodes * sizeof(NodeRec),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        HCK(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < m.nnodes; ++i) {
+            if (node_obbs) {
+                std::memcpy(node_obbs + size_t(i) * 24, recs[i].box[0], 12 * sizeof(float));
+                std::memcpy(node_obbs + size_t(i) * 24 + 12, recs[i].box[1], 12 * sizeof(float));
+            }
+            if (node_links) {
+                node_links[size_t(i) * 4 + 0] = recs[i].link[0];
+                node_links[size_t(i) * 4 + 1] = recs[i].link[1];
+                node_links[size_t(i) * 4 + 2] = recs[i].cnt & 0xFFFFu;
+                node_links[size_t(i) * 4 + 3] = recs[i].cnt >> 16;
+            }
+        }
+    }
+    if (perm && m.ntri) {
+        HCK(ctx, hipMemcpyAsync(perm, ctx->d_perm.as<uint32_t>() + m.tri_base, m.ntri * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        HCK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return CD_OK;
+}
+
+int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode, int32_t* flags, float* seg6) {
+    if (!ctx) return CD_E_INVALID;
+    if (n && (!a9 || !b9 || !flags)) return fail(ctx, CD_E_INVALID, "null argument");
+    if (mode != CD_MODE_PLAIN) return fail(ctx, CD_E_INVALID, "unknown mode");
+    hipStream_t st = ctx->stream;
+    HCK(ctx, ctx->d_tt_a.ensure(n * 9 * sizeof(float) + 4));
+    HCK(ctx, ctx->d_tt_b.ensure(n * 9 * sizeof(float) + 4));
+    HCK(ctx, ctx->d_tt_flags.ensure(n * sizeof(int32_t) + 4));
+    HCK(ctx, ctx->d_tt_seg.ensure(n * 6 * sizeof(float) + 4));
+    if (n) {
+        HCK(ctx, hipMemcpyAsync(ctx->d_tt_a.p, a9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_tt_b.p, b9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    HCK(ctx, hipEventRecord(ctx->ev[0], st));
+    HCK(ctx, launch_tritri_plain(ctx->d_tt_a.as<float>(), ctx->d_tt_b.as<float>(), n, ctx->d_tt_flags.as<int32_t>(),
+                                 seg6 ? ctx->d_tt_seg.as<float>() : nullptr, st));
+    HCK(ctx, hipEventRecord(ctx->ev[1], st));
+    if (n) {
+        HCK(ctx, hipMemcpyAsync(flags, ctx->d_tt_flags.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        if (seg6) HCK(ctx, hipMemcpyAsync(seg6, ctx->d_tt_seg.p, n * 6 * sizeof(float), hipMemcpyDeviceToHost, st));
+    }
+    HCK(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->timings = cd_timings{};
+    ctx->timings.total_ms = ms;
+    ctx->timings.narrow_ms = ms;
+    ctx->timings.launches = 1;
+    return CD_OK;
+}
+
+int cd_query_batch(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t mode, cd_contact* out, uint64_t cap,
+                   uint64_t* n_out) {
+    if (!ctx || !n_out) return CD_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    int rc = stage(ctx, q, nq);
+    if (rc) return rc;
+    uint64_t n = 0;
+    rc = run_queries(ctx, nq, mode, &n);
+    if (rc) return rc;
+    ctx->staged_contacts = n;
+    *n_out = n;
+    if (n > cap || (n && !out)) return fail(ctx, CD_E_CAPACITY, "contact buffer too small");
+    if (n) HCK(ctx, hipMemcpyAsync(out, ctx->d_contacts.p, n * sizeof(cd_contact), hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+int cd_query_leafpairs(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t* out4, uint64_t cap, uint64_t* n_out) {
+    if (!ctx || !n_out) return CD_E_INVALID;
+    int rc = stage(ctx, q, nq);
+    if (rc) return rc;
+    uint64_t n = 0;
+    rc = run_plain(ctx, nq, &n);
+    if (rc) return rc;
+    const uint64_t nlp = ctx->counters.leaf_pairs;
+    *n_out = nlp;
+    if (nlp > cap || (nlp && !out4)) return fail(ctx, CD_E_CAPACITY, "leaf-pair buffer too small");
+    if (nlp) HCK(ctx, hipMemcpyAsync(out4, ctx->d_leafpairs.p, nlp * sizeof(uint4), hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+int cd_stage_queries(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
+    if (!ctx) return CD_E_INVALID;
+    return stage(ctx, q, nq);
+}
+
+int cd_run_staged(cd_ctx* ctx, uint32_t mode, uint64_t* n_contacts) {
+    if (!ctx || !n_contacts) return CD_E_INVALID;
+    uint64_t n = 0;
+    int rc = run_queries(ctx, ctx->staged_nq, mode, &n);
+    if (rc) return rc;
+    ctx->staged_contacts = n;
+    *n_contacts = n;
+    return CD_OK;
+}
+
+int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_out) {
+    if (!ctx || !n_out) return CD_E_INVALID;
+    const uint64_t n = ctx->staged_contacts;
+    *n_out = n;
+    if (n > cap || (n && !out)) return fail(ctx, CD_E_CAPACITY, "contact buffer too small");
+    if (n) HCK(ctx, hipMemcpyAsync(out, ctx->d_contacts.p, n * sizeof(cd_contact), hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+int cd_frame(cd_ctx* ctx, const cd_entity*, uint32_t, uint32_t, cd_pair*, uint32_t, uint32_t*, cd_contact*, uint64_t,
+             uint64_t*) {
+    return fail(ctx, CD_E_INVALID, "cd_frame: broad phase not available in this build");
+}
+
+int cd_broad_phase(cd_ctx* ctx, const cd_entity*, uint32_t, uint32_t*, uint32_t, uint32_t*) {
+    return fail(ctx, CD_E_INVALID, "cd_broad_phase: not available in this build");
+}
+
+int cd_get_counters(cd_ctx* ctx, cd_counters* out) {
+    if (!ctx || !out) return CD_E_INVALID;
+    *out = ctx->counters;
+    return CD_OK;
+}
+
+int cd_get_timings(cd_ctx* ctx, cd_timings* out) {
+    if (!ctx || !out) return CD_E_INVALID;
+    *out = ctx->timings;
+    return CD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+// 3x3 symmetric eigen-solver in FP64 for per-node PCA on the GPU.
+//
+// Same semantics and operation order as eig3/eig3.cpp:21-265 of the
+// reference (JAMA / EISPACK tred2 + tql2, eps = 2^-52, no iteration cap,
+// eigenvalues ascending, eigenvectors in the columns of V), so that under
+// -ffp-contract=off the device result is bit-identical to the host one.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+namespace cdk {
+
+__host__ __device__ inline void eig3_tridiag(double V[3][3], double d[3], double e[3]) {
+    const int n = 3;
+    for (int j = 0; j < n; ++j) d[j] = V[n - 1][j];
+    for (int i = n - 1; i > 0; --i) {
+        double scale = 0.0, h = 0.0;
+        for (int k = 0; k < i; ++k) scale = scale + fabs(d[k]);
+        if (scale == 0.0) {
+            e[i] = d[i - 1];
+            for (int j = 0; j < i; ++j) {
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+                V[j][i] = 0.0;
+            }
+        } else {
+            for (int k = 0; k < i; ++k) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            double f = d[i - 1];
+            double g = sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h = h - f * g;
+            d[i - 1] = f - g;
+            for (int j = 0; j < i; ++j) e[j] = 0.0;
+            for (int j = 0; j < i; ++j) {
+                f = d[j];
+                V[j][i] = f;
+                g = e[j] + V[j][j] * f;
+                for (int k = j + 1; k <= i - 1; ++k) {
+                    g += V[k][j] * d[k];
+                    e[k] += V[k][j] * f;
+                }
+                e[j] = g;
+            }
+            f = 0.0;
+            for (int j = 0; j < i; ++j) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            double hh = f / (h + h);
+            for (int j = 0; j < i; ++j) e[j] -= hh * d[j];
+            for (int j = 0; j < i; ++j) {
+                f = d[j];
+                g = e[j];
+                for (int k = j; k <= i - 1; ++k) V[k][j] -= (f * e[k] + g * d[k]);
+                d[j] = V[i - 1][j];
+                V[i][j] = 0.0;
+            }
+        }
+        d[i] = h;
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        V[n - 1][i] = V[i][i];
+        V[i][i] = 1.0;
+        double h = d[i + 1];
+        if (h != 0.0) {
+            for (int k = 0; k <= i; ++k) d[k] = V[k][i + 1] / h;
+            for (int j = 0; j <= i; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= i; ++k) g += V[k][i + 1] * V[k][j];
+                for (int k = 0; k <= i; ++k) V[k][j] -= g * d[k];
+            }
+        }
+        for (int k = 0; k <= i; ++k) V[k][i + 1] = 0.0;
+    }
+    for (int j = 0; j < n; ++j) {
+        d[j] = V[n - 1][j];
+        V[n - 1][j] = 0.0;
+    }
+    V[n - 1][n - 1] = 1.0;
+    e[0] = 0.0;
+}
+
+__host__ __device__ inline double eig3_hyp(double x, double y) { return sqrt(x * x + y * y); }
+
+__host__ __device__ inline void eig3_ql(double V[3][3], double d[3], double e[3]) {
+    const int n = 3;
+    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+    e[n - 1] = 0.0;
+    double f = 0.0, tst1 = 0.0;
+    const double eps = 2.220446049250313080847263336181640625e-16;  // 2^-52
+    for (int l = 0; l < n; ++l) {
+        double cand = fabs(d[l]) + fabs(e[l]);
+        tst1 = (tst1 > cand) ? tst1 : cand;
+        int m = l;
+        while (m < n) {
+            if (fabs(e[m]) <= eps * tst1) break;
+            ++m;
+        }
+        if (m > l) {
+            do {
+                double g = d[l];
+                double p = (d[l + 1] - g) / (2.0 * e[l]);
+                double r = eig3_hyp(p, 1.0);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                double dl1 = d[l + 1];
+                double h = g - d[l];
+                for (int i = l + 2; i < n; ++i) d[i] -= h;
+                f = f + h;
+                p = d[m];
+                double c = 1.0, c2 = c, c3 = c;
+                double el1 = e[l + 1];
+                double s = 0.0, s2 = 0.0;
+                for (int i = m - 1; i >= l; --i) {
+                    c3 = c2;
+                    c2 = c;
+                    s2 = s;
+                    g = c * e[i];
+                    h = c * p;
+                    r = eig3_hyp(p, e[i]);
+                    e[i + 1] = s * r;
+                    s = e[i] / r;
+                    c = p / r;
+                    p = c * d[i] - s * g;
+                    d[i + 1] = h + s * (c * g + s * d[i]);
+                    for (int k = 0; k < n; ++k) {
+                        h = V[k][i + 1];
+                        V[k][i + 1] = s * V[k][i] + c * h;
+                        V[k][i] = c * V[k][i] - s * h;
+                    }
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
+            } while (fabs(e[l]) > eps * tst1);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0.0;
+    }
+    for (int i = 0; i < n - 1; ++i) {
+        int k = i;
+        double p = d[i];
+        for (int j = i + 1; j < n; ++j)
+            if (d[j] < p) {
+                k = j;
+                p = d[j];
+            }
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int j = 0; j < n; ++j) {
+                p = V[j][i];
+                V[j][i] = V[j][k];
+                V[j][k] = p;
+            }
+        }
+    }
+}
+
+__host__ __device__ inline void eig3(const double A[3][3], double V[3][3], double d[3]) {
+    double e[3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) V[i][j] = A[i][j];
+    eig3_tridiag(V, d, e);
+    eig3_ql(V, d, e);
+}
+
+}  // namespace cdk

@@ -1,0 +1,233 @@
+// Exact-sign orientation predicate for float32 inputs and the exact-semantics
+// triangle-triangle test built on it.
+//
+// side(a,b,c,d) = sign( ((a-c) x (b-c)) . (d-c) ).
+// Stage 1: float32 evaluation with a forward error bound (semi-static filter
+//          in the style of Shewchuk 1997, "Adaptive Precision Floating-Point
+//          Arithmetic and Fast Robust Geometric Predicates");
+// Stage 2: the same expression in float64 with its bound;
+// Stage 3: exact — the determinant expanded over the original coordinates is
+//          a sum of 24 products of three floats; each product is two doubles
+//          exactly (the float*float product is exact in double, the third
+//          factor goes through an FMA-based two-product), and the 48 doubles
+//          are summed exactly as a non-overlapping expansion with zero
+//          elimination (Shewchuk's GROW-EXPANSION).  The sign of the sum is
+//          the sign of its most significant component.
+// Only stage 1 runs for almost every predicate; FP64 is confined to the
+// predicates whose float32 value is inside its error bound.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "geom.hpp"
+
+namespace cdk {
+
+struct PredStats {
+    uint32_t fp64 = 0, exact = 0;
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double& x, double& y) {
+    x = a + b;
+    double bv = x - a;
+    double av = x - bv;
+    double br = b - bv;
+    double ar = a - av;
+    y = ar + br;
+}
+
+// Sum of +-u_i*v_j*w_k terms, exactly.
+__device__ __noinline__ int orient_exact_slow(f3 a, f3 b, f3 c, f3 d) {
+    double e[48];
+    int n = 0;
+    auto grow = [&](double q) {
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            double s, h;
+            two_sum(q, e[i], s, h);
+            q = s;
+            if (h != 0.0) e[m++] = h;
+        }
+        if (q != 0.0) e[m++] = q;
+        n = m;
+    };
+    auto triple = [&](f3 u, f3 v, f3 w, double sg) {
+        // u . (v x w) = ux(vy wz - vz wy) + uy(vz wx - vx wz) + uz(vx wy - vy wx)
+        const double ux = u.x, uy = u.y, uz = u.z, vx = v.x, vy = v.y, vz = v.z, wx = w.x, wy = w.y, wz = w.z;
+        const double t[6][3] = {{ux * vy, wz, 1}, {ux * vz, wy, -1}, {uy * vz, wx, 1},
+                                {uy * vx, wz, -1}, {uz * vx, wy, 1}, {uz * vy, wx, -1}};
+        for (int k = 0; k < 6; ++k) {
+            double p = t[k][0] * t[k][1];
+            double err = fma(t[k][0], t[k][1], -p);
+            double s = sg * t[k][2];
+            grow(s * p);
+            grow(s * err);
+        }
+    };
+    // [a-c, b-c, d-c] = [a,b,d] - [a,b,c] - [a,c,d] + [b,c,d]
+    triple(a, b, d, 1.0);
+    triple(a, b, c, -1.0);
+    triple(a, c, d, -1.0);
+    triple(b, c, d, 1.0);
+    for (int i = n - 1; i >= 0; --i)
+        if (e[i] != 0.0) return e[i] > 0.0 ? 1 : -1;
+    return 0;
+}
+
+__device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats& st) {
+    {
+        const float ax = a.x - c.x, ay = a.y - c.y, az = a.z - c.z;
+        const float bx = b.x - c.x, by = b.y - c.y, bz = b.z - c.z;
+        const float dx = d.x - c.x, dy = d.y - c.y, dz = d.z - c.z;
+        const float p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
+        const float det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
+        const float perm = (fabsf(p0) + fabsf(p1)) * fabsf(dx) + (fabsf(p2) + fabsf(p3)) * fabsf(dy) +
+                           (fabsf(p4) + fabsf(p5)) * fabsf(dz);
+        const float bound = 9.5367431640625e-07f * perm + 1e-30f;  // 16 * 2^-24
+        if (det > bound) return 1;
+        if (det < -bound) return -1;
+    }
+    ++st.fp64;
+    {
+        const double ax = double(a.x) - c.x, ay = double(a.y) - c.y, az = double(a.z) - c.z;
+        const double bx = double(b.x) - c.x, by = double(b.y) - c.y, bz = double(b.z) - c.z;
+        const double dx = double(d.x) - c.x, dy = double(d.y) - c.y, dz = double(d.z) - c.z;
+        const double p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
+        const double det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
+        const double perm = (fabs(p0) + fabs(p1)) * fabs(dx) + (fabs(p2) + fabs(p3)) * fabs(dy) +
+                            (fabs(p4) + fabs(p5)) * fabs(dz);
+        const double bound = 1.7763568394002505e-15 * perm;  // 16 * 2^-53
+        if (det > bound) return 1;
+        if (det < -bound) return -1;
+    }
+    ++st.exact;
+    return orient_exact_slow(a, b, c, d);
+}
+
+// Guigue & Devillers 2003 decision structure ("Fast and robust
+// triangle-triangle overlap test using orientation predicates", JGT 8(1)),
+// every sign exact.  Returns 1 = intersect (closed triangles, not coplanar),
+// 0 = disjoint, 2 = coplanar.  dsign[0..2] / dsign[3..5] receive the exact
+// sides of U's vertices w.r.t. V's plane and of V's w.r.t. U's plane (used
+// to place the contact segment).
+__device__ __forceinline__ int gd_check(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, PredStats& st) {
+    if (orient_filtered(p2, p1, q1, q2, st) > 0) return 0;
+    if (orient_filtered(p2, r1, p1, r2, st) > 0) return 0;
+    return 1;
+}
+__device__ inline int gd_canon2(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, int dp2, int dq2, int dr2, PredStats& st) {
+    if (dp2 > 0) {
+        if (dq2 > 0) return gd_check(p1, r1, q1, r2, p2, q2, st);
+        if (dr2 > 0) return gd_check(p1, r1, q1, q2, r2, p2, st);
+        return gd_check(p1, q1, r1, p2, q2, r2, st);
+    }
+    if (dp2 < 0) {
+        if (dq2 < 0) return gd_check(p1, q1, r1, r2, p2, q2, st);
+        if (dr2 < 0) return gd_check(p1, q1, r1, q2, r2, p2, st);
+        return gd_check(p1, r1, q1, p2, q2, r2, st);
+    }
+    if (dq2 < 0) {
+        if (dr2 >= 0) return gd_check(p1, r1, q1, q2, r2, p2, st);
+        return gd_check(p1, q1, r1, p2, q2, r2, st);
+    }
+    if (dq2 > 0) {
+        if (dr2 > 0) return gd_check(p1, r1, q1, p2, q2, r2, st);
+        return gd_check(p1, q1, r1, q2, r2, p2, st);
+    }
+    if (dr2 > 0) return gd_check(p1, q1, r1, r2, p2, q2, st);
+    if (dr2 < 0) return gd_check(p1, r1, q1, r2, p2, q2, st);
+    return 2;
+}
+__device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, int dsign[6], PredStats& st) {
+    const int dp1 = orient_filtered(p2, q2, r2, p1, st);
+    const int dq1 = orient_filtered(p2, q2, r2, q1, st);
+    const int dr1 = orient_filtered(p2, q2, r2, r1, st);
+    dsign[3] = dp1;
+    dsign[4] = dq1;
+    dsign[5] = dr1;
+    if (dp1 * dq1 > 0 && dp1 * dr1 > 0) return 0;
+    const int dp2 = orient_filtered(q1, r1, p1, p2, st);
+    const int dq2 = orient_filtered(q1, r1, p1, q2, st);
+    const int dr2 = orient_filtered(q1, r1, p1, r2, st);
+    dsign[0] = dp2;
+    dsign[1] = dq2;
+    dsign[2] = dr2;
+    if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
+    if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
+    if (dp1 > 0) {
+        if (dq1 > 0) return gd_canon2(r1, p1, q1, p2, r2, q2, dp2, dr2, dq2, st);
+        if (dr1 > 0) return gd_canon2(q1, r1, p1, p2, r2, q2, dp2, dr2, dq2, st);
+        return gd_canon2(p1, q1, r1, p2, q2, r2, dp2, dq2, dr2, st);
+    }
+    if (dp1 < 0) {
+        if (dq1 < 0) return gd_canon2(r1, p1, q1, p2, q2, r2, dp2, dq2, dr2, st);
+        if (dr1 < 0) return gd_canon2(q1, r1, p1, p2, q2, r2, dp2, dq2, dr2, st);
+        return gd_canon2(p1, q1, r1, p2, r2, q2, dp2, dr2, dq2, st);
+    }
+    if (dq1 < 0) {
+        if (dr1 >= 0) return gd_canon2(q1, r1, p1, p2, r2, q2, dp2, dr2, dq2, st);
+        return gd_canon2(p1, q1, r1, p2, q2, r2, dp2, dq2, dr2, st);
+    }
+    if (dq1 > 0) {
+        if (dr1 > 0) return gd_canon2(p1, q1, r1, p2, r2, q2, dp2, dr2, dq2, st);
+        return gd_canon2(q1, r1, p1, p2, q2, r2, dp2, dq2, dr2, st);
+    }
+    if (dr1 > 0) return gd_canon2(r1, p1, q1, p2, q2, r2, dp2, dq2, dr2, st);
+    if (dr1 < 0) return gd_canon2(r1, p1, q1, p2, r2, q2, dp2, dr2, dq2, st);
+    return 2;
+}
+
+// Contact segment of an intersecting, non-coplanar pair, with Möller's
+// interval construction (Triangle.cpp:764-1001) driven by the exact sides:
+// a float distance whose sign disagrees with the exact side is replaced by a
+// tiny value of the exact sign, an exact zero becomes 0.  When the float
+// intervals fail to overlap (a touching pair), the segment collapses to the
+// midpoint of the closest ends.
+__device__ inline float signed_like(float v, int s) {
+    if (s == 0) return 0.0f;
+    if (s > 0) return v > 0.0f ? v : 1e-30f;
+    return v < 0.0f ? v : -1e-30f;
+}
+__device__ inline void exact_segment(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, const int ds[6], f3& s, f3& t) {
+    const f3 N1 = cross(sub(V1, V0), sub(V2, V0));
+    const float d1 = -dot(N1, V0);
+    const float du0 = signed_like(dot(N1, U0) + d1, ds[0]);
+    const float du1 = signed_like(dot(N1, U1) + d1, ds[1]);
+    const float du2 = signed_like(dot(N1, U2) + d1, ds[2]);
+    const f3 N2 = cross(sub(U1, U0), sub(U2, U0));
+    const float d2 = -dot(N2, U0);
+    const float dv0 = signed_like(dot(N2, V0) + d2, ds[3]);
+    const float dv1 = signed_like(dot(N2, V1) + d2, ds[4]);
+    const float dv2 = signed_like(dot(N2, V2) + d2, ds[5]);
+    const f3 D = cross(N1, N2);
+    float mx = fabsf(D.x);
+    int index = 0;
+    if (fabsf(D.y) > mx) { mx = fabsf(D.y); index = 1; }
+    if (fabsf(D.z) > mx) { index = 2; }
+    float i10, i11, i20, i21;
+    f3 A1, A2, B1, B2;
+    if (intervals(V0, V1, V2, comp(V0, index), comp(V1, index), comp(V2, index), dv0, dv1, dv2, dv0 * dv1, dv0 * dv2,
+                  i10, i11, A1, A2) ||
+        intervals(U0, U1, U2, comp(U0, index), comp(U1, index), comp(U2, index), du0, du1, du2, du0 * du1, du0 * du2,
+                  i20, i21, B1, B2)) {
+        s = t = V0;
+        return;
+    }
+    int sm1 = 0, sm2 = 0;
+    if (i10 > i11) { float q = i10; i10 = i11; i11 = q; sm1 = 1; }
+    if (i20 > i21) { float q = i20; i20 = i21; i21 = q; sm2 = 1; }
+    if (i11 < i20 || i21 < i10) {
+        const f3 a = (i11 < i20) ? (sm1 == 0 ? A2 : A1) : (sm2 == 0 ? B2 : B1);
+        const f3 b = (i11 < i20) ? (sm2 == 0 ? B1 : B2) : (sm1 == 0 ? A1 : A2);
+        s = t = mk(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z));
+        return;
+    }
+    if (i20 < i10) {
+        s = sm1 == 0 ? A1 : A2;
+        t = (i21 < i11) ? (sm2 == 0 ? B2 : B1) : (sm1 == 0 ? A2 : A1);
+    } else {
+        s = sm2 == 0 ? B1 : B2;
+        t = (i21 > i11) ? (sm1 == 0 ? A2 : A1) : (sm2 == 0 ? B2 : B1);
+    }
+}
+
+}  // namespace cdk

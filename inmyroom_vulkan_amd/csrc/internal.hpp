@@ -1,0 +1,125 @@
+// Internal layouts of the collision library (device arenas and records).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mi355x_collide.h"
+#include "geom.hpp"
+
+namespace cdk {
+
+// One internal OBB-tree node, 128 B (one L2 line).  Same content as the
+// reference's OBBtreeNode (include/Geometry/OBBtree.h:61-75) plus the surface
+// of each child box, which the traversal's descent rule compares
+// (OBBtree.cpp:426) and which is a pure function of the stored box.
+struct alignas(128) NodeRec {
+    float box[2][12];   // left, right child boxes: centre, u, v, w
+    uint32_t link[2];   // child node index (count 0) or mesh-local triangle offset
+    uint32_t cnt;       // left count | right count << 16 (0 = internal child)
+    float surf[2];      // surface(box[0]), surface(box[1])
+    uint32_t pad[3];
+};
+static_assert(sizeof(NodeRec) == 128, "node record must stay one 128-byte line");
+
+// Mesh table entry (device and host).
+struct MeshRec {
+    uint32_t node_base;   // first NodeRec of this mesh in the node arena
+    uint32_t tri_base;    // first triangle of this mesh in the triangle arena (tree order)
+    uint32_t ntri, nnodes;
+    float root[12];       // root OBB (OBBtree.h:129)
+    float root_surf;
+    uint32_t depth;       // internal levels below the root
+    float slack;          // max distance a vertex lies outside a box meant to hold it
+    float radius;         // max |vertex|
+};
+static_assert(sizeof(MeshRec) == 80, "mesh record layout");
+
+// Device copy of cd_query.
+struct QueryRec {
+    uint32_t mesh_a, mesh_b;
+    float rel[16];
+};
+static_assert(sizeof(QueryRec) == sizeof(cd_query), "query layout");
+
+constexpr uint32_t ROOT_CODE = 0xFFFFFFFFu;  // traversal code of a tree's root
+constexpr int PLAIN_STACK = 64;              // per-lane stack entries of the plain traversal
+
+__host__ __device__ inline Box box_from12(const float* p) {
+    Box b;
+    b.c = mk(p[0], p[1], p[2]);
+    b.u = mk(p[3], p[4], p[5]);
+    b.v = mk(p[6], p[7], p[8]);
+    b.w = mk(p[9], p[10], p[11]);
+    return b;
+}
+__host__ __device__ inline void box_to12(const Box& b, float* p) {
+    p[0] = b.c.x; p[1] = b.c.y; p[2] = b.c.z;
+    p[3] = b.u.x; p[4] = b.u.y; p[5] = b.u.z;
+    p[6] = b.v.x; p[7] = b.v.y; p[8] = b.v.z;
+    p[9] = b.w.x; p[10] = b.w.y; p[11] = b.w.z;
+}
+
+// Growable device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t want = need < 4096 ? 4096 : need;
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device-side counters of one run (zeroed by a fill kernel, never by a
+// memset, so a run can be captured into a HIP graph).
+struct RunCounters {
+    unsigned long long sat, leafpairs, tritests, contacts, fp64, exact, ub, err;
+};
+
+// Build of a batch of triangle soups into the arenas (build.hip).
+struct BuildResult {
+    std::vector<MeshRec> meshes;   // node_base / tri_base relative to the arenas
+    std::string err;
+};
+
+}  // namespace cdk
+
+// The context (opaque in the C ABI).
+struct cd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t leaf_max = 4;
+    std::string err;
+
+    std::vector<cdk::MeshRec> meshes;      // host copy of the mesh table
+    cdk::DevBuf d_meshes, d_nodes, d_tris, d_perm;
+    size_t nodes_used = 0, tris_used = 0;
+
+    cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg;
+    cdk::DevBuf d_work[8];                  // scratch of the batched path and the broad phase
+    uint32_t staged_nq = 0;
+    uint64_t staged_contacts = 0;
+    uint64_t lp_cap = 0, contact_cap = 0;
+
+    cd_counters counters{};
+    cd_timings timings{};
+    hipEvent_t ev[8] = {};
+};

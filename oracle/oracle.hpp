@@ -1,0 +1,66 @@
+// TEST INFRASTRUCTURE ONLY — the oracle.  Nothing in the shipped library may
+// include, link or call this code; tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py use it as the checker.
+//
+// CPU restatement of inMyRoom_vulkan's collision path (broad phase, OBB-tree
+// build, OBB-tree traversal, Möller triangle-triangle test, contact output),
+// written from the reference's behaviour.  Strict IEEE semantics: build with
+// -O2 -ffp-contract=off and no fast-math, so every float/double operation is
+// rounded exactly once, in the order written.
+//
+// glm (an un-vendored submodule of the reference, fork thesmallcreeper/glm,
+// pinned commit unknown; public glm 0.9.9 semantics) is restated here as the
+// handful of formulas the path uses.  Their expression order follows glm 0.9.9:
+//   dot(a,b)        = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   cross(a,b)      = (a.y*b.z - b.y*a.z, a.z*b.x - b.z*a.x, a.x*b.y - b.x*a.y)
+//   length(v)       = sqrt(dot(v,v));  normalize(v) = v * (1 / sqrt(dot(v,v)))
+//   mat4 * vec4     = (m0*x + m1*y) + (m2*z + m3*w)           (pairwise)
+//   mat4 * mat4     = ((A0*b0 + A1*b1) + A2*b2) + A3*b3       (per column)
+//   inverse(mat4)   = cofactor expansion, 1/det scaling
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+
+namespace orc {
+
+struct V3 { float x, y, z; };
+struct D3 { double x, y, z; };
+struct M4 { float m[16]; };  // column-major: m[c*4 + r]
+
+// A "Paralgram" / OBB: centre plus three half-extent side vectors
+// (inMyRoom_vulkan/include/Geometry/Paralgram.h:29-35).
+struct Box { V3 c, u, v, w; };
+
+// One flattened internal node (inMyRoom_vulkan/include/Geometry/OBBtree.h:61-75).
+struct Node {
+    Box obb[2];          // [0] = left child, [1] = right child
+    uint32_t link[2];    // child node index (cnt==0) or triangle offset (cnt>0)
+    uint16_t cnt[2];     // 0 = internal child, else triangles in the leaf
+};
+
+struct Tree {
+    Box root;
+    std::vector<Node> nodes;       // DFS preorder of internal nodes; node 0 = root
+    std::vector<uint32_t> perm;    // tree-order slot -> input triangle index
+    std::vector<float> pos;        // tree-order triangle positions, 9 floats each
+    int depth = 0;                 // levels below the root (root leaf = 0)
+    double slack = 0.0;            // max distance of a vertex outside a box that should contain it
+    double radius = 0.0;           // max |vertex| in model space
+};
+
+struct LeafPair { uint32_t offA, cntA, offB, cntB; };
+
+// Contact record: one colliding (intersecting, non-coplanar) triangle pair,
+// expressed in the first entity's model space.
+struct Contact {
+    uint32_t query;
+    uint32_t triA, triB;  // tree-order triangle slots
+    uint32_t flags;       // bits 0-2: A vertex k outside B's plane; bits 3-5: B vertex k outside A's plane
+    float s[3], t[3];     // intersection segment (source, target)
+    float na[3], nb[3];   // unit face normals of A's and (transformed) B's triangle
+};
+
+struct Counters { uint64_t sat = 0, leafpairs = 0, tritests = 0, contacts = 0, ub = 0; };
+
+}  // namespace orc
