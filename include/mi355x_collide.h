@@ -111,6 +111,18 @@ typedef struct cd_timings {   /* device time of the last run, from HIP events */
     uint32_t pad;
 } cd_timings;
 
+/* Launch parameters of the batched path (no counterpart in the reference:
+ * the reference has no GPU).  Chosen per workload by the autotuner
+ * (inmyroom_vulkan_amd/autotune.py); zero fields mean "default".            */
+typedef struct cd_tuning {
+    uint32_t split_levels;   /* breadth-first levels before the persistent kernel (work-split granularity) */
+    uint32_t stack_entries;  /* LDS stack entries per wavefront: 512, 1024 or 2048 */
+    uint32_t pop_width;      /* node pairs a wavefront pops per step, 1..64 */
+    uint32_t waves_per_cu;   /* persistent wavefronts per compute unit */
+    uint32_t use_graph;      /* 1: capture the batched pipeline in a HIP graph and replay it */
+    uint32_t reserved[3];
+} cd_tuning;
+
 /* ---- context --------------------------------------------------------------- */
 /* Replaces CollisionDetection::CollisionDetection (src/CollisionDetection/CollisionDetection.cpp:5-26). */
 int cd_create(cd_ctx** out, const cd_config* cfg);
@@ -179,6 +191,9 @@ int cd_run_staged(cd_ctx* ctx, uint32_t mode, uint64_t* n_contacts);
 int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_out);
 
 int cd_get_counters(cd_ctx* ctx, cd_counters* out);
+/* Tuning of the batched path (no counterpart in the reference). */
+int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t);
+int cd_get_tuning(cd_ctx* ctx, cd_tuning* t);
 int cd_get_timings(cd_ctx* ctx, cd_timings* out);
 
 #ifdef __cplusplus

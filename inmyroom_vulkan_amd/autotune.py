@@ -1,0 +1,104 @@
+"""Autotuner of the batched path's launch parameters.
+
+Searches split levels (work-split granularity), LDS stack entries per
+wavefront (stack depth), node pairs popped per step (lanes per step of the
+wavefront traversal) and persistent wavefronts per CU, timing each candidate
+with the library's own HIP events on staged queries, and caches the winner
+per (device architecture, workload size class) in tuning_gfx950.json.
+
+    python -m inmyroom_vulkan_amd.autotune --nq 4096
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import math
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning_gfx950.json")
+
+SPACE = {
+    "split_levels": [4, 6, 8, 10, 12],
+    "stack_entries": [512, 1024, 2048],
+    "pop_width": [32, 64],
+    "waves_per_cu": [4, 8, 16],
+}
+
+
+def size_class(nq: int) -> str:
+    return f"nq<=2^{max(0, math.ceil(math.log2(max(nq, 1))))}"
+
+
+def lookup(nq: int) -> dict | None:
+    if not os.path.exists(TABLE):
+        return None
+    with open(TABLE) as f:
+        table = json.load(f)
+    entry = table.get(size_class(nq))
+    return dict(entry["tuning"]) if entry else None
+
+
+def tune(gpu, queries, reps: int = 3, space: dict | None = None, log=print) -> dict:
+    """Time every candidate on the staged queries; return the fastest."""
+    from .collide import MODE_BATCHED
+
+    space = space or SPACE
+    gpu.stage(queries)
+    best, best_ms = None, math.inf
+    results = []
+    keys = list(space)
+    for combo in itertools.product(*(space[k] for k in keys)):
+        t = dict(zip(keys, combo))
+        gpu.set_tuning(**t, use_graph=0)
+        gpu.run_staged(MODE_BATCHED)  # warm
+        ms = []
+        for _ in range(reps):
+            gpu.run_staged(MODE_BATCHED)
+            ms.append(gpu.timings()["total_ms"])
+        m = sorted(ms)[len(ms) // 2]
+        results.append((m, t))
+        if m < best_ms:
+            best, best_ms = t, m
+        log(f"{t} -> {m:.3f} ms")
+    gpu.set_tuning()
+    return {"tuning": best, "ms": best_ms, "candidates": len(results)}
+
+
+def save(nq: int, result: dict) -> None:
+    table = {}
+    if os.path.exists(TABLE):
+        with open(TABLE) as f:
+            table = json.load(f)
+    table[size_class(nq)] = {"tuning": result["tuning"], "ms": result["ms"], "when": time.strftime("%Y-%m-%d")}
+    with open(TABLE, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def main():
+    import numpy as np
+
+    from . import QUERY_DTYPE, Collider, scenes
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--nq", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--quick", action="store_true")
+    args = ap.parse_args()
+    soups, qa, qb, rel, _ = scenes.config3(args.nq, seed=args.seed)
+    gpu = Collider(leaf_max=4)
+    ids = np.array(gpu.register_soups(soups), np.uint32)
+    q = np.zeros(args.nq, QUERY_DTYPE)
+    q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
+    space = None
+    if args.quick:
+        space = {"split_levels": [6, 10], "stack_entries": [1024], "pop_width": [64], "waves_per_cu": [8]}
+    res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
+    save(args.nq, res)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

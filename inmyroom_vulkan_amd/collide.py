@@ -28,6 +28,7 @@ EXPORTS = (
     "cd_create", "cd_destroy", "cd_last_error", "cd_register_mesh", "cd_register_soups", "cd_mesh_info_get",
     "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
     "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
+    "cd_set_tuning", "cd_get_tuning",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -60,6 +61,11 @@ class cd_counters(C.Structure):
 class cd_timings(C.Structure):
     _fields_ = [("total_ms", C.c_float), ("traverse_ms", C.c_float), ("narrow_ms", C.c_float),
                 ("broad_ms", C.c_float), ("launches", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class cd_tuning(C.Structure):
+    _fields_ = [("split_levels", C.c_uint32), ("stack_entries", C.c_uint32), ("pop_width", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class CollideError(RuntimeError):
@@ -99,6 +105,8 @@ def load_library() -> C.CDLL:
     lib.cd_fetch_contacts.argtypes = [vp, vp, u64, P(u64)]
     lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
     lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
+    lib.cd_set_tuning.argtypes = [vp, P(cd_tuning)]
+    lib.cd_get_tuning.argtypes = [vp, P(cd_tuning)]
     for name in EXPORTS:
         if name != "cd_last_error":
             getattr(lib, name).restype = C.c_int
@@ -274,6 +282,16 @@ class Collider:
         c = cd_counters()
         self._check(self.lib.cd_get_counters(self.h, C.byref(c)), "cd_get_counters")
         return {f: getattr(c, f) for f, _ in cd_counters._fields_}
+
+    def set_tuning(self, split_levels=0, stack_entries=0, pop_width=0, waves_per_cu=0, use_graph=0):
+        t = cd_tuning(split_levels=split_levels, stack_entries=stack_entries, pop_width=pop_width,
+                      waves_per_cu=waves_per_cu, use_graph=use_graph)
+        self._check(self.lib.cd_set_tuning(self.h, C.byref(t)), "cd_set_tuning")
+
+    def tuning(self) -> dict:
+        t = cd_tuning()
+        self._check(self.lib.cd_get_tuning(self.h, C.byref(t)), "cd_get_tuning")
+        return {f: getattr(t, f) for f, _ in cd_tuning._fields_ if f != "reserved"}
 
     def timings(self) -> dict:
         t = cd_timings()

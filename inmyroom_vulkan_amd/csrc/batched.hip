@@ -1,0 +1,511 @@
+// Batched path: thousands of uneven mesh-pair queries in a few launches.
+//
+//  k_setup        per query: relative matrix and the slack that makes the
+//                 float32 separating-axis test conservative
+//  k_seed         one root node pair per query
+//  k_split_level  breadth-first expansion of node pairs for the first
+//                 levels (work splitting: a large query becomes many tasks)
+//  k_dfs          persistent wavefronts; each pops up to 64 node pairs at a
+//                 time from a stack in LDS (wave-private spill to HBM), tests
+//                 them, pushes children with ballot/prefix compaction, and
+//                 takes new tasks from a global counter when it runs low
+//  k_narrow_exact candidate leaf pairs expanded densely to triangle tests
+//                 (wave prefix scan), exact-sign triangle test, contacts
+//                 compacted with one atomic per wavefront
+//
+// Node-pair semantics follow OBBtree::IntersectOBBtreesRecursive
+// (inMyRoom_vulkan/src/Geometry/OBBtree.cpp:414-477) with the separating-axis
+// test of Paralgram.cpp:17-173 made conservative: an axis separates only when
+// the float gap exceeds a forward error bound plus the containment slack of
+// both trees, so no pair of triangles that truly intersect is culled.
+#include "exact.hpp"
+#include "kernels.hpp"
+
+namespace cdk {
+
+namespace {
+
+__device__ __forceinline__ unsigned long long lanes_below_b() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+__device__ __forceinline__ unsigned long long reserve_b(unsigned long long mask, unsigned long long* counter) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<unsigned long long>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    return base + static_cast<unsigned long long>(__popcll(mask & lanes_below_b()));
+}
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+constexpr float U32 = 5.9604644775390625e-08f;  // 2^-24
+
+__device__ __forceinline__ float mag1(const Box& b) {
+    return (fabsf(b.c.x) + fabsf(b.c.y) + fabsf(b.c.z)) + (fabsf(b.u.x) + fabsf(b.u.y) + fabsf(b.u.z)) +
+           (fabsf(b.v.x) + fabsf(b.v.y) + fabsf(b.v.z)) + (fabsf(b.w.x) + fabsf(b.w.y) + fabsf(b.w.z));
+}
+
+// Conservative 15-axis test (same axes and order as Paralgram.cpp:17-173).
+__device__ __forceinline__ bool axis_may_overlap(const Box& l, const Box& r, f3 a, float k) {
+    float l0, l1, r0, r1;
+    project(l, a, l0, l1);
+    project(r, a, r0, r1);
+    const float tol = (fabsf(a.x) + fabsf(a.y) + fabsf(a.z)) * k;
+    return !((r0 - l1 > tol) || (l0 - r1 > tol));
+}
+__device__ inline bool sat_conservative(const Box& l, const Box& r, float sigma) {
+    const float k = 16.f * U32 * (mag1(l) + mag1(r)) + sigma;
+    if (!axis_may_overlap(l, r, cross(l.v, l.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.u, l.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.u, l.v), k)) return false;
+    if (!axis_may_overlap(l, r, cross(r.v, r.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(r.u, r.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(r.u, r.v), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.u, r.u), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.u, r.v), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.u, r.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.v, r.u), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.v, r.v), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.v, r.w), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.w, r.u), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.w, r.v), k)) return false;
+    if (!axis_may_overlap(l, r, cross(l.w, r.w), k)) return false;
+    return true;
+}
+
+struct Code {  // resolved traversal code
+    Box box;
+    float surf;
+    bool leaf;
+    uint32_t link, cnt;
+};
+__device__ __forceinline__ Code resolve(const MeshRec* mesh, const NodeRec* nodes, uint32_t code) {
+    Code r;
+    if (code == ROOT_CODE) {
+        r.box = box_from12(mesh->root);
+        r.surf = mesh->root_surf;
+        r.leaf = mesh->nnodes == 0;
+        r.link = 0;
+        r.cnt = r.leaf ? mesh->ntri : 0;
+        return r;
+    }
+    const NodeRec& n = nodes[code >> 1];
+    const uint32_t side = code & 1;
+    r.box = box_from12(n.box[side]);
+    r.surf = n.surf[side];
+    r.link = n.link[side];
+    r.cnt = (n.cnt >> (16 * side)) & 0xFFFFu;
+    r.leaf = r.cnt != 0;
+    return r;
+}
+
+enum : int { PAIR_NONE = 0, PAIR_LEAF = 1, PAIR_SPLIT = 2 };
+
+// One node pair: conservative test, then leaf pair or the two child pairs
+// (first child on top), with the reference's descent rule.
+__device__ __forceinline__ int eval_pair(const QueryPre& Q, const MeshRec* meshes, const NodeRec* nodes, uint32_t ca,
+                                         uint32_t cb, uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b,
+                                         uint4& leaf, uint32_t qi) {
+    const Code A = resolve(&meshes[Q.mesh_a], nodes + Q.node_base_a, ca);
+    const Code B0 = resolve(&meshes[Q.mesh_b], nodes + Q.node_base_b, cb);
+    M4 M;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) M.m[k] = Q.m[k];
+    const Box sb = xform_box(M, B0.box);
+    if (!sat_conservative(A.box, sb, Q.sigma)) return PAIR_NONE;
+    if (A.leaf && B0.leaf) {
+        leaf = make_uint4(qi, A.link, B0.link, A.cnt | (B0.cnt << 16));
+        return PAIR_LEAF;
+    }
+    const bool split_a = !A.leaf && (B0.leaf || A.surf >= surface(sb));
+    if (split_a) {
+        n0a = (A.link << 1) | 0u;
+        n1a = (A.link << 1) | 1u;
+        n0b = n1b = cb;
+    } else {
+        n0b = (B0.link << 1) | 0u;
+        n1b = (B0.link << 1) | 1u;
+        n0a = n1a = ca;
+    }
+    return PAIR_SPLIT;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+__global__ void k_setup(const QueryRec* __restrict__ qs, uint32_t nq, const MeshRec* __restrict__ meshes,
+                        QueryPre* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const QueryRec q = qs[i];
+    const MeshRec& A = meshes[q.mesh_a];
+    const MeshRec& B = meshes[q.mesh_b];
+    QueryPre p;
+    float kinf = 0.f, fro = 0.f, tmax = 0.f;
+    for (int r = 0; r < 3; ++r) {
+        float row = 0.f;
+        for (int c = 0; c < 3; ++c) {
+            row += fabsf(q.rel[c * 4 + r]);
+            fro += q.rel[c * 4 + r] * q.rel[c * 4 + r];
+        }
+        kinf = fmaxf(kinf, row);
+        tmax = fmaxf(tmax, fabsf(q.rel[12 + r]));
+    }
+    // Triangles of B sit within slack_B of B's boxes; M stretches that by at
+    // most its Frobenius norm; rounding of M * p (vertices, box centre and
+    // sides, all |.| <= 2 radius) is bounded with room to spare by
+    // 16u (10 |M|_inf R_B + 2 |t|_max).  Twice the A-side slack, plus a
+    // little for the float rounding of the slack values themselves.
+    const float rb = A.slack + sqrtf(fro) * 1.0001f * B.slack + 16.f * U32 * (10.f * kinf * B.radius + 2.f * tmax) +
+                     16.f * U32 * 2.f * A.radius;
+    for (int k = 0; k < 16; ++k) p.m[k] = q.rel[k];
+    p.sigma = rb * 1.0001f + 1e-30f;
+    p.mesh_a = q.mesh_a;
+    p.mesh_b = q.mesh_b;
+    p.node_base_a = A.node_base;
+    p.node_base_b = B.node_base;
+    p.tri_base_a = A.tri_base;
+    p.tri_base_b = B.tri_base;
+    p.pad = 0;
+    out[i] = p;
+}
+
+__global__ void k_seed(uint32_t nq, uint3* __restrict__ frontier, unsigned int* __restrict__ count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) frontier[i] = make_uint3(i, ROOT_CODE, ROOT_CODE);
+    if (i == 0) *count = nq;
+}
+
+__global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ in, const unsigned int* __restrict__ nin,
+                                                     uint3* __restrict__ out, unsigned int* __restrict__ nout,
+                                                     uint32_t out_cap, const QueryPre* __restrict__ qp,
+                                                     const MeshRec* __restrict__ meshes,
+                                                     const NodeRec* __restrict__ nodes, uint4* __restrict__ lp_out,
+                                                     uint64_t lp_cap, RunCounters* __restrict__ ctr) {
+    const uint32_t n = min(*nin, out_cap);  // both frontier buffers hold out_cap entries
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long nsat = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        int kind = PAIR_NONE;
+        uint32_t n0a = 0, n0b = 0, n1a = 0, n1b = 0;
+        uint4 leaf = make_uint4(0, 0, 0, 0);
+        uint32_t qi = 0;
+        if (i < n) {
+            const uint3 e = in[i];
+            qi = e.x;
+            ++nsat;
+            kind = eval_pair(qp[qi], meshes, nodes, e.y, e.z, n0a, n0b, n1a, n1b, leaf, qi);
+        }
+        const unsigned long long mL = __ballot(kind == PAIR_LEAF);
+        if (mL) {
+            const unsigned long long s = reserve_b(mL, &ctr->leafpairs);
+            if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
+        }
+        const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
+        if (mS) {
+            unsigned int base2 = 0;
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll(static_cast<long long>(mS)) - 1;
+            if (lane == leader) base2 = atomicAdd(nout, 2u * unsigned(__popcll(mS)));
+            base2 = __shfl(base2, leader, 64);
+            const unsigned int slot = base2 + 2u * unsigned(__popcll(mS & lanes_below_b()));
+            if (kind == PAIR_SPLIT) {
+                if (slot + 1 < out_cap) {
+                    out[slot] = make_uint3(qi, n0a, n0b);
+                    out[slot + 1] = make_uint3(qi, n1a, n1b);
+                } else {
+                    atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
+                }
+            }
+        }
+    }
+    nsat = wave_sum(nsat);
+    if ((threadIdx.x & 63) == 0 && nsat) atomicAdd(&ctr->sat, nsat);
+}
+
+template <int CAP>
+__global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, const unsigned int* __restrict__ ntasks_p,
+                                            unsigned int* __restrict__ head, const QueryPre* __restrict__ qp,
+                                            const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
+                                            uint4* __restrict__ lp_out, uint64_t lp_cap, uint3* __restrict__ spill_base,
+                                            uint32_t spill_cap, uint32_t pop_width, uint32_t task_cap,
+                                            RunCounters* __restrict__ ctr) {
+    __shared__ uint32_t sq[CAP], sa[CAP], sb[CAP];
+    const int lane = threadIdx.x;
+    uint3* spill = spill_base + size_t(blockIdx.x) * spill_cap;
+    const uint32_t ntasks = min(*ntasks_p, task_cap);
+    uint32_t size = 0, nspill = 0;
+    bool tasks_done = false, overflow = false;
+    unsigned long long nsat = 0;
+    const unsigned long long below = lanes_below_b();
+    for (;;) {
+        if (size < 64 && nspill > 0) {  // bring back the spilled bottom block underneath
+            const uint32_t k = min(uint32_t(CAP / 2), nspill);
+            uint32_t tq = 0, ta = 0, tb = 0;
+            if (uint32_t(lane) < size) { tq = sq[lane]; ta = sa[lane]; tb = sb[lane]; }
+            __syncthreads();
+            if (uint32_t(lane) < size) { sq[k + lane] = tq; sa[k + lane] = ta; sb[k + lane] = tb; }
+            for (uint32_t i = lane; i < k; i += 64) {
+                const uint3 e = spill[nspill - k + i];
+                sq[i] = e.x; sa[i] = e.y; sb[i] = e.z;
+            }
+            nspill -= k;
+            size += k;
+            __syncthreads();
+        }
+        if (size < 64 && nspill == 0 && !tasks_done) {
+            const uint32_t want = 64 - size;
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(head, want);
+            t0 = __shfl(t0, 0, 64);
+            const uint32_t got = t0 < ntasks ? min(want, ntasks - t0) : 0;
+            if (got < want) tasks_done = true;
+            if (uint32_t(lane) < got) {
+                const uint3 e = tasks[t0 + lane];
+                sq[size + lane] = e.x; sa[size + lane] = e.y; sb[size + lane] = e.z;
+            }
+            size += got;
+            __syncthreads();
+        }
+        if (size == 0) {
+            if (tasks_done && nspill == 0) break;
+            continue;
+        }
+        const uint32_t p = min(size, pop_width);
+        const bool have = uint32_t(lane) < p;
+        uint32_t qi = 0, ca = 0, cb = 0;
+        if (have) {
+            const uint32_t idx = size - p + lane;
+            qi = sq[idx]; ca = sa[idx]; cb = sb[idx];
+        }
+        size -= p;
+        __syncthreads();
+        int kind = PAIR_NONE;
+        uint32_t n0a = 0, n0b = 0, n1a = 0, n1b = 0;
+        uint4 leaf = make_uint4(0, 0, 0, 0);
+        if (have) {
+            ++nsat;
+            kind = eval_pair(qp[qi], meshes, nodes, ca, cb, n0a, n0b, n1a, n1b, leaf, qi);
+        }
+        const unsigned long long mL = __ballot(kind == PAIR_LEAF);
+        if (mL) {
+            const unsigned long long s = reserve_b(mL, &ctr->leafpairs);
+            if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
+        }
+        const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
+        const uint32_t ns = uint32_t(__popcll(mS));
+        if (size + 2 * ns > uint32_t(CAP)) {  // spill the bottom half of the stack to this wave's HBM area
+            const uint32_t k = CAP / 2;
+            if (nspill + k <= spill_cap) {
+                for (uint32_t i = lane; i < k; i += 64) spill[nspill + i] = make_uint3(sq[i], sa[i], sb[i]);
+                __syncthreads();
+                for (uint32_t i = 0; i + k < size; i += 64) {
+                    const uint32_t j = i + lane;
+                    uint32_t tq = 0, ta = 0, tb = 0;
+                    if (j + k < size) { tq = sq[j + k]; ta = sa[j + k]; tb = sb[j + k]; }
+                    __syncthreads();
+                    if (j + k < size) { sq[j] = tq; sa[j] = ta; sb[j] = tb; }
+                    __syncthreads();
+                }
+                nspill += k;
+                size -= k;
+            } else {
+                overflow = true;  // cannot happen below ~spill_cap/64 levels; reported, never silently dropped
+                break;
+            }
+        }
+        if (kind == PAIR_SPLIT) {
+            const uint32_t pos = size + 2u * uint32_t(__popcll(mS & below));
+            sq[pos] = qi; sa[pos] = n1a; sb[pos] = n1b;
+            sq[pos + 1] = qi; sa[pos + 1] = n0a; sb[pos + 1] = n0b;
+        }
+        size += 2 * ns;
+        __syncthreads();
+    }
+    nsat = wave_sum(nsat);
+    if (lane == 0) {
+        atomicAdd(&ctr->sat, nsat);
+        if (overflow) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 4u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ lps,
+                                                      const unsigned long long* __restrict__ nlp_p, uint64_t lp_cap,
+                                                      const QueryPre* __restrict__ qp, const float* __restrict__ tris,
+                                                      cd_contact* __restrict__ out, uint64_t cap,
+                                                      RunCounters* __restrict__ ctr) {
+    const uint64_t nlp = min(*nlp_p, lp_cap);
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    unsigned long long ntt = 0, nfp64 = 0, nexact = 0;
+    for (uint64_t chunk = wave; chunk * 64 < nlp; chunk += nwaves) {
+        const uint64_t li = chunk * 64 + lane;
+        uint4 lp = make_uint4(0, 0, 0, 0);
+        if (li < nlp) lp = lps[li];
+        const uint32_t cA = lp.w & 0xFFFFu, cB = lp.w >> 16;
+        const uint32_t cnt = cA * cB;
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const uint32_t excl = incl - cnt;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const bool active = t < total;
+            int k = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = k + step;
+                const uint32_t oc = __shfl(excl, cand & 63, 64);
+                if (cand < 64 && oc <= t) k = cand;
+            }
+            const uint32_t kq = __shfl(lp.x, k, 64), ka = __shfl(lp.y, k, 64), kb = __shfl(lp.z, k, 64);
+            const uint32_t kw = __shfl(lp.w, k, 64), ko = __shfl(excl, k, 64);
+            bool hit = false;
+            cd_contact c;
+            if (active) {
+                ++ntt;
+                const uint32_t kcB = kw >> 16;
+                const uint32_t r = t - ko;
+                const uint32_t i = r / kcB, j = r % kcB;
+                const QueryPre& Q = qp[kq];
+                M4 M;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) M.m[e] = Q.m[e];
+                const float* a = tris + (size_t(Q.tri_base_a) + ka + i) * 9;
+                const float* b = tris + (size_t(Q.tri_base_b) + kb + j) * 9;
+                const f3 A0 = mk(a[0], a[1], a[2]), A1 = mk(a[3], a[4], a[5]), A2 = mk(a[6], a[7], a[8]);
+                const f3 B0 = xform_point(M, mk(b[0], b[1], b[2]));
+                const f3 B1 = xform_point(M, mk(b[3], b[4], b[5]));
+                const f3 B2 = xform_point(M, mk(b[6], b[7], b[8]));
+                PredStats st;
+                int ds[6];
+                const int res = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st);
+                nfp64 += st.fp64;
+                nexact += st.exact;
+                if (res == 1) {
+                    hit = true;
+                    f3 s, tt;
+                    exact_segment(A0, A1, A2, B0, B1, B2, ds, s, tt);
+                    const PlaneF pa = plane_of(A0, A1, A2), pb = plane_of(B0, B1, B2);
+                    c.query = kq;
+                    c.tri_a = ka + i;
+                    c.tri_b = kb + j;
+                    c.flags = outside_flags(pa, pb, A0, A1, A2, B0, B1, B2);
+                    c.s[0] = s.x; c.s[1] = s.y; c.s[2] = s.z;
+                    c.t[0] = tt.x; c.t[1] = tt.y; c.t[2] = tt.z;
+                    c.na[0] = pa.n.x; c.na[1] = pa.n.y; c.na[2] = pa.n.z;
+                    c.nb[0] = pb.n.x; c.nb[1] = pb.n.y; c.nb[2] = pb.n.z;
+                }
+            }
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                const unsigned long long slot = reserve_b(m, &ctr->contacts);
+                if (hit && slot < cap) out[slot] = c;
+            }
+        }
+    }
+    ntt = wave_sum(ntt);
+    nfp64 = wave_sum(nfp64);
+    nexact = wave_sum(nexact);
+    if (lane == 0) {
+        if (ntt) atomicAdd(&ctr->tritests, ntt);
+        if (nfp64) atomicAdd(&ctr->fp64, nfp64);
+        if (nexact) atomicAdd(&ctr->exact, nexact);
+    }
+}
+
+// Exact-semantics stream of triangle pairs: bit0 intersect (non-coplanar), bit1 coplanar.
+__global__ __launch_bounds__(256) void k_tritri_exact(const float* __restrict__ a9, const float* __restrict__ b9,
+                                                      uint64_t n, int32_t* __restrict__ flags,
+                                                      float* __restrict__ seg6, RunCounters* __restrict__ ctr) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long nfp64 = 0, nexact = 0;
+    if (i < n) {
+        const float* a = a9 + i * 9;
+        const float* b = b9 + i * 9;
+        const f3 A0 = mk(a[0], a[1], a[2]), A1 = mk(a[3], a[4], a[5]), A2 = mk(a[6], a[7], a[8]);
+        const f3 B0 = mk(b[0], b[1], b[2]), B1 = mk(b[3], b[4], b[5]), B2 = mk(b[6], b[7], b[8]);
+        PredStats st;
+        int ds[6];
+        const int res = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st);
+        nfp64 = st.fp64;
+        nexact = st.exact;
+        flags[i] = res == 1 ? 1 : (res == 2 ? 2 : 0);
+        if (seg6) {
+            f3 s = mk(0.f, 0.f, 0.f), t = mk(0.f, 0.f, 0.f);
+            if (res == 1) exact_segment(A0, A1, A2, B0, B1, B2, ds, s, t);
+            float* o = seg6 + i * 6;
+            o[0] = s.x; o[1] = s.y; o[2] = s.z;
+            o[3] = t.x; o[4] = t.y; o[5] = t.z;
+        }
+    }
+    nfp64 = wave_sum(nfp64);
+    nexact = wave_sum(nexact);
+    if ((threadIdx.x & 63) == 0) {
+        if (nfp64) atomicAdd(&ctr->fp64, nfp64);
+        if (nexact) atomicAdd(&ctr->exact, nexact);
+    }
+}
+
+// ---- host launchers -------------------------------------------------------
+hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st) {
+    if (!nq) return hipSuccess;
+    hipLaunchKernelGGL(k_setup, dim3((nq + 127) / 128), dim3(128), 0, st, qs, nq, meshes, out);
+    return hipGetLastError();
+}
+hipError_t launch_seed(uint32_t nq, uint3* frontier, unsigned int* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed, dim3((std::max(nq, 1u) + 255) / 256), dim3(256), 0, st, nq, frontier, count);
+    return hipGetLastError();
+}
+hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
+                              const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
+                              uint64_t lp_cap, RunCounters* ctr, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_split_level, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
+                       lp_cap, ctr);
+    return hipGetLastError();
+}
+hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
+                      unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
+                      uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
+                      RunCounters* ctr, hipStream_t st) {
+    switch (cap_entries) {
+        case 512:
+            hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
+                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+            break;
+        case 2048:
+            hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
+                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+            break;
+        default:
+            hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
+                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+            break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
+                               const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, ctr);
+    return hipGetLastError();
+}
+hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
+                               RunCounters* ctr, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_tritri_exact, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, a9, b9, n, flags, seg6, ctr);
+    return hipGetLastError();
+}
+
+}  // namespace cdk

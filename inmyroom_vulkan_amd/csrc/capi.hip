@@ -1,4 +1,5 @@
 // C ABI of the collision library (declared in include/mi355x_collide.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -125,8 +126,150 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
 }
 
+cd_tuning effective_tuning(const cd_ctx* ctx) {
+    cd_tuning t = ctx->tuning;
+    if (t.split_levels == 0) t.split_levels = 8;
+    if (t.split_levels > 16) t.split_levels = 16;
+    if (t.stack_entries != 512 && t.stack_entries != 2048) t.stack_entries = 1024;
+    if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
+    if (t.waves_per_cu == 0) t.waves_per_cu = 8;
+    return t;
+}
+
+constexpr uint32_t SPILL_CAP = 16384;  // entries of each persistent wavefront's HBM spill area
+
+// Work buffers of the batched path (d_work slots).
+enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL };
+
+// Enqueue the whole batched pipeline on the stream (no host synchronisation
+// inside, so it can be captured in a graph).
+int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
+    hipStream_t st = ctx->stream;
+    const uint32_t L = t.split_levels;
+    const uint64_t fcap = std::max<uint64_t>(uint64_t(nq) << L, 64);
+    const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
+    QueryPre* qp = ctx->d_work[W_QPRE].as<QueryPre>();
+    uint3* F[2] = {ctx->d_work[W_FRONT0].as<uint3>(), ctx->d_work[W_FRONT1].as<uint3>()};
+    unsigned int* small = ctx->d_work[W_SMALL].as<unsigned int>();  // [0..L] level counts, [L+1] task head
+    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    const NodeRec* dn = ctx->d_nodes.as<NodeRec>();
+    RunCounters* ctr = dev_counters(ctx);
+    const uint32_t grid = uint32_t(ctx->num_cu) * 8;
+    HCK(ctx, launch_zero(ctr, st));
+    HCK(ctx, launch_fill_u32(small, L + 2, 0u, st));
+    if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
+    HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dm, qp, st));
+    HCK(ctx, launch_seed(nq, F[0], &small[0], st));
+    for (uint32_t l = 0; l < L; ++l)
+        HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)),
+                                    qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctr, grid, st));
+    if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
+    HCK(ctx, launch_dfs(int(t.stack_entries), F[L & 1], &small[L], uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)), &small[L + 1], qp, dm, dn,
+                        ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
+                        t.pop_width, nwaves, ctr, st));
+    if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
+    HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->leafpairs, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
+                                 ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
+    if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
+    return CD_OK;
+}
+
+int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
+    hipStream_t st = ctx->stream;
+    const cd_tuning t = effective_tuning(ctx);
+    const uint32_t L = t.split_levels;
+    const uint64_t fcap = std::max<uint64_t>(uint64_t(nq) << L, 64);
+    const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
+    if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
+    if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
+    HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
+    HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
+    HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
+    HCK(ctx, ctx->d_work[W_SMALL].ensure(64 * sizeof(unsigned int)));
+    HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
+    RunCounters h{};
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
+        HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
+        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, ctx->lp_cap, ctx->contact_cap,
+                                     uint64_t(ctx->leaf_max)};
+        for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
+                                &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
+                                &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL]})
+            key.push_back(reinterpret_cast<uint64_t>(b->p));
+        if (t.use_graph) {
+            bool same = ctx->graph && key == ctx->graph_key;
+            if (!same) {
+                if (ctx->graph) {
+                    (void)hipGraphExecDestroy(ctx->graph);
+                    ctx->graph = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                HCK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_batched(ctx, nq, t, false);
+                hipError_t ec = hipStreamEndCapture(st, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
+                hipError_t ei = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
+                ctx->graph_key = key;
+            }
+            HCK(ctx, hipEventRecord(ctx->ev[4], st));
+            HCK(ctx, hipGraphLaunch(ctx->graph, st));
+            HCK(ctx, hipEventRecord(ctx->ev[5], st));
+        } else {
+            int rc = enqueue_batched(ctx, nq, t, true);
+            if (rc) return rc;
+        }
+        int rc = read_counters(ctx, h);
+        if (rc) return rc;
+        if (h.err) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
+        bool retry = false;
+        if (h.leafpairs > ctx->lp_cap) {
+            ctx->lp_cap = h.leafpairs + h.leafpairs / 4;
+            retry = true;
+        }
+        if (h.contacts > ctx->contact_cap) {
+            ctx->contact_cap = h.contacts + h.contacts / 4;
+            retry = true;
+        }
+        if (retry) continue;
+        ctx->timings = cd_timings{};
+        if (t.use_graph) {
+            float total = 0.f;
+            HCK(ctx, hipEventElapsedTime(&total, ctx->ev[4], ctx->ev[5]));
+            ctx->timings.total_ms = total;
+        } else {
+            float t_split = 0.f, t_dfs = 0.f, t_narrow = 0.f;
+            HCK(ctx, hipEventElapsedTime(&t_split, ctx->ev[0], ctx->ev[1]));
+            HCK(ctx, hipEventElapsedTime(&t_dfs, ctx->ev[1], ctx->ev[2]));
+            HCK(ctx, hipEventElapsedTime(&t_narrow, ctx->ev[2], ctx->ev[3]));
+            ctx->timings.traverse_ms = t_split + t_dfs;
+            ctx->timings.narrow_ms = t_narrow;
+            ctx->timings.broad_ms = t_split;
+            ctx->timings.total_ms = t_split + t_dfs + t_narrow;
+        }
+        ctx->timings.launches = 1;
+        ctx->counters = cd_counters{};
+        ctx->counters.sat_tests = h.sat;
+        ctx->counters.leaf_pairs = h.leafpairs;
+        ctx->counters.tri_tests = h.tritests;
+        ctx->counters.contacts = h.contacts;
+        ctx->counters.fp64_fallbacks = h.fp64;
+        ctx->counters.exact_fallbacks = h.exact;
+        *n_contacts = h.contacts;
+        return CD_OK;
+    }
+    return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
+}
+
 int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
     if (mode == CD_MODE_PLAIN) return run_plain(ctx, nq, n_contacts);
+    if (mode == CD_MODE_BATCHED) return run_batched(ctx, nq, n_contacts);
     return fail(ctx, CD_E_INVALID, "unknown mode");
 }
 
@@ -177,6 +320,9 @@ int cd_create(cd_ctx** out, const cd_config* cfg) {
         delete ctx;
         return CD_E_NOMEM;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cu = prop.multiProcessorCount;
     *out = ctx;
     return CD_OK;
 }
@@ -189,6 +335,7 @@ int cd_destroy(cd_ctx* ctx) {
                       &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg})
         b->release();
     for (auto& b : ctx->d_work) b.release();
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -267,7 +414,7 @@ int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, 
 int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode, int32_t* flags, float* seg6) {
     if (!ctx) return CD_E_INVALID;
     if (n && (!a9 || !b9 || !flags)) return fail(ctx, CD_E_INVALID, "null argument");
-    if (mode != CD_MODE_PLAIN) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
     hipStream_t st = ctx->stream;
     HCK(ctx, ctx->d_tt_a.ensure(n * 9 * sizeof(float) + 4));
     HCK(ctx, ctx->d_tt_b.ensure(n * 9 * sizeof(float) + 4));
@@ -277,9 +424,14 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
         HCK(ctx, hipMemcpyAsync(ctx->d_tt_a.p, a9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
         HCK(ctx, hipMemcpyAsync(ctx->d_tt_b.p, b9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
     }
+    HCK(ctx, launch_zero(dev_counters(ctx), st));
     HCK(ctx, hipEventRecord(ctx->ev[0], st));
-    HCK(ctx, launch_tritri_plain(ctx->d_tt_a.as<float>(), ctx->d_tt_b.as<float>(), n, ctx->d_tt_flags.as<int32_t>(),
-                                 seg6 ? ctx->d_tt_seg.as<float>() : nullptr, st));
+    if (mode == CD_MODE_PLAIN)
+        HCK(ctx, launch_tritri_plain(ctx->d_tt_a.as<float>(), ctx->d_tt_b.as<float>(), n, ctx->d_tt_flags.as<int32_t>(),
+                                     seg6 ? ctx->d_tt_seg.as<float>() : nullptr, st));
+    else
+        HCK(ctx, launch_tritri_exact(ctx->d_tt_a.as<float>(), ctx->d_tt_b.as<float>(), n, ctx->d_tt_flags.as<int32_t>(),
+                                     seg6 ? ctx->d_tt_seg.as<float>() : nullptr, dev_counters(ctx), st));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
     if (n) {
         HCK(ctx, hipMemcpyAsync(flags, ctx->d_tt_flags.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -288,6 +440,13 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
     HCK(ctx, hipStreamSynchronize(st));
     float ms = 0.f;
     HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    RunCounters h{};
+    int rc = read_counters(ctx, h);
+    if (rc) return rc;
+    ctx->counters = cd_counters{};
+    ctx->counters.tri_tests = n;
+    ctx->counters.fp64_fallbacks = h.fp64;
+    ctx->counters.exact_fallbacks = h.exact;
     ctx->timings = cd_timings{};
     ctx->timings.total_ms = ms;
     ctx->timings.narrow_ms = ms;
@@ -364,6 +523,23 @@ int cd_broad_phase(cd_ctx* ctx, const cd_entity*, uint32_t, uint32_t*, uint32_t,
 int cd_get_counters(cd_ctx* ctx, cd_counters* out) {
     if (!ctx || !out) return CD_E_INVALID;
     *out = ctx->counters;
+    return CD_OK;
+}
+
+int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
+    if (!ctx || !t) return CD_E_INVALID;
+    if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
+        return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
+    if (t->pop_width > 64 || t->split_levels > 16 || t->waves_per_cu > 64)
+        return fail(ctx, CD_E_INVALID, "tuning out of range");
+    ctx->tuning = *t;
+    return CD_OK;
+}
+
+int cd_get_tuning(cd_ctx* ctx, cd_tuning* t) {
+    if (!ctx || !t) return CD_E_INVALID;
+    *t = effective_tuning(ctx);
+    t->use_graph = ctx->tuning.use_graph;
     return CD_OK;
 }
 

@@ -45,6 +45,15 @@ struct QueryRec {
 };
 static_assert(sizeof(QueryRec) == sizeof(cd_query), "query layout");
 
+// Per-query record of the batched path: relative matrix, the slack that
+// makes its float32 separating-axis test conservative, and arena bases.
+struct QueryPre {
+    float m[16];
+    float sigma;
+    uint32_t mesh_a, mesh_b, node_base_a, node_base_b, tri_base_a, tri_base_b, pad;
+};
+static_assert(sizeof(QueryPre) == 96, "query record layout");
+
 constexpr uint32_t ROOT_CODE = 0xFFFFFFFFu;  // traversal code of a tree's root
 constexpr int PLAIN_STACK = 64;              // per-lane stack entries of the plain traversal
 
@@ -121,5 +130,9 @@ struct cd_ctx {
 
     cd_counters counters{};
     cd_timings timings{};
+    cd_tuning tuning{};
+    int num_cu = 256;
+    hipGraphExec_t graph = nullptr;        // captured batched pipeline
+    std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured
     hipEvent_t ev[8] = {};
 };

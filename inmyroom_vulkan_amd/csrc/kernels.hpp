@@ -18,4 +18,21 @@ hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max
                                const MeshRec* meshes, const float* tris, cd_contact* out, uint64_t cap,
                                RunCounters* ctr, hipStream_t st);
 
+// batched.hip
+hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st);
+hipError_t launch_seed(uint32_t nq, uint3* frontier, unsigned int* count, hipStream_t st);
+hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
+                              const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
+                              uint64_t lp_cap, RunCounters* ctr, uint32_t grid, hipStream_t st);
+hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
+                      unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
+                      uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
+                      RunCounters* ctr, hipStream_t st);
+hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
+                               const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                               hipStream_t st);
+hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
+                               RunCounters* ctr, hipStream_t st);
+hipError_t launch_fill_u32(unsigned int* p, uint32_t n, unsigned int v, hipStream_t st);
+
 }  // namespace cdk

@@ -216,7 +216,17 @@ __global__ void k_zero_counters(RunCounters* c) {
     if (threadIdx.x == 0) *c = RunCounters{0, 0, 0, 0, 0, 0, 0, 0};
 }
 
+__global__ void k_fill_u32(unsigned int* p, uint32_t n, unsigned int v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---- host launchers -------------------------------------------------------
+hipError_t launch_fill_u32(unsigned int* p, uint32_t n, unsigned int v, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
 hipError_t launch_zero(RunCounters* c, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, c);
     return hipGetLastError();

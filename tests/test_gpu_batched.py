@@ -1,0 +1,118 @@
+"""Batched path on the GPU against the exact-semantics oracle.
+
+The batched path evaluates every orientation predicate with its exact sign
+(float32 filter, float64 fallback, exact expansion), and its traversal is
+conservative, so its colliding pair set must equal the oracle's exact set
+(bignum orientation, conservative double traversal) with zero mismatches.
+"""
+import numpy as np
+import pytest
+
+from conftest import case, golden
+from inmyroom_vulkan_amd import MODE_BATCHED, MODE_PLAIN, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _keys(c):
+    return {(int(x["query"]), int(x["tri_a"]), int(x["tri_b"])) for x in c}
+
+
+def test_tritri_exact_stream_matches_oracle(gpu, orc):
+    g = golden("moller_ref.npz")
+    a1, b1, _ = scenes.config4(100000, seed=51)
+    a2, b2, _ = scenes.config4(100000, seed=52, span=1.0)
+    a9 = np.concatenate([g["a9"], a1, a2])
+    b9 = np.concatenate([g["b9"], b1, b2])
+    case(f"tritri exact n={len(a9)}")
+    f, seg = gpu.tritri(a9, b9, MODE_BATCHED)
+    want = orc.tritri_exact(a9, b9)
+    assert np.array_equal(f, want)
+    cnt = gpu.counters()
+    assert cnt["fp64_fallbacks"] > 0  # the near-degenerate cases do reach the fallback
+    # segments of clearly transversal pairs agree with the Möller segment
+    fm, sm = orc.tritri(a9, b9)
+    both = (f == 1) & ((fm & 3) == 1)
+    d = np.abs(seg[both] - sm[both]).max(axis=1)
+    scale = np.abs(sm[both]).max(axis=1) + 1e-6
+    assert np.quantile(d / scale, 0.99) < 1e-5
+
+
+def _exact_oracle(orc, soups, qa, qb, rel, L):
+    trees = [orc.tree(s, L) for s in soups]
+    return orc.batch(trees, qa, qb, rel, mode=1)
+
+
+def test_batched_queries_config1(gpu, orc):
+    mesh, rel = scenes.config1(300, seed=1)
+    case("batched queries config1 n=300")
+    mid = gpu.register_soups([mesh])[0]
+    q = gpu.make_queries(mid, mid, rel)
+    got = gpu.query_batch(q, MODE_BATCHED)
+    z = np.zeros(len(rel), np.uint32)
+    want = _exact_oracle(orc, [mesh], z, z, rel, 4)
+    assert _keys(got) == _keys(want.contacts)
+    assert len(got) > 100
+    # same pair set as the plain path away from degenerate contacts
+    plain = gpu.query_batch(q, MODE_PLAIN)
+    diff = _keys(got) ^ _keys(plain)
+    assert len(diff) <= max(3, len(plain) // 500)
+
+
+def test_batched_queries_uneven_subset(gpu, orc):
+    soups, qa, qb, rel, _ = scenes.config3(128, seed=3, freq_cap=16)
+    case(f"batched queries config3 subset nq={len(qa)}")
+    ids = np.array(gpu.register_soups(soups), np.uint32)
+    q = np.zeros(len(qa), dtype=gpu.make_queries(0, 0, rel[:1]).dtype)
+    q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
+    got = gpu.query_batch(q, MODE_BATCHED)
+    want = _exact_oracle(orc, soups, qa, qb, rel, 4)
+    assert _keys(got) == _keys(want.contacts)
+    assert len(got) > 100
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(split_levels=0, stack_entries=512, pop_width=64, waves_per_cu=2),   # no split; spills in LDS stack
+    dict(split_levels=4, stack_entries=2048, pop_width=16, waves_per_cu=4),
+    dict(split_levels=12, stack_entries=1024, pop_width=64, waves_per_cu=8),
+    dict(split_levels=6, stack_entries=512, pop_width=1, waves_per_cu=1),
+])
+def test_batched_tuning_variants_agree(gpu, tuning):
+    soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
+    case(f"batched tuning variant {tuning}")
+    ids = np.array(gpu.register_soups(soups), np.uint32)
+    q = np.zeros(len(qa), dtype=gpu.make_queries(0, 0, rel[:1]).dtype)
+    q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
+    gpu.set_tuning()
+    base = _keys(gpu.query_batch(q, MODE_BATCHED))
+    base_cnt = gpu.counters()
+    try:
+        gpu.set_tuning(**tuning)
+        got = _keys(gpu.query_batch(q, MODE_BATCHED))
+        cnt = gpu.counters()
+    finally:
+        gpu.set_tuning()
+    assert got == base
+    assert cnt["leaf_pairs"] == base_cnt["leaf_pairs"] and cnt["tri_tests"] == base_cnt["tri_tests"]
+
+
+def test_batched_graph_replays_match_eager(gpu):
+    soups, qa, qb, rel, _ = scenes.config3(96, seed=7, freq_cap=20)
+    case("batched graph replays")
+    ids = np.array(gpu.register_soups(soups), np.uint32)
+    q = np.zeros(len(qa), dtype=gpu.make_queries(0, 0, rel[:1]).dtype)
+    q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
+    gpu.set_tuning()
+    eager = _keys(gpu.query_batch(q, MODE_BATCHED))
+    try:
+        gpu.set_tuning(use_graph=1)
+        gpu.stage(q)
+        for rep in range(4):
+            gpu.run_staged(MODE_BATCHED)
+            got = _keys(gpu.fetch())
+            assert got == eager, f"replay {rep}"
+            # eager work of another kind between replays
+            gpu.query_batch(q[:8], MODE_PLAIN)
+            gpu.stage(q)
+    finally:
+        gpu.set_tuning()
