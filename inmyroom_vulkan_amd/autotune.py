@@ -67,13 +67,13 @@ def tune(gpu, queries, reps: int = 3, space: dict | None = None, log=print) -> d
     return {"tuning": best, "ms": best_ms, "candidates": len(results)}
 
 
-def save(nq: int, result: dict) -> None:
+def save(nq: int, result: dict, path: str = TABLE) -> None:
     table = {}
-    if os.path.exists(TABLE):
-        with open(TABLE) as f:
+    if os.path.exists(path):
+        with open(path) as f:
             table = json.load(f)
     table[size_class(nq)] = {"tuning": result["tuning"], "ms": result["ms"], "when": time.strftime("%Y-%m-%d")}
-    with open(TABLE, "w") as f:
+    with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
 
 
@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--nq", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=TABLE, help="table to update (default: the package's table)")
     args = ap.parse_args()
     soups, qa, qb, rel, _ = scenes.config3(args.nq, seed=args.seed)
     gpu = Collider(leaf_max=4)
@@ -96,7 +97,7 @@ def main():
     if args.quick:
         space = {"split_levels": [6, 10], "stack_entries": [1024], "pop_width": [64], "waves_per_cu": [8]}
     res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
-    save(args.nq, res)
+    save(args.nq, res, args.out)
     print(json.dumps(res), flush=True)
 
 

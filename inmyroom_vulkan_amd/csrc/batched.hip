@@ -240,6 +240,7 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     const int lane = threadIdx.x;
     uint3* spill = spill_base + size_t(blockIdx.x) * spill_cap;
     const uint32_t ntasks = min(*ntasks_p, task_cap);
+    const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
     uint32_t size = 0, nspill = 0;
     bool tasks_done = false, overflow = false;
     unsigned long long nsat = 0;
@@ -260,7 +261,9 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
             __syncthreads();
         }
         if (size < 64 && nspill == 0 && !tasks_done) {
-            const uint32_t want = 64 - size;
+            // take a fair share, not a full stack: a wave that grabbed 64 tasks
+            // at once would leave most of the grid idle
+            const uint32_t want = min(64 - size, grab);
             uint32_t t0 = 0;
             if (lane == 0) t0 = atomicAdd(head, want);
             t0 = __shfl(t0, 0, 64);
@@ -372,7 +375,8 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
             const uint32_t kq = __shfl(lp.x, k, 64), ka = __shfl(lp.y, k, 64), kb = __shfl(lp.z, k, 64);
             const uint32_t kw = __shfl(lp.w, k, 64), ko = __shfl(excl, k, 64);
             bool hit = false;
-            cd_contact c;
+            f3 A0, A1, A2, B0, B1, B2;
+            uint32_t ds = 0, ta = 0, tb = 0;
             if (active) {
                 ++ntt;
                 const uint32_t kcB = kw >> 16;
@@ -382,36 +386,25 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
                 M4 M;
 #pragma unroll
                 for (int e = 0; e < 16; ++e) M.m[e] = Q.m[e];
-                const float* a = tris + (size_t(Q.tri_base_a) + ka + i) * 9;
-                const float* b = tris + (size_t(Q.tri_base_b) + kb + j) * 9;
-                const f3 A0 = mk(a[0], a[1], a[2]), A1 = mk(a[3], a[4], a[5]), A2 = mk(a[6], a[7], a[8]);
-                const f3 B0 = xform_point(M, mk(b[0], b[1], b[2]));
-                const f3 B1 = xform_point(M, mk(b[3], b[4], b[5]));
-                const f3 B2 = xform_point(M, mk(b[6], b[7], b[8]));
+                ta = ka + i;
+                tb = kb + j;
+                const float* a = tris + (size_t(Q.tri_base_a) + ta) * 9;
+                const float* b = tris + (size_t(Q.tri_base_b) + tb) * 9;
+                A0 = mk(a[0], a[1], a[2]);
+                A1 = mk(a[3], a[4], a[5]);
+                A2 = mk(a[6], a[7], a[8]);
+                B0 = xform_point(M, mk(b[0], b[1], b[2]));
+                B1 = xform_point(M, mk(b[3], b[4], b[5]));
+                B2 = xform_point(M, mk(b[6], b[7], b[8]));
                 PredStats st;
-                int ds[6];
-                const int res = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st);
+                hit = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st) == 1;
                 nfp64 += st.fp64;
                 nexact += st.exact;
-                if (res == 1) {
-                    hit = true;
-                    f3 s, tt;
-                    exact_segment(A0, A1, A2, B0, B1, B2, ds, s, tt);
-                    const PlaneF pa = plane_of(A0, A1, A2), pb = plane_of(B0, B1, B2);
-                    c.query = kq;
-                    c.tri_a = ka + i;
-                    c.tri_b = kb + j;
-                    c.flags = outside_flags(pa, pb, A0, A1, A2, B0, B1, B2);
-                    c.s[0] = s.x; c.s[1] = s.y; c.s[2] = s.z;
-                    c.t[0] = tt.x; c.t[1] = tt.y; c.t[2] = tt.z;
-                    c.na[0] = pa.n.x; c.na[1] = pa.n.y; c.na[2] = pa.n.z;
-                    c.nb[0] = pb.n.x; c.nb[1] = pb.n.y; c.nb[2] = pb.n.z;
-                }
             }
             const unsigned long long m = __ballot(hit);
             if (m) {
                 const unsigned long long slot = reserve_b(m, &ctr->contacts);
-                if (hit && slot < cap) out[slot] = c;
+                if (hit && slot < cap) write_contact(out + slot, A0, A1, A2, B0, B1, B2, ds, kq, ta, tb);
             }
         }
     }
@@ -437,7 +430,7 @@ __global__ __launch_bounds__(256) void k_tritri_exact(const float* __restrict__ 
         const f3 A0 = mk(a[0], a[1], a[2]), A1 = mk(a[3], a[4], a[5]), A2 = mk(a[6], a[7], a[8]);
         const f3 B0 = mk(b[0], b[1], b[2]), B1 = mk(b[3], b[4], b[5]), B2 = mk(b[6], b[7], b[8]);
         PredStats st;
-        int ds[6];
+        uint32_t ds = 0;
         const int res = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st);
         nfp64 = st.fp64;
         nexact = st.exact;

@@ -138,8 +138,9 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
 
 constexpr uint32_t SPILL_CAP = 16384;  // entries of each persistent wavefront's HBM spill area
 
-// Work buffers of the batched path (d_work slots).
-enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL };
+// Work buffers (d_work slots): batched path, then broad phase / frame.
+enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT };
 
 // Enqueue the whole batched pipeline on the stream (no host synchronisation
 // inside, so it can be captured in a graph).
@@ -282,6 +283,68 @@ int stage(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
     if (nq) HCK(ctx, hipMemcpyAsync(ctx->d_queries.p, qr, size_t(nq) * sizeof(QueryRec), hipMemcpyHostToDevice, ctx->stream));
     HCK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->staged_nq = nq;
+    return CD_OK;
+}
+
+// Sweep and prune on the GPU.  Leaves the oriented pairs, sorted by
+// (first, second), in *hp and the matching queries (relative matrices) in
+// d_queries.
+int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out, std::vector<uint2>* hp) {
+    if (!ents) return fail(ctx, CD_E_INVALID, "null entity array");
+    if (n >= 0xFFFFu) return fail(ctx, CD_E_INVALID, "more entities than the 16-bit Entity type allows");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (ents[i].mesh_id >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "entity names an unknown mesh");
+        if (!all_finite(ents[i].current, 16)) return fail(ctx, CD_E_INVALID, "non-finite entity matrix");
+    }
+    hipStream_t st = ctx->stream;
+    HCK(ctx, ctx->d_work[W_ENTS].ensure(size_t(n) * sizeof(cd_entity)));
+    HCK(ctx, ctx->d_work[W_IV].ensure(size_t(n) * 6 * sizeof(float)));
+    for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure(size_t(n) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_PCOUNT].ensure(sizeof(unsigned int)));
+    HCK(ctx, hipMemcpyAsync(ctx->d_work[W_ENTS].p, ents, size_t(n) * sizeof(cd_entity), hipMemcpyHostToDevice, st));
+    const cd_entity* de = ctx->d_work[W_ENTS].as<cd_entity>();
+    uint32_t* k0 = ctx->d_work[W_K0].as<uint32_t>();
+    uint32_t* v0 = ctx->d_work[W_V0].as<uint32_t>();
+    uint32_t* k1 = ctx->d_work[W_K1].as<uint32_t>();
+    uint32_t* v1 = ctx->d_work[W_V1].as<uint32_t>();
+    unsigned int* cnt = ctx->d_work[W_PCOUNT].as<unsigned int>();
+    uint32_t cap = std::max<uint32_t>(4 * n, 1024);
+    uint32_t np = 0;
+    HCK(ctx, hipEventRecord(ctx->ev[6], st));
+    HCK(ctx, launch_sap_project(de, n, ctx->d_meshes.as<MeshRec>(), sap_axes(), ctx->d_work[W_IV].as<float>(), k0, v0, st));
+    HCK(ctx, launch_radix_sort(k0, v0, k1, v1, n, 32, st));
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        HCK(ctx, ctx->d_work[W_PAIRS].ensure(size_t(cap) * sizeof(uint2)));
+        HCK(ctx, launch_fill_u32(cnt, 1, 0u, st));
+        HCK(ctx, launch_sap_sweep(v0, ctx->d_work[W_IV].as<float>(), de, n, ctx->d_work[W_PAIRS].as<uint2>(), cap, cnt, st));
+        HCK(ctx, hipMemcpyAsync(&np, cnt, sizeof np, hipMemcpyDeviceToHost, st));
+        HCK(ctx, hipStreamSynchronize(st));
+        if (np <= cap) break;
+        cap = np + np / 4;
+    }
+    if (np > cap) return fail(ctx, CD_E_NOMEM, "pair buffer sizing did not converge");
+    if (np) {
+        for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure(size_t(np) * sizeof(uint32_t)));
+        k0 = ctx->d_work[W_K0].as<uint32_t>();
+        v0 = ctx->d_work[W_V0].as<uint32_t>();
+        k1 = ctx->d_work[W_K1].as<uint32_t>();
+        v1 = ctx->d_work[W_V1].as<uint32_t>();
+        HCK(ctx, ctx->d_work[W_SPAIRS].ensure(size_t(np) * sizeof(uint2)));
+        HCK(ctx, ctx->d_queries.ensure(size_t(np) * sizeof(QueryRec)));
+        HCK(ctx, launch_pair_keys(ctx->d_work[W_PAIRS].as<uint2>(), np, k0, v0, st));
+        HCK(ctx, launch_radix_sort(k0, v0, k1, v1, np, 32, st));
+        HCK(ctx, launch_pair_queries(ctx->d_work[W_PAIRS].as<uint2>(), v0, np, de, ctx->d_queries.as<QueryRec>(),
+                                     ctx->d_work[W_SPAIRS].as<uint2>(), st));
+    }
+    HCK(ctx, hipEventRecord(ctx->ev[7], st));
+    hp->assign(np, make_uint2(0, 0));
+    if (np) HCK(ctx, hipMemcpyAsync(hp->data(), ctx->d_work[W_SPAIRS].p, size_t(np) * sizeof(uint2), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+    ctx->timings = cd_timings{};
+    ctx->timings.broad_ms = ms;
+    *np_out = np;
     return CD_OK;
 }
 
@@ -511,13 +574,88 @@ int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_ou
     return CD_OK;
 }
 
-int cd_frame(cd_ctx* ctx, const cd_entity*, uint32_t, uint32_t, cd_pair*, uint32_t, uint32_t*, cd_contact*, uint64_t,
-             uint64_t*) {
-    return fail(ctx, CD_E_INVALID, "cd_frame: broad phase not available in this build");
+int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
+             uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+    if (!ctx || !n_pairs || !n_contacts) return CD_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    *n_pairs = 0;
+    *n_contacts = 0;
+    if (n < 2) {  // CollisionDetection.cpp:40
+        ctx->counters = cd_counters{};
+        return CD_OK;
+    }
+    uint32_t np = 0;
+    std::vector<uint2> hp;
+    int rc = broad_phase(ctx, entities, n, &np, &hp);
+    if (rc) return rc;
+    const float broad_ms = ctx->timings.broad_ms;
+    if (np == 0) {
+        ctx->counters = cd_counters{};
+        ctx->timings = cd_timings{};
+        ctx->timings.broad_ms = broad_ms;
+        ctx->timings.total_ms = broad_ms;
+        return CD_OK;
+    }
+    if (mode == CD_MODE_PLAIN)
+        for (uint32_t i = 0; i < np; ++i)
+            if (ctx->meshes[entities[hp[i].x].mesh_id].depth + ctx->meshes[entities[hp[i].y].mesh_id].depth + 2 >
+                uint32_t(PLAIN_STACK))
+                return fail(ctx, CD_E_STACK, "trees too deep for the plain traversal stack");
+    ctx->staged_nq = np;
+    uint64_t nc = 0;
+    rc = run_queries(ctx, np, mode, &nc);
+    if (rc) return rc;
+    ctx->counters.broad_pairs = np;
+    ctx->timings.broad_ms = broad_ms;
+    ctx->timings.total_ms += broad_ms;
+    ctx->staged_contacts = nc;
+    // contacts grouped by pair, ordered by (pair, tri_a, tri_b)
+    std::vector<cd_contact> hc(nc);
+    if (nc) {
+        if (nc > 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "too many contacts in one frame");
+        hipStream_t st = ctx->stream;
+        HCK(ctx, ctx->d_work[W_CSORT].ensure(nc * sizeof(cd_contact)));
+        for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure(nc * sizeof(uint32_t)));
+        HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), uint32_t(nc), ctx->d_work[W_CSORT].as<cd_contact>(),
+                                      ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
+                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
+        HCK(ctx, hipMemcpyAsync(hc.data(), ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost, st));
+        HCK(ctx, hipStreamSynchronize(st));
+    }
+    // pairs that produced contacts (CollisionDetection.cpp:52, :63 keep only those)
+    std::vector<cd_pair> out_pairs;
+    for (uint64_t i = 0; i < nc;) {
+        uint64_t j = i;
+        while (j < nc && hc[j].query == hc[i].query) ++j;
+        const uint2 pr = hp[hc[i].query];
+        out_pairs.push_back(cd_pair{pr.x, pr.y, uint32_t(i), uint32_t(j - i)});
+        i = j;
+    }
+    *n_pairs = uint32_t(out_pairs.size());
+    *n_contacts = nc;
+    if (out_pairs.size() > pair_cap || nc > contact_cap || (nc && (!pairs || !contacts)))
+        return fail(ctx, CD_E_CAPACITY, "pair or contact buffer too small");
+    if (!out_pairs.empty()) std::memcpy(pairs, out_pairs.data(), out_pairs.size() * sizeof(cd_pair));
+    if (nc) std::memcpy(contacts, hc.data(), nc * sizeof(cd_contact));
+    return CD_OK;
 }
 
-int cd_broad_phase(cd_ctx* ctx, const cd_entity*, uint32_t, uint32_t*, uint32_t, uint32_t*) {
-    return fail(ctx, CD_E_INVALID, "cd_broad_phase: not available in this build");
+int cd_broad_phase(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t* pairs2, uint32_t cap, uint32_t* n_pairs) {
+    if (!ctx || !n_pairs) return CD_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    *n_pairs = 0;
+    if (n < 2) return CD_OK;
+    uint32_t np = 0;
+    std::vector<uint2> hp;
+    int rc = broad_phase(ctx, entities, n, &np, &hp);
+    if (rc) return rc;
+    *n_pairs = np;
+    if (np > cap || (np && !pairs2)) return fail(ctx, CD_E_CAPACITY, "pair buffer too small");
+    for (uint32_t i = 0; i < np; ++i) {
+        pairs2[2 * i] = hp[i].x;
+        pairs2[2 * i + 1] = hp[i].y;
+    }
+    return CD_OK;
 }
 
 int cd_get_counters(cd_ctx* ctx, cd_counters* out) {

@@ -18,6 +18,7 @@
 #pragma once
 #pragma clang fp contract(off)
 
+#include "../../include/mi355x_collide.h"
 #include "geom.hpp"
 
 namespace cdk {
@@ -73,34 +74,37 @@ __device__ __noinline__ int orient_exact_slow(f3 a, f3 b, f3 c, f3 d) {
     return 0;
 }
 
+// Stages 2 and 3, out of line so the float32 fast path stays small.
+// Returns (sign + 1) | 4 when the exact stage was needed.
+__device__ __noinline__ int orient_slow(f3 a, f3 b, f3 c, f3 d) {
+    const double ax = double(a.x) - c.x, ay = double(a.y) - c.y, az = double(a.z) - c.z;
+    const double bx = double(b.x) - c.x, by = double(b.y) - c.y, bz = double(b.z) - c.z;
+    const double dx = double(d.x) - c.x, dy = double(d.y) - c.y, dz = double(d.z) - c.z;
+    const double p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
+    const double det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
+    const double perm = (fabs(p0) + fabs(p1)) * fabs(dx) + (fabs(p2) + fabs(p3)) * fabs(dy) +
+                        (fabs(p4) + fabs(p5)) * fabs(dz);
+    const double bound = 1.7763568394002505e-15 * perm;  // 16 * 2^-53
+    if (det > bound) return 2;
+    if (det < -bound) return 0;
+    return (orient_exact_slow(a, b, c, d) + 1) | 4;
+}
+
 __device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats& st) {
-    {
-        const float ax = a.x - c.x, ay = a.y - c.y, az = a.z - c.z;
-        const float bx = b.x - c.x, by = b.y - c.y, bz = b.z - c.z;
-        const float dx = d.x - c.x, dy = d.y - c.y, dz = d.z - c.z;
-        const float p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
-        const float det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
-        const float perm = (fabsf(p0) + fabsf(p1)) * fabsf(dx) + (fabsf(p2) + fabsf(p3)) * fabsf(dy) +
-                           (fabsf(p4) + fabsf(p5)) * fabsf(dz);
-        const float bound = 9.5367431640625e-07f * perm + 1e-30f;  // 16 * 2^-24
-        if (det > bound) return 1;
-        if (det < -bound) return -1;
-    }
+    const float ax = a.x - c.x, ay = a.y - c.y, az = a.z - c.z;
+    const float bx = b.x - c.x, by = b.y - c.y, bz = b.z - c.z;
+    const float dx = d.x - c.x, dy = d.y - c.y, dz = d.z - c.z;
+    const float p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
+    const float det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
+    const float perm = (fabsf(p0) + fabsf(p1)) * fabsf(dx) + (fabsf(p2) + fabsf(p3)) * fabsf(dy) +
+                       (fabsf(p4) + fabsf(p5)) * fabsf(dz);
+    const float bound = 9.5367431640625e-07f * perm + 1e-30f;  // 16 * 2^-24
+    if (det > bound) return 1;
+    if (det < -bound) return -1;
     ++st.fp64;
-    {
-        const double ax = double(a.x) - c.x, ay = double(a.y) - c.y, az = double(a.z) - c.z;
-        const double bx = double(b.x) - c.x, by = double(b.y) - c.y, bz = double(b.z) - c.z;
-        const double dx = double(d.x) - c.x, dy = double(d.y) - c.y, dz = double(d.z) - c.z;
-        const double p0 = ay * bz, p1 = by * az, p2 = az * bx, p3 = bz * ax, p4 = ax * by, p5 = bx * ay;
-        const double det = (p0 - p1) * dx + (p2 - p3) * dy + (p4 - p5) * dz;
-        const double perm = (fabs(p0) + fabs(p1)) * fabs(dx) + (fabs(p2) + fabs(p3)) * fabs(dy) +
-                            (fabs(p4) + fabs(p5)) * fabs(dz);
-        const double bound = 1.7763568394002505e-15 * perm;  // 16 * 2^-53
-        if (det > bound) return 1;
-        if (det < -bound) return -1;
-    }
-    ++st.exact;
-    return orient_exact_slow(a, b, c, d);
+    const int r = orient_slow(a, b, c, d);
+    if (r & 4) ++st.exact;
+    return (r & 3) - 1;
 }
 
 // Guigue & Devillers 2003 decision structure ("Fast and robust
@@ -137,20 +141,19 @@ __device__ inline int gd_canon2(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, int dp
     if (dr2 < 0) return gd_check(p1, r1, q1, r2, p2, q2, st);
     return 2;
 }
-__device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, int dsign[6], PredStats& st) {
+// dsides packs the exact sides (+1 each, 2 bits): U's vertices vs V's plane
+// in bits 0-5, V's vertices vs U's plane in bits 6-11.
+__device__ __forceinline__ int side_of(uint32_t packed, int k) { return int((packed >> (2 * k)) & 3u) - 1; }
+__device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
     const int dp1 = orient_filtered(p2, q2, r2, p1, st);
     const int dq1 = orient_filtered(p2, q2, r2, q1, st);
     const int dr1 = orient_filtered(p2, q2, r2, r1, st);
-    dsign[3] = dp1;
-    dsign[4] = dq1;
-    dsign[5] = dr1;
+    dsides = (uint32_t(dp1 + 1) << 6) | (uint32_t(dq1 + 1) << 8) | (uint32_t(dr1 + 1) << 10);
     if (dp1 * dq1 > 0 && dp1 * dr1 > 0) return 0;
     const int dp2 = orient_filtered(q1, r1, p1, p2, st);
     const int dq2 = orient_filtered(q1, r1, p1, q2, st);
     const int dr2 = orient_filtered(q1, r1, p1, r2, st);
-    dsign[0] = dp2;
-    dsign[1] = dq2;
-    dsign[2] = dr2;
+    dsides |= uint32_t(dp2 + 1) | (uint32_t(dq2 + 1) << 2) | (uint32_t(dr2 + 1) << 4);
     if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
     if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
     if (dp1 > 0) {
@@ -187,17 +190,17 @@ __device__ inline float signed_like(float v, int s) {
     if (s > 0) return v > 0.0f ? v : 1e-30f;
     return v < 0.0f ? v : -1e-30f;
 }
-__device__ inline void exact_segment(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, const int ds[6], f3& s, f3& t) {
+__device__ inline void exact_segment(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, uint32_t ds, f3& s, f3& t) {
     const f3 N1 = cross(sub(V1, V0), sub(V2, V0));
     const float d1 = -dot(N1, V0);
-    const float du0 = signed_like(dot(N1, U0) + d1, ds[0]);
-    const float du1 = signed_like(dot(N1, U1) + d1, ds[1]);
-    const float du2 = signed_like(dot(N1, U2) + d1, ds[2]);
+    const float du0 = signed_like(dot(N1, U0) + d1, side_of(ds, 0));
+    const float du1 = signed_like(dot(N1, U1) + d1, side_of(ds, 1));
+    const float du2 = signed_like(dot(N1, U2) + d1, side_of(ds, 2));
     const f3 N2 = cross(sub(U1, U0), sub(U2, U0));
     const float d2 = -dot(N2, U0);
-    const float dv0 = signed_like(dot(N2, V0) + d2, ds[3]);
-    const float dv1 = signed_like(dot(N2, V1) + d2, ds[4]);
-    const float dv2 = signed_like(dot(N2, V2) + d2, ds[5]);
+    const float dv0 = signed_like(dot(N2, V0) + d2, side_of(ds, 3));
+    const float dv1 = signed_like(dot(N2, V1) + d2, side_of(ds, 4));
+    const float dv2 = signed_like(dot(N2, V2) + d2, side_of(ds, 5));
     const f3 D = cross(N1, N2);
     float mx = fabsf(D.x);
     int index = 0;
@@ -228,6 +231,25 @@ __device__ inline void exact_segment(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, c
         s = sm2 == 0 ? B1 : B2;
         t = (i21 > i11) ? (sm1 == 0 ? A2 : A1) : (sm2 == 0 ? B2 : B1);
     }
+}
+
+// Contact record of a colliding pair, written in place (out of line: only
+// hit lanes run it).
+__device__ __noinline__ void write_contact(cd_contact* dst, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2, uint32_t ds,
+                                           uint32_t q, uint32_t ta, uint32_t tb) {
+    f3 s, t;
+    exact_segment(A0, A1, A2, B0, B1, B2, ds, s, t);
+    const PlaneF pa = plane_of(A0, A1, A2), pb = plane_of(B0, B1, B2);
+    cd_contact c;
+    c.query = q;
+    c.tri_a = ta;
+    c.tri_b = tb;
+    c.flags = outside_flags(pa, pb, A0, A1, A2, B0, B1, B2);
+    c.s[0] = s.x; c.s[1] = s.y; c.s[2] = s.z;
+    c.t[0] = t.x; c.t[1] = t.y; c.t[2] = t.z;
+    c.na[0] = pa.n.x; c.na[1] = pa.n.y; c.na[2] = pa.n.z;
+    c.nb[0] = pb.n.x; c.nb[1] = pb.n.y; c.nb[2] = pb.n.z;
+    *dst = c;
 }
 
 }  // namespace cdk

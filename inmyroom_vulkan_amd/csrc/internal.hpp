@@ -54,6 +54,18 @@ struct QueryPre {
 };
 static_assert(sizeof(QueryPre) == 96, "query record layout");
 
+// The fixed sweep frame of the broad phase (CollisionDetection.cpp:9-11).
+struct SapAxes {
+    f3 u, v, w;
+};
+__host__ __device__ inline SapAxes sap_axes() {
+    SapAxes a;
+    a.u = normalize(mk(0.8f, -0.2f, 0.f));
+    a.w = normalize(cross(a.u, mk(0.f, -1.f, 0.f)));
+    a.v = normalize(cross(a.w, a.u));
+    return a;
+}
+
 constexpr uint32_t ROOT_CODE = 0xFFFFFFFFu;  // traversal code of a tree's root
 constexpr int PLAIN_STACK = 64;              // per-lane stack entries of the plain traversal
 
@@ -123,7 +135,7 @@ struct cd_ctx {
     size_t nodes_used = 0, tris_used = 0;
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg;
-    cdk::DevBuf d_work[8];                  // scratch of the batched path and the broad phase
+    cdk::DevBuf d_work[24];                 // scratch of the batched path and the broad phase
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
     uint64_t lp_cap = 0, contact_cap = 0;

@@ -35,4 +35,17 @@ hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int
                                RunCounters* ctr, hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, uint32_t n, unsigned int v, hipStream_t st);
 
+// broad.hip
+hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* meshes, const SapAxes& ax, float* iv,
+                              uint32_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
+                             uint32_t key_bits, hipStream_t st);
+hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
+                            uint32_t cap, unsigned int* count, hipStream_t st);
+hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32_t n, const cd_entity* ents,
+                               QueryRec* qs, uint2* sorted_pairs, hipStream_t st);
+hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, uint32_t* k0, uint32_t* v0,
+                                uint32_t* k1, uint32_t* v1, hipStream_t st);
+
 }  // namespace cdk

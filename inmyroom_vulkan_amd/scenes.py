@@ -176,6 +176,57 @@ def config1(n_poses: int = 1000, seed: int = 1, freq: int = 8):
 
 
 # --------------------------------------------------------------------------
+# config 2: scene of rigid entities
+# --------------------------------------------------------------------------
+def quat_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    aw, ax, ay, az = a[..., 0], a[..., 1], a[..., 2], a[..., 3]
+    bw, bx, by, bz = b[..., 0], b[..., 1], b[..., 2], b[..., 3]
+    return np.stack([aw * bw - ax * bx - ay * by - az * bz, aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx, aw * bz + ax * by - ay * bx + az * bw], -1)
+
+
+class Scene:
+    """Rigid entities drawn from three mesh kinds, integrated at fixed dt."""
+
+    def __init__(self, n_entities: int = 512, seed: int = 2, cube: float = 50.0, dt: float = 1.0 / 60.0,
+                 torus_grid: tuple[int, int] = (128, 32)):
+        rng = np.random.default_rng(seed)
+        self.soups = [box_soup(), unit_icosphere_soup(8), torus_soup(rng, *torus_grid)]
+        n = int(n_entities)
+        self.mesh_of = rng.integers(0, 3, size=n).astype(np.uint32)
+        self.scale = rng.uniform(0.5, 3.0, size=n)
+        self.x = rng.uniform(0.0, cube, size=(n, 3))
+        self.q = random_quaternions(rng, n)
+        self.v = rng.normal(0.0, 2.0, size=(n, 3))
+        self.w = rng.normal(0.0, 1.0, size=(n, 3))
+        self.dt = dt
+        self.should_callback = np.ones(n, np.uint8)
+        self.prev = self.matrices()
+
+    def matrices(self) -> np.ndarray:
+        return pose_matrices(self.x, self.q, self.scale)
+
+    def step(self):
+        """x += v dt; q <- q * exp(w dt / 2), renormalised."""
+        self.prev = self.matrices()
+        self.x = self.x + self.v * self.dt
+        ang = np.linalg.norm(self.w, axis=1) * self.dt / 2
+        axis = self.w / np.maximum(np.linalg.norm(self.w, axis=1, keepdims=True), 1e-30)
+        dq = np.concatenate([np.cos(ang)[:, None], axis * np.sin(ang)[:, None]], axis=1)
+        q = quat_mul(self.q, dq)
+        self.q = q / np.linalg.norm(q, axis=1, keepdims=True)
+
+    def entities(self, mesh_ids, dtype) -> np.ndarray:
+        e = np.zeros(len(self.mesh_of), dtype)
+        e["current"] = self.matrices()
+        e["previous"] = self.prev
+        e["mesh_id"] = np.asarray(mesh_ids, np.uint32)[self.mesh_of]
+        e["entity"] = np.arange(len(self.mesh_of), dtype=np.uint16)
+        e["should_callback"] = self.should_callback
+        return e
+
+
+# --------------------------------------------------------------------------
 # config 3: 4,096 uneven mesh-pair queries
 # --------------------------------------------------------------------------
 def config3_sizes(n_queries: int = 4096, seed: int = 3, tmin: float = 12.0, tmax: float = 65536.0):

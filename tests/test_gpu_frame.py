@@ -1,0 +1,84 @@
+"""Broad phase and whole frames on the GPU against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import case
+from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(n, seed, cube=25.0):
+    return scenes.Scene(n_entities=n, seed=seed, cube=cube, torus_grid=(32, 16))
+
+
+def test_broad_phase_matches_oracle(gpu, orc):
+    sc = _scene(300, 21)
+    sc.should_callback[::7] = 0  # a pair is kept only if one of its entities wants callbacks
+    case("broad phase n=300")
+    ids = gpu.register_soups(sc.soups)
+    trees = [orc.tree(s, 4) for s in sc.soups]
+    for frame in range(3):
+        ents = sc.entities(ids, ENTITY_DTYPE)
+        got = [tuple(p) for p in gpu.broad_phase(ents).tolist()]
+        want = [tuple(p) for p in orc.frame(trees, sc.mesh_of, ents["current"], sc.should_callback, mode=0).pairs.tolist()]
+        assert got == want, frame  # same pairs, same orientation, same (first, second) order
+        assert len(want) > 5
+        sc.step()
+
+
+def test_frame_plain_matches_oracle(gpu, orc):
+    sc = _scene(200, 22, cube=18.0)
+    case("frame plain n=200")
+    ids = gpu.register_soups(sc.soups)
+    trees = [orc.tree(s, 4) for s in sc.soups]
+    for frame in range(3):
+        ents = sc.entities(ids, ENTITY_DTYPE)
+        pairs, cons = gpu.frame(ents, MODE_PLAIN)
+        res = orc.frame(trees, sc.mesh_of, ents["current"], sc.should_callback, mode=0)
+        want = {}
+        for c in res.contacts:
+            a, b = res.pairs[c["query"]]
+            want[(int(a), int(b), int(c["tri_a"]), int(c["tri_b"]))] = c
+        got = {}
+        for p in pairs:
+            for c in cons[p["contact_begin"]: p["contact_begin"] + p["contact_count"]]:
+                got[(int(p["first"]), int(p["second"]), int(c["tri_a"]), int(c["tri_b"]))] = c
+        assert set(got) == set(want), frame
+        for k, c in want.items():
+            g = got[k]
+            assert g["s"].tobytes() == c["s"].tobytes() and g["t"].tobytes() == c["t"].tobytes()
+            assert g["flags"] == c["flags"] and g["na"].tobytes() == c["na"].tobytes()
+        # pairs without contacts are dropped, the rest are in (first, second) order
+        keys = [(int(p["first"]), int(p["second"])) for p in pairs]
+        assert keys == sorted(keys) and all(p["contact_count"] > 0 for p in pairs)
+        assert len(want) > 0
+        sc.step()
+
+
+def test_frame_batched_matches_exact_oracle(gpu, orc):
+    sc = _scene(200, 23, cube=18.0)
+    case("frame batched n=200")
+    ids = gpu.register_soups(sc.soups)
+    trees = [orc.tree(s, 4) for s in sc.soups]
+    ents = sc.entities(ids, ENTITY_DTYPE)
+    pairs, cons = gpu.frame(ents, MODE_BATCHED)
+    res = orc.frame(trees, sc.mesh_of, ents["current"], sc.should_callback, mode=1)
+    want = {(int(res.pairs[c["query"]][0]), int(res.pairs[c["query"]][1]), int(c["tri_a"]), int(c["tri_b"]))
+            for c in res.contacts}
+    got = set()
+    for p in pairs:
+        for c in cons[p["contact_begin"]: p["contact_begin"] + p["contact_count"]]:
+            got.add((int(p["first"]), int(p["second"]), int(c["tri_a"]), int(c["tri_b"])))
+    assert got == want and len(want) > 0
+
+
+def test_frame_small_inputs(gpu):
+    case("frame with fewer than two entities")
+    ids = gpu.register_soups([scenes.box_soup()])
+    e = np.zeros(1, ENTITY_DTYPE)
+    e["current"] = scenes.identity16()
+    e["mesh_id"] = ids[0]
+    pairs, cons = gpu.frame(e, MODE_PLAIN)
+    assert len(pairs) == 0 and len(cons) == 0
+    assert len(gpu.broad_phase(e[:0])) == 0
