@@ -147,6 +147,26 @@ int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out);
 int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, uint32_t* node_links,
                    uint32_t* perm);
 
+/* Tree-order triangle positions of a mesh (9 floats each; after a refit, the
+ * skinned positions).  The reference keeps them in OBBtree::triangles_position
+ * (include/Geometry/OBBtree.h:132).                                         */
+int cd_export_triangles(cd_ctx* ctx, uint32_t mesh, float* tri9);
+
+/* ---- deformable meshes ---------------------------------------------------------- */
+/* No counterpart in the reference: it gives trees only to non-skinned
+ * primitives (src/Graphics/Meshes/PrimitivesOfMeshes.cpp:642) and never
+ * refits.  A skinned mesh is registered in its bind pose (the tree is built
+ * as cd_register_mesh builds it) with 4 joints and 4 weights per vertex.    */
+int cd_register_skinned(cd_ctx* ctx, const float* bind_positions, uint32_t nvert, const uint32_t* indices,
+                        uint32_t ntri, const uint16_t* joints4, const float* weights4, uint32_t nbones,
+                        uint32_t* mesh_id);
+/* Skin the listed meshes with their bone matrices (nbones x 16 floats per
+ * mesh, concatenated in list order; each = joint matrix * inverse bind, the
+ * product the skinning shader forms, shaders/dynamicMeshShader_glsl.comp:120-145)
+ * and refit every box of their trees over the skinned triangles (same PCA
+ * construction as the build; topology kept).                               */
+int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float* bone_matrices);
+
 /* ---- queries ------------------------------------------------------------------- */
 /* Triangle::IntersectTriangles (src/Geometry/Triangle.cpp:85-110) over n
  * pairs.  flags: bit0 hit, bit1 coplanar, bit2 undefined-in-reference case

@@ -28,7 +28,7 @@ EXPORTS = (
     "cd_create", "cd_destroy", "cd_last_error", "cd_register_mesh", "cd_register_soups", "cd_mesh_info_get",
     "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
     "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
-    "cd_set_tuning", "cd_get_tuning",
+    "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -106,6 +106,9 @@ def load_library() -> C.CDLL:
     lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
     lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
     lib.cd_set_tuning.argtypes = [vp, P(cd_tuning)]
+    lib.cd_export_triangles.argtypes = [vp, u32, vp]
+    lib.cd_register_skinned.argtypes = [vp, vp, u32, vp, u32, vp, vp, u32, P(u32)]
+    lib.cd_skin_refit.argtypes = [vp, vp, u32, vp]
     lib.cd_get_tuning.argtypes = [vp, P(cd_tuning)]
     for name in EXPORTS:
         if name != "cd_last_error":
@@ -184,6 +187,27 @@ class Collider:
         self._check(self.lib.cd_export_tree(self.h, mesh, _ptr(root), _ptr(obbs), _ptr(links), _ptr(perm)),
                     "cd_export_tree")
         return {"root": root, "obbs": obbs[:n], "links": links[:n], "perm": perm[:t], "depth": info["depth"]}
+
+    def export_triangles(self, mesh: int) -> np.ndarray:
+        n = self.mesh_info(mesh)["ntri"]
+        out = np.zeros((max(n, 1), 9), np.float32)
+        self._check(self.lib.cd_export_triangles(self.h, mesh, _ptr(out)), "cd_export_triangles")
+        return out[:n]
+
+    def register_skinned(self, positions, indices, joints, weights, nbones: int) -> int:
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        jt = np.ascontiguousarray(joints, np.uint16).reshape(-1, 4)
+        wt = np.ascontiguousarray(weights, np.float32).reshape(-1, 4)
+        mid = C.c_uint32()
+        self._check(self.lib.cd_register_skinned(self.h, _ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(jt), _ptr(wt),
+                                                 nbones, C.byref(mid)), "cd_register_skinned")
+        return mid.value
+
+    def skin_refit(self, mesh_ids, bone_matrices):
+        ids = np.ascontiguousarray(mesh_ids, np.uint32)
+        bones = np.ascontiguousarray(bone_matrices, np.float32)
+        self._check(self.lib.cd_skin_refit(self.h, _ptr(ids), len(ids), _ptr(bones)), "cd_skin_refit")
 
     # -- queries --------------------------------------------------------------
     def tritri(self, a9, b9, mode: int = MODE_PLAIN):

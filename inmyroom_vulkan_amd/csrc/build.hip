@@ -371,7 +371,9 @@ __global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t*
         }
     }
     rec.cnt = cnt;
-    rec.pad[0] = rec.pad[1] = rec.pad[2] = 0;
+    rec.pad[0] = nd.toff;    // this node's subtree: triangles [toff, toff + tcount) in tree order
+    rec.pad[1] = nd.tcount;
+    rec.pad[2] = 0;
     out[node_base[nd.mesh] + nd.pre] = rec;
 }
 

@@ -286,6 +286,27 @@ int stage(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
     return CD_OK;
 }
 
+// Grow a device buffer, keeping its contents.
+hipError_t grow_keep(DevBuf& b, size_t need, hipStream_t st) {
+    if (need <= b.bytes) return hipSuccess;
+    const size_t want = std::max(need, b.bytes * 2);
+    void* np = nullptr;
+    hipError_t e = hipMalloc(&np, want);
+    if (e != hipSuccess) return e;
+    if (b.p) {
+        e = hipMemcpyAsync(np, b.p, b.bytes, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            (void)hipFree(np);
+            return e;
+        }
+        (void)hipFree(b.p);
+    }
+    b.p = np;
+    b.bytes = want;
+    return hipSuccess;
+}
+
 // Sweep and prune on the GPU.  Leaves the oriented pairs, sorted by
 // (first, second), in *hp and the matching queries (relative matrices) in
 // d_queries.
@@ -395,7 +416,9 @@ int cd_destroy(cd_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_perm, &ctx->d_queries, &ctx->d_leafpairs,
-                      &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg})
+                      &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg,
+                      &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
+                      &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits})
         b->release();
     for (auto& b : ctx->d_work) b.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -655,6 +678,146 @@ int cd_broad_phase(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t*
         pairs2[2 * i] = hp[i].x;
         pairs2[2 * i + 1] = hp[i].y;
     }
+    return CD_OK;
+}
+
+int cd_export_triangles(cd_ctx* ctx, uint32_t mesh, float* tri9) {
+    if (!ctx) return CD_E_INVALID;
+    if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");
+    const MeshRec& m = ctx->meshes[mesh];
+    if (m.ntri && !tri9) return fail(ctx, CD_E_INVALID, "null output");
+    if (m.ntri)
+        HCK(ctx, hipMemcpyAsync(tri9, ctx->d_tris.as<float>() + size_t(m.tri_base) * 9, size_t(m.ntri) * 9 * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+int cd_register_skinned(cd_ctx* ctx, const float* bind_positions, uint32_t nvert, const uint32_t* indices,
+                        uint32_t ntri, const uint16_t* joints4, const float* weights4, uint32_t nbones,
+                        uint32_t* mesh_id) {
+    if (!ctx || !mesh_id) return CD_E_INVALID;
+    if (nbones == 0 || nbones > 0xFFFFu) return fail(ctx, CD_E_INVALID, "bone count out of range");
+    if (nvert && (!bind_positions || !joints4 || !weights4)) return fail(ctx, CD_E_INVALID, "null skin arrays");
+    for (size_t i = 0; i < size_t(nvert) * 4; ++i)
+        if (joints4[i] >= nbones) return fail(ctx, CD_E_INVALID, "joint index out of range");
+    if (!all_finite(weights4, size_t(nvert) * 4)) return fail(ctx, CD_E_INVALID, "non-finite weight");
+    uint32_t id = 0;
+    int rc = cd_register_mesh(ctx, bind_positions, nvert, indices, ntri, &id);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    const size_t vbase = ctx->skin_verts, vidx_base = ctx->skin_vidx;
+    if (vbase + nvert >= 0xFFFFFFF0ull || vidx_base + size_t(ntri) * 3 >= 0xFFFFFFF0ull)
+        return fail(ctx, CD_E_INVALID, "skinned vertex arena exceeds 32-bit indexing");
+    HCK(ctx, grow_keep(ctx->d_bind, (vbase + nvert + 1) * 3 * sizeof(float), st));
+    HCK(ctx, grow_keep(ctx->d_joints, (vbase + nvert + 1) * sizeof(ushort4), st));
+    HCK(ctx, grow_keep(ctx->d_weights, (vbase + nvert + 1) * sizeof(float4), st));
+    HCK(ctx, grow_keep(ctx->d_vidx, (vidx_base + size_t(ntri) * 3 + 1) * sizeof(uint32_t), st));
+    HCK(ctx, grow_keep(ctx->d_skinpos, (vbase + nvert + 1) * 3 * sizeof(float), st));
+    if (nvert) {
+        HCK(ctx, hipMemcpyAsync(ctx->d_bind.as<float>() + vbase * 3, bind_positions, size_t(nvert) * 3 * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_skinpos.as<float>() + vbase * 3, bind_positions, size_t(nvert) * 3 * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_joints.as<ushort4>() + vbase, joints4, size_t(nvert) * sizeof(ushort4),
+                                hipMemcpyHostToDevice, st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_weights.as<float4>() + vbase, weights4, size_t(nvert) * sizeof(float4),
+                                hipMemcpyHostToDevice, st));
+    }
+    if (ntri) {
+        DevBuf idx;
+        HCK(ctx, idx.ensure(size_t(ntri) * 3 * sizeof(uint32_t)));
+        HCK(ctx, hipMemcpyAsync(idx.p, indices, size_t(ntri) * 3 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        const MeshRec& m = ctx->meshes[id];
+        HCK(ctx, launch_tree_vidx(ctx->d_perm.as<uint32_t>() + m.tri_base, ntri, idx.as<uint32_t>(), uint32_t(vbase),
+                                  ctx->d_vidx.as<uint32_t>() + vidx_base, st));
+        HCK(ctx, hipStreamSynchronize(st));
+        idx.release();
+    }
+    ctx->skins.push_back({id, cd_ctx::SkinHost{uint32_t(vbase), nvert, nbones, uint32_t(vidx_base)}});
+    ctx->skin_verts += nvert;
+    ctx->skin_vidx += size_t(ntri) * 3;
+    *mesh_id = id;
+    return CD_OK;
+}
+
+int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float* bone_matrices) {
+    if (!ctx) return CD_E_INVALID;
+    if (n == 0) return CD_OK;
+    if (!mesh_ids || !bone_matrices) return fail(ctx, CD_E_INVALID, "null argument");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    std::vector<SkinRec> recs(n);
+    std::vector<uint32_t> vstart(n), tstart(n);
+    uint32_t vtot = 0, ttot = 0, btot = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const cd_ctx::SkinHost* sh = nullptr;
+        for (const auto& kv : ctx->skins)
+            if (kv.first == mesh_ids[i]) sh = &kv.second;
+        if (!sh) return fail(ctx, CD_E_INVALID, "mesh is not a registered skinned mesh");
+        const MeshRec& m = ctx->meshes[mesh_ids[i]];
+        recs[i] = SkinRec{mesh_ids[i], sh->vbase, sh->nvert, sh->nbones, btot, m.tri_base, m.ntri, sh->vidx_base};
+        vstart[i] = vtot;
+        tstart[i] = ttot;
+        vtot += sh->nvert;
+        ttot += m.ntri;
+        btot += sh->nbones;
+    }
+    if (!all_finite(bone_matrices, size_t(btot) * 16)) return fail(ctx, CD_E_INVALID, "non-finite bone matrix");
+    const size_t rec_bytes = n * sizeof(SkinRec), pre_bytes = n * sizeof(uint32_t);
+    HCK(ctx, ctx->d_skinrec.ensure(rec_bytes + 2 * pre_bytes));
+    char* base = ctx->d_skinrec.as<char>();
+    HCK(ctx, hipMemcpyAsync(base, recs.data(), rec_bytes, hipMemcpyHostToDevice, st));
+    HCK(ctx, hipMemcpyAsync(base + rec_bytes, vstart.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    HCK(ctx, hipMemcpyAsync(base + rec_bytes + pre_bytes, tstart.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    HCK(ctx, ctx->d_bones.ensure(size_t(btot) * 16 * sizeof(float)));
+    HCK(ctx, hipMemcpyAsync(ctx->d_bones.p, bone_matrices, size_t(btot) * 16 * sizeof(float), hipMemcpyHostToDevice, st));
+    const SkinRec* dsk = reinterpret_cast<const SkinRec*>(base);
+    const uint32_t* dvs = reinterpret_cast<const uint32_t*>(base + rec_bytes);
+    const uint32_t* dts = reinterpret_cast<const uint32_t*>(base + rec_bytes + pre_bytes);
+    // refit items: the root and both child boxes of every internal node (cached per mesh list)
+    std::vector<uint32_t> key(mesh_ids, mesh_ids + n);
+    if (key != ctx->refit_key) {
+        std::vector<uint2> items;
+        for (uint32_t i = 0; i < n; ++i) {
+            const MeshRec& m = ctx->meshes[mesh_ids[i]];
+            items.push_back(make_uint2(mesh_ids[i], ROOT_CODE));
+            for (uint32_t nd = 0; nd < m.nnodes; ++nd) {
+                items.push_back(make_uint2(mesh_ids[i], nd << 1));
+                items.push_back(make_uint2(mesh_ids[i], (nd << 1) | 1u));
+            }
+        }
+        HCK(ctx, ctx->d_refit_items.ensure(items.size() * sizeof(uint2) + n * sizeof(uint32_t)));
+        HCK(ctx, hipMemcpyAsync(ctx->d_refit_items.p, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_refit_items.as<char>() + items.size() * sizeof(uint2), mesh_ids,
+                                n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        ctx->refit_key = key;
+        ctx->refit_nitems = uint32_t(items.size());
+    }
+    const uint32_t nm = uint32_t(ctx->meshes.size());
+    HCK(ctx, ctx->d_refit_bits.ensure(2 * size_t(nm) * sizeof(unsigned int)));
+    unsigned int* bits = ctx->d_refit_bits.as<unsigned int>();
+    HCK(ctx, hipEventRecord(ctx->ev[0], st));
+    HCK(ctx, launch_skin(dsk, n, dvs, ctx->d_bind.as<float>(), ctx->d_joints.as<ushort4>(), ctx->d_weights.as<float4>(),
+                         ctx->d_bones.as<float>(), vtot, ctx->d_skinpos.as<float>(), st));
+    HCK(ctx, launch_gather_tris(dsk, n, dts, ctx->d_vidx.as<uint32_t>(), ctx->d_skinpos.as<float>(), ctx->d_tris.as<float>(),
+                                ttot, st));
+    HCK(ctx, launch_fill_u32(bits, 2 * nm, 0u, st));
+    const uint2* items = ctx->d_refit_items.as<uint2>();
+    const uint32_t* ids = reinterpret_cast<const uint32_t*>(ctx->d_refit_items.as<char>() + ctx->refit_nitems * sizeof(uint2));
+    HCK(ctx, launch_refit(items, ctx->refit_nitems, ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
+                          ctx->d_tris.as<float>(), bits, bits + nm, ids, n, st));
+    HCK(ctx, hipEventRecord(ctx->ev[1], st));
+    // keep the host copy of the mesh table in step (root boxes, slack, radius)
+    for (uint32_t i = 0; i < n; ++i)
+        HCK(ctx, hipMemcpyAsync(&ctx->meshes[mesh_ids[i]], ctx->d_meshes.as<MeshRec>() + mesh_ids[i], sizeof(MeshRec),
+                                hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->timings = cd_timings{};
+    ctx->timings.total_ms = ms;
+    ctx->timings.broad_ms = ms;
     return CD_OK;
 }
 

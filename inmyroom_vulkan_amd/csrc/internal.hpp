@@ -21,7 +21,7 @@ struct alignas(128) NodeRec {
     uint32_t link[2];   // child node index (count 0) or mesh-local triangle offset
     uint32_t cnt;       // left count | right count << 16 (0 = internal child)
     float surf[2];      // surface(box[0]), surface(box[1])
-    uint32_t pad[3];
+    uint32_t pad[3];    // [0], [1]: this node's own triangle range (offset, count); used by refit
 };
 static_assert(sizeof(NodeRec) == 128, "node record must stay one 128-byte line");
 
@@ -65,6 +65,11 @@ __host__ __device__ inline SapAxes sap_axes() {
     a.v = normalize(cross(a.w, a.u));
     return a;
 }
+
+// One skinned mesh in a skin/refit launch.
+struct SkinRec {
+    uint32_t mesh, vbase, nvert, nbones, bone_base, tri_base, ntri, vidx_base;
+};
 
 constexpr uint32_t ROOT_CODE = 0xFFFFFFFFu;  // traversal code of a tree's root
 constexpr int PLAIN_STACK = 64;              // per-lane stack entries of the plain traversal
@@ -144,6 +149,14 @@ struct cd_ctx {
     cd_timings timings{};
     cd_tuning tuning{};
     int num_cu = 256;
+    // skinned meshes: bind pose, joints, weights, tree-order vertex indices
+    struct SkinHost { uint32_t vbase, nvert, nbones, vidx_base; };
+    std::vector<std::pair<uint32_t, SkinHost>> skins;  // (mesh id, data)
+    cdk::DevBuf d_bind, d_joints, d_weights, d_vidx, d_skinpos, d_bones, d_skinrec, d_refit_items, d_refit_bits;
+    size_t skin_verts = 0, skin_vidx = 0;
+    std::vector<uint32_t> refit_key;       // mesh list the cached refit items are for
+    uint32_t refit_nitems = 0;
+
     hipGraphExec_t graph = nullptr;        // captured batched pipeline
     std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured
     hipEvent_t ev[8] = {};

@@ -48,4 +48,15 @@ hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32
 hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, uint32_t* k0, uint32_t* v0,
                                 uint32_t* k1, uint32_t* v1, hipStream_t st);
 
+// refit.hip
+hipError_t launch_tree_vidx(const uint32_t* perm, uint32_t ntri, const uint32_t* indices, uint32_t vbase,
+                            uint32_t* out, hipStream_t st);
+hipError_t launch_skin(const SkinRec* sk, uint32_t nsk, const uint32_t* vstart, const float* bind, const ushort4* joints,
+                       const float4* weights, const float* bones, uint32_t total, float* out, hipStream_t st);
+hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* tstart, const uint32_t* vidx,
+                              const float* pos, float* tris, uint32_t total, hipStream_t st);
+hipError_t launch_refit(const uint2* items, uint32_t nitems, MeshRec* meshes, NodeRec* nodes, const float* tris,
+                        unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids, uint32_t nids,
+                        hipStream_t st);
+
 }  // namespace cdk

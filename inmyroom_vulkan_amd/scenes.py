@@ -259,6 +259,109 @@ def config3(n_queries: int = 4096, seed: int = 3, amp: float = 0.1, overlap_p: f
 
 
 # --------------------------------------------------------------------------
+# config 5: deformable batch against a terrain
+# --------------------------------------------------------------------------
+def torus_indexed(nu: int, nv: int, R: float = 1.0, r: float = 0.3):
+    u = np.arange(nu) * (2 * np.pi / nu)
+    v = np.arange(nv) * (2 * np.pi / nv)
+    U, Vv = np.meshgrid(u, v, indexing="ij")
+    P = np.stack([(R + r * np.cos(Vv)) * np.cos(U), (R + r * np.cos(Vv)) * np.sin(U), r * np.sin(Vv)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a = i * nv + j
+    b = ((i + 1) % nu) * nv + j
+    c = ((i + 1) % nu) * nv + (j + 1) % nv
+    d = i * nv + (j + 1) % nv
+    F = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)]).astype(np.uint32)
+    return P, F, U.reshape(-1)
+
+
+def heightfield(n: int = 316, size: float = 100.0, amp: float = 5.0, octaves: int = 6, seed: int = 5):
+    """n x n vertex grid over [0, size]^2 with fBm heights in [-amp, amp]:
+    2 (n-1)^2 triangles."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(0.0, size, n)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    h = np.zeros_like(X)
+    a = 1.0
+    for o in range(octaves):
+        res = 2 ** (o + 2)
+        g = rng.uniform(-1, 1, size=(res + 1, res + 1))
+        fx, fy = X / size * res, Y / size * res
+        ix, iy = np.minimum(fx.astype(int), res - 1), np.minimum(fy.astype(int), res - 1)
+        tx, ty = fx - ix, fy - iy
+        tx, ty = tx * tx * (3 - 2 * tx), ty * ty * (3 - 2 * ty)
+        v = (g[ix, iy] * (1 - tx) * (1 - ty) + g[ix + 1, iy] * tx * (1 - ty) + g[ix, iy + 1] * (1 - tx) * ty
+             + g[ix + 1, iy + 1] * tx * ty)
+        h += a * v
+        a *= 0.5
+    h *= amp / np.abs(h).max()
+    P = np.stack([X, Y, h], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a0, b0, c0, d0 = i * n + j, (i + 1) * n + j, (i + 1) * n + j + 1, i * n + j + 1
+    F = np.concatenate([np.stack([a0, b0, c0], -1).reshape(-1, 3), np.stack([a0, c0, d0], -1).reshape(-1, 3)]).astype(np.uint32)
+    return P, F, h
+
+
+def rotation_about(axis: np.ndarray, angle: np.ndarray) -> np.ndarray:
+    axis = axis / np.linalg.norm(axis, axis=-1, keepdims=True)
+    h = angle / 2
+    return quat_to_mat3(np.concatenate([np.cos(h)[..., None], axis * np.sin(h)[..., None]], -1))
+
+
+class DeformableBatch:
+    """n_meshes skinned tori (nu x nv grid, 2 nu nv triangles) with n_bones
+    bones on the ring; each vertex takes its 4 nearest bones with
+    Dirichlet(1,1,1,1) weights; every frame each bone turns about its rest
+    point by a random angle in [0, max_deg] degrees.  The tori sit on an fBm
+    heightfield terrain."""
+
+    def __init__(self, n_meshes: int = 64, nu: int = 100, nv: int = 100, n_bones: int = 64, seed: int = 5,
+                 terrain_n: int = 316, terrain_size: float = 100.0, max_deg: float = 30.0):
+        rng = np.random.default_rng(seed)
+        self.rng = rng
+        self.P, self.F, ring = torus_indexed(nu, nv)
+        self.n_meshes, self.n_bones, self.max_deg = n_meshes, n_bones, max_deg
+        theta = np.arange(n_bones) * (2 * np.pi / n_bones)
+        self.bone_pos = np.stack([np.cos(theta), np.sin(theta), np.zeros(n_bones)], -1)
+        d = np.abs(((ring[:, None] - theta[None, :]) + np.pi) % (2 * np.pi) - np.pi)
+        self.joints = np.argsort(d, axis=1)[:, :4].astype(np.uint16)
+        self.weights = rng.dirichlet(np.ones(4), size=len(self.P)).astype(np.float32)
+        self.tP, self.tF, h = heightfield(terrain_n, terrain_size, 5.0, seed=seed + 1)
+        # world placement: uniform scale 1-2, on the terrain, random heading, half sunk
+        self.scale = rng.uniform(1.0, 2.0, n_meshes)
+        xy = rng.uniform(10.0, terrain_size - 10.0, size=(n_meshes, 2))
+        gi = np.clip(np.rint(xy / terrain_size * (terrain_n - 1)).astype(int), 0, terrain_n - 1)
+        z = h[gi[:, 0], gi[:, 1]]
+        self.world_t = np.concatenate([xy, z[:, None]], 1)
+        self.world_q = random_quaternions(rng, n_meshes)
+
+    def bone_matrices(self) -> np.ndarray:
+        """(n_meshes, n_bones, 16) column-major: World * T(b) * R * T(-b)."""
+        nm, nb = self.n_meshes, self.n_bones
+        axis = self.rng.normal(size=(nm, nb, 3))
+        ang = np.radians(self.rng.uniform(0.0, self.max_deg, size=(nm, nb)))
+        R = rotation_about(axis, ang)
+        Wr = quat_to_mat3(self.world_q) * self.scale[:, None, None]
+        M = np.zeros((nm, nb, 4, 4))
+        b = self.bone_pos[None]
+        M[..., :3, :3] = Wr[:, None] @ R
+        local_t = b - (R @ b[..., None])[..., 0]  # T(b) R T(-b) translation
+        M[..., :3, 3] = (Wr[:, None] @ local_t[..., None])[..., 0] + self.world_t[:, None]
+        M[..., 3, 3] = 1.0
+        return np.ascontiguousarray(M.transpose(0, 1, 3, 2).reshape(nm, nb, 16).astype(np.float32))
+
+    @staticmethod
+    def skin_reference(P, joints, weights, bones) -> np.ndarray:
+        """float64 linear blend skinning, for tests."""
+        B = bones.astype(np.float64).reshape(-1, 4, 4).transpose(0, 2, 1)  # row-major
+        ph = np.concatenate([P, np.ones((len(P), 1))], 1)
+        out = np.zeros((len(P), 3))
+        for k in range(4):
+            out += weights[:, k:k + 1].astype(np.float64) * (B[joints[:, k].astype(int)] @ ph[..., None])[:, :3, 0]
+        return out
+
+
+# --------------------------------------------------------------------------
 # config 4: precision stress
 # --------------------------------------------------------------------------
 def config4(n_pairs: int = 1 << 24, seed: int = 4, span: float = 1e3):
