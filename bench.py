@@ -60,6 +60,24 @@ def algorithmic_bytes(counters: dict, leafpairs: np.ndarray, nq: int) -> tuple[f
     return trav, narrow
 
 
+def copy_probe(nbytes: int, reps: int = 3) -> None:
+    """Device-to-device copies of a known size through the HIP runtime, so a
+    rocprofv3 --pmc run of this process can scale FETCH_SIZE / WRITE_SIZE of
+    the collision kernels by what the counters read for a plain copy."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {reps} x {nbytes} bytes device to device")
+
+
 def cpu_baseline(soups, qa, qb, rel, budget_s: float, leaf_max: int):
     """Oracle, one thread, on an evenly spaced sample of the queries."""
     from oracle import Oracle
@@ -98,6 +116,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--exact-sample", type=int, default=48, help="queries checked against the exact oracle")
+    ap.add_argument("--copy-probe", type=int, default=0, help="bytes of a calibration copy run first (PMC runs)")
+    ap.add_argument("--tuning", default="", help="override tuning fields, e.g. passes=1,split_levels=8")
+    ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5),
+                    help="3: the headline batch; 2, 4, 5: scene frames, triangle-pair stream, deformable batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the batch shards by query "
@@ -106,6 +128,13 @@ def main() -> int:
 
     from inmyroom_vulkan_amd import MODE_BATCHED, MODE_PLAIN, QUERY_DTYPE, Collider, autotune
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+    if args.config != 3:
+        import bench_configs
+
+        print(json.dumps(bench_configs.run(args)), flush=True)
+        return 0
     soups, qa, qb, rel, overlap = make_workload(args.nq, args.seed)
     gpu = Collider(leaf_max=4)
     t0 = time.time()
@@ -138,6 +167,9 @@ def main() -> int:
     # ---- batched: tuning from the cached table, else defaults ----
     tuning = autotune.lookup(args.nq)
     tuning = dict(tuning or {})
+    for kv in filter(None, args.tuning.split(",")):
+        k, v = kv.split("=")
+        tuning[k.strip()] = int(v)
     tuning["use_graph"] = 0
     gpu.set_tuning(**tuning)
     gpu.run_staged(MODE_BATCHED)

@@ -120,7 +120,10 @@ typedef struct cd_tuning {
     uint32_t pop_width;      /* node pairs a wavefront pops per step, 1..64 */
     uint32_t waves_per_cu;   /* persistent wavefronts per compute unit */
     uint32_t use_graph;      /* 1: capture the batched pipeline in a HIP graph and replay it */
-    uint32_t reserved[3];
+    uint32_t passes;         /* persistent-kernel launches, 1..8; before the last, busy wavefronts hand their
+                                stacks on once idle_pct percent of the grid has run out of work */
+    uint32_t idle_pct;       /* 1..100 */
+    uint32_t reserved;
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

@@ -2,7 +2,9 @@
 
 Searches split levels (work-split granularity), LDS stack entries per
 wavefront (stack depth), node pairs popped per step (lanes per step of the
-wavefront traversal) and persistent wavefronts per CU, timing each candidate
+wavefront traversal), persistent wavefronts per CU and the rebalancing
+schedule (traversal launches, and the share of idle wavefronts at which the
+busy ones hand their stacks to the next launch), timing each candidate
 with the library's own HIP events on staged queries, and caches the winner
 per (device architecture, workload size class) in tuning_gfx950.json.
 
@@ -21,11 +23,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
 SPACE = {
-    "split_levels": [4, 6, 8, 10, 12],
-    "stack_entries": [512, 1024, 2048],
-    "pop_width": [32, 64],
-    "waves_per_cu": [4, 8, 16],
+    "split_levels": [6, 8, 10, 14],
+    "stack_entries": [512, 1024],
+    "pop_width": [64],
+    "waves_per_cu": [4, 8],
+    # joint keys: each value sets both fields
+    "passes/idle_pct": [(1, 50), (2, 50), (3, 25), (3, 50), (3, 75), (5, 50)],
 }
+
+
+def _expand(keys, combo) -> dict:
+    t = {}
+    for k, v in zip(keys, combo):
+        if "/" in k:
+            t.update(zip(k.split("/"), v))
+        else:
+            t[k] = v
+    return t
 
 
 def size_class(nq: int) -> str:
@@ -51,7 +65,7 @@ def tune(gpu, queries, reps: int = 3, space: dict | None = None, log=print) -> d
     results = []
     keys = list(space)
     for combo in itertools.product(*(space[k] for k in keys)):
-        t = dict(zip(keys, combo))
+        t = _expand(keys, combo)
         gpu.set_tuning(**t, use_graph=0)
         gpu.run_staged(MODE_BATCHED)  # warm
         ms = []
@@ -95,7 +109,8 @@ def main():
     q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
     space = None
     if args.quick:
-        space = {"split_levels": [6, 10], "stack_entries": [1024], "pop_width": [64], "waves_per_cu": [8]}
+        space = {"split_levels": [6, 10], "stack_entries": [512], "pop_width": [64], "waves_per_cu": [4],
+                 "passes/idle_pct": [(1, 50), (3, 50)]}
     res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
     save(args.nq, res, args.out)
     print(json.dumps(res), flush=True)

@@ -8,7 +8,11 @@
 //  k_dfs          persistent wavefronts; each pops up to 64 node pairs at a
 //                 time from a stack in LDS (wave-private spill to HBM), tests
 //                 them, pushes children with ballot/prefix compaction, and
-//                 takes new tasks from a global counter when it runs low
+//                 takes new tasks from a global counter when it runs low.
+//                 Launched a few times in a row: in all but the last launch,
+//                 once a set number of wavefronts has gone idle the busy ones
+//                 dump their stacks, and the next launch shares them out
+//                 (no wavefront ever waits on another)
 //  k_narrow_exact candidate leaf pairs expanded densely to triangle tests
 //                 (wave prefix scan), exact-sign triangle test, contacts
 //                 compacted with one atomic per wavefront
@@ -78,61 +82,86 @@ __device__ inline bool sat_conservative(const Box& l, const Box& r, float sigma)
     return true;
 }
 
+// Traversal codes of the batched path are absolute, so a node pair's two
+// records can be fetched straight from the codes, together with the query's
+// record, in one round trip to memory: ROOT_FLAG | mesh for a tree's root
+// (its box lives in the mesh record), else (node << 1) | side with node an
+// index into the whole node arena.
+constexpr uint32_t ROOT_FLAG = 0x80000000u;
+
 struct Code {  // resolved traversal code
     Box box;
     float surf;
     bool leaf;
-    uint32_t link, cnt;
+    uint32_t link, cnt;  // link: mesh-local child node index, or triangle offset of a leaf
 };
-__device__ __forceinline__ Code resolve(const MeshRec* mesh, const NodeRec* nodes, uint32_t code) {
+__device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
+                                        uint32_t code) {
+    // Both layouts are read through selected addresses, so every lane runs
+    // the same load instructions whatever kind of code it holds.
+    const bool root = (code & ROOT_FLAG) != 0u;
+    const uint32_t side = code & 1u;
+    const MeshRec* mr = meshes + (root ? (code & ~ROOT_FLAG) : 0u);
+    const NodeRec* nr = nodes + (root ? 0u : (code >> 1));
+    const float* bx = root ? mr->root : nr->box[side];
+    const float* sf = root ? &mr->root_surf : &nr->surf[side];
+    const uint32_t* w1 = root ? &mr->ntri : &nr->link[side];
+    const uint32_t* w2 = root ? &mr->nnodes : &nr->cnt;
     Code r;
-    if (code == ROOT_CODE) {
-        r.box = box_from12(mesh->root);
-        r.surf = mesh->root_surf;
-        r.leaf = mesh->nnodes == 0;
-        r.link = 0;
-        r.cnt = r.leaf ? mesh->ntri : 0;
-        return r;
-    }
-    const NodeRec& n = nodes[code >> 1];
-    const uint32_t side = code & 1;
-    r.box = box_from12(n.box[side]);
-    r.surf = n.surf[side];
-    r.link = n.link[side];
-    r.cnt = (n.cnt >> (16 * side)) & 0xFFFFu;
-    r.leaf = r.cnt != 0;
+    r.box = box_from12(bx);
+    r.surf = *sf;
+    const uint32_t a = *w1, b = *w2;
+    // a root is a leaf when its tree has no internal node; it then holds all
+    // ntri triangles from offset 0.  Both readings are computed and one is
+    // selected: a branch here would make the lanes wait for these loads
+    // before the next record's loads are issued.
+    const uint32_t cnt_root = a & (0u - uint32_t(b == 0u));
+    const uint32_t cnt_node = (b >> (16 * side)) & 0xFFFFu;
+    const uint32_t rm = 0u - uint32_t(root);
+    r.link = a & ~rm;
+    r.cnt = (cnt_root & rm) | (cnt_node & ~rm);
+    r.leaf = r.cnt != 0u;
     return r;
 }
 
 enum : int { PAIR_NONE = 0, PAIR_LEAF = 1, PAIR_SPLIT = 2 };
 
+constexpr uint32_t LP_CHUNK = 256;  // leaf-pair slots a persistent wavefront reserves at a time (>= 64)
+
+// Unused slots [next, end) of a wavefront's chunk become empty leaf pairs.
+__device__ __forceinline__ void close_chunk(uint4* __restrict__ lp_out, uint64_t lp_cap, uint64_t next, uint64_t end,
+                                            int lane) {
+    for (uint64_t i = next + uint64_t(lane); i < end; i += 64)
+        if (i < lp_cap) lp_out[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // One node pair: conservative test, then leaf pair or the two child pairs
 // (first child on top), with the reference's descent rule.
-__device__ __forceinline__ int eval_pair(const QueryPre& Q, const MeshRec* meshes, const NodeRec* nodes, uint32_t ca,
-                                         uint32_t cb, uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b,
-                                         uint4& leaf, uint32_t qi) {
-    const Code A = resolve(&meshes[Q.mesh_a], nodes + Q.node_base_a, ca);
-    const Code B0 = resolve(&meshes[Q.mesh_b], nodes + Q.node_base_b, cb);
+__device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
+                                         const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
+                                         uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf) {
+    const QueryPre& Q = qp[qi];
     M4 M;
 #pragma unroll
     for (int k = 0; k < 16; ++k) M.m[k] = Q.m[k];
+    const float sigma = Q.sigma;
+    const uint32_t base_a = Q.node_base_a, base_b = Q.node_base_b;
+    const Code A = resolve(meshes, nodes, ca);
+    const Code B0 = resolve(meshes, nodes, cb);
     const Box sb = xform_box(M, B0.box);
-    if (!sat_conservative(A.box, sb, Q.sigma)) return PAIR_NONE;
-    if (A.leaf && B0.leaf) {
-        leaf = make_uint4(qi, A.link, B0.link, A.cnt | (B0.cnt << 16));
-        return PAIR_LEAF;
-    }
+    const bool hit = sat_conservative(A.box, sb, sigma);
+    // Outputs are formed whatever the test says (selects, no branches), so
+    // the compiler keeps every record load ahead of the test instead of
+    // sinking some of them behind it into a second round trip.
+    const bool both_leaf = A.leaf && B0.leaf;
+    leaf = make_uint4(qi, A.link, B0.link, A.cnt | (B0.cnt << 16));
     const bool split_a = !A.leaf && (B0.leaf || A.surf >= surface(sb));
-    if (split_a) {
-        n0a = (A.link << 1) | 0u;
-        n1a = (A.link << 1) | 1u;
-        n0b = n1b = cb;
-    } else {
-        n0b = (B0.link << 1) | 0u;
-        n1b = (B0.link << 1) | 1u;
-        n0a = n1a = ca;
-    }
-    return PAIR_SPLIT;
+    const uint32_t ka = (base_a + A.link) << 1, kb = (base_b + B0.link) << 1;
+    n0a = split_a ? ka : ca;
+    n1a = split_a ? (ka | 1u) : ca;
+    n0b = split_a ? cb : kb;
+    n1b = split_a ? cb : (kb | 1u);
+    return hit ? (both_leaf ? PAIR_LEAF : PAIR_SPLIT) : PAIR_NONE;
 }
 
 }  // namespace
@@ -175,9 +204,10 @@ __global__ void k_setup(const QueryRec* __restrict__ qs, uint32_t nq, const Mesh
     out[i] = p;
 }
 
-__global__ void k_seed(uint32_t nq, uint3* __restrict__ frontier, unsigned int* __restrict__ count) {
+__global__ void k_seed(uint32_t nq, const QueryPre* __restrict__ qp, uint3* __restrict__ frontier,
+                       unsigned int* __restrict__ count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nq) frontier[i] = make_uint3(i, ROOT_CODE, ROOT_CODE);
+    if (i < nq) frontier[i] = make_uint3(i, ROOT_FLAG | qp[i].mesh_a, ROOT_FLAG | qp[i].mesh_b);
     if (i == 0) *count = nq;
 }
 
@@ -189,7 +219,7 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
                                                      uint64_t lp_cap, RunCounters* __restrict__ ctr) {
     const uint32_t n = min(*nin, out_cap);  // both frontier buffers hold out_cap entries
     const uint32_t stride = gridDim.x * blockDim.x;
-    unsigned long long nsat = 0;
+    unsigned long long nsat = 0, nleaf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         int kind = PAIR_NONE;
@@ -200,12 +230,13 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
             const uint3 e = in[i];
             qi = e.x;
             ++nsat;
-            kind = eval_pair(qp[qi], meshes, nodes, e.y, e.z, n0a, n0b, n1a, n1b, leaf, qi);
+            kind = eval_pair(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf);
         }
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
         if (mL) {
-            const unsigned long long s = reserve_b(mL, &ctr->leafpairs);
+            const unsigned long long s = reserve_b(mL, &ctr->lpslots);
             if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
+            nleaf += unsigned(__popcll(mL));
         }
         const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
         if (mS) {
@@ -226,7 +257,10 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
         }
     }
     nsat = wave_sum(nsat);
-    if ((threadIdx.x & 63) == 0 && nsat) atomicAdd(&ctr->sat, nsat);
+    if ((threadIdx.x & 63) == 0) {
+        if (nsat) atomicAdd(&ctr->sat, nsat);
+        if (nleaf) atomicAdd(&ctr->leafpairs, nleaf);
+    }
 }
 
 template <int CAP>
@@ -235,6 +269,8 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
                                             const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
                                             uint4* __restrict__ lp_out, uint64_t lp_cap, uint3* __restrict__ spill_base,
                                             uint32_t spill_cap, uint32_t pop_width, uint32_t task_cap,
+                                            unsigned int* __restrict__ exited, uint3* __restrict__ dump,
+                                            unsigned int* __restrict__ ndump, uint32_t dump_cap, uint32_t dump_after,
                                             RunCounters* __restrict__ ctr) {
     __shared__ uint32_t sq[CAP], sa[CAP], sb[CAP];
     const int lane = threadIdx.x;
@@ -243,9 +279,46 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
     uint32_t size = 0, nspill = 0;
     bool tasks_done = false, overflow = false;
-    unsigned long long nsat = 0;
+    unsigned int gone_seen = 0;  // idle wavefronts seen by lane 0 in the previous step
+    unsigned long long nsat = 0, nleaf = 0;
+    // Leaf-pair slots are taken LP_CHUNK at a time, so the shared counter
+    // sees one atomic per chunk rather than one per step.
+    uint64_t lp_next = 0, lp_end = 0;
     const unsigned long long below = lanes_below_b();
     for (;;) {
+        // Rebalancing: once dump_after wavefronts have run out of work, the
+        // ones still busy hand their whole stack (LDS and spill) to the dump
+        // list and leave; the next launch spreads that list over the grid.
+        // The idle count is read one step ahead so the load's latency hides
+        // behind the step's own loads.
+        if (dump_after != 0xFFFFFFFFu) {
+            const unsigned int gone = __shfl(gone_seen, 0, 64);
+            if (lane == 0) gone_seen = __hip_atomic_load(exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (gone >= dump_after && (size + nspill) > 0) {
+                const uint32_t n = size + nspill;
+                unsigned int base = 0xFFFFFFFFu;
+                if (lane == 0) {  // reserve without overshooting the capacity (each failed CAS means another wave won)
+                    unsigned int cur = __hip_atomic_load(ndump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    while (cur + n <= dump_cap) {
+                        const unsigned int prev = atomicCAS(ndump, cur, cur + n);
+                        if (prev == cur) {
+                            base = cur;
+                            break;
+                        }
+                        cur = prev;
+                    }
+                }
+                base = __shfl(base, 0, 64);
+                if (base != 0xFFFFFFFFu) {
+                    for (uint32_t i = lane; i < size; i += 64) dump[base + i] = make_uint3(sq[i], sa[i], sb[i]);
+                    for (uint32_t i = lane; i < nspill; i += 64) dump[base + size + i] = spill[i];
+                    size = 0;
+                    nspill = 0;
+                    __syncthreads();
+                }
+                // no room: keep working on this stack
+            }
+        }
         if (size < 64 && nspill > 0) {  // bring back the spilled bottom block underneath
             const uint32_t k = min(uint32_t(CAP / 2), nspill);
             uint32_t tq = 0, ta = 0, tb = 0;
@@ -294,12 +367,22 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         uint4 leaf = make_uint4(0, 0, 0, 0);
         if (have) {
             ++nsat;
-            kind = eval_pair(qp[qi], meshes, nodes, ca, cb, n0a, n0b, n1a, n1b, leaf, qi);
+            kind = eval_pair(qp, meshes, nodes, qi, ca, cb, n0a, n0b, n1a, n1b, leaf);
         }
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
         if (mL) {
-            const unsigned long long s = reserve_b(mL, &ctr->leafpairs);
+            const uint32_t nl = uint32_t(__popcll(mL));
+            if (lp_next + nl > lp_end) {
+                close_chunk(lp_out, lp_cap, lp_next, lp_end, lane);
+                unsigned long long c0 = 0;
+                if (lane == 0) c0 = atomicAdd(&ctr->lpslots, static_cast<unsigned long long>(LP_CHUNK));
+                lp_next = __shfl(c0, 0, 64);
+                lp_end = lp_next + LP_CHUNK;
+            }
+            const uint64_t s = lp_next + uint64_t(__popcll(mL & below));
             if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
+            lp_next += nl;
+            nleaf += nl;
         }
         const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
         const uint32_t ns = uint32_t(__popcll(mS));
@@ -331,9 +414,12 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         size += 2 * ns;
         __syncthreads();
     }
+    close_chunk(lp_out, lp_cap, lp_next, lp_end, lane);
     nsat = wave_sum(nsat);
     if (lane == 0) {
         atomicAdd(&ctr->sat, nsat);
+        if (nleaf) atomicAdd(&ctr->leafpairs, nleaf);
+        atomicAdd(exited, 1u);
         if (overflow) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 4u);
     }
 }
@@ -457,8 +543,8 @@ hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, 
     hipLaunchKernelGGL(k_setup, dim3((nq + 127) / 128), dim3(128), 0, st, qs, nq, meshes, out);
     return hipGetLastError();
 }
-hipError_t launch_seed(uint32_t nq, uint3* frontier, unsigned int* count, hipStream_t st) {
-    hipLaunchKernelGGL(k_seed, dim3((std::max(nq, 1u) + 255) / 256), dim3(256), 0, st, nq, frontier, count);
+hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigned int* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed, dim3((std::max(nq, 1u) + 255) / 256), dim3(256), 0, st, nq, qp, frontier, count);
     return hipGetLastError();
 }
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
@@ -471,19 +557,23 @@ hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* o
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
                       unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                       uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
+                      unsigned int* exited, uint3* dump, unsigned int* ndump, uint32_t dump_cap, uint32_t dump_after,
                       RunCounters* ctr, hipStream_t st) {
     switch (cap_entries) {
         case 512:
             hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
+                               dump_after, ctr);
             break;
         case 2048:
             hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
+                               dump_after, ctr);
             break;
         default:
             hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, ctr);
+                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
+                               dump_after, ctr);
             break;
     }
     return hipGetLastError();

@@ -556,8 +556,9 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
             total_internal += roots[m].icount;
             tri_base[m] = uint32_t(ctx->tris_used + tri_begin[m]);
         }
-        if (ctx->nodes_used + total_internal >= 0xFFFFFFF0ull || ctx->tris_used + T >= 0xFFFFFFF0ull) {
-            res.err = "arena exceeds 32-bit indexing";
+        // node indices travel as (node << 1) | side below a flag bit in the batched traversal
+        if (ctx->nodes_used + total_internal >= (1ull << 30) || ctx->tris_used + T >= 0xFFFFFFF0ull) {
+            res.err = "arena exceeds 32-bit indexing (at most 2^30 internal nodes)";
             goto fail;
         }
         // grow the arenas (keeping what is already registered)

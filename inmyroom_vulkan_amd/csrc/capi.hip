@@ -126,50 +126,75 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
 }
 
+constexpr uint32_t MAX_PASSES = 8;
 cd_tuning effective_tuning(const cd_ctx* ctx) {
     cd_tuning t = ctx->tuning;
     if (t.split_levels == 0) t.split_levels = 8;
-    if (t.split_levels > 16) t.split_levels = 16;
+    if (t.split_levels > 64) t.split_levels = 64;
     if (t.stack_entries != 512 && t.stack_entries != 2048) t.stack_entries = 1024;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
     if (t.waves_per_cu == 0) t.waves_per_cu = 8;
+    if (t.passes == 0) t.passes = 3;
+    if (t.passes > MAX_PASSES) t.passes = MAX_PASSES;
+    if (t.idle_pct == 0 || t.idle_pct > 100) t.idle_pct = 50;
     return t;
 }
 
 constexpr uint32_t SPILL_CAP = 16384;  // entries of each persistent wavefront's HBM spill area
+constexpr uint32_t DUMP_CAP = 1u << 20;  // node pairs handed from one traversal launch to the next
+// W_SMALL layout: [0..L] split-level counts; from PASS_BASE, four words per
+// traversal launch: task head, idle wavefronts, dump count, unused.
+constexpr uint32_t PASS_BASE = 72;
+constexpr uint32_t SMALL_WORDS = PASS_BASE + 4 * MAX_PASSES;
 
 // Work buffers (d_work slots): batched path, then broad phase / frame.
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
-       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT };
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DUMP1 };
 
 // Enqueue the whole batched pipeline on the stream (no host synchronisation
 // inside, so it can be captured in a graph).
+uint64_t frontier_cap(const cd_ctx* ctx, uint32_t nq, uint32_t L) {
+    // worst case nq * 2^L, capped; a level that overflows sets an error bit and the run is repeated larger
+    const uint64_t worst = (L >= 40) ? (uint64_t(1) << 40) : (uint64_t(std::max(nq, 1u)) << L);
+    return std::max<uint64_t>(64, std::min<uint64_t>(worst, std::max<uint64_t>(ctx->frontier_cap, 1u << 20)));
+}
+
 int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     hipStream_t st = ctx->stream;
     const uint32_t L = t.split_levels;
-    const uint64_t fcap = std::max<uint64_t>(uint64_t(nq) << L, 64);
+    const uint64_t fcap = frontier_cap(ctx, nq, L);
     const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
     QueryPre* qp = ctx->d_work[W_QPRE].as<QueryPre>();
     uint3* F[2] = {ctx->d_work[W_FRONT0].as<uint3>(), ctx->d_work[W_FRONT1].as<uint3>()};
-    unsigned int* small = ctx->d_work[W_SMALL].as<unsigned int>();  // [0..L] level counts, [L+1] task head
+    unsigned int* small = ctx->d_work[W_SMALL].as<unsigned int>();
+    uint3* D[2] = {ctx->d_work[W_DUMP0].as<uint3>(), ctx->d_work[W_DUMP1].as<uint3>()};
     const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
     const NodeRec* dn = ctx->d_nodes.as<NodeRec>();
     RunCounters* ctr = dev_counters(ctx);
     const uint32_t grid = uint32_t(ctx->num_cu) * 8;
     HCK(ctx, launch_zero(ctr, st));
-    HCK(ctx, launch_fill_u32(small, L + 2, 0u, st));
+    HCK(ctx, launch_fill_u32(small, SMALL_WORDS, 0u, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dm, qp, st));
-    HCK(ctx, launch_seed(nq, F[0], &small[0], st));
+    HCK(ctx, launch_seed(nq, qp, F[0], &small[0], st));
     for (uint32_t l = 0; l < L; ++l)
         HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)),
                                     qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctr, grid, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
-    HCK(ctx, launch_dfs(int(t.stack_entries), F[L & 1], &small[L], uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)), &small[L + 1], qp, dm, dn,
-                        ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
-                        t.pop_width, nwaves, ctr, st));
+    const uint32_t dump_after = std::max(1u, uint32_t(uint64_t(nwaves) * t.idle_pct / 100));
+    for (uint32_t p = 0; p < t.passes; ++p) {
+        unsigned int* pw = &small[PASS_BASE + 4 * p];
+        const bool first = p == 0, last = p + 1 == t.passes;
+        const uint3* tasks = first ? F[L & 1] : D[(p - 1) & 1];
+        const unsigned int* ntasks = first ? &small[L] : &small[PASS_BASE + 4 * (p - 1) + 2];
+        const uint32_t task_cap = first ? uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)) : DUMP_CAP;
+        HCK(ctx, launch_dfs(int(t.stack_entries), tasks, ntasks, task_cap, &pw[0], qp, dm, dn,
+                            ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
+                            t.pop_width, nwaves, &pw[1], D[p & 1], &pw[2], DUMP_CAP,
+                            last ? 0xFFFFFFFFu : dump_after, ctr, st));
+    }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
-    HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->leafpairs, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
+    HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
                                  ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
     return CD_OK;
@@ -179,24 +204,29 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     hipStream_t st = ctx->stream;
     const cd_tuning t = effective_tuning(ctx);
     const uint32_t L = t.split_levels;
-    const uint64_t fcap = std::max<uint64_t>(uint64_t(nq) << L, 64);
     const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
     if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
     if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
     HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
-    HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
-    HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
-    HCK(ctx, ctx->d_work[W_SMALL].ensure(64 * sizeof(unsigned int)));
+    HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
+    if (t.passes > 1) {
+        HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DUMP_CAP) * sizeof(uint3)));
+        HCK(ctx, ctx->d_work[W_DUMP1].ensure(size_t(DUMP_CAP) * sizeof(uint3)));
+    }
     RunCounters h{};
-    for (int attempt = 0; attempt < 4; ++attempt) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        const uint64_t fcap = frontier_cap(ctx, nq, L);
+        HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
+        HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
-        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, ctx->lp_cap, ctx->contact_cap,
-                                     uint64_t(ctx->leaf_max)};
+        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.passes, t.idle_pct, ctx->lp_cap,
+                                     ctx->contact_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
-                                &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL]})
+                                &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
+                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_DUMP1]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;
@@ -228,10 +258,14 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         }
         int rc = read_counters(ctx, h);
         if (rc) return rc;
-        if (h.err) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
         bool retry = false;
-        if (h.leafpairs > ctx->lp_cap) {
-            ctx->lp_cap = h.leafpairs + h.leafpairs / 4;
+        if (h.err & 2u) {  // a split level ran out of frontier: repeat with twice the room
+            ctx->frontier_cap = 2 * fcap;
+            retry = true;
+        }
+        if (h.err & ~2ull) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
+        if (h.lpslots > ctx->lp_cap) {
+            ctx->lp_cap = h.lpslots + h.lpslots / 4;
             retry = true;
         }
         if (h.contacts > ctx->contact_cap) {
@@ -418,7 +452,7 @@ int cd_destroy(cd_ctx* ctx) {
     for (DevBuf* b : {&ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_perm, &ctx->d_queries, &ctx->d_leafpairs,
                       &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg,
                       &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
-                      &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits})
+                      &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits, &ctx->d_refit_scratch})
         b->release();
     for (auto& b : ctx->d_work) b.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -794,6 +828,7 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
         ctx->refit_key = key;
         ctx->refit_nitems = uint32_t(items.size());
     }
+    HCK(ctx, ctx->d_refit_scratch.ensure(size_t(ctx->refit_nitems) * 24 * sizeof(double)));
     const uint32_t nm = uint32_t(ctx->meshes.size());
     HCK(ctx, ctx->d_refit_bits.ensure(2 * size_t(nm) * sizeof(unsigned int)));
     unsigned int* bits = ctx->d_refit_bits.as<unsigned int>();
@@ -806,12 +841,12 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
     const uint2* items = ctx->d_refit_items.as<uint2>();
     const uint32_t* ids = reinterpret_cast<const uint32_t*>(ctx->d_refit_items.as<char>() + ctx->refit_nitems * sizeof(uint2));
     HCK(ctx, launch_refit(items, ctx->refit_nitems, ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
-                          ctx->d_tris.as<float>(), bits, bits + nm, ids, n, st));
+                          ctx->d_tris.as<float>(), ctx->d_refit_scratch.as<double>(), bits, bits + nm, ids, n, st));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
-    // keep the host copy of the mesh table in step (root boxes, slack, radius)
-    for (uint32_t i = 0; i < n; ++i)
-        HCK(ctx, hipMemcpyAsync(&ctx->meshes[mesh_ids[i]], ctx->d_meshes.as<MeshRec>() + mesh_ids[i], sizeof(MeshRec),
-                                hipMemcpyDeviceToHost, st));
+    // keep the host copy of the mesh table in step (root boxes, slack, radius): one copy of the id span
+    const uint32_t lo_id = *std::min_element(mesh_ids, mesh_ids + n), hi_id = *std::max_element(mesh_ids, mesh_ids + n);
+    HCK(ctx, hipMemcpyAsync(&ctx->meshes[lo_id], ctx->d_meshes.as<MeshRec>() + lo_id,
+                            size_t(hi_id - lo_id + 1) * sizeof(MeshRec), hipMemcpyDeviceToHost, st));
     HCK(ctx, hipStreamSynchronize(st));
     float ms = 0.f;
     HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
@@ -831,7 +866,8 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
     if (!ctx || !t) return CD_E_INVALID;
     if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
-    if (t->pop_width > 64 || t->split_levels > 16 || t->waves_per_cu > 64)
+    if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->passes > MAX_PASSES ||
+        t->idle_pct > 100)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;

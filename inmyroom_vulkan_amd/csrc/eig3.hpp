@@ -1,7 +1,7 @@
 // 3x3 symmetric eigen-solver in FP64 for per-node PCA on the GPU.
 //
 // Same semantics and operation order as eig3/eig3.cpp:21-265 of the
-// reference (JAMA / EISPACK tred2 + tql2, eps = 2^-52, no iteration cap,
+// reference (JAMA / EISPACK tred2 + tql2, eps = 2^-52, an iteration guard far past convergence,
 // eigenvalues ascending, eigenvectors in the columns of V), so that under
 // -ffp-contract=off the device result is bit-identical to the host one.
 #pragma once
@@ -102,6 +102,11 @@ __host__ __device__ inline void eig3_ql(double V[3][3], double d[3], double e[3]
             ++m;
         }
         if (m > l) {
+            // The reference iterates until convergence with no cap; QL with
+            // this shift converges in a handful of steps, and the guard only
+            // keeps a GPU thread from spinning on input that would hang the
+            // reference too.  Results are unchanged whenever it converges.
+            int guard = 0;
             do {
                 double g = d[l];
                 double p = (d[l + 1] - g) / (2.0 * e[l]);
@@ -138,7 +143,7 @@ __host__ __device__ inline void eig3_ql(double V[3][3], double d[3], double e[3]
                 p = -s * s2 * c3 * el1 * e[l] / dl1;
                 e[l] = s * p;
                 d[l] = c * p;
-            } while (fabs(e[l]) > eps * tst1);
+            } while (fabs(e[l]) > eps * tst1 && ++guard < 1000);
         }
         d[l] = d[l] + f;
         e[l] = 0.0;

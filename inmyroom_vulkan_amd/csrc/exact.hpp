@@ -110,40 +110,21 @@ __device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats
 // Guigue & Devillers 2003 decision structure ("Fast and robust
 // triangle-triangle overlap test using orientation predicates", JGT 8(1)),
 // every sign exact.  Returns 1 = intersect (closed triangles, not coplanar),
-// 0 = disjoint, 2 = coplanar.  dsign[0..2] / dsign[3..5] receive the exact
-// sides of U's vertices w.r.t. V's plane and of V's w.r.t. U's plane (used
-// to place the contact segment).
-__device__ __forceinline__ int gd_check(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, PredStats& st) {
-    if (orient_filtered(p2, p1, q1, q2, st) > 0) return 0;
-    if (orient_filtered(p2, r1, p1, r2, st) > 0) return 0;
-    return 1;
-}
-__device__ inline int gd_canon2(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, int dp2, int dq2, int dr2, PredStats& st) {
-    if (dp2 > 0) {
-        if (dq2 > 0) return gd_check(p1, r1, q1, r2, p2, q2, st);
-        if (dr2 > 0) return gd_check(p1, r1, q1, q2, r2, p2, st);
-        return gd_check(p1, q1, r1, p2, q2, r2, st);
-    }
-    if (dp2 < 0) {
-        if (dq2 < 0) return gd_check(p1, q1, r1, r2, p2, q2, st);
-        if (dr2 < 0) return gd_check(p1, q1, r1, q2, r2, p2, st);
-        return gd_check(p1, r1, q1, p2, q2, r2, st);
-    }
-    if (dq2 < 0) {
-        if (dr2 >= 0) return gd_check(p1, r1, q1, q2, r2, p2, st);
-        return gd_check(p1, q1, r1, p2, q2, r2, st);
-    }
-    if (dq2 > 0) {
-        if (dr2 > 0) return gd_check(p1, r1, q1, p2, q2, r2, st);
-        return gd_check(p1, q1, r1, q2, r2, p2, st);
-    }
-    if (dr2 > 0) return gd_check(p1, q1, r1, r2, p2, q2, st);
-    if (dr2 < 0) return gd_check(p1, r1, q1, r2, p2, q2, st);
-    return 2;
-}
+// 0 = disjoint, 2 = coplanar.
+//
+// The published test is a tree of cases that permute the two triangles into
+// a canonical form and then evaluate two final orientations.  Written as
+// that tree, every case inlines its own copy of the predicate and the
+// function no longer fits the instruction cache; here the same cases are
+// reduced to two small permutations (a cyclic rotation of T1 plus an
+// optional q/r flip, a q/r swap of T2 plus a cyclic rotation) and the final
+// two predicates are evaluated once.
+//
 // dsides packs the exact sides (+1 each, 2 bits): U's vertices vs V's plane
 // in bits 0-5, V's vertices vs U's plane in bits 6-11.
 __device__ __forceinline__ int side_of(uint32_t packed, int k) { return int((packed >> (2 * k)) & 3u) - 1; }
+__device__ __forceinline__ f3 pick3(f3 a, f3 b, f3 c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
+
 __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
     const int dp1 = orient_filtered(p2, q2, r2, p1, st);
     const int dq1 = orient_filtered(p2, q2, r2, q1, st);
@@ -156,27 +137,62 @@ __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, ui
     dsides |= uint32_t(dp2 + 1) | (uint32_t(dq2 + 1) << 2) | (uint32_t(dr2 + 1) << 4);
     if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
     if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
+    // stage 1: rotate T1 so that its first vertex is alone on its side of
+    // T2's plane (rot1 = index of that vertex); swap2 exchanges q2 and r2
+    int rot1, swap2;
     if (dp1 > 0) {
-        if (dq1 > 0) return gd_canon2(r1, p1, q1, p2, r2, q2, dp2, dr2, dq2, st);
-        if (dr1 > 0) return gd_canon2(q1, r1, p1, p2, r2, q2, dp2, dr2, dq2, st);
-        return gd_canon2(p1, q1, r1, p2, q2, r2, dp2, dq2, dr2, st);
+        if (dq1 > 0) { rot1 = 2; swap2 = 1; }
+        else if (dr1 > 0) { rot1 = 1; swap2 = 1; }
+        else { rot1 = 0; swap2 = 0; }
+    } else if (dp1 < 0) {
+        if (dq1 < 0) { rot1 = 2; swap2 = 0; }
+        else if (dr1 < 0) { rot1 = 1; swap2 = 0; }
+        else { rot1 = 0; swap2 = 1; }
+    } else {
+        if (dq1 < 0) {
+            if (dr1 >= 0) { rot1 = 1; swap2 = 1; } else { rot1 = 0; swap2 = 0; }
+        } else if (dq1 > 0) {
+            if (dr1 > 0) { rot1 = 0; swap2 = 1; } else { rot1 = 1; swap2 = 0; }
+        } else {
+            rot1 = 2;
+            swap2 = dr1 > 0 ? 0 : 1;  // dr1 == 0 is the coplanar case, excluded above
+        }
     }
-    if (dp1 < 0) {
-        if (dq1 < 0) return gd_canon2(r1, p1, q1, p2, q2, r2, dp2, dq2, dr2, st);
-        if (dr1 < 0) return gd_canon2(q1, r1, p1, p2, q2, r2, dp2, dq2, dr2, st);
-        return gd_canon2(p1, q1, r1, p2, r2, q2, dp2, dr2, dq2, st);
+    // T2 after the optional swap, with its signs
+    const f3 s_q2 = swap2 ? r2 : q2, s_r2 = swap2 ? q2 : r2;
+    const int e_p = dp2, e_q = swap2 ? dr2 : dq2, e_r = swap2 ? dq2 : dr2;
+    // stage 2: rotate T2 the same way (rot2), flip1 exchanges T1's q and r
+    int rot2, flip1;
+    if (e_p > 0) {
+        if (e_q > 0) { flip1 = 1; rot2 = 2; }
+        else if (e_r > 0) { flip1 = 1; rot2 = 1; }
+        else { flip1 = 0; rot2 = 0; }
+    } else if (e_p < 0) {
+        if (e_q < 0) { flip1 = 0; rot2 = 2; }
+        else if (e_r < 0) { flip1 = 0; rot2 = 1; }
+        else { flip1 = 1; rot2 = 0; }
+    } else {
+        if (e_q < 0) {
+            if (e_r >= 0) { flip1 = 1; rot2 = 1; } else { flip1 = 0; rot2 = 0; }
+        } else if (e_q > 0) {
+            if (e_r > 0) { flip1 = 1; rot2 = 0; } else { flip1 = 0; rot2 = 1; }
+        } else {
+            if (e_r == 0) return 2;
+            flip1 = e_r > 0 ? 0 : 1;
+            rot2 = 2;
+        }
     }
-    if (dq1 < 0) {
-        if (dr1 >= 0) return gd_canon2(q1, r1, p1, p2, r2, q2, dp2, dr2, dq2, st);
-        return gd_canon2(p1, q1, r1, p2, q2, r2, dp2, dq2, dr2, st);
-    }
-    if (dq1 > 0) {
-        if (dr1 > 0) return gd_canon2(p1, q1, r1, p2, r2, q2, dp2, dr2, dq2, st);
-        return gd_canon2(q1, r1, p1, p2, q2, r2, dp2, dq2, dr2, st);
-    }
-    if (dr1 > 0) return gd_canon2(r1, p1, q1, p2, q2, r2, dp2, dq2, dr2, st);
-    if (dr1 < 0) return gd_canon2(r1, p1, q1, p2, r2, q2, dp2, dr2, dq2, st);
-    return 2;
+    const f3 P1 = pick3(p1, q1, r1, rot1);
+    const f3 b1 = pick3(p1, q1, r1, rot1 == 2 ? 0 : rot1 + 1);
+    const f3 c1 = pick3(p1, q1, r1, rot1 == 0 ? 2 : rot1 - 1);
+    const f3 Q1 = flip1 ? c1 : b1, R1 = flip1 ? b1 : c1;
+    const f3 P2 = pick3(p2, s_q2, s_r2, rot2);
+    const f3 Q2 = pick3(p2, s_q2, s_r2, rot2 == 2 ? 0 : rot2 + 1);
+    const f3 R2 = pick3(p2, s_q2, s_r2, rot2 == 0 ? 2 : rot2 - 1);
+    // the two final orientations (the check of the canonical form)
+    if (orient_filtered(P2, P1, Q1, Q2, st) > 0) return 0;
+    if (orient_filtered(P2, R1, P1, R2, st) > 0) return 0;
+    return 1;
 }
 
 // Contact segment of an intersecting, non-coplanar pair, with Möller's

@@ -118,6 +118,9 @@ struct DevBuf {
 // memset, so a run can be captured into a HIP graph).
 struct RunCounters {
     unsigned long long sat, leafpairs, tritests, contacts, fp64, exact, ub, err;
+    // leaf-pair slots handed out by the batched traversal; slots beyond the
+    // leaf pairs it found hold empty entries (zero triangle counts)
+    unsigned long long lpslots;
 };
 
 // Build of a batch of triangle soups into the arenas (build.hip).
@@ -143,7 +146,7 @@ struct cd_ctx {
     cdk::DevBuf d_work[24];                 // scratch of the batched path and the broad phase
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
-    uint64_t lp_cap = 0, contact_cap = 0;
+    uint64_t lp_cap = 0, contact_cap = 0, frontier_cap = 0;
 
     cd_counters counters{};
     cd_timings timings{};
@@ -152,7 +155,8 @@ struct cd_ctx {
     // skinned meshes: bind pose, joints, weights, tree-order vertex indices
     struct SkinHost { uint32_t vbase, nvert, nbones, vidx_base; };
     std::vector<std::pair<uint32_t, SkinHost>> skins;  // (mesh id, data)
-    cdk::DevBuf d_bind, d_joints, d_weights, d_vidx, d_skinpos, d_bones, d_skinrec, d_refit_items, d_refit_bits;
+    cdk::DevBuf d_bind, d_joints, d_weights, d_vidx, d_skinpos, d_bones, d_skinrec, d_refit_items, d_refit_bits,
+        d_refit_scratch;
     size_t skin_verts = 0, skin_vidx = 0;
     std::vector<uint32_t> refit_key;       // mesh list the cached refit items are for
     uint32_t refit_nitems = 0;

@@ -20,13 +20,14 @@ hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max
 
 // batched.hip
 hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st);
-hipError_t launch_seed(uint32_t nq, uint3* frontier, unsigned int* count, hipStream_t st);
+hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigned int* count, hipStream_t st);
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, RunCounters* ctr, uint32_t grid, hipStream_t st);
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
                       unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                       uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
+                      unsigned int* exited, uint3* dump, unsigned int* ndump, uint32_t dump_cap, uint32_t dump_after,
                       RunCounters* ctr, hipStream_t st);
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
@@ -56,7 +57,7 @@ hipError_t launch_skin(const SkinRec* sk, uint32_t nsk, const uint32_t* vstart, 
 hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* tstart, const uint32_t* vidx,
                               const float* pos, float* tris, uint32_t total, hipStream_t st);
 hipError_t launch_refit(const uint2* items, uint32_t nitems, MeshRec* meshes, NodeRec* nodes, const float* tris,
-                        unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids, uint32_t nids,
-                        hipStream_t st);
+                        double* scratch, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
+                        uint32_t nids, hipStream_t st);
 
 }  // namespace cdk

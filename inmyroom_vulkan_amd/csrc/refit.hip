@@ -7,15 +7,14 @@
 //                skinning shader (shaders/dynamicMeshShader_glsl.comp:120-145):
 //                p' = sum_k w_k (S_k * (p, 1)), S_k = joint matrix * inverse bind
 //  k_gather_tris rebuilds the tree-order triangle arena from skinned vertices
-//  k_refit       every box of the tree (root and both children of every
+//  k_refit_*     every box of the tree (root and both children of every
 //                internal node) is recomputed over its own triangles with the
 //                PCA construction of OBB.cpp:33-166 (mean, covariance, eig3,
 //                axes = rows of V, extents + 2 FLT_EPSILON); the topology built
 //                at registration is kept.  The FP64 sums are parallel
-//                workgroup reductions, so the bits differ from a sequential
-//                build; boxes still contain their triangles (checked by the
-//                slack pass that follows).
-//  k_refit_slack containment slack and radius of the refit meshes
+//                wavefront reductions, so the bits differ from a sequential
+//                build; boxes still contain their triangles, and the
+//                containment slack and radius of each mesh are recomputed.
 #include <float.h>
 
 #include "eig3.hpp"
@@ -100,92 +99,127 @@ __device__ __forceinline__ void item_range(const MeshRec& m, const NodeRec* node
     }
 }
 
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Wavefront reductions (butterfly: min and max are the same on every lane;
+// sums are read from lane 0 only).
+__device__ __forceinline__ double wave_sum_d(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < NT / 64; ++i) t += red[i];
-    return t;
+    return v;
 }
-template <int NT, bool MAX>
-__device__ __forceinline__ double block_ext(double v, double* red) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const double u = __shfl_xor(v, o, 64);
-        v = MAX ? fmax(v, u) : fmin(v, u);
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Each box is refit in three launches, so the serial eigen-solve runs one
+// box per lane instead of one lane per workgroup:
+//   k_refit_moments  one wavefront per box: distinct-point check, mean and
+//                    covariance (two passes, FP64)
+//   k_refit_eig      one thread per box: eig3, axes = rows of V
+//   k_refit_box      one wavefront per box: extents along the axes, the
+//                    box and its surface, then the containment slack
+// Scratch per box: RS_WORDS doubles.
+constexpr int RS_MEAN = 0, RS_COV = 3, RS_AX = 9, RS_KIND = 18, RS_WORDS = 24;
+constexpr int REFIT_WAVES = 4;  // boxes per 256-thread workgroup
+
+__global__ __launch_bounds__(64 * REFIT_WAVES) void k_refit_moments(const uint2* __restrict__ items, uint32_t nitems,
+                                                                    const MeshRec* __restrict__ meshes,
+                                                                    const NodeRec* __restrict__ nodes,
+                                                                    const float* __restrict__ tris,
+                                                                    double* __restrict__ scratch) {
+    const uint32_t it = blockIdx.x * REFIT_WAVES + (threadIdx.x >> 6);
+    if (it >= nitems) return;  // whole wavefronts leave; no workgroup barrier follows
+    const int lane = threadIdx.x & 63;
+    const uint2 item = items[it];
+    const MeshRec& m = meshes[item.x];
+    uint32_t off, cnt;
+    item_range(m, nodes, item.y, off, cnt);
+    const float* P = tris + (size_t(m.tri_base) + off) * 9;
+    const uint32_t npts = 3 * cnt;
+    double* out = scratch + size_t(it) * RS_WORDS;
+    // distinct-point check (OBB.cpp:35-50): more than one distinct point -> PCA
+    bool differs = false;
+    for (uint32_t k = 1 + lane; k < npts; k += 64)
+        differs |= !(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2]);
+    const bool multi = __ballot(differs) != 0ull;
+    if (npts == 0 || !multi) {
+        if (lane == 0) out[RS_KIND] = npts == 0 ? 0.0 : 1.0;
+        return;
     }
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    double t = red[0];
-    for (int i = 1; i < NT / 64; ++i) t = MAX ? fmax(t, red[i]) : fmin(t, red[i]);
-    return t;
+    const double N = double(npts);
+    double sx = 0, sy = 0, sz = 0;
+    for (uint32_t k = lane; k < npts; k += 64) {
+        sx += double(P[3 * k]);
+        sy += double(P[3 * k + 1]);
+        sz += double(P[3 * k + 2]);
+    }
+    // every lane needs the mean: take lane 0's sums
+    const double mx = __shfl(wave_sum_d(sx), 0, 64) / N;
+    const double my = __shfl(wave_sum_d(sy), 0, 64) / N;
+    const double mz = __shfl(wave_sum_d(sz), 0, 64) / N;
+    double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
+    for (uint32_t k = lane; k < npts; k += 64) {
+        const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+        cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
+        cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
+    }
+    cxx = wave_sum_d(cxx); cyy = wave_sum_d(cyy); czz = wave_sum_d(czz);
+    cxy = wave_sum_d(cxy); cxz = wave_sum_d(cxz); cyz = wave_sum_d(cyz);
+    if (lane == 0) {
+        out[RS_MEAN] = mx; out[RS_MEAN + 1] = my; out[RS_MEAN + 2] = mz;
+        out[RS_COV] = cxx / N; out[RS_COV + 1] = cyy / N; out[RS_COV + 2] = czz / N;
+        out[RS_COV + 3] = cxy / N; out[RS_COV + 4] = cxz / N; out[RS_COV + 5] = cyz / N;
+        out[RS_KIND] = 2.0;
+    }
 }
 
-constexpr int REFIT_NT = 256;
-
-__global__ __launch_bounds__(REFIT_NT) void k_refit(const uint2* __restrict__ items, uint32_t nitems,
-                                                    MeshRec* __restrict__ meshes, NodeRec* __restrict__ nodes,
-                                                    const float* __restrict__ tris) {
-    __shared__ double red[REFIT_NT / 64];
-    __shared__ double s_ax[9];
-    const uint32_t it = blockIdx.x;
+__global__ __launch_bounds__(256) void k_refit_eig(uint32_t nitems, double* __restrict__ scratch) {
+    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= nitems) return;
+    double* s = scratch + size_t(it) * RS_WORDS;
+    if (s[RS_KIND] != 2.0) return;
+    const double xx = s[RS_COV], yy = s[RS_COV + 1], zz = s[RS_COV + 2];
+    const double xy = s[RS_COV + 3], xz = s[RS_COV + 4], yz = s[RS_COV + 5];
+    double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+    double V[3][3], d[3];
+    eig3(A, V, d);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s[RS_AX + r * 3 + c] = V[r][c];  // rows of V (OBB.cpp:86-87)
+}
+
+__global__ __launch_bounds__(64 * REFIT_WAVES) void k_refit_box(const uint2* __restrict__ items, uint32_t nitems,
+                                                                MeshRec* __restrict__ meshes,
+                                                                NodeRec* __restrict__ nodes,
+                                                                const float* __restrict__ tris,
+                                                                const double* __restrict__ scratch,
+                                                                unsigned int* __restrict__ slack_bits,
+                                                                unsigned int* __restrict__ radius_bits) {
+    const uint32_t it = blockIdx.x * REFIT_WAVES + (threadIdx.x >> 6);
+    if (it >= nitems) return;
+    const int lane = threadIdx.x & 63;
     const uint2 item = items[it];
     MeshRec& m = meshes[item.x];
     uint32_t off, cnt;
     item_range(m, nodes, item.y, off, cnt);
     const float* P = tris + (size_t(m.tri_base) + off) * 9;
     const uint32_t npts = 3 * cnt;
-    const int tid = threadIdx.x;
-    // distinct-point check (OBB.cpp:35-50)
-    int differs = 0;
-    for (uint32_t k = 1 + tid; k < npts; k += REFIT_NT)
-        if (!(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2])) differs = 1;
-    const bool multi = block_sum<REFIT_NT>(double(differs), red) > 0.0;
+    const double* s = scratch + size_t(it) * RS_WORDS;
     Box box;
-    if (npts == 0 || !multi) {
+    if (s[RS_KIND] != 2.0) {  // OBB::EmptyOBB (OBB.cpp:168-178), centred on the single point if there is one
         const float e = FLT_EPSILON;
         box.c = npts ? mk(P[0], P[1], P[2]) : mk(0.f, 0.f, 0.f);
         box.u = mk(e, 0.f, 0.f);
         box.v = mk(0.f, e, 0.f);
         box.w = mk(0.f, 0.f, e);
     } else {
-        const double N = double(npts);
-        double sx = 0, sy = 0, sz = 0;
-        for (uint32_t k = tid; k < npts; k += REFIT_NT) {
-            sx += double(P[3 * k]);
-            sy += double(P[3 * k + 1]);
-            sz += double(P[3 * k + 2]);
-        }
-        const double mx = block_sum<REFIT_NT>(sx, red) / N;
-        const double my = block_sum<REFIT_NT>(sy, red) / N;
-        const double mz = block_sum<REFIT_NT>(sz, red) / N;
-        double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
-        for (uint32_t k = tid; k < npts; k += REFIT_NT) {
-            const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
-            cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
-            cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
-        }
-        const double xx = block_sum<REFIT_NT>(cxx, red) / N, yy = block_sum<REFIT_NT>(cyy, red) / N;
-        const double zz = block_sum<REFIT_NT>(czz, red) / N, xy = block_sum<REFIT_NT>(cxy, red) / N;
-        const double xz = block_sum<REFIT_NT>(cxz, red) / N, yz = block_sum<REFIT_NT>(cyz, red) / N;
-        if (tid == 0) {
-            double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
-            double V[3][3], d[3];
-            eig3(A, V, d);
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) s_ax[r * 3 + c] = V[r][c];  // rows of V (OBB.cpp:86-87)
-        }
-        __syncthreads();
         double ax[9];
-        for (int t = 0; t < 9; ++t) ax[t] = s_ax[t];
+        for (int t = 0; t < 9; ++t) ax[t] = s[RS_AX + t];
+        // OBB::CreateAABBfromPoints (OBB.cpp:105-166)
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t k = tid; k < npts; k += REFIT_NT) {
+        for (uint32_t k = lane; k < npts; k += 64) {
             const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
             for (int q = 0; q < 3; ++q) {
                 const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
@@ -194,8 +228,8 @@ __global__ __launch_bounds__(REFIT_NT) void k_refit(const uint2* __restrict__ it
             }
         }
         for (int q = 0; q < 3; ++q) {
-            lo[q] = block_ext<REFIT_NT, false>(lo[q], red);
-            hi[q] = block_ext<REFIT_NT, true>(hi[q], red);
+            lo[q] = wave_min_d(lo[q]);
+            hi[q] = wave_max_d(hi[q]);
         }
         double cx = 0.0, cy = 0.0, cz = 0.0;
         float side[3][3];
@@ -215,43 +249,33 @@ __global__ __launch_bounds__(REFIT_NT) void k_refit(const uint2* __restrict__ it
         box.v = mk(side[1][0], side[1][1], side[1][2]);
         box.w = mk(side[2][0], side[2][1], side[2][2]);
     }
-    if (tid == 0) {
-        const float s = surface(box);
+    // every lane holds the same box (min and max reductions are exact)
+    if (lane == 0) {
+        const float sf = surface(box);
         if (item.y == ROOT_CODE) {
             box_to12(box, m.root);
-            m.root_surf = s;
+            m.root_surf = sf;
         } else {
             NodeRec& n = nodes[m.node_base + (item.y >> 1)];
             box_to12(box, n.box[item.y & 1]);
-            n.surf[item.y & 1] = s;
+            n.surf[item.y & 1] = sf;
         }
     }
-}
-
-__global__ __launch_bounds__(64) void k_refit_slack(const uint2* __restrict__ items, uint32_t nitems,
-                                                    const MeshRec* __restrict__ meshes,
-                                                    const NodeRec* __restrict__ nodes, const float* __restrict__ tris,
-                                                    unsigned int* __restrict__ slack_bits,
-                                                    unsigned int* __restrict__ radius_bits) {
-    const uint32_t it = blockIdx.x;
-    if (it >= nitems) return;
-    const uint2 item = items[it];
-    const MeshRec& m = meshes[item.x];
-    uint32_t off, cnt;
-    item_range(m, nodes, item.y, off, cnt);
-    const float* bx = item.y == ROOT_CODE ? m.root : nodes[m.node_base + (item.y >> 1)].box[item.y & 1];
-    const double c[3] = {bx[0], bx[1], bx[2]};
-    const double u[3] = {bx[3], bx[4], bx[5]}, v[3] = {bx[6], bx[7], bx[8]}, w[3] = {bx[9], bx[10], bx[11]};
+    // containment slack: how far any of the box's points lies outside it
+    // (the batched traversal widens its separating-axis test by this much)
+    const double c[3] = {box.c.x, box.c.y, box.c.z};
+    const double u[3] = {box.u.x, box.u.y, box.u.z}, v[3] = {box.v.x, box.v.y, box.v.z};
+    const double w[3] = {box.w.x, box.w.y, box.w.z};
     auto det3 = [](const double* a, const double* b, const double* cc) {
-        return a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) + a[2] * (b[0] * cc[1] - b[1] * cc[0]);
+        return a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) +
+               a[2] * (b[0] * cc[1] - b[1] * cc[0]);
     };
     const double D = det3(u, v, w);
     const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const double lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-    const float* P = tris + (size_t(m.tri_base) + off) * 9;
     double worst = 0.0, rad = 0.0;
-    for (uint32_t k = threadIdx.x; k < 3 * cnt; k += 64) {
+    for (uint32_t k = lane; k < npts; k += 64) {
         const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
         const double r[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
         double ex = INFINITY;
@@ -262,11 +286,9 @@ __global__ __launch_bounds__(64) void k_refit_slack(const uint2* __restrict__ it
         worst = fmax(worst, ex);
         rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        worst = fmax(worst, __shfl_xor(worst, o, 64));
-        rad = fmax(rad, __shfl_xor(rad, o, 64));
-    }
-    if (threadIdx.x == 0) {
+    worst = wave_max_d(worst);
+    rad = wave_max_d(rad);
+    if (lane == 0) {
         float wf = float(worst);
         if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
         float rf = float(rad);
@@ -312,14 +334,18 @@ hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* t
     return hipGetLastError();
 }
 hipError_t launch_refit(const uint2* items, uint32_t nitems, MeshRec* meshes, NodeRec* nodes, const float* tris,
-                        unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids, uint32_t nids,
-                        hipStream_t st) {
+                        double* scratch, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
+                        uint32_t nids, hipStream_t st) {
     if (!nitems) return hipSuccess;
-    hipLaunchKernelGGL(k_refit, dim3(nitems), dim3(REFIT_NT), 0, st, items, nitems, meshes, nodes, tris);
+    const dim3 gw((nitems + REFIT_WAVES - 1) / REFIT_WAVES);
+    hipLaunchKernelGGL(k_refit_moments, gw, dim3(64 * REFIT_WAVES), 0, st, items, nitems, meshes, nodes, tris, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_refit_slack, dim3(nitems), dim3(64), 0, st, items, nitems, meshes, nodes, tris, slack_bits,
-                       radius_bits);
+    hipLaunchKernelGGL(k_refit_eig, g256r(nitems), dim3(256), 0, st, nitems, scratch);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_refit_box, gw, dim3(64 * REFIT_WAVES), 0, st, items, nitems, meshes, nodes, tris, scratch,
+                       slack_bits, radius_bits);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_store_slack, g256r(nids), dim3(256), 0, st, meshes, ids, nids, slack_bits, radius_bits);

@@ -76,6 +76,10 @@ def test_batched_queries_uneven_subset(gpu, orc):
     dict(split_levels=4, stack_entries=2048, pop_width=16, waves_per_cu=4),
     dict(split_levels=12, stack_entries=1024, pop_width=64, waves_per_cu=8),
     dict(split_levels=6, stack_entries=512, pop_width=1, waves_per_cu=1),
+    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, passes=1),
+    # busy wavefronts hand their stacks on as soon as 1% of the grid is idle, every launch but the last
+    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, passes=8, idle_pct=1),
+    dict(split_levels=0, stack_entries=1024, pop_width=32, waves_per_cu=2, passes=5, idle_pct=90),
 ])
 def test_batched_tuning_variants_agree(gpu, tuning):
     soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
@@ -94,6 +98,7 @@ def test_batched_tuning_variants_agree(gpu, tuning):
         gpu.set_tuning()
     assert got == base
     assert cnt["leaf_pairs"] == base_cnt["leaf_pairs"] and cnt["tri_tests"] == base_cnt["tri_tests"]
+    assert cnt["sat_tests"] == base_cnt["sat_tests"]  # every node pair tested exactly once, whichever launch takes it
 
 
 def test_batched_graph_replays_match_eager(gpu):

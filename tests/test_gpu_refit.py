@@ -52,9 +52,12 @@ def test_refit_identity_matches_build(gpu):
 
 
 def _contains(box, pts, tol):
+    """Every point within distance tol of the box (float boxes at |x| ~ 20 round their centre by ~1e-6,
+    the same order as the 2 FLT_EPSILON padding, so containment is a distance, as the slack is)."""
     c, S = box[:3].astype(np.float64), box[3:].reshape(3, 3).astype(np.float64)
     coef = np.linalg.solve(S.T, (pts.astype(np.float64) - c).T).T
-    return np.all(np.abs(coef) <= 1 + tol)
+    excess = (np.maximum(0.0, np.abs(coef) - 1.0) * np.linalg.norm(S, axis=1)).sum(axis=1)
+    return np.all(excess <= tol)
 
 
 def test_refit_boxes_contain_their_triangles(gpu):
