@@ -84,6 +84,27 @@ typedef struct cd_pair {
     uint32_t contact_begin, contact_count;
 } cd_pair;
 
+/* One uncollide ray (include/Geometry/Ray.h): origin and direction in the
+ * first entity's model space, for the rays of either entity.              */
+typedef struct cd_ray {
+    float origin[3];
+    float direction[3];
+} cd_ray;
+
+/* CDentriesUncollideRays (include/CollisionDetection/CreateUncollideRays.h:6-14)
+ * of one query or frame pair: its rays sit at rays[begin_first ..
+ * + n_first) (rays_from_first_to_second) and rays[begin_second .. +
+ * n_second) (rays_from_second_to_first).                                   */
+typedef struct cd_pair_rays {
+    uint32_t query;          /* query index, or pair index of the last cd_frame */
+    uint32_t first, second;  /* mesh ids (queries) or entity indices (cd_frame) */
+    uint32_t n_first, n_second;
+    uint32_t pad;
+    uint64_t begin_first, begin_second;
+    float avg_first[3];      /* average_point_first_modelspace */
+    float avg_second[3];     /* average_point_second_modelspace (the second's model space) */
+} cd_pair_rays;
+
 typedef struct cd_mesh_info {
     uint32_t ntri, nnodes, depth, leaf_max;
     float root[12];          /* root OBB: centre, u, v, w half-extent sides */
@@ -199,6 +220,27 @@ int cd_query_leafpairs(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t* ou
 int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs,
              uint32_t pair_cap, uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap,
              uint64_t* n_contacts);
+
+/* Rays of the last run's contacts (cd_frame, cd_query_batch or
+ * cd_run_staged): CreateUncollideRays::ExecuteCreateUncollideRays
+ * (src/CollisionDetection/CreateUncollideRays.cpp:60-200).  Per colliding
+ * triangle: weight |s - t|, weighted midpoint and the vertices not outside
+ * the other triangle's plane, merged over leaf pairs; then one ray per
+ * such vertex (de-duplicated by vertex id per pair and side, direction
+ * minus the vertex normal) or, when none is inside, one at the weighted
+ * contact point with the barycentric normal.  Only pairs with rays are
+ * listed (CollisionDetection.cpp:63).  Where several triangles claim one
+ * vertex the reference keeps the first its unordered_map walk meets; here
+ * the lowest tree-order triangle keeps it (this only matters for a mesh
+ * with face normals).  CD_E_CAPACITY writes the needed counts back.       */
+int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
+                      uint64_t ray_cap, uint64_t* n_rays);
+
+/* Vertex normals of a mesh for its rays (TriangleNormal::CreateTriangleNormalList,
+ * src/Geometry/Triangle.cpp:214-234): nvert x 3 floats indexed like the
+ * vertices given to cd_register_mesh (for a soup: 3 t + k).  A mesh without
+ * them uses face normals, as the reference does for a mesh without normals. */
+int cd_set_mesh_normals(cd_ctx* ctx, uint32_t mesh, const float* normals, uint32_t nvert);
 
 /* Broad phase alone (src/CollisionDetection/SweepAndPrune.cpp:15-88):
  * oriented overlapping pairs, sorted by (first, second).                  */

@@ -29,6 +29,7 @@ EXPORTS = (
     "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
     "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
+    "cd_uncollide_rays", "cd_set_mesh_normals",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -41,7 +42,13 @@ ENTITY_DTYPE = np.dtype([
     ("should_callback", "u1"), ("pad", "u1"),
 ])
 PAIR_DTYPE = np.dtype([("first", "<u4"), ("second", "<u4"), ("contact_begin", "<u4"), ("contact_count", "<u4")])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("direction", "<f4", 3)])
+PAIR_RAYS_DTYPE = np.dtype([
+    ("query", "<u4"), ("first", "<u4"), ("second", "<u4"), ("n_first", "<u4"), ("n_second", "<u4"), ("pad", "<u4"),
+    ("begin_first", "<u8"), ("begin_second", "<u8"), ("avg_first", "<f4", 3), ("avg_second", "<f4", 3),
+])
 assert CONTACT_DTYPE.itemsize == 64 and QUERY_DTYPE.itemsize == 72 and ENTITY_DTYPE.itemsize == 136
+assert RAY_DTYPE.itemsize == 24 and PAIR_RAYS_DTYPE.itemsize == 64
 
 
 class cd_config(C.Structure):
@@ -104,6 +111,8 @@ def load_library() -> C.CDLL:
     lib.cd_stage_queries.argtypes = [vp, vp, u32]
     lib.cd_run_staged.argtypes = [vp, u32, P(u64)]
     lib.cd_fetch_contacts.argtypes = [vp, vp, u64, P(u64)]
+    lib.cd_uncollide_rays.argtypes = [vp, vp, u32, P(u32), vp, u64, P(u64)]
+    lib.cd_set_mesh_normals.argtypes = [vp, u32, vp, u32]
     lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
     lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
     lib.cd_set_tuning.argtypes = [vp, P(cd_tuning)]
@@ -289,6 +298,26 @@ class Collider:
                 continue
             self._check(rc, "cd_frame")
             return pairs[: npairs.value], cons[: ncons.value]
+
+    def uncollide_rays(self):
+        """Rays of the last run's contacts: (pairs [PAIR_RAYS_DTYPE], rays [RAY_DTYPE])."""
+        pcap, rcap = 1 << 10, 1 << 14
+        while True:
+            pairs = np.zeros(pcap, PAIR_RAYS_DTYPE)
+            rays = np.zeros(rcap, RAY_DTYPE)
+            npairs, nrays = C.c_uint32(), C.c_uint64()
+            rc = self.lib.cd_uncollide_rays(self.h, _ptr(pairs), pcap, C.byref(npairs), _ptr(rays), rcap,
+                                            C.byref(nrays))
+            if rc == CD_E_CAPACITY:
+                pcap = max(pcap, int(npairs.value))
+                rcap = max(rcap, int(nrays.value))
+                continue
+            self._check(rc, "cd_uncollide_rays")
+            return pairs[: npairs.value], rays[: nrays.value]
+
+    def set_mesh_normals(self, mesh: int, normals: np.ndarray) -> None:
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self._check(self.lib.cd_set_mesh_normals(self.h, int(mesh), _ptr(n), len(n)), "cd_set_mesh_normals")
 
     def broad_phase(self, entities: np.ndarray) -> np.ndarray:
         ents = np.ascontiguousarray(entities, ENTITY_DTYPE)

@@ -212,5 +212,15 @@ hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* ou
     hipLaunchKernelGGL(k_gather_contacts, g256(n), dim3(256), 0, st, in, v0, n, out);
     return hipGetLastError();
 }
+hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo, uint32_t* k0,
+                            uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st) {
+    hipLaunchKernelGGL(k_iota, g256(n), dim3(256), 0, st, v0, n);
+    for (int field : {f_lo, f_mid, f_hi}) {
+        hipLaunchKernelGGL(k_contact_keys, g256(n), dim3(256), 0, st, in, n, field, v0, k0);
+        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, 32, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
 
 }  // namespace cdk

@@ -378,7 +378,7 @@ __global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t*
 }
 
 __global__ void k_emit_tris(const BNode* nodes, uint32_t count, const uint32_t* tri_base, const float* pos_fin,
-                            const uint32_t* id_fin, float* tris, uint32_t* perm) {
+                            const uint32_t* id_fin, float* tris, uint32_t* perm, uint32_t* leafof) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const BNode& nd = nodes[i];
@@ -389,6 +389,7 @@ __global__ void k_emit_tris(const BNode* nodes, uint32_t count, const uint32_t* 
         float* o = tris + (dst0 + t) * 9;
         for (int q = 0; q < 9; ++q) o[q] = s[q];
         perm[dst0 + t] = id_fin[nd.beg + t];
+        leafof[dst0 + t] = nd.toff;  // the leaf's first slot names the leaf
     }
 }
 
@@ -582,6 +583,7 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
         CK(grow(ctx->d_nodes, (ctx->nodes_used + total_internal + 1) * sizeof(NodeRec)));
         CK(grow(ctx->d_tris, (ctx->tris_used + T + 1) * 9 * sizeof(float)));
         CK(grow(ctx->d_perm, (ctx->tris_used + T + 1) * sizeof(uint32_t)));
+        CK(grow(ctx->d_leafof, (ctx->tris_used + T + 1) * sizeof(uint32_t)));
         CK(hipMemcpyAsync(base_nodes.p, node_base.data(), nmesh * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         CK(hipMemcpyAsync(base_tris.p, tri_base.data(), nmesh * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_emit_nodes, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total,
@@ -589,7 +591,7 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
         CK(hipGetLastError());
         hipLaunchKernelGGL(k_emit_tris, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total,
                            base_tris.as<uint32_t>(), pos_fin.as<float>(), id_fin.as<uint32_t>(), ctx->d_tris.as<float>(),
-                           ctx->d_perm.as<uint32_t>());
+                           ctx->d_perm.as<uint32_t>(), ctx->d_leafof.as<uint32_t>());
         CK(hipGetLastError());
         hipLaunchKernelGGL(k_slack, dim3(total), dim3(64), 0, st, nodes.as<BNode>(), total, pos_fin.as<float>(),
                            slack_bits.as<unsigned int>(), radius_bits.as<unsigned int>());

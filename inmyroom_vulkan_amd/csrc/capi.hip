@@ -39,6 +39,36 @@ int upload_mesh_table(cd_ctx* ctx) {
     return CD_OK;
 }
 
+hipError_t grow_keep(DevBuf& b, size_t need, hipStream_t st);
+
+int upload_attrs(cd_ctx* ctx) {
+    const size_t bytes = ctx->attrs.size() * sizeof(uint2);
+    HCK(ctx, ctx->d_attrs.ensure(bytes + sizeof(uint2)));
+    if (bytes) HCK(ctx, hipMemcpyAsync(ctx->d_attrs.p, ctx->attrs.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    HCK(ctx, hipStreamSynchronize(ctx->stream));
+    return CD_OK;
+}
+
+// Index-buffer vertex ids of a registered mesh, in tree order (for the
+// vertex de-duplication of the rays, CreateUncollideRays.cpp:146-152).
+int store_vertex_ids(cd_ctx* ctx, uint32_t id, const uint32_t* indices, uint32_t ntri, uint32_t nvert) {
+    hipStream_t st = ctx->stream;
+    const MeshRec& m = ctx->meshes[id];
+    const size_t n = size_t(ntri) * 3;
+    if (ctx->vid_used + n >= 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "vertex-id arena exceeds 32-bit indexing");
+    HCK(ctx, grow_keep(ctx->d_vid, (ctx->vid_used + n + 1) * sizeof(uint32_t), st));
+    ScopedBuf tmp;
+    HCK(ctx, tmp.ensure((n + 1) * sizeof(uint32_t)));
+    if (n) HCK(ctx, hipMemcpyAsync(tmp.p, indices, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HCK(ctx, launch_tree_vidx(ctx->d_perm.as<uint32_t>() + m.tri_base, ntri, tmp.as<uint32_t>(), 0u,
+                              ctx->d_vid.as<uint32_t>() + ctx->vid_used, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    ctx->attrs[id].x = uint32_t(ctx->vid_used);
+    ctx->mesh_nvert[id] = nvert;
+    ctx->vid_used += n;
+    return upload_attrs(ctx);
+}
+
 int register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh, uint32_t* first_id) {
     uint64_t T = 0;
     for (uint32_t m = 0; m < nmesh; ++m) T += counts[m];
@@ -50,7 +80,12 @@ int register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint3
     if (!r.err.empty()) return fail(ctx, CD_E_HIP, "tree build: " + r.err);
     if (first_id) *first_id = uint32_t(ctx->meshes.size());
     ctx->meshes.insert(ctx->meshes.end(), r.meshes.begin(), r.meshes.end());
-    return upload_mesh_table(ctx);
+    for (uint32_t m = 0; m < nmesh; ++m) {  // a soup: vertex ids 3 t + k, face normals
+        ctx->attrs.push_back(make_uint2(~0u, ~0u));
+        ctx->mesh_nvert.push_back(3u * counts[m]);
+    }
+    int rc = upload_mesh_table(ctx);
+    return rc ? rc : upload_attrs(ctx);
 }
 
 int check_queries(cd_ctx* ctx, const QueryRec* q, uint32_t nq) {
@@ -149,7 +184,10 @@ constexpr uint32_t SMALL_WORDS = PASS_BASE + 4 * MAX_PASSES;
 
 // Work buffers (d_work slots): batched path, then broad phase / frame.
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
-       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DUMP1 };
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DUMP1,
+       W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
+       W_RAYS };
+static_assert(W_RAYS < 40, "work slots");
 
 // Enqueue the whole batched pipeline on the stream (no host synchronisation
 // inside, so it can be captured in a graph).
@@ -302,7 +340,15 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
 }
 
+int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts);
 int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
+    ctx->last_run = 0;
+    ctx->contacts_sorted = false;
+    int rc = run_queries_(ctx, nq, mode, n_contacts);
+    if (rc == CD_OK) ctx->last_run = 1;
+    return rc;
+}
+int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
     if (mode == CD_MODE_PLAIN) return run_plain(ctx, nq, n_contacts);
     if (mode == CD_MODE_BATCHED) return run_batched(ctx, nq, n_contacts);
     return fail(ctx, CD_E_INVALID, "unknown mode");
@@ -317,6 +363,9 @@ int stage(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
     if (nq) HCK(ctx, hipMemcpyAsync(ctx->d_queries.p, qr, size_t(nq) * sizeof(QueryRec), hipMemcpyHostToDevice, ctx->stream));
     HCK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->staged_nq = nq;
+    ctx->staged_ids.resize(nq);
+    for (uint32_t i = 0; i < nq; ++i) ctx->staged_ids[i] = make_uint2(qr[i].mesh_a, qr[i].mesh_b);
+    ctx->last_run = 0;
     return CD_OK;
 }
 
@@ -452,7 +501,8 @@ int cd_destroy(cd_ctx* ctx) {
     for (DevBuf* b : {&ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_perm, &ctx->d_queries, &ctx->d_leafpairs,
                       &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg,
                       &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
-                      &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits, &ctx->d_refit_scratch})
+                      &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits, &ctx->d_refit_scratch,
+                      &ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm})
         b->release();
     for (auto& b : ctx->d_work) b.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
@@ -476,7 +526,32 @@ int cd_register_mesh(cd_ctx* ctx, const float* positions, uint32_t nvert, const 
             if (vi >= nvert) return fail(ctx, CD_E_INVALID, "index out of range");
             std::memcpy(&soup[size_t(t) * 9 + k * 3], positions + size_t(vi) * 3, 3 * sizeof(float));
         }
-    return register_soups(ctx, soup.data(), &ntri, 1, mesh_id);
+    uint32_t id = 0;
+    int rc = register_soups(ctx, soup.data(), &ntri, 1, &id);
+    if (rc) return rc;
+    rc = store_vertex_ids(ctx, id, indices, ntri, nvert);
+    if (rc) return rc;
+    if (mesh_id) *mesh_id = id;
+    return CD_OK;
+}
+
+int cd_set_mesh_normals(cd_ctx* ctx, uint32_t mesh, const float* normals, uint32_t nvert) {
+    if (!ctx) return CD_E_INVALID;
+    if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");
+    if (nvert != ctx->mesh_nvert[mesh]) return fail(ctx, CD_E_INVALID, "normal count differs from the mesh's vertex count");
+    if (nvert && !normals) return fail(ctx, CD_E_INVALID, "null normals");
+    if (!all_finite(normals, size_t(nvert) * 3)) return fail(ctx, CD_E_INVALID, "non-finite normal");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const size_t n = size_t(nvert) * 3;
+    if (ctx->nrm_used + n >= 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "normal arena exceeds 32-bit indexing");
+    HCK(ctx, grow_keep(ctx->d_nrm, (ctx->nrm_used + n + 1) * sizeof(float), st));
+    if (n) HCK(ctx, hipMemcpyAsync(ctx->d_nrm.as<float>() + ctx->nrm_used, normals, n * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    ctx->attrs[mesh].y = uint32_t(ctx->nrm_used);
+    ctx->nrm_used += n;
+    return upload_attrs(ctx);
 }
 
 int cd_register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* tri_counts, uint32_t nmesh, uint32_t* first_id) {
@@ -651,6 +726,10 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
         ctx->timings = cd_timings{};
         ctx->timings.broad_ms = broad_ms;
         ctx->timings.total_ms = broad_ms;
+        ctx->staged_nq = 0;
+        ctx->staged_contacts = 0;
+        ctx->staged_ids.clear();
+        ctx->last_run = 2;
         return CD_OK;
     }
     if (mode == CD_MODE_PLAIN)
@@ -679,6 +758,9 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
         HCK(ctx, hipMemcpyAsync(hc.data(), ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost, st));
         HCK(ctx, hipStreamSynchronize(st));
     }
+    ctx->contacts_sorted = true;
+    ctx->last_run = 2;
+    ctx->staged_ids = hp;
     // pairs that produced contacts (CollisionDetection.cpp:52, :63 keep only those)
     std::vector<cd_pair> out_pairs;
     for (uint64_t i = 0; i < nc;) {
@@ -694,6 +776,119 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
         return fail(ctx, CD_E_CAPACITY, "pair or contact buffer too small");
     if (!out_pairs.empty()) std::memcpy(pairs, out_pairs.data(), out_pairs.size() * sizeof(cd_pair));
     if (nc) std::memcpy(contacts, hc.data(), nc * sizeof(cd_contact));
+    return CD_OK;
+}
+
+// ---- rays (CreateUncollideRays.cpp:60-200) -----------------------------------
+int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
+                      uint64_t ray_cap, uint64_t* n_rays) {
+    if (!ctx || !n_pairs || !n_rays) return CD_E_INVALID;
+    *n_pairs = 0;
+    *n_rays = 0;
+    if (ctx->last_run == 0) return fail(ctx, CD_E_INVALID, "no run to take contacts from");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const uint32_t nq = ctx->staged_nq;
+    const uint64_t nc64 = ctx->staged_contacts;
+    if (nc64 == 0 || nq == 0) return CD_OK;
+    if (nc64 > 0x3FFFFFFFull) return fail(ctx, CD_E_CAPACITY, "too many contacts for ray generation");
+    const uint32_t nc = uint32_t(nc64);
+    for (int w : {W_CSORT}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(cd_contact)));
+    for (int w : {W_K0, W_V0, W_K1, W_V1, W_RSTART, W_RPOS, W_RCNT, W_RPOS2})
+        HCK(ctx, ctx->d_work[w].ensure((size_t(nc) + 1) * sizeof(uint32_t)));
+    for (int w : {W_RCAND, W_RDENSE}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(RayCand)));
+    HCK(ctx, ctx->d_work[W_RSUMS].ensure((size_t(nc) / 1024 + 2) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_RQ].ensure(size_t(nq) * 4 * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_RAVG].ensure(size_t(nq) * 6 * sizeof(float)));
+    HCK(ctx, ctx->d_work[W_RTOT].ensure(64 * sizeof(uint32_t)));
+    uint64_t H = 64;
+    while (H < 6ull * nc) H <<= 1;  // more slots than vertices offered: every probe ends
+    HCK(ctx, ctx->d_work[W_RHKEY].ensure(H * sizeof(unsigned long long)));
+    HCK(ctx, ctx->d_work[W_RHVAL].ensure(H * sizeof(unsigned long long)));
+    const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
+    if (!ctx->contacts_sorted) {
+        HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), nc, ctx->d_work[W_CSORT].as<cd_contact>(),
+                                      ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
+                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
+        ctx->contacts_sorted = true;
+    }
+    // second side's order: (query, tri_b, tri_a)
+    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
+                              ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
+    const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
+    const QueryRec* dq = ctx->d_queries.as<QueryRec>();
+    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    uint32_t* tot = ctx->d_work[W_RTOT].as<uint32_t>();   // [0] candidates, [1] rays, per side at +2 * side
+    uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();      // per side: count[nq], begin[nq]
+    float* avg = ctx->d_work[W_RAVG].as<float>();          // per side: nq x 3
+    HCK(ctx, launch_fill_u32(qw, 4 * nq, 0u, st));
+    HCK(ctx, launch_fill_u32(tot, 64, 0u, st));
+    uint32_t side_rays[2] = {0, 0};
+    for (int side = 0; side < 2; ++side) {
+        uint32_t* t2 = tot + 2 * side;
+        HCK(ctx, launch_cand(sorted, side ? orderB : nullptr, nc, side, dq, dm, ctx->d_leafof.as<uint32_t>(),
+                             ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(), st));
+        HCK(ctx, launch_scan(ctx->d_work[W_RSTART].as<uint32_t>(), nc, ctx->d_work[W_RPOS].as<uint32_t>(),
+                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[0], st));
+        HCK(ctx, launch_compact(ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(),
+                                ctx->d_work[W_RPOS].as<uint32_t>(), nc, ctx->d_work[W_RDENSE].as<RayCand>(), st));
+        auto* hkey = ctx->d_work[W_RHKEY].as<unsigned long long>();
+        auto* hval = ctx->d_work[W_RHVAL].as<unsigned long long>();
+        HCK(ctx, launch_fill_u64(hkey, H, ~0ull, st));
+        HCK(ctx, launch_fill_u64(hval, H, ~0ull, st));
+        HCK(ctx, launch_fill_u32(ctx->d_work[W_RCNT].as<uint32_t>(), nc + 1, 0u, st));
+        const RayCand* cand = ctx->d_work[W_RDENSE].as<RayCand>();
+        HCK(ctx, launch_vert_claim(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                                   ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, st));
+        uint32_t* qcount = qw + 2 * size_t(nq) * side;
+        uint32_t* qbegin = qcount + nq;
+        HCK(ctx, launch_ray_count(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                                  ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, ctx->d_work[W_RCNT].as<uint32_t>(),
+                                  qcount, st));
+        HCK(ctx, launch_scan(ctx->d_work[W_RCNT].as<uint32_t>(), nc, ctx->d_work[W_RPOS2].as<uint32_t>(),
+                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[1], st));
+        HCK(ctx, hipMemcpyAsync(&side_rays[side], &t2[1], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HCK(ctx, hipStreamSynchronize(st));
+        const uint64_t base = side ? side_rays[0] : 0;
+        HCK(ctx, grow_keep(ctx->d_work[W_RAYS], (base + side_rays[side] + 1) * sizeof(cd_ray), st));
+        HCK(ctx, launch_ray_emit(cand, &t2[0], nc, side, dq, dm, ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(),
+                                 ctx->d_vid.as<uint32_t>(), ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), hkey, hval,
+                                 H - 1, ctx->d_work[W_RPOS2].as<uint32_t>(), base, ctx->d_work[W_RAYS].as<cd_ray>(), qbegin,
+                                 st));
+        HCK(ctx, launch_ray_avg(ctx->d_work[W_RAYS].as<cd_ray>(), base, qbegin, qcount, nq, side, dq,
+                                avg + size_t(nq) * 3 * side, st));
+    }
+    std::vector<uint32_t> hq(size_t(nq) * 4);
+    std::vector<float> ha(size_t(nq) * 6);
+    HCK(ctx, hipMemcpyAsync(hq.data(), qw, hq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipMemcpyAsync(ha.data(), avg, ha.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    // pairs with rays (CollisionDetection.cpp:63); rays of the first sides, then of the second sides
+    std::vector<cd_pair_rays> out;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint32_t c0 = hq[q], c1 = hq[2 * size_t(nq) + q];
+        if (c0 == 0 && c1 == 0) continue;
+        cd_pair_rays r{};
+        r.query = q;
+        r.first = ctx->staged_ids[q].x;
+        r.second = ctx->staged_ids[q].y;
+        r.n_first = c0;
+        r.n_second = c1;
+        r.begin_first = hq[size_t(nq) + q];
+        r.begin_second = uint64_t(side_rays[0]) + hq[3 * size_t(nq) + q];
+        for (int k = 0; k < 3; ++k) {
+            r.avg_first[k] = ha[3 * size_t(q) + k];
+            r.avg_second[k] = ha[3 * (size_t(nq) + q) + k];
+        }
+        out.push_back(r);
+    }
+    const uint64_t total = uint64_t(side_rays[0]) + side_rays[1];
+    *n_pairs = uint32_t(out.size());
+    *n_rays = total;
+    if (out.size() > pair_cap || total > ray_cap || (total && (!pairs || !rays)))
+        return fail(ctx, CD_E_CAPACITY, "pair or ray buffer too small");
+    if (!out.empty()) std::memcpy(pairs, out.data(), out.size() * sizeof(cd_pair_rays));
+    if (total) HCK(ctx, hipMemcpy(rays, ctx->d_work[W_RAYS].p, total * sizeof(cd_ray), hipMemcpyDeviceToHost));
     return CD_OK;
 }
 

@@ -66,6 +66,15 @@ __host__ __device__ inline SapAxes sap_axes() {
     return a;
 }
 
+// One colliding triangle of one side of a query, aggregated over its
+// contacts (TriangleCandidateRays, CreateUncollideRays.cpp:13-58).
+struct RayCand {
+    uint32_t query, tri;   // tree-order triangle slot of this side's mesh
+    uint32_t mask, pad;    // vertex k not outside every plane it was tested against
+    float w, wp[3];        // sum of weights, sum of weighted midpoints
+};
+static_assert(sizeof(RayCand) == 32, "candidate layout");
+
 // One skinned mesh in a skin/refit launch.
 struct SkinRec {
     uint32_t mesh, vbase, nvert, nbones, bone_base, tri_base, ntri, vidx_base;
@@ -113,6 +122,13 @@ struct DevBuf {
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
+// A device buffer freed when it goes out of scope (temporaries).
+struct ScopedBuf : DevBuf {
+    ScopedBuf() = default;
+    ScopedBuf(const ScopedBuf&) = delete;
+    ScopedBuf& operator=(const ScopedBuf&) = delete;
+    ~ScopedBuf() { release(); }
+};
 
 // Device-side counters of one run (zeroed by a fill kernel, never by a
 // memset, so a run can be captured into a HIP graph).
@@ -141,9 +157,20 @@ struct cd_ctx {
     std::vector<cdk::MeshRec> meshes;      // host copy of the mesh table
     cdk::DevBuf d_meshes, d_nodes, d_tris, d_perm;
     size_t nodes_used = 0, tris_used = 0;
+    // ray-generation attributes: leaf of every tree-order triangle (its leaf's
+    // first slot), and per mesh {vertex-id base or ~0 (soup: 3 t + k),
+    // normal base or ~0 (face normals)} with the arenas they index
+    cdk::DevBuf d_leafof, d_attrs, d_vid, d_nrm;
+    std::vector<uint2> attrs;
+    std::vector<uint32_t> mesh_nvert;      // vertex ids per mesh (3 * ntri for a soup)
+    size_t vid_used = 0, nrm_used = 0;
+    // what the last run left on the device, for cd_uncollide_rays
+    int last_run = 0;                      // 0 none, 1 staged queries, 2 frame
+    bool contacts_sorted = false;          // sorted copy in d_work[W_CSORT]
+    std::vector<uint2> staged_ids;         // per query: (mesh_a, mesh_b), or per frame pair: (first, second)
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg;
-    cdk::DevBuf d_work[24];                 // scratch of the batched path and the broad phase
+    cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
     uint64_t lp_cap = 0, contact_cap = 0, frontier_cap = 0;

@@ -48,6 +48,33 @@ hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32
                                QueryRec* qs, uint2* sorted_pairs, hipStream_t st);
 hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, uint32_t* k0, uint32_t* v0,
                                 uint32_t* k1, uint32_t* v1, hipStream_t st);
+// Stable order of contacts by three fields (0 query, 1 tri_a, 2 tri_b), most
+// significant first; the permutation ends in v0.
+hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo, uint32_t* k0,
+                            uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st);
+
+// rays.hip
+hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
+                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st);
+hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st);
+hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
+                          hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
+hipError_t launch_vert_claim(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                             const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
+                             const uint32_t* perm, unsigned long long* hkey, unsigned long long* hval, uint64_t hmask,
+                             hipStream_t st);
+hipError_t launch_ray_count(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                            const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
+                            const uint32_t* perm, const unsigned long long* hkey, const unsigned long long* hval,
+                            uint64_t hmask, uint32_t* cnt, uint32_t* qcount, hipStream_t st);
+hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                           const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
+                           const uint32_t* vid, const float* nrm, const uint32_t* perm,
+                           const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
+                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st);
+hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
+                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st);
 
 // refit.hip
 hipError_t launch_tree_vidx(const uint32_t* perm, uint32_t ntri, const uint32_t* indices, uint32_t vbase,

@@ -1,0 +1,446 @@
+// Contact aggregation and uncollide-ray generation
+// (CreateUncollideRays::ExecuteCreateUncollideRays,
+//  inMyRoom_vulkan/src/CollisionDetection/CreateUncollideRays.cpp:60-200).
+//
+// Input: the contacts of a run sorted by (query, tri_a, tri_b), and a
+// permutation of them sorted by (query, tri_b, tri_a).  For each side:
+//  k_cand        one thread per run of contacts with the same (query,
+//                triangle): weight |s - t|, weighted midpoint and the
+//                vertex-not-outside mask, summed per leaf of the other tree
+//                and merged leaf after leaf (:74-127).  The reference merges
+//                in the order its DFS emits leaf pairs, and for a fixed leaf
+//                of one tree that order meets the other tree's leaves in
+//                tree order, which is the order of the sorted run: these
+//                sums are the reference's, bit for bit.  A leaf whose
+//                contacts all have zero weight is skipped, as IsNull() skips
+//                it (:119, :125).
+//  k_compact     dense candidate list (exclusive scan of the run starts)
+//  k_vert_claim  the not-outside vertices go into a hash set keyed by
+//                (query, side, vertex id); the smallest (triangle, corner)
+//                claims each vertex (:146-152 keep the first one the
+//                unordered_map walk meets, an order no GPU can reproduce)
+//  k_ray_count / k_ray_emit  one ray per claimed vertex (origin the vertex,
+//                direction minus its normal), or one at the weighted contact
+//                point with the barycentric normal when no vertex is
+//                inside (:155-167); the second entity's rays are in the
+//                first's space (:135, :146, :160)
+//  k_ray_avg     average origin per query and side (:179-197)
+#include "kernels.hpp"
+
+namespace cdk {
+
+namespace {
+
+// glm::adjointTranspose(mat3(M)) (Triangle.cpp:175-178): the cofactor matrix
+// of the upper-left 3x3, column-major like glm.
+struct M3 { float m[9]; };
+__device__ __forceinline__ M3 adjoint_transpose(const float* M) {
+    auto a = [&](int c, int r) { return M[c * 4 + r]; };
+    M3 R;
+    R.m[0] = +(a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2));
+    R.m[1] = -(a(1, 0) * a(2, 2) - a(2, 0) * a(1, 2));
+    R.m[2] = +(a(1, 0) * a(2, 1) - a(2, 0) * a(1, 1));
+    R.m[3] = -(a(0, 1) * a(2, 2) - a(2, 1) * a(0, 2));
+    R.m[4] = +(a(0, 0) * a(2, 2) - a(2, 0) * a(0, 2));
+    R.m[5] = -(a(0, 0) * a(2, 1) - a(2, 0) * a(0, 1));
+    R.m[6] = +(a(0, 1) * a(1, 2) - a(1, 1) * a(0, 2));
+    R.m[7] = -(a(0, 0) * a(1, 2) - a(1, 0) * a(0, 2));
+    R.m[8] = +(a(0, 0) * a(1, 1) - a(1, 0) * a(0, 1));
+    return R;
+}
+__device__ __forceinline__ f3 mul3(const M3& M, f3 v) {
+    return mk((M.m[0] * v.x + M.m[3] * v.y) + M.m[6] * v.z, (M.m[1] * v.x + M.m[4] * v.y) + M.m[7] * v.z,
+              (M.m[2] * v.x + M.m[5] * v.y) + M.m[8] * v.z);
+}
+__device__ __forceinline__ f3 scale(f3 v, float s) { return mk(v.x * s, v.y * s, v.z * s); }
+__device__ __forceinline__ f3 divs(f3 v, float s) { return mk(v.x / s, v.y / s, v.z / s); }
+
+__device__ __forceinline__ const cd_contact& at(const cd_contact* c, const uint32_t* order, uint32_t k) {
+    return c[order ? order[k] : k];
+}
+__device__ __forceinline__ uint32_t own_tri(const cd_contact& c, int side) { return side ? c.tri_b : c.tri_a; }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+constexpr uint64_t EMPTY = ~0ull;
+
+struct Corner {  // one corner of a candidate triangle, in the space its rays use
+    f3 p[3];     // triangle positions (the second's taken into the first's space)
+    uint32_t mesh, tri;
+};
+
+__device__ __forceinline__ uint32_t vertex_id(const MeshRec& m, const uint2& attr, const uint32_t* vid,
+                                              const uint32_t* perm, uint32_t tri, int k) {
+    // index-buffer value of the corner, or 3 t + k for a triangle soup
+    return attr.x == ~0u ? 3u * perm[m.tri_base + tri] + uint32_t(k) : vid[attr.x + 3u * tri + uint32_t(k)];
+}
+
+// Corner normal before the normal matrix: the mesh's vertex normal, or the
+// face normal of the model-space triangle (TriangleNormal, Triangle.cpp:140-146, 214-234).
+__device__ __forceinline__ f3 corner_normal(const uint2& attr, const float* nrm, uint32_t v, const float* tri9) {
+    if (attr.y != ~0u) {
+        const float* n = nrm + size_t(attr.y) + 3u * size_t(v);
+        return mk(n[0], n[1], n[2]);
+    }
+    const f3 p0 = mk(tri9[0], tri9[1], tri9[2]), p1 = mk(tri9[3], tri9[4], tri9[5]), p2 = mk(tri9[6], tri9[7], tri9[8]);
+    return normalize(cross(sub(p1, p0), sub(p2, p0)));
+}
+
+}  // namespace
+
+__global__ void k_cand(const cd_contact* __restrict__ c, const uint32_t* __restrict__ order, uint32_t n, int side,
+                       const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+                       const uint32_t* __restrict__ leafof, RayCand* __restrict__ out, uint32_t* __restrict__ start) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cd_contact& ci = at(c, order, i);
+    const uint32_t tri = own_tri(ci, side);
+    bool first = i == 0;
+    if (!first) {
+        const cd_contact& cp = at(c, order, i - 1);
+        first = cp.query != ci.query || own_tri(cp, side) != tri;
+    }
+    start[i] = 0u;
+    if (!first) return;
+    const QueryRec& q = qs[ci.query];
+    const uint32_t other_base = meshes[side ? q.mesh_a : q.mesh_b].tri_base;
+    const uint32_t shift = side ? 3u : 0u;
+    // merged candidate (the unordered_map entry) and the current leaf's local one
+    bool have = false;
+    f3 twp = mk(0.f, 0.f, 0.f);
+    float tw = 0.f;
+    uint32_t tmask = 7u;
+    f3 gwp = mk(0.f, 0.f, 0.f);
+    float gw = 0.f;
+    uint32_t gmask = 7u, leaf = 0u;
+    auto flush = [&]() {
+        if (gw == 0.f) return;  // IsNull (:19-22, :119)
+        if (!have) {
+            twp = gwp;
+            tw = gw;
+            tmask = gmask;
+            have = true;
+        } else {  // Merge (:52-57)
+            tmask &= gmask;
+            twp = add(twp, gwp);
+            tw += gw;
+        }
+    };
+    for (uint32_t j = i; j < n; ++j) {
+        const cd_contact& cj = at(c, order, j);
+        if (cj.query != ci.query || own_tri(cj, side) != tri) break;
+        const uint32_t other = side ? cj.tri_a : cj.tri_b;
+        const uint32_t lf = leafof[size_t(other_base) + other];
+        if (j > i && lf != leaf) {
+            flush();
+            gwp = mk(0.f, 0.f, 0.f);
+            gw = 0.f;
+            gmask = 7u;
+        }
+        leaf = lf;
+        const f3 s = mk(cj.s[0], cj.s[1], cj.s[2]), t = mk(cj.t[0], cj.t[1], cj.t[2]);
+        const float weight = vlength(sub(s, t));                 // :93
+        const f3 avg = divs(scale(add(s, t), weight), 2.f);      // :94
+        gwp = add(gwp, avg);
+        gw += weight;
+        gmask &= ~((cj.flags >> shift) & 7u);                   // :102-112
+    }
+    flush();
+    if (have) {
+        RayCand r;
+        r.query = ci.query;
+        r.tri = tri;
+        r.mask = tmask;
+        r.pad = 0u;
+        r.w = tw;
+        r.wp[0] = twp.x;
+        r.wp[1] = twp.y;
+        r.wp[2] = twp.z;
+        out[i] = r;
+        start[i] = 1u;
+    }
+}
+
+// Exclusive scan, three launches: per-block sums, one block over the sums
+// (in chunks), then the per-element offsets.  1024 elements per block.
+constexpr uint32_t SCAN_BLOCK = 1024;
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o, 64);
+        if (lane >= uint32_t(o)) incl += u;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
+        if (k < w) base += s_w[k];
+        all += s_w[k];
+    }
+    __syncthreads();
+    total = all;
+    return base + incl - v;
+}
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t n,
+                                                   uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = b * SCAN_BLOCK + tid * 4 + k;
+        if (i < n) v += in[i];
+    }
+    uint32_t total = 0;
+    (void)block_excl_scan(v, s_w, total);
+    if (tid == 0) sums[b] = total;
+}
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_w[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t t = 0;
+        const uint32_t ex = block_excl_scan(v, s_w, t);
+        if (i < nb) sums[i] = carry + ex;
+        carry += t;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                    const uint32_t* __restrict__ sums, uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_w[4];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    uint32_t v[4], s = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = b * SCAN_BLOCK + tid * 4 + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t total = 0;
+    uint32_t run = sums[b] + block_excl_scan(s, s_w, total);
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = b * SCAN_BLOCK + tid * 4 + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+}
+
+__global__ void k_compact(const RayCand* __restrict__ in, const uint32_t* __restrict__ start,
+                          const uint32_t* __restrict__ pos, uint32_t n, RayCand* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && start[i]) out[pos[i]] = in[i];
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Hash set of claimed vertices: key (query, side, vertex id), value the
+// smallest (triangle << 2 | corner) that wants the vertex.  Linear probing in
+// a power-of-two table with more slots than keys, so every probe ends.
+__device__ __forceinline__ uint64_t vkey(uint32_t query, int side, uint32_t v) {
+    return (uint64_t(query) << 33) | (uint64_t(side) << 32) | uint64_t(v);
+}
+__global__ void k_vert_claim(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
+                             const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+                             const uint2* __restrict__ attrs, const uint32_t* __restrict__ vid,
+                             const uint32_t* __restrict__ perm, unsigned long long* __restrict__ hkey,
+                             unsigned long long* __restrict__ hval, uint64_t hmask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *ncand_p) return;
+    const RayCand r = cand[i];
+    if (r.mask == 0u) return;
+    const uint32_t mesh = side ? qs[r.query].mesh_b : qs[r.query].mesh_a;
+    const MeshRec& m = meshes[mesh];
+    for (int k = 0; k < 3; ++k) {
+        if (!(r.mask & (1u << k))) continue;
+        const uint64_t key = vkey(r.query, side, vertex_id(m, attrs[mesh], vid, perm, r.tri, k));
+        uint64_t h = mix64(key) & hmask;
+        for (;;) {
+            const unsigned long long prev = atomicCAS(&hkey[h], static_cast<unsigned long long>(EMPTY),
+                                                      static_cast<unsigned long long>(key));
+            if (prev == EMPTY || prev == key) break;
+            h = (h + 1) & hmask;
+        }
+        atomicMin(&hval[h], (static_cast<unsigned long long>(r.tri) << 2) | static_cast<unsigned long long>(k));
+    }
+}
+__device__ __forceinline__ bool claimed(const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
+                                        uint64_t key, uint32_t tri, int k) {
+    uint64_t h = mix64(key) & hmask;
+    while (hkey[h] != key) h = (h + 1) & hmask;  // present: inserted by k_vert_claim
+    return hval[h] == ((static_cast<unsigned long long>(tri) << 2) | static_cast<unsigned long long>(k));
+}
+
+__global__ void k_ray_count(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
+                            const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+                            const uint2* __restrict__ attrs, const uint32_t* __restrict__ vid,
+                            const uint32_t* __restrict__ perm, const unsigned long long* __restrict__ hkey,
+                            const unsigned long long* __restrict__ hval, uint64_t hmask, uint32_t* __restrict__ cnt,
+                            uint32_t* __restrict__ qcount) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nc = *ncand_p;
+    if (i >= nc) return;
+    const RayCand r = cand[i];
+    uint32_t k_rays = 1u;  // fallback ray (:155-167)
+    if (r.mask != 0u) {
+        const uint32_t mesh = side ? qs[r.query].mesh_b : qs[r.query].mesh_a;
+        const MeshRec& m = meshes[mesh];
+        k_rays = 0u;
+        for (int k = 0; k < 3; ++k)
+            if ((r.mask & (1u << k)) &&
+                claimed(hkey, hval, hmask, vkey(r.query, side, vertex_id(m, attrs[mesh], vid, perm, r.tri, k)), r.tri, k))
+                ++k_rays;
+    }
+    cnt[i] = k_rays;
+    atomicAdd(&qcount[r.query], k_rays);
+}
+
+__global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
+                           const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+                           const float* __restrict__ tris, const uint2* __restrict__ attrs,
+                           const uint32_t* __restrict__ vid, const float* __restrict__ nrm,
+                           const uint32_t* __restrict__ perm, const unsigned long long* __restrict__ hkey,
+                           const unsigned long long* __restrict__ hval, uint64_t hmask,
+                           const uint32_t* __restrict__ pos, uint64_t ray_base, cd_ray* __restrict__ rays,
+                           uint32_t* __restrict__ qbegin) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nc = *ncand_p;
+    if (i >= nc) return;
+    const RayCand r = cand[i];
+    if (i == 0 || cand[i - 1].query != r.query) qbegin[r.query] = pos[i];  // candidates are in query order
+    const QueryRec& q = qs[r.query];
+    const uint32_t mesh = side ? q.mesh_b : q.mesh_a;
+    const MeshRec& m = meshes[mesh];
+    const uint2 attr = attrs[mesh];
+    M4 M;
+    for (int e = 0; e < 16; ++e) M.m[e] = q.rel[e];
+    const float* t9 = tris + (size_t(m.tri_base) + r.tri) * 9;
+    f3 P[3];
+    for (int k = 0; k < 3; ++k) {
+        const f3 p = mk(t9[3 * k], t9[3 * k + 1], t9[3 * k + 2]);
+        P[k] = side ? mul4(M, p, 1.f) : p;  // :135-136
+    }
+    cd_ray* out = rays + ray_base + pos[i];
+    uint32_t w = 0;
+    if (r.mask != 0u) {
+        const M3 NM = adjoint_transpose(q.rel);
+        for (int k = 0; k < 3; ++k) {
+            if (!(r.mask & (1u << k))) continue;
+            const uint32_t v = vertex_id(m, attr, vid, perm, r.tri, k);
+            if (!claimed(hkey, hval, hmask, vkey(r.query, side, v), r.tri, k)) continue;
+            const f3 n = corner_normal(attr, nrm, v, t9);
+            const f3 nn = side ? normalize(mul3(NM, n)) : normalize(n);  // Triangle.cpp:180-193
+            cd_ray ray;
+            ray.origin[0] = P[k].x; ray.origin[1] = P[k].y; ray.origin[2] = P[k].z;
+            ray.direction[0] = -nn.x; ray.direction[1] = -nn.y; ray.direction[2] = -nn.z;
+            out[w++] = ray;
+        }
+    } else {
+        // weighted average point, barycentric normal (Triangle.cpp:148-163, 195-212)
+        const f3 pt = divs(mk(r.wp[0], r.wp[1], r.wp[2]), r.w);
+        const f3 v0 = sub(P[1], P[0]), v1 = sub(P[2], P[0]), v2 = sub(pt, P[0]);
+        const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+        const float denom = d00 * d11 - d01 * d01;
+        const float bx = (d11 * d20 - d01 * d21) / denom;
+        const float by = (d00 * d21 - d01 * d20) / denom;
+        f3 n[3];
+        for (int k = 0; k < 3; ++k) n[k] = corner_normal(attr, nrm, vertex_id(m, attr, vid, perm, r.tri, k), t9);
+        const f3 in = add(add(scale(n[0], (1.f - bx) - by), scale(n[1], bx)), scale(n[2], by));
+        const f3 nn = side ? normalize(mul3(adjoint_transpose(q.rel), in)) : normalize(in);
+        cd_ray ray;
+        ray.origin[0] = pt.x; ray.origin[1] = pt.y; ray.origin[2] = pt.z;
+        ray.direction[0] = -nn.x; ray.direction[1] = -nn.y; ray.direction[2] = -nn.z;
+        out[0] = ray;
+    }
+}
+
+// Average origin of each query's rays on one side (:179-197); the second's is
+// taken back to its own model space by inverse(rel).
+__global__ void k_ray_avg(const cd_ray* __restrict__ rays, uint64_t ray_base, const uint32_t* __restrict__ qbegin,
+                          const uint32_t* __restrict__ qcount, uint32_t nq, int side, const QueryRec* __restrict__ qs,
+                          float* __restrict__ avg) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const uint32_t n = qcount[qi];
+    f3 acc = mk(0.f, 0.f, 0.f);
+    if (n) {
+        const cd_ray* r = rays + ray_base + qbegin[qi];
+        for (uint32_t k = 0; k < n; ++k) acc = add(acc, mk(r[k].origin[0], r[k].origin[1], r[k].origin[2]));
+    }
+    f3 a = divs(acc, float(n));
+    if (side) {
+        M4 M;
+        for (int e = 0; e < 16; ++e) M.m[e] = qs[qi].rel[e];
+        a = mul4(mat_inverse(M), a, 1.f);
+    }
+    avg[3 * qi] = a.x;
+    avg[3 * qi + 1] = a.y;
+    avg[3 * qi + 2] = a.z;
+}
+
+// ---- launchers ---------------------------------------------------------------
+static inline dim3 gr(uint64_t n) { return dim3(uint32_t((std::max<uint64_t>(n, 1) + 255) / 256)); }
+
+hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
+                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st) {
+    hipLaunchKernelGGL(k_cand, gr(n), dim3(256), 0, st, c, order, n, side, qs, meshes, leafof, out, start);
+    return hipGetLastError();
+}
+hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st) {
+    const uint32_t nb = std::max(1u, (n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, sums, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, in, n, sums, out);
+    return hipGetLastError();
+}
+hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_compact, gr(n), dim3(256), 0, st, in, start, pos, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u64, gr(n), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_vert_claim(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                             const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
+                             const uint32_t* perm, unsigned long long* hkey, unsigned long long* hval, uint64_t hmask,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_vert_claim, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, attrs, vid, perm,
+                       hkey, hval, hmask);
+    return hipGetLastError();
+}
+hipError_t launch_ray_count(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                            const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
+                            const uint32_t* perm, const unsigned long long* hkey, const unsigned long long* hval,
+                            uint64_t hmask, uint32_t* cnt, uint32_t* qcount, hipStream_t st) {
+    hipLaunchKernelGGL(k_ray_count, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, attrs, vid, perm,
+                       hkey, hval, hmask, cnt, qcount);
+    return hipGetLastError();
+}
+hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
+                           const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
+                           const uint32_t* vid, const float* nrm, const uint32_t* perm,
+                           const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
+                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st) {
+    hipLaunchKernelGGL(k_ray_emit, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, tris, attrs, vid,
+                       nrm, perm, hkey, hval, hmask, pos, ray_base, rays, qbegin);
+    return hipGetLastError();
+}
+hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
+                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st) {
+    hipLaunchKernelGGL(k_ray_avg, gr(nq), dim3(256), 0, st, rays, ray_base, qbegin, qcount, nq, side, qs, avg);
+    return hipGetLastError();
+}
+
+}  // namespace cdk

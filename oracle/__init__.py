@@ -94,6 +94,15 @@ class Oracle:
         lib.orc_result_leafpairs.argtypes = [vp, _U32, C.c_void_p]
         lib.orc_result_pairs.argtypes = [vp, _U32]
         lib.orc_result_per_query.argtypes = [vp, _U64]
+        lib.orc_tree_set_attrs.argtypes = [vp, C.c_void_p, C.c_void_p]
+        lib.orc_rays_queries.argtypes = [C.POINTER(vp), _U32, _U32, _F, C.c_int64]
+        lib.orc_rays_queries.restype = vp
+        lib.orc_rays_frame.argtypes = [C.POINTER(vp), _U32, _F, _U8, C.c_int64]
+        lib.orc_rays_frame.restype = vp
+        lib.orc_rays_count.argtypes = [vp, C.c_int]
+        lib.orc_rays_count.restype = C.c_int64
+        lib.orc_rays_get.argtypes = [vp, _U32, _F, _F]
+        lib.orc_rays_free.argtypes = [vp]
 
     # -- small functions -------------------------------------------------
     def eig3(self, A):
@@ -180,6 +189,44 @@ class Oracle:
         return OracleResult(self, r, nq=None)
 
 
+    def rays_queries(self, trees, qa, qb, mats):
+        """CDentriesUncollideRays of each query (CreateUncollideRays.cpp:60-200)."""
+        handles = (C.c_void_p * len(trees))(*[t.h for t in trees])
+        mats = np.ascontiguousarray(mats, np.float32).reshape(-1, 16)
+        r = self.lib.orc_rays_queries(handles, np.ascontiguousarray(qa, np.uint32),
+                                      np.ascontiguousarray(qb, np.uint32), mats.reshape(-1), len(mats))
+        return OracleRays(self, r)
+
+    def rays_frame(self, trees, mesh_of, cur16, should_cb):
+        """Pairs of a frame that produce rays, with their rays (CollisionDetection.cpp:38-67)."""
+        handles = (C.c_void_p * len(trees))(*[t.h for t in trees])
+        r = self.lib.orc_rays_frame(handles, np.ascontiguousarray(mesh_of, np.uint32),
+                                    np.ascontiguousarray(cur16, np.float32).reshape(-1),
+                                    np.ascontiguousarray(should_cb, np.uint8), len(mesh_of))
+        return OracleRays(self, r)
+
+
+class OracleRays:
+    """pairs: (first, second, n_from_first, n_from_second); avg: (n, 2, 3);
+    rays: per pair the first's rays then the second's, (origin, direction)."""
+
+    def __init__(self, orc: Oracle, h):
+        lib = orc.lib
+        n, nr = int(lib.orc_rays_count(h, 0)), int(lib.orc_rays_count(h, 1))
+        self.pairs = np.zeros((n, 4), np.uint32)
+        self.avg = np.zeros((n, 2, 3), np.float32)
+        self.rays = np.zeros((nr, 2, 3), np.float32)
+        if n:
+            lib.orc_rays_get(h, self.pairs.reshape(-1), self.avg.reshape(-1), self.rays.reshape(-1))
+        lib.orc_rays_free(h)
+
+    def split(self, i: int):
+        """Rays from the first and from the second entity of pair i."""
+        begin = int(self.pairs[:i, 2:].sum())
+        nf, ns = int(self.pairs[i, 2]), int(self.pairs[i, 3])
+        return self.rays[begin:begin + nf], self.rays[begin + nf:begin + nf + ns]
+
+
 class OracleTree:
     def __init__(self, orc: Oracle, tri9, leaf_max: int):
         self.orc = orc
@@ -203,6 +250,14 @@ class OracleTree:
     @property
     def slack(self) -> float:
         return float(self.orc.lib.orc_tree_slack(self.h))
+
+    def set_attrs(self, normals9=None, vid3=None):
+        """Corner normals (input order, 9 floats per triangle; None = face normals) and
+        index-buffer vertex ids (3 per triangle; None = 3 t + k) for ray generation."""
+        self._nrm = None if normals9 is None else np.ascontiguousarray(normals9, np.float32).reshape(-1, 9)
+        self._vid = None if vid3 is None else np.ascontiguousarray(vid3, np.uint32).reshape(-1, 3)
+        self.orc.lib.orc_tree_set_attrs(self.h, None if self._nrm is None else self._nrm.ctypes.data,
+                                        None if self._vid is None else self._vid.ctypes.data)
 
     def export(self):
         n = self.nodes

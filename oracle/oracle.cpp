@@ -12,6 +12,8 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <unordered_map>
+#include <unordered_set>
 #include <utility>
 
 namespace orc {
@@ -847,6 +849,182 @@ void narrow(const Tree& A, const Tree& B, const M4& M, const std::vector<LeafPai
 }
 
 // ---------------------------------------------------------------------------
+// Ray generation: CreateUncollideRays::ExecuteCreateUncollideRays
+// (inMyRoom_vulkan/src/CollisionDetection/CreateUncollideRays.cpp:60-200)
+// with Plane (src/Geometry/Plane.cpp:5-28) and the triangle helpers of
+// src/Geometry/Triangle.cpp:140-212.  The candidate maps and the emplaced
+// vertex set are std::unordered_map / std::unordered_set keyed by uint32_t,
+// filled in the reference's order, so their iteration order is the one a
+// libstdc++ build of the reference sees.
+// glm formulas used here beyond the header's list (parity unpinned: no glm
+// source is present): mat3(mat4) = upper-left 3x3; adjointTranspose = the
+// cofactor matrix, each entry a 2x2 minor a*b - c*d; mat3 * vec3 =
+// (m0*x + m1*y) + m2*z per row.
+// ---------------------------------------------------------------------------
+struct Cand {  // TriangleCandidateRays (CreateUncollideRays.cpp:13-58)
+    uint8_t notout = 7;  // std::bitset<3> point_is_not_outside = -1
+    V3 wp = {0.f, 0.f, 0.f};
+    float w = 0.f;
+    bool null() const { return w == 0.f; }
+    void merge(const Cand& r) {
+        notout &= r.notout;
+        wp = add(wp, r.wp);
+        w += r.w;
+    }
+};
+struct M3 { float m[9]; };  // column-major: m[c*3 + r]
+static M3 adjoint_transpose(const M4& M) {  // glm::adjointTranspose(mat3(M)) (Triangle.cpp:175-178)
+    auto a = [&](int c, int r) { return M.m[c * 4 + r]; };
+    M3 R;
+    R.m[0 * 3 + 0] = +(a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2));
+    R.m[0 * 3 + 1] = -(a(1, 0) * a(2, 2) - a(2, 0) * a(1, 2));
+    R.m[0 * 3 + 2] = +(a(1, 0) * a(2, 1) - a(2, 0) * a(1, 1));
+    R.m[1 * 3 + 0] = -(a(0, 1) * a(2, 2) - a(2, 1) * a(0, 2));
+    R.m[1 * 3 + 1] = +(a(0, 0) * a(2, 2) - a(2, 0) * a(0, 2));
+    R.m[1 * 3 + 2] = -(a(0, 0) * a(2, 1) - a(2, 0) * a(0, 1));
+    R.m[2 * 3 + 0] = +(a(0, 1) * a(1, 2) - a(1, 1) * a(0, 2));
+    R.m[2 * 3 + 1] = -(a(0, 0) * a(1, 2) - a(1, 0) * a(0, 2));
+    R.m[2 * 3 + 2] = +(a(0, 0) * a(1, 1) - a(1, 0) * a(0, 1));
+    return R;
+}
+static V3 mul3(const M3& M, V3 v) {
+    float o[3];
+    for (int r = 0; r < 3; ++r) o[r] = (M.m[0 + r] * v.x + M.m[3 + r] * v.y) + M.m[6 + r] * v.z;
+    return {o[0], o[1], o[2]};
+}
+static V3 scale(V3 v, float s) { return {v.x * s, v.y * s, v.z * s}; }
+static V3 divs(V3 v, float s) { return {v.x / s, v.y / s, v.z / s}; }
+
+// Face normal of a triangle (Triangle.cpp:140-146).
+static V3 face_normal(const V3 P[3]) { return normalize(cross(sub(P[1], P[0]), sub(P[2], P[0]))); }
+
+// Barycentric (u, v) of a point (Triangle.cpp:148-163).
+static void barycentric(const V3 P[3], V3 point, float& bx, float& by) {
+    V3 v0 = sub(P[1], P[0]), v1 = sub(P[2], P[0]), v2 = sub(point, P[0]);
+    float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    float denom = d00 * d11 - d01 * d01;
+    bx = (d11 * d20 - d01 * d21) / denom;
+    by = (d00 * d21 - d01 * d20) / denom;
+}
+
+static V3 corner_normal(const Tree& T, uint32_t t, int k) {
+    const float* n = &T.nrm[size_t(t) * 9 + size_t(k) * 3];
+    return {n[0], n[1], n[2]};
+}
+
+PairRays uncollide_rays(const Tree& A, const Tree& B, const M4& M, const std::vector<LeafPair>& lps) {
+    const M3 NM = adjoint_transpose(M);
+    std::unordered_map<uint32_t, Cand> first_map, second_map;
+    for (const LeafPair& lp : lps) {
+        std::vector<Cand> fc(lp.cntA), sc(lp.cntB);
+        for (uint32_t i = 0; i < lp.cntA; ++i)
+            for (uint32_t j = 0; j < lp.cntB; ++j) {
+                const uint32_t ta = lp.offA + i, tb = lp.offB + j;
+                V3 PA[3], PB[3];
+                for (int q = 0; q < 3; ++q) {
+                    const float* a = &A.pos[size_t(ta) * 9 + q * 3];
+                    const float* b = &B.pos[size_t(tb) * 9 + q * 3];
+                    PA[q] = {a[0], a[1], a[2]};
+                    PB[q] = mul4(M, V3{b[0], b[1], b[2]}, 1.f);
+                }
+                int cop = 0, ub = 0;
+                float s[3], t[3];
+                int hit = tri_tri_isect(&PA[0].x, &PA[1].x, &PA[2].x, &PB[0].x, &PB[1].x, &PB[2].x, &cop, &ub, s, t);
+                if (!(hit && !cop)) continue;  // :88
+                PlaneF pa = plane_of(PA), pb = plane_of(PB);
+                V3 S = {s[0], s[1], s[2]}, Tt = {t[0], t[1], t[2]};
+                float weight = length(sub(S, Tt));                  // :93
+                V3 avg = divs(scale(add(S, Tt), weight), 2.f);       // :94  weight * (s + t) / 2.f
+                fc[i].wp = add(fc[i].wp, avg);
+                fc[i].w += weight;
+                sc[j].wp = add(sc[j].wp, avg);
+                sc[j].w += weight;
+                for (int q = 0; q < 3; ++q) {  // :102-112
+                    if (outside(pb, PA[q])) fc[i].notout &= uint8_t(~(1u << q));
+                    if (outside(pa, PB[q])) sc[j].notout &= uint8_t(~(1u << q));
+                }
+            }
+        for (uint32_t i = 0; i < lp.cntA; ++i)  // :117-121
+            if (!fc[i].null()) {
+                auto r = first_map.emplace(lp.offA + i, fc[i]);
+                if (!r.second) r.first->second.merge(fc[i]);
+            }
+        for (uint32_t j = 0; j < lp.cntB; ++j)  // :123-127
+            if (!sc[j].null()) {
+                auto r = second_map.emplace(lp.offB + j, sc[j]);
+                if (!r.second) r.first->second.merge(sc[j]);
+            }
+    }
+    auto find_rays = [&](const std::unordered_map<uint32_t, Cand>& cands, const Tree& T, bool xf) {  // :131-175
+        std::vector<RayRec> rays;
+        std::unordered_set<uint32_t> emplaced;
+        for (const auto& kv : cands) {
+            const uint32_t t = kv.first;
+            const Cand& c = kv.second;
+            V3 P[3];
+            for (int q = 0; q < 3; ++q) {
+                const float* p = &T.pos[size_t(t) * 9 + q * 3];
+                P[q] = xf ? mul4(M, V3{p[0], p[1], p[2]}, 1.f) : V3{p[0], p[1], p[2]};
+            }
+            if (c.notout != 0) {
+                for (int q = 0; q < 3; ++q) {
+                    if (!(c.notout & (1u << q))) continue;
+                    if (!emplaced.insert(T.vid[size_t(t) * 3 + q]).second) continue;
+                    V3 n = corner_normal(T, t, q);
+                    V3 nn = xf ? normalize(mul3(NM, n)) : normalize(n);
+                    rays.push_back(RayRec{{P[q].x, P[q].y, P[q].z}, {-nn.x, -nn.y, -nn.z}});
+                }
+            } else {
+                V3 pos = divs(c.wp, c.w);
+                float bx, by;
+                barycentric(P, pos, bx, by);
+                V3 n0 = corner_normal(T, t, 0), n1 = corner_normal(T, t, 1), n2 = corner_normal(T, t, 2);
+                V3 in = add(add(scale(n0, (1.f - bx) - by), scale(n1, bx)), scale(n2, by));
+                V3 nn = xf ? normalize(mul3(NM, in)) : normalize(in);
+                rays.push_back(RayRec{{pos.x, pos.y, pos.z}, {-nn.x, -nn.y, -nn.z}});
+            }
+        }
+        return rays;
+    };
+    PairRays out;
+    auto average = [](const std::vector<RayRec>& rays, float* avg) {  // :183-188, std::accumulate then / float(size)
+        V3 acc = {0.f, 0.f, 0.f};
+        for (const RayRec& r : rays) acc = add(acc, V3{r.o[0], r.o[1], r.o[2]});
+        V3 a = divs(acc, float(rays.size()));
+        avg[0] = a.x; avg[1] = a.y; avg[2] = a.z;
+    };
+    out.from_first = find_rays(first_map, A, false);
+    average(out.from_first, out.avg_first);
+    out.from_second = find_rays(second_map, B, true);
+    average(out.from_second, out.avg_second);
+    V3 back = mul4(mat_inverse(M), V3{out.avg_second[0], out.avg_second[1], out.avg_second[2]}, 1.f);  // :196-197
+    out.avg_second[0] = back.x; out.avg_second[1] = back.y; out.avg_second[2] = back.z;
+    return out;
+}
+
+// Tree attributes for rays (Triangle.cpp:214-280): vertex normals through the
+// index buffer, or the face normal when the mesh has none; vertex ids are the
+// index-buffer values (a soup's are 3 t + k).
+void set_attrs(Tree& T, const float* nrm9, const uint32_t* vid3) {
+    const size_t n = T.perm.size();
+    T.nrm.assign(n * 9, 0.f);
+    T.vid.assign(n * 3, 0u);
+    for (size_t s = 0; s < n; ++s) {
+        const uint32_t t = T.perm[s];
+        V3 P[3];
+        for (int q = 0; q < 3; ++q) P[q] = {T.pos[s * 9 + q * 3], T.pos[s * 9 + q * 3 + 1], T.pos[s * 9 + q * 3 + 2]};
+        const V3 fn = face_normal(P);
+        for (int q = 0; q < 3; ++q) {
+            V3 nq = nrm9 ? V3{nrm9[size_t(t) * 9 + q * 3], nrm9[size_t(t) * 9 + q * 3 + 1], nrm9[size_t(t) * 9 + q * 3 + 2]} : fn;
+            T.nrm[s * 9 + q * 3] = nq.x;
+            T.nrm[s * 9 + q * 3 + 1] = nq.y;
+            T.nrm[s * 9 + q * 3 + 2] = nq.z;
+            T.vid[s * 3 + q] = vid3 ? vid3[size_t(t) * 3 + q] : uint32_t(3 * t + q);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Broad phase: sweep and prune on a fixed rotated frame
 // (inMyRoom_vulkan/src/CollisionDetection/CollisionDetection.cpp:9-13,
 //  inMyRoom_vulkan/src/CollisionDetection/SweepAndPrune.cpp:15-88)
@@ -1381,6 +1559,96 @@ void orc_tree_export(void* tp, float* root12, float* node_obbs, uint32_t* node_l
     }
     std::memcpy(perm, T.perm.data(), T.perm.size() * sizeof(uint32_t));
 }
+
+// ---- ray generation ----
+void orc_tree_set_attrs(void* tp, const float* nrm9, const uint32_t* vid3) {
+    set_attrs(*static_cast<Tree*>(tp), nrm9, vid3);
+}
+struct OrcRays { std::vector<PairRays> pairs; };
+static Tree& with_attrs(void* tp) {  // a tree without attributes gets face normals and soup vertex ids
+    Tree& T = *static_cast<Tree*>(tp);
+    if (T.nrm.size() != T.perm.size() * 9) set_attrs(T, nullptr, nullptr);
+    return T;
+}
+// Per query (mesh pair + relative matrix): CDentriesUncollideRays of every
+// query, rays or not.
+void* orc_rays_queries(void** trees, const uint32_t* qa, const uint32_t* qb, const float* m16, int64_t nq) {
+    auto* R = new OrcRays();
+    for (int64_t q = 0; q < nq; ++q) {
+        const Tree& A = with_attrs(trees[qa[q]]);
+        const Tree& B = with_attrs(trees[qb[q]]);
+        M4 M;
+        std::memcpy(M.m, m16 + q * 16, sizeof M.m);
+        Counters k;
+        std::vector<LeafPair> lps;
+        traverse(A, B, M, lps, k);
+        PairRays pr = uncollide_rays(A, B, M, lps);
+        pr.first = qa[q];
+        pr.second = qb[q];
+        R->pairs.push_back(std::move(pr));
+    }
+    return R;
+}
+// A frame (CollisionDetection.cpp:38-67): broad phase, mid phase, rays; pairs
+// without a ray are dropped (:63).
+void* orc_rays_frame(void** trees, const uint32_t* mesh_of, const float* cur16, const uint8_t* cb, int64_t n) {
+    auto* R = new OrcRays();
+    if (n < 2) return R;
+    std::vector<M4> cur(static_cast<size_t>(n));
+    std::vector<const Box*> roots(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+        std::memcpy(cur[size_t(i)].m, cur16 + i * 16, sizeof(float) * 16);
+        roots[size_t(i)] = &static_cast<Tree*>(trees[mesh_of[i]])->root;
+    }
+    for (const auto& pr : sap(cur.data(), roots.data(), cb, size_t(n))) {
+        const Tree& A = with_attrs(trees[mesh_of[pr.first]]);
+        const Tree& B = with_attrs(trees[mesh_of[pr.second]]);
+        M4 M = mat_mul(mat_inverse(cur[pr.first]), cur[pr.second]);
+        Counters k;
+        std::vector<LeafPair> lps;
+        traverse(A, B, M, lps, k);
+        if (lps.empty()) continue;  // :52
+        PairRays r = uncollide_rays(A, B, M, lps);
+        if (r.from_first.empty() && r.from_second.empty()) continue;  // :63
+        r.first = pr.first;
+        r.second = pr.second;
+        R->pairs.push_back(std::move(r));
+    }
+    return R;
+}
+// what: 0 pairs, 1 rays (both directions, all pairs)
+int64_t orc_rays_count(void* rp, int what) {
+    const OrcRays& R = *static_cast<OrcRays*>(rp);
+    if (what == 0) return int64_t(R.pairs.size());
+    int64_t n = 0;
+    for (const auto& p : R.pairs) n += int64_t(p.from_first.size() + p.from_second.size());
+    return n;
+}
+// pairs4: first, second, rays from first, rays from second; avg6: average
+// point of each side (first's and second's model space); rays6: per pair the
+// first's rays then the second's (origin, direction)
+void orc_rays_get(void* rp, uint32_t* pairs4, float* avg6, float* rays6) {
+    const OrcRays& R = *static_cast<OrcRays*>(rp);
+    size_t k = 0;
+    for (size_t i = 0; i < R.pairs.size(); ++i) {
+        const PairRays& p = R.pairs[i];
+        pairs4[i * 4 + 0] = p.first;
+        pairs4[i * 4 + 1] = p.second;
+        pairs4[i * 4 + 2] = uint32_t(p.from_first.size());
+        pairs4[i * 4 + 3] = uint32_t(p.from_second.size());
+        for (int c = 0; c < 3; ++c) {
+            avg6[i * 6 + c] = p.avg_first[c];
+            avg6[i * 6 + 3 + c] = p.avg_second[c];
+        }
+        for (const auto* v : {&p.from_first, &p.from_second})
+            for (const RayRec& r : *v) {
+                std::memcpy(rays6 + k * 6, r.o, sizeof r.o);
+                std::memcpy(rays6 + k * 6 + 3, r.d, sizeof r.d);
+                ++k;
+            }
+    }
+}
+void orc_rays_free(void* rp) { delete static_cast<OrcRays*>(rp); }
 
 // Batch of mesh-pair queries.  mode 0 = reference semantics (Möller), mode 1
 // = exact semantics.  keep_leafpairs stores the candidate list (mode 0).

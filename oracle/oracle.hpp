@@ -47,6 +47,11 @@ struct Tree {
     int depth = 0;                 // levels below the root (root leaf = 0)
     double slack = 0.0;            // max distance of a vertex outside a box that should contain it
     double radius = 0.0;           // max |vertex| in model space
+    // Ray-generation attributes (Triangle.cpp:214-280), tree order; empty
+    // until orc_tree_set_attrs: corner normals (9 floats per triangle, the
+    // mesh's vertex normals or the face normal) and index-buffer vertex ids.
+    std::vector<float> nrm;
+    std::vector<uint32_t> vid;
 };
 
 struct LeafPair { uint32_t offA, cntA, offB, cntB; };
@@ -62,5 +67,16 @@ struct Contact {
 };
 
 struct Counters { uint64_t sat = 0, leafpairs = 0, tritests = 0, contacts = 0, ub = 0; };
+
+// One uncollide ray (Ray.h): origin and direction, both in the first
+// entity's model space.
+struct RayRec { float o[3], d[3]; };
+
+// CDentriesUncollideRays (CreateUncollideRays.h:6-14) for one entity pair.
+struct PairRays {
+    uint32_t first = 0, second = 0;
+    std::vector<RayRec> from_first, from_second;
+    float avg_first[3] = {0, 0, 0}, avg_second[3] = {0, 0, 0};
+};
 
 }  // namespace orc
