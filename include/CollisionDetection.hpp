@@ -7,9 +7,10 @@
 // entry type of inMyRoom_vulkan/include/ECS/ECStypes.h:149-156 restated on
 // plain floats (the OBBtree pointer becomes a registered mesh id).
 // ExecuteCollisionDetection runs phases 1-3 of
-// src/CollisionDetection/CollisionDetection.cpp:38-67 on the GPU and keeps the
-// colliding entity pairs with their contacts; ray shooting and callbacks
-// (CollisionDetection.cpp:69-141) stay with the engine.
+// src/CollisionDetection/CollisionDetection.cpp:38-67 on the GPU, including
+// the uncollide rays of CreateUncollideRays.cpp:60-200, and keeps the entity
+// pairs that produced rays (:63) with their contacts and rays; ray shooting
+// and callbacks (CollisionDetection.cpp:69-141) stay with the engine.
 //
 // Error behaviour: the reference has no error returns; this mirror throws
 // std::runtime_error with the library's message when a call fails (the C ABI
@@ -39,6 +40,11 @@ struct CollidingPair {
     Entity first, second;          // oriented as SweepAndPrune.cpp:63 orients them
     uint32_t firstIndex, secondIndex;  // positions in the frame's entry list
     std::vector<cd_contact> contacts;  // in the first entity's model space
+    // CDentriesUncollideRays (include/CollisionDetection/CreateUncollideRays.h:6-14);
+    // every ray in the first entity's model space
+    std::vector<cd_ray> rays_from_first_to_second, rays_from_second_to_first;
+    float average_point_first_modelspace[3];
+    float average_point_second_modelspace[3];
 };
 
 class CollisionDetection {
@@ -60,6 +66,12 @@ public:
         uint32_t id = 0;
         check(cd_register_mesh(ctx_, positions, nvert, indices, ntri, &id), "cd_register_mesh");
         return id;
+    }
+
+    // Vertex normals for a mesh's rays (Triangle.cpp:214-234); without them the
+    // face normals are used, as for a glTF mesh without normals.
+    void SetMeshNormals(uint32_t mesh, const float* normals, uint32_t nvert) {
+        check(cd_set_mesh_normals(ctx_, mesh, normals, nvert), "cd_set_mesh_normals");
     }
 
     void Reset() { entries_.clear(); }  // CollisionDetection.cpp:28-31
@@ -89,13 +101,33 @@ public:
         check(rc, "cd_frame");
         pairs_.resize(np);
         contacts_.resize(nc);
-        for (const cd_pair& p : pairs_) {
+        // phase 3 (CreateUncollideRays), on the contacts the frame left on the device
+        uint32_t nrp = 0;
+        uint64_t nr = 0;
+        rc = cd_uncollide_rays(ctx_, nullptr, 0, &nrp, nullptr, 0, &nr);
+        if (rc == CD_E_CAPACITY) {
+            ray_pairs_.resize(nrp);
+            rays_.resize(nr);
+            rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), nrp, &nrp, rays_.data(), nr, &nr);
+        }
+        check(rc, "cd_uncollide_rays");
+        ray_pairs_.resize(nrp);
+        size_t k = 0;  // both lists follow the frame's pair order
+        for (const cd_pair_rays& rp : ray_pairs_) {
+            while (k < pairs_.size() && (pairs_[k].first != rp.first || pairs_[k].second != rp.second)) ++k;
             CollidingPair r;
-            r.firstIndex = p.first;
-            r.secondIndex = p.second;
-            r.first = entries_[p.first].entity;
-            r.second = entries_[p.second].entity;
-            r.contacts.assign(contacts_.begin() + p.contact_begin, contacts_.begin() + p.contact_begin + p.contact_count);
+            r.firstIndex = rp.first;
+            r.secondIndex = rp.second;
+            r.first = entries_[rp.first].entity;
+            r.second = entries_[rp.second].entity;
+            if (k < pairs_.size())
+                r.contacts.assign(contacts_.begin() + pairs_[k].contact_begin,
+                                  contacts_.begin() + pairs_[k].contact_begin + pairs_[k].contact_count);
+            r.rays_from_first_to_second.assign(rays_.begin() + rp.begin_first, rays_.begin() + rp.begin_first + rp.n_first);
+            r.rays_from_second_to_first.assign(rays_.begin() + rp.begin_second,
+                                               rays_.begin() + rp.begin_second + rp.n_second);
+            std::memcpy(r.average_point_first_modelspace, rp.avg_first, sizeof rp.avg_first);
+            std::memcpy(r.average_point_second_modelspace, rp.avg_second, sizeof rp.avg_second);
             results_.push_back(std::move(r));
         }
     }
@@ -113,6 +145,8 @@ private:
     std::vector<cd_entity> entries_;
     std::vector<cd_pair> pairs_;
     std::vector<cd_contact> contacts_;
+    std::vector<cd_pair_rays> ray_pairs_;
+    std::vector<cd_ray> rays_;
     std::vector<CollidingPair> results_;
 };
 

@@ -146,6 +146,7 @@ __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const 
     for (int k = 0; k < 16; ++k) M.m[k] = Q.m[k];
     const float sigma = Q.sigma;
     const uint32_t base_a = Q.node_base_a, base_b = Q.node_base_b;
+    const uint32_t tbase_a = Q.tri_base_a, tbase_b = Q.tri_base_b;
     const Code A = resolve(meshes, nodes, ca);
     const Code B0 = resolve(meshes, nodes, cb);
     const Box sb = xform_box(M, B0.box);
@@ -154,7 +155,9 @@ __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const 
     // the compiler keeps every record load ahead of the test instead of
     // sinking some of them behind it into a second round trip.
     const bool both_leaf = A.leaf && B0.leaf;
-    leaf = make_uint4(qi, A.link, B0.link, A.cnt | (B0.cnt << 16));
+    // leaf pair with arena-global triangle offsets: the narrow phase can fetch
+    // triangles without first reading the query's record
+    leaf = make_uint4(qi, tbase_a + A.link, tbase_b + B0.link, A.cnt | (B0.cnt << 16));
     const bool split_a = !A.leaf && (B0.leaf || A.surf >= surface(sb));
     const uint32_t ka = (base_a + A.link) << 1, kb = (base_b + B0.link) << 1;
     n0a = split_a ? ka : ca;
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
     uint32_t size = 0, nspill = 0;
     bool tasks_done = false, overflow = false;
-    unsigned int gone_seen = 0;  // idle wavefronts seen by lane 0 in the previous step
+    unsigned int gone_seen = 0, gone = 0, step = 0;  // idle wavefronts: lane 0's last read, and its broadcast
     unsigned long long nsat = 0, nleaf = 0;
     // Leaf-pair slots are taken LP_CHUNK at a time, so the shared counter
     // sees one atomic per chunk rather than one per step.
@@ -289,11 +292,15 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         // Rebalancing: once dump_after wavefronts have run out of work, the
         // ones still busy hand their whole stack (LDS and spill) to the dump
         // list and leave; the next launch spreads that list over the grid.
-        // The idle count is read one step ahead so the load's latency hides
-        // behind the step's own loads.
+        // The idle count is read every 8th step, one step ahead of its use,
+        // so the (device-coherent, hence slow) load hides behind the step's
+        // own loads and a thousand wavefronts do not queue on one address.
         if (dump_after != 0xFFFFFFFFu) {
-            const unsigned int gone = __shfl(gone_seen, 0, 64);
-            if (lane == 0) gone_seen = __hip_atomic_load(exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((step & 7u) == 0u) {
+                gone = __shfl(gone_seen, 0, 64);
+                if (lane == 0) gone_seen = __hip_atomic_load(exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            ++step;
             if (gone >= dump_after && (size + nspill) > 0) {
                 const uint32_t n = size + nspill;
                 unsigned int base = 0xFFFFFFFFu;
@@ -424,13 +431,32 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     }
 }
 
+constexpr uint32_t CSTAGE = 64;  // contacts a wavefront stages in LDS before one atomic reserves their slots
+
+// Copy a wavefront's staged contacts to the output (16 words each).
+__device__ __forceinline__ void flush_contacts(cd_contact* __restrict__ out, uint64_t cap, const cd_contact* staged,
+                                               uint32_t n, RunCounters* __restrict__ ctr, int lane) {
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&ctr->contacts, static_cast<unsigned long long>(n));
+    base = __shfl(base, 0, 64);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(staged);
+    for (uint32_t e = uint32_t(lane); e < 16u * n; e += 64u) {
+        const uint64_t slot = base + e / 16u;
+        if (slot < cap) reinterpret_cast<uint32_t*>(out + slot)[e % 16u] = src[e];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ lps,
                                                       const unsigned long long* __restrict__ nlp_p, uint64_t lp_cap,
                                                       const QueryPre* __restrict__ qp, const float* __restrict__ tris,
                                                       cd_contact* __restrict__ out, uint64_t cap,
                                                       RunCounters* __restrict__ ctr) {
+    __shared__ cd_contact s_con[4][CSTAGE];
     const uint64_t nlp = min(*nlp_p, lp_cap);
     const int lane = threadIdx.x & 63;
+    cd_contact* stage = s_con[threadIdx.x >> 6];
+    uint32_t staged = 0;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
     unsigned long long ntt = 0, nfp64 = 0, nexact = 0;
@@ -472,10 +498,10 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
                 M4 M;
 #pragma unroll
                 for (int e = 0; e < 16; ++e) M.m[e] = Q.m[e];
-                ta = ka + i;
-                tb = kb + j;
-                const float* a = tris + (size_t(Q.tri_base_a) + ta) * 9;
-                const float* b = tris + (size_t(Q.tri_base_b) + tb) * 9;
+                const float* a = tris + (size_t(ka) + i) * 9;  // ka, kb: arena-global triangle offsets
+                const float* b = tris + (size_t(kb) + j) * 9;
+                ta = ka + i - Q.tri_base_a;  // mesh-local slots for the contact
+                tb = kb + j - Q.tri_base_b;
                 A0 = mk(a[0], a[1], a[2]);
                 A1 = mk(a[3], a[4], a[5]);
                 A2 = mk(a[6], a[7], a[8]);
@@ -489,10 +515,21 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
             }
             const unsigned long long m = __ballot(hit);
             if (m) {
-                const unsigned long long slot = reserve_b(m, &ctr->contacts);
-                if (hit && slot < cap) write_contact(out + slot, A0, A1, A2, B0, B1, B2, ds, kq, ta, tb);
+                const uint32_t nh = uint32_t(__popcll(m));
+                if (staged + nh > CSTAGE) {
+                    __threadfence_block();  // this wavefront's LDS writes are complete before other lanes copy them
+                    flush_contacts(out, cap, stage, staged, ctr, lane);
+                    staged = 0;
+                    __threadfence_block();
+                }
+                if (hit) write_contact(stage + staged + uint32_t(__popcll(m & below)), A0, A1, A2, B0, B1, B2, ds, kq, ta, tb);
+                staged += nh;
             }
         }
+    }
+    if (staged) {
+        __threadfence_block();
+        flush_contacts(out, cap, stage, staged, ctr, lane);
     }
     ntt = wave_sum(ntt);
     nfp64 = wave_sum(nfp64);

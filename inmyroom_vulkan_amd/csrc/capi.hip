@@ -502,6 +502,7 @@ int cd_destroy(cd_ctx* ctx) {
                       &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg,
                       &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
                       &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits, &ctx->d_refit_scratch,
+                      &ctx->d_refit_ranges, &ctx->d_refit_plan, &ctx->d_refit_partial,
                       &ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm})
         b->release();
     for (auto& b : ctx->d_work) b.release();
@@ -1020,10 +1021,44 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
         HCK(ctx, hipMemcpyAsync(ctx->d_refit_items.p, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
         HCK(ctx, hipMemcpyAsync(ctx->d_refit_items.as<char>() + items.size() * sizeof(uint2), mesh_ids,
                                 n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        // plan: each box's triangle range, then small boxes and chunks of the large ones
+        const uint32_t ni = uint32_t(items.size());
+        HCK(ctx, ctx->d_refit_ranges.ensure((size_t(ni) + 1) * sizeof(uint2)));
+        HCK(ctx, launch_item_ranges(ctx->d_refit_items.as<uint2>(), ni, ctx->d_meshes.as<MeshRec>(),
+                                    ctx->d_nodes.as<NodeRec>(), ctx->d_refit_ranges.as<uint2>(), st));
+        std::vector<uint2> ranges(ni);
+        HCK(ctx, hipMemcpyAsync(ranges.data(), ctx->d_refit_ranges.p, ni * sizeof(uint2), hipMemcpyDeviceToHost, st));
+        HCK(ctx, hipStreamSynchronize(st));
+        std::vector<uint32_t> small;
+        std::vector<uint4> chunks, big;
+        const uint32_t smax = refit_small_max(), ct = refit_chunk_tris();
+        for (uint32_t it = 0; it < ni; ++it) {
+            if (ranges[it].y <= smax) {
+                small.push_back(it);
+                continue;
+            }
+            big.push_back(make_uint4(it, uint32_t(chunks.size()), (ranges[it].y + ct - 1) / ct, 0u));
+            for (uint32_t t = 0; t < ranges[it].y; t += ct)
+                chunks.push_back(make_uint4(it, ranges[it].x + t, std::min(ct, ranges[it].y - t), 0u));
+        }
+        const size_t bs = small.size() * sizeof(uint32_t), bc = chunks.size() * sizeof(uint4),
+                     bb = big.size() * sizeof(uint4);
+        HCK(ctx, ctx->d_refit_plan.ensure(bs + bc + bb + 64));
+        char* pb = ctx->d_refit_plan.as<char>();
+        const size_t oc = (bs + 15) / 16 * 16, ob = oc + bc;
+        if (bs) HCK(ctx, hipMemcpyAsync(pb, small.data(), bs, hipMemcpyHostToDevice, st));
+        if (bc) HCK(ctx, hipMemcpyAsync(pb + oc, chunks.data(), bc, hipMemcpyHostToDevice, st));
+        if (bb) HCK(ctx, hipMemcpyAsync(pb + ob, big.data(), bb, hipMemcpyHostToDevice, st));
+        HCK(ctx, hipStreamSynchronize(st));
+        ctx->refit_plan = RefitPlan{ctx->d_refit_items.as<uint2>(), ctx->d_refit_ranges.as<uint2>(),
+                                    reinterpret_cast<const uint32_t*>(pb), uint32_t(small.size()),
+                                    reinterpret_cast<const uint4*>(pb + oc), uint32_t(chunks.size()),
+                                    reinterpret_cast<const uint4*>(pb + ob), uint32_t(big.size())};
+        HCK(ctx, ctx->d_refit_partial.ensure((chunks.size() + 1) * 8 * sizeof(double)));
         ctx->refit_key = key;
-        ctx->refit_nitems = uint32_t(items.size());
+        ctx->refit_nitems = ni;
     }
-    HCK(ctx, ctx->d_refit_scratch.ensure(size_t(ctx->refit_nitems) * 24 * sizeof(double)));
+    HCK(ctx, ctx->d_refit_scratch.ensure((size_t(ctx->refit_nitems) + 1) * 24 * sizeof(double)));
     const uint32_t nm = uint32_t(ctx->meshes.size());
     HCK(ctx, ctx->d_refit_bits.ensure(2 * size_t(nm) * sizeof(unsigned int)));
     unsigned int* bits = ctx->d_refit_bits.as<unsigned int>();
@@ -1035,8 +1070,10 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
     HCK(ctx, launch_fill_u32(bits, 2 * nm, 0u, st));
     const uint2* items = ctx->d_refit_items.as<uint2>();
     const uint32_t* ids = reinterpret_cast<const uint32_t*>(ctx->d_refit_items.as<char>() + ctx->refit_nitems * sizeof(uint2));
-    HCK(ctx, launch_refit(items, ctx->refit_nitems, ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
-                          ctx->d_tris.as<float>(), ctx->d_refit_scratch.as<double>(), bits, bits + nm, ids, n, st));
+    (void)items;
+    HCK(ctx, launch_refit(ctx->refit_plan, ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
+                          ctx->d_tris.as<float>(), ctx->d_refit_scratch.as<double>(),
+                          ctx->d_refit_partial.as<double>(), bits, bits + nm, ids, n, st));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
     // keep the host copy of the mesh table in step (root boxes, slack, radius): one copy of the id span
     const uint32_t lo_id = *std::min_element(mesh_ids, mesh_ids + n), hi_id = *std::max_element(mesh_ids, mesh_ids + n);

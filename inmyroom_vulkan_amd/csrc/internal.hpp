@@ -75,6 +75,18 @@ struct RayCand {
 };
 static_assert(sizeof(RayCand) == 32, "candidate layout");
 
+// A refit plan: small boxes (one thread each) and large boxes split into
+// chunks of triangles (device arrays, cached per mesh list).
+struct RefitPlan {
+    const uint2* items;    // (mesh, traversal code) per box
+    const uint2* ranges;   // (first triangle, triangles) per box, tree order, mesh-local
+    const uint32_t* small; // boxes done whole by one thread
+    uint32_t nsmall;
+    const uint4* chunks;   // (box, first triangle, triangles, 0)
+    uint32_t nchunks;
+    const uint4* big;      // (box, first chunk, chunks, 0)
+    uint32_t nbig;
+};
 // One skinned mesh in a skin/refit launch.
 struct SkinRec {
     uint32_t mesh, vbase, nvert, nbones, bone_base, tri_base, ntri, vidx_base;
@@ -183,10 +195,11 @@ struct cd_ctx {
     struct SkinHost { uint32_t vbase, nvert, nbones, vidx_base; };
     std::vector<std::pair<uint32_t, SkinHost>> skins;  // (mesh id, data)
     cdk::DevBuf d_bind, d_joints, d_weights, d_vidx, d_skinpos, d_bones, d_skinrec, d_refit_items, d_refit_bits,
-        d_refit_scratch;
+        d_refit_scratch, d_refit_ranges, d_refit_plan, d_refit_partial;
     size_t skin_verts = 0, skin_vidx = 0;
     std::vector<uint32_t> refit_key;       // mesh list the cached refit items are for
     uint32_t refit_nitems = 0;
+    cdk::RefitPlan refit_plan{};
 
     hipGraphExec_t graph = nullptr;        // captured batched pipeline
     std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured

@@ -83,8 +83,12 @@ hipError_t launch_skin(const SkinRec* sk, uint32_t nsk, const uint32_t* vstart, 
                        const float4* weights, const float* bones, uint32_t total, float* out, hipStream_t st);
 hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* tstart, const uint32_t* vidx,
                               const float* pos, float* tris, uint32_t total, hipStream_t st);
-hipError_t launch_refit(const uint2* items, uint32_t nitems, MeshRec* meshes, NodeRec* nodes, const float* tris,
-                        double* scratch, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
+hipError_t launch_item_ranges(const uint2* items, uint32_t nitems, const MeshRec* meshes, const NodeRec* nodes,
+                              uint2* ranges, hipStream_t st);
+hipError_t launch_refit(const RefitPlan& p, MeshRec* meshes, NodeRec* nodes, const float* tris, double* scratch,
+                        double* partial, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
                         uint32_t nids, hipStream_t st);
+uint32_t refit_small_max();
+uint32_t refit_chunk_tris();
 
 }  // namespace cdk

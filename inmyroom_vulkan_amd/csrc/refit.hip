@@ -7,12 +7,12 @@
 //                skinning shader (shaders/dynamicMeshShader_glsl.comp:120-145):
 //                p' = sum_k w_k (S_k * (p, 1)), S_k = joint matrix * inverse bind
 //  k_gather_tris rebuilds the tree-order triangle arena from skinned vertices
-//  k_refit_*     every box of the tree (root and both children of every
+//  k_rf_*        every box of the tree (root and both children of every
 //                internal node) is recomputed over its own triangles with the
 //                PCA construction of OBB.cpp:33-166 (mean, covariance, eig3,
 //                axes = rows of V, extents + 2 FLT_EPSILON); the topology built
-//                at registration is kept.  The FP64 sums are parallel
-//                wavefront reductions, so the bits differ from a sequential
+//                at registration is kept.  The FP64 sums of a large box are
+//                chunked parallel reductions, so the bits differ from a sequential
 //                build; boxes still contain their triangles, and the
 //                containment slack and radius of each mesh are recomputed.
 #include <float.h>
@@ -114,110 +114,242 @@ __device__ __forceinline__ double wave_max_d(double v) {
     return v;
 }
 
-// Each box is refit in three launches, so the serial eigen-solve runs one
-// box per lane instead of one lane per workgroup:
-//   k_refit_moments  one wavefront per box: distinct-point check, mean and
-//                    covariance (two passes, FP64)
-//   k_refit_eig      one thread per box: eig3, axes = rows of V
-//   k_refit_box      one wavefront per box: extents along the axes, the
-//                    box and its surface, then the containment slack
-// Scratch per box: RS_WORDS doubles.
+// Boxes are refit by size class, so that neither the serial eigen-solve nor
+// the passes over a large box's points sit on one wavefront:
+//   k_rf_small   one thread per small box (<= RF_SMALL triangles): the whole
+//                construction in registers
+//   large boxes  their triangles split into chunks of RF_CHUNK; per pass one
+//                wavefront per chunk writes a partial result and one thread
+//                per box combines its chunks in chunk order:
+//                sums -> mean, centred products -> covariance and eig3,
+//                projections -> extents and the box, then the slack
+// Containment slack and radius end in per-mesh atomic maxima.
+constexpr uint32_t RF_SMALL = 16, RF_CHUNK = 512;
 constexpr int RS_MEAN = 0, RS_COV = 3, RS_AX = 9, RS_KIND = 18, RS_WORDS = 24;
-constexpr int REFIT_WAVES = 4;  // boxes per 256-thread workgroup
+constexpr int PW = 8;  // doubles per chunk partial
 
-__global__ __launch_bounds__(64 * REFIT_WAVES) void k_refit_moments(const uint2* __restrict__ items, uint32_t nitems,
-                                                                    const MeshRec* __restrict__ meshes,
-                                                                    const NodeRec* __restrict__ nodes,
-                                                                    const float* __restrict__ tris,
-                                                                    double* __restrict__ scratch) {
-    const uint32_t it = blockIdx.x * REFIT_WAVES + (threadIdx.x >> 6);
-    if (it >= nitems) return;  // whole wavefronts leave; no workgroup barrier follows
-    const int lane = threadIdx.x & 63;
-    const uint2 item = items[it];
-    const MeshRec& m = meshes[item.x];
-    uint32_t off, cnt;
-    item_range(m, nodes, item.y, off, cnt);
-    const float* P = tris + (size_t(m.tri_base) + off) * 9;
-    const uint32_t npts = 3 * cnt;
-    double* out = scratch + size_t(it) * RS_WORDS;
-    // distinct-point check (OBB.cpp:35-50): more than one distinct point -> PCA
-    bool differs = false;
-    for (uint32_t k = 1 + lane; k < npts; k += 64)
-        differs |= !(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2]);
-    const bool multi = __ballot(differs) != 0ull;
-    if (npts == 0 || !multi) {
-        if (lane == 0) out[RS_KIND] = npts == 0 ? 0.0 : 1.0;
-        return;
-    }
-    const double N = double(npts);
-    double sx = 0, sy = 0, sz = 0;
-    for (uint32_t k = lane; k < npts; k += 64) {
-        sx += double(P[3 * k]);
-        sy += double(P[3 * k + 1]);
-        sz += double(P[3 * k + 2]);
-    }
-    // every lane needs the mean: take lane 0's sums
-    const double mx = __shfl(wave_sum_d(sx), 0, 64) / N;
-    const double my = __shfl(wave_sum_d(sy), 0, 64) / N;
-    const double mz = __shfl(wave_sum_d(sz), 0, 64) / N;
-    double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
-    for (uint32_t k = lane; k < npts; k += 64) {
-        const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
-        cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
-        cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
-    }
-    cxx = wave_sum_d(cxx); cyy = wave_sum_d(cyy); czz = wave_sum_d(czz);
-    cxy = wave_sum_d(cxy); cxz = wave_sum_d(cxz); cyz = wave_sum_d(cyz);
-    if (lane == 0) {
-        out[RS_MEAN] = mx; out[RS_MEAN + 1] = my; out[RS_MEAN + 2] = mz;
-        out[RS_COV] = cxx / N; out[RS_COV + 1] = cyy / N; out[RS_COV + 2] = czz / N;
-        out[RS_COV + 3] = cxy / N; out[RS_COV + 4] = cxz / N; out[RS_COV + 5] = cyz / N;
-        out[RS_KIND] = 2.0;
-    }
+__device__ __forceinline__ const float* item_tris(const MeshRec& m, const float* tris, uint32_t off) {
+    return tris + (size_t(m.tri_base) + off) * 9;
 }
 
-__global__ __launch_bounds__(256) void k_refit_eig(uint32_t nitems, double* __restrict__ scratch) {
-    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= nitems) return;
-    double* s = scratch + size_t(it) * RS_WORDS;
-    if (s[RS_KIND] != 2.0) return;
-    const double xx = s[RS_COV], yy = s[RS_COV + 1], zz = s[RS_COV + 2];
-    const double xy = s[RS_COV + 3], xz = s[RS_COV + 4], yz = s[RS_COV + 5];
+// OBB::CreateAABBfromPoints tail (OBB.cpp:120-166): box from axes and extents.
+__device__ inline Box box_from_extents(const double* ax, const double* lo, const double* hi) {
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    float side[3][3];
+    for (int q = 0; q < 3; ++q) {
+        const double delta = (hi[q] - lo[q]) + 2.0 * double(FLT_EPSILON);
+        const double mid = (hi[q] + lo[q]) / 2.0;
+        cx = cx + mid * ax[3 * q];
+        cy = cy + mid * ax[3 * q + 1];
+        cz = cz + mid * ax[3 * q + 2];
+        const double half = delta / 2.0;
+        side[q][0] = float(half * ax[3 * q]);
+        side[q][1] = float(half * ax[3 * q + 1]);
+        side[q][2] = float(half * ax[3 * q + 2]);
+    }
+    Box box;
+    box.c = mk(float(cx), float(cy), float(cz));
+    box.u = mk(side[0][0], side[0][1], side[0][2]);
+    box.v = mk(side[1][0], side[1][1], side[1][2]);
+    box.w = mk(side[2][0], side[2][1], side[2][2]);
+    return box;
+}
+// OBB::EmptyOBB (OBB.cpp:168-178), centred on the single point if there is one.
+__device__ inline Box empty_box(const float* P, uint32_t npts) {
+    const float e = FLT_EPSILON;
+    Box box;
+    box.c = npts ? mk(P[0], P[1], P[2]) : mk(0.f, 0.f, 0.f);
+    box.u = mk(e, 0.f, 0.f);
+    box.v = mk(0.f, e, 0.f);
+    box.w = mk(0.f, 0.f, e);
+    return box;
+}
+__device__ inline void axes_of(double xx, double yy, double zz, double xy, double xz, double yz, double* ax) {
     double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
     double V[3][3], d[3];
     eig3(A, V, d);
     for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) s[RS_AX + r * 3 + c] = V[r][c];  // rows of V (OBB.cpp:86-87)
+        for (int c = 0; c < 3; ++c) ax[r * 3 + c] = V[r][c];  // rows of V (OBB.cpp:86-87)
+}
+__device__ inline void store_box(MeshRec& m, NodeRec* nodes, uint32_t code, const Box& box) {
+    const float sf = surface(box);
+    if (code == ROOT_CODE) {
+        box_to12(box, m.root);
+        m.root_surf = sf;
+    } else {
+        NodeRec& n = nodes[m.node_base + (code >> 1)];
+        box_to12(box, n.box[code & 1]);
+        n.surf[code & 1] = sf;
+    }
+}
+// How far a point lies outside a box (distance along each side beyond the
+// box), in FP64 from the stored float box.
+struct SlackFrame {
+    double c[3], u[3], v[3], w[3], D, lu, lv, lw;
+};
+__device__ inline double det3(const double* a, const double* b, const double* cc) {
+    return a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) + a[2] * (b[0] * cc[1] - b[1] * cc[0]);
+}
+__device__ inline SlackFrame slack_frame(const float* bx) {
+    SlackFrame f;
+    for (int k = 0; k < 3; ++k) {
+        f.c[k] = bx[k];
+        f.u[k] = bx[3 + k];
+        f.v[k] = bx[6 + k];
+        f.w[k] = bx[9 + k];
+    }
+    f.D = det3(f.u, f.v, f.w);
+    f.lu = sqrt(f.u[0] * f.u[0] + f.u[1] * f.u[1] + f.u[2] * f.u[2]);
+    f.lv = sqrt(f.v[0] * f.v[0] + f.v[1] * f.v[1] + f.v[2] * f.v[2]);
+    f.lw = sqrt(f.w[0] * f.w[0] + f.w[1] * f.w[1] + f.w[2] * f.w[2]);
+    return f;
+}
+__device__ inline double excess(const SlackFrame& f, const double* p) {
+    const double r[3] = {p[0] - f.c[0], p[1] - f.c[1], p[2] - f.c[2]};
+    if (!(fabs(f.D) > 0.0)) return INFINITY;
+    const double al = det3(r, f.v, f.w) / f.D, be = det3(f.u, r, f.w) / f.D, ga = det3(f.u, f.v, r) / f.D;
+    return fmax(0.0, fabs(al) - 1.0) * f.lu + fmax(0.0, fabs(be) - 1.0) * f.lv + fmax(0.0, fabs(ga) - 1.0) * f.lw;
+}
+__device__ inline void post_slack(unsigned int* slack_bits, unsigned int* radius_bits, uint32_t mesh, bool root,
+                                  double worst, double rad) {
+    float wf = float(worst);
+    if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
+    float rf = float(rad);
+    if (double(rf) < rad) rf = nextafterf(rf, INFINITY);
+    atomicMax(&slack_bits[mesh], __float_as_uint(wf));
+    if (root) atomicMax(&radius_bits[mesh], __float_as_uint(rf));
+}
+__device__ inline const float* box_ptr(const MeshRec& m, const NodeRec* nodes, uint32_t code) {
+    return code == ROOT_CODE ? m.root : nodes[m.node_base + (code >> 1)].box[code & 1];
 }
 
-__global__ __launch_bounds__(64 * REFIT_WAVES) void k_refit_box(const uint2* __restrict__ items, uint32_t nitems,
-                                                                MeshRec* __restrict__ meshes,
-                                                                NodeRec* __restrict__ nodes,
-                                                                const float* __restrict__ tris,
-                                                                const double* __restrict__ scratch,
-                                                                unsigned int* __restrict__ slack_bits,
-                                                                unsigned int* __restrict__ radius_bits) {
-    const uint32_t it = blockIdx.x * REFIT_WAVES + (threadIdx.x >> 6);
+// Triangle range of every item (tree order, mesh-local).
+__global__ void k_item_ranges(const uint2* __restrict__ items, uint32_t nitems, const MeshRec* __restrict__ meshes,
+                              const NodeRec* __restrict__ nodes, uint2* __restrict__ ranges) {
+    const uint32_t it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= nitems) return;
-    const int lane = threadIdx.x & 63;
+    const uint2 item = items[it];
+    uint32_t off, cnt;
+    item_range(meshes[item.x], nodes, item.y, off, cnt);
+    ranges[it] = make_uint2(off, cnt);
+}
+
+// A small box, whole, in one thread.
+__global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ items, const uint32_t* __restrict__ list,
+                                                  uint32_t n, const uint2* __restrict__ ranges,
+                                                  MeshRec* __restrict__ meshes, NodeRec* __restrict__ nodes,
+                                                  const float* __restrict__ tris, unsigned int* __restrict__ slack_bits,
+                                                  unsigned int* __restrict__ radius_bits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t it = list[i];
     const uint2 item = items[it];
     MeshRec& m = meshes[item.x];
-    uint32_t off, cnt;
-    item_range(m, nodes, item.y, off, cnt);
-    const float* P = tris + (size_t(m.tri_base) + off) * 9;
-    const uint32_t npts = 3 * cnt;
-    const double* s = scratch + size_t(it) * RS_WORDS;
+    const uint2 rg = ranges[it];
+    const float* P = item_tris(m, tris, rg.x);
+    const uint32_t npts = 3 * rg.y;
+    bool multi = false;
+    for (uint32_t k = 1; k < npts && !multi; ++k)
+        multi = !(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2]);
     Box box;
-    if (s[RS_KIND] != 2.0) {  // OBB::EmptyOBB (OBB.cpp:168-178), centred on the single point if there is one
-        const float e = FLT_EPSILON;
-        box.c = npts ? mk(P[0], P[1], P[2]) : mk(0.f, 0.f, 0.f);
-        box.u = mk(e, 0.f, 0.f);
-        box.v = mk(0.f, e, 0.f);
-        box.w = mk(0.f, 0.f, e);
+    if (!multi) {
+        box = empty_box(P, npts);
     } else {
+        const double N = double(npts);
+        double sx = 0, sy = 0, sz = 0;
+        for (uint32_t k = 0; k < npts; ++k) {
+            sx += double(P[3 * k]);
+            sy += double(P[3 * k + 1]);
+            sz += double(P[3 * k + 2]);
+        }
+        const double mx = sx / N, my = sy / N, mz = sz / N;
+        double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
+        for (uint32_t k = 0; k < npts; ++k) {
+            const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+            cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
+            cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
+        }
+        double ax[9];
+        axes_of(cxx / N, cyy / N, czz / N, cxy / N, cxz / N, cyz / N, ax);
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t k = 0; k < npts; ++k) {
+            const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+            for (int q = 0; q < 3; ++q) {
+                const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
+                lo[q] = fmin(lo[q], pr);
+                hi[q] = fmax(hi[q], pr);
+            }
+        }
+        box = box_from_extents(ax, lo, hi);
+    }
+    store_box(m, nodes, item.y, box);
+    float b12[12];
+    box_to12(box, b12);
+    const SlackFrame f = slack_frame(b12);
+    double worst = 0.0, rad = 0.0;
+    for (uint32_t k = 0; k < npts; ++k) {
+        const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+        worst = fmax(worst, excess(f, p));
+        rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+    }
+    post_slack(slack_bits, radius_bits, item.x, item.y == ROOT_CODE, worst, rad);
+}
+
+// ---- large boxes: chunk passes ----
+// chunk = {item, first triangle (mesh-local), triangles}
+enum : int { RF_SUM = 0, RF_COV = 1, RF_EXT = 2, RF_SLACK = 3 };
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ items, const uint4* __restrict__ chunks,
+                                                  uint32_t nchunks, const uint2* __restrict__ ranges,
+                                                  const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
+                                                  const float* __restrict__ tris, const double* __restrict__ scratch,
+                                                  double* __restrict__ partial, unsigned int* __restrict__ slack_bits,
+                                                  unsigned int* __restrict__ radius_bits) {
+    const uint32_t ci = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ci >= nchunks) return;  // whole wavefronts leave; no workgroup barrier follows
+    const int lane = threadIdx.x & 63;
+    const uint4 ch = chunks[ci];
+    const uint2 item = items[ch.x];
+    const MeshRec& m = meshes[item.x];
+    const float* P = item_tris(m, tris, ch.y);
+    const uint32_t npts = 3 * ch.z;
+    const double* s = scratch + size_t(ch.x) * RS_WORDS;
+    double* out = partial + size_t(ci) * PW;
+    if (PASS == RF_SUM) {
+        const float* P0 = item_tris(m, tris, ranges[ch.x].x);  // the box's first point
+        bool differs = false;
+        double sx = 0, sy = 0, sz = 0;
+        for (uint32_t k = lane; k < npts; k += 64) {
+            differs |= !(P[3 * k] == P0[0] && P[3 * k + 1] == P0[1] && P[3 * k + 2] == P0[2]);
+            sx += double(P[3 * k]);
+            sy += double(P[3 * k + 1]);
+            sz += double(P[3 * k + 2]);
+        }
+        const bool any = __ballot(differs) != 0ull;
+        sx = wave_sum_d(sx); sy = wave_sum_d(sy); sz = wave_sum_d(sz);
+        if (lane == 0) {
+            out[0] = sx; out[1] = sy; out[2] = sz;
+            out[7] = any ? 1.0 : 0.0;
+        }
+    } else if (PASS == RF_COV) {
+        if (s[RS_KIND] != 2.0) return;
+        const double mx = s[RS_MEAN], my = s[RS_MEAN + 1], mz = s[RS_MEAN + 2];
+        double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
+        for (uint32_t k = lane; k < npts; k += 64) {
+            const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+            cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
+            cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
+        }
+        cxx = wave_sum_d(cxx); cyy = wave_sum_d(cyy); czz = wave_sum_d(czz);
+        cxy = wave_sum_d(cxy); cxz = wave_sum_d(cxz); cyz = wave_sum_d(cyz);
+        if (lane == 0) {
+            out[0] = cxx; out[1] = cyy; out[2] = czz; out[3] = cxy; out[4] = cxz; out[5] = cyz;
+        }
+    } else if (PASS == RF_EXT) {
+        if (s[RS_KIND] != 2.0) return;
         double ax[9];
         for (int t = 0; t < 9; ++t) ax[t] = s[RS_AX + t];
-        // OBB::CreateAABBfromPoints (OBB.cpp:105-166)
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t k = lane; k < npts; k += 64) {
             const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
@@ -231,70 +363,71 @@ __global__ __launch_bounds__(64 * REFIT_WAVES) void k_refit_box(const uint2* __r
             lo[q] = wave_min_d(lo[q]);
             hi[q] = wave_max_d(hi[q]);
         }
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        float side[3][3];
-        for (int q = 0; q < 3; ++q) {
-            const double delta = (hi[q] - lo[q]) + 2.0 * double(FLT_EPSILON);
-            const double mid = (hi[q] + lo[q]) / 2.0;
-            cx = cx + mid * ax[3 * q];
-            cy = cy + mid * ax[3 * q + 1];
-            cz = cz + mid * ax[3 * q + 2];
-            const double half = delta / 2.0;
-            side[q][0] = float(half * ax[3 * q]);
-            side[q][1] = float(half * ax[3 * q + 1]);
-            side[q][2] = float(half * ax[3 * q + 2]);
+        if (lane == 0)
+            for (int q = 0; q < 3; ++q) {
+                out[q] = lo[q];
+                out[3 + q] = hi[q];
+            }
+    } else {
+        const SlackFrame f = slack_frame(box_ptr(m, nodes, item.y));
+        double worst = 0.0, rad = 0.0;
+        for (uint32_t k = lane; k < npts; k += 64) {
+            const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+            worst = fmax(worst, excess(f, p));
+            rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
         }
-        box.c = mk(float(cx), float(cy), float(cz));
-        box.u = mk(side[0][0], side[0][1], side[0][2]);
-        box.v = mk(side[1][0], side[1][1], side[1][2]);
-        box.w = mk(side[2][0], side[2][1], side[2][2]);
+        worst = wave_max_d(worst);
+        rad = wave_max_d(rad);
+        if (lane == 0) post_slack(slack_bits, radius_bits, item.x, item.y == ROOT_CODE, worst, rad);
     }
-    // every lane holds the same box (min and max reductions are exact)
-    if (lane == 0) {
-        const float sf = surface(box);
-        if (item.y == ROOT_CODE) {
-            box_to12(box, m.root);
-            m.root_surf = sf;
+}
+
+// One thread per large box: combine its chunks (in chunk order) after a pass.
+// big = {item, first chunk, chunks}
+template <int PASS>
+__global__ __launch_bounds__(256) void k_rf_big(const uint2* __restrict__ items, const uint4* __restrict__ big,
+                                                uint32_t nbig, const uint2* __restrict__ ranges,
+                                                MeshRec* __restrict__ meshes, NodeRec* __restrict__ nodes,
+                                                const float* __restrict__ tris, const double* __restrict__ partial,
+                                                double* __restrict__ scratch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbig) return;
+    const uint4 b = big[i];
+    double* s = scratch + size_t(b.x) * RS_WORDS;
+    const double* pp = partial + size_t(b.y) * PW;
+    const uint2 rg = ranges[b.x];
+    const double N = 3.0 * double(rg.y);
+    if (PASS == RF_SUM) {
+        double sx = 0, sy = 0, sz = 0;
+        bool any = false;
+        for (uint32_t c = 0; c < b.z; ++c) {
+            sx += pp[c * PW]; sy += pp[c * PW + 1]; sz += pp[c * PW + 2];
+            any |= pp[c * PW + 7] != 0.0;
+        }
+        s[RS_MEAN] = sx / N; s[RS_MEAN + 1] = sy / N; s[RS_MEAN + 2] = sz / N;
+        s[RS_KIND] = any ? 2.0 : 1.0;
+    } else if (PASS == RF_COV) {
+        if (s[RS_KIND] != 2.0) return;
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t c = 0; c < b.z; ++c)
+            for (int k = 0; k < 6; ++k) v[k] += pp[c * PW + k];
+        axes_of(v[0] / N, v[1] / N, v[2] / N, v[3] / N, v[4] / N, v[5] / N, s + RS_AX);
+    } else {  // RF_EXT: the box
+        const uint2 item = items[b.x];
+        MeshRec& m = meshes[item.x];
+        Box box;
+        if (s[RS_KIND] != 2.0) {
+            box = empty_box(item_tris(m, tris, rg.x), 3 * rg.y);
         } else {
-            NodeRec& n = nodes[m.node_base + (item.y >> 1)];
-            box_to12(box, n.box[item.y & 1]);
-            n.surf[item.y & 1] = sf;
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t c = 0; c < b.z; ++c)
+                for (int q = 0; q < 3; ++q) {
+                    lo[q] = fmin(lo[q], pp[c * PW + q]);
+                    hi[q] = fmax(hi[q], pp[c * PW + 3 + q]);
+                }
+            box = box_from_extents(s + RS_AX, lo, hi);
         }
-    }
-    // containment slack: how far any of the box's points lies outside it
-    // (the batched traversal widens its separating-axis test by this much)
-    const double c[3] = {box.c.x, box.c.y, box.c.z};
-    const double u[3] = {box.u.x, box.u.y, box.u.z}, v[3] = {box.v.x, box.v.y, box.v.z};
-    const double w[3] = {box.w.x, box.w.y, box.w.z};
-    auto det3 = [](const double* a, const double* b, const double* cc) {
-        return a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) +
-               a[2] * (b[0] * cc[1] - b[1] * cc[0]);
-    };
-    const double D = det3(u, v, w);
-    const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    const double lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-    double worst = 0.0, rad = 0.0;
-    for (uint32_t k = lane; k < npts; k += 64) {
-        const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
-        const double r[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
-        double ex = INFINITY;
-        if (fabs(D) > 0.0) {
-            const double al = det3(r, v, w) / D, be = det3(u, r, w) / D, ga = det3(u, v, r) / D;
-            ex = fmax(0.0, fabs(al) - 1.0) * lu + fmax(0.0, fabs(be) - 1.0) * lv + fmax(0.0, fabs(ga) - 1.0) * lw;
-        }
-        worst = fmax(worst, ex);
-        rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
-    }
-    worst = wave_max_d(worst);
-    rad = wave_max_d(rad);
-    if (lane == 0) {
-        float wf = float(worst);
-        if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
-        float rf = float(rad);
-        if (double(rf) < rad) rf = nextafterf(rf, INFINITY);
-        atomicMax(&slack_bits[item.x], __float_as_uint(wf));
-        if (item.y == ROOT_CODE) atomicMax(&radius_bits[item.x], __float_as_uint(rf));
+        store_box(m, nodes, item.y, box);
     }
 }
 
@@ -333,23 +466,40 @@ hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* t
     hipLaunchKernelGGL(k_gather_tris, g256r(total), dim3(256), 0, st, sk, nsk, tstart, vidx, pos, tris, total);
     return hipGetLastError();
 }
-hipError_t launch_refit(const uint2* items, uint32_t nitems, MeshRec* meshes, NodeRec* nodes, const float* tris,
-                        double* scratch, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
+hipError_t launch_item_ranges(const uint2* items, uint32_t nitems, const MeshRec* meshes, const NodeRec* nodes,
+                              uint2* ranges, hipStream_t st) {
+    hipLaunchKernelGGL(k_item_ranges, g256r(nitems), dim3(256), 0, st, items, nitems, meshes, nodes, ranges);
+    return hipGetLastError();
+}
+hipError_t launch_refit(const RefitPlan& p, MeshRec* meshes, NodeRec* nodes, const float* tris, double* scratch,
+                        double* partial, unsigned int* slack_bits, unsigned int* radius_bits, const uint32_t* ids,
                         uint32_t nids, hipStream_t st) {
-    if (!nitems) return hipSuccess;
-    const dim3 gw((nitems + REFIT_WAVES - 1) / REFIT_WAVES);
-    hipLaunchKernelGGL(k_refit_moments, gw, dim3(64 * REFIT_WAVES), 0, st, items, nitems, meshes, nodes, tris, scratch);
+    if (p.nsmall)
+        hipLaunchKernelGGL(k_rf_small, g256r(p.nsmall), dim3(256), 0, st, p.items, p.small, p.nsmall, p.ranges, meshes,
+                           nodes, tris, slack_bits, radius_bits);
+    if (p.nbig) {
+        const dim3 gc((p.nchunks + 3) / 4), gb = g256r(p.nbig);
+        hipLaunchKernelGGL(k_rf_chunk<RF_SUM>, gc, dim3(256), 0, st, p.items, p.chunks, p.nchunks, p.ranges, meshes,
+                           nodes, tris, scratch, partial, slack_bits, radius_bits);
+        hipLaunchKernelGGL(k_rf_big<RF_SUM>, gb, dim3(256), 0, st, p.items, p.big, p.nbig, p.ranges, meshes, nodes,
+                           tris, partial, scratch);
+        hipLaunchKernelGGL(k_rf_chunk<RF_COV>, gc, dim3(256), 0, st, p.items, p.chunks, p.nchunks, p.ranges, meshes,
+                           nodes, tris, scratch, partial, slack_bits, radius_bits);
+        hipLaunchKernelGGL(k_rf_big<RF_COV>, gb, dim3(256), 0, st, p.items, p.big, p.nbig, p.ranges, meshes, nodes,
+                           tris, partial, scratch);
+        hipLaunchKernelGGL(k_rf_chunk<RF_EXT>, gc, dim3(256), 0, st, p.items, p.chunks, p.nchunks, p.ranges, meshes,
+                           nodes, tris, scratch, partial, slack_bits, radius_bits);
+        hipLaunchKernelGGL(k_rf_big<RF_EXT>, gb, dim3(256), 0, st, p.items, p.big, p.nbig, p.ranges, meshes, nodes,
+                           tris, partial, scratch);
+        hipLaunchKernelGGL(k_rf_chunk<RF_SLACK>, gc, dim3(256), 0, st, p.items, p.chunks, p.nchunks, p.ranges, meshes,
+                           nodes, tris, scratch, partial, slack_bits, radius_bits);
+    }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_refit_eig, g256r(nitems), dim3(256), 0, st, nitems, scratch);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_refit_box, gw, dim3(64 * REFIT_WAVES), 0, st, items, nitems, meshes, nodes, tris, scratch,
-                       slack_bits, radius_bits);
-    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_store_slack, g256r(nids), dim3(256), 0, st, meshes, ids, nids, slack_bits, radius_bits);
     return hipGetLastError();
 }
+uint32_t refit_small_max() { return RF_SMALL; }
+uint32_t refit_chunk_tris() { return RF_CHUNK; }
 
 }  // namespace cdk
