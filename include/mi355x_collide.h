@@ -144,7 +144,8 @@ typedef struct cd_tuning {
     uint32_t passes;         /* persistent-kernel launches, 1..8; before the last, busy wavefronts hand their
                                 stacks on once idle_pct percent of the grid has run out of work */
     uint32_t idle_pct;       /* 1..100 */
-    uint32_t reserved;
+    uint32_t split_tris;     /* a split level keeps splitting only node pairs over this many triangles
+                                (0: all of them); the others go to the persistent kernel */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

@@ -1,6 +1,7 @@
 """Autotuner of the batched path's launch parameters.
 
-Searches split levels (work-split granularity), LDS stack entries per
+Searches split levels and the size above which a node pair keeps being
+split (work-split granularity), LDS stack entries per
 wavefront (stack depth), node pairs popped per step (lanes per step of the
 wavefront traversal), persistent wavefronts per CU and the rebalancing
 schedule (traversal launches, and the share of idle wavefronts at which the
@@ -23,12 +24,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
 SPACE = {
-    "split_levels": [6, 8, 10, 14],
-    "stack_entries": [512, 1024],
+    # joint keys: each value sets both fields
+    "split_levels/split_tris": [(6, 0), (8, 0), (10, 0), (14, 0), (16, 512), (20, 512), (24, 2048), (32, 2048),
+                                (32, 8192)],
+    "stack_entries": [512],
     "pop_width": [64],
     "waves_per_cu": [4, 8],
-    # joint keys: each value sets both fields
-    "passes/idle_pct": [(1, 50), (2, 50), (3, 25), (3, 50), (3, 75), (5, 50)],
+    "passes/idle_pct": [(1, 50), (3, 50)],
 }
 
 
@@ -109,8 +111,8 @@ def main():
     q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
     space = None
     if args.quick:
-        space = {"split_levels": [6, 10], "stack_entries": [512], "pop_width": [64], "waves_per_cu": [4],
-                 "passes/idle_pct": [(1, 50), (3, 50)]}
+        space = {"split_levels/split_tris": [(10, 0), (24, 2048)], "stack_entries": [512], "pop_width": [64],
+                 "waves_per_cu": [4], "passes/idle_pct": [(1, 50)]}
     res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
     save(args.nq, res, args.out)
     print(json.dumps(res), flush=True)

@@ -94,6 +94,7 @@ struct Code {  // resolved traversal code
     float surf;
     bool leaf;
     uint32_t link, cnt;  // link: mesh-local child node index, or triangle offset of a leaf
+    uint32_t tris;       // triangles under the box
 };
 __device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
                                         uint32_t code) {
@@ -107,6 +108,8 @@ __device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, cons
     const float* sf = root ? &mr->root_surf : &nr->surf[side];
     const uint32_t* w1 = root ? &mr->ntri : &nr->link[side];
     const uint32_t* w2 = root ? &mr->nnodes : &nr->cnt;
+    const uint32_t* w3 = root ? &mr->ntri : &nr->pad[2];  // left child's triangles
+    const uint32_t* w4 = root ? &mr->ntri : &nr->pad[1];  // the node's triangles
     Code r;
     r.box = box_from12(bx);
     r.surf = *sf;
@@ -121,6 +124,8 @@ __device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, cons
     r.link = a & ~rm;
     r.cnt = (cnt_root & rm) | (cnt_node & ~rm);
     r.leaf = r.cnt != 0u;
+    const uint32_t left = *w3, all = *w4;
+    r.tris = root ? a : (side ? all - left : left);
     return r;
 }
 
@@ -139,7 +144,8 @@ __device__ __forceinline__ void close_chunk(uint4* __restrict__ lp_out, uint64_t
 // (first child on top), with the reference's descent rule.
 __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
                                          const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
-                                         uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf) {
+                                         uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf,
+                                         uint32_t& cost) {
     const QueryPre& Q = qp[qi];
     M4 M;
 #pragma unroll
@@ -164,6 +170,7 @@ __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const 
     n1a = split_a ? (ka | 1u) : ca;
     n0b = split_a ? cb : kb;
     n1b = split_a ? cb : (kb | 1u);
+    cost = A.tris + B0.tris;  // size of the pair's subtrees: how much work splitting it further would share out
     return hit ? (both_leaf ? PAIR_LEAF : PAIR_SPLIT) : PAIR_NONE;
 }
 
@@ -219,8 +226,14 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
                                                      uint32_t out_cap, const QueryPre* __restrict__ qp,
                                                      const MeshRec* __restrict__ meshes,
                                                      const NodeRec* __restrict__ nodes, uint4* __restrict__ lp_out,
-                                                     uint64_t lp_cap, RunCounters* __restrict__ ctr) {
-    const uint32_t n = min(*nin, out_cap);  // both frontier buffers hold out_cap entries
+                                                     uint64_t lp_cap, uint3* __restrict__ tasks,
+                                                     unsigned int* __restrict__ ntasks, uint32_t split_tris,
+                                                     RunCounters* __restrict__ ctr) {
+    // A pair that passes is split; its two children go on to the next level
+    // when the pair's subtrees hold more than split_tris triangles, and to the
+    // persistent kernel's task list otherwise.  The task list is written level
+    // by level, so read backwards it starts with the largest tasks.
+    const uint32_t n = min(*nin, out_cap);  // frontier buffers and the task list hold out_cap entries each
     const uint32_t stride = gridDim.x * blockDim.x;
     unsigned long long nsat = 0, nleaf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -228,12 +241,12 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
         int kind = PAIR_NONE;
         uint32_t n0a = 0, n0b = 0, n1a = 0, n1b = 0;
         uint4 leaf = make_uint4(0, 0, 0, 0);
-        uint32_t qi = 0;
+        uint32_t qi = 0, cost = 0;
         if (i < n) {
             const uint3 e = in[i];
             qi = e.x;
             ++nsat;
-            kind = eval_pair(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf);
+            kind = eval_pair(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf, cost);
         }
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
         if (mL) {
@@ -241,18 +254,23 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
             if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
             nleaf += unsigned(__popcll(mL));
         }
-        const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
-        if (mS) {
+        const bool go_on = cost > split_tris;
+        for (int dest = 0; dest < 2; ++dest) {  // 0: next level, 1: task list
+            const bool mine = kind == PAIR_SPLIT && go_on == (dest == 0);
+            const unsigned long long mS = __ballot(mine);
+            if (!mS) continue;
+            unsigned int* cnt_p = dest == 0 ? nout : ntasks;
+            uint3* dst = dest == 0 ? out : tasks;
             unsigned int base2 = 0;
             const int lane = threadIdx.x & 63;
             const int leader = __ffsll(static_cast<long long>(mS)) - 1;
-            if (lane == leader) base2 = atomicAdd(nout, 2u * unsigned(__popcll(mS)));
+            if (lane == leader) base2 = atomicAdd(cnt_p, 2u * unsigned(__popcll(mS)));
             base2 = __shfl(base2, leader, 64);
             const unsigned int slot = base2 + 2u * unsigned(__popcll(mS & lanes_below_b()));
-            if (kind == PAIR_SPLIT) {
+            if (mine) {
                 if (slot + 1 < out_cap) {
-                    out[slot] = make_uint3(qi, n0a, n0b);
-                    out[slot + 1] = make_uint3(qi, n1a, n1b);
+                    dst[slot] = make_uint3(qi, n0a, n0b);
+                    dst[slot + 1] = make_uint3(qi, n1a, n1b);
                 } else {
                     atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
                 }
@@ -350,7 +368,7 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
             const uint32_t got = t0 < ntasks ? min(want, ntasks - t0) : 0;
             if (got < want) tasks_done = true;
             if (uint32_t(lane) < got) {
-                const uint3 e = tasks[t0 + lane];
+                const uint3 e = tasks[ntasks - 1u - (t0 + lane)];  // backwards: largest tasks first
                 sq[size + lane] = e.x; sa[size + lane] = e.y; sb[size + lane] = e.z;
             }
             size += got;
@@ -374,7 +392,8 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         uint4 leaf = make_uint4(0, 0, 0, 0);
         if (have) {
             ++nsat;
-            kind = eval_pair(qp, meshes, nodes, qi, ca, cb, n0a, n0b, n1a, n1b, leaf);
+            uint32_t cost = 0;
+            kind = eval_pair(qp, meshes, nodes, qi, ca, cb, n0a, n0b, n1a, n1b, leaf, cost);
         }
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
         if (mL) {
@@ -586,9 +605,10 @@ hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigne
 }
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
-                              uint64_t lp_cap, RunCounters* ctr, uint32_t grid, hipStream_t st) {
+                              uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
+                              RunCounters* ctr, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_split_level, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
-                       lp_cap, ctr);
+                       lp_cap, tasks, ntasks, split_tris, ctr);
     return hipGetLastError();
 }
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,

@@ -373,7 +373,7 @@ __global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t*
     rec.cnt = cnt;
     rec.pad[0] = nd.toff;    // this node's subtree: triangles [toff, toff + tcount) in tree order
     rec.pad[1] = nd.tcount;
-    rec.pad[2] = 0;
+    rec.pad[2] = nodes[nd.kid[0]].tcount;  // of which the left child holds the first pad[2]
     out[node_base[nd.mesh] + nd.pre] = rec;
 }
 

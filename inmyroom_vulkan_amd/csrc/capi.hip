@@ -169,7 +169,7 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
     if (t.stack_entries != 512 && t.stack_entries != 2048) t.stack_entries = 1024;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
     if (t.waves_per_cu == 0) t.waves_per_cu = 8;
-    if (t.passes == 0) t.passes = 3;
+    if (t.passes == 0) t.passes = 1;
     if (t.passes > MAX_PASSES) t.passes = MAX_PASSES;
     if (t.idle_pct == 0 || t.idle_pct > 100) t.idle_pct = 50;
     return t;
@@ -177,8 +177,10 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
 
 constexpr uint32_t SPILL_CAP = 16384;  // entries of each persistent wavefront's HBM spill area
 constexpr uint32_t DUMP_CAP = 1u << 20;  // node pairs handed from one traversal launch to the next
-// W_SMALL layout: [0..L] split-level counts; from PASS_BASE, four words per
+// W_SMALL layout: [0..L] split-level counts; [TASK_COUNT] tasks the split
+// levels hand to the persistent kernel; from PASS_BASE, four words per
 // traversal launch: task head, idle wavefronts, dump count, unused.
+constexpr uint32_t TASK_COUNT = 70;
 constexpr uint32_t PASS_BASE = 72;
 constexpr uint32_t SMALL_WORDS = PASS_BASE + 4 * MAX_PASSES;
 
@@ -186,8 +188,8 @@ constexpr uint32_t SMALL_WORDS = PASS_BASE + 4 * MAX_PASSES;
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DUMP1,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS };
-static_assert(W_RAYS < 40, "work slots");
+       W_RAYS, W_TASKS };
+static_assert(W_TASKS < 40, "work slots");
 
 // Enqueue the whole batched pipeline on the stream (no host synchronisation
 // inside, so it can be captured in a graph).
@@ -215,17 +217,20 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dm, qp, st));
     HCK(ctx, launch_seed(nq, qp, F[0], &small[0], st));
-    for (uint32_t l = 0; l < L; ++l)
-        HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)),
-                                    qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctr, grid, st));
+    uint3* T = ctx->d_work[W_TASKS].as<uint3>();
+    const uint32_t cap32 = uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull));
+    for (uint32_t l = 0; l < L; ++l)  // the last level sends every split to the task list
+        HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], cap32, qp, dm, dn,
+                                    ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT],
+                                    l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
     const uint32_t dump_after = std::max(1u, uint32_t(uint64_t(nwaves) * t.idle_pct / 100));
     for (uint32_t p = 0; p < t.passes; ++p) {
         unsigned int* pw = &small[PASS_BASE + 4 * p];
         const bool first = p == 0, last = p + 1 == t.passes;
-        const uint3* tasks = first ? F[L & 1] : D[(p - 1) & 1];
-        const unsigned int* ntasks = first ? &small[L] : &small[PASS_BASE + 4 * (p - 1) + 2];
-        const uint32_t task_cap = first ? uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull)) : DUMP_CAP;
+        const uint3* tasks = first ? (L ? T : F[0]) : D[(p - 1) & 1];
+        const unsigned int* ntasks = first ? &small[L ? TASK_COUNT : 0] : &small[PASS_BASE + 4 * (p - 1) + 2];
+        const uint32_t task_cap = first ? cap32 : DUMP_CAP;
         HCK(ctx, launch_dfs(int(t.stack_entries), tasks, ntasks, task_cap, &pw[0], qp, dm, dn,
                             ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
                             t.pop_width, nwaves, &pw[1], D[p & 1], &pw[2], DUMP_CAP,
@@ -257,14 +262,16 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         const uint64_t fcap = frontier_cap(ctx, nq, L);
         HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
+        HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
-        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.passes, t.idle_pct, ctx->lp_cap,
+        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.passes, t.idle_pct, t.split_tris,
+                                     ctx->lp_cap,
                                      ctx->contact_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
-                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_DUMP1]})
+                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_DUMP1], &ctx->d_work[W_TASKS]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;

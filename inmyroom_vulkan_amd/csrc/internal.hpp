@@ -21,7 +21,7 @@ struct alignas(128) NodeRec {
     uint32_t link[2];   // child node index (count 0) or mesh-local triangle offset
     uint32_t cnt;       // left count | right count << 16 (0 = internal child)
     float surf[2];      // surface(box[0]), surface(box[1])
-    uint32_t pad[3];    // [0], [1]: this node's own triangle range (offset, count); used by refit
+    uint32_t pad[3];    // [0], [1]: this node's own triangle range (offset, count); [2]: the left child's count
 };
 static_assert(sizeof(NodeRec) == 128, "node record must stay one 128-byte line");
 

@@ -23,7 +23,8 @@ hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, 
 hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigned int* count, hipStream_t st);
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
-                              uint64_t lp_cap, RunCounters* ctr, uint32_t grid, hipStream_t st);
+                              uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
+                              RunCounters* ctr, uint32_t grid, hipStream_t st);
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
                       unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                       uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,

@@ -80,6 +80,9 @@ def test_batched_queries_uneven_subset(gpu, orc):
     # busy wavefronts hand their stacks on as soon as 1% of the grid is idle, every launch but the last
     dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, passes=8, idle_pct=1),
     dict(split_levels=0, stack_entries=1024, pop_width=32, waves_per_cu=2, passes=5, idle_pct=90),
+    # deep, selective splitting: only pairs over split_tris triangles go on to the next level
+    dict(split_levels=24, split_tris=64, stack_entries=512, pop_width=64, waves_per_cu=4),
+    dict(split_levels=40, split_tris=1, stack_entries=512, pop_width=64, waves_per_cu=4),
 ])
 def test_batched_tuning_variants_agree(gpu, tuning):
     soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
