@@ -27,6 +27,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -76,6 +77,23 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
     hip.hipFree(a)
     hip.hipFree(b)
     log(f"copy probe: {reps} x {nbytes} bytes device to device")
+
+
+PMC_SUMMARY = "profiles/traversal_narrow_4096q_pmc.md"
+
+
+def measured_traffic(kind: str, args):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a plain
+    copy there); None for other workloads."""
+    if args.nq != 4096 or args.seed != 3:
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), PMC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    key = "traffic_bytes_per_step" if kind == "traversal" else "narrow_traffic_bytes_per_launch"
+    m = re.search(key + r": (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(soups, qa, qb, rel, budget_s: float, leaf_max: int):
@@ -210,6 +228,7 @@ def main() -> int:
         dom, dom_ms, dom_bytes = "narrow (k_narrow_exact)", t_narrow, narrow_b
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     pipeline_gbs = (trav_b + narrow_b) / (ms_per_step * 1e-3) / 1e9
+    traffic = measured_traffic("traversal" if dom.startswith("traversal") else "narrow", args)
 
     value = args.nq / (ms_per_step * 1e-3)
     plain_qps = args.nq / (np.median(plain_ms) * 1e-3)
@@ -243,7 +262,8 @@ def main() -> int:
                       "plain_traversal": round(float(np.median(plain_trav_ms)), 4),
                       "plain_narrow": round(float(np.median(plain_narrow_ms)), 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_source": PMC_SUMMARY if traffic else None,
                      "algorithmic_bytes_per_launch": int(dom_bytes), "launch_ms": round(dom_ms, 4),
                      "pipeline_algorithmic_GBps": round(pipeline_gbs, 2)},
         "tuning": gpu.tuning(),
