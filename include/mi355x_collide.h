@@ -141,9 +141,10 @@ typedef struct cd_tuning {
     uint32_t pop_width;      /* node pairs a wavefront pops per step, 1..64 */
     uint32_t waves_per_cu;   /* persistent wavefronts per compute unit */
     uint32_t use_graph;      /* 1: capture the batched pipeline in a HIP graph and replay it */
-    uint32_t passes;         /* persistent-kernel launches, 1..8; before the last, busy wavefronts hand their
-                                stacks on once idle_pct percent of the grid has run out of work */
-    uint32_t idle_pct;       /* 1..100 */
+    uint32_t work_sharing;   /* 1: wavefronts that run out of work take node pairs busy ones donate;
+                                2: once half the grid is done, busy wavefronts hand their stacks to a
+                                   second launch that shares them out */
+    uint32_t spin_polls;     /* polls an idle wavefront waits for a donation before it leaves (0: 4096) */
     uint32_t split_tris;     /* a split level keeps splitting only node pairs over this many triangles
                                 (0: all of them); the others go to the persistent kernel */
 } cd_tuning;

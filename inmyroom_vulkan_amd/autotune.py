@@ -3,9 +3,8 @@
 Searches split levels and the size above which a node pair keeps being
 split (work-split granularity), LDS stack entries per
 wavefront (stack depth), node pairs popped per step (lanes per step of the
-wavefront traversal), persistent wavefronts per CU and the rebalancing
-schedule (traversal launches, and the share of idle wavefronts at which the
-busy ones hand their stacks to the next launch), timing each candidate
+wavefront traversal), persistent wavefronts per CU and work sharing (idle
+wavefronts taking node pairs that busy ones donate), timing each candidate
 with the library's own HIP events on staged queries, and caches the winner
 per (device architecture, workload size class) in tuning_gfx950.json.
 
@@ -30,7 +29,7 @@ SPACE = {
     "stack_entries": [512],
     "pop_width": [64],
     "waves_per_cu": [4, 8],
-    "passes/idle_pct": [(1, 50), (3, 50)],
+    "work_sharing": [0, 1],
 }
 
 
@@ -112,7 +111,7 @@ def main():
     space = None
     if args.quick:
         space = {"split_levels/split_tris": [(10, 0), (24, 2048)], "stack_entries": [512], "pop_width": [64],
-                 "waves_per_cu": [4], "passes/idle_pct": [(1, 50)]}
+                 "waves_per_cu": [4], "work_sharing": [0, 1]}
     res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
     save(args.nq, res, args.out)
     print(json.dumps(res), flush=True)

@@ -9,10 +9,9 @@
 //                 time from a stack in LDS (wave-private spill to HBM), tests
 //                 them, pushes children with ballot/prefix compaction, and
 //                 takes new tasks from a global counter when it runs low.
-//                 Launched a few times in a row: in all but the last launch,
-//                 once a set number of wavefronts has gone idle the busy ones
-//                 dump their stacks, and the next launch shares them out
-//                 (no wavefront ever waits on another)
+//                 With work sharing, wavefronts that run dry take stack
+//                 bottoms donated by busy ones (bounded waits only), and a
+//                 second launch takes whatever nobody claimed
 //  k_narrow_exact candidate leaf pairs expanded densely to triangle tests
 //                 (wave prefix scan), exact-sign triangle test, contacts
 //                 compacted with one atomic per wavefront
@@ -284,48 +283,115 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
     }
 }
 
+// Work sharing (share != 0).  A wavefront that runs out of tasks registers
+// as idle and polls a donation queue (the first POLLERS of them; later ones
+// leave); busy wavefronts that see idle ones push the bottom half of their
+// LDS stack (the shallowest,
+// hence largest, node pairs) into it.  Entries are written and read with
+// atomics only (no cache-wide fences, which write back or drop a whole L2),
+// the ready word last.  Every wait is bounded: an idle
+// wavefront leaves after spin_polls empty polls (or as soon as every
+// wavefront is idle), and a claimed entry whose ready word has not appeared
+// after READY_POLLS polls is reported (err bit 8) instead of waited for.  A
+// donor writes its entries right after reserving them and waits on nothing,
+// so neither bound is reached in practice.  Entries nobody claimed are taken
+// by a second launch with sharing off.
+// dq_ctl: [0] entries reserved by donors, [1] entries claimed, [2] wavefronts
+// idle or gone, [3] pollers waiting.
+constexpr uint32_t READY_POLLS = 1u << 16;
+constexpr uint32_t DONATE_MAX = 256;
+constexpr uint32_t POLLERS = 128;  // idle wavefronts that wait for donations; later ones leave
+
+// Read a word other wavefronts update, through the device's coherence point.
+// The L2 caches of the dies are not coherent with each other for ordinary
+// device memory, so a plain (or relaxed atomic) load can keep returning a
+// line this die cached earlier; a read-modify-write cannot.
+__device__ __forceinline__ unsigned int coherent_read(unsigned int* p) {
+    unsigned int zero = 0u;
+    asm volatile("" : "+v"(zero));  // opaque to the optimiser, which would turn "add 0" into a plain load
+    return atomicAdd(p, zero);
+}
+
 template <int CAP>
-__global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, const unsigned int* __restrict__ ntasks_p,
+__global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, const uint4* __restrict__ tasks4,
+                                            const unsigned int* __restrict__ task_lo_p,
+                                            const unsigned int* __restrict__ ntasks_p, uint32_t task_cap,
                                             unsigned int* __restrict__ head, const QueryPre* __restrict__ qp,
                                             const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
                                             uint4* __restrict__ lp_out, uint64_t lp_cap, uint3* __restrict__ spill_base,
-                                            uint32_t spill_cap, uint32_t pop_width, uint32_t task_cap,
-                                            unsigned int* __restrict__ exited, uint3* __restrict__ dump,
-                                            unsigned int* __restrict__ ndump, uint32_t dump_cap, uint32_t dump_after,
-                                            RunCounters* __restrict__ ctr) {
+                                            uint32_t spill_cap, uint32_t pop_width, uint4* __restrict__ dq,
+                                            unsigned int* __restrict__ dq_ctl, uint32_t dq_cap, uint32_t share,
+                                            uint32_t spin_polls, RunCounters* __restrict__ ctr) {
     __shared__ uint32_t sq[CAP], sa[CAP], sb[CAP];
     const int lane = threadIdx.x;
     uint3* spill = spill_base + size_t(blockIdx.x) * spill_cap;
-    const uint32_t ntasks = min(*ntasks_p, task_cap);
+    // tasks [lo, hi): the split levels' list, or (second launch) what nobody claimed from the queue
+    const uint32_t lo = task_lo_p ? *task_lo_p : 0u;
+    const uint32_t hi = min(*ntasks_p, task_cap);
+    const uint32_t ntasks = hi > lo ? hi - lo : 0u;
     const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
     uint32_t size = 0, nspill = 0;
-    bool tasks_done = false, overflow = false;
-    unsigned int gone_seen = 0, gone = 0, step = 0;  // idle wavefronts: lane 0's last read, and its broadcast
+    bool tasks_done = false, overflow = false, lost = false, poller = false;
+    unsigned int idle_read = 0, idle_seen = 0, step = 0;  // lane 0's last read of the idle count, and its broadcast
     unsigned long long nsat = 0, nleaf = 0;
     // Leaf-pair slots are taken LP_CHUNK at a time, so the shared counter
     // sees one atomic per chunk rather than one per step.
     uint64_t lp_next = 0, lp_end = 0;
     const unsigned long long below = lanes_below_b();
     for (;;) {
-        // Rebalancing: once dump_after wavefronts have run out of work, the
-        // ones still busy hand their whole stack (LDS and spill) to the dump
-        // list and leave; the next launch spreads that list over the grid.
         // The idle count is read every 8th step, one step ahead of its use,
         // so the (device-coherent, hence slow) load hides behind the step's
         // own loads and a thousand wavefronts do not queue on one address.
-        if (dump_after != 0xFFFFFFFFu) {
+        if (share == 2u) {
+            // Hand-off: once half the grid has finished, the wavefronts still
+            // busy write their whole stack (LDS and spill) to the queue and
+            // leave; the second launch shares the queue out over the grid.
             if ((step & 7u) == 0u) {
-                gone = __shfl(gone_seen, 0, 64);
-                if (lane == 0) gone_seen = __hip_atomic_load(exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                idle_seen = __shfl(idle_read, 0, 64);
+                if (lane == 0) idle_read = coherent_read(&dq_ctl[2]);  // wavefronts finished
             }
             ++step;
-            if (gone >= dump_after && (size + nspill) > 0) {
+            if (idle_seen >= gridDim.x / 2u && size + nspill > 0u) {
                 const uint32_t n = size + nspill;
                 unsigned int base = 0xFFFFFFFFu;
+                if (lane == 0) {
+                    unsigned int cur = coherent_read(&dq_ctl[0]);
+                    while (cur + n <= dq_cap) {
+                        const unsigned int prev = atomicCAS(&dq_ctl[0], cur, cur + n);
+                        if (prev == cur) {
+                            base = cur;
+                            break;
+                        }
+                        cur = prev;
+                    }
+                }
+                base = __shfl(base, 0, 64);
+                if (base != 0xFFFFFFFFu) {  // read by the next launch only: plain stores suffice
+                    for (uint32_t i = lane; i < size; i += 64) dq[base + i] = make_uint4(sq[i], sa[i], sb[i], 1u);
+                    for (uint32_t i = lane; i < nspill; i += 64) {
+                        const uint3 e = spill[i];
+                        dq[base + size + i] = make_uint4(e.x, e.y, e.z, 1u);
+                    }
+                    size = 0;
+                    nspill = 0;
+                    tasks_done = true;  // the task list is empty by now: half the grid ran out of it
+                    break;
+                }
+            }
+        }
+        if (share == 1u) {
+            if ((step & 7u) == 0u) {
+                idle_seen = __shfl(idle_read, 0, 64);
+                if (lane == 0) idle_read = coherent_read(&dq_ctl[3]);  // pollers waiting
+            }
+            ++step;
+            if (idle_seen > 0 && size >= 128u) {  // donate the bottom half of the stack
+                const uint32_t k = min(size / 2u, DONATE_MAX);
+                unsigned int base = 0xFFFFFFFFu;
                 if (lane == 0) {  // reserve without overshooting the capacity (each failed CAS means another wave won)
-                    unsigned int cur = __hip_atomic_load(ndump, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    while (cur + n <= dump_cap) {
-                        const unsigned int prev = atomicCAS(ndump, cur, cur + n);
+                    unsigned int cur = coherent_read(&dq_ctl[0]);
+                    while (cur + k <= dq_cap) {
+                        const unsigned int prev = atomicCAS(&dq_ctl[0], cur, cur + k);
                         if (prev == cur) {
                             base = cur;
                             break;
@@ -335,13 +401,28 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
                 }
                 base = __shfl(base, 0, 64);
                 if (base != 0xFFFFFFFFu) {
-                    for (uint32_t i = lane; i < size; i += 64) dump[base + i] = make_uint3(sq[i], sa[i], sb[i]);
-                    for (uint32_t i = lane; i < nspill; i += 64) dump[base + size + i] = spill[i];
-                    size = 0;
-                    nspill = 0;
+                    for (uint32_t i = lane; i < k; i += 64) {
+                        // every word goes through the coherence point (atomics),
+                        // the ready word only after the others have been performed
+                        unsigned int* e = reinterpret_cast<unsigned int*>(dq + base + i);
+                        const unsigned int r0 = atomicExch(&e[0], sq[i]);
+                        const unsigned int r1 = atomicExch(&e[1], sa[i]);
+                        const unsigned int r2 = atomicExch(&e[2], sb[i]);
+                        asm volatile("" ::"v"(r0), "v"(r1), "v"(r2));  // wait for their completion
+                        atomicExch(&e[3], 1u);
+                    }
                     __syncthreads();
+                    for (uint32_t i = 0; i + k < size; i += 64) {  // shift the rest down
+                        const uint32_t j = i + lane;
+                        uint32_t tq = 0, ta = 0, tb = 0;
+                        if (j + k < size) { tq = sq[j + k]; ta = sa[j + k]; tb = sb[j + k]; }
+                        __syncthreads();
+                        if (j + k < size) { sq[j] = tq; sa[j] = ta; sb[j] = tb; }
+                        __syncthreads();
+                    }
+                    size -= k;
+                    idle_seen = 0;  // once per read of the idle count
                 }
-                // no room: keep working on this stack
             }
         }
         if (size < 64 && nspill > 0) {  // bring back the spilled bottom block underneath
@@ -368,14 +449,90 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
             const uint32_t got = t0 < ntasks ? min(want, ntasks - t0) : 0;
             if (got < want) tasks_done = true;
             if (uint32_t(lane) < got) {
-                const uint3 e = tasks[ntasks - 1u - (t0 + lane)];  // backwards: largest tasks first
+                const uint32_t idx = lo + ntasks - 1u - (t0 + lane);  // backwards: largest tasks first
+                uint3 e;
+                if (tasks4) {
+                    const uint4 f = tasks4[idx];
+                    e = make_uint3(f.x, f.y, f.z);
+                } else {
+                    e = tasks[idx];
+                }
                 sq[size + lane] = e.x; sa[size + lane] = e.y; sb[size + lane] = e.z;
             }
             size += got;
             __syncthreads();
         }
         if (size == 0) {
-            if (tasks_done && nspill == 0) break;
+            if (!(tasks_done && nspill == 0)) continue;
+            if (share != 1u) break;
+            // Out of work.  Only the first POLLERS idle wavefronts wait
+            // (boundedly) for donations; the others leave at once: a thousand
+            // wavefronts polling one line would slow every atomic the busy
+            // ones issue.
+            unsigned int idle_before = 0;
+            if (lane == 0) idle_before = atomicAdd(&dq_ctl[2], 1u);
+            idle_before = __shfl(idle_before, 0, 64);
+            if (!poller && idle_before >= POLLERS) break;
+            poller = true;  // a poller keeps coming back here after each donation it works off
+            if (lane == 0) atomicAdd(&dq_ctl[3], 1u);  // a poller is waiting
+            bool leave = false;
+            for (uint32_t polls = 0;; ++polls) {
+                unsigned int base = 0, got = 0;
+                if (lane == 0) {
+                    const unsigned int t = min(coherent_read(&dq_ctl[0]), dq_cap);
+                    unsigned int h = t ? coherent_read(&dq_ctl[1]) : 0u;
+                    while (h < t) {  // each failed CAS means another wave claimed
+                        const unsigned int k = min(t - h, 64u);
+                        const unsigned int prev = atomicCAS(&dq_ctl[1], h, h + k);
+                        if (prev == h) {
+                            base = h;
+                            got = k;
+                            break;
+                        }
+                        h = prev;
+                    }
+                }
+                got = __shfl(got, 0, 64);
+                base = __shfl(base, 0, 64);
+                if (got) {
+                    bool ok = false;
+                    uint4 e = make_uint4(0, 0, 0, 0);
+                    if (uint32_t(lane) < got) {
+                        unsigned int* w = reinterpret_cast<unsigned int*>(dq + base + lane);
+                        for (uint32_t r = 0; r < READY_POLLS; ++r) {
+                            if (coherent_read(&w[3])) {
+                                ok = true;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        if (ok) e = make_uint4(coherent_read(&w[0]), coherent_read(&w[1]), coherent_read(&w[2]), 1u);
+                    }
+                    const unsigned long long mok = __ballot(ok);
+                    if (__ballot(uint32_t(lane) < got && !ok)) lost = true;
+                    if (ok) {
+                        const uint32_t pos = uint32_t(__popcll(mok & below));
+                        sq[pos] = e.x; sa[pos] = e.y; sb[pos] = e.z;
+                    }
+                    size = uint32_t(__popcll(mok));
+                    __syncthreads();
+                    if (lane == 0) {
+                        atomicSub(&dq_ctl[3], 1u);
+                        atomicSub(&dq_ctl[2], 1u);
+                    }
+                    break;
+                }
+                unsigned int idle_now = 0;
+                if (lane == 0) idle_now = coherent_read(&dq_ctl[2]);
+                idle_now = __shfl(idle_now, 0, 64);
+                if (idle_now >= gridDim.x || polls + 1 >= spin_polls) {  // everyone idle or gone, or waited long enough
+                    if (lane == 0) atomicSub(&dq_ctl[3], 1u);
+                    leave = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(64);
+            }
+            if (leave) break;
             continue;
         }
         const uint32_t p = min(size, pop_width);
@@ -445,8 +602,9 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     if (lane == 0) {
         atomicAdd(&ctr->sat, nsat);
         if (nleaf) atomicAdd(&ctr->leafpairs, nleaf);
-        atomicAdd(exited, 1u);
         if (overflow) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 4u);
+        if (lost) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 8u);
+        if (share == 2u) atomicAdd(&dq_ctl[2], 1u);
     }
 }
 
@@ -611,28 +769,19 @@ hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* o
                        lp_cap, tasks, ntasks, split_tris, ctr);
     return hipGetLastError();
 }
-hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
-                      unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
-                      uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
-                      unsigned int* exited, uint3* dump, unsigned int* ndump, uint32_t dump_cap, uint32_t dump_after,
-                      RunCounters* ctr, hipStream_t st) {
+hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,
+                      const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
+                      const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
+                      uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
+                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, RunCounters* ctr, hipStream_t st) {
+#define CD_DFS_ARGS tasks, tasks4, task_lo, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, \
+                    pop_width, dq, dq_ctl, dq_cap, share, spin_polls, ctr
     switch (cap_entries) {
-        case 512:
-            hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
-                               dump_after, ctr);
-            break;
-        case 2048:
-            hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
-                               dump_after, ctr);
-            break;
-        default:
-            hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, tasks, ntasks, head, qp, meshes, nodes, lp,
-                               lp_cap, spill, spill_cap, pop_width, task_cap, exited, dump, ndump, dump_cap,
-                               dump_after, ctr);
-            break;
+        case 512: hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
+        case 2048: hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
+        default: hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
     }
+#undef CD_DFS_ARGS
     return hipGetLastError();
 }
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,

@@ -161,7 +161,6 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
 }
 
-constexpr uint32_t MAX_PASSES = 8;
 cd_tuning effective_tuning(const cd_ctx* ctx) {
     cd_tuning t = ctx->tuning;
     if (t.split_levels == 0) t.split_levels = 8;
@@ -169,24 +168,22 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
     if (t.stack_entries != 512 && t.stack_entries != 2048) t.stack_entries = 1024;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
     if (t.waves_per_cu == 0) t.waves_per_cu = 8;
-    if (t.passes == 0) t.passes = 1;
-    if (t.passes > MAX_PASSES) t.passes = MAX_PASSES;
-    if (t.idle_pct == 0 || t.idle_pct > 100) t.idle_pct = 50;
+    if (t.spin_polls == 0) t.spin_polls = 4096;
     return t;
 }
 
 constexpr uint32_t SPILL_CAP = 16384;  // entries of each persistent wavefront's HBM spill area
-constexpr uint32_t DUMP_CAP = 1u << 20;  // node pairs handed from one traversal launch to the next
+constexpr uint32_t DQ_CAP = 1u << 20;  // node pairs the queue of the work-sharing traversal holds
 // W_SMALL layout: [0..L] split-level counts; [TASK_COUNT] tasks the split
-// levels hand to the persistent kernel; from PASS_BASE, four words per
-// traversal launch: task head, idle wavefronts, dump count, unused.
-constexpr uint32_t TASK_COUNT = 70;
-constexpr uint32_t PASS_BASE = 72;
-constexpr uint32_t SMALL_WORDS = PASS_BASE + 4 * MAX_PASSES;
+// levels hand to the persistent kernel; [HEAD0], [HEAD1] task heads of the
+// traversal launch and of the launch that takes unclaimed donations;
+// [DQ_CTL .. +3] donation queue: reserved, claimed, wavefronts idle or gone, pollers waiting.
+constexpr uint32_t TASK_COUNT = 70, HEAD0 = 72, HEAD1 = 73, DQ_CTL = 76;
+constexpr uint32_t SMALL_WORDS = 80;
 
 // Work buffers (d_work slots): batched path, then broad phase / frame.
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
-       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DUMP1,
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_UNUSED1,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
        W_RAYS, W_TASKS };
 static_assert(W_TASKS < 40, "work slots");
@@ -207,13 +204,15 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     QueryPre* qp = ctx->d_work[W_QPRE].as<QueryPre>();
     uint3* F[2] = {ctx->d_work[W_FRONT0].as<uint3>(), ctx->d_work[W_FRONT1].as<uint3>()};
     unsigned int* small = ctx->d_work[W_SMALL].as<unsigned int>();
-    uint3* D[2] = {ctx->d_work[W_DUMP0].as<uint3>(), ctx->d_work[W_DUMP1].as<uint3>()};
+    uint4* dq = ctx->d_work[W_DUMP0].as<uint4>();
     const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
     const NodeRec* dn = ctx->d_nodes.as<NodeRec>();
     RunCounters* ctr = dev_counters(ctx);
     const uint32_t grid = uint32_t(ctx->num_cu) * 8;
     HCK(ctx, launch_zero(ctr, st));
     HCK(ctx, launch_fill_u32(small, SMALL_WORDS, 0u, st));
+    if (t.work_sharing == 1)  // ready words of the donation queue start clear
+        HCK(ctx, launch_fill_u32(reinterpret_cast<unsigned int*>(dq), 4 * DQ_CAP, 0u, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dm, qp, st));
     HCK(ctx, launch_seed(nq, qp, F[0], &small[0], st));
@@ -224,18 +223,16 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
                                     ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT],
                                     l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
-    const uint32_t dump_after = std::max(1u, uint32_t(uint64_t(nwaves) * t.idle_pct / 100));
-    for (uint32_t p = 0; p < t.passes; ++p) {
-        unsigned int* pw = &small[PASS_BASE + 4 * p];
-        const bool first = p == 0, last = p + 1 == t.passes;
-        const uint3* tasks = first ? (L ? T : F[0]) : D[(p - 1) & 1];
-        const unsigned int* ntasks = first ? &small[L ? TASK_COUNT : 0] : &small[PASS_BASE + 4 * (p - 1) + 2];
-        const uint32_t task_cap = first ? cap32 : DUMP_CAP;
-        HCK(ctx, launch_dfs(int(t.stack_entries), tasks, ntasks, task_cap, &pw[0], qp, dm, dn,
-                            ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
-                            t.pop_width, nwaves, &pw[1], D[p & 1], &pw[2], DUMP_CAP,
-                            last ? 0xFFFFFFFFu : dump_after, ctr, st));
-    }
+    const uint32_t share = t.work_sharing;
+    HCK(ctx, launch_dfs(int(t.stack_entries), L ? T : F[0], nullptr, nullptr, &small[L ? TASK_COUNT : 0], cap32,
+                        &small[HEAD0], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
+                        ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP,
+                        share, t.spin_polls, ctr, st));
+    if (share)  // donations nobody claimed, [claimed, reserved), or the stacks handed off
+        HCK(ctx, launch_dfs(int(t.stack_entries), nullptr, dq, &small[DQ_CTL + 1], &small[DQ_CTL], DQ_CAP,
+                            &small[HEAD1], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
+                            ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL],
+                            DQ_CAP, 0u, t.spin_polls, ctr, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
     HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
                                  ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
@@ -253,10 +250,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
     HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
-    if (t.passes > 1) {
-        HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DUMP_CAP) * sizeof(uint3)));
-        HCK(ctx, ctx->d_work[W_DUMP1].ensure(size_t(DUMP_CAP) * sizeof(uint3)));
-    }
+    if (t.work_sharing) HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DQ_CAP) * sizeof(uint4)));
     RunCounters h{};
     for (int attempt = 0; attempt < 8; ++attempt) {
         const uint64_t fcap = frontier_cap(ctx, nq, L);
@@ -265,13 +259,13 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
-        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.passes, t.idle_pct, t.split_tris,
+        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls, t.split_tris,
                                      ctx->lp_cap,
                                      ctx->contact_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
-                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_DUMP1], &ctx->d_work[W_TASKS]})
+                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;
@@ -308,6 +302,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
             ctx->frontier_cap = 2 * fcap;
             retry = true;
         }
+        if (h.err & 8ull) return fail(ctx, CD_E_HIP, "batched traversal: a donated node pair was never published");
         if (h.err & ~2ull) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
         if (h.lpslots > ctx->lp_cap) {
             ctx->lp_cap = h.lpslots + h.lpslots / 4;
@@ -1105,8 +1100,7 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
     if (!ctx || !t) return CD_E_INVALID;
     if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
-    if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->passes > MAX_PASSES ||
-        t->idle_pct > 100)
+    if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 2)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;

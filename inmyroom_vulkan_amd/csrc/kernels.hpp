@@ -25,11 +25,11 @@ hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* o
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
                               RunCounters* ctr, uint32_t grid, hipStream_t st);
-hipError_t launch_dfs(int cap_entries, const uint3* tasks, const unsigned int* ntasks, uint32_t task_cap,
-                      unsigned int* head, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
-                      uint64_t lp_cap, uint3* spill, uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves,
-                      unsigned int* exited, uint3* dump, unsigned int* ndump, uint32_t dump_cap, uint32_t dump_after,
-                      RunCounters* ctr, hipStream_t st);
+hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,
+                      const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
+                      const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
+                      uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
+                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, RunCounters* ctr, hipStream_t st);
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
                                hipStream_t st);

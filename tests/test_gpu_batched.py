@@ -76,10 +76,11 @@ def test_batched_queries_uneven_subset(gpu, orc):
     dict(split_levels=4, stack_entries=2048, pop_width=16, waves_per_cu=4),
     dict(split_levels=12, stack_entries=1024, pop_width=64, waves_per_cu=8),
     dict(split_levels=6, stack_entries=512, pop_width=1, waves_per_cu=1),
-    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, passes=1),
-    # busy wavefronts hand their stacks on as soon as 1% of the grid is idle, every launch but the last
-    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, passes=8, idle_pct=1),
-    dict(split_levels=0, stack_entries=1024, pop_width=32, waves_per_cu=2, passes=5, idle_pct=90),
+    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, work_sharing=0),
+    # idle wavefronts take donated node pairs; a tiny spin budget leaves donations to the second launch
+    dict(split_levels=2, stack_entries=512, pop_width=64, waves_per_cu=4, work_sharing=1),
+    dict(split_levels=0, stack_entries=1024, pop_width=32, waves_per_cu=2, work_sharing=1, spin_polls=1),
+    dict(split_levels=10, stack_entries=512, pop_width=64, waves_per_cu=8, work_sharing=1, spin_polls=8),
     # deep, selective splitting: only pairs over split_tris triangles go on to the next level
     dict(split_levels=24, split_tris=64, stack_entries=512, pop_width=64, waves_per_cu=4),
     dict(split_levels=40, split_tris=1, stack_entries=512, pop_width=64, waves_per_cu=4),
