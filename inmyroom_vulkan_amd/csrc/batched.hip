@@ -608,6 +608,22 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     }
 }
 
+// Adds a per-lane count to a device counter with one atomic per block
+// (256 threads; every thread of the block calls it).  Same-address atomics
+// serialise in L2, so one per wavefront over a long stream costs more than
+// the arithmetic.
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* __restrict__ dst) {
+    __shared__ unsigned long long part[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 constexpr uint32_t CSTAGE = 64;  // contacts a wavefront stages in LDS before one atomic reserves their slots
 
 // Copy a wavefront's staged contacts to the output (16 words each).
@@ -623,10 +639,69 @@ __device__ __forceinline__ void flush_contacts(cd_contact* __restrict__ out, uin
     }
 }
 
+// Triangle pair of a test: A in its mesh's space, B moved into it by the
+// query's matrix; ga, gb are arena-global triangle slots.
+__device__ __forceinline__ void narrow_load(const QueryPre& Q, const float* __restrict__ tris, uint32_t ga, uint32_t gb,
+                                            f3& A0, f3& A1, f3& A2, f3& B0, f3& B1, f3& B2) {
+    M4 M;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) M.m[e] = Q.m[e];
+    const float* a = tris + size_t(ga) * 9;
+    const float* b = tris + size_t(gb) * 9;
+    A0 = mk(a[0], a[1], a[2]);
+    A1 = mk(a[3], a[4], a[5]);
+    A2 = mk(a[6], a[7], a[8]);
+    B0 = xform_point(M, mk(b[0], b[1], b[2]));
+    B1 = xform_point(M, mk(b[3], b[4], b[5]));
+    B2 = xform_point(M, mk(b[6], b[7], b[8]));
+}
+
+// Stages the contacts of the lanes with `hit` set in the wavefront's LDS
+// area, flushing it first when they do not fit.  Every lane calls it.
+__device__ __forceinline__ void stage_hits(bool hit, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2, uint32_t ds, uint32_t q,
+                                           uint32_t ta, uint32_t tb, cd_contact* stage, uint32_t& staged,
+                                           cd_contact* __restrict__ out, uint64_t cap, RunCounters* __restrict__ ctr,
+                                           int lane, unsigned long long below) {
+    const unsigned long long m = __ballot(hit);
+    if (!m) return;
+    const uint32_t nh = uint32_t(__popcll(m));
+    if (staged + nh > CSTAGE) {
+        __threadfence_block();  // this wavefront's LDS writes are complete before other lanes copy them
+        flush_contacts(out, cap, stage, staged, ctr, lane);
+        staged = 0;
+        __threadfence_block();
+    }
+    if (hit) write_contact(stage + staged + uint32_t(__popcll(m & below)), A0, A1, A2, B0, B1, B2, ds, q, ta, tb);
+    staged += nh;
+}
+
+// Appends the tests of the lanes with `defer` set to a rerun list (one
+// atomic per wavefront).  The count keeps growing past the capacity, so the
+// host sees how much room the run needed.  Every lane calls it.
+__device__ __forceinline__ void defer_tests(bool defer, uint4 e, uint4* __restrict__ list, uint64_t list_cap,
+                                            unsigned long long* __restrict__ count, int lane,
+                                            unsigned long long below) {
+    const unsigned long long m = __ballot(defer);
+    if (!m) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, 0, 64);
+    if (defer) {
+        const uint64_t s = base + uint64_t(__popcll(m & below));
+        if (s < list_cap) list[s] = e;
+    }
+}
+
+// Triangle tests of the leaf pairs, float32 predicates only: a test that a
+// float32 predicate cannot decide goes to the rerun list (entries {query,
+// global A slot, global B slot}), which k_narrow_rerun<1> and <2> finish in
+// float64 and exactly.  Keeping the slow stages out of this kernel keeps its
+// registers and scratch small.
 __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ lps,
                                                       const unsigned long long* __restrict__ nlp_p, uint64_t lp_cap,
                                                       const QueryPre* __restrict__ qp, const float* __restrict__ tris,
                                                       cd_contact* __restrict__ out, uint64_t cap,
+                                                      uint4* __restrict__ defer_list, uint64_t defer_cap,
                                                       RunCounters* __restrict__ ctr) {
     __shared__ cd_contact s_con[4][CSTAGE];
     const uint64_t nlp = min(*nlp_p, lp_cap);
@@ -636,7 +711,7 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-    unsigned long long ntt = 0, nfp64 = 0, nexact = 0;
+    unsigned long long ntt = 0;
     for (uint64_t chunk = wave; chunk * 64 < nlp; chunk += nwaves) {
         const uint64_t li = chunk * 64 + lane;
         uint4 lp = make_uint4(0, 0, 0, 0);
@@ -663,92 +738,227 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
             }
             const uint32_t kq = __shfl(lp.x, k, 64), ka = __shfl(lp.y, k, 64), kb = __shfl(lp.z, k, 64);
             const uint32_t kw = __shfl(lp.w, k, 64), ko = __shfl(excl, k, 64);
-            bool hit = false;
+            bool hit = false, defer = false;
             f3 A0, A1, A2, B0, B1, B2;
-            uint32_t ds = 0, ta = 0, tb = 0;
+            uint32_t ds = 0, ga = 0, gb = 0, ta = 0, tb = 0;
             if (active) {
                 ++ntt;
                 const uint32_t kcB = kw >> 16;
                 const uint32_t r = t - ko;
-                const uint32_t i = r / kcB, j = r % kcB;
+                ga = ka + r / kcB;  // ka, kb: arena-global triangle offsets
+                gb = kb + r % kcB;
                 const QueryPre& Q = qp[kq];
-                M4 M;
-#pragma unroll
-                for (int e = 0; e < 16; ++e) M.m[e] = Q.m[e];
-                const float* a = tris + (size_t(ka) + i) * 9;  // ka, kb: arena-global triangle offsets
-                const float* b = tris + (size_t(kb) + j) * 9;
-                ta = ka + i - Q.tri_base_a;  // mesh-local slots for the contact
-                tb = kb + j - Q.tri_base_b;
-                A0 = mk(a[0], a[1], a[2]);
-                A1 = mk(a[3], a[4], a[5]);
-                A2 = mk(a[6], a[7], a[8]);
-                B0 = xform_point(M, mk(b[0], b[1], b[2]));
-                B1 = xform_point(M, mk(b[3], b[4], b[5]));
-                B2 = xform_point(M, mk(b[6], b[7], b[8]));
+                ta = ga - Q.tri_base_a;  // mesh-local slots for the contact
+                tb = gb - Q.tri_base_b;
+                narrow_load(Q, tris, ga, gb, A0, A1, A2, B0, B1, B2);
                 PredStats st;
-                hit = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st) == 1;
-                nfp64 += st.fp64;
-                nexact += st.exact;
+                const int res = tri_tri_exact<0>(A0, A1, A2, B0, B1, B2, ds, st);
+                hit = res == 1;
+                defer = res == DEFERRED;
             }
-            const unsigned long long m = __ballot(hit);
-            if (m) {
-                const uint32_t nh = uint32_t(__popcll(m));
-                if (staged + nh > CSTAGE) {
-                    __threadfence_block();  // this wavefront's LDS writes are complete before other lanes copy them
-                    flush_contacts(out, cap, stage, staged, ctr, lane);
-                    staged = 0;
-                    __threadfence_block();
-                }
-                if (hit) write_contact(stage + staged + uint32_t(__popcll(m & below)), A0, A1, A2, B0, B1, B2, ds, kq, ta, tb);
-                staged += nh;
-            }
+            defer_tests(defer, make_uint4(kq, ga, gb, 0), defer_list, defer_cap, &ctr->deferred, lane, below);
+            stage_hits(hit, A0, A1, A2, B0, B1, B2, ds, kq, ta, tb, stage, staged, out, cap, ctr, lane, below);
         }
     }
     if (staged) {
         __threadfence_block();
         flush_contacts(out, cap, stage, staged, ctr, lane);
     }
-    ntt = wave_sum(ntt);
-    nfp64 = wave_sum(nfp64);
-    nexact = wave_sum(nexact);
-    if (lane == 0) {
-        if (ntt) atomicAdd(&ctr->tritests, ntt);
-        if (nfp64) atomicAdd(&ctr->fp64, nfp64);
-        if (nexact) atomicAdd(&ctr->exact, nexact);
+    block_add(ntt, &ctr->tritests);
+}
+
+// Rerun of deferred triangle tests at precision level L (1: float64, whose
+// undecided tests go on to `next`; 2: exact).  Wavefront-stride over the
+// list, whose length is read on the device.
+template <int L>
+__global__ __launch_bounds__(256) void k_narrow_rerun(const uint4* __restrict__ list,
+                                                      const unsigned long long* __restrict__ nlist_p,
+                                                      uint64_t list_cap, const QueryPre* __restrict__ qp,
+                                                      const float* __restrict__ tris, cd_contact* __restrict__ out,
+                                                      uint64_t cap, uint4* __restrict__ next,
+                                                      unsigned long long* __restrict__ nnext,
+                                                      RunCounters* __restrict__ ctr) {
+    __shared__ cd_contact s_con[4][CSTAGE];
+    const uint64_t n = min(*nlist_p, list_cap);
+    const int lane = threadIdx.x & 63;
+    cd_contact* stage = s_con[threadIdx.x >> 6];
+    uint32_t staged = 0;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    unsigned long long nfp64 = 0, nexact = 0;
+    for (uint64_t chunk = wave; chunk * 64 < n; chunk += nwaves) {
+        const uint64_t li = chunk * 64 + lane;
+        bool hit = false, defer = false;
+        f3 A0, A1, A2, B0, B1, B2;
+        uint32_t ds = 0, ta = 0, tb = 0;
+        uint4 e = make_uint4(0, 0, 0, 0);
+        if (li < n) {
+            e = list[li];
+            const QueryPre& Q = qp[e.x];
+            ta = e.y - Q.tri_base_a;
+            tb = e.z - Q.tri_base_b;
+            narrow_load(Q, tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
+            PredStats st;
+            const int res = tri_tri_exact<L>(A0, A1, A2, B0, B1, B2, ds, st);
+            hit = res == 1;
+            defer = res == DEFERRED;
+            if (!defer) {
+                nfp64 += st.fp64;
+                nexact += st.exact;
+            }
+        }
+        if (L < 2) defer_tests(defer, e, next, list_cap, nnext, lane, below);
+        stage_hits(hit, A0, A1, A2, B0, B1, B2, ds, e.x, ta, tb, stage, staged, out, cap, ctr, lane, below);
+    }
+    if (staged) {
+        __threadfence_block();
+        flush_contacts(out, cap, stage, staged, ctr, lane);
+    }
+    block_add(nfp64, &ctr->fp64);
+    block_add(nexact, &ctr->exact);
+}
+
+// Exact-semantics stream of triangle pairs: flag 1 intersect (non-coplanar),
+// 2 coplanar, 0 disjoint.
+//
+// A cascade of three launches, one precision level each, so that every
+// wavefront runs one kind of arithmetic: the float32 pass decides almost
+// every pair and appends the few it cannot decide to a list; the float64
+// pass reruns those, appending what it cannot decide to a second list; the
+// last pass reruns that remainder with the exact expansion.  A pair that
+// needs float64 in one predicate would otherwise hold its whole wavefront on
+// the slow path.  Each rerun repeats the pair from the start, so a pair's
+// counters come from the pass that decides it and total the same as a
+// single full-precision pass.
+//
+// The passes are persistent (grid-stride, block-uniform loops).  A block
+// stages the pairs it defers in LDS and moves them to the list BSTAGE at a
+// time with one atomic, so lists are dense and a pass makes few atomics.
+constexpr uint32_t BSTAGE = 2048;
+
+struct BlockStage {
+    uint32_t buf[BSTAGE];
+    uint32_t n;
+    unsigned long long base;
+};
+
+__device__ __forceinline__ void stage_flush(BlockStage& bs, uint32_t* __restrict__ list, uint64_t cap,
+                                            unsigned long long* __restrict__ count) {
+    const uint32_t c = bs.n;  // every thread reads it after a barrier
+    if (c == 0) return;
+    if (threadIdx.x == 0) bs.base = atomicAdd(count, static_cast<unsigned long long>(c));
+    __syncthreads();
+    const unsigned long long b = bs.base;
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x)
+        if (b + i < cap) list[b + i] = bs.buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) bs.n = 0;
+    __syncthreads();
+}
+
+// Stages v for the threads with `on` set; every thread of the block calls
+// it once per loop iteration (at most blockDim.x entries per call).
+__device__ __forceinline__ void stage_push(bool on, uint32_t v, BlockStage& bs, uint32_t* __restrict__ list,
+                                           uint64_t cap, unsigned long long* __restrict__ count, int lane,
+                                           unsigned long long below) {
+    const unsigned long long m = __ballot(on);
+    uint32_t off = 0;
+    if (lane == 0 && m) off = atomicAdd(&bs.n, uint32_t(__popcll(m)));
+    off = __shfl(off, 0, 64);
+    if (on) bs.buf[off + uint32_t(__popcll(m & below))] = v;
+    __syncthreads();
+    const uint32_t c = bs.n;
+    __syncthreads();  // every thread holds c before any wavefront adds to bs.n again
+    if (c > BSTAGE - blockDim.x) stage_flush(bs, list, cap, count);
+}
+
+__device__ __forceinline__ void tt_load(const float* __restrict__ a9, const float* __restrict__ b9, uint64_t i, f3& A0,
+                                        f3& A1, f3& A2, f3& B0, f3& B1, f3& B2) {
+    const float* a = a9 + i * 9;
+    const float* b = b9 + i * 9;
+    A0 = mk(a[0], a[1], a[2]); A1 = mk(a[3], a[4], a[5]); A2 = mk(a[6], a[7], a[8]);
+    B0 = mk(b[0], b[1], b[2]); B1 = mk(b[3], b[4], b[5]); B2 = mk(b[6], b[7], b[8]);
+}
+
+__device__ __forceinline__ void tt_store(uint64_t i, int res, uint32_t ds, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2,
+                                         int32_t* __restrict__ flags, float* __restrict__ seg6) {
+    flags[i] = res == 1 ? 1 : (res == 2 ? 2 : 0);
+    if (seg6) {
+        f3 s = mk(0.f, 0.f, 0.f), t = mk(0.f, 0.f, 0.f);
+        if (res == 1) exact_segment(A0, A1, A2, B0, B1, B2, ds, s, t);
+        float* o = seg6 + i * 6;
+        o[0] = s.x; o[1] = s.y; o[2] = s.z;
+        o[3] = t.x; o[4] = t.y; o[5] = t.z;
     }
 }
 
-// Exact-semantics stream of triangle pairs: bit0 intersect (non-coplanar), bit1 coplanar.
-__global__ __launch_bounds__(256) void k_tritri_exact(const float* __restrict__ a9, const float* __restrict__ b9,
-                                                      uint64_t n, int32_t* __restrict__ flags,
-                                                      float* __restrict__ seg6, RunCounters* __restrict__ ctr) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    unsigned long long nfp64 = 0, nexact = 0;
-    if (i < n) {
-        const float* a = a9 + i * 9;
-        const float* b = b9 + i * 9;
-        const f3 A0 = mk(a[0], a[1], a[2]), A1 = mk(a[3], a[4], a[5]), A2 = mk(a[6], a[7], a[8]);
-        const f3 B0 = mk(b[0], b[1], b[2]), B1 = mk(b[3], b[4], b[5]), B2 = mk(b[6], b[7], b[8]);
-        PredStats st;
-        uint32_t ds = 0;
-        const int res = tri_tri_exact(A0, A1, A2, B0, B1, B2, ds, st);
-        nfp64 = st.fp64;
-        nexact = st.exact;
-        flags[i] = res == 1 ? 1 : (res == 2 ? 2 : 0);
-        if (seg6) {
-            f3 s = mk(0.f, 0.f, 0.f), t = mk(0.f, 0.f, 0.f);
-            if (res == 1) exact_segment(A0, A1, A2, B0, B1, B2, ds, s, t);
-            float* o = seg6 + i * 6;
-            o[0] = s.x; o[1] = s.y; o[2] = s.z;
-            o[3] = t.x; o[4] = t.y; o[5] = t.z;
+__global__ __launch_bounds__(256) void k_tritri_f32(const float* __restrict__ a9, const float* __restrict__ b9,
+                                                    uint64_t n, int32_t* __restrict__ flags,
+                                                    float* __restrict__ seg6, uint32_t* __restrict__ list,
+                                                    unsigned long long* __restrict__ nlist) {
+    __shared__ BlockStage bs;
+    if (threadIdx.x == 0) bs.n = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < n; base += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t i = base + threadIdx.x;
+        bool defer = false;
+        if (i < n) {
+            f3 A0, A1, A2, B0, B1, B2;
+            tt_load(a9, b9, i, A0, A1, A2, B0, B1, B2);
+            PredStats st;
+            uint32_t ds = 0;
+            const int res = tri_tri_exact<0>(A0, A1, A2, B0, B1, B2, ds, st);
+            defer = res == DEFERRED;
+            if (!defer) tt_store(i, res, ds, A0, A1, A2, B0, B1, B2, flags, seg6);
         }
+        stage_push(defer, uint32_t(i), bs, list, n, nlist, lane, below);
     }
-    nfp64 = wave_sum(nfp64);
-    nexact = wave_sum(nexact);
-    if ((threadIdx.x & 63) == 0) {
-        if (nfp64) atomicAdd(&ctr->fp64, nfp64);
-        if (nexact) atomicAdd(&ctr->exact, nexact);
+    stage_flush(bs, list, n, nlist);
+}
+
+// Reruns of listed pairs at precision level L (grid-stride; the list length
+// is read on the device).
+template <int L>
+__global__ __launch_bounds__(256) void k_tritri_rerun(const float* __restrict__ a9, const float* __restrict__ b9,
+                                                      const uint32_t* __restrict__ list,
+                                                      const unsigned long long* __restrict__ nlist,
+                                                      int32_t* __restrict__ flags, float* __restrict__ seg6,
+                                                      uint64_t n, uint32_t* __restrict__ next,
+                                                      unsigned long long* __restrict__ nnext,
+                                                      RunCounters* __restrict__ ctr) {
+    __shared__ BlockStage bs;
+    if (L < 2 && threadIdx.x == 0) bs.n = 0;
+    __syncthreads();
+    const uint64_t count = min(static_cast<uint64_t>(*nlist), n);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long nfp64 = 0, nexact = 0;
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x; base < count; base += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t k = base + threadIdx.x;
+        bool defer = false;
+        uint32_t i = 0;
+        if (k < count) {
+            i = list[k];
+            f3 A0, A1, A2, B0, B1, B2;
+            tt_load(a9, b9, i, A0, A1, A2, B0, B1, B2);
+            PredStats st;
+            uint32_t ds = 0;
+            const int res = tri_tri_exact<L>(A0, A1, A2, B0, B1, B2, ds, st);
+            defer = res == DEFERRED;
+            if (!defer) {
+                tt_store(i, res, ds, A0, A1, A2, B0, B1, B2, flags, seg6);
+                nfp64 += st.fp64;
+                nexact += st.exact;
+            }
+        }
+        if (L < 2) stage_push(defer, i, bs, next, n, nnext, lane, below);
     }
+    if (L < 2) stage_flush(bs, next, n, nnext);
+    block_add(nfp64, &ctr->fp64);
+    block_add(nexact, &ctr->exact);
 }
 
 // ---- host launchers -------------------------------------------------------
@@ -785,15 +995,31 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
     return hipGetLastError();
 }
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
-                               const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
-                               hipStream_t st) {
-    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, ctr);
+                               const float* tris, cd_contact* out, uint64_t cap, uint4* defer, uint64_t defer_cap,
+                               RunCounters* ctr, uint32_t grid, uint32_t rerun_grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, defer,
+                       defer_cap, ctr);
+    hipLaunchKernelGGL(k_narrow_rerun<1>, dim3(rerun_grid), dim3(256), 0, st, defer, &ctr->deferred, defer_cap, qp,
+                       tris, out, cap, defer + defer_cap, &ctr->deferred2, ctr);
+    hipLaunchKernelGGL(k_narrow_rerun<2>, dim3(rerun_grid), dim3(256), 0, st, defer + defer_cap, &ctr->deferred2,
+                       defer_cap, qp, tris, out, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
     return hipGetLastError();
 }
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
-                               RunCounters* ctr, hipStream_t st) {
+                               uint32_t* lists, uint32_t grid, RunCounters* ctr, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_tritri_exact, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, a9, b9, n, flags, seg6, ctr);
+    // lists: words 0-3 the two list lengths (64-bit, zeroed here), the
+    // float32 pass's list (n entries) at word 64, the float64 pass's after it
+    unsigned long long* nl = reinterpret_cast<unsigned long long*>(lists);
+    uint32_t* l1 = lists + 64;
+    uint32_t* l2 = lists + 64 + n;
+    hipError_t e = launch_fill_u32(lists, 4, 0, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tritri_f32, dim3(grid), dim3(256), 0, st, a9, b9, n, flags, seg6, l1, nl);
+    hipLaunchKernelGGL(k_tritri_rerun<1>, dim3(grid), dim3(256), 0, st, a9, b9, l1, nl, flags, seg6, n, l2, nl + 1,
+                       ctr);
+    hipLaunchKernelGGL(k_tritri_rerun<2>, dim3(grid), dim3(256), 0, st, a9, b9, l2, nl + 1, flags, seg6, n,
+                       (uint32_t*)nullptr, (unsigned long long*)nullptr, ctr);
     return hipGetLastError();
 }
 

@@ -183,7 +183,7 @@ constexpr uint32_t SMALL_WORDS = 80;
 
 // Work buffers (d_work slots): batched path, then broad phase / frame.
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
-       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_UNUSED1,
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
        W_RAYS, W_TASKS };
 static_assert(W_TASKS < 40, "work slots");
@@ -235,7 +235,8 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
                             DQ_CAP, 0u, t.spin_polls, ctr, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
     HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
-                                 ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
+                                 ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctx->d_work[W_DEFER].as<uint4>(),
+                                 ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu), st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
     return CD_OK;
 }
@@ -247,6 +248,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
     if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
     if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
+    if (ctx->defer_cap == 0) ctx->defer_cap = 1u << 16;
     HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
     HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
@@ -259,13 +261,14 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
-        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls, t.split_tris,
-                                     ctx->lp_cap,
-                                     ctx->contact_cap, uint64_t(ctx->leaf_max), fcap};
+        HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
+        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
+                                     t.split_tris, ctx->lp_cap, ctx->contact_cap, ctx->defer_cap,
+                                     uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
-                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS]})
+                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;
@@ -310,6 +313,11 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         }
         if (h.contacts > ctx->contact_cap) {
             ctx->contact_cap = h.contacts + h.contacts / 4;
+            retry = true;
+        }
+        if (h.deferred > ctx->defer_cap || h.deferred2 > ctx->defer_cap) {  // rerun lists ran out of room
+            const uint64_t need = std::max(h.deferred, h.deferred2);
+            ctx->defer_cap = need + need / 4;
             retry = true;
         }
         if (retry) continue;
@@ -502,7 +510,7 @@ int cd_destroy(cd_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_perm, &ctx->d_queries, &ctx->d_leafpairs,
                       &ctx->d_contacts, &ctx->d_counters, &ctx->d_tt_a, &ctx->d_tt_b, &ctx->d_tt_flags, &ctx->d_tt_seg,
-                      &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
+                      &ctx->d_tt_list, &ctx->d_bind, &ctx->d_joints, &ctx->d_weights, &ctx->d_vidx, &ctx->d_skinpos, &ctx->d_bones,
                       &ctx->d_skinrec, &ctx->d_refit_items, &ctx->d_refit_bits, &ctx->d_refit_scratch,
                       &ctx->d_refit_ranges, &ctx->d_refit_plan, &ctx->d_refit_partial,
                       &ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm})
@@ -618,6 +626,11 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
     HCK(ctx, ctx->d_tt_b.ensure(n * 9 * sizeof(float) + 4));
     HCK(ctx, ctx->d_tt_flags.ensure(n * sizeof(int32_t) + 4));
     HCK(ctx, ctx->d_tt_seg.ensure(n * 6 * sizeof(float) + 4));
+    const uint32_t tt_grid = uint32_t(ctx->num_cu) * 8;
+    if (mode == CD_MODE_BATCHED) {
+        if (n >= 0xFFFFFFFFull) return fail(ctx, CD_E_INVALID, "2^32 - 1 or more pairs in one exact batch");
+        HCK(ctx, ctx->d_tt_list.ensure((64 + 2 * n) * sizeof(uint32_t)));
+    }
     if (n) {
         HCK(ctx, hipMemcpyAsync(ctx->d_tt_a.p, a9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
         HCK(ctx, hipMemcpyAsync(ctx->d_tt_b.p, b9, n * 9 * sizeof(float), hipMemcpyHostToDevice, st));
@@ -629,13 +642,19 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
                                      seg6 ? ctx->d_tt_seg.as<float>() : nullptr, st));
     else
         HCK(ctx, launch_tritri_exact(ctx->d_tt_a.as<float>(), ctx->d_tt_b.as<float>(), n, ctx->d_tt_flags.as<int32_t>(),
-                                     seg6 ? ctx->d_tt_seg.as<float>() : nullptr, dev_counters(ctx), st));
+                                     seg6 ? ctx->d_tt_seg.as<float>() : nullptr, ctx->d_tt_list.as<uint32_t>(),
+                                     tt_grid, dev_counters(ctx), st));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
     if (n) {
         HCK(ctx, hipMemcpyAsync(flags, ctx->d_tt_flags.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
         if (seg6) HCK(ctx, hipMemcpyAsync(seg6, ctx->d_tt_seg.p, n * 6 * sizeof(float), hipMemcpyDeviceToHost, st));
     }
+    unsigned long long lens[2] = {0, 0};
+    if (mode == CD_MODE_BATCHED && n)
+        HCK(ctx, hipMemcpyAsync(lens, ctx->d_tt_list.p, sizeof lens, hipMemcpyDeviceToHost, st));
     HCK(ctx, hipStreamSynchronize(st));
+    if (std::max(lens[0], lens[1]) > n)  // a pass defers each pair at most once: unreachable
+        return fail(ctx, CD_E_HIP, "exact stream: rerun list overflow");
     float ms = 0.f;
     HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     RunCounters h{};
@@ -648,7 +667,7 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
     ctx->timings = cd_timings{};
     ctx->timings.total_ms = ms;
     ctx->timings.narrow_ms = ms;
-    ctx->timings.launches = 1;
+    ctx->timings.launches = mode == CD_MODE_PLAIN ? 1 : 4;
     return CD_OK;
 }
 

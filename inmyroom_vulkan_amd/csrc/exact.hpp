@@ -9,12 +9,15 @@
 // Stage 3: exact — the determinant expanded over the original coordinates is
 //          a sum of 24 products of three floats; each product is two doubles
 //          exactly (the float*float product is exact in double, the third
-//          factor goes through an FMA-based two-product), and the 48 doubles
-//          are summed exactly as a non-overlapping expansion with zero
-//          elimination (Shewchuk's GROW-EXPANSION).  The sign of the sum is
-//          the sign of its most significant component.
+//          factor goes through an FMA-based two-product), and the sign of the
+//          48 doubles' exact sum is found by certified error-free vector sums
+//          in registers, backed by a non-overlapping expansion with zero
+//          elimination (Shewchuk's GROW-EXPANSION).
 // Only stage 1 runs for almost every predicate; FP64 is confined to the
-// predicates whose float32 value is inside its error bound.
+// predicates whose float32 value is inside its error bound.  Streaming
+// kernels run the stages as separate passes (orient_at's levels): a test
+// that stage 1 cannot decide is deferred to a float64 pass and from there
+// to an exact pass, so no wavefront mixes the three kinds of arithmetic.
 #pragma once
 #pragma clang fp contract(off)
 
@@ -36,11 +39,45 @@ __device__ __forceinline__ void two_sum(double a, double b, double& x, double& y
     y = ar + br;
 }
 
-// Sum of +-u_i*v_j*w_k terms, exactly.
+// The determinant over the original coordinates as 48 doubles whose exact
+// sum it is: [a-c, b-c, d-c] = [a,b,d] - [a,b,c] - [a,c,d] + [b,c,d], each
+// triple product six terms +-u_i*v_j*w_k, each term the double product
+// u_i*v_j (exact: two floats) times w_k split into its rounded value and
+// the FMA remainder.  Every term is a multiple of 2^-447 (three factors
+// that are multiples of 2^-149) and below 2^384, so nothing underflows or
+// overflows here or in any exact sum of them.
+__device__ __forceinline__ void det_terms(f3 a, f3 b, f3 c, f3 d, double (&t)[48]) {
+    int n = 0;
+    auto triple = [&](f3 u, f3 v, f3 w, double sg) {
+        // u . (v x w) = ux(vy wz - vz wy) + uy(vz wx - vx wz) + uz(vx wy - vy wx)
+        const double ux = u.x, uy = u.y, uz = u.z, vx = v.x, vy = v.y, vz = v.z, wx = w.x, wy = w.y, wz = w.z;
+        const double q[6][3] = {{ux * vy, wz, 1}, {ux * vz, wy, -1}, {uy * vz, wx, 1},
+                                {uy * vx, wz, -1}, {uz * vx, wy, 1}, {uz * vy, wx, -1}};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double p = q[k][0] * q[k][1];
+            const double err = fma(q[k][0], q[k][1], -p);
+            const double s = sg * q[k][2];
+            t[n++] = s * p;
+            t[n++] = s * err;
+        }
+    };
+    triple(a, b, d, 1.0);
+    triple(a, b, c, -1.0);
+    triple(a, c, d, -1.0);
+    triple(b, c, d, 1.0);
+}
+
+// Sign of the exact sum by growing a non-overlapping expansion with zero
+// elimination (Shewchuk's GROW-EXPANSION); its array lives in scratch
+// memory, so it only backs up orient_exact.
 __device__ __noinline__ int orient_exact_slow(f3 a, f3 b, f3 c, f3 d) {
+    double t[48];
+    det_terms(a, b, c, d, t);
     double e[48];
     int n = 0;
-    auto grow = [&](double q) {
+    for (int k = 0; k < 48; ++k) {
+        double q = t[k];
         int m = 0;
         for (int i = 0; i < n; ++i) {
             double s, h;
@@ -50,33 +87,46 @@ __device__ __noinline__ int orient_exact_slow(f3 a, f3 b, f3 c, f3 d) {
         }
         if (q != 0.0) e[m++] = q;
         n = m;
-    };
-    auto triple = [&](f3 u, f3 v, f3 w, double sg) {
-        // u . (v x w) = ux(vy wz - vz wy) + uy(vz wx - vx wz) + uz(vx wy - vy wx)
-        const double ux = u.x, uy = u.y, uz = u.z, vx = v.x, vy = v.y, vz = v.z, wx = w.x, wy = w.y, wz = w.z;
-        const double t[6][3] = {{ux * vy, wz, 1}, {ux * vz, wy, -1}, {uy * vz, wx, 1},
-                                {uy * vx, wz, -1}, {uz * vx, wy, 1}, {uz * vy, wx, -1}};
-        for (int k = 0; k < 6; ++k) {
-            double p = t[k][0] * t[k][1];
-            double err = fma(t[k][0], t[k][1], -p);
-            double s = sg * t[k][2];
-            grow(s * p);
-            grow(s * err);
-        }
-    };
-    // [a-c, b-c, d-c] = [a,b,d] - [a,b,c] - [a,c,d] + [b,c,d]
-    triple(a, b, d, 1.0);
-    triple(a, b, c, -1.0);
-    triple(a, c, d, -1.0);
-    triple(b, c, d, 1.0);
+    }
     for (int i = n - 1; i >= 0; --i)
         if (e[i] != 0.0) return e[i] > 0.0 ? 1 : -1;
     return 0;
 }
 
-// Stages 2 and 3, out of line so the float32 fast path stays small.
-// Returns (sign + 1) | 4 when the exact stage was needed.
-__device__ __noinline__ int orient_slow(f3 a, f3 b, f3 c, f3 d) {
+// Exact sign, in registers: iterated error-free vector sums (VecSum,
+// Ogita, Rump & Oishi 2005, "Accurate sum and dot product", SIAM J. Sci.
+// Comput. 26(6), Algorithm 4.3).  A pass replaces p[i-1], p[i] by the sum
+// and the rounding error of the pair, walking up the array, so the terms
+// keep their exact sum and p[47] becomes its floating-point value.  Once
+// |p[47]| exceeds the magnitude of the other 47 terms together, their sum
+// cannot change its sign: the float sum r of their magnitudes is within
+// gamma_46 < 5.2e-15 of the true one, so |p[47]| > r (1 + 1e-13) certifies
+// it; r == 0 leaves p[47] as the exact sum.  Degenerate inputs (shared
+// vertices, points on edges, 1-ulp offsets) certify within three passes;
+// a case still open after eight takes the expansion.
+__device__ __forceinline__ int orient_exact(f3 a, f3 b, f3 c, f3 d) {
+    double p[48];
+    det_terms(a, b, c, d, p);
+#pragma unroll 1
+    for (int pass = 0; pass < 8; ++pass) {
+#pragma unroll
+        for (int i = 1; i < 48; ++i) {
+            double s, e;
+            two_sum(p[i], p[i - 1], s, e);
+            p[i] = s;
+            p[i - 1] = e;
+        }
+        double r = 0.0;
+#pragma unroll
+        for (int i = 0; i < 47; ++i) r += fabs(p[i]);
+        if (r == 0.0 || fabs(p[47]) > r * (1.0 + 1e-13)) return p[47] > 0.0 ? 1 : (p[47] < 0.0 ? -1 : 0);
+    }
+    return orient_exact_slow(a, b, c, d);
+}
+
+// Float64 stage: +1 / -1, or UNDECIDED when |det| is inside its bound.
+constexpr int UNDECIDED = 2;
+__device__ __forceinline__ int orient_f64(f3 a, f3 b, f3 c, f3 d) {
     const double ax = double(a.x) - c.x, ay = double(a.y) - c.y, az = double(a.z) - c.z;
     const double bx = double(b.x) - c.x, by = double(b.y) - c.y, bz = double(b.z) - c.z;
     const double dx = double(d.x) - c.x, dy = double(d.y) - c.y, dz = double(d.z) - c.z;
@@ -85,12 +135,13 @@ __device__ __noinline__ int orient_slow(f3 a, f3 b, f3 c, f3 d) {
     const double perm = (fabs(p0) + fabs(p1)) * fabs(dx) + (fabs(p2) + fabs(p3)) * fabs(dy) +
                         (fabs(p4) + fabs(p5)) * fabs(dz);
     const double bound = 1.7763568394002505e-15 * perm;  // 16 * 2^-53
-    if (det > bound) return 2;
-    if (det < -bound) return 0;
-    return (orient_exact_slow(a, b, c, d) + 1) | 4;
+    if (det > bound) return 1;
+    if (det < -bound) return -1;
+    return UNDECIDED;
 }
 
-__device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats& st) {
+// Float32 stage: +1 / -1, or UNDECIDED.
+__device__ __forceinline__ int orient_f32(f3 a, f3 b, f3 c, f3 d) {
     const float ax = a.x - c.x, ay = a.y - c.y, az = a.z - c.z;
     const float bx = b.x - c.x, by = b.y - c.y, bz = b.z - c.z;
     const float dx = d.x - c.x, dy = d.y - c.y, dz = d.z - c.z;
@@ -101,10 +152,37 @@ __device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats
     const float bound = 9.5367431640625e-07f * perm + 1e-30f;  // 16 * 2^-24
     if (det > bound) return 1;
     if (det < -bound) return -1;
+    return UNDECIDED;
+}
+
+// Stages 2 and 3 out of line, for kernels where the fallback is rare and
+// the float32 path should stay small.  Returns (sign + 1) | 4 when the exact
+// stage was needed.
+__device__ __noinline__ int orient_slow(f3 a, f3 b, f3 c, f3 d) {
+    const int r = orient_f64(a, b, c, d);
+    if (r != UNDECIDED) return r + 1;
+    return (orient_exact(a, b, c, d) + 1) | 4;
+}
+
+// Orientation at a precision level:
+//   0  float32 only (UNDECIDED when inside the float32 bound),
+//   1  float32, then float64 inline (UNDECIDED when inside both bounds),
+//   2  float32, float64, exact: always decided.
+// st.fp64 counts predicates that left the float32 stage, st.exact those that
+// reached the exact stage.
+template <int L>
+__device__ __forceinline__ int orient_at(f3 a, f3 b, f3 c, f3 d, PredStats& st) {
+    const int r = orient_f32(a, b, c, d);
+    if (L == 0 || r != UNDECIDED) return r;
     ++st.fp64;
-    const int r = orient_slow(a, b, c, d);
-    if (r & 4) ++st.exact;
-    return (r & 3) - 1;
+    if (L == 1) return orient_f64(a, b, c, d);
+    const int s = orient_slow(a, b, c, d);
+    if (s & 4) ++st.exact;
+    return (s & 3) - 1;
+}
+
+__device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats& st) {
+    return orient_at<2>(a, b, c, d, st);
 }
 
 // Guigue & Devillers 2003 decision structure ("Fast and robust
@@ -122,18 +200,31 @@ __device__ __forceinline__ int orient_filtered(f3 a, f3 b, f3 c, f3 d, PredStats
 //
 // dsides packs the exact sides (+1 each, 2 bits): U's vertices vs V's plane
 // in bits 0-5, V's vertices vs U's plane in bits 6-11.
+//
+// At precision level L < 2 (see orient_at) a predicate can stay undecided;
+// the test then stops and returns DEFERRED, for a rerun at a higher level.
 __device__ __forceinline__ int side_of(uint32_t packed, int k) { return int((packed >> (2 * k)) & 3u) - 1; }
 __device__ __forceinline__ f3 pick3(f3 a, f3 b, f3 c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
 
+constexpr int DEFERRED = -1;
+#define CD_DECIDE(v) \
+    if (L < 2 && (v) == UNDECIDED) return DEFERRED
+template <int L = 2>
 __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
-    const int dp1 = orient_filtered(p2, q2, r2, p1, st);
-    const int dq1 = orient_filtered(p2, q2, r2, q1, st);
-    const int dr1 = orient_filtered(p2, q2, r2, r1, st);
+    const int dp1 = orient_at<L>(p2, q2, r2, p1, st);
+    CD_DECIDE(dp1);
+    const int dq1 = orient_at<L>(p2, q2, r2, q1, st);
+    CD_DECIDE(dq1);
+    const int dr1 = orient_at<L>(p2, q2, r2, r1, st);
+    CD_DECIDE(dr1);
     dsides = (uint32_t(dp1 + 1) << 6) | (uint32_t(dq1 + 1) << 8) | (uint32_t(dr1 + 1) << 10);
     if (dp1 * dq1 > 0 && dp1 * dr1 > 0) return 0;
-    const int dp2 = orient_filtered(q1, r1, p1, p2, st);
-    const int dq2 = orient_filtered(q1, r1, p1, q2, st);
-    const int dr2 = orient_filtered(q1, r1, p1, r2, st);
+    const int dp2 = orient_at<L>(q1, r1, p1, p2, st);
+    CD_DECIDE(dp2);
+    const int dq2 = orient_at<L>(q1, r1, p1, q2, st);
+    CD_DECIDE(dq2);
+    const int dr2 = orient_at<L>(q1, r1, p1, r2, st);
+    CD_DECIDE(dr2);
     dsides |= uint32_t(dp2 + 1) | (uint32_t(dq2 + 1) << 2) | (uint32_t(dr2 + 1) << 4);
     if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
     if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
@@ -190,10 +281,15 @@ __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, ui
     const f3 Q2 = pick3(p2, s_q2, s_r2, rot2 == 2 ? 0 : rot2 + 1);
     const f3 R2 = pick3(p2, s_q2, s_r2, rot2 == 0 ? 2 : rot2 - 1);
     // the two final orientations (the check of the canonical form)
-    if (orient_filtered(P2, P1, Q1, Q2, st) > 0) return 0;
-    if (orient_filtered(P2, R1, P1, R2, st) > 0) return 0;
+    const int f1 = orient_at<L>(P2, P1, Q1, Q2, st);
+    CD_DECIDE(f1);
+    if (f1 > 0) return 0;
+    const int f2 = orient_at<L>(P2, R1, P1, R2, st);
+    CD_DECIDE(f2);
+    if (f2 > 0) return 0;
     return 1;
 }
+#undef CD_DECIDE
 
 // Contact segment of an intersecting, non-coplanar pair, with Möller's
 // interval construction (Triangle.cpp:764-1001) driven by the exact sides:
@@ -249,9 +345,9 @@ __device__ inline void exact_segment(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, u
     }
 }
 
-// Contact record of a colliding pair, written in place (out of line: only
-// hit lanes run it).
-__device__ __noinline__ void write_contact(cd_contact* dst, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2, uint32_t ds,
+// Contact record of a colliding pair, written in place.  Inline: as a call
+// it would keep the caller's live registers across the call boundary.
+__device__ __forceinline__ void write_contact(cd_contact* dst, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2, uint32_t ds,
                                            uint32_t q, uint32_t ta, uint32_t tb) {
     f3 s, t;
     exact_segment(A0, A1, A2, B0, B1, B2, ds, s, t);

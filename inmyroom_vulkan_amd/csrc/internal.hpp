@@ -149,6 +149,9 @@ struct RunCounters {
     // leaf-pair slots handed out by the batched traversal; slots beyond the
     // leaf pairs it found hold empty entries (zero triangle counts)
     unsigned long long lpslots;
+    // triangle tests the batched narrow phase left to its float64 and exact
+    // reruns (list lengths; entries beyond the list capacity are not stored)
+    unsigned long long deferred, deferred2;
 };
 
 // Build of a batch of triangle soups into the arenas (build.hip).
@@ -181,11 +184,12 @@ struct cd_ctx {
     bool contacts_sorted = false;          // sorted copy in d_work[W_CSORT]
     std::vector<uint2> staged_ids;         // per query: (mesh_a, mesh_b), or per frame pair: (first, second)
 
-    cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg;
+    cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
     uint64_t lp_cap = 0, contact_cap = 0, frontier_cap = 0;
+    uint64_t defer_cap = 0;  // entries of each rerun list of the batched narrow phase
 
     cd_counters counters{};
     cd_timings timings{};

@@ -30,11 +30,13 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
                       uint32_t dq_cap, uint32_t share, uint32_t spin_polls, RunCounters* ctr, hipStream_t st);
+// defer: 2 * defer_cap entries, the float64 and the exact rerun lists
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
-                               const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
-                               hipStream_t st);
+                               const float* tris, cd_contact* out, uint64_t cap, uint4* defer, uint64_t defer_cap,
+                               RunCounters* ctr, uint32_t grid, uint32_t rerun_grid, hipStream_t st);
+// lists: 64 + 2n words of device scratch (two rerun lists of n entries)
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
-                               RunCounters* ctr, hipStream_t st);
+                               uint32_t* lists, uint32_t grid, RunCounters* ctr, hipStream_t st);
 hipError_t launch_fill_u32(unsigned int* p, uint32_t n, unsigned int v, hipStream_t st);
 
 // broad.hip

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_narrow_plain(const uint4* __restrict__ 
 }
 
 __global__ void k_zero_counters(RunCounters* c) {
-    if (threadIdx.x == 0) *c = RunCounters{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (threadIdx.x == 0) *c = RunCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 }
 
 __global__ void k_fill_u32(unsigned int* p, uint32_t n, unsigned int v) {

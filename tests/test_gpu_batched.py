@@ -71,6 +71,26 @@ def test_batched_queries_uneven_subset(gpu, orc):
     assert len(got) > 100
 
 
+def test_batched_self_contact_deferred_tests(gpu, orc):
+    """A mesh against itself at the identity: neighbouring triangles share
+    vertices, so their predicates are exactly zero and every such test goes
+    from the float32 pass through the float64 rerun to the exact one; there
+    are more of them than the rerun lists first hold, so the run is repeated
+    with longer lists."""
+    P, F = scenes.icosphere(32)
+    soup = P.astype(np.float32)[F].reshape(-1, 9)
+    case(f"batched self contact at identity, {len(soup)} triangles")
+    mid = gpu.register_soups([soup])[0]
+    rel = scenes.identity16()[None]
+    got = gpu.query_batch(gpu.make_queries(mid, mid, rel), MODE_BATCHED)
+    cnt = gpu.counters()
+    z = np.zeros(1, np.uint32)
+    want = _exact_oracle(orc, [soup], z, z, rel, 4)
+    assert _keys(got) == _keys(want.contacts)
+    assert cnt["exact_fallbacks"] > 65536
+    assert len(got) > 3 * len(soup)  # each triangle touches its edge and vertex neighbours
+
+
 @pytest.mark.parametrize("tuning", [
     dict(split_levels=0, stack_entries=512, pop_width=64, waves_per_cu=2),   # no split; spills in LDS stack
     dict(split_levels=4, stack_entries=2048, pop_width=16, waves_per_cu=4),
