@@ -128,8 +128,8 @@ def config4(args):
                  extra={"plain_tests_per_s": round(n / (res["plain"]["kernel_ms"] * 1e-3), 1),
                         "fp64_fallbacks": res["batched"]["fp64"], "exact_fallbacks": res["batched"]["exact"],
                         "mismatches_vs_exact_oracle": mism,
-                        "roofline": {"bound": "hbm", "kernel": "k_tritri_f32 + k_tritri_rerun<1> + k_tritri_rerun<2>", "achieved": round(achieved, 2),
-                                     "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
+                        "roofline": {"bound": "hbm", "kernel": "k_tritri_f32 + k_tritri_rerun<1> + k_tritri_rerun<2>",
+                                     "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
                                      "traffic": None, "algorithmic_bytes_per_launch": int(bytes_per_launch)},
                         "cpu_baseline": {"value": round(sample / t_cpu, 1), "unit": "tests/s", "cores": 1,
                                          "kind": "port", "sample": f"{sample:,} pairs, reference Möller, single thread"},
