@@ -6,6 +6,10 @@
 //  k_radix_sort   stable LSD radix sort of (key, value) pairs, 4-bit digits,
 //                 one workgroup; ranks within a wavefront from ballots on
 //                 the digit bits, across wavefronts from an LDS table
+//                 (up to RS_SMALL keys)
+//  k_rs_*         the same sort over many workgroups for longer arrays
+//                 (contacts): 8-bit digits, per-tile digit counts, one
+//                 scan of the count table, stable scatter
 //  k_sap_sweep    one lane per entity in U order: every later entity whose
 //                 U.min <= this U.max, kept when V and W overlap and either
 //                 entity wants callbacks (SweepAndPrune.cpp:50-85); the pair
@@ -15,6 +19,8 @@
 //
 // Ties on U.min are ordered by entity index (the reference's std::sort
 // leaves them unspecified); +0 and -0 are the same key.
+#include <utility>
+
 #include "internal.hpp"
 #include "kernels.hpp"
 
@@ -105,6 +111,83 @@ __global__ __launch_bounds__(1024) void k_radix_sort(uint32_t* keys, uint32_t* v
     }
 }
 
+// Multi-workgroup pass of the LSD sort: tiles of RS_TILE keys; the count
+// table is digit-major ([digit][tile]) so that one exclusive scan gives
+// every tile the start of each of its digit runs in tile order (stable).
+constexpr uint32_t RS_TILE = 4096;
+constexpr uint32_t RS_SMALL = 4096;
+
+__global__ __launch_bounds__(256) void k_rs_count(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                  uint32_t* __restrict__ table) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t lo = blockIdx.x * RS_TILE, hi = min(lo + RS_TILE, n);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    __syncthreads();
+    table[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the m-entry count table in place, one workgroup.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ table, uint32_t m) {
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (m + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, m);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += table[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the partial sums
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t c = table[i];
+        table[i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv,
+                                                    uint32_t n, uint32_t shift, const uint32_t* __restrict__ table,
+                                                    uint32_t* __restrict__ dk, uint32_t* __restrict__ dv) {
+    __shared__ uint32_t start[256];    // next output slot of each digit for this tile
+    __shared__ uint32_t wcnt[4][256];  // keys of each digit per wavefront in the current round
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    start[tid] = table[size_t(tid) * gridDim.x + blockIdx.x];
+    const uint32_t lo = blockIdx.x * RS_TILE, hi = min(lo + RS_TILE, n);
+    for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
+        const uint32_t i = r0 + tid;
+        const bool valid = i < hi;
+        const uint32_t k = valid ? sk[i] : 0u, v = valid ? sv[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bm = __ballot(valid && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? bm : ~bm;
+        }
+        const uint32_t rank = uint32_t(__popcll(peers & below));
+        for (uint32_t e = tid; e < 4 * 256; e += 256) (&wcnt[0][0])[e] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wcnt[wave][d] = uint32_t(__popcll(peers));
+        __syncthreads();
+        if (valid) {
+            uint32_t off = start[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][d];
+            dk[off] = k;
+            dv[off] = v;
+        }
+        __syncthreads();
+        start[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+}
+
 __global__ void k_sap_sweep(const uint32_t* __restrict__ order, const float* __restrict__ iv,
                             const cd_entity* __restrict__ ents, uint32_t n, uint2* __restrict__ pairs, uint32_t cap,
                             unsigned int* __restrict__ count) {
@@ -179,11 +262,38 @@ hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* 
     hipLaunchKernelGGL(k_sap_project, g256(n), dim3(256), 0, st, ents, n, meshes, ax, iv, keys, vals);
     return hipGetLastError();
 }
+uint64_t radix_table_words(uint32_t n) { return uint64_t(256) * ((uint64_t(n) + RS_TILE - 1) / RS_TILE); }
+
+uint32_t key_bits_below(uint64_t bound) {
+    uint32_t b = 1;
+    while (b < 32 && (uint64_t(1) << b) < bound) ++b;
+    return b;
+}
+
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-                             uint32_t key_bits, hipStream_t st) {
-    uint32_t passes = (key_bits + 3) / 4;
-    passes += passes & 1u;  // even, so the result lands back in keys/vals
-    hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(1024), 0, st, keys, vals, keys_tmp, vals_tmp, n, passes);
+                             uint32_t key_bits, uint32_t* table, hipStream_t st) {
+    if (n < 2) return hipSuccess;
+    if (n <= RS_SMALL || !table) {
+        uint32_t passes = (key_bits + 3) / 4;
+        passes += passes & 1u;  // even, so the result lands back in keys/vals
+        hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(1024), 0, st, keys, vals, keys_tmp, vals_tmp, n, passes);
+        return hipGetLastError();
+    }
+    const uint32_t tiles = uint32_t((uint64_t(n) + RS_TILE - 1) / RS_TILE);
+    const uint32_t m = 256 * tiles;
+    uint32_t *sk = keys, *sv = vals, *dk = keys_tmp, *dv = vals_tmp;
+    for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(256), 0, st, sk, n, shift, table);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, table, m);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(256), 0, st, sk, sv, n, shift, table, dk, dv);
+        std::swap(sk, dk);
+        std::swap(sv, dv);
+    }
+    if (sk != keys) {  // odd number of passes: the result is in the temporaries
+        hipError_t e = hipMemcpyAsync(keys, sk, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(vals, sv, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
@@ -200,26 +310,23 @@ hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32
     hipLaunchKernelGGL(k_pair_queries, g256(n), dim3(256), 0, st, pairs, order, n, ents, qs, sorted_pairs);
     return hipGetLastError();
 }
-hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, uint32_t* k0, uint32_t* v0,
-                                uint32_t* k1, uint32_t* v1, hipStream_t st) {
-    // stable LSD over (query, tri_a, tri_b): least significant field first
+hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo,
+                            const uint32_t field_bits[3], uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                            uint32_t* table, hipStream_t st) {
     hipLaunchKernelGGL(k_iota, g256(n), dim3(256), 0, st, v0, n);
-    for (int field = 2; field >= 0; --field) {
+    for (int field : {f_lo, f_mid, f_hi}) {  // stable LSD: least significant field first
         hipLaunchKernelGGL(k_contact_keys, g256(n), dim3(256), 0, st, in, n, field, v0, k0);
-        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, 32, st);
+        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, field_bits[field], table, st);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_gather_contacts, g256(n), dim3(256), 0, st, in, v0, n, out);
     return hipGetLastError();
 }
-hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo, uint32_t* k0,
-                            uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st) {
-    hipLaunchKernelGGL(k_iota, g256(n), dim3(256), 0, st, v0, n);
-    for (int field : {f_lo, f_mid, f_hi}) {
-        hipLaunchKernelGGL(k_contact_keys, g256(n), dim3(256), 0, st, in, n, field, v0, k0);
-        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, 32, st);
-        if (e != hipSuccess) return e;
-    }
+hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, const uint32_t field_bits[3],
+                                uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* table,
+                                hipStream_t st) {
+    hipError_t e = launch_sort_perm(in, n, 0, 1, 2, field_bits, k0, v0, k1, v1, table, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gather_contacts, g256(n), dim3(256), 0, st, in, v0, n, out);
     return hipGetLastError();
 }
 

@@ -185,8 +185,23 @@ constexpr uint32_t SMALL_WORDS = 80;
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS };
-static_assert(W_TASKS < 40, "work slots");
+       W_RAYS, W_TASKS, W_SORTT };
+static_assert(W_SORTT < 40, "work slots");
+
+// Count table of the multi-workgroup radix sort for n keys.
+hipError_t sort_table(cd_ctx* ctx, uint32_t n, uint32_t** table) {
+    hipError_t e = ctx->d_work[W_SORTT].ensure(radix_table_words(n) * sizeof(uint32_t));
+    *table = ctx->d_work[W_SORTT].as<uint32_t>();
+    return e;
+}
+
+// Key bits of the contact fields query, tri_a, tri_b for nq queries.
+void contact_field_bits(const cd_ctx* ctx, uint32_t nq, uint32_t bits[3]) {
+    uint64_t max_tris = 1;
+    for (const MeshRec& m : ctx->meshes) max_tris = std::max<uint64_t>(max_tris, m.ntri);
+    bits[0] = key_bits_below(nq);
+    bits[1] = bits[2] = key_bits_below(max_tris);
+}
 
 // Enqueue the whole batched pipeline on the stream (no host synchronisation
 // inside, so it can be captured in a graph).
@@ -426,7 +441,9 @@ int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out
     uint32_t np = 0;
     HCK(ctx, hipEventRecord(ctx->ev[6], st));
     HCK(ctx, launch_sap_project(de, n, ctx->d_meshes.as<MeshRec>(), sap_axes(), ctx->d_work[W_IV].as<float>(), k0, v0, st));
-    HCK(ctx, launch_radix_sort(k0, v0, k1, v1, n, 32, st));
+    uint32_t* table = nullptr;
+    HCK(ctx, sort_table(ctx, n, &table));
+    HCK(ctx, launch_radix_sort(k0, v0, k1, v1, n, 32, table, st));
     for (int attempt = 0; attempt < 3; ++attempt) {
         HCK(ctx, ctx->d_work[W_PAIRS].ensure(size_t(cap) * sizeof(uint2)));
         HCK(ctx, launch_fill_u32(cnt, 1, 0u, st));
@@ -446,7 +463,8 @@ int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out
         HCK(ctx, ctx->d_work[W_SPAIRS].ensure(size_t(np) * sizeof(uint2)));
         HCK(ctx, ctx->d_queries.ensure(size_t(np) * sizeof(QueryRec)));
         HCK(ctx, launch_pair_keys(ctx->d_work[W_PAIRS].as<uint2>(), np, k0, v0, st));
-        HCK(ctx, launch_radix_sort(k0, v0, k1, v1, np, 32, st));
+        HCK(ctx, sort_table(ctx, np, &table));
+        HCK(ctx, launch_radix_sort(k0, v0, k1, v1, np, key_bits_below(uint64_t(n) << 16), table, st));
         HCK(ctx, launch_pair_queries(ctx->d_work[W_PAIRS].as<uint2>(), v0, np, de, ctx->d_queries.as<QueryRec>(),
                                      ctx->d_work[W_SPAIRS].as<uint2>(), st));
     }
@@ -516,6 +534,7 @@ int cd_destroy(cd_ctx* ctx) {
                       &ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm})
         b->release();
     for (auto& b : ctx->d_work) b.release();
+    ctx->h_stage.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -767,18 +786,25 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
     ctx->timings.broad_ms = broad_ms;
     ctx->timings.total_ms += broad_ms;
     ctx->staged_contacts = nc;
-    // contacts grouped by pair, ordered by (pair, tri_a, tri_b)
-    std::vector<cd_contact> hc(nc);
+    // contacts grouped by pair, ordered by (pair, tri_a, tri_b), staged in
+    // pinned host memory
+    const cd_contact* hc = nullptr;
     if (nc) {
         if (nc > 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "too many contacts in one frame");
         hipStream_t st = ctx->stream;
         HCK(ctx, ctx->d_work[W_CSORT].ensure(nc * sizeof(cd_contact)));
         for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure(nc * sizeof(uint32_t)));
+        uint32_t* table = nullptr;
+        HCK(ctx, sort_table(ctx, uint32_t(nc), &table));
+        uint32_t bits[3];
+        contact_field_bits(ctx, np, bits);
         HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), uint32_t(nc), ctx->d_work[W_CSORT].as<cd_contact>(),
-                                      ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
-                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
-        HCK(ctx, hipMemcpyAsync(hc.data(), ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost, st));
+                                      bits, ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
+                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), table, st));
+        HCK(ctx, ctx->h_stage.ensure(nc * sizeof(cd_contact)));
+        HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost, st));
         HCK(ctx, hipStreamSynchronize(st));
+        hc = static_cast<const cd_contact*>(ctx->h_stage.p);
     }
     ctx->contacts_sorted = true;
     ctx->last_run = 2;
@@ -797,7 +823,7 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
     if (out_pairs.size() > pair_cap || nc > contact_cap || (nc && (!pairs || !contacts)))
         return fail(ctx, CD_E_CAPACITY, "pair or contact buffer too small");
     if (!out_pairs.empty()) std::memcpy(pairs, out_pairs.data(), out_pairs.size() * sizeof(cd_pair));
-    if (nc) std::memcpy(contacts, hc.data(), nc * sizeof(cd_contact));
+    if (nc) std::memcpy(contacts, hc, nc * sizeof(cd_contact));
     return CD_OK;
 }
 
@@ -828,15 +854,20 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
     HCK(ctx, ctx->d_work[W_RHKEY].ensure(H * sizeof(unsigned long long)));
     HCK(ctx, ctx->d_work[W_RHVAL].ensure(H * sizeof(unsigned long long)));
     const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
+    uint32_t* table = nullptr;
+    HCK(ctx, sort_table(ctx, nc, &table));
+    uint32_t bits[3];
+    contact_field_bits(ctx, nq, bits);
     if (!ctx->contacts_sorted) {
-        HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), nc, ctx->d_work[W_CSORT].as<cd_contact>(),
+        HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), nc, ctx->d_work[W_CSORT].as<cd_contact>(), bits,
                                       ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
-                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
+                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), table, st));
         ctx->contacts_sorted = true;
     }
     // second side's order: (query, tri_b, tri_a)
-    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
-                              ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), st));
+    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
+                              ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
+                              ctx->d_work[W_V1].as<uint32_t>(), table, st));
     const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
     const QueryRec* dq = ctx->d_queries.as<QueryRec>();
     const MeshRec* dm = ctx->d_meshes.as<MeshRec>();

@@ -116,13 +116,18 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t need) {
         if (need <= bytes) return hipSuccess;
+        const size_t bytes_before = bytes;
         if (p) {
             hipError_t e = hipFree(p);
             if (e != hipSuccess) return e;
             p = nullptr;
             bytes = 0;
         }
+        // at least 1.5x the old size: sizes that creep up from call to call
+        // (contacts per frame) would otherwise free and allocate every call,
+        // and hipFree waits for the device
         size_t want = need < 4096 ? 4096 : need;
+        if (want < bytes_before + bytes_before / 2) want = bytes_before + bytes_before / 2;
         hipError_t e = hipMalloc(&p, want);
         if (e == hipSuccess) bytes = want;
         return e;
@@ -133,6 +138,27 @@ struct DevBuf {
         bytes = 0;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
+};
+// Pinned host memory, grown like DevBuf.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        const size_t bytes_before = bytes;
+        release();
+        size_t want = need < 4096 ? 4096 : need;
+        if (want < bytes_before + bytes_before / 2) want = bytes_before + bytes_before / 2;
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = want;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
 };
 // A device buffer freed when it goes out of scope (temporaries).
 struct ScopedBuf : DevBuf {
@@ -185,6 +211,7 @@ struct cd_ctx {
     std::vector<uint2> staged_ids;         // per query: (mesh_a, mesh_b), or per frame pair: (first, second)
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
+    cdk::HostBuf h_stage;  // pinned staging of results copied back to the host
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;

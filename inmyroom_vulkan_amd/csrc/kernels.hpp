@@ -42,19 +42,29 @@ hipError_t launch_fill_u32(unsigned int* p, uint32_t n, unsigned int v, hipStrea
 // broad.hip
 hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* meshes, const SapAxes& ax, float* iv,
                               uint32_t* keys, uint32_t* vals, hipStream_t st);
+// Stable sort of (key, value) pairs by keys below 2^key_bits; the result is
+// in keys/vals.  Longer arrays run over many workgroups and need `table`,
+// radix_table_words(n) words of device scratch (without it they run in one).
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-                             uint32_t key_bits, hipStream_t st);
+                             uint32_t key_bits, uint32_t* table, hipStream_t st);
+uint64_t radix_table_words(uint32_t n);
+// Bits that hold every value below `bound` (at least 1).
+uint32_t key_bits_below(uint64_t bound);
 hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
                             uint32_t cap, unsigned int* count, hipStream_t st);
 hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
 hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32_t n, const cd_entity* ents,
                                QueryRec* qs, uint2* sorted_pairs, hipStream_t st);
-hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, uint32_t* k0, uint32_t* v0,
-                                uint32_t* k1, uint32_t* v1, hipStream_t st);
+// Contacts ordered by (query, tri_a, tri_b) into out.  field_bits: key bits
+// of query, tri_a and tri_b (values below 2^bits).
+hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, const uint32_t field_bits[3],
+                                uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* table,
+                                hipStream_t st);
 // Stable order of contacts by three fields (0 query, 1 tri_a, 2 tri_b), most
 // significant first; the permutation ends in v0.
-hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo, uint32_t* k0,
-                            uint32_t* v0, uint32_t* k1, uint32_t* v1, hipStream_t st);
+hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo,
+                            const uint32_t field_bits[3], uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                            uint32_t* table, hipStream_t st);
 
 // rays.hip
 hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,

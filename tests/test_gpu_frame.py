@@ -73,6 +73,36 @@ def test_frame_batched_matches_exact_oracle(gpu, orc):
     assert got == want and len(want) > 0
 
 
+def test_frame_many_contacts_sorted(gpu, orc):
+    """The 512-entity scene: tens of thousands of contacts go through the
+    multi-workgroup radix sort; they come back grouped by pair in (first,
+    second) order and by (tri_a, tri_b) within a pair, and as a set equal
+    the exact oracle's."""
+    sc = scenes.Scene(n_entities=512, seed=2)
+    case("frame batched, 512 entities")
+    ids = gpu.register_soups(sc.soups)
+    trees = [orc.tree(s, 4) for s in sc.soups]
+    ents = sc.entities(ids, ENTITY_DTYPE)
+    pairs, cons = gpu.frame(ents, MODE_BATCHED)
+    assert len(cons) > 4096
+    keys = [(int(p["first"]), int(p["second"])) for p in pairs]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+    begin = 0
+    for p in pairs:
+        assert p["contact_begin"] == begin
+        c = cons[begin: begin + p["contact_count"]]
+        tk = [(int(x["tri_a"]), int(x["tri_b"])) for x in c]
+        assert tk == sorted(tk) and len(set(tk)) == len(tk)
+        begin += int(p["contact_count"])
+    assert begin == len(cons)
+    res = orc.frame(trees, sc.mesh_of, ents["current"], sc.should_callback, mode=1)
+    want = {(int(res.pairs[c["query"]][0]), int(res.pairs[c["query"]][1]), int(c["tri_a"]), int(c["tri_b"]))
+            for c in res.contacts}
+    got = {(int(p["first"]), int(p["second"]), int(x["tri_a"]), int(x["tri_b"]))
+           for p in pairs for x in cons[p["contact_begin"]: p["contact_begin"] + p["contact_count"]]}
+    assert got == want
+
+
 def test_frame_small_inputs(gpu):
     case("frame with fewer than two entities")
     ids = gpu.register_soups([scenes.box_soup()])
