@@ -48,37 +48,68 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 constexpr float U32 = 5.9604644775390625e-08f;  // 2^-24
 
-__device__ __forceinline__ float mag1(const Box& b) {
-    return (fabsf(b.c.x) + fabsf(b.c.y) + fabsf(b.c.z)) + (fabsf(b.u.x) + fabsf(b.u.y) + fabsf(b.u.z)) +
-           (fabsf(b.v.x) + fabsf(b.v.y) + fabsf(b.v.z)) + (fabsf(b.w.x) + fabsf(b.w.y) + fabsf(b.w.z));
-}
+__device__ __forceinline__ float norm1(f3 v) { return (fabsf(v.x) + fabsf(v.y)) + fabsf(v.z); }
 
-// Conservative 15-axis test (same axes and order as Paralgram.cpp:17-173).
-__device__ __forceinline__ bool axis_may_overlap(const Box& l, const Box& r, f3 a, float k) {
-    float l0, l1, r0, r1;
-    project(l, a, l0, l1);
-    project(r, a, r0, r1);
-    const float tol = (fabsf(a.x) + fabsf(a.y) + fabsf(a.z)) * k;
-    return !((r0 - l1 > tol) || (l0 - r1 > tol));
-}
+// Conservative separating-axis test of two boxes (centre + three half-axis
+// vectors each) on the 15 axes of Paralgram.cpp:17-173.  On an axis L the
+// boxes are disjoint when |L.T| > sum_k |L.a_k| + sum_l |L.b_l|, T = r.c - l.c,
+// a_k and b_l the two boxes' half-axes.  Every one of these dot products is a
+// scalar triple product of box vectors and T, so the 15 axes share them:
+//   L = a_p x a_q (face of l):  |L.a_m| = |det a|, |L.b_j| = D1[m][j] = |(a_p x a_q).b_j|
+//   L = b_p x b_q (face of r):  |L.b_n| = |det b|, |L.a_i| = D2[i][n] = |a_i.(b_p x b_q)|
+//   L = a_i x b_j (edges):      |L.a_k| = D1[m][j] for the m != i,
+//                               |L.b_l| = D2[i][n] for the n != j,
+//                               |L.T| = |a_i.(b_j x T)|
+// (the remaining terms, L.a_p for an axis built from a_p, are exactly zero).
+// Rounding: a float triple product x.(y z) is within 5u |x|_1 |y|_1 |z|_1 of
+// the exact one and |x cross y|_1 <= |x|_1 |y|_1, so 16u S |x|_1 |y|_1, S the
+// sum of the 1-norms of T and the six half-axes, bounds every error of an
+// axis built from x and y, the sums included; sigma |x|_1 |y|_1 bounds how far
+// the triangles stick out of their boxes along it.  An axis rejects only
+// beyond both: no intersecting pair is culled.
 __device__ inline bool sat_conservative(const Box& l, const Box& r, float sigma) {
-    const float k = 16.f * U32 * (mag1(l) + mag1(r)) + sigma;
-    if (!axis_may_overlap(l, r, cross(l.v, l.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.u, l.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.u, l.v), k)) return false;
-    if (!axis_may_overlap(l, r, cross(r.v, r.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(r.u, r.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(r.u, r.v), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.u, r.u), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.u, r.v), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.u, r.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.v, r.u), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.v, r.v), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.v, r.w), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.w, r.u), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.w, r.v), k)) return false;
-    if (!axis_may_overlap(l, r, cross(l.w, r.w), k)) return false;
-    return true;
+    const f3 a[3] = {l.u, l.v, l.w};
+    const f3 b[3] = {r.u, r.v, r.w};
+    const f3 T = sub(r.c, l.c);
+    float na[3], nb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        na[i] = norm1(a[i]);
+        nb[i] = norm1(b[i]);
+    }
+    const float S = norm1(T) + ((na[0] + na[1]) + na[2]) + ((nb[0] + nb[1]) + nb[2]);
+    const float K = 16.f * U32 * S + sigma;
+    // face normals in the reference's order: v x w, u x w, u x v
+    const f3 NA[3] = {cross(a[1], a[2]), cross(a[0], a[2]), cross(a[0], a[1])};
+    const f3 NB[3] = {cross(b[1], b[2]), cross(b[0], b[2]), cross(b[0], b[1])};
+    const float fa[3] = {na[1] * na[2], na[0] * na[2], na[0] * na[1]};
+    const float fb[3] = {nb[1] * nb[2], nb[0] * nb[2], nb[0] * nb[1]};
+    const float detA = fabsf(dot(NA[0], a[0])), detB = fabsf(dot(NB[0], b[0]));
+    float D1[3][3], D2[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            D1[m][j] = fabsf(dot(NA[m], b[j]));
+            D2[m][j] = fabsf(dot(a[m], NB[j]));
+        }
+    bool sep = false;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        sep |= fabsf(dot(NA[m], T)) - (detA + ((D1[m][0] + D1[m][1]) + D1[m][2])) > fa[m] * K;
+        sep |= fabsf(dot(NB[m], T)) - (((D2[0][m] + D2[1][m]) + D2[2][m]) + detB) > fb[m] * K;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const f3 X = cross(b[j], T);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float rA = D1[i == 0 ? 1 : 0][j] + D1[i == 2 ? 1 : 2][j];
+            const float rB = D2[i][j == 0 ? 1 : 0] + D2[i][j == 2 ? 1 : 2];
+            sep |= fabsf(dot(a[i], X)) - (rA + rB) > (na[i] * nb[j]) * K;
+        }
+    }
+    return !sep;
 }
 
 // Traversal codes of the batched path are absolute, so a node pair's two
