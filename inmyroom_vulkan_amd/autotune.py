@@ -24,12 +24,11 @@ TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
 SPACE = {
     # joint keys: each value sets both fields
-    "split_levels/split_tris": [(6, 0), (8, 0), (10, 0), (14, 0), (16, 512), (20, 512), (24, 2048), (32, 2048),
-                                (32, 8192)],
-    "stack_entries": [512],
+    "split_levels/split_tris": [(6, 0), (8, 0), (10, 0), (12, 0), (16, 512), (24, 2048), (32, 8192)],
+    "stack_entries": [512, 1024],
     "pop_width": [64],
-    "waves_per_cu": [4, 8],
-    "work_sharing": [0, 1],
+    "waves_per_cu": [2, 4, 8],
+    "work_sharing": [0],  # 1 and 2 are correct but have not won on any workload measured
 }
 
 

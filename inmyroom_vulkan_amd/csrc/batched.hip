@@ -46,6 +46,22 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// Adds a per-lane count to a device counter with one atomic per block
+// (256 threads; every thread of the block calls it).  Same-address atomics
+// serialise in L2, so one per wavefront over a long stream costs more than
+// the arithmetic.
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* __restrict__ dst) {
+    __shared__ unsigned long long part[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+}
+
 constexpr float U32 = 5.9604644775390625e-08f;  // 2^-24
 
 __device__ __forceinline__ float norm1(f3 v) { return (fabsf(v.x) + fabsf(v.y)) + fabsf(v.z); }
@@ -265,6 +281,11 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
     // by level, so read backwards it starts with the largest tasks.
     const uint32_t n = min(*nin, out_cap);  // frontier buffers and the task list hold out_cap entries each
     const uint32_t stride = gridDim.x * blockDim.x;
+    // slots are reserved per block: one atomic per output list and iteration
+    __shared__ unsigned int s_n[4][3];  // per wavefront: leaf pairs, next-level entries, tasks
+    __shared__ unsigned long long s_base[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = lanes_below_b();
     unsigned long long nsat = 0, nleaf = 0;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -278,40 +299,54 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
             ++nsat;
             kind = eval_pair(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf, cost);
         }
-        const unsigned long long mL = __ballot(kind == PAIR_LEAF);
-        if (mL) {
-            const unsigned long long s = reserve_b(mL, &ctr->lpslots);
-            if (kind == PAIR_LEAF && s < lp_cap) lp_out[s] = leaf;
-            nleaf += unsigned(__popcll(mL));
-        }
         const bool go_on = cost > split_tris;
-        for (int dest = 0; dest < 2; ++dest) {  // 0: next level, 1: task list
-            const bool mine = kind == PAIR_SPLIT && go_on == (dest == 0);
-            const unsigned long long mS = __ballot(mine);
-            if (!mS) continue;
-            unsigned int* cnt_p = dest == 0 ? nout : ntasks;
-            uint3* dst = dest == 0 ? out : tasks;
-            unsigned int base2 = 0;
-            const int lane = threadIdx.x & 63;
-            const int leader = __ffsll(static_cast<long long>(mS)) - 1;
-            if (lane == leader) base2 = atomicAdd(cnt_p, 2u * unsigned(__popcll(mS)));
-            base2 = __shfl(base2, leader, 64);
-            const unsigned int slot = base2 + 2u * unsigned(__popcll(mS & lanes_below_b()));
-            if (mine) {
-                if (slot + 1 < out_cap) {
-                    dst[slot] = make_uint3(qi, n0a, n0b);
-                    dst[slot + 1] = make_uint3(qi, n1a, n1b);
-                } else {
-                    atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
-                }
+        const bool is_leaf = kind == PAIR_LEAF;
+        const bool to_next = kind == PAIR_SPLIT && go_on, to_task = kind == PAIR_SPLIT && !go_on;
+        const unsigned long long mL = __ballot(is_leaf), mN = __ballot(to_next), mT = __ballot(to_task);
+        nleaf += is_leaf ? 1u : 0u;
+        if (lane == 0) {
+            s_n[wave][0] = unsigned(__popcll(mL));
+            s_n[wave][1] = 2u * unsigned(__popcll(mN));
+            s_n[wave][2] = 2u * unsigned(__popcll(mT));
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int c = int(threadIdx.x);
+            const unsigned int tot = (s_n[0][c] + s_n[1][c]) + (s_n[2][c] + s_n[3][c]);
+            unsigned long long b0 = 0;
+            if (tot) {
+                if (c == 0) b0 = atomicAdd(&ctr->lpslots, static_cast<unsigned long long>(tot));
+                else b0 = atomicAdd(c == 1 ? nout : ntasks, tot);
+            }
+            s_base[c] = b0;
+        }
+        __syncthreads();
+        unsigned long long first[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            unsigned long long f = s_base[c];
+            for (int w = 0; w < wave; ++w) f += s_n[w][c];
+            first[c] = f;
+        }
+        if (is_leaf) {
+            const unsigned long long sl = first[0] + unsigned(__popcll(mL & below));
+            if (sl < lp_cap) lp_out[sl] = leaf;
+        }
+        if (to_next || to_task) {
+            const unsigned long long m = to_next ? mN : mT;
+            const unsigned long long slot = first[to_next ? 1 : 2] + 2u * unsigned(__popcll(m & below));
+            uint3* dst = to_next ? out : tasks;
+            if (slot + 1 < out_cap) {
+                dst[slot] = make_uint3(qi, n0a, n0b);
+                dst[slot + 1] = make_uint3(qi, n1a, n1b);
+            } else {
+                atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
             }
         }
+        __syncthreads();  // s_n and s_base are rewritten by the next iteration
     }
-    nsat = wave_sum(nsat);
-    if ((threadIdx.x & 63) == 0) {
-        if (nsat) atomicAdd(&ctr->sat, nsat);
-        if (nleaf) atomicAdd(&ctr->leafpairs, nleaf);
-    }
+    block_add(nsat, &ctr->sat);
+    block_add(nleaf, &ctr->leafpairs);
 }
 
 // Work sharing (share != 0).  A wavefront that runs out of tasks registers
@@ -637,22 +672,6 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         if (lost) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 8u);
         if (share == 2u) atomicAdd(&dq_ctl[2], 1u);
     }
-}
-
-// Adds a per-lane count to a device counter with one atomic per block
-// (256 threads; every thread of the block calls it).  Same-address atomics
-// serialise in L2, so one per wavefront over a long stream costs more than
-// the arithmetic.
-__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* __restrict__ dst) {
-    __shared__ unsigned long long part[4];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
-        if (t) atomicAdd(dst, t);
-    }
-    __syncthreads();
 }
 
 constexpr uint32_t CSTAGE = 64;  // contacts a wavefront stages in LDS before one atomic reserves their slots

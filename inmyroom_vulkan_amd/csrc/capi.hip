@@ -233,10 +233,14 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     HCK(ctx, launch_seed(nq, qp, F[0], &small[0], st));
     uint3* T = ctx->d_work[W_TASKS].as<uint3>();
     const uint32_t cap32 = uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull));
-    for (uint32_t l = 0; l < L; ++l)  // the last level sends every split to the task list
+    for (uint32_t l = 0; l < L; ++l) {  // the last level sends every split to the task list
+        // level l holds at most nq * 2^l pairs: no more workgroups than that needs
+        const uint64_t bound = l >= 32 ? uint64_t(cap32) : std::min<uint64_t>(cap32, uint64_t(std::max(nq, 1u)) << l);
+        const uint32_t lgrid = uint32_t(std::min<uint64_t>(grid, (bound + 255) / 256));
         HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], cap32, qp, dm, dn,
                                     ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT],
-                                    l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st));
+                                    l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, lgrid, st));
+    }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
     const uint32_t share = t.work_sharing;
     HCK(ctx, launch_dfs(int(t.stack_entries), L ? T : F[0], nullptr, nullptr, &small[L ? TASK_COUNT : 0], cap32,
