@@ -143,10 +143,16 @@ typedef struct cd_tuning {
     uint32_t use_graph;      /* 1: capture the batched pipeline in a HIP graph and replay it */
     uint32_t work_sharing;   /* 1: wavefronts that run out of work take node pairs busy ones donate;
                                 2: once half the grid is done, busy wavefronts hand their stacks to a
-                                   second launch that shares them out */
+                                   second launch that shares them out;
+                                3: budgeted pass: a wavefront hands its stack on after handoff_pct
+                                   percent of the previous run's node-pair tests per wavefront; what
+                                   is handed on is split again breadth first (resplit_levels) and
+                                   shared out by a second launch */
     uint32_t spin_polls;     /* polls an idle wavefront waits for a donation before it leaves (0: 4096) */
     uint32_t split_tris;     /* a split level keeps splitting only node pairs over this many triangles
                                 (0: all of them); the others go to the persistent kernel */
+    uint32_t handoff_pct;    /* work_sharing 3: the budget, percent of the mean (0: 100) */
+    uint32_t resplit_levels; /* work_sharing 3: breadth-first levels over what was handed on (0: 4) */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

@@ -277,8 +277,7 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
                                                      RunCounters* __restrict__ ctr) {
     // A pair that passes is split; its two children go on to the next level
     // when the pair's subtrees hold more than split_tris triangles, and to the
-    // persistent kernel's task list otherwise.  The task list is written level
-    // by level, so read backwards it starts with the largest tasks.
+    // persistent kernel's task list otherwise.
     const uint32_t n = min(*nin, out_cap);  // frontier buffers and the task list hold out_cap entries each
     const uint32_t stride = gridDim.x * blockDim.x;
     // slots are reserved per block: one atomic per output list and iteration
@@ -387,7 +386,10 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
                                             uint4* __restrict__ lp_out, uint64_t lp_cap, uint3* __restrict__ spill_base,
                                             uint32_t spill_cap, uint32_t pop_width, uint4* __restrict__ dq,
                                             unsigned int* __restrict__ dq_ctl, uint32_t dq_cap, uint32_t share,
-                                            uint32_t spin_polls, RunCounters* __restrict__ ctr) {
+                                            uint32_t spin_polls, const unsigned long long* __restrict__ budget_p,
+                                            uint32_t budget_pct, uint3* __restrict__ dump,
+                                            unsigned int* __restrict__ ndump, uint32_t dump_cap,
+                                            RunCounters* __restrict__ ctr) {
     __shared__ uint32_t sq[CAP], sa[CAP], sb[CAP];
     const int lane = threadIdx.x;
     uint3* spill = spill_base + size_t(blockIdx.x) * spill_cap;
@@ -398,6 +400,12 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
     uint32_t size = 0, nspill = 0;
     bool tasks_done = false, overflow = false, lost = false, poller = false;
+    // Budgeted pass (share == 3): after `budget` node-pair tests a wavefront
+    // hands what is left on its stack to `dump` and leaves; the budget is
+    // budget_pct percent of the previous run's tests per wavefront (none on
+    // a first run).
+    uint64_t budget = 0, wave_tests = 0;
+    if (share == 3u && budget_p) budget = (*budget_p / gridDim.x) * budget_pct / 100u;
     unsigned int idle_read = 0, idle_seen = 0, step = 0;  // lane 0's last read of the idle count, and its broadcast
     unsigned long long nsat = 0, nleaf = 0;
     // Leaf-pair slots are taken LP_CHUNK at a time, so the shared counter
@@ -515,7 +523,14 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
             const uint32_t got = t0 < ntasks ? min(want, ntasks - t0) : 0;
             if (got < want) tasks_done = true;
             if (uint32_t(lane) < got) {
-                const uint32_t idx = lo + ntasks - 1u - (t0 + lane);  // backwards: largest tasks first
+                // Tasks are taken in a fixed pseudo-random order, k -> k P mod n
+                // (P = 2^31 - 1 is prime, so for n < P this is a permutation):
+                // neighbouring tasks come from the same query and cost alike, and
+                // runs of them would pile one query's heavy subtrees onto a few
+                // wavefronts.
+                const uint32_t k = t0 + lane;
+                const uint32_t idx =
+                    lo + (ntasks < 2147483647u ? uint32_t((uint64_t(k) * 2147483647ull) % ntasks) : k);
                 uint3 e;
                 if (tasks4) {
                     const uint4 f = tasks4[idx];
@@ -601,7 +616,23 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
             if (leave) break;
             continue;
         }
+        if (budget && wave_tests >= budget) {  // hand the rest of this stack on, then leave
+            const uint32_t n = size + nspill;
+            unsigned int base = 0;
+            if (lane == 0) base = atomicAdd(ndump, n);
+            base = __shfl(base, 0, 64);
+            if (base + n <= dump_cap && base + n >= base) {
+                for (uint32_t i = lane; i < size; i += 64) dump[base + i] = make_uint3(sq[i], sa[i], sb[i]);
+                for (uint32_t i = lane; i < nspill; i += 64) dump[base + size + i] = spill[i];
+            } else if (lane == 0) {
+                atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);  // out of frontier room: rerun larger
+            }
+            size = 0;
+            nspill = 0;
+            break;
+        }
         const uint32_t p = min(size, pop_width);
+        wave_tests += p;
         const bool have = uint32_t(lane) < p;
         uint32_t qi = 0, ca = 0, cb = 0;
         if (have) {
@@ -1011,6 +1042,11 @@ __global__ __launch_bounds__(256) void k_tritri_rerun(const float* __restrict__ 
     block_add(nexact, &ctr->exact);
 }
 
+// Node-pair tests of this run, for the next run's budgeted pass.
+__global__ void k_keep_budget(const RunCounters* __restrict__ ctr, unsigned long long* __restrict__ budget) {
+    if (threadIdx.x == 0) *budget = ctr->sat;
+}
+
 // ---- host launchers -------------------------------------------------------
 hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st) {
     if (!nq) return hipSuccess;
@@ -1033,15 +1069,21 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
                       const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
-                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, RunCounters* ctr, hipStream_t st) {
+                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
+                      uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
+                      hipStream_t st) {
 #define CD_DFS_ARGS tasks, tasks4, task_lo, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, \
-                    pop_width, dq, dq_ctl, dq_cap, share, spin_polls, ctr
+                    pop_width, dq, dq_ctl, dq_cap, share, spin_polls, budget, budget_pct, dump, ndump, dump_cap, ctr
     switch (cap_entries) {
         case 512: hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
         case 2048: hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
         default: hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
     }
 #undef CD_DFS_ARGS
+    return hipGetLastError();
+}
+hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget, hipStream_t st) {
+    hipLaunchKernelGGL(k_keep_budget, dim3(1), dim3(64), 0, st, ctr, budget);
     return hipGetLastError();
 }
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,

@@ -169,6 +169,8 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
     if (t.waves_per_cu == 0) t.waves_per_cu = 8;
     if (t.spin_polls == 0) t.spin_polls = 4096;
+    if (t.handoff_pct == 0) t.handoff_pct = 100;
+    if (t.resplit_levels == 0) t.resplit_levels = 4;
     return t;
 }
 
@@ -177,16 +179,20 @@ constexpr uint32_t DQ_CAP = 1u << 20;  // node pairs the queue of the work-shari
 // W_SMALL layout: [0..L] split-level counts; [TASK_COUNT] tasks the split
 // levels hand to the persistent kernel; [HEAD0], [HEAD1] task heads of the
 // traversal launch and of the launch that takes unclaimed donations;
-// [DQ_CTL .. +3] donation queue: reserved, claimed, wavefronts idle or gone, pollers waiting.
+// [DQ_CTL .. +3] donation queue: reserved, claimed, wavefronts idle or gone, pollers waiting;
+// budgeted pass: [DUMP_COUNT] node pairs handed on, [RS_BASE + r] re-split
+// level r's output, [TASK2_COUNT], [HEAD2] the second launch's tasks and head.
 constexpr uint32_t TASK_COUNT = 70, HEAD0 = 72, HEAD1 = 73, DQ_CTL = 76;
-constexpr uint32_t SMALL_WORDS = 80;
+constexpr uint32_t DUMP_COUNT = 80, RS_BASE = 81, TASK2_COUNT = 100, HEAD2 = 101;
+constexpr uint32_t SMALL_WORDS = 104;
+constexpr uint32_t MAX_RESPLIT = 16;
 
 // Work buffers (d_work slots): batched path, then broad phase / frame.
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS, W_SORTT };
-static_assert(W_SORTT < 40, "work slots");
+       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2 };
+static_assert(W_BUD2 < 40, "work slots");
 
 // Count table of the multi-workgroup radix sort for n keys.
 hipError_t sort_table(cd_ctx* ctx, uint32_t n, uint32_t** table) {
@@ -243,15 +249,35 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
     const uint32_t share = t.work_sharing;
+    unsigned long long* budget = ctx->d_budget.as<unsigned long long>();
+    // budgeted pass: the node pairs handed on, the re-split levels' frontier and their tasks
+    uint3* B3[3] = {ctx->d_work[W_BUD0].as<uint3>(), ctx->d_work[W_BUD1].as<uint3>(), ctx->d_work[W_BUD2].as<uint3>()};
+    const uint32_t dcap = uint32_t(std::min<uint64_t>(ctx->dump_cap, 0xFFFFFFF0ull));
     HCK(ctx, launch_dfs(int(t.stack_entries), L ? T : F[0], nullptr, nullptr, &small[L ? TASK_COUNT : 0], cap32,
                         &small[HEAD0], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
                         ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP,
-                        share, t.spin_polls, ctr, st));
-    if (share)  // donations nobody claimed, [claimed, reserved), or the stacks handed off
+                        share, t.spin_polls, budget, t.handoff_pct, B3[0], &small[DUMP_COUNT], dcap, ctr, st));
+    if (share == 1 || share == 2)  // donations nobody claimed, [claimed, reserved), or the stacks handed off
         HCK(ctx, launch_dfs(int(t.stack_entries), nullptr, dq, &small[DQ_CTL + 1], &small[DQ_CTL], DQ_CAP,
                             &small[HEAD1], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
                             ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL],
-                            DQ_CAP, 0u, t.spin_polls, ctr, st));
+                            DQ_CAP, 0u, t.spin_polls, nullptr, 0u, nullptr, nullptr, 0u, ctr, st));
+    if (share == 3) {
+        // what the budgeted launch handed on is split again breadth first and
+        // shared out by a second launch over the new task list
+        const uint32_t R = std::min(t.resplit_levels, MAX_RESPLIT);
+        for (uint32_t r = 0; r < R; ++r)
+            HCK(ctx, launch_split_level(B3[r & 1], &small[r ? RS_BASE + r : DUMP_COUNT], B3[(r + 1) & 1],
+                                        &small[RS_BASE + r + 1], dcap, qp, dm, dn, ctx->d_leafpairs.as<uint4>(),
+                                        ctx->lp_cap, B3[2], &small[TASK2_COUNT],
+                                        r + 1 == R ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st));
+        HCK(ctx, launch_dfs(int(t.stack_entries), R ? B3[2] : B3[0], nullptr, nullptr,
+                            &small[R ? TASK2_COUNT : DUMP_COUNT], dcap, &small[HEAD2], qp, dm, dn,
+                            ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
+                            t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP, 0u, t.spin_polls, nullptr, 0u, nullptr,
+                            nullptr, 0u, ctr, st));
+        HCK(ctx, launch_keep_budget(ctr, budget, st));
+    }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
     HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
                                  ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctx->d_work[W_DEFER].as<uint4>(),
@@ -271,23 +297,33 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
     HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
-    if (t.work_sharing) HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DQ_CAP) * sizeof(uint4)));
+    if (t.work_sharing == 1 || t.work_sharing == 2)
+        HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DQ_CAP) * sizeof(uint4)));
+    if (!ctx->d_budget.p) {  // node-pair tests of the last run, kept across runs (none yet)
+        HCK(ctx, ctx->d_budget.ensure(sizeof(unsigned long long)));
+        HCK(ctx, hipMemsetAsync(ctx->d_budget.p, 0, sizeof(unsigned long long), st));
+    }
     RunCounters h{};
     for (int attempt = 0; attempt < 8; ++attempt) {
         const uint64_t fcap = frontier_cap(ctx, nq, L);
         HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
+        if (t.work_sharing == 3) {
+            if (ctx->dump_cap == 0) ctx->dump_cap = 1u << 20;
+            for (int w : {W_BUD0, W_BUD1, W_BUD2}) HCK(ctx, ctx->d_work[w].ensure(ctx->dump_cap * sizeof(uint3)));
+        }
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
         std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
-                                     t.split_tris, ctx->lp_cap, ctx->contact_cap, ctx->defer_cap,
-                                     uint64_t(ctx->leaf_max), fcap};
+                                     t.split_tris, t.handoff_pct, t.resplit_levels, ctx->lp_cap, ctx->contact_cap,
+                                     ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
-                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER]})
+                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER], &ctx->d_budget,
+                                &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;
@@ -320,8 +356,9 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         int rc = read_counters(ctx, h);
         if (rc) return rc;
         bool retry = false;
-        if (h.err & 2u) {  // a split level ran out of frontier: repeat with twice the room
+        if (h.err & 2u) {  // a split level or the hand-off ran out of room: repeat with twice as much
             ctx->frontier_cap = 2 * fcap;
+            if (t.work_sharing == 3) ctx->dump_cap = 2 * std::max<uint64_t>(ctx->dump_cap, 1u << 20);
             retry = true;
         }
         if (h.err & 8ull) return fail(ctx, CD_E_HIP, "batched traversal: a donated node pair was never published");
@@ -538,6 +575,7 @@ int cd_destroy(cd_ctx* ctx) {
                       &ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm})
         b->release();
     for (auto& b : ctx->d_work) b.release();
+    ctx->d_budget.release();
     ctx->h_stage.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto& e : ctx->ev)
@@ -1154,7 +1192,8 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
     if (!ctx || !t) return CD_E_INVALID;
     if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
-    if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 2)
+    if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 3 ||
+        t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;

@@ -212,11 +212,13 @@ struct cd_ctx {
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
     cdk::HostBuf h_stage;  // pinned staging of results copied back to the host
+    cdk::DevBuf d_budget;  // node-pair tests of the last batched run (budgeted traversal pass)
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
     uint64_t lp_cap = 0, contact_cap = 0, frontier_cap = 0;
     uint64_t defer_cap = 0;  // entries of each rerun list of the batched narrow phase
+    uint64_t dump_cap = 0;   // entries of the budgeted traversal's hand-off and re-split buffers
 
     cd_counters counters{};
     cd_timings timings{};

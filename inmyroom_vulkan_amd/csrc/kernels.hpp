@@ -29,7 +29,11 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
                       const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
-                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, RunCounters* ctr, hipStream_t st);
+                      uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
+                      uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
+                      hipStream_t st);
+// Keeps this run's node-pair test count for the next run's budgeted pass.
+hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget, hipStream_t st);
 // defer: 2 * defer_cap entries, the float64 and the exact rerun lists
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, cd_contact* out, uint64_t cap, uint4* defer, uint64_t defer_cap,

@@ -125,6 +125,36 @@ def test_batched_tuning_variants_agree(gpu, tuning):
     assert cnt["sat_tests"] == base_cnt["sat_tests"]  # every node pair tested exactly once, whichever launch takes it
 
 
+@pytest.mark.parametrize("tuning", [
+    dict(split_levels=16, split_tris=512, stack_entries=1024, waves_per_cu=8, work_sharing=3, handoff_pct=10,
+         resplit_levels=2),  # a tiny budget: most of the work is handed on and split again
+    dict(split_levels=0, stack_entries=512, waves_per_cu=4, work_sharing=3, handoff_pct=100, resplit_levels=4),
+    dict(split_levels=8, stack_entries=512, waves_per_cu=2, work_sharing=3, handoff_pct=50, resplit_levels=16),
+    dict(split_levels=6, stack_entries=512, waves_per_cu=4, work_sharing=3, handoff_pct=30, resplit_levels=0),
+])
+def test_batched_budgeted_pass_agrees(gpu, tuning):
+    """Budgeted traversal: each run's budget comes from the run before, so
+    the later runs hand work on; every run tests every node pair once."""
+    soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
+    case(f"budgeted pass {tuning}")
+    ids = np.array(gpu.register_soups(soups), np.uint32)
+    q = np.zeros(len(qa), dtype=gpu.make_queries(0, 0, rel[:1]).dtype)
+    q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
+    gpu.set_tuning()
+    base = _keys(gpu.query_batch(q, MODE_BATCHED))
+    base_cnt = gpu.counters()
+    try:
+        for use_graph in (0, 0, 1, 1):
+            gpu.set_tuning(**tuning, use_graph=use_graph)
+            got = _keys(gpu.query_batch(q, MODE_BATCHED))
+            cnt = gpu.counters()
+            assert got == base
+            for k in ("sat_tests", "leaf_pairs", "tri_tests"):
+                assert cnt[k] == base_cnt[k], k
+    finally:
+        gpu.set_tuning()
+
+
 def test_batched_graph_replays_match_eager(gpu):
     soups, qa, qb, rel, _ = scenes.config3(96, seed=7, freq_cap=20)
     case("batched graph replays")
