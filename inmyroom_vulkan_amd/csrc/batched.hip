@@ -705,19 +705,36 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     }
 }
 
-constexpr uint32_t CSTAGE = 64;  // contacts a wavefront stages in LDS before one atomic reserves their slots
+// Hits of the batched narrow phase are recorded as {query, global A slot,
+// global B slot, exact sides}; k_contacts turns record i into contact i.
+// Building the contact record inside the test kernels would set their
+// register peak (and so their occupancy) for the 1-2% of tests that hit.
+constexpr uint32_t HSTAGE = 64;  // hit records a wavefront stages in LDS before one atomic reserves their slots
 
-// Copy a wavefront's staged contacts to the output (16 words each).
-__device__ __forceinline__ void flush_contacts(cd_contact* __restrict__ out, uint64_t cap, const cd_contact* staged,
-                                               uint32_t n, RunCounters* __restrict__ ctr, int lane) {
+__device__ __forceinline__ void flush_hits(uint4* __restrict__ hits, uint64_t cap, const uint4* staged, uint32_t n,
+                                           RunCounters* __restrict__ ctr, int lane) {
     unsigned long long base = 0;
     if (lane == 0) base = atomicAdd(&ctr->contacts, static_cast<unsigned long long>(n));
     base = __shfl(base, 0, 64);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(staged);
-    for (uint32_t e = uint32_t(lane); e < 16u * n; e += 64u) {
-        const uint64_t slot = base + e / 16u;
-        if (slot < cap) reinterpret_cast<uint32_t*>(out + slot)[e % 16u] = src[e];
+    if (uint32_t(lane) < n && base + lane < cap) hits[base + lane] = staged[lane];
+}
+
+// Stages the records of the lanes with `hit` set, flushing first when they
+// do not fit.  Every lane calls it.
+__device__ __forceinline__ void stage_hits(bool hit, uint4 rec, uint4* stage, uint32_t& staged,
+                                           uint4* __restrict__ hits, uint64_t cap, RunCounters* __restrict__ ctr,
+                                           int lane, unsigned long long below) {
+    const unsigned long long m = __ballot(hit);
+    if (!m) return;
+    const uint32_t nh = uint32_t(__popcll(m));
+    if (staged + nh > HSTAGE) {
+        __threadfence_block();  // this wavefront's LDS writes are complete before other lanes copy them
+        flush_hits(hits, cap, stage, staged, ctr, lane);
+        staged = 0;
+        __threadfence_block();
     }
+    if (hit) stage[staged + uint32_t(__popcll(m & below))] = rec;
+    staged += nh;
 }
 
 // Triangle pair of a test: A in its mesh's space, B moved into it by the
@@ -735,25 +752,6 @@ __device__ __forceinline__ void narrow_load(const QueryPre& Q, const float* __re
     B0 = xform_point(M, mk(b[0], b[1], b[2]));
     B1 = xform_point(M, mk(b[3], b[4], b[5]));
     B2 = xform_point(M, mk(b[6], b[7], b[8]));
-}
-
-// Stages the contacts of the lanes with `hit` set in the wavefront's LDS
-// area, flushing it first when they do not fit.  Every lane calls it.
-__device__ __forceinline__ void stage_hits(bool hit, f3 A0, f3 A1, f3 A2, f3 B0, f3 B1, f3 B2, uint32_t ds, uint32_t q,
-                                           uint32_t ta, uint32_t tb, cd_contact* stage, uint32_t& staged,
-                                           cd_contact* __restrict__ out, uint64_t cap, RunCounters* __restrict__ ctr,
-                                           int lane, unsigned long long below) {
-    const unsigned long long m = __ballot(hit);
-    if (!m) return;
-    const uint32_t nh = uint32_t(__popcll(m));
-    if (staged + nh > CSTAGE) {
-        __threadfence_block();  // this wavefront's LDS writes are complete before other lanes copy them
-        flush_contacts(out, cap, stage, staged, ctr, lane);
-        staged = 0;
-        __threadfence_block();
-    }
-    if (hit) write_contact(stage + staged + uint32_t(__popcll(m & below)), A0, A1, A2, B0, B1, B2, ds, q, ta, tb);
-    staged += nh;
 }
 
 // Appends the tests of the lanes with `defer` set to a rerun list (one
@@ -778,17 +776,71 @@ __device__ __forceinline__ void defer_tests(bool defer, uint4 e, uint4* __restri
 // global A slot, global B slot}), which k_narrow_rerun<1> and <2> finish in
 // float64 and exactly.  Keeping the slow stages out of this kernel keeps its
 // registers and scratch small.
+//
+// The test's three parts run as separate full-width batches: part 1 over
+// every test, part 2 over the tests part 1 leaves open (~30%), part 3 over
+// those part 2 leaves open (~6%).  Open tests wait in two queues per
+// wavefront in LDS ({query, A slot, B slot, signs so far}); a queue is
+// worked off 64 at a time once it holds 64, and drained at the end.  Run as
+// one sequence per lane, nearly every wavefront would carry a lane through
+// all three parts, and every test would pay for them.
+constexpr uint32_t NQCAP = 128;  // entries of each queue: under 64 before a batch adds at most 64
+
+// Appends e for the lanes with `on` set to a wavefront queue of n entries.
+__device__ __forceinline__ void queue_push(bool on, uint4 e, uint4* q, uint32_t& n, unsigned long long below) {
+    const unsigned long long m = __ballot(on);
+    if (on) q[n + uint32_t(__popcll(m & below))] = e;
+    n += uint32_t(__popcll(m));
+    __threadfence_block();  // the entries are in LDS before any lane reads them
+}
+
+// One batch of part 2 (W == 2) or part 3 (W == 3) over the top (up to) 64
+// entries of queue q; part 2 moves what stays open to queue q3.
+template <int W>
+__device__ __forceinline__ void narrow_batch(uint4* q, uint32_t& n, uint4* q3, uint32_t& n3,
+                                             const QueryPre* __restrict__ qp, const float* __restrict__ tris,
+                                             uint4* __restrict__ hits, uint64_t cap,
+                                             uint4* __restrict__ defer_list, uint64_t defer_cap,
+                                             RunCounters* __restrict__ ctr, uint4* stage, uint32_t& staged,
+                                             int lane, unsigned long long below) {
+    const uint32_t m = min(n, 64u);
+    const bool have = uint32_t(lane) < m;
+    uint4 e = make_uint4(0, 0, 0, 0);
+    if (have) e = q[n - m + uint32_t(lane)];
+    n -= m;
+    __threadfence_block();  // read before the slots are written again
+    bool hit = false, defer = false, more = false;
+    uint32_t ds = e.w;
+    if (have) {
+        f3 A0, A1, A2, B0, B1, B2;
+        narrow_load(qp[e.x], tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
+        PredStats st;
+        const int res = W == 2 ? tt_part2<0>(A0, A1, A2, B0, B1, B2, ds, st)
+                               : tt_part3<0>(A0, A1, A2, B0, B1, B2, ds, st);
+        defer = res == DEFERRED;
+        more = res == TT_MORE;
+        hit = res == 1;
+    }
+    defer_tests(defer, make_uint4(e.x, e.y, e.z, 0), defer_list, defer_cap, &ctr->deferred, lane, below);
+    if (W == 2) queue_push(more, make_uint4(e.x, e.y, e.z, ds), q3, n3, below);
+    if (W == 3) stage_hits(hit, make_uint4(e.x, e.y, e.z, ds), stage, staged, hits, cap, ctr, lane, below);
+}
+
 __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ lps,
                                                       const unsigned long long* __restrict__ nlp_p, uint64_t lp_cap,
                                                       const QueryPre* __restrict__ qp, const float* __restrict__ tris,
-                                                      cd_contact* __restrict__ out, uint64_t cap,
+                                                      uint4* __restrict__ hits, uint64_t cap,
                                                       uint4* __restrict__ defer_list, uint64_t defer_cap,
                                                       RunCounters* __restrict__ ctr) {
-    __shared__ cd_contact s_con[4][CSTAGE];
+    __shared__ uint4 s_hit[4][HSTAGE];
+    __shared__ uint4 s_q2[4][NQCAP], s_q3[4][NQCAP];
     const uint64_t nlp = min(*nlp_p, lp_cap);
     const int lane = threadIdx.x & 63;
-    cd_contact* stage = s_con[threadIdx.x >> 6];
-    uint32_t staged = 0;
+    const int w = threadIdx.x >> 6;
+    uint4* stage = s_hit[w];
+    uint4* q2 = s_q2[w];
+    uint4* q3 = s_q3[w];
+    uint32_t staged = 0, n2 = 0, n3 = 0;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
@@ -819,31 +871,42 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
             }
             const uint32_t kq = __shfl(lp.x, k, 64), ka = __shfl(lp.y, k, 64), kb = __shfl(lp.z, k, 64);
             const uint32_t kw = __shfl(lp.w, k, 64), ko = __shfl(excl, k, 64);
-            bool hit = false, defer = false;
-            f3 A0, A1, A2, B0, B1, B2;
-            uint32_t ds = 0, ga = 0, gb = 0, ta = 0, tb = 0;
+            bool defer = false, more = false;
+            uint32_t ds = 0, ga = 0, gb = 0;
             if (active) {
                 ++ntt;
                 const uint32_t kcB = kw >> 16;
                 const uint32_t r = t - ko;
                 ga = ka + r / kcB;  // ka, kb: arena-global triangle offsets
                 gb = kb + r % kcB;
-                const QueryPre& Q = qp[kq];
-                ta = ga - Q.tri_base_a;  // mesh-local slots for the contact
-                tb = gb - Q.tri_base_b;
-                narrow_load(Q, tris, ga, gb, A0, A1, A2, B0, B1, B2);
+                f3 A0, A1, A2, B0, B1, B2;
+                narrow_load(qp[kq], tris, ga, gb, A0, A1, A2, B0, B1, B2);
                 PredStats st;
-                const int res = tri_tri_exact<0>(A0, A1, A2, B0, B1, B2, ds, st);
-                hit = res == 1;
+                const int res = tt_part1<0>(A0, A1, A2, B0, B1, B2, ds, st);
                 defer = res == DEFERRED;
+                more = res == TT_MORE;
             }
             defer_tests(defer, make_uint4(kq, ga, gb, 0), defer_list, defer_cap, &ctr->deferred, lane, below);
-            stage_hits(hit, A0, A1, A2, B0, B1, B2, ds, kq, ta, tb, stage, staged, out, cap, ctr, lane, below);
+            queue_push(more, make_uint4(kq, ga, gb, ds), q2, n2, below);
+            if (n2 >= 64)
+                narrow_batch<2>(q2, n2, q3, n3, qp, tris, hits, cap, defer_list, defer_cap, ctr, stage, staged, lane,
+                                below);
+            if (n3 >= 64)
+                narrow_batch<3>(q3, n3, q3, n3, qp, tris, hits, cap, defer_list, defer_cap, ctr, stage, staged, lane,
+                                below);
         }
     }
+    while (n2 > 0) {
+        narrow_batch<2>(q2, n2, q3, n3, qp, tris, hits, cap, defer_list, defer_cap, ctr, stage, staged, lane, below);
+        if (n3 >= 64)
+            narrow_batch<3>(q3, n3, q3, n3, qp, tris, hits, cap, defer_list, defer_cap, ctr, stage, staged, lane,
+                            below);
+    }
+    while (n3 > 0)
+        narrow_batch<3>(q3, n3, q3, n3, qp, tris, hits, cap, defer_list, defer_cap, ctr, stage, staged, lane, below);
     if (staged) {
         __threadfence_block();
-        flush_contacts(out, cap, stage, staged, ctr, lane);
+        flush_hits(hits, cap, stage, staged, ctr, lane);
     }
     block_add(ntt, &ctr->tritests);
 }
@@ -855,14 +918,14 @@ template <int L>
 __global__ __launch_bounds__(256) void k_narrow_rerun(const uint4* __restrict__ list,
                                                       const unsigned long long* __restrict__ nlist_p,
                                                       uint64_t list_cap, const QueryPre* __restrict__ qp,
-                                                      const float* __restrict__ tris, cd_contact* __restrict__ out,
+                                                      const float* __restrict__ tris, uint4* __restrict__ hits,
                                                       uint64_t cap, uint4* __restrict__ next,
                                                       unsigned long long* __restrict__ nnext,
                                                       RunCounters* __restrict__ ctr) {
-    __shared__ cd_contact s_con[4][CSTAGE];
+    __shared__ uint4 s_hit[4][HSTAGE];
     const uint64_t n = min(*nlist_p, list_cap);
     const int lane = threadIdx.x & 63;
-    cd_contact* stage = s_con[threadIdx.x >> 6];
+    uint4* stage = s_hit[threadIdx.x >> 6];
     uint32_t staged = 0;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -871,15 +934,12 @@ __global__ __launch_bounds__(256) void k_narrow_rerun(const uint4* __restrict__ 
     for (uint64_t chunk = wave; chunk * 64 < n; chunk += nwaves) {
         const uint64_t li = chunk * 64 + lane;
         bool hit = false, defer = false;
-        f3 A0, A1, A2, B0, B1, B2;
-        uint32_t ds = 0, ta = 0, tb = 0;
+        uint32_t ds = 0;
         uint4 e = make_uint4(0, 0, 0, 0);
         if (li < n) {
             e = list[li];
-            const QueryPre& Q = qp[e.x];
-            ta = e.y - Q.tri_base_a;
-            tb = e.z - Q.tri_base_b;
-            narrow_load(Q, tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
+            f3 A0, A1, A2, B0, B1, B2;
+            narrow_load(qp[e.x], tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
             PredStats st;
             const int res = tri_tri_exact<L>(A0, A1, A2, B0, B1, B2, ds, st);
             hit = res == 1;
@@ -890,14 +950,29 @@ __global__ __launch_bounds__(256) void k_narrow_rerun(const uint4* __restrict__ 
             }
         }
         if (L < 2) defer_tests(defer, e, next, list_cap, nnext, lane, below);
-        stage_hits(hit, A0, A1, A2, B0, B1, B2, ds, e.x, ta, tb, stage, staged, out, cap, ctr, lane, below);
+        stage_hits(hit, make_uint4(e.x, e.y, e.z, ds), stage, staged, hits, cap, ctr, lane, below);
     }
     if (staged) {
         __threadfence_block();
-        flush_contacts(out, cap, stage, staged, ctr, lane);
+        flush_hits(hits, cap, stage, staged, ctr, lane);
     }
     block_add(nfp64, &ctr->fp64);
     block_add(nexact, &ctr->exact);
+}
+
+// Contact record of every hit record (record i -> contact i).
+__global__ __launch_bounds__(256) void k_contacts(const uint4* __restrict__ hits,
+                                                  const unsigned long long* __restrict__ nhits_p, uint64_t cap,
+                                                  const QueryPre* __restrict__ qp, const float* __restrict__ tris,
+                                                  cd_contact* __restrict__ out) {
+    const uint64_t n = min(*nhits_p, cap);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint4 e = hits[i];
+        const QueryPre& Q = qp[e.x];
+        f3 A0, A1, A2, B0, B1, B2;
+        narrow_load(Q, tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
+        write_contact(out + i, A0, A1, A2, B0, B1, B2, e.w, e.x, e.y - Q.tri_base_a, e.z - Q.tri_base_b);
+    }
 }
 
 // Exact-semantics stream of triangle pairs: flag 1 intersect (non-coplanar),
@@ -1087,14 +1162,16 @@ hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget
     return hipGetLastError();
 }
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
-                               const float* tris, cd_contact* out, uint64_t cap, uint4* defer, uint64_t defer_cap,
-                               RunCounters* ctr, uint32_t grid, uint32_t rerun_grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, defer,
+                               const float* tris, uint4* hits, cd_contact* out, uint64_t cap, uint4* defer,
+                               uint64_t defer_cap, RunCounters* ctr, uint32_t grid, uint32_t rerun_grid,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, hits, cap, defer,
                        defer_cap, ctr);
     hipLaunchKernelGGL(k_narrow_rerun<1>, dim3(rerun_grid), dim3(256), 0, st, defer, &ctr->deferred, defer_cap, qp,
-                       tris, out, cap, defer + defer_cap, &ctr->deferred2, ctr);
+                       tris, hits, cap, defer + defer_cap, &ctr->deferred2, ctr);
     hipLaunchKernelGGL(k_narrow_rerun<2>, dim3(rerun_grid), dim3(256), 0, st, defer + defer_cap, &ctr->deferred2,
-                       defer_cap, qp, tris, out, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
+                       defer_cap, qp, tris, hits, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
+    hipLaunchKernelGGL(k_contacts, dim3(grid), dim3(256), 0, st, hits, &ctr->contacts, cap, qp, tris, out);
     return hipGetLastError();
 }
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,

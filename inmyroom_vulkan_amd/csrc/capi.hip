@@ -191,8 +191,8 @@ constexpr uint32_t MAX_RESPLIT = 16;
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2 };
-static_assert(W_BUD2 < 40, "work slots");
+       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS };
+static_assert(W_HITS < 40, "work slots");
 
 // Count table of the multi-workgroup radix sort for n keys.
 hipError_t sort_table(cd_ctx* ctx, uint32_t n, uint32_t** table) {
@@ -280,8 +280,9 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
     HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
-                                 ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctx->d_work[W_DEFER].as<uint4>(),
-                                 ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu), st));
+                                 ctx->d_work[W_HITS].as<uint4>(), ctx->d_contacts.as<cd_contact>(), ctx->contact_cap,
+                                 ctx->d_work[W_DEFER].as<uint4>(), ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu),
+                                 st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
     return CD_OK;
 }
@@ -315,6 +316,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         }
         HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
+        HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
         std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
                                      t.split_tris, t.handoff_pct, t.resplit_levels, ctx->lp_cap, ctx->contact_cap,
@@ -323,7 +325,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
                                 &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER], &ctx->d_budget,
-                                &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2]})
+                                &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2], &ctx->d_work[W_HITS]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
             bool same = ctx->graph && key == ctx->graph_key;

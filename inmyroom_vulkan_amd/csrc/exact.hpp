@@ -209,8 +209,16 @@ __device__ __forceinline__ f3 pick3(f3 a, f3 b, f3 c, int i) { return i == 0 ? a
 constexpr int DEFERRED = -1;
 #define CD_DECIDE(v) \
     if (L < 2 && (v) == UNDECIDED) return DEFERRED
-template <int L = 2>
-__device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
+//
+// The test runs in three parts that streaming kernels can also run as
+// separate batches (most pairs end in the first or second part):
+//   tt_part1  T1's vertices against T2's plane: 0 (all on one side) or MORE;
+//   tt_part2  T2's vertices against T1's plane: 0, 2 (coplanar) or MORE;
+//   tt_part3  the canonical form and its two final orientations: 0 or 1.
+// Each may also return DEFERRED at a level below 2.
+constexpr int TT_MORE = 3;
+template <int L>
+__device__ __forceinline__ int tt_part1(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
     const int dp1 = orient_at<L>(p2, q2, r2, p1, st);
     CD_DECIDE(dp1);
     const int dq1 = orient_at<L>(p2, q2, r2, q1, st);
@@ -219,6 +227,11 @@ __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, ui
     CD_DECIDE(dr1);
     dsides = (uint32_t(dp1 + 1) << 6) | (uint32_t(dq1 + 1) << 8) | (uint32_t(dr1 + 1) << 10);
     if (dp1 * dq1 > 0 && dp1 * dr1 > 0) return 0;
+    return TT_MORE;
+}
+template <int L>
+__device__ __forceinline__ int tt_part2(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
+    const int dp1 = side_of(dsides, 3), dq1 = side_of(dsides, 4), dr1 = side_of(dsides, 5);
     const int dp2 = orient_at<L>(q1, r1, p1, p2, st);
     CD_DECIDE(dp2);
     const int dq2 = orient_at<L>(q1, r1, p1, q2, st);
@@ -228,6 +241,12 @@ __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, ui
     dsides |= uint32_t(dp2 + 1) | (uint32_t(dq2 + 1) << 2) | (uint32_t(dr2 + 1) << 4);
     if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
     if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
+    return TT_MORE;
+}
+template <int L>
+__device__ __forceinline__ int tt_part3(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t dsides, PredStats& st) {
+    const int dp1 = side_of(dsides, 3), dq1 = side_of(dsides, 4), dr1 = side_of(dsides, 5);
+    const int dp2 = side_of(dsides, 0), dq2 = side_of(dsides, 1), dr2 = side_of(dsides, 2);
     // stage 1: rotate T1 so that its first vertex is alone on its side of
     // T2's plane (rot1 = index of that vertex); swap2 exchanges q2 and r2
     int rot1, swap2;
@@ -290,6 +309,15 @@ __device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, ui
     return 1;
 }
 #undef CD_DECIDE
+
+template <int L = 2>
+__device__ inline int tri_tri_exact(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
+    int r = tt_part1<L>(p1, q1, r1, p2, q2, r2, dsides, st);
+    if (r != TT_MORE) return r;
+    r = tt_part2<L>(p1, q1, r1, p2, q2, r2, dsides, st);
+    if (r != TT_MORE) return r;
+    return tt_part3<L>(p1, q1, r1, p2, q2, r2, dsides, st);
+}
 
 // Contact segment of an intersecting, non-coplanar pair, with Möller's
 // interval construction (Triangle.cpp:764-1001) driven by the exact sides:
