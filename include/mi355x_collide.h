@@ -105,6 +105,26 @@ typedef struct cd_pair_rays {
     float avg_second[3];     /* average_point_second_modelspace (the second's model space) */
 } cd_pair_rays;
 
+/* Response parameters (ShootUncollideRays::ShootUncollideRays,
+ * src/CollisionDetection/ShootUncollideRays.cpp:4-12; CollisionDetection.cpp:21-24
+ * passes radians(40), radians(65), 1.01, which a NULL cd_shoot_params selects). */
+typedef struct cd_shoot_params {
+    float smoothstep_start_rads;   /* max_cos_from_force_response_smoothstep_start (an angle) */
+    float smoothstep_finish_rads;  /* ..._finish */
+    float distance_bias;           /* ray_distance_bias_multiplier */
+} cd_shoot_params;
+
+/* One entity pair's response (CollisionDetection.cpp:70-99). */
+typedef struct cd_pair_response {
+    uint32_t first, second;  /* entity indices of the last cd_frame */
+    uint32_t shot;           /* 0: neither entity moved since the previous frame (:80-81), deltas zero */
+    uint32_t pad;
+    float delta[3];          /* ExecuteShootUncollideRays: the uncollide delta, world space */
+    float delta_first[3];    /* CollisionCallbackData::deltaVector for the first entity: -delta * its share */
+    float delta_second[3];   /* ... for the second: +delta * its share */
+    float pad2[3];
+} cd_pair_response;
+
 typedef struct cd_mesh_info {
     uint32_t ntri, nnodes, depth, leaf_max;
     float root[12];          /* root OBB: centre, u, v, w half-extent sides */
@@ -243,6 +263,25 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
  * with face normals).  CD_E_CAPACITY writes the needed counts back.       */
 int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
                       uint64_t ray_cap, uint64_t* n_rays);
+
+/* Response of the last cd_frame, from the rays cd_uncollide_rays made:
+ * ShootUncollideRays::ExecuteShootUncollideRays
+ * (src/CollisionDetection/ShootUncollideRays.cpp:14-179, Ray::IntersectOBBtree
+ * src/Geometry/Ray.cpp:135-236) and the movement split of
+ * CollisionDetection::ExecuteCollisionDetection (CollisionDetection.cpp:70-99,
+ * PointMovementBetweenFrames :143-150).  Each ray is shot at the other
+ * entity's tree and, when it leaves it through a back face before re-entering
+ * its own, reflected and shot back; the pair's delta comes from the force
+ * responses and push-out vectors of all its rays, and is split between the
+ * entities by how far each one's average contact point moved since the
+ * previous frame (cd_entity.previous).  One record per pair that has rays, in
+ * cd_uncollide_rays' order.  glm::intersectRayTriangle's backface output
+ * (a fork of glm that is not available) is taken as the negative-determinant
+ * branch of glm 0.9.9's test.  The callbacks' fan-out to entity ancestors
+ * (:101-121) needs the engine's entity tree and is left to the caller.
+ * Params NULL: the reference's values.  CD_E_CAPACITY writes the count back. */
+int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* out, uint32_t cap,
+                  uint32_t* n_out);
 
 /* Vertex normals of a mesh for its rays (TriangleNormal::CreateTriangleNormalList,
  * src/Geometry/Triangle.cpp:214-234): nvert x 3 floats indexed like the

@@ -6,8 +6,8 @@ Möller triangle-triangle test with contact output, sweep-and-prune broad
 phase, contact aggregation into uncollide rays, skinning and refit.  ``collide.Collider`` is the Python binding; ``scenes`` generates the
 synthetic workloads.
 """
-from .collide import (CONTACT_DTYPE, ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, PAIR_DTYPE, PAIR_RAYS_DTYPE, QUERY_DTYPE,
-                      RAY_DTYPE, CollideError, Collider, load_library)
+from .collide import (CONTACT_DTYPE, ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, PAIR_DTYPE, PAIR_RAYS_DTYPE,
+                      PAIR_RESPONSE_DTYPE, QUERY_DTYPE, RAY_DTYPE, CollideError, Collider, load_library)
 
 __all__ = ["Collider", "CollideError", "load_library", "CONTACT_DTYPE", "QUERY_DTYPE", "ENTITY_DTYPE", "PAIR_DTYPE",
-           "RAY_DTYPE", "PAIR_RAYS_DTYPE", "MODE_PLAIN", "MODE_BATCHED"]
+           "RAY_DTYPE", "PAIR_RAYS_DTYPE", "PAIR_RESPONSE_DTYPE", "MODE_PLAIN", "MODE_BATCHED"]

@@ -7,6 +7,7 @@ no CPU fallback anywhere behind this interface.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -29,7 +30,7 @@ EXPORTS = (
     "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
     "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
-    "cd_uncollide_rays", "cd_set_mesh_normals",
+    "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -47,8 +48,17 @@ PAIR_RAYS_DTYPE = np.dtype([
     ("query", "<u4"), ("first", "<u4"), ("second", "<u4"), ("n_first", "<u4"), ("n_second", "<u4"), ("pad", "<u4"),
     ("begin_first", "<u8"), ("begin_second", "<u8"), ("avg_first", "<f4", 3), ("avg_second", "<f4", 3),
 ])
+PAIR_RESPONSE_DTYPE = np.dtype([
+    ("first", "<u4"), ("second", "<u4"), ("shot", "<u4"), ("pad", "<u4"), ("delta", "<f4", 3),
+    ("delta_first", "<f4", 3), ("delta_second", "<f4", 3), ("pad2", "<f4", 3),
+])
 assert CONTACT_DTYPE.itemsize == 64 and QUERY_DTYPE.itemsize == 72 and ENTITY_DTYPE.itemsize == 136
-assert RAY_DTYPE.itemsize == 24 and PAIR_RAYS_DTYPE.itemsize == 64
+assert RAY_DTYPE.itemsize == 24 and PAIR_RAYS_DTYPE.itemsize == 64 and PAIR_RESPONSE_DTYPE.itemsize == 64
+
+
+class cd_shoot_params(C.Structure):
+    _fields_ = [("smoothstep_start_rads", C.c_float), ("smoothstep_finish_rads", C.c_float),
+                ("distance_bias", C.c_float)]
 
 
 class cd_config(C.Structure):
@@ -113,6 +123,7 @@ def load_library() -> C.CDLL:
     lib.cd_run_staged.argtypes = [vp, u32, P(u64)]
     lib.cd_fetch_contacts.argtypes = [vp, vp, u64, P(u64)]
     lib.cd_uncollide_rays.argtypes = [vp, vp, u32, P(u32), vp, u64, P(u64)]
+    lib.cd_shoot_rays.argtypes = [vp, P(cd_shoot_params), vp, u32, P(u32)]
     lib.cd_set_mesh_normals.argtypes = [vp, u32, vp, u32]
     lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
     lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
@@ -315,6 +326,26 @@ class Collider:
                 continue
             self._check(rc, "cd_uncollide_rays")
             return pairs[: npairs.value], rays[: nrays.value]
+
+    def shoot_rays(self, start_rads=None, finish_rads=None, bias=None):
+        """Response of the last frame's pairs with rays (PAIR_RESPONSE_DTYPE);
+        after frame() and uncollide_rays().  No arguments: the reference's
+        smoother-step angles (40, 65 degrees) and distance bias (1.01)."""
+        p = None
+        if start_rads is not None or finish_rads is not None or bias is not None:
+            p = C.byref(cd_shoot_params(math.radians(40.0) if start_rads is None else start_rads,
+                                        math.radians(65.0) if finish_rads is None else finish_rads,
+                                        1.01 if bias is None else bias))
+        cap = 1 << 10
+        while True:
+            out = np.zeros(cap, PAIR_RESPONSE_DTYPE)
+            n = C.c_uint32()
+            rc = self.lib.cd_shoot_rays(self.h, p, _ptr(out), cap, C.byref(n))
+            if rc == CD_E_CAPACITY:
+                cap = max(cap, int(n.value))
+                continue
+            self._check(rc, "cd_shoot_rays")
+            return out[: n.value]
 
     def set_mesh_normals(self, mesh: int, normals: np.ndarray) -> None:
         n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)

@@ -191,8 +191,8 @@ constexpr uint32_t MAX_RESPLIT = 16;
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS };
-static_assert(W_HITS < 40, "work slots");
+       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS };
+static_assert(W_PIDS < 40, "work slots");
 
 // Count table of the multi-workgroup radix sort for n keys.
 hipError_t sort_table(cd_ctx* ctx, uint32_t n, uint32_t** table) {
@@ -412,6 +412,7 @@ int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts);
 int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
     ctx->last_run = 0;
     ctx->contacts_sorted = false;
+    ctx->rays_ready = false;
     int rc = run_queries_(ctx, nq, mode, n_contacts);
     if (rc == CD_OK) ctx->last_run = 1;
     return rc;
@@ -434,6 +435,7 @@ int stage(cd_ctx* ctx, const cd_query* q, uint32_t nq) {
     ctx->staged_ids.resize(nq);
     for (uint32_t i = 0; i < nq; ++i) ctx->staged_ids[i] = make_uint2(qr[i].mesh_a, qr[i].mesh_b);
     ctx->last_run = 0;
+    ctx->rays_ready = false;
     return CD_OK;
 }
 
@@ -803,8 +805,12 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
     }
     uint32_t np = 0;
     std::vector<uint2> hp;
+    ctx->last_run = 0;
+    ctx->rays_ready = false;
     int rc = broad_phase(ctx, entities, n, &np, &hp);
     if (rc) return rc;
+    ctx->frame_mesh.resize(n);
+    for (uint32_t i = 0; i < n; ++i) ctx->frame_mesh[i] = entities[i].mesh_id;
     const float broad_ms = ctx->timings.broad_ms;
     if (np == 0) {
         ctx->counters = cd_counters{};
@@ -882,7 +888,13 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
     hipStream_t st = ctx->stream;
     const uint32_t nq = ctx->staged_nq;
     const uint64_t nc64 = ctx->staged_contacts;
-    if (nc64 == 0 || nq == 0) return CD_OK;
+    ctx->rays_ready = false;
+    if (nc64 == 0 || nq == 0) {
+        ctx->ray_total = 0;
+        ctx->ray_counts.assign(2 * size_t(nq), 0u);
+        ctx->rays_ready = true;
+        return CD_OK;
+    }
     if (nc64 > 0x3FFFFFFFull) return fail(ctx, CD_E_CAPACITY, "too many contacts for ray generation");
     const uint32_t nc = uint32_t(nc64);
     for (int w : {W_CSORT}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(cd_contact)));
@@ -980,12 +992,82 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
         out.push_back(r);
     }
     const uint64_t total = uint64_t(side_rays[0]) + side_rays[1];
+    ctx->ray_base1 = side_rays[0];
+    ctx->ray_total = total;
+    ctx->ray_counts.resize(2 * size_t(nq));
+    for (uint32_t q = 0; q < nq; ++q) {
+        ctx->ray_counts[q] = hq[q];
+        ctx->ray_counts[size_t(nq) + q] = hq[2 * size_t(nq) + q];
+    }
+    ctx->rays_ready = true;
     *n_pairs = uint32_t(out.size());
     *n_rays = total;
     if (out.size() > pair_cap || total > ray_cap || (total && (!pairs || !rays)))
         return fail(ctx, CD_E_CAPACITY, "pair or ray buffer too small");
     if (!out.empty()) std::memcpy(pairs, out.data(), out.size() * sizeof(cd_pair_rays));
     if (total) HCK(ctx, hipMemcpy(rays, ctx->d_work[W_RAYS].p, total * sizeof(cd_ray), hipMemcpyDeviceToHost));
+    return CD_OK;
+}
+
+// ---- response (ShootUncollideRays.cpp:14-179, CollisionDetection.cpp:70-99) ----
+int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
+    if (!ctx || !n_out) return CD_E_INVALID;
+    *n_out = 0;
+    if (ctx->last_run != 2) return fail(ctx, CD_E_INVALID, "ray shooting needs the entity matrices of a cd_frame");
+    if (!ctx->rays_ready) return fail(ctx, CD_E_INVALID, "no rays: call cd_uncollide_rays after cd_frame");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const uint32_t nq = ctx->staged_nq;
+    uint32_t npairs = 0;
+    for (uint32_t q = 0; q < nq; ++q) npairs += (ctx->ray_counts[q] + ctx->ray_counts[size_t(nq) + q]) ? 1u : 0u;
+    *n_out = npairs;
+    if (npairs == 0) return CD_OK;
+    if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
+    for (uint32_t q = 0; q < nq; ++q)  // the closest-hit stack holds a tree's depth + 2 entries
+        for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y})
+            if (ctx->meshes[ctx->frame_mesh[e]].depth + 2 > uint32_t(RAY_STACK_ENTRIES))
+                return fail(ctx, CD_E_STACK, "tree too deep for the ray traversal stack");
+    // radians and cosines as the reference computes them (glm::radians, std::cos on float)
+    const float deg = 0.01745329251994329576923690768489f;
+    const float a0 = params ? params->smoothstep_start_rads : 40.f * deg;
+    const float a1 = params ? params->smoothstep_finish_rads : 65.f * deg;
+    const float bias = params ? params->distance_bias : 1.01f;
+    const float cos_start = std::cos(a0), cos_finish = std::cos(a1);
+    HCK(ctx, ctx->d_work[W_SHOTS].ensure(std::max<uint64_t>(ctx->ray_total, 1) * sizeof(RayShot)));
+    HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(nq) * sizeof(cd_pair_response)));
+    // pair ids, then per side the first ray of every pair (an exclusive scan
+    // of the counts: the device keeps a begin only for pairs with rays)
+    std::vector<uint32_t> up(4 * size_t(nq));
+    for (uint32_t q = 0; q < nq; ++q) {
+        up[2 * size_t(q)] = ctx->staged_ids[q].x;
+        up[2 * size_t(q) + 1] = ctx->staged_ids[q].y;
+    }
+    for (int side = 0; side < 2; ++side) {
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < nq; ++q) {
+            up[2 * size_t(nq) + size_t(nq) * side + q] = run;
+            run += ctx->ray_counts[size_t(nq) * side + q];
+        }
+    }
+    HCK(ctx, ctx->d_work[W_PIDS].ensure(up.size() * sizeof(uint32_t)));
+    HCK(ctx, hipMemcpyAsync(ctx->d_work[W_PIDS].p, up.data(), up.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    const uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();
+    const uint32_t* pbegin = ctx->d_work[W_PIDS].as<uint32_t>() + 2 * size_t(nq);
+    HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
+                          ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
+                          ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                          ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(), st));
+    HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
+                            ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
+                            ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,
+                            ctx->d_work[W_RESP].as<cd_pair_response>(), st));
+    std::vector<cd_pair_response> all(nq);
+    HCK(ctx, hipMemcpyAsync(all.data(), ctx->d_work[W_RESP].p, size_t(nq) * sizeof(cd_pair_response),
+                            hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    uint32_t k = 0;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (ctx->ray_counts[q] + ctx->ray_counts[size_t(nq) + q]) out[k++] = all[q];
     return CD_OK;
 }
 

@@ -75,6 +75,16 @@ struct RayCand {
 };
 static_assert(sizeof(RayCand) == 32, "candidate layout");
 
+// One uncollide ray's two Hermann passes (the second shoots the reflected
+// ray back): each pass's contribution to the summed force response and its
+// push-out vector, as the response step accumulates them.
+struct RayShot {
+    float force[2][3], push[2][3];
+    uint32_t ok;    // bit p: pass p succeeded
+    uint32_t pair;  // query (entity pair) of the ray
+};
+constexpr int RAY_STACK_ENTRIES = 64;  // closest-hit traversal stack: a tree's depth + 2
+
 // A refit plan: small boxes (one thread each) and large boxes split into
 // chunks of triangles (device arrays, cached per mesh list).
 struct RefitPlan {
@@ -209,6 +219,12 @@ struct cd_ctx {
     int last_run = 0;                      // 0 none, 1 staged queries, 2 frame
     bool contacts_sorted = false;          // sorted copy in d_work[W_CSORT]
     std::vector<uint2> staged_ids;         // per query: (mesh_a, mesh_b), or per frame pair: (first, second)
+    std::vector<uint32_t> frame_mesh;      // mesh of each entity of the last cd_frame
+    // rays of the last cd_uncollide_rays, still on the device, for cd_shoot_rays
+    bool rays_ready = false;
+    uint32_t ray_base1 = 0;                // first ray of the second sides
+    uint64_t ray_total = 0;
+    std::vector<uint32_t> ray_counts;      // per query: rays of the first side, then (at +nq) of the second
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
     cdk::HostBuf h_stage;  // pinned staging of results copied back to the host

@@ -1,0 +1,418 @@
+// Response: shooting the uncollide rays and splitting the delta
+// (ShootUncollideRays::ExecuteShootUncollideRays,
+//  inMyRoom_vulkan/src/CollisionDetection/ShootUncollideRays.cpp:14-179;
+//  Ray::IntersectOBBtree, Ray.cpp:13-236; CollisionDetection.cpp:70-99, 143-150).
+//
+//  k_shoot    one lane per uncollide ray.  Pass 1 shoots the ray at the other
+//             entity's tree (closest hit); it succeeds when that hit is a
+//             back face reached before the ray re-enters its own tree (the
+//             Hermann pass, :129-158).  On success the ray is reflected (origin
+//             moved by the response, direction minus the hit's normal,
+//             :104-110) and pass 2 shoots it the other way.  Each pass's force
+//             response and push-out vector are kept per ray.
+//  k_respond  one lane per entity pair: force responses summed in the
+//             reference's ray order (rays from the first entity, each with its
+//             reflected pass, then those from the second), FindResponse
+//             (:160-169) over the push-out vectors, the world-space delta, and
+//             its split by how far each entity's average contact point moved
+//             since the previous frame.
+//
+// The closest-hit traversal keeps an explicit stack.  At an internal node
+// both children's entry/exit distances are computed, the farther child is
+// pushed below the nearer, and a child is entered only if, when it is
+// popped, its entry distance is below the best hit so far and its exit
+// distance is not negative: the nodes visited, and the order in which equal
+// distances are met, are those of the reference's recursion.
+#include "rayattr.hpp"
+
+namespace cdk {
+
+namespace {
+
+constexpr float FLT_EPS = 1.1920928955078125e-07f;  // std::numeric_limits<float>::epsilon()
+constexpr int RAY_STACK = RAY_STACK_ENTRIES;  // holds a tree's depth + 2 entries
+
+__device__ __forceinline__ float max_std(float a, float b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ float min_std(float a, float b) { return (b < a) ? b : a; }  // std::min
+
+// glm::intersectRayTriangle from the origin (glm 0.9.9's Möller-Trumbore
+// form); back = the negative-determinant branch, the backface output of the
+// reference's glm fork.
+__device__ __forceinline__ bool ray_triangle(f3 dir, f3 v0, f3 v1, f3 v2, float& bx, float& by, float& dist,
+                                             bool& back) {
+    const f3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    const f3 p = cross(dir, e2);
+    const float det = dot(e1, p);
+    const f3 d0 = sub(mk(0.f, 0.f, 0.f), v0);
+    f3 perp;
+    if (det > FLT_EPS) {
+        bx = dot(d0, p);
+        if (bx < 0.f || bx > det) return false;
+        perp = cross(d0, e1);
+        by = dot(dir, perp);
+        if (by < 0.f || bx + by > det) return false;
+        back = false;
+    } else if (det < -FLT_EPS) {
+        bx = dot(d0, p);
+        if (bx > 0.f || bx < det) return false;
+        perp = cross(d0, e1);
+        by = dot(dir, perp);
+        if (by > 0.f || bx + by < det) return false;
+        back = true;
+    } else {
+        return false;
+    }
+    const float inv = 1.f / det;
+    dist = dot(e2, perp) * inv;
+    bx = bx * inv;
+    by = by * inv;
+    return true;
+}
+
+// Ray::IntersectParalgram from the origin (Ray.cpp:36-133): the three slabs
+// of the box, normals v x w, w x u, u x v.
+__device__ __forceinline__ bool ray_box(f3 dir, const Box& b, float& tmin, float& tmax) {
+    tmin = -__builtin_huge_valf();
+    tmax = __builtin_huge_valf();
+    const f3 ro = sub(mk(0.f, 0.f, 0.f), b.c);
+    const f3 side[3] = {b.u, b.v, b.w};
+    const f3 pa[3] = {b.v, b.w, b.u}, pb[3] = {b.w, b.u, b.v};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 n = normalize(cross(pa[k], pb[k]));
+        const float d = -fabsf(dot(n, side[k]));
+        const float hc = dot(n, ro);
+        const float vn1 = hc + d;
+        const float vn2 = -hc + d;
+        const float vd = dot(n, dir);
+        if (fabsf(vd) >= FLT_EPS) {
+            const float inv = 1.f / vd;
+            float t1 = -vn1 * inv;
+            float t2 = vn2 * inv;
+            if (t1 > t2) {
+                const float t = t1;
+                t1 = t2;
+                t2 = t;
+            }
+            tmin = max_std(t1, tmin);
+            tmax = min_std(t2, tmax);
+            if (tmin > tmax || tmax < 0.f) return false;
+        } else if (vn1 > 0.f || vn2 > 0.f) {
+            return false;
+        }
+    }
+    return true;
+}
+
+struct Hit {
+    bool hit, back;
+    float dist, bx, by;
+    uint32_t tri;  // mesh-local tree-order slot
+};
+
+struct StackEntry {
+    uint32_t link, cnt;  // cnt 0: internal node `link`; else a leaf of cnt triangles from `link`
+    float t0, t1;
+};
+
+__device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, const M4& C, f3 dir, uint32_t off,
+                          uint32_t cnt, Hit& best) {
+    for (uint32_t i = off; i != off + cnt; ++i) {
+        const float* t9 = tris + (size_t(m.tri_base) + i) * 9;
+        const f3 P0 = mul4(C, mk(t9[0], t9[1], t9[2]), 1.f);
+        const f3 P1 = mul4(C, mk(t9[3], t9[4], t9[5]), 1.f);
+        const f3 P2 = mul4(C, mk(t9[6], t9[7], t9[8]), 1.f);
+        float bx = 0.f, by = 0.f, dist = __builtin_huge_valf();
+        bool back = false;
+        if (ray_triangle(dir, P0, P1, P2, bx, by, dist, back) && dist > 0.f && dist < best.dist) {
+            best.hit = true;
+            best.back = back;
+            best.dist = dist;
+            best.tri = i;
+            best.bx = bx;
+            best.by = by;
+        }
+    }
+}
+
+// Closest hit of the ray (o, dir) on mesh m moved by M (Ray.cpp:135-236).
+__device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
+                        f3 o, f3 dir) {
+    C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
+    C.m[13] = C.m[13] - o.y;
+    C.m[14] = C.m[14] - o.z;
+    C.m[15] = C.m[15] - 0.f;
+    Hit best = {false, false, __builtin_huge_valf(), 0.f, 0.f, 0xFFFFFFFFu};
+    float r0, r1;
+    if (!(ray_box(dir, xform_box(C, box_from12(m.root)), r0, r1) && r1 >= 0.f)) return best;
+    if (m.nnodes == 0) {  // the root is a leaf
+        leaf_hits(tris, m, C, dir, 0, m.ntri, best);
+        return best;
+    }
+    const NodeRec* nb = nodes + m.node_base;
+    StackEntry st[RAY_STACK];
+    int sp = 0;
+    uint32_t node = 0;  // the root, entered without a distance check
+    for (;;) {
+        const NodeRec& n = nb[node];
+        const uint32_t cl = n.cnt & 0xFFFFu, cr = n.cnt >> 16;
+        float l0, l1, q0, q1;
+        const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1);
+        const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1);
+        const StackEntry L = {n.link[0], cl, l0, l1}, R = {n.link[1], cr, q0, q1};
+        if (sp + 2 > RAY_STACK) {  // deeper than the host allows (it checks tree depths first)
+            best.hit = false;
+            break;
+        }
+        if (lh && rh) {  // the farther below the nearer (Ray.cpp:175-199)
+            if (l0 < q0) {
+                st[sp++] = R;
+                st[sp++] = L;
+            } else {
+                st[sp++] = L;
+                st[sp++] = R;
+            }
+        } else if (lh) {
+            st[sp++] = L;
+        } else if (rh) {
+            st[sp++] = R;
+        }
+        // next internal node to enter; leaves are tested as they are popped
+        bool found = false;
+        while (sp > 0) {
+            const StackEntry e = st[--sp];
+            if (!(e.t0 < best.dist && e.t1 >= 0.f)) continue;
+            if (e.cnt == 0u) {
+                node = e.link;
+                found = true;
+                break;
+            }
+            leaf_hits(tris, m, C, dir, e.link, e.cnt, best);
+        }
+        if (!found) break;
+    }
+    return best;
+}
+
+// Ray::MoveOriginEpsilonTowardsDirection (Ray.cpp:13-21).
+__device__ __forceinline__ f3 move_eps(f3 o, f3 d, float factor) {
+    float big = fabsf(o.x);
+    if (big < fabsf(o.y)) big = fabsf(o.y);
+    if (big < fabsf(o.z)) big = fabsf(o.z);
+    const float e = big * FLT_EPS;
+    return add(o, scale(d, factor * e));
+}
+
+struct Pass {
+    bool ok;
+    f3 response, normal;
+};
+
+struct MeshView {
+    const MeshRec* meshes;
+    const NodeRec* nodes;
+    const float* tris;
+    const uint2* attrs;
+    const uint32_t* vid;
+    const float* nrm;
+    const uint32_t* perm;
+};
+
+// HermannPass (ShootUncollideRays.cpp:129-158).
+__device__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
+                        const M3& Bn, f3 o, f3 d) {
+    Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
+    const MeshRec mb = V.meshes[meshB];
+    const Hit p2 = ray_tree(mb, V.nodes, V.tris, Bmat, o, d);
+    if (!(p2.hit && p2.back)) return r;
+    const f3 o2 = move_eps(o, d, 4.f);
+    const float eps = vlength(sub(o2, o));
+    const Hit p3 = ray_tree(V.meshes[meshA], V.nodes, V.tris, Amat, o2, d);
+    if (p2.dist <= p3.dist + eps) {
+        r.ok = true;
+        r.response = scale(d, p2.dist);
+        const uint2 attr = V.attrs[meshB];
+        const float* t9 = V.tris + (size_t(mb.tri_base) + p2.tri) * 9;
+        f3 nk[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nk[k] = corner_normal(attr, V.nrm, vertex_id(mb, attr, V.vid, V.perm, p2.tri, k), t9);
+        const f3 ni = add(add(scale(nk[0], (1.f - p2.bx) - p2.by), scale(nk[1], p2.bx)), scale(nk[2], p2.by));
+        r.normal = normalize(mul3(Bn, ni));
+    }
+    return r;
+}
+
+// CalcForceResponse (:118-127).
+__device__ __forceinline__ f3 force_response(const Pass& p) {
+    const f3 dir = normalize(p.response);
+    const float len = vlength(p.response);
+    const float c = dot(p.normal, dir);
+    const float c2 = c * c;
+    return scale(dir, c2 * len);
+}
+
+__device__ __forceinline__ f3 neg(f3 v) { return mk(-v.x, -v.y, -v.z); }  // glm's unary minus
+__device__ __forceinline__ void put3(float* o, f3 v) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+__device__ __forceinline__ f3 get3(const float* o) { return mk(o[0], o[1], o[2]); }
+
+}  // namespace
+
+// Per-ray result: for each of the two passes, the contribution to the summed
+// force response and the push-out vector, as the reference accumulates them
+// (a first-to-second pass subtracts its force and pushes -response, a
+// second-to-first pass adds and pushes +response).
+// pbegin: per side, the first ray of every pair (non-decreasing; a pair
+// without rays has the next pair's begin)
+__global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, uint32_t nrays, uint32_t base1,
+                                              const uint32_t* __restrict__ pbegin, uint32_t nq,
+                                              const QueryRec* __restrict__ qs, MeshView V,
+                                              RayShot* __restrict__ out) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrays) return;
+    const int side = r >= base1 ? 1 : 0;
+    const uint32_t local = r - (side ? base1 : 0u);
+    const uint32_t* begin = pbegin + size_t(nq) * side;
+    // the pair: the last one whose rays begin at or before this ray (of equal
+    // begins only the last can hold rays)
+    uint32_t lo = 0, hi = nq;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (begin[mid] <= local) lo = mid;
+        else hi = mid;
+    }
+    const QueryRec q = qs[lo];
+    M4 M, I;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        M.m[e] = q.rel[e];
+        I.m[e] = (e % 5 == 0) ? 1.f : 0.f;
+    }
+    const M3 Mn = adjoint_transpose(q.rel);
+    M3 I3;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) I3.m[e] = (e % 4 == 0) ? 1.f : 0.f;
+    RayShot s = {};
+    f3 o = get3(rays[r].origin), d = get3(rays[r].direction);
+    // side 0: first-to-second, then the reflected ray second-to-first; side 1 the other way round
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool f2s = (pass == 0) == (side == 0);
+        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d)
+                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d);
+        if (!p.ok) break;
+        const f3 F = force_response(p);
+        put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
+        put3(s.push[pass], f2s ? neg(p.response) : p.response);
+        s.ok |= 1u << pass;
+        o = add(o, p.response);  // ReflectHermannResult (:104-110)
+        d = neg(p.normal);
+    }
+    s.pair = lo;
+    out[r] = s;
+}
+
+__device__ __forceinline__ float smoother_step(float a, float b, float x) {  // :171-175
+    float t = (x - a) / (b - a);
+    t = (t < 0.f) ? 0.f : ((1.f < t) ? 1.f : t);
+    return t * t * t * (t * (t * 6.f - 15.f) + 10.f);
+}
+__device__ __forceinline__ bool same_matrix(const float* a, const float* b) {
+    for (int i = 0; i < 16; ++i)
+        if (!(a[i] == b[i])) return false;
+    return true;
+}
+
+__global__ void k_respond(const RayShot* __restrict__ shots, uint32_t base1, const uint32_t* __restrict__ qw,
+                          uint32_t nq, const float* __restrict__ avg, const QueryRec* __restrict__ qs,
+                          const uint2* __restrict__ pair_ids, const cd_entity* __restrict__ ents,
+                          float cos_start, float cos_finish, float bias, cd_pair_response* __restrict__ out) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint2 ids = pair_ids[q];
+    cd_pair_response res = {};
+    res.first = ids.x;
+    res.second = ids.y;
+    const float* c1 = ents[ids.x].current;
+    const float* c2 = ents[ids.y].current;
+    const float* p1 = ents[ids.x].previous;
+    const float* p2 = ents[ids.y].previous;
+    const uint32_t n0 = qw[q], b0 = qw[nq + q];
+    const uint32_t n1 = qw[2 * size_t(nq) + q], b1 = base1 + qw[3 * size_t(nq) + q];
+    if ((n0 == 0 && n1 == 0) || (same_matrix(c1, p1) && same_matrix(c2, p2))) {  // :63, :80-81
+        out[q] = res;
+        return;
+    }
+    res.shot = 1;
+    // the summed force responses, in the reference's order
+    f3 sum = mk(0.f, 0.f, 0.f);
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t b = side ? b1 : b0, n = side ? n1 : n0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const RayShot& s = shots[b + i];
+            for (int pass = 0; pass < 2; ++pass)
+                if (s.ok & (1u << pass)) sum = add(sum, get3(s.force[pass]));
+        }
+    }
+    f3 delta = mk(0.f, 0.f, 0.f);
+    if (!(sum.x == 0.f && sum.y == 0.f && sum.z == 0.f)) {  // :87
+        const f3 nf = normalize(sum);
+        float max_len = 0.f;  // FindResponse (:160-169)
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t b = side ? b1 : b0, n = side ? n1 : n0;
+            for (uint32_t i = 0; i < n; ++i) {
+                const RayShot& s = shots[b + i];
+                for (int pass = 0; pass < 2; ++pass) {
+                    if (!(s.ok & (1u << pass))) continue;
+                    const f3 r = get3(s.push[pass]);
+                    const float c = dot(normalize(r), nf);
+                    const float need = vlength(r) / c;
+                    max_len = max_std(max_len, smoother_step(cos_finish, cos_start, c) * need);
+                }
+            }
+        }
+        M4 F;
+        for (int e = 0; e < 16; ++e) F.m[e] = c1[e];
+        delta = mul4(F, scale(scale(nf, max_len), bias), 0.f);
+    }
+    // the split by each entity's movement (CollisionDetection.cpp:86-97, 143-150)
+    M4 C1, P1, C2, P2;
+    for (int e = 0; e < 16; ++e) {
+        C1.m[e] = c1[e];
+        P1.m[e] = p1[e];
+        C2.m[e] = c2[e];
+        P2.m[e] = p2[e];
+    }
+    const f3 a1 = get3(avg + 3 * size_t(q)), a2 = get3(avg + 3 * (size_t(nq) + q));
+    const float m1 = vlength(sub(mul4(C1, a1, 1.f), mul4(P1, a1, 1.f)));
+    const float m2 = vlength(sub(mul4(C2, a2, 1.f), mul4(P2, a2, 1.f)));
+    const float total = m1 + m2;
+    put3(res.delta, delta);
+    put3(res.delta_first, scale(neg(delta), m1 / total));
+    put3(res.delta_second, scale(delta, m2 / total));
+    out[q] = res;
+}
+
+hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
+                        const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
+                        const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
+                        hipStream_t st) {
+    if (!nrays) return hipSuccess;
+    const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
+    hipLaunchKernelGGL(k_shoot, dim3((nrays + 63) / 64), dim3(64), 0, st, rays, nrays, base1, pbegin, nq, qs, V,
+                       shots);
+    return hipGetLastError();
+}
+hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
+                          const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
+                          float cos_finish, float bias, cd_pair_response* out, hipStream_t st) {
+    if (!nq) return hipSuccess;
+    hipLaunchKernelGGL(k_respond, dim3((nq + 63) / 64), dim3(64), 0, st, shots, base1, qw, nq, avg, qs, pair_ids, ents,
+                       cos_start, cos_finish, bias, out);
+    return hipGetLastError();
+}
+
+}  // namespace cdk

@@ -103,6 +103,16 @@ class Oracle:
         lib.orc_rays_count.restype = C.c_int64
         lib.orc_rays_get.argtypes = [vp, _U32, _F, _F]
         lib.orc_rays_free.argtypes = [vp]
+        lib.orc_ray_triangle.argtypes = [_F, _F, _F]
+        lib.orc_ray_box.argtypes = [_F, _F, _F]
+        lib.orc_ray_box.restype = C.c_int
+        lib.orc_ray_tree.argtypes = [vp, _F, _F, _F, _F]
+        lib.orc_shoot_frame.argtypes = [C.POINTER(vp), _U32, _F, _F, _U8, C.c_int64, C.c_float, C.c_float,
+                                        C.c_float]
+        lib.orc_shoot_frame.restype = vp
+        lib.orc_shoot_get.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_uint64)]
+        lib.orc_shoot_get.restype = C.c_int64
+        lib.orc_shoot_free.argtypes = [vp]
 
     # -- small functions -------------------------------------------------
     def eig3(self, A):
@@ -204,6 +214,47 @@ class Oracle:
                                     np.ascontiguousarray(cur16, np.float32).reshape(-1),
                                     np.ascontiguousarray(should_cb, np.uint8), len(mesh_of))
         return OracleRays(self, r)
+
+
+    # -- response: shooting the uncollide rays (ShootUncollideRays.cpp, Ray.cpp) --
+    def ray_triangle(self, direction, tri9):
+        """(hit, backface, distance, bary_u, bary_v) of a ray from the origin."""
+        out = np.zeros(6, np.float32)
+        self.lib.orc_ray_triangle(np.ascontiguousarray(direction, np.float32), np.ascontiguousarray(tri9, np.float32),
+                                  out)
+        return bool(out[0]), bool(out[1]), float(out[2]), float(out[3]), float(out[4])
+
+    def ray_box(self, direction, box12):
+        """(hit, t_enter, t_exit) of a ray from the origin through a box (centre, half-axes)."""
+        out = np.zeros(2, np.float32)
+        hit = self.lib.orc_ray_box(np.ascontiguousarray(direction, np.float32), np.ascontiguousarray(box12, np.float32),
+                                   out)
+        return bool(hit), float(out[0]), float(out[1])
+
+    def ray_tree(self, tree, m16, origin, direction):
+        """Closest hit (hit, backface, distance, tree-order triangle or -1, u, v) of a ray on a moved tree."""
+        out = np.zeros(8, np.float32)
+        self.lib.orc_ray_tree(tree.h, np.ascontiguousarray(m16, np.float32).reshape(-1),
+                              np.ascontiguousarray(origin, np.float32), np.ascontiguousarray(direction, np.float32), out)
+        return bool(out[0]), bool(out[1]), float(out[2]), int(out[3]), float(out[4]), float(out[5])
+
+    def shoot_frame(self, trees, mesh_of, cur16, prev16, should_cb, cos_start, cos_finish, bias):
+        """Per pair with rays: ids (first, second, shot) and deltas (world delta,
+        first's share, second's share) (CollisionDetection.cpp:70-99), plus
+        (passes, successful passes)."""
+        handles = (C.c_void_p * len(trees))(*[t.h for t in trees])
+        h = self.lib.orc_shoot_frame(handles, np.ascontiguousarray(mesh_of, np.uint32),
+                                     np.ascontiguousarray(cur16, np.float32).reshape(-1),
+                                     np.ascontiguousarray(prev16, np.float32).reshape(-1),
+                                     np.ascontiguousarray(should_cb, np.uint8), len(mesh_of), cos_start, cos_finish, bias)
+        n = int(self.lib.orc_shoot_get(h, None, None, None))
+        ids = np.zeros((n, 3), np.uint32)
+        d = np.zeros((n, 3, 3), np.float32)
+        st = np.zeros(2, np.uint64)
+        self.lib.orc_shoot_get(h, ids.ctypes.data_as(C.POINTER(C.c_uint32)), d.ctypes.data_as(C.POINTER(C.c_float)),
+                               st.ctypes.data_as(C.POINTER(C.c_uint64)))
+        self.lib.orc_shoot_free(h)
+        return ids, d, (int(st[0]), int(st[1]))
 
 
 class OracleRays:

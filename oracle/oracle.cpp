@@ -1005,6 +1005,269 @@ PairRays uncollide_rays(const Tree& A, const Tree& B, const M4& M, const std::ve
 // Tree attributes for rays (Triangle.cpp:214-280): vertex normals through the
 // index buffer, or the face normal when the mesh has none; vertex ids are the
 // index-buffer values (a soup's are 3 t + k).
+// ---------------------------------------------------------------------------
+// Response: shooting the uncollide rays (ShootUncollideRays.cpp:14-179,
+// Ray.cpp:13-236) and the movement split (CollisionDetection.cpp:80-99,
+// :143-150).
+// ---------------------------------------------------------------------------
+
+// glm::intersectRayTriangle with the ray origin at 0 (every call in the
+// reference is made with a centred ray, Ray.cpp:137-155).  glm 0.9.9's
+// Möller-Trumbore form; the reference's glm fork adds a backface output,
+// restated here as the negative-determinant branch (the ray leaves through
+// the triangle's back: its direction points along the winding normal).
+bool ray_triangle(V3 dir, V3 v0, V3 v1, V3 v2, float& bx, float& by, float& dist, bool& back) {
+    const V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    const V3 p = cross(dir, e2);
+    const float det = dot(e1, p);
+    const float eps = std::numeric_limits<float>::epsilon();
+    const V3 d0 = sub(V3{0.f, 0.f, 0.f}, v0);  // orig - vert0
+    V3 perp{0.f, 0.f, 0.f};
+    if (det > eps) {
+        bx = dot(d0, p);
+        if (bx < 0.f || bx > det) return false;
+        perp = cross(d0, e1);
+        by = dot(dir, perp);
+        if (by < 0.f || bx + by > det) return false;
+        back = false;
+    } else if (det < -eps) {
+        bx = dot(d0, p);
+        if (bx > 0.f || bx < det) return false;
+        perp = cross(d0, e1);
+        by = dot(dir, perp);
+        if (by > 0.f || bx + by < det) return false;
+        back = true;
+    } else {
+        return false;
+    }
+    const float inv = 1.f / det;
+    dist = dot(e2, perp) * inv;
+    bx = bx * inv;
+    by = by * inv;
+    return true;
+}
+
+// Ray::IntersectParalgram (Ray.cpp:36-133), origin 0: entry and exit
+// distances of the ray through the box's three slabs (Haines).
+static inline float fmax_std(float a, float b) { return (a < b) ? b : a; }  // std::max(a, b)
+static inline float fmin_std(float a, float b) { return (b < a) ? b : a; }  // std::min(a, b)
+bool ray_box(V3 dir, const Box& b, float& tmin, float& tmax) {
+    tmin = -std::numeric_limits<float>::infinity();
+    tmax = +std::numeric_limits<float>::infinity();
+    const V3 ro = sub(V3{0.f, 0.f, 0.f}, b.c);
+    const V3 sides[3] = {b.u, b.v, b.w};
+    const V3 pa[3] = {b.v, b.w, b.u}, pb[3] = {b.w, b.u, b.v};  // slab normals v x w, w x u, u x v
+    for (int k = 0; k < 3; ++k) {
+        const V3 n = normalize(cross(pa[k], pb[k]));
+        const float d = -std::fabs(dot(n, sides[k]));
+        const float hc = dot(n, ro);
+        const float vn1 = hc + d;
+        const float vn2 = -hc + d;
+        const float vd = dot(n, dir);
+        if (std::fabs(vd) >= std::numeric_limits<float>::epsilon()) {
+            const float inv = 1.f / vd;
+            float t1 = -vn1 * inv;
+            float t2 = vn2 * inv;
+            if (t1 > t2) std::swap(t1, t2);
+            tmin = fmax_std(t1, tmin);
+            tmax = fmin_std(t2, tmax);
+            if (tmin > tmax || tmax < 0.f) return false;
+        } else if (vn1 > 0 || vn2 > 0) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// Closest hit of a ray on a tree (Ray::IntersectOBBtree, Ray.cpp:135-236):
+// the tree moved by M with the ray origin subtracted from M's translation,
+// children visited nearer entry first, each only while its entry distance
+// is below the best hit so far.
+static void ray_tree_rec(const Tree& T, const Trav& t, const M4& M, V3 dir, RayHit& best) {
+    if (!t.leaf) {
+        const Trav L = child_trav(T, t, 0), R = child_trav(T, t, 1);
+        float l0, l1, r0, r1;
+        const bool lh = ray_box(dir, xform_box(M, *L.box), l0, l1);
+        const bool rh = ray_box(dir, xform_box(M, *R.box), r0, r1);
+        auto visit = [&](const Trav& c, float c0, float c1) {
+            if (c0 < best.dist && c1 >= 0.f) ray_tree_rec(T, c, M, dir, best);
+        };
+        if (lh && rh) {
+            if (l0 < r0) {
+                visit(L, l0, l1);
+                visit(R, r0, r1);
+            } else {
+                visit(R, r0, r1);
+                visit(L, l0, l1);
+            }
+        } else if (lh) {
+            visit(L, l0, l1);
+        } else if (rh) {
+            visit(R, r0, r1);
+        }
+        return;
+    }
+    for (uint32_t i = t.off; i != t.off + t.cnt; ++i) {
+        V3 P[3];
+        for (int k = 0; k < 3; ++k) P[k] = mul4(M, tri_pt(T.pos.data(), i, k), 1.f);
+        float bx = 0.f, by = 0.f, dist = std::numeric_limits<float>::infinity();
+        bool back = false;
+        if (ray_triangle(dir, P[0], P[1], P[2], bx, by, dist, back) && dist > 0.f && dist < best.dist) {
+            best.hit = true;
+            best.back = back;
+            best.dist = dist;
+            best.tri = i;
+            best.bx = bx;
+            best.by = by;
+        }
+    }
+}
+RayHit ray_tree(const Tree& T, const M4& M, V3 o, V3 d) {
+    M4 C = M;  // centred: the ray starts at 0 (Ray.cpp:150-154)
+    C.m[12] = C.m[12] - o.x;
+    C.m[13] = C.m[13] - o.y;
+    C.m[14] = C.m[14] - o.z;
+    C.m[15] = C.m[15] - 0.f;
+    RayHit best;
+    const Trav root = root_trav(T);
+    float t0, t1;
+    if (ray_box(d, xform_box(C, *root.box), t0, t1) && t1 >= 0.f) ray_tree_rec(T, root, C, d, best);
+    return best;
+}
+
+// Ray::MoveOriginEpsilonTowardsDirection (Ray.cpp:13-21).
+static inline V3 neg(V3 v) { return {-v.x, -v.y, -v.z}; }  // glm's unary minus
+static V3 move_eps(V3 o, V3 d, float factor) {
+    const float big = std::max({std::fabs(o.x), std::fabs(o.y), std::fabs(o.z)});
+    const float e = big * std::numeric_limits<float>::epsilon();
+    return add(o, scale(d, factor * e));
+}
+
+struct Hermann {
+    bool ok = false;
+    V3 response{0.f, 0.f, 0.f}, normal{0.f, 0.f, 0.f};
+};
+// ShootUncollideRays::HermannPass (:129-158): the ray must leave B through a
+// back face before it re-enters A (within the origin's epsilon step).
+static Hermann hermann(const Tree& A, const M4& Amat, const Tree& B, const M4& Bmat, const M3& Bn, V3 o, V3 d) {
+    Hermann h;
+    const RayHit p2 = ray_tree(B, Bmat, o, d);
+    if (!(p2.hit && p2.back)) return h;
+    const V3 o2 = move_eps(o, d, 4.f);
+    const float eps = length(sub(o2, o));
+    const RayHit p3 = ray_tree(A, Amat, o2, d);
+    if (p2.dist <= p3.dist + eps) {
+        h.ok = true;
+        h.response = scale(d, p2.dist);
+        const V3 n0 = corner_normal(B, p2.tri, 0), n1 = corner_normal(B, p2.tri, 1), n2 = corner_normal(B, p2.tri, 2);
+        const float w0 = (1.f - p2.bx) - p2.by;
+        const V3 ni = add(add(scale(n0, w0), scale(n1, p2.bx)), scale(n2, p2.by));
+        h.normal = normalize(mul3(Bn, ni));
+    }
+    return h;
+}
+// :118-127
+static V3 force_response(const Hermann& h) {
+    const V3 dir = normalize(h.response);
+    const float len = length(h.response);
+    const float c = dot(h.normal, dir);
+    const float c2 = c * c;
+    return scale(dir, c2 * len);
+}
+// :171-175
+static float smoother_step(float a, float b, float x) {
+    float t = (x - a) / (b - a);
+    t = (t < 0.f) ? 0.f : ((1.f < t) ? 1.f : t);  // std::clamp(t, 0, 1)
+    return t * t * t * (t * (t * 6.f - 15.f) + 10.f);
+}
+
+// ExecuteShootUncollideRays (:14-102) for one pair, in the first entity's
+// model space until the final transform by the first's current matrix.
+V3 shoot_pair(const Tree& first, const Tree& second, const M4& first_cur, const M4& M, const PairRays& pr,
+              float cos_start, float cos_finish, float bias, ShootStats* stats) {
+    M4 I{};
+    I.m[0] = I.m[5] = I.m[10] = I.m[15] = 1.f;
+    M3 I3{};
+    I3.m[0] = I3.m[4] = I3.m[8] = 1.f;
+    const M3 Mn = adjoint_transpose(M);
+    V3 avg{0.f, 0.f, 0.f};
+    std::vector<V3> responses;
+    // returns whether the pass succeeded and, if so, the reflected ray
+    auto f2s = [&](V3 o, V3 d, V3& ro, V3& rd) {
+        const Hermann h = hermann(first, I, second, M, Mn, o, d);
+        if (stats) ++stats->passes;
+        if (!h.ok) return false;
+        avg = sub(avg, force_response(h));
+        responses.push_back(neg(h.response));
+        ro = add(o, h.response);
+        rd = neg(h.normal);
+        return true;
+    };
+    auto s2f = [&](V3 o, V3 d, V3& ro, V3& rd) {
+        const Hermann h = hermann(second, M, first, I, I3, o, d);
+        if (stats) ++stats->passes;
+        if (!h.ok) return false;
+        avg = add(avg, force_response(h));
+        responses.push_back(h.response);
+        ro = add(o, h.response);
+        rd = neg(h.normal);
+        return true;
+    };
+    for (const RayRec& r : pr.from_first) {
+        V3 ro, rd, xo, xd;
+        if (f2s({r.o[0], r.o[1], r.o[2]}, {r.d[0], r.d[1], r.d[2]}, ro, rd)) s2f(ro, rd, xo, xd);
+    }
+    for (const RayRec& r : pr.from_second) {
+        V3 ro, rd, xo, xd;
+        if (s2f({r.o[0], r.o[1], r.o[2]}, {r.d[0], r.d[1], r.d[2]}, ro, rd)) f2s(ro, rd, xo, xd);
+    }
+    if (stats) stats->successes += responses.size();
+    if (avg.x == 0.f && avg.y == 0.f && avg.z == 0.f) return {0.f, 0.f, 0.f};
+    const V3 n = normalize(avg);
+    float max_len = 0.f;  // FindResponse (:160-169)
+    for (const V3& r : responses) {
+        const V3 rn = normalize(r);
+        const float c = dot(rn, n);
+        const float need = length(r) / c;
+        const float f = smoother_step(cos_finish, cos_start, c);
+        max_len = fmax_std(max_len, f * need);
+    }
+    const V3 local = scale(scale(n, max_len), bias);  // bias * (max_len * n): glm scalar * vec3, per component
+    return mul4(first_cur, local, 0.f);
+}
+
+// PointMovementBetweenFrames (CollisionDetection.cpp:143-150).
+static float point_movement(V3 p, const M4& cur, const M4& prev) {
+    return length(sub(mul4(cur, p, 1.f), mul4(prev, p, 1.f)));
+}
+static bool mat_equal(const M4& a, const M4& b) {
+    for (int i = 0; i < 16; ++i)
+        if (!(a.m[i] == b.m[i])) return false;
+    return true;
+}
+PairResponse respond(const Tree& first, const Tree& second, const M4& first_cur, const M4& first_prev,
+                     const M4& second_cur, const M4& second_prev, const PairRays& pr, float cos_start,
+                     float cos_finish, float bias, ShootStats* stats) {
+    PairResponse out;
+    out.first = pr.first;
+    out.second = pr.second;
+    if (mat_equal(first_cur, first_prev) && mat_equal(second_cur, second_prev)) return out;  // :80-81
+    out.shot = true;
+    const M4 M = mat_mul(mat_inverse(first_cur), second_cur);
+    const V3 delta = shoot_pair(first, second, first_cur, M, pr, cos_start, cos_finish, bias, stats);
+    const float m1 = point_movement({pr.avg_first[0], pr.avg_first[1], pr.avg_first[2]}, first_cur, first_prev);
+    const float m2 = point_movement({pr.avg_second[0], pr.avg_second[1], pr.avg_second[2]}, second_cur, second_prev);
+    const float total = m1 + m2;
+    const V3 nd = neg(delta);
+    const V3 d1 = scale(nd, m1 / total), d2 = scale(delta, m2 / total);
+    for (int k = 0; k < 3; ++k) {
+        out.delta[k] = (&delta.x)[k];
+        out.delta_first[k] = (&d1.x)[k];
+        out.delta_second[k] = (&d2.x)[k];
+    }
+    return out;
+}
+
 void set_attrs(Tree& T, const float* nrm9, const uint32_t* vid3) {
     const size_t n = T.perm.size();
     T.nrm.assign(n * 9, 0.f);
@@ -1616,6 +1879,81 @@ void* orc_rays_frame(void** trees, const uint32_t* mesh_of, const float* cur16, 
     }
     return R;
 }
+// Closed-form probes of the ray pieces.  out6: hit, back, dist, bx, by, (unused)
+void orc_ray_triangle(const float* dir3, const float* tri9, float* out6) {
+    float bx = 0.f, by = 0.f, dist = std::numeric_limits<float>::infinity();
+    bool back = false;
+    const bool hit = ray_triangle({dir3[0], dir3[1], dir3[2]}, {tri9[0], tri9[1], tri9[2]}, {tri9[3], tri9[4], tri9[5]},
+                                  {tri9[6], tri9[7], tri9[8]}, bx, by, dist, back);
+    out6[0] = hit ? 1.f : 0.f;
+    out6[1] = back ? 1.f : 0.f;
+    out6[2] = dist;
+    out6[3] = bx;
+    out6[4] = by;
+    out6[5] = 0.f;
+}
+int orc_ray_box(const float* dir3, const float* b12, float* out2) {
+    Box b;
+    std::memcpy(&b, b12, sizeof b);
+    return ray_box({dir3[0], dir3[1], dir3[2]}, b, out2[0], out2[1]) ? 1 : 0;
+}
+// out8: hit, back, dist, tri, bx, by
+void orc_ray_tree(void* tp, const float* m16, const float* o3, const float* d3, float* out8) {
+    M4 M;
+    std::memcpy(M.m, m16, sizeof M.m);
+    const RayHit h = ray_tree(*static_cast<Tree*>(tp), M, {o3[0], o3[1], o3[2]}, {d3[0], d3[1], d3[2]});
+    out8[0] = h.hit ? 1.f : 0.f;
+    out8[1] = h.back ? 1.f : 0.f;
+    out8[2] = h.dist;
+    out8[3] = float(h.tri == 0xFFFFFFFFu ? -1.0 : double(h.tri));
+    out8[4] = h.bx;
+    out8[5] = h.by;
+    out8[6] = out8[7] = 0.f;
+}
+// A frame's response: broad and mid phase, rays, then for every pair with
+// rays the shot delta and its split (CollisionDetection.cpp:38-99).
+struct OrcShoot {
+    std::vector<PairResponse> pairs;
+    ShootStats stats;
+};
+void* orc_shoot_frame(void** trees, const uint32_t* mesh_of, const float* cur16, const float* prev16, const uint8_t* cb,
+                      int64_t n, float cos_start, float cos_finish, float bias) {
+    auto* S = new OrcShoot();
+    auto* R = static_cast<OrcRays*>(orc_rays_frame(trees, mesh_of, cur16, cb, n));
+    for (const PairRays& pr : R->pairs) {
+        M4 c1, p1, c2, p2;
+        std::memcpy(c1.m, cur16 + size_t(pr.first) * 16, sizeof c1.m);
+        std::memcpy(p1.m, prev16 + size_t(pr.first) * 16, sizeof p1.m);
+        std::memcpy(c2.m, cur16 + size_t(pr.second) * 16, sizeof c2.m);
+        std::memcpy(p2.m, prev16 + size_t(pr.second) * 16, sizeof p2.m);
+        S->pairs.push_back(respond(with_attrs(trees[mesh_of[pr.first]]), with_attrs(trees[mesh_of[pr.second]]), c1, p1,
+                                   c2, p2, pr, cos_start, cos_finish, bias, &S->stats));
+    }
+    delete R;
+    return S;
+}
+// out: per pair first, second, shot (u32 x 3) and delta, delta_first, delta_second (f32 x 9); stats2: passes, successes
+int64_t orc_shoot_get(void* sp, uint32_t* ids3, float* d9, uint64_t* stats2) {
+    const OrcShoot& S = *static_cast<OrcShoot*>(sp);
+    if (ids3 && d9)
+        for (size_t i = 0; i < S.pairs.size(); ++i) {
+            const PairResponse& r = S.pairs[i];
+            ids3[i * 3 + 0] = r.first;
+            ids3[i * 3 + 1] = r.second;
+            ids3[i * 3 + 2] = r.shot ? 1u : 0u;
+            for (int k = 0; k < 3; ++k) {
+                d9[i * 9 + k] = r.delta[k];
+                d9[i * 9 + 3 + k] = r.delta_first[k];
+                d9[i * 9 + 6 + k] = r.delta_second[k];
+            }
+        }
+    if (stats2) {
+        stats2[0] = S.stats.passes;
+        stats2[1] = S.stats.successes;
+    }
+    return int64_t(S.pairs.size());
+}
+void orc_shoot_free(void* sp) { delete static_cast<OrcShoot*>(sp); }
 // what: 0 pairs, 1 rays (both directions, all pairs)
 int64_t orc_rays_count(void* rp, int what) {
     const OrcRays& R = *static_cast<OrcRays*>(rp);

@@ -79,4 +79,27 @@ struct PairRays {
     float avg_first[3] = {0, 0, 0}, avg_second[3] = {0, 0, 0};
 };
 
+// Closest ray hit on a tree (Ray.h:17-24).
+struct RayHit {
+    bool hit = false, back = false;
+    float dist = __builtin_huge_valf();
+    uint32_t tri = 0xFFFFFFFFu;  // tree-order slot
+    float bx = 0.f, by = 0.f;
+};
+// One entity pair's response (CollisionDetection.cpp:70-99): the uncollide
+// delta in world space and its split between the two entities.
+struct PairResponse {
+    uint32_t first = 0, second = 0;
+    bool shot = false;  // false when neither entity moved: all deltas zero
+    float delta[3] = {0, 0, 0}, delta_first[3] = {0, 0, 0}, delta_second[3] = {0, 0, 0};
+};
+struct ShootStats { uint64_t passes = 0, successes = 0; };
+
+bool ray_triangle(V3 dir, V3 v0, V3 v1, V3 v2, float& bx, float& by, float& dist, bool& back);
+bool ray_box(V3 dir, const Box& b, float& tmin, float& tmax);
+RayHit ray_tree(const Tree& T, const M4& M, V3 o, V3 d);
+PairResponse respond(const Tree& first, const Tree& second, const M4& first_cur, const M4& first_prev,
+                     const M4& second_cur, const M4& second_prev, const PairRays& pr, float cos_start,
+                     float cos_finish, float bias, ShootStats* stats);
+
 }  // namespace orc
