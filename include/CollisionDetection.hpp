@@ -6,11 +6,15 @@
 // (Reset, AddCollisionDetectionEntry, ExecuteCollisionDetection), with the
 // entry type of inMyRoom_vulkan/include/ECS/ECStypes.h:149-156 restated on
 // plain floats (the OBBtree pointer becomes a registered mesh id).
-// ExecuteCollisionDetection runs phases 1-3 of
-// src/CollisionDetection/CollisionDetection.cpp:38-67 on the GPU, including
-// the uncollide rays of CreateUncollideRays.cpp:60-200, and keeps the entity
-// pairs that produced rays (:63) with their contacts and rays; ray shooting
-// and callbacks (CollisionDetection.cpp:69-141) stay with the engine.
+// ExecuteCollisionDetection runs src/CollisionDetection/CollisionDetection.cpp:38-141
+// on the GPU: phases 1-3 (broad, mid, narrow), the uncollide rays of
+// CreateUncollideRays.cpp:60-200, the ray shooting of ShootUncollideRays.cpp
+// with the movement split (:69-99), then the callback fan-out to entity
+// ancestors (:101-121) and MakeCallbacks (:124-134).  The two engine
+// services those last steps use are hooks: the entity's ancestor list
+// (EntitiesHandler::GetEntityAncestors) and the components' CollisionCallback
+// (ComponentBaseClass.h:25), which receives the callbacks as the reference
+// hands them over.
 //
 // Error behaviour: the reference has no error returns; this mirror throws
 // std::runtime_error with the library's message when a call fails (the C ABI
@@ -18,8 +22,12 @@
 #pragma once
 
 #include <cstring>
+#include <functional>
+#include <iterator>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "mi355x_collide.h"
@@ -36,6 +44,13 @@ struct CollisionDetectionEntry {  // ECStypes.h:149-156
     Entity entity;
 };
 
+struct CollisionCallbackData {  // ECStypes.h:158-163
+    Entity familyEntity;
+    Entity collideWithEntity;
+    float deltaVector[3] = {0.f, 0.f, 0.f};
+};
+typedef std::vector<std::pair<Entity, std::vector<CollisionCallbackData>>> CallbackEntityDataPairs;
+
 struct CollidingPair {
     Entity first, second;          // oriented as SweepAndPrune.cpp:63 orients them
     uint32_t firstIndex, secondIndex;  // positions in the frame's entry list
@@ -45,6 +60,10 @@ struct CollidingPair {
     std::vector<cd_ray> rays_from_first_to_second, rays_from_second_to_first;
     float average_point_first_modelspace[3];
     float average_point_second_modelspace[3];
+    // the response (CollisionDetection.cpp:70-99): zero when neither entity moved
+    bool shot = false;
+    float delta[3] = {0.f, 0.f, 0.f};
+    CollisionCallbackData first_callback, second_callback;
 };
 
 class CollisionDetection {
@@ -130,12 +149,54 @@ public:
             std::memcpy(r.average_point_second_modelspace, rp.avg_second, sizeof rp.avg_second);
             results_.push_back(std::move(r));
         }
+        if (results_.empty()) {
+            MakeCallbacks({});
+            return;
+        }
+        // ray shooting and the movement split (:69-99), one record per pair with rays
+        std::vector<cd_pair_response> resp(results_.size());
+        uint32_t nresp = 0;
+        check(cd_shoot_rays(ctx_, nullptr, resp.data(), uint32_t(resp.size()), &nresp), "cd_shoot_rays");
+        std::unordered_map<Entity, std::vector<CollisionCallbackData>> callbacks_to_be_made;
+        for (size_t i = 0; i < results_.size() && i < nresp; ++i) {
+            CollidingPair& r = results_[i];
+            r.shot = resp[i].shot != 0;
+            std::memcpy(r.delta, resp[i].delta, sizeof r.delta);
+            r.first_callback.familyEntity = r.first;
+            r.first_callback.collideWithEntity = r.second;
+            std::memcpy(r.first_callback.deltaVector, resp[i].delta_first, sizeof r.first_callback.deltaVector);
+            r.second_callback.familyEntity = r.second;
+            r.second_callback.collideWithEntity = r.first;
+            std::memcpy(r.second_callback.deltaVector, resp[i].delta_second, sizeof r.second_callback.deltaVector);
+            // the callback goes to every ancestor the two entities do not share (:101-121)
+            const std::vector<Entity> fa = Ancestors(r.first), sa = Ancestors(r.second);
+            for (size_t k = 0; k != fa.size(); ++k)
+                if (k >= sa.size() || fa[k] != sa[k]) callbacks_to_be_made[fa[k]].emplace_back(r.first_callback);
+            for (size_t k = 0; k != sa.size(); ++k)
+                if (k >= fa.size() || fa[k] != sa[k]) callbacks_to_be_made[sa[k]].emplace_back(r.second_callback);
+        }
+        MakeCallbacks(std::move(callbacks_to_be_made));
     }
+
+    // Engine hooks for the last steps (CollisionDetection.cpp:101-134).
+    // Without an ancestor provider an entity is its own only ancestor.
+    void SetAncestorsProvider(std::function<std::vector<Entity>(Entity)> f) { ancestors_ = std::move(f); }
+    void SetCallbackSink(std::function<void(const CallbackEntityDataPairs&)> f) { sink_ = std::move(f); }
 
     const std::vector<CollidingPair>& GetCollidingPairs() const { return results_; }
     cd_ctx* Context() const { return ctx_; }
 
 private:
+    std::vector<Entity> Ancestors(Entity e) const { return ancestors_ ? ancestors_(e) : std::vector<Entity>{e}; }
+
+    // CollisionDetection::MakeCallbacks (:124-134): the map becomes a vector
+    // of (entity, callbacks) pairs handed to the components.
+    void MakeCallbacks(std::unordered_map<Entity, std::vector<CollisionCallbackData>>&& callbacks_to_be_made) const {
+        CallbackEntityDataPairs v(std::make_move_iterator(callbacks_to_be_made.begin()),
+                                  std::make_move_iterator(callbacks_to_be_made.end()));
+        if (sink_) sink_(v);
+    }
+
     void check(int rc, const char* what) const {
         if (rc != CD_OK) throw std::runtime_error(std::string(what) + ": " + (ctx_ ? cd_last_error(ctx_) : "no context"));
     }
@@ -148,6 +209,8 @@ private:
     std::vector<cd_pair_rays> ray_pairs_;
     std::vector<cd_ray> rays_;
     std::vector<CollidingPair> results_;
+    std::function<std::vector<Entity>(Entity)> ancestors_;
+    std::function<void(const CallbackEntityDataPairs&)> sink_;
 };
 
 }  // namespace mi355x
