@@ -1,7 +1,8 @@
 """Timing of the other workloads with bench.py's harness (`bench.py --config N`).
 
   2  scene step: 512 rigid entities (boxes, icospheres, noisy tori) in a 50 m
-     cube, 100 frames at 1/60 s, broad + mid + narrow each frame (cd_frame)
+     cube, 100 frames at 1/60 s, broad + mid + narrow each frame (cd_frame);
+     also timed with the uncollide rays and the response (the whole frame)
   4  precision stress: 2^24 triangle pairs (80% random, 10% near-coplanar,
      10% touching); plain float32 Möller vs the filtered exact test, each
      checked against the exact oracle on a sample
@@ -58,6 +59,19 @@ def config2(args):
         out[name] = {"ms_per_frame": round(wall, 4), "device_ms_per_frame": round(dev / len(frames), 4),
                      "pairs_per_frame": npairs / len(frames), "contacts_per_frame": ncons / len(frames)}
         log(f"config 2 {name}: {wall:.3f} ms/frame wall, {dev / len(frames):.3f} ms device")
+        # the whole ExecuteCollisionDetection: frame, uncollide rays, ray shooting and the split
+        t = time.perf_counter()
+        nrays = nshot = 0
+        for f in frames:
+            gpu.frame(f, mode)
+            _, rays = gpu.uncollide_rays()
+            resp = gpu.shoot_rays()
+            nrays += len(rays)
+            nshot += int(resp["shot"].sum())
+        full = (time.perf_counter() - t) * 1e3 / len(frames)
+        out[name].update({"full_ms_per_frame": round(full, 4), "rays_per_frame": nrays / len(frames),
+                          "pairs_shot_per_frame": nshot / len(frames)})
+        log(f"config 2 {name}: {full:.3f} ms/frame with rays and response")
     cpu = None
     if not args.no_cpu:
         from oracle import Oracle
@@ -129,7 +143,8 @@ def config4(args):
                         "fp64_fallbacks": res["batched"]["fp64"], "exact_fallbacks": res["batched"]["exact"],
                         "mismatches_vs_exact_oracle": mism,
                         "roofline": {"bound": "hbm", "kernel": "k_tritri_f32 + k_tritri_rerun<1> + k_tritri_rerun<2>",
-                                     "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 5),
+                                     "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                                     "frac": round(achieved / 8000.0, 5),
                                      "traffic": None, "algorithmic_bytes_per_launch": int(bytes_per_launch)},
                         "cpu_baseline": {"value": round(sample / t_cpu, 1), "unit": "tests/s", "cores": 1,
                                          "kind": "port", "sample": f"{sample:,} pairs, reference Möller, single thread"},

@@ -114,7 +114,7 @@ __global__ __launch_bounds__(1024) void k_radix_sort(uint32_t* keys, uint32_t* v
 // Multi-workgroup pass of the LSD sort: tiles of RS_TILE keys; the count
 // table is digit-major ([digit][tile]) so that one exclusive scan gives
 // every tile the start of each of its digit runs in tile order (stable).
-constexpr uint32_t RS_TILE = 4096;
+constexpr uint32_t RS_TILE = 1024;  // keys per workgroup per pass: 4 rounds of 256
 constexpr uint32_t RS_SMALL = 4096;
 
 __global__ __launch_bounds__(256) void k_rs_count(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,

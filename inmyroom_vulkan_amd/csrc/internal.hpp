@@ -83,7 +83,7 @@ struct RayShot {
     uint32_t ok;    // bit p: pass p succeeded
     uint32_t pair;  // query (entity pair) of the ray
 };
-constexpr int RAY_STACK_ENTRIES = 64;  // closest-hit traversal stack: a tree's depth + 2
+constexpr int RAY_STACK_ENTRIES = 48;  // closest-hit traversal stack per lane (LDS): a tree's depth + 2
 
 // A refit plan: small boxes (one thread each) and large boxes split into
 // chunks of triangles (device arrays, cached per mesh list).

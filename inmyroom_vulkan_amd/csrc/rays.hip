@@ -24,7 +24,8 @@
 //                point with the barycentric normal when no vertex is
 //                inside (:155-167); the second entity's rays are in the
 //                first's space (:135, :146, :160)
-//  k_ray_avg     average origin per query and side (:179-197)
+//  k_ray_avg     average origin per query and side (:179-197), one
+//                wavefront per query
 #include "rayattr.hpp"
 
 namespace cdk {
@@ -325,17 +326,26 @@ __global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __r
 
 // Average origin of each query's rays on one side (:179-197); the second's is
 // taken back to its own model space by inverse(rel).
-__global__ void k_ray_avg(const cd_ray* __restrict__ rays, uint64_t ray_base, const uint32_t* __restrict__ qbegin,
-                          const uint32_t* __restrict__ qcount, uint32_t nq, int side, const QueryRec* __restrict__ qs,
-                          float* __restrict__ avg) {
-    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per query: lane-strided partial sums, then a fixed shuffle
+// tree, so the sum is the same on every run (the reference sums in its
+// unordered_map's order, which no GPU reproduces).
+__global__ __launch_bounds__(64) void k_ray_avg(const cd_ray* __restrict__ rays, uint64_t ray_base,
+                                                const uint32_t* __restrict__ qbegin,
+                                                const uint32_t* __restrict__ qcount, uint32_t nq, int side,
+                                                const QueryRec* __restrict__ qs, float* __restrict__ avg) {
+    const uint32_t qi = blockIdx.x;
+    const int lane = threadIdx.x;
     if (qi >= nq) return;
     const uint32_t n = qcount[qi];
     f3 acc = mk(0.f, 0.f, 0.f);
     if (n) {
         const cd_ray* r = rays + ray_base + qbegin[qi];
-        for (uint32_t k = 0; k < n; ++k) acc = add(acc, mk(r[k].origin[0], r[k].origin[1], r[k].origin[2]));
+        for (uint32_t k = uint32_t(lane); k < n; k += 64) acc = add(acc, mk(r[k].origin[0], r[k].origin[1], r[k].origin[2]));
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        acc = add(acc, mk(__shfl_down(acc.x, off, 64), __shfl_down(acc.y, off, 64), __shfl_down(acc.z, off, 64)));
+    if (lane != 0) return;
     f3 a = divs(acc, float(n));
     if (side) {
         M4 M;
@@ -398,7 +408,8 @@ hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t 
 }
 hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
                           uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st) {
-    hipLaunchKernelGGL(k_ray_avg, gr(nq), dim3(256), 0, st, rays, ray_base, qbegin, qcount, nq, side, qs, avg);
+    if (!nq) return hipSuccess;
+    hipLaunchKernelGGL(k_ray_avg, dim3(nq), dim3(64), 0, st, rays, ray_base, qbegin, qcount, nq, side, qs, avg);
     return hipGetLastError();
 }
 

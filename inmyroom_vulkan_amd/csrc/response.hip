@@ -10,18 +10,16 @@
 //             moved by the response, direction minus the hit's normal,
 //             :104-110) and pass 2 shoots it the other way.  Each pass's force
 //             response and push-out vector are kept per ray.
-//  k_respond  one lane per entity pair: force responses summed in the
-//             reference's ray order (rays from the first entity, each with its
-//             reflected pass, then those from the second), FindResponse
-//             (:160-169) over the push-out vectors, the world-space delta, and
-//             its split by how far each entity's average contact point moved
-//             since the previous frame.
+//  k_respond  one wavefront per entity pair: the force responses summed,
+//             FindResponse (:160-169) over the push-out vectors, the
+//             world-space delta, and its split by how far each entity's
+//             average contact point moved since the previous frame.
 //
-// The closest-hit traversal keeps an explicit stack.  At an internal node
-// both children's entry/exit distances are computed, the farther child is
-// pushed below the nearer, and a child is entered only if, when it is
-// popped, its entry distance is below the best hit so far and its exit
-// distance is not negative: the nodes visited, and the order in which equal
+// The closest-hit traversal keeps an explicit stack in LDS.  At an internal
+// node both children's entry/exit distances are computed, the farther child
+// is pushed below the nearer, and a child is entered only if its exit
+// distance is not negative and, when it is popped, its entry distance is
+// below the best hit so far: the nodes visited, and the order in which equal
 // distances are met, are those of the reference's recursion.
 #include "rayattr.hpp"
 
@@ -30,7 +28,7 @@ namespace cdk {
 namespace {
 
 constexpr float FLT_EPS = 1.1920928955078125e-07f;  // std::numeric_limits<float>::epsilon()
-constexpr int RAY_STACK = RAY_STACK_ENTRIES;  // holds a tree's depth + 2 entries
+constexpr int RAY_STACK = RAY_STACK_ENTRIES;  // entries per lane: a tree's depth + 2
 
 __device__ __forceinline__ float max_std(float a, float b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ float min_std(float a, float b) { return (b < a) ? b : a; }  // std::min
@@ -110,11 +108,6 @@ struct Hit {
     uint32_t tri;  // mesh-local tree-order slot
 };
 
-struct StackEntry {
-    uint32_t link, cnt;  // cnt 0: internal node `link`; else a leaf of cnt triangles from `link`
-    float t0, t1;
-};
-
 __device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, const M4& C, f3 dir, uint32_t off,
                           uint32_t cnt, Hit& best) {
     for (uint32_t i = off; i != off + cnt; ++i) {
@@ -135,9 +128,27 @@ __device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, cons
     }
 }
 
+// Per-lane traversal stack in LDS, entry-major (entry e of lane l at
+// e * 64 + l, so a wavefront's accesses fall in distinct banks).  An entry
+// is a child still to visit: cnt 0 for internal node `link`, else a leaf of
+// cnt triangles from `link`; t0 its entry distance.  A child whose exit
+// distance is negative is never pushed (that check does not depend on the
+// best hit, so making it at push time is the reference's check).
+struct RayStack {
+    uint32_t* link;
+    uint32_t* cnt;
+    float* t0;
+    int lane;
+    __device__ __forceinline__ void put(int i, uint32_t l, uint32_t c, float t) {
+        link[i * 64 + lane] = l;
+        cnt[i * 64 + lane] = c;
+        t0[i * 64 + lane] = t;
+    }
+};
+
 // Closest hit of the ray (o, dir) on mesh m moved by M (Ray.cpp:135-236).
 __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
-                        f3 o, f3 dir) {
+                        f3 o, f3 dir, RayStack& S) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
@@ -150,44 +161,46 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
         return best;
     }
     const NodeRec* nb = nodes + m.node_base;
-    StackEntry st[RAY_STACK];
     int sp = 0;
     uint32_t node = 0;  // the root, entered without a distance check
     for (;;) {
         const NodeRec& n = nb[node];
-        const uint32_t cl = n.cnt & 0xFFFFu, cr = n.cnt >> 16;
         float l0, l1, q0, q1;
-        const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1);
-        const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1);
-        const StackEntry L = {n.link[0], cl, l0, l1}, R = {n.link[1], cr, q0, q1};
+        const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1) && l1 >= 0.f;
+        const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1) && q1 >= 0.f;
         if (sp + 2 > RAY_STACK) {  // deeper than the host allows (it checks tree depths first)
             best.hit = false;
             break;
         }
-        if (lh && rh) {  // the farther below the nearer (Ray.cpp:175-199)
-            if (l0 < q0) {
-                st[sp++] = R;
-                st[sp++] = L;
+        const uint32_t cl = n.cnt & 0xFFFFu, cr = n.cnt >> 16;
+        // the farther below the nearer (Ray.cpp:175-199); order set by the entry distances alone
+        const bool left_first = !(lh && rh) || l0 < q0;
+        if (lh && rh) {
+            if (left_first) {
+                S.put(sp++, n.link[1], cr, q0);
+                S.put(sp++, n.link[0], cl, l0);
             } else {
-                st[sp++] = L;
-                st[sp++] = R;
+                S.put(sp++, n.link[0], cl, l0);
+                S.put(sp++, n.link[1], cr, q0);
             }
         } else if (lh) {
-            st[sp++] = L;
+            S.put(sp++, n.link[0], cl, l0);
         } else if (rh) {
-            st[sp++] = R;
+            S.put(sp++, n.link[1], cr, q0);
         }
         // next internal node to enter; leaves are tested as they are popped
         bool found = false;
         while (sp > 0) {
-            const StackEntry e = st[--sp];
-            if (!(e.t0 < best.dist && e.t1 >= 0.f)) continue;
-            if (e.cnt == 0u) {
-                node = e.link;
+            --sp;
+            const uint32_t el = S.link[sp * 64 + S.lane], ec = S.cnt[sp * 64 + S.lane];
+            const float et = S.t0[sp * 64 + S.lane];
+            if (!(et < best.dist)) continue;
+            if (ec == 0u) {
+                node = el;
                 found = true;
                 break;
             }
-            leaf_hits(tris, m, C, dir, e.link, e.cnt, best);
+            leaf_hits(tris, m, C, dir, el, ec, best);
         }
         if (!found) break;
     }
@@ -220,14 +233,14 @@ struct MeshView {
 
 // HermannPass (ShootUncollideRays.cpp:129-158).
 __device__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
-                        const M3& Bn, f3 o, f3 d) {
+                        const M3& Bn, f3 o, f3 d, RayStack& S) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const MeshRec mb = V.meshes[meshB];
-    const Hit p2 = ray_tree(mb, V.nodes, V.tris, Bmat, o, d);
+    const Hit p2 = ray_tree(mb, V.nodes, V.tris, Bmat, o, d, S);
     if (!(p2.hit && p2.back)) return r;
     const f3 o2 = move_eps(o, d, 4.f);
     const float eps = vlength(sub(o2, o));
-    const Hit p3 = ray_tree(V.meshes[meshA], V.nodes, V.tris, Amat, o2, d);
+    const Hit p3 = ray_tree(V.meshes[meshA], V.nodes, V.tris, Amat, o2, d, S);
     if (p2.dist <= p3.dist + eps) {
         r.ok = true;
         r.response = scale(d, p2.dist);
@@ -271,6 +284,9 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
                                               RayShot* __restrict__ out) {
+    __shared__ uint32_t s_link[RAY_STACK * 64], s_cnt[RAY_STACK * 64];
+    __shared__ float s_t0[RAY_STACK * 64];
+    RayStack S = {s_link, s_cnt, s_t0, int(threadIdx.x)};
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrays) return;
     const int side = r >= base1 ? 1 : 0;
@@ -301,8 +317,8 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         const bool f2s = (pass == 0) == (side == 0);
-        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d)
-                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d);
+        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S)
+                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S);
         if (!p.ok) break;
         const f3 F = force_response(p);
         put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
@@ -326,11 +342,22 @@ __device__ __forceinline__ bool same_matrix(const float* a, const float* b) {
     return true;
 }
 
-__global__ void k_respond(const RayShot* __restrict__ shots, uint32_t base1, const uint32_t* __restrict__ qw,
-                          uint32_t nq, const float* __restrict__ avg, const QueryRec* __restrict__ qs,
-                          const uint2* __restrict__ pair_ids, const cd_entity* __restrict__ ents,
-                          float cos_start, float cos_finish, float bias, cd_pair_response* __restrict__ out) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per pair.  The force responses are summed lane-strided
+// (each shot's first pass, then its second) and then over a fixed shuffle
+// tree: a sum that is the same on every run.  The reference sums them in its
+// ray order, which follows an unordered_map walk no GPU reproduces.  The
+// maximum of FindResponse does not depend on the order.
+__device__ __forceinline__ f3 shfl_down3(f3 v, int off) {
+    return mk(__shfl_down(v.x, off, 64), __shfl_down(v.y, off, 64), __shfl_down(v.z, off, 64));
+}
+__global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shots, uint32_t base1,
+                                                const uint32_t* __restrict__ qw, uint32_t nq,
+                                                const float* __restrict__ avg, const QueryRec* __restrict__ qs,
+                                                const uint2* __restrict__ pair_ids, const cd_entity* __restrict__ ents,
+                                                float cos_start, float cos_finish, float bias,
+                                                cd_pair_response* __restrict__ out) {
+    const uint32_t q = blockIdx.x;
+    const int lane = threadIdx.x;
     if (q >= nq) return;
     const uint2 ids = pair_ids[q];
     cd_pair_response res = {};
@@ -343,41 +370,41 @@ __global__ void k_respond(const RayShot* __restrict__ shots, uint32_t base1, con
     const uint32_t n0 = qw[q], b0 = qw[nq + q];
     const uint32_t n1 = qw[2 * size_t(nq) + q], b1 = base1 + qw[3 * size_t(nq) + q];
     if ((n0 == 0 && n1 == 0) || (same_matrix(c1, p1) && same_matrix(c2, p2))) {  // :63, :80-81
-        out[q] = res;
+        if (lane == 0) out[q] = res;
         return;
     }
     res.shot = 1;
-    // the summed force responses, in the reference's order
+    const uint32_t n = n0 + n1;
     f3 sum = mk(0.f, 0.f, 0.f);
-    for (int side = 0; side < 2; ++side) {
-        const uint32_t b = side ? b1 : b0, n = side ? n1 : n0;
-        for (uint32_t i = 0; i < n; ++i) {
-            const RayShot& s = shots[b + i];
-            for (int pass = 0; pass < 2; ++pass)
-                if (s.ok & (1u << pass)) sum = add(sum, get3(s.force[pass]));
-        }
+    for (uint32_t i = uint32_t(lane); i < n; i += 64) {
+        const RayShot& s = shots[i < n0 ? b0 + i : b1 + (i - n0)];
+        for (int pass = 0; pass < 2; ++pass)
+            if (s.ok & (1u << pass)) sum = add(sum, get3(s.force[pass]));
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum = add(sum, shfl_down3(sum, off));
+    sum = mk(__shfl(sum.x, 0, 64), __shfl(sum.y, 0, 64), __shfl(sum.z, 0, 64));
     f3 delta = mk(0.f, 0.f, 0.f);
     if (!(sum.x == 0.f && sum.y == 0.f && sum.z == 0.f)) {  // :87
         const f3 nf = normalize(sum);
         float max_len = 0.f;  // FindResponse (:160-169)
-        for (int side = 0; side < 2; ++side) {
-            const uint32_t b = side ? b1 : b0, n = side ? n1 : n0;
-            for (uint32_t i = 0; i < n; ++i) {
-                const RayShot& s = shots[b + i];
-                for (int pass = 0; pass < 2; ++pass) {
-                    if (!(s.ok & (1u << pass))) continue;
-                    const f3 r = get3(s.push[pass]);
-                    const float c = dot(normalize(r), nf);
-                    const float need = vlength(r) / c;
-                    max_len = max_std(max_len, smoother_step(cos_finish, cos_start, c) * need);
-                }
+        for (uint32_t i = uint32_t(lane); i < n; i += 64) {
+            const RayShot& s = shots[i < n0 ? b0 + i : b1 + (i - n0)];
+            for (int pass = 0; pass < 2; ++pass) {
+                if (!(s.ok & (1u << pass))) continue;
+                const f3 r = get3(s.push[pass]);
+                const float c = dot(normalize(r), nf);
+                const float need = vlength(r) / c;
+                max_len = max_std(max_len, smoother_step(cos_finish, cos_start, c) * need);
             }
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) max_len = max_std(max_len, __shfl_down(max_len, off, 64));
         M4 F;
         for (int e = 0; e < 16; ++e) F.m[e] = c1[e];
         delta = mul4(F, scale(scale(nf, max_len), bias), 0.f);
     }
+    if (lane != 0) return;
     // the split by each entity's movement (CollisionDetection.cpp:86-97, 143-150)
     M4 C1, P1, C2, P2;
     for (int e = 0; e < 16; ++e) {
@@ -410,8 +437,8 @@ hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* 
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
                           float cos_finish, float bias, cd_pair_response* out, hipStream_t st) {
     if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_respond, dim3((nq + 63) / 64), dim3(64), 0, st, shots, base1, qw, nq, avg, qs, pair_ids, ents,
-                       cos_start, cos_finish, bias, out);
+    hipLaunchKernelGGL(k_respond, dim3(nq), dim3(64), 0, st, shots, base1, qw, nq, avg, qs, pair_ids, ents, cos_start,
+                       cos_finish, bias, out);
     return hipGetLastError();
 }
 
