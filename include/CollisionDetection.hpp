@@ -213,4 +213,50 @@ private:
     std::function<void(const CallbackEntityDataPairs&)> sink_;
 };
 
+// The collision part of MeshesOfNodes (include/Graphics/Meshes/MeshesOfNodes.h:
+// 28, 48-51): AddMeshesOfModel gives every mesh of a glTF model its tree
+// (src/Graphics/Meshes/MeshesOfNodes.cpp:29-64), GetMeshIndexOffsetOfModel
+// (:66-73) and GetMeshInfo(i).boundBoxTree name it.  A model is named by its
+// path where the reference keys it by its tinygltf::Model; boundBoxTree is
+// the mesh id entries carry (CD_NO_MESH for a mesh with no collision
+// triangles: skinned or morphed only, which the reference never attaches a
+// collision component to, src/GameImporter.cpp:320-321).
+class MeshesOfNodes {
+public:
+    struct MeshInfo {
+        uint32_t boundBoxTree = CD_NO_MESH;
+    };
+
+    explicit MeshesOfNodes(CollisionDetection& collisionDetection) : cd_(collisionDetection) {}
+
+    void AddMeshesOfModel(const std::string& path) {
+        cd_gltf* doc = nullptr;
+        char err[512];
+        if (cd_gltf_open(path.c_str(), &doc, err, sizeof err) != CD_OK)
+            throw std::runtime_error("cd_gltf_open: " + std::string(err));
+        uint32_t n = 0;
+        cd_gltf_mesh_count(doc, &n);
+        std::vector<uint32_t> ids(n);
+        const int rc = cd_register_gltf(cd_.Context(), doc, ids.data(), n);
+        cd_gltf_close(doc);
+        if (rc != CD_OK) throw std::runtime_error(std::string("cd_register_gltf: ") + cd_last_error(cd_.Context()));
+        modelToMeshIndexOffset_.emplace(path, meshes_.size());
+        for (uint32_t id : ids) meshes_.push_back(MeshInfo{id});
+    }
+
+    size_t GetMeshIndexOffsetOfModel(const std::string& path) const {
+        auto it = modelToMeshIndexOffset_.find(path);
+        if (it == modelToMeshIndexOffset_.end()) throw std::runtime_error("model not added: " + path);
+        return it->second;
+    }
+
+    const MeshInfo& GetMeshInfo(size_t this_mesh_index) const { return meshes_.at(this_mesh_index); }
+    size_t size() const { return meshes_.size(); }
+
+private:
+    CollisionDetection& cd_;
+    std::vector<MeshInfo> meshes_;
+    std::unordered_map<std::string, size_t> modelToMeshIndexOffset_;
+};
+
 }  // namespace mi355x

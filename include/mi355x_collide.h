@@ -28,6 +28,9 @@ extern "C" {
 #define CD_E_HIP (-3)      /* HIP runtime error */
 #define CD_E_STACK (-4)    /* a traversal stack would overflow (tree deeper than supported) */
 #define CD_E_NOMEM (-5)    /* device allocation failed */
+#define CD_E_FORMAT (-6)   /* malformed file, or a layout the mesh ingest does not read */
+
+#define CD_NO_MESH 0xFFFFFFFFu /* a glTF mesh without collision triangles */
 
 /* ---- modes --------------------------------------------------------------- */
 /* Plain: reference semantics bit for bit (Möller test with its 1e-6 snap,
@@ -203,6 +206,60 @@ int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, 
  * skinned positions).  The reference keeps them in OBBtree::triangles_position
  * (include/Geometry/OBBtree.h:132).                                         */
 int cd_export_triangles(cd_ctx* ctx, uint32_t mesh, float* tri9);
+
+/* ---- glTF mesh ingest ---------------------------------------------------------------
+ * A parsed .gltf / .glb document in host memory.  Parsing needs no GPU.
+ * Replaces tinygltf's LoadASCIIFromFile / LoadBinaryFromFile as
+ * GameImporter::LoadglTFmodel calls them (src/GameImporter.cpp:468-500) and
+ * the POSITION / NORMAL / index reads of PrimitiveInitializationData
+ * (src/Graphics/Meshes/PrimitivesOfMeshes.cpp:43-137, 528-552, 695-752):
+ * positions and normals come out as (x, -y, -z); a line loop is read as a
+ * line strip; skinned (JOINTS_0) and position-morphed primitives are marked
+ * excluded and give no triangles (:637-671).  Interleaved (strided) views,
+ * sparse accessors and index types other than unsigned short / unsigned int
+ * are rejected with CD_E_FORMAT.                                            */
+typedef struct cd_gltf cd_gltf;
+
+typedef struct {
+    uint32_t nprim;            /* primitives, excluded ones included */
+    uint32_t with_normals;     /* included primitives with NORMAL */
+    uint32_t without_normals;  /* included primitives without */
+    uint32_t pad;
+    uint64_t nvert;            /* vertices of the included primitives */
+    uint64_t ntri;             /* collision triangles */
+} cd_gltf_mesh_info;
+
+typedef struct {
+    int32_t gltf_mode;         /* as written; -1 when absent */
+    uint32_t mode;             /* draw mode used (absent -> 4, line loop 2 -> line strip 3) */
+    uint32_t excluded;         /* 1: skinned or morphed */
+    uint32_t nvert;
+    uint64_t nindex;
+    uint64_t ntri;             /* triangles its mode makes (Triangle.cpp:10-62) */
+    const float* positions;    /* nvert x 3; owned by the document */
+    const float* normals;      /* nvert x 3, or NULL */
+    const uint32_t* indices;   /* nindex (0 .. nvert-1 for a primitive without indices) */
+} cd_gltf_primitive;
+
+/* err (may be NULL) receives a message of at most err_cap - 1 bytes. */
+int cd_gltf_open(const char* path, cd_gltf** doc, char* err, uint32_t err_cap);
+int cd_gltf_close(cd_gltf* doc);
+int cd_gltf_mesh_count(const cd_gltf* doc, uint32_t* n);
+int cd_gltf_mesh_info_get(const cd_gltf* doc, uint32_t mesh, cd_gltf_mesh_info* out);
+/* The k-th primitive of a mesh in the order its triangles are concatenated
+ * (MeshesOfNodes::AddMeshesOfModel's sort, src/Graphics/Meshes/MeshesOfNodes.cpp:41-43). */
+int cd_gltf_primitive_get(const cd_gltf* doc, uint32_t mesh, uint32_t k, cd_gltf_primitive* out);
+/* Builds the collision tree of every mesh of the document in one GPU pass:
+ * mesh_ids[m] (cap >= mesh count) receives glTF mesh m's id, or CD_NO_MESH
+ * when it has no collision triangles.  Replaces MeshesOfNodes::AddMeshesOfModel
+ * (MeshesOfNodes.cpp:29-64) -> PrimitivesOfMeshes::GetOBBtreeAndReset
+ * (PrimitivesOfMeshes.cpp:840-863) -> Triangle::CreateTriangleList
+ * (src/Geometry/Triangle.cpp:259-280) -> OBBtree::OBBtree.  Vertex ids (the
+ * rays' de-duplication keys) are the index-list entries offset by the
+ * primitive's first vertex in its mesh; vertex normals are the primitives'
+ * NORMAL, or face normals for a mesh without them (a mesh that mixes the two
+ * is rejected with CD_E_FORMAT).                                            */
+int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32_t cap);
 
 /* ---- deformable meshes ---------------------------------------------------------- */
 /* No counterpart in the reference: it gives trees only to non-skinned

@@ -21,6 +21,8 @@ CD_E_CAPACITY = -2
 CD_E_HIP = -3
 CD_E_STACK = -4
 CD_E_NOMEM = -5
+CD_E_FORMAT = -6
+CD_NO_MESH = 0xFFFFFFFF
 MODE_PLAIN = 0
 MODE_BATCHED = 1
 
@@ -30,7 +32,8 @@ EXPORTS = (
     "cd_export_tree", "cd_tritri_batch", "cd_query_batch", "cd_query_leafpairs", "cd_frame", "cd_broad_phase",
     "cd_stage_queries", "cd_run_staged", "cd_fetch_contacts", "cd_get_counters", "cd_get_timings",
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
-    "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays",
+    "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays", "cd_gltf_open", "cd_gltf_close",
+    "cd_gltf_mesh_count", "cd_gltf_mesh_info_get", "cd_gltf_primitive_get", "cd_register_gltf",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -87,6 +90,17 @@ class cd_tuning(C.Structure):
                 ("resplit_levels", C.c_uint32)]
 
 
+class cd_gltf_mesh_info(C.Structure):
+    _fields_ = [("nprim", C.c_uint32), ("with_normals", C.c_uint32), ("without_normals", C.c_uint32),
+                ("pad", C.c_uint32), ("nvert", C.c_uint64), ("ntri", C.c_uint64)]
+
+
+class cd_gltf_primitive(C.Structure):
+    _fields_ = [("gltf_mode", C.c_int32), ("mode", C.c_uint32), ("excluded", C.c_uint32), ("nvert", C.c_uint32),
+                ("nindex", C.c_uint64), ("ntri", C.c_uint64), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("indices", C.POINTER(C.c_uint32))]
+
+
 class CollideError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{msg} (status {code})")
@@ -132,6 +146,12 @@ def load_library() -> C.CDLL:
     lib.cd_register_skinned.argtypes = [vp, vp, u32, vp, u32, vp, vp, u32, P(u32)]
     lib.cd_skin_refit.argtypes = [vp, vp, u32, vp]
     lib.cd_get_tuning.argtypes = [vp, P(cd_tuning)]
+    lib.cd_gltf_open.argtypes = [C.c_char_p, P(vp), C.c_char_p, u32]
+    lib.cd_gltf_close.argtypes = [vp]
+    lib.cd_gltf_mesh_count.argtypes = [vp, P(u32)]
+    lib.cd_gltf_mesh_info_get.argtypes = [vp, u32, P(cd_gltf_mesh_info)]
+    lib.cd_gltf_primitive_get.argtypes = [vp, u32, u32, P(cd_gltf_primitive)]
+    lib.cd_register_gltf.argtypes = [vp, vp, vp, u32]
     for name in EXPORTS:
         if name != "cd_last_error":
             getattr(lib, name).restype = C.c_int
@@ -141,6 +161,65 @@ def load_library() -> C.CDLL:
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data
+
+
+class GltfDocument:
+    """A parsed .gltf / .glb file (cd_gltf_open); host memory only, no GPU needed."""
+
+    def __init__(self, path: str):
+        self.lib = load_library()
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = self.lib.cd_gltf_open(os.fsencode(path), C.byref(h), err, len(err))
+        if rc != CD_OK:
+            raise CollideError(rc, f"cd_gltf_open: {err.value.decode(errors='replace')}")
+        self.h = h
+        self.path = path
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cd_gltf_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self) -> int:
+        n = C.c_uint32()
+        if self.lib.cd_gltf_mesh_count(self.h, C.byref(n)) != CD_OK:
+            raise CollideError(CD_E_INVALID, "cd_gltf_mesh_count")
+        return n.value
+
+    def mesh_info(self, mesh: int) -> dict:
+        info = cd_gltf_mesh_info()
+        if self.lib.cd_gltf_mesh_info_get(self.h, mesh, C.byref(info)) != CD_OK:
+            raise CollideError(CD_E_INVALID, f"cd_gltf_mesh_info_get: no mesh {mesh}")
+        return {k: getattr(info, k) for k in ("nprim", "with_normals", "without_normals", "nvert", "ntri")}
+
+    def primitives(self, mesh: int) -> list[dict]:
+        """The mesh's primitives in concatenation order, as numpy copies."""
+        out = []
+        for k in range(self.mesh_info(mesh)["nprim"]):
+            p = cd_gltf_primitive()
+            if self.lib.cd_gltf_primitive_get(self.h, mesh, k, C.byref(p)) != CD_OK:
+                raise CollideError(CD_E_INVALID, f"cd_gltf_primitive_get: mesh {mesh} primitive {k}")
+            arr = lambda ptr, n, dt: (np.ctypeslib.as_array(ptr, (n,)).astype(dt, copy=True) if n and ptr
+                                      else np.zeros(0, dt))
+            rec = {"gltf_mode": p.gltf_mode, "mode": p.mode, "excluded": bool(p.excluded), "nvert": p.nvert,
+                   "ntri": p.ntri, "positions": None, "normals": None, "indices": None}
+            if not p.excluded:
+                rec["positions"] = arr(p.positions, 3 * p.nvert, np.float32).reshape(-1, 3)
+                rec["indices"] = arr(p.indices, p.nindex, np.uint32)
+                if p.normals:
+                    rec["normals"] = arr(p.normals, 3 * p.nvert, np.float32).reshape(-1, 3)
+            out.append(rec)
+        return out
 
 
 class Collider:
@@ -192,6 +271,12 @@ class Collider:
         self._check(self.lib.cd_register_soups(self.h, _ptr(tri), _ptr(counts), len(counts), C.byref(first)),
                     "cd_register_soups")
         return list(range(first.value, first.value + len(soups)))
+
+    def register_gltf(self, doc: GltfDocument) -> np.ndarray:
+        """Trees of every mesh of a parsed document; CD_NO_MESH where a mesh has no collision triangles."""
+        ids = np.zeros(max(1, len(doc)), np.uint32)
+        self._check(self.lib.cd_register_gltf(self.h, doc.h, _ptr(ids), len(ids)), "cd_register_gltf")
+        return ids[: len(doc)]
 
     def mesh_info(self, mesh: int) -> dict:
         info = cd_mesh_info()

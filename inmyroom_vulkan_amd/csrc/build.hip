@@ -454,7 +454,7 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
 static inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
 
 // Build every soup of the batch and append the trees to the context arenas.
-BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh) {
+BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh, bool tri9_on_device) {
     BuildResult res;
     hipStream_t st = ctx->stream;
     uint64_t T = 0, node_cap = 0;
@@ -491,7 +491,9 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     CK(base_nodes.ensure(nmesh * sizeof(uint32_t)));
     CK(base_tris.ensure(nmesh * sizeof(uint32_t)));
 
-    if (T) CK(hipMemcpyAsync(pos[0].p, tri9, T * 9 * sizeof(float), hipMemcpyHostToDevice, st));
+    if (T)
+        CK(hipMemcpyAsync(pos[0].p, tri9, T * 9 * sizeof(float),
+                          tri9_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     host_ids.resize(T);
     for (uint32_t m = 0; m < nmesh; ++m)
         for (uint32_t k = 0; k < counts[m]; ++k) host_ids[tri_begin[m] + k] = k;

@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "gltf.hpp"
 #include "kernels.hpp"
 
 using namespace cdk;
@@ -635,6 +636,201 @@ int cd_register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* tri_counts
     if (nmesh && !tri_counts) return fail(ctx, CD_E_INVALID, "null counts");
     (void)hipSetDevice(ctx->device);
     return register_soups(ctx, tri9, tri_counts, nmesh, first_id);
+}
+
+// ---- glTF mesh ingest ------------------------------------------------------------
+struct cd_gltf {
+    cdk::gltf::Document doc;
+};
+
+int cd_gltf_open(const char* path, cd_gltf** out, char* err, uint32_t err_cap) {
+    auto report = [&](const std::string& m) {
+        if (err && err_cap) {
+            const size_t n = std::min<size_t>(m.size(), err_cap - 1);
+            std::memcpy(err, m.data(), n);
+            err[n] = 0;
+        }
+    };
+    if (!path || !out) {
+        report("null argument");
+        return CD_E_INVALID;
+    }
+    *out = nullptr;
+    cd_gltf* g = new (std::nothrow) cd_gltf;
+    if (!g) {
+        report("out of host memory");
+        return CD_E_NOMEM;
+    }
+    std::string msg;
+    bool ok;
+    try {
+        ok = cdk::gltf::load(path, g->doc, msg);
+    } catch (const std::bad_alloc&) {
+        ok = false;
+        msg = "out of host memory";
+    }
+    if (!ok) {
+        delete g;
+        report(msg);
+        return CD_E_FORMAT;
+    }
+    report("");
+    *out = g;
+    return CD_OK;
+}
+
+int cd_gltf_close(cd_gltf* doc) {
+    delete doc;
+    return CD_OK;
+}
+
+int cd_gltf_mesh_count(const cd_gltf* doc, uint32_t* n) {
+    if (!doc || !n) return CD_E_INVALID;
+    *n = uint32_t(doc->doc.meshes.size());
+    return CD_OK;
+}
+
+int cd_gltf_mesh_info_get(const cd_gltf* doc, uint32_t mesh, cd_gltf_mesh_info* out) {
+    if (!doc || !out || mesh >= doc->doc.meshes.size()) return CD_E_INVALID;
+    const cdk::gltf::Mesh& m = doc->doc.meshes[mesh];
+    out->nprim = uint32_t(m.prims.size());
+    out->with_normals = m.with_normals;
+    out->without_normals = m.without_normals;
+    out->pad = 0;
+    out->nvert = m.nvert;
+    out->ntri = m.ntri;
+    return CD_OK;
+}
+
+int cd_gltf_primitive_get(const cd_gltf* doc, uint32_t mesh, uint32_t k, cd_gltf_primitive* out) {
+    if (!doc || !out || mesh >= doc->doc.meshes.size() || k >= doc->doc.meshes[mesh].prims.size()) return CD_E_INVALID;
+    const cdk::gltf::Primitive& p = doc->doc.meshes[mesh].prims[k];
+    out->gltf_mode = p.gltf_mode;
+    out->mode = p.mode;
+    out->excluded = p.excluded ? 1u : 0u;
+    out->nvert = p.nvert;
+    out->nindex = p.idx.size();
+    out->ntri = p.ntri;
+    out->positions = p.pos.empty() ? nullptr : p.pos.data();
+    out->normals = p.nrm.empty() ? nullptr : p.nrm.data();
+    out->indices = p.idx.empty() ? nullptr : p.idx.data();
+    return CD_OK;
+}
+
+int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32_t cap) {
+    if (!ctx) return CD_E_INVALID;
+    if (!doc) return fail(ctx, CD_E_INVALID, "null document");
+    const auto& meshes = doc->doc.meshes;
+    if (cap < meshes.size() || (!mesh_ids && !meshes.empty()))
+        return fail(ctx, CD_E_CAPACITY, "mesh id array shorter than the document's " + std::to_string(meshes.size()) +
+                                            " meshes");
+    // what one tree build takes: every mesh with triangles, its primitives in order
+    std::vector<uint32_t> reg, counts;
+    std::vector<PrimSpan> spans;
+    uint64_t T = 0, V = 0, N = 0;
+    for (uint32_t m = 0; m < meshes.size(); ++m) {
+        const auto& mesh = meshes[m];
+        if (mesh.ntri == 0) continue;
+        if (mesh.with_normals && mesh.without_normals)
+            return fail(ctx, CD_E_FORMAT, "mesh " + std::to_string(m) +
+                                              ": some primitives have NORMAL and some do not; the ray attributes "
+                                              "need one kind per mesh");
+        if (mesh.ntri > 0x7FFFFFFFull || mesh.nvert > 0x7FFFFFFFull)
+            return fail(ctx, CD_E_INVALID, "mesh " + std::to_string(m) + " too large");
+        uint64_t vid = 0;
+        for (const auto& p : mesh.prims) {
+            if (p.excluded) continue;
+            if (p.ntri) {
+                PrimSpan s{};
+                s.tri_off = uint32_t(T);
+                s.ntri = uint32_t(p.ntri);
+                s.idx_off = uint32_t(N);
+                s.mode = p.mode;
+                s.vbase = uint32_t(V);
+                s.vid_base = uint32_t(vid);
+                spans.push_back(s);
+            }
+            T += p.ntri;
+            V += p.nvert;
+            N += p.idx.size();
+            vid += p.nvert;
+            if (T >= 0xFFFFFFF0ull || V >= 0x55555550ull || N >= 0xFFFFFFF0ull)
+                return fail(ctx, CD_E_INVALID, "document too large for one ingest (32-bit indexing)");
+        }
+        reg.push_back(m);
+        counts.push_back(uint32_t(mesh.ntri));
+    }
+    for (uint32_t m = 0; m < meshes.size(); ++m) mesh_ids[m] = CD_NO_MESH;
+    if (reg.empty()) return CD_OK;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    // concatenated vertices and index lists of the included primitives
+    std::vector<float> pos;
+    std::vector<uint32_t> idx;
+    pos.reserve(size_t(V) * 3);
+    idx.reserve(size_t(N));
+    for (uint32_t m : reg)
+        for (const auto& p : meshes[m].prims) {
+            if (p.excluded) continue;
+            pos.insert(pos.end(), p.pos.begin(), p.pos.end());
+            idx.insert(idx.end(), p.idx.begin(), p.idx.end());
+        }
+    ScopedBuf d_pos, d_idx, d_spans, d_soup, d_vid3;
+    HCK(ctx, d_pos.ensure(pos.size() * sizeof(float) + 16));
+    HCK(ctx, d_idx.ensure(idx.size() * sizeof(uint32_t) + 16));
+    HCK(ctx, d_spans.ensure(spans.size() * sizeof(PrimSpan) + 16));
+    HCK(ctx, d_soup.ensure(size_t(T) * 9 * sizeof(float) + 16));
+    HCK(ctx, d_vid3.ensure(size_t(T) * 3 * sizeof(uint32_t) + 16));
+    HCK(ctx, hipMemcpyAsync(d_pos.p, pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HCK(ctx, hipMemcpyAsync(d_idx.p, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HCK(ctx, hipMemcpyAsync(d_spans.p, spans.data(), spans.size() * sizeof(PrimSpan), hipMemcpyHostToDevice, st));
+    HCK(ctx, launch_assemble(d_spans.as<PrimSpan>(), uint32_t(spans.size()), d_idx.as<uint32_t>(), d_pos.as<float>(),
+                             uint32_t(T), d_soup.as<float>(), d_vid3.as<uint32_t>(), st));
+    BuildResult r = build_soups(ctx, d_soup.as<float>(), counts.data(), uint32_t(reg.size()), true);
+    if (!r.err.empty()) return fail(ctx, CD_E_HIP, "tree build: " + r.err);
+    const uint32_t first = uint32_t(ctx->meshes.size());
+    ctx->meshes.insert(ctx->meshes.end(), r.meshes.begin(), r.meshes.end());
+    for (uint32_t m : reg) {  // plain soups (vertex ids 3 t + k) until their attributes are in
+        ctx->attrs.push_back(make_uint2(~0u, ~0u));
+        ctx->mesh_nvert.push_back(3u * uint32_t(meshes[m].ntri));
+    }
+    int rc = upload_mesh_table(ctx);
+    if (!rc) rc = upload_attrs(ctx);
+    if (rc) return rc;
+    // vertex ids in tree order, then the vertex normals of meshes that have them
+    const size_t nvid = size_t(T) * 3;
+    if (ctx->vid_used + nvid >= 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "vertex-id arena exceeds 32-bit indexing");
+    HCK(ctx, grow_keep(ctx->d_vid, (ctx->vid_used + nvid + 1) * sizeof(uint32_t), st));
+    uint64_t tri_at = 0;
+    for (size_t i = 0; i < reg.size(); ++i) {
+        const MeshRec& mr = ctx->meshes[first + i];
+        HCK(ctx, launch_tree_vidx(ctx->d_perm.as<uint32_t>() + mr.tri_base, mr.ntri, d_vid3.as<uint32_t>() + tri_at * 3,
+                                  0u, ctx->d_vid.as<uint32_t>() + ctx->vid_used, st));
+        const auto& mesh = meshes[reg[i]];
+        uint2 attr = make_uint2(uint32_t(ctx->vid_used), ~0u);
+        ctx->vid_used += size_t(mr.ntri) * 3;
+        tri_at += mr.ntri;
+        if (mesh.with_normals) {
+            const size_t n = size_t(mesh.nvert) * 3;
+            if (ctx->nrm_used + n >= 0xFFFFFFF0ull)
+                return fail(ctx, CD_E_CAPACITY, "normal arena exceeds 32-bit indexing");
+            HCK(ctx, grow_keep(ctx->d_nrm, (ctx->nrm_used + n + 1) * sizeof(float), st));
+            size_t at = ctx->nrm_used;
+            for (const auto& p : mesh.prims) {
+                if (p.excluded) continue;
+                HCK(ctx, hipMemcpyAsync(ctx->d_nrm.as<float>() + at, p.nrm.data(), p.nrm.size() * sizeof(float),
+                                        hipMemcpyHostToDevice, st));
+                at += p.nrm.size();
+            }
+            attr.y = uint32_t(ctx->nrm_used);
+            ctx->nrm_used += n;
+        }
+        ctx->attrs[first + i] = attr;
+        ctx->mesh_nvert[first + i] = uint32_t(mesh.nvert);
+        mesh_ids[reg[i]] = first + uint32_t(i);
+    }
+    HCK(ctx, hipStreamSynchronize(st));
+    return upload_attrs(ctx);
 }
 
 int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out) {

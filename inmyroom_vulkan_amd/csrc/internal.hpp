@@ -178,6 +178,14 @@ struct ScopedBuf : DevBuf {
     ~ScopedBuf() { release(); }
 };
 
+// One glTF primitive of the mesh ingest: its triangles' place in the output,
+// its index list and its vertices (in the concatenated arrays), and the id of
+// its first vertex within its mesh.
+struct PrimSpan {
+    uint32_t tri_off, ntri, idx_off, mode, vbase, vid_base, pad[2];
+};
+static_assert(sizeof(PrimSpan) == 32, "primitive span layout");
+
 // Device-side counters of one run (zeroed by a fill kernel, never by a
 // memset, so a run can be captured into a HIP graph).
 struct RunCounters {

@@ -5,8 +5,13 @@
 
 namespace cdk {
 
-// build.hip
-BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh);
+// build.hip (tri9 in host memory, or already on the device)
+BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh,
+                        bool tri9_on_device = false);
+
+// ingest.hip
+hipError_t launch_assemble(const PrimSpan* prims, uint32_t nprim, const uint32_t* idx, const float* pos, uint32_t total,
+                           float* tri9, uint32_t* vid3, hipStream_t st);
 
 // plain.hip
 hipError_t launch_zero(RunCounters* c, hipStream_t st);

@@ -106,3 +106,58 @@ def test_mirror_frame_to_callbacks(tmp_path, orc):
     assert (c5[3], c5[5]) == ("5", "7") and (c7[3], c7[5]) == ("7", "5")
     assert all(abs(float(v)) == 0.0 for v in c5[6:9])  # the static cube takes no share
     assert [float(v) for v in c7[6:9]] == delta  # the moving cube takes all of it
+
+
+GLTF_PROGRAM = r'''
+#include "CollisionDetection.hpp"
+#include <cstdio>
+int main(int argc, char** argv) {
+    mi355x::CollisionDetection cd(4, CD_MODE_PLAIN);
+    mi355x::MeshesOfNodes meshes(cd);
+    meshes.AddMeshesOfModel(argv[1]);  // Apple: one mesh
+    meshes.AddMeshesOfModel(argv[2]);  // Table_room: 46 meshes, 4 skinned
+    const size_t apple = meshes.GetMeshIndexOffsetOfModel(argv[1]);
+    const size_t room = meshes.GetMeshIndexOffsetOfModel(argv[2]);
+    std::printf("meshes %zu apple %zu room %zu\n", meshes.size(), apple, room);
+    int none = 0;
+    for (size_t i = room; i < meshes.size(); ++i) none += meshes.GetMeshInfo(i).boundBoxTree == CD_NO_MESH;
+    std::printf("without trees %d\n", none);
+    auto entry = [&](mi355x::Entity ent, float x) {
+        mi355x::CollisionDetectionEntry e{};
+        for (int i = 0; i < 16; ++i) e.currentGlobalMatrix[i] = e.previousGlobalMatrix[i] = (i % 5 == 0) ? 1.f : 0.f;
+        e.currentGlobalMatrix[12] = x;
+        e.previousGlobalMatrix[12] = x + 0.5f;
+        e.meshId = meshes.GetMeshInfo(apple).boundBoxTree;
+        e.shouldCallback = true;
+        e.entity = ent;
+        return e;
+    };
+    cd.Reset();
+    cd.AddCollisionDetectionEntry(entry(1, 0.f));
+    cd.AddCollisionDetectionEntry(entry(2, 1.2f));   // overlaps the first
+    cd.AddCollisionDetectionEntry(entry(3, 10.f));   // far away
+    cd.ExecuteCollisionDetection();
+    for (const auto& r : cd.GetCollidingPairs()) std::printf("pair %d %d shot %d\n", r.first, r.second, int(r.shot));
+    return 0;
+}
+'''
+
+
+def test_mirror_meshes_of_nodes_gltf(tmp_path):
+    case("C++ mirror, MeshesOfNodes from glTF files")
+    from conftest import GOLDEN
+    from inmyroom_vulkan_amd import collide
+    src = tmp_path / "gltf.cpp"
+    src.write_text(GLTF_PROGRAM)
+    exe = tmp_path / "gltf"
+    libdir = os.path.dirname(collide.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir,
+                    "-lmi355x_collide", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    apple = os.path.join(GOLDEN, "gltf", "apple", "scene.gltf")
+    room = os.path.join(GOLDEN, "gltf", "table_room.gltf")
+    out = subprocess.run([str(exe), apple, room], check=True, capture_output=True, text=True, timeout=60).stdout
+    lines = out.split("\n")
+    assert lines[0] == "meshes 47 apple 0 room 1"
+    assert lines[1] == "without trees 4"
+    pairs = [l.split() for l in lines if l.startswith("pair")]
+    assert [p[1:3] for p in pairs] == [["1", "2"]] and pairs[0][4] == "1"
