@@ -136,8 +136,9 @@ def main() -> int:
     ap.add_argument("--exact-sample", type=int, default=48, help="queries checked against the exact oracle")
     ap.add_argument("--copy-probe", type=int, default=0, help="bytes of a calibration copy run first (PMC runs)")
     ap.add_argument("--tuning", default="", help="override tuning fields, e.g. work_sharing=1,split_levels=8")
-    ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5),
-                    help="3: the headline batch; 2, 4, 5: scene frames, triangle-pair stream, deformable batch")
+    ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5, 6),
+                    help="3: the headline batch; 2, 4, 5, 6: scene frames, triangle-pair stream, deformable batch, "
+                         "glTF ingest")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the batch shards by query "

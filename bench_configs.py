@@ -9,6 +9,9 @@
   5  deformable batch: 64 skinned tori (20,000 triangles, 64 bones, 4 weights
      per vertex) skinned and refit every frame, then tested against a
      316 x 316 fBm terrain in one batched launch
+  6  glTF mesh ingest: the three shipped scenes (Apple, Table_room, Sponza
+     environmentTest: 50 meshes, 64,842 collision triangles) parsed and
+     given their trees, one document after another, as a model load does
 
 Each prints one JSON line in bench.py's format.  The oracle (CPU
 restatement) only appears as the cpu_baseline / mismatch checker.
@@ -25,10 +28,10 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _line(metric, value, unit, ms, args, config, dtype="f32", extra=None):
+def _line(metric, value, unit, ms, args, config, dtype="f32", extra=None, data="synthetic"):
     d = {"metric": metric, "value": round(value, 2), "unit": unit, "n_gpus": 1, "steps": args.steps,
          "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-         "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+         "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
     d.update(extra or {})
     return d
 
@@ -187,9 +190,68 @@ def config5(args):
                         "contacts_per_frame": ncon / max(1, len(wall))})
 
 
+def config6(args):
+    import os
+
+    from inmyroom_vulkan_amd import Collider, GltfDocument
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    paths = [os.path.join(here, "tests", "golden", "gltf", p)
+             for p in ("apple/scene.gltf", "table_room.gltf", "environment.gltf")]
+    gpu = Collider(leaf_max=4)
+    parse_ms, reg_ms, ntri = [], [], 0
+    for k in range(args.warmup + args.steps):
+        pm = rm = 0.0
+        n = 0
+        for p in paths:
+            t = time.perf_counter()
+            doc = GltfDocument(p)
+            t1 = time.perf_counter()
+            ids = gpu.register_gltf(doc)
+            t2 = time.perf_counter()
+            n += sum(doc.mesh_info(m)["ntri"] for m in range(len(doc)))
+            doc.close()
+            pm += (t1 - t) * 1e3
+            rm += (t2 - t1) * 1e3
+        if k >= args.warmup:
+            parse_ms.append(pm)
+            reg_ms.append(rm)
+            ntri = n
+    ms = float(np.median(np.array(parse_ms) + np.array(reg_ms)))
+    log(f"config 6: {ms:.3f} ms per load of {ntri:,} triangles (parse {np.median(parse_ms):.3f}, "
+        f"assemble + tree build {np.median(reg_ms):.3f})")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import Oracle
+        from oracle import gltf as og
+
+        orc = Oracle()
+        t = time.perf_counter()
+        n = loads = 0
+        while True:
+            for p in paths:
+                for prims in og.load(p):
+                    soup = og.mesh_triangles(prims)[0]
+                    if len(soup):
+                        orc.tree(soup, 4)
+                        n += len(soup)
+            loads += 1
+            if time.perf_counter() - t > args.cpu_seconds:
+                break
+        el = time.perf_counter() - t
+        cpu = {"value": round(n / el, 1), "unit": "triangles/s", "cores": 1, "kind": "port",
+               "sample": f"{loads} loads of the three scenes: numpy decode and the oracle's tree build, single thread"}
+    return _line("glTF ingest triangles/s (parse, assemble, OBB-tree build)", ntri / (ms * 1e-3), "triangles/s", ms,
+                 args, {"workload": "Apple + Table_room + Sponza environmentTest glTF scenes: 50 meshes, "
+                                    f"{ntri:,} collision triangles", "parallelism": "single GPU"},
+                 extra={"parse_ms": round(float(np.median(parse_ms)), 4),
+                        "assemble_build_ms": round(float(np.median(reg_ms)), 4), "cpu_baseline": cpu},
+                 data="the reference's shipped glTF scenes (tests/golden/gltf)")
+
+
 def run(args):
     if not hasattr(args, "pairs"):
         args.pairs = 1 << 24
     if not hasattr(args, "oracle_pairs"):
         args.oracle_pairs = 1 << 18
-    return {2: config2, 4: config4, 5: config5}[args.config](args)
+    return {2: config2, 4: config4, 5: config5, 6: config6}[args.config](args)

@@ -261,6 +261,38 @@ int cd_gltf_primitive_get(const cd_gltf* doc, uint32_t mesh, uint32_t k, cd_gltf
  * is rejected with CD_E_FORMAT).                                            */
 int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32_t cap);
 
+/* ---- node transforms ----------------------------------------------------------------
+ * A node's NodeData (include/ECS/GeneralCompEntities/NodeDataCompEntity.h:47-50). */
+#define CD_NO_PARENT 0xFFFFFFFFu
+
+typedef struct {
+    float local_scale[3];
+    float local_rotation[4];     /* unit quaternion (x, y, z, w) */
+    float local_translation[3];
+    float global_translation[3];
+    uint32_t parent;             /* index in the same array, or CD_NO_PARENT */
+    uint32_t pad[2];
+} cd_node;                       /* 64 bytes */
+
+/* Global matrices of a node hierarchy on the GPU, 16 floats (column-major)
+ * per node: gT * parent_global * T * R * S, from each root down.  Replaces
+ * NodeDataCompEntity::GetGlobalMatrix (src/ECS/GeneralCompEntities/
+ * NodeDataCompEntity.cpp:63-72) as Early/LateNodeGlobalMatrixCompEntity::
+ * Update apply it (EarlyNodeGlobalMatrixCompEntity.cpp:63-78).  Nodes may be
+ * in any order; a chain deeper than 256 or a cycle is rejected.           */
+int cd_node_global_matrices(cd_ctx* ctx, const cd_node* nodes, uint32_t n, float* global16);
+
+/* The glTF nodes as the reference imports them (GameImporter::
+ * CreatePositionInitMap, src/GameImporter.cpp:503-552, and NodeDataCompEntity::
+ * CreateComponentEntityByMap, NodeDataCompEntity.cpp:19-61): TRS in
+ * (x, -y, -z) space, a matrix node decomposed as glm::decompose does, the
+ * rotation normalized; parents from the children lists.  mesh[i] is node i's
+ * glTF mesh or -1; collides[i] is 1 when the node gets a collision component
+ * (a mesh that is neither skinned nor morphed, GameImporter.cpp:320-326).
+ * Any of nodes, mesh, collides may be NULL.                                 */
+int cd_gltf_node_count(const cd_gltf* doc, uint32_t* n);
+int cd_gltf_nodes(const cd_gltf* doc, cd_node* nodes, int32_t* mesh, uint32_t* collides, uint32_t cap);
+
 /* ---- deformable meshes ---------------------------------------------------------- */
 /* No counterpart in the reference: it gives trees only to non-skinned
  * primitives (src/Graphics/Meshes/PrimitivesOfMeshes.cpp:642) and never

@@ -7,8 +7,8 @@ phase, contact aggregation into uncollide rays, skinning and refit, glTF mesh in
 synthetic workloads.
 """
 from .collide import (CONTACT_DTYPE, ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, PAIR_DTYPE, PAIR_RAYS_DTYPE,
-                      PAIR_RESPONSE_DTYPE, QUERY_DTYPE, RAY_DTYPE, CD_NO_MESH, CollideError, Collider, GltfDocument,
-                      load_library)
+                      PAIR_RESPONSE_DTYPE, QUERY_DTYPE, RAY_DTYPE, NODE_DTYPE, CD_NO_MESH, CD_NO_PARENT, CollideError,
+                      Collider, GltfDocument, load_library)
 
-__all__ = ["Collider", "CollideError", "GltfDocument", "CD_NO_MESH", "load_library", "CONTACT_DTYPE", "QUERY_DTYPE", "ENTITY_DTYPE", "PAIR_DTYPE",
+__all__ = ["Collider", "CollideError", "GltfDocument", "CD_NO_MESH", "CD_NO_PARENT", "NODE_DTYPE", "load_library", "CONTACT_DTYPE", "QUERY_DTYPE", "ENTITY_DTYPE", "PAIR_DTYPE",
            "RAY_DTYPE", "PAIR_RAYS_DTYPE", "PAIR_RESPONSE_DTYPE", "MODE_PLAIN", "MODE_BATCHED"]

@@ -34,6 +34,7 @@ EXPORTS = (
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
     "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays", "cd_gltf_open", "cd_gltf_close",
     "cd_gltf_mesh_count", "cd_gltf_mesh_info_get", "cd_gltf_primitive_get", "cd_register_gltf",
+    "cd_node_global_matrices", "cd_gltf_node_count", "cd_gltf_nodes",
 )
 
 CONTACT_DTYPE = np.dtype([
@@ -55,7 +56,13 @@ PAIR_RESPONSE_DTYPE = np.dtype([
     ("first", "<u4"), ("second", "<u4"), ("shot", "<u4"), ("pad", "<u4"), ("delta", "<f4", 3),
     ("delta_first", "<f4", 3), ("delta_second", "<f4", 3), ("pad2", "<f4", 3),
 ])
+NODE_DTYPE = np.dtype([
+    ("local_scale", "<f4", 3), ("local_rotation", "<f4", 4), ("local_translation", "<f4", 3),
+    ("global_translation", "<f4", 3), ("parent", "<u4"), ("pad", "<u4", 2),
+])
+CD_NO_PARENT = 0xFFFFFFFF
 assert CONTACT_DTYPE.itemsize == 64 and QUERY_DTYPE.itemsize == 72 and ENTITY_DTYPE.itemsize == 136
+assert NODE_DTYPE.itemsize == 64
 assert RAY_DTYPE.itemsize == 24 and PAIR_RAYS_DTYPE.itemsize == 64 and PAIR_RESPONSE_DTYPE.itemsize == 64
 
 
@@ -152,6 +159,9 @@ def load_library() -> C.CDLL:
     lib.cd_gltf_mesh_info_get.argtypes = [vp, u32, P(cd_gltf_mesh_info)]
     lib.cd_gltf_primitive_get.argtypes = [vp, u32, u32, P(cd_gltf_primitive)]
     lib.cd_register_gltf.argtypes = [vp, vp, vp, u32]
+    lib.cd_node_global_matrices.argtypes = [vp, vp, u32, vp]
+    lib.cd_gltf_node_count.argtypes = [vp, P(u32)]
+    lib.cd_gltf_nodes.argtypes = [vp, vp, vp, vp, u32]
     for name in EXPORTS:
         if name != "cd_last_error":
             getattr(lib, name).restype = C.c_int
@@ -201,6 +211,19 @@ class GltfDocument:
         if self.lib.cd_gltf_mesh_info_get(self.h, mesh, C.byref(info)) != CD_OK:
             raise CollideError(CD_E_INVALID, f"cd_gltf_mesh_info_get: no mesh {mesh}")
         return {k: getattr(info, k) for k in ("nprim", "with_normals", "without_normals", "nvert", "ntri")}
+
+    def nodes(self):
+        """(NODE_DTYPE array, glTF mesh per node or -1, collides per node) as the nodes are imported."""
+        n = C.c_uint32()
+        if self.lib.cd_gltf_node_count(self.h, C.byref(n)) != CD_OK:
+            raise CollideError(CD_E_INVALID, "cd_gltf_node_count")
+        nodes = np.zeros(n.value, NODE_DTYPE)
+        mesh = np.zeros(n.value, np.int32)
+        coll = np.zeros(n.value, np.uint32)
+        rc = self.lib.cd_gltf_nodes(self.h, _ptr(nodes), _ptr(mesh), _ptr(coll), n.value)
+        if rc != CD_OK:
+            raise CollideError(rc, "cd_gltf_nodes")
+        return nodes, mesh, coll.astype(bool)
 
     def primitives(self, mesh: int) -> list[dict]:
         """The mesh's primitives in concatenation order, as numpy copies."""
@@ -277,6 +300,13 @@ class Collider:
         ids = np.zeros(max(1, len(doc)), np.uint32)
         self._check(self.lib.cd_register_gltf(self.h, doc.h, _ptr(ids), len(ids)), "cd_register_gltf")
         return ids[: len(doc)]
+
+    def node_globals(self, nodes: np.ndarray) -> np.ndarray:
+        """Global matrices (n, 16) of a node hierarchy (NODE_DTYPE records)."""
+        nd = np.ascontiguousarray(nodes, NODE_DTYPE)
+        out = np.zeros((len(nd), 16), np.float32)
+        self._check(self.lib.cd_node_global_matrices(self.h, _ptr(nd), len(nd), _ptr(out)), "cd_node_global_matrices")
+        return out
 
     def mesh_info(self, mesh: int) -> dict:
         info = cd_mesh_info()

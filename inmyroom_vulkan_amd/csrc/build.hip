@@ -9,7 +9,7 @@
 //   * the distinct-point check is a wave ballot;
 //   * the mean and the six covariance sums are SEQUENTIAL double chains in
 //     point order (the reference's std::accumulate / std::inner_product),
-//     one lane per chain, fed from LDS-staged chunks of the node's points;
+//     one lane per chain adding terms the whole wave precomputes per LDS chunk;
 //   * eig3 runs in FP64 on one lane;
 //   * the three min/max projections are wave reductions whose tie rule
 //     reproduces std::min/std::max visiting the points in order;
@@ -50,9 +50,60 @@ struct LevelArgs {
     uint32_t leaf_max;
 };
 
-constexpr int CHUNK = 256;  // points staged in LDS per step of the sequential chains
+constexpr int CHUNK = 256;                 // points per step of the sequential chains
+constexpr int CHUNK_REGS = CHUNK * 3 / 64;  // floats each lane fetches per chunk
+static_assert(CHUNK * 3 % 64 == 0 && CHUNK % 64 == 0, "chunk split over a wave");
 
-__device__ inline void wave_min_last(double& v, int& i) {
+// The reference's sequential double sums over a node's points (OBB.cpp:
+// 41-73: std::accumulate, then the six inner products), one chain per lane,
+// in point order.  Only the final additions form the chain; the terms do
+// not depend on it.  So the wave streams the points through LDS a chunk at a
+// time: every lane fetches its share of the next chunk, all lanes turn the
+// current chunk into its K terms per point (term(p, c, out) for the points
+// lane, lane + 64, ...), and the first K lanes add their term of each point
+// in order while the fetch is in flight.  The block is one wave.
+template <int K, class Term>
+__device__ __forceinline__ double stream_chain(const float* __restrict__ P, uint32_t npts, Term term, float* s_raw,
+                                               double* s_val) {
+    const int lane = threadIdx.x;
+    float r[CHUNK_REGS];
+    auto fetch = [&](uint32_t base) {
+        const uint32_t m3 = 3u * min(uint32_t(CHUNK), npts - base);
+#pragma unroll
+        for (int j = 0; j < CHUNK_REGS; ++j) {
+            const uint32_t t = uint32_t(lane) + 64u * j;
+            r[j] = t < m3 ? P[3 * size_t(base) + t] : 0.f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < CHUNK_REGS; ++j) s_raw[lane + 64 * j] = r[j];
+    };
+    fetch(0);
+    store();
+    __syncthreads();
+    double acc = 0.0;
+    for (uint32_t base = 0; base < npts; base += CHUNK) {
+        const uint32_t next = base + CHUNK;
+        const uint32_t m = min(uint32_t(CHUNK), npts - base);
+        if (next < npts) fetch(next);
+#pragma unroll
+        for (int j = 0; j < CHUNK / 64; ++j) {
+            const uint32_t k = uint32_t(lane) + 64u * j;
+            if (k < m) term(s_raw + 3 * k, s_val + K * k);
+        }
+        __syncthreads();
+        if (lane < K) {
+            const double* q = s_val + lane;
+#pragma unroll 16
+            for (uint32_t k = 0; k < m; ++k) acc = acc + q[K * k];
+        }
+        __syncthreads();
+        if (next < npts) store();
+        __syncthreads();
+    }
+    return acc;
+}__device__ inline void wave_min_last(double& v, int& i) {
     // smaller value wins; equal values (== , so -0 and +0 too): larger index wins
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -118,7 +169,8 @@ __device__ inline bool goes_left(const float* T, f3 axis, float cproj) {
 }
 
 __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint32_t next_first) {
-    __shared__ float s_pts[CHUNK * 3];
+    __shared__ float s_raw[CHUNK * 3];
+    __shared__ double s_val[CHUNK * 6];
     __shared__ double s_mean[3];
     __shared__ double s_cov[6];
     __shared__ double s_ax[9];
@@ -132,11 +184,16 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
     const float* P = a.pos_in + size_t(beg) * 9;
 
     // ---- OBB::CreateOBBfromPoints (OBB.cpp:33-89) ----
-    int differs = 0;
-    for (uint32_t k = 1 + lane; k < npts; k += 64) {
-        if (!(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2])) differs = 1;
+    // more than one distinct point?  (the first wave-wide chunk that shows one ends the scan)
+    bool multi = false;
+    if (npts > 1) {
+        const float x0 = P[0], y0 = P[1], z0 = P[2];
+        for (uint32_t k0 = 1; k0 < npts && !multi; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            const bool d = k < npts && !(P[3 * k] == x0 && P[3 * k + 1] == y0 && P[3 * k + 2] == z0);
+            multi = __ballot(d) != 0;
+        }
     }
-    const bool multi = __ballot(differs) != 0;
     Box box;
     if (npts == 0 || !multi) {
         const float e = FLT_EPSILON;  // OBB::EmptyOBB (OBB.cpp:168-178)
@@ -146,38 +203,30 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
         box.w = mk(0.f, 0.f, e);
     } else {
         const double N = double(npts);
-        double acc = 0.0;
-        for (uint32_t base = 0; base < npts; base += CHUNK) {
-            const uint32_t m = min(uint32_t(CHUNK), npts - base);
-            for (uint32_t t = lane; t < 3 * m; t += 64) s_pts[t] = P[3 * size_t(base) + t];
-            __syncthreads();
-            if (lane < 3) {
-#pragma unroll 8
-                for (uint32_t k = 0; k < m; ++k) acc = acc + double(s_pts[3 * k + lane]);
-            }
-            __syncthreads();
-        }
+        double acc = stream_chain<3>(
+            P, npts,
+            [](const float* p, double* o) {
+                o[0] = double(p[0]);
+                o[1] = double(p[1]);
+                o[2] = double(p[2]);
+            },
+            s_raw, s_val);
         if (lane < 3) s_mean[lane] = acc / N;
         __syncthreads();
         // covariance order xx, yy, zz, xy, xz, yz (OBB.cpp:56-73)
-        const int ca = lane == 3 || lane == 4 ? 0 : (lane == 5 ? 1 : (lane < 3 ? lane : 0));
-        const int cb = lane == 3 ? 1 : (lane == 4 || lane == 5 ? 2 : (lane < 3 ? lane : 0));
-        const double ma = s_mean[ca], mb = s_mean[cb];
-        acc = 0.0;
-        for (uint32_t base = 0; base < npts; base += CHUNK) {
-            const uint32_t m = min(uint32_t(CHUNK), npts - base);
-            for (uint32_t t = lane; t < 3 * m; t += 64) s_pts[t] = P[3 * size_t(base) + t];
-            __syncthreads();
-            if (lane < 6) {
-#pragma unroll 8
-                for (uint32_t k = 0; k < m; ++k) {
-                    double da = double(s_pts[3 * k + ca]) - ma;
-                    double db = double(s_pts[3 * k + cb]) - mb;
-                    acc = acc + da * db;
-                }
-            }
-            __syncthreads();
-        }
+        const double mx = s_mean[0], my = s_mean[1], mz = s_mean[2];
+        acc = stream_chain<6>(
+            P, npts,
+            [mx, my, mz](const float* p, double* o) {
+                const double dx = double(p[0]) - mx, dy = double(p[1]) - my, dz = double(p[2]) - mz;
+                o[0] = dx * dx;
+                o[1] = dy * dy;
+                o[2] = dz * dz;
+                o[3] = dx * dy;
+                o[4] = dx * dz;
+                o[5] = dy * dz;
+            },
+            s_raw, s_val);
         if (lane < 6) s_cov[lane] = acc / N;
         __syncthreads();
         if (lane == 0) {
@@ -203,6 +252,7 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
             loi[q] = -1;
             hii[q] = -1;
         }
+#pragma unroll 4
         for (uint32_t k = lane; k < npts; k += 64) {
             const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
 #pragma unroll
@@ -413,6 +463,7 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
     double worst = 0.0, rad = 0.0;
     const uint32_t npts = 3 * (nd.end - nd.beg);
     const float* P = pos_fin + size_t(nd.beg) * 9;
+#pragma unroll 4
     for (uint32_t k = lane; k < npts; k += 64) {
         const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
         const double r[3] = {p[0] - nd.box[0], p[1] - nd.box[1], p[2] - nd.box[2]};

@@ -833,6 +833,74 @@ int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32
     return upload_attrs(ctx);
 }
 
+int cd_gltf_node_count(const cd_gltf* doc, uint32_t* n) {
+    if (!doc || !n) return CD_E_INVALID;
+    *n = uint32_t(doc->doc.nodes.size());
+    return CD_OK;
+}
+
+int cd_gltf_nodes(const cd_gltf* doc, cd_node* nodes, int32_t* mesh, uint32_t* collides, uint32_t cap) {
+    if (!doc) return CD_E_INVALID;
+    const auto& nv = doc->doc.nodes;
+    if (cap < nv.size()) return CD_E_CAPACITY;
+    for (size_t i = 0; i < nv.size(); ++i) {
+        if (nodes) {
+            cd_node& o = nodes[i];
+            std::memset(&o, 0, sizeof o);
+            std::memcpy(o.local_scale, nv[i].scale, sizeof o.local_scale);
+            std::memcpy(o.local_rotation, nv[i].rotation, sizeof o.local_rotation);
+            std::memcpy(o.local_translation, nv[i].translation, sizeof o.local_translation);
+            o.parent = nv[i].parent;
+        }
+        if (mesh) mesh[i] = nv[i].mesh;
+        if (collides) collides[i] = nv[i].collides ? 1u : 0u;
+    }
+    return CD_OK;
+}
+
+int cd_node_global_matrices(cd_ctx* ctx, const cd_node* nodes, uint32_t n, float* global16) {
+    if (!ctx) return CD_E_INVALID;
+    if (n && (!nodes || !global16)) return fail(ctx, CD_E_INVALID, "null node arrays");
+    // depth of every node; parents out of range, cycles and chains deeper than 256 are rejected
+    constexpr uint32_t MAX_DEPTH = 256;
+    std::vector<uint32_t> depth(n, ~0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!all_finite(nodes[i].local_scale, 13)) return fail(ctx, CD_E_INVALID, "non-finite node transform");
+        if (nodes[i].parent != CD_NO_PARENT && nodes[i].parent >= n)
+            return fail(ctx, CD_E_INVALID, "node " + std::to_string(i) + ": parent out of range");
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t a = i, steps = 0;
+        while (depth[a] == ~0u && nodes[a].parent != CD_NO_PARENT) {
+            a = nodes[a].parent;
+            if (++steps > MAX_DEPTH) return fail(ctx, CD_E_INVALID, "node hierarchy deeper than 256 or cyclic");
+        }
+        uint32_t d = depth[a] == ~0u ? 0u : depth[a];
+        if (depth[a] == ~0u) depth[a] = 0;
+        // walk again from i, writing depths top-down
+        std::vector<uint32_t> chain;
+        for (uint32_t b = i; b != a; b = nodes[b].parent) chain.push_back(b);
+        for (size_t k = chain.size(); k-- > 0;) {
+            d += 1;
+            if (d > MAX_DEPTH) return fail(ctx, CD_E_INVALID, "node hierarchy deeper than 256 or cyclic");
+            depth[chain[k]] = d;
+        }
+    }
+    if (n == 0) return CD_OK;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    ScopedBuf d_nodes, d_depth, d_out;
+    HCK(ctx, d_nodes.ensure(size_t(n) * sizeof(cd_node)));
+    HCK(ctx, d_depth.ensure(size_t(n) * sizeof(uint32_t)));
+    HCK(ctx, d_out.ensure(size_t(n) * 16 * sizeof(float)));
+    HCK(ctx, hipMemcpyAsync(d_nodes.p, nodes, size_t(n) * sizeof(cd_node), hipMemcpyHostToDevice, st));
+    HCK(ctx, hipMemcpyAsync(d_depth.p, depth.data(), size_t(n) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HCK(ctx, launch_node_globals(d_nodes.as<cd_node>(), d_depth.as<uint32_t>(), n, d_out.as<float>(), st));
+    HCK(ctx, hipMemcpyAsync(global16, d_out.p, size_t(n) * 16 * sizeof(float), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    return CD_OK;
+}
+
 int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out) {
     if (!ctx || !out) return CD_E_INVALID;
     if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");

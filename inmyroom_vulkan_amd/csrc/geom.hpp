@@ -73,6 +73,61 @@ CD_HD M4 mat_mul(const M4& A, const M4& B) {
     return R;
 }
 
+// Node transforms (NodeDataCompEntity::GetGlobalMatrix, src/ECS/
+// GeneralCompEntities/NodeDataCompEntity.cpp:63-72) with glm's builders
+// applied to the identity exactly as written there, zero signs included:
+// translate: column 3 = ((I0 v.x + I1 v.y) + I2 v.z) + I3; scale: column i =
+// Ii v[i]; toMat4 of a quaternion (x, y, z, w) = mat3_cast's nine terms.
+CD_HD M4 identity4() {
+    M4 I;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) I.m[i] = (i % 5 == 0) ? 1.f : 0.f;
+    return I;
+}
+CD_HD M4 translate_identity(float x, float y, float z) {
+    M4 R = identity4();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float i0 = r == 0 ? 1.f : 0.f, i1 = r == 1 ? 1.f : 0.f, i2 = r == 2 ? 1.f : 0.f, i3 = r == 3 ? 1.f : 0.f;
+        R.m[12 + r] = ((i0 * x + i1 * y) + i2 * z) + i3;
+    }
+    return R;
+}
+CD_HD M4 scale_identity(float x, float y, float z) {
+    M4 R = identity4();
+    const float v[3] = {x, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) R.m[c * 4 + r] = (r == c ? 1.f : 0.f) * v[c];
+    return R;
+}
+CD_HD M4 quat_to_mat4(float qx, float qy, float qz, float qw) {
+    const float qxx = qx * qx, qyy = qy * qy, qzz = qz * qz;
+    const float qxz = qx * qz, qxy = qx * qy, qyz = qy * qz;
+    const float qwx = qw * qx, qwy = qw * qy, qwz = qw * qz;
+    M4 R = identity4();
+    R.m[0] = 1.f - 2.f * (qyy + qzz);
+    R.m[1] = 2.f * (qxy + qwz);
+    R.m[2] = 2.f * (qxz - qwy);
+    R.m[4] = 2.f * (qxy - qwz);
+    R.m[5] = 1.f - 2.f * (qxx + qzz);
+    R.m[6] = 2.f * (qyz + qwx);
+    R.m[8] = 2.f * (qxz + qwy);
+    R.m[9] = 2.f * (qyz - qwx);
+    R.m[10] = 1.f - 2.f * (qxx + qyy);
+    return R;
+}
+// gT * parent * T * R * S, multiplied left to right as the reference writes it.
+// n: scale[3], rotation (x, y, z, w), translation[3], global translation[3].
+CD_HD M4 node_global(const float* n, const M4& parent) {
+    const M4 S = scale_identity(n[0], n[1], n[2]);
+    const M4 R = quat_to_mat4(n[3], n[4], n[5], n[6]);
+    const M4 T = translate_identity(n[7], n[8], n[9]);
+    const M4 gT = translate_identity(n[10], n[11], n[12]);
+    return mat_mul(mat_mul(mat_mul(mat_mul(gT, parent), T), R), S);
+}
+
 // glm 0.9.9 compute_inverse<4,4> (used by OBBtreesCollision.cpp:15).
 CD_HD M4 mat_inverse(const M4& M) {
 #define MM(c, r) M.m[(c) * 4 + (r)]

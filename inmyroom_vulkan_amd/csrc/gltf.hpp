@@ -24,6 +24,7 @@
 #pragma once
 
 #include <cctype>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "geom.hpp"
 #include "json.hpp"
 
 namespace cdk {
@@ -42,6 +44,8 @@ struct Primitive {
     int32_t gltf_mode = -1;      // as written, -1 when absent
     uint32_t mode = TRIANGLES;   // draw mode after the line-loop fallback
     bool excluded = false;       // skinned or morphed: no collision triangles
+    bool skin_attr = false;      // JOINTS_0 or WEIGHTS_0
+    bool targets = false;        // morph targets of any attribute
     uint32_t nvert = 0;
     std::vector<float> pos;      // nvert x 3, (x, -y, -z)
     std::vector<float> nrm;      // nvert x 3, (x, -y, -z), or empty
@@ -54,10 +58,25 @@ struct Mesh {
     std::vector<Primitive> prims;  // in the order the tree concatenates them
     uint64_t ntri = 0, nvert = 0;  // over the primitives that are not excluded
     uint32_t with_normals = 0, without_normals = 0;  // included primitives with / without NORMAL
+    bool skinned = false;  // some primitive has JOINTS_0 or WEIGHTS_0 (PrimitivesOfMeshes.cpp:1189-1198)
+    bool morphed = false;  // some primitive has morph targets (:1200-1210)
+};
+
+// A node as its NodeData component holds it after import
+// (GameImporter::CreatePositionInitMap, src/GameImporter.cpp:503-552, then
+// NodeDataCompEntity::CreateComponentEntityByMap, NodeDataCompEntity.cpp:19-61).
+struct Node {
+    float scale[3] = {1.f, 1.f, 1.f};
+    float rotation[4] = {0.f, 0.f, 0.f, 1.f};  // unit quaternion (x, y, z, w)
+    float translation[3] = {0.f, 0.f, 0.f};
+    uint32_t parent = 0xFFFFFFFFu;
+    int32_t mesh = -1;
+    bool collides = false;  // gets a ModelCollision component (GameImporter.cpp:320-326)
 };
 
 struct Document {
     std::vector<Mesh> meshes;
+    std::vector<Node> nodes;
 };
 
 // Triangles Triangle::CreateIndicesTriplets makes from n indices in a mode
@@ -255,8 +274,12 @@ inline bool primitive(Ctx& c, const json::Value& pv, Primitive& pr, const std::s
     if (!attrs || attrs->kind != json::Value::Object) { c.err = where + ": no attributes"; return false; }
     const bool skinned = attrs->find("JOINTS_0") != nullptr;
     bool morphed = false;
-    if (const json::Value* t = pv.find("targets"))
+    pr.skin_attr = skinned || attrs->find("WEIGHTS_0") != nullptr;
+    if (const json::Value* t = pv.find("targets")) {
         morphed = t->kind == json::Value::Array && !t->arr.empty() && t->arr[0].find("POSITION") != nullptr;
+        pr.targets = t->kind == json::Value::Array && !t->arr.empty() && t->arr[0].kind == json::Value::Object &&
+                     !t->arr[0].obj.empty();
+    }
     if (skinned || morphed) {
         pr.excluded = true;
         return true;
@@ -299,6 +322,147 @@ inline bool primitive(Ctx& c, const json::Value& pv, Primitive& pr, const std::s
         for (uint32_t v = 0; v < pr.nvert; ++v) pr.idx[v] = v;
     }
     pr.ntri = triangles_of(pr.mode, pr.idx.size());
+    return true;
+}
+
+// glm::normalize of a quaternion: 1 / sqrt((w w + x x) + (y y + z z)) times
+// each component; (1, 0, 0, 0) when the length is not positive.
+inline void quat_normalize(float q[4]) {  // q = (x, y, z, w)
+    const float d = (q[3] * q[3] + q[0] * q[0]) + (q[1] * q[1] + q[2] * q[2]);
+    const float len = std::sqrt(d);
+    if (!(len > 0.f)) {
+        q[0] = q[1] = q[2] = 0.f;
+        q[3] = 1.f;
+        return;
+    }
+    const float inv = 1.f / len;
+    const float w = q[3] * inv, x = q[0] * inv, y = q[1] * inv, z = q[2] * inv;
+    q[0] = x;
+    q[1] = y;
+    q[2] = z;
+    q[3] = w;
+}
+
+// glm::decompose (gtx/matrix_decompose, after Thomas, "Decomposing a matrix
+// into simple transformations", Graphics Gems II) of an affine float matrix,
+// keeping the parts the node uses: scale, rotation, translation.  Returns
+// false, leaving the outputs alone, where glm gives up (w or the 3x3
+// determinant within FLT_EPSILON of zero).
+inline bool decompose(M4 L, float scale[3], float q[4], float t[3]) {
+    const float eps = 1.1920928955078125e-7f;
+    if (std::fabs(L.m[15]) < eps) return false;
+    for (int i = 0; i < 16; ++i) L.m[i] = L.m[i] / L.m[15];
+    const f3 c0 = mk(L.m[0], L.m[1], L.m[2]), c1 = mk(L.m[4], L.m[5], L.m[6]), c2 = mk(L.m[8], L.m[9], L.m[10]);
+    if (std::fabs(dot(c0, cross(c1, c2))) < eps) return false;
+    if (std::fabs(L.m[3]) >= eps || std::fabs(L.m[7]) >= eps || std::fabs(L.m[11]) >= eps) {
+        L.m[3] = L.m[7] = L.m[11] = 0.f;  // perspective part: solved for and dropped
+        L.m[15] = 1.f;
+    }
+    t[0] = L.m[12];
+    t[1] = L.m[13];
+    t[2] = L.m[14];
+    f3 row[3] = {c0, c1, c2};
+    auto unit = [](f3 v) {
+        const float len = vlength(v);
+        return mk(v.x / len, v.y / len, v.z / len);
+    };
+    auto combine = [](f3 a, f3 b, float bs) { return mk(a.x + b.x * bs, a.y + b.y * bs, a.z + b.z * bs); };
+    scale[0] = vlength(row[0]);
+    row[0] = unit(row[0]);
+    float skew_z = dot(row[0], row[1]);
+    row[1] = combine(row[1], row[0], -skew_z);
+    scale[1] = vlength(row[1]);
+    row[1] = unit(row[1]);
+    float skew_y = dot(row[0], row[2]);
+    row[2] = combine(row[2], row[0], -skew_y);
+    float skew_x = dot(row[1], row[2]);
+    row[2] = combine(row[2], row[1], -skew_x);
+    scale[2] = vlength(row[2]);
+    row[2] = unit(row[2]);
+    if (dot(row[0], cross(row[1], row[2])) < 0.f) {
+        for (int i = 0; i < 3; ++i) {
+            scale[i] = scale[i] * -1.f;
+            row[i] = mk(row[i].x * -1.f, row[i].y * -1.f, row[i].z * -1.f);
+        }
+    }
+    auto at = [&](int i, int j) { const f3& r = row[i]; return j == 0 ? r.x : (j == 1 ? r.y : r.z); };
+    const float trace = (row[0].x + row[1].y) + row[2].z;
+    if (trace > 0.f) {
+        float root = std::sqrt(trace + 1.f);
+        q[3] = 0.5f * root;
+        root = 0.5f / root;
+        q[0] = root * (row[1].z - row[2].y);
+        q[1] = root * (row[2].x - row[0].z);
+        q[2] = root * (row[0].y - row[1].x);
+    } else {
+        const int next[3] = {1, 2, 0};
+        int i = 0;
+        if (row[1].y > row[0].x) i = 1;
+        if (row[2].z > at(i, i)) i = 2;
+        const int j = next[i], k = next[j];
+        float root = std::sqrt(((at(i, i) - at(j, j)) - at(k, k)) + 1.f);
+        q[i] = 0.5f * root;
+        root = 0.5f / root;
+        q[j] = root * (at(i, j) + at(j, i));
+        q[k] = root * (at(i, k) + at(k, i));
+        q[3] = root * (at(j, k) - at(k, j));
+    }
+    return true;
+}
+
+inline bool number_array(const json::Value& o, const char* key, size_t n, double* out, std::string& err,
+                         const std::string& where) {
+    const json::Value* v = o.find(key);
+    if (!v) return false;
+    if (v->kind != json::Value::Array || v->arr.size() != n) {
+        err = where + ": \"" + key + "\" is not an array of " + std::to_string(n) + " numbers";
+        return true;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (v->arr[i].kind != json::Value::Number) {
+            err = where + ": \"" + key + "\" holds a non-number";
+            return true;
+        }
+        out[i] = v->arr[i].num;
+    }
+    return true;
+}
+
+// The node's NodeData after import: a matrix is turned into (x, -y, -z)
+// space by diag(1, -1, -1, 1) on both sides and decomposed; otherwise TRS with
+// the rotation (w, x, -y, -z) and the translation (x, -y, -z); the rotation
+// is then normalized (NodeDataCompEntity.cpp:33-41).
+inline bool node(const json::Value& nv, Node& nd, std::string& err, const std::string& where) {
+    double m[16], v[4];
+    err.clear();
+    if (number_array(nv, "matrix", 16, m, err, where)) {
+        if (!err.empty()) return false;
+        M4 A, F = identity4();
+        for (int i = 0; i < 16; ++i) A.m[i] = float(m[i]);
+        F.m[5] = -1.f;
+        F.m[10] = -1.f;
+        const M4 TRS = mat_mul(mat_mul(F, A), F);
+        decompose(TRS, nd.scale, nd.rotation, nd.translation);
+    } else {
+        if (number_array(nv, "scale", 3, v, err, where)) {
+            if (!err.empty()) return false;
+            for (int i = 0; i < 3; ++i) nd.scale[i] = float(v[i]);
+        }
+        if (number_array(nv, "rotation", 4, v, err, where)) {
+            if (!err.empty()) return false;
+            nd.rotation[0] = float(v[0]);
+            nd.rotation[1] = float(-v[1]);
+            nd.rotation[2] = float(-v[2]);
+            nd.rotation[3] = float(v[3]);
+        }
+        if (number_array(nv, "translation", 3, v, err, where)) {
+            if (!err.empty()) return false;
+            nd.translation[0] = float(v[0]);
+            nd.translation[1] = float(-v[1]);
+            nd.translation[2] = float(-v[2]);
+        }
+    }
+    quat_normalize(nd.rotation);
     return true;
 }
 
@@ -390,10 +554,10 @@ inline bool load(const std::string& path, Document& doc, std::string& err) {
     c.root = &root;
     if (!load_buffers(c, dir, glb ? &bin : nullptr)) { err = c.err; return false; }
     doc.meshes.clear();
+    doc.nodes.clear();
     const json::Value* meshes = root.find("meshes");
-    if (!meshes) return true;
-    if (meshes->kind != json::Value::Array) { err = "\"meshes\" is not an array"; return false; }
-    for (size_t m = 0; m < meshes->arr.size(); ++m) {
+    if (meshes && meshes->kind != json::Value::Array) { err = "\"meshes\" is not an array"; return false; }
+    for (size_t m = 0; meshes && m < meshes->arr.size(); ++m) {
         const json::Value& mv = meshes->arr[m];
         Mesh mesh;
         if (const json::Value* nm = mv.find("name"))
@@ -411,6 +575,8 @@ inline bool load(const std::string& path, Document& doc, std::string& err) {
         }
         for (uint32_t p : tree_order(modes)) {
             Primitive& pr = in[p];
+            mesh.skinned = mesh.skinned || pr.skin_attr;
+            mesh.morphed = mesh.morphed || pr.targets;
             if (!pr.excluded) {
                 mesh.ntri += pr.ntri;
                 mesh.nvert += pr.nvert;
@@ -419,6 +585,39 @@ inline bool load(const std::string& path, Document& doc, std::string& err) {
             mesh.prims.push_back(std::move(pr));
         }
         doc.meshes.push_back(std::move(mesh));
+    }
+    if (const json::Value* nodes = root.find("nodes")) {
+        if (nodes->kind != json::Value::Array) { err = "\"nodes\" is not an array"; return false; }
+        doc.nodes.resize(nodes->arr.size());
+        for (size_t i = 0; i < nodes->arr.size(); ++i) {
+            const json::Value& nv = nodes->arr[i];
+            const std::string where = "node " + std::to_string(i);
+            Node& nd = doc.nodes[i];
+            if (!node(nv, nd, err, where)) return false;
+            int64_t mi, sk;
+            if (!get_int(nv, "mesh", mi, -1, false, err, where) || !get_int(nv, "skin", sk, -1, false, err, where))
+                return false;
+            if (mi >= int64_t(doc.meshes.size()) || mi < -1) { err = where + ": no such mesh"; return false; }
+            nd.mesh = int32_t(mi);
+            nd.collides = mi >= 0 && !doc.meshes[size_t(mi)].skinned && !doc.meshes[size_t(mi)].morphed;
+        }
+        for (size_t i = 0; i < nodes->arr.size(); ++i) {
+            const json::Value* ch = nodes->arr[i].find("children");
+            if (!ch) continue;
+            if (ch->kind != json::Value::Array) { err = "node " + std::to_string(i) + ": bad children"; return false; }
+            for (const json::Value& cv : ch->arr) {
+                if (!cv.is_int() || cv.num < 0 || cv.num >= double(doc.nodes.size())) {
+                    err = "node " + std::to_string(i) + ": child out of range";
+                    return false;
+                }
+                Node& kid = doc.nodes[size_t(cv.num)];
+                if (kid.parent != 0xFFFFFFFFu || size_t(cv.num) == i) {
+                    err = "node " + std::to_string(size_t(cv.num)) + ": more than one parent";
+                    return false;
+                }
+                kid.parent = uint32_t(i);
+            }
+        }
     }
     return true;
 }

@@ -185,6 +185,7 @@ struct PrimSpan {
     uint32_t tri_off, ntri, idx_off, mode, vbase, vid_base, pad[2];
 };
 static_assert(sizeof(PrimSpan) == 32, "primitive span layout");
+static_assert(sizeof(cd_node) == 64, "node layout");
 
 // Device-side counters of one run (zeroed by a fill kernel, never by a
 // memset, so a run can be captured into a HIP graph).

@@ -197,6 +197,9 @@ class Parser {
         const std::string tok(s_, p);  // strtod needs a terminated copy
         v.kind = Value::Number;
         v.num = std::strtod(tok.c_str(), nullptr);
+        // A number written as an integer is an integer, as the JSON libraries
+        // glTF loaders use keep it: "-0" is the integer 0, read as +0.0.
+        if (tok.find_first_of(".eE") == std::string::npos && v.num == 0.0) v.num = 0.0;
         s_ = p;
         return true;
     }

@@ -9,6 +9,9 @@ namespace cdk {
 BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh,
                         bool tri9_on_device = false);
 
+// nodes.hip
+hipError_t launch_node_globals(const cd_node* nodes, const uint32_t* depth, uint32_t n, float* out, hipStream_t st);
+
 // ingest.hip
 hipError_t launch_assemble(const PrimSpan* prims, uint32_t nprim, const uint32_t* idx, const float* pos, uint32_t total,
                            float* tri9, uint32_t* vid3, hipStream_t st);

@@ -63,6 +63,8 @@ class Oracle:
         lib.orc_surface.restype = C.c_float
         lib.orc_xform_box.argtypes = [_F, _F, _F]
         lib.orc_mat_inverse.argtypes = [_F, _F]
+        lib.orc_node_globals.argtypes = [_F, _U32, C.c_int64, _F]
+        lib.orc_node_globals.restype = C.c_int
         lib.orc_mat_mul.argtypes = [_F, _F, _F]
         lib.orc_sap_axes.argtypes = [_F]
         lib.orc_tritri.argtypes = [_F, _F, C.c_int64, _I32, C.c_void_p]
@@ -148,6 +150,17 @@ class Oracle:
         out = np.zeros(16, np.float32)
         self.lib.orc_mat_mul(np.ascontiguousarray(a16, np.float32).reshape(16),
                              np.ascontiguousarray(b16, np.float32).reshape(16), out)
+        return out
+
+    def node_globals(self, trs13, parent):
+        """GetGlobalMatrix down a hierarchy: trs13 (n, 13) = scale, rotation
+        (x, y, z, w), translation, global translation; parent (n,) with ~0 for
+        roots.  Returns (n, 16) column-major float32."""
+        t = np.ascontiguousarray(trs13, np.float32).reshape(-1, 13)
+        p = np.ascontiguousarray(parent, np.uint32).reshape(-1)
+        out = np.zeros((len(t), 16), np.float32)
+        if self.lib.orc_node_globals(t, p, len(t), out) != 0:
+            raise ValueError("cyclic node hierarchy")
         return out
 
     def sap_axes(self):

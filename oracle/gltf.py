@@ -184,3 +184,151 @@ def mesh_triangles(prims: list[dict]):
         return z, None, np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.uint32)
     normals = np.concatenate(N) if len(N) == len(P) else None
     return np.concatenate(P), normals, np.concatenate(I), np.concatenate(Io)
+
+
+# ---------------------------------------------------------------------------
+# Nodes: GameImporter::CreatePositionInitMap (src/GameImporter.cpp:503-552)
+# then NodeDataCompEntity::CreateComponentEntityByMap
+# (ECS/GeneralCompEntities/NodeDataCompEntity.cpp:19-61), in float32 scalar
+# operations.  glm (an empty submodule of the reference) supplies
+# glm::decompose, glm::normalize and the quaternion dot; they are restated
+# from glm 0.9.9's published source order, so the matrix-node path and the
+# normalization are pinned by round trips and invariants, not by glm itself.
+# ---------------------------------------------------------------------------
+_f = np.float32
+
+
+def _mat_mul(A, B):
+    R = [_f(0)] * 16
+    for c in range(4):
+        for r in range(4):
+            t0 = _f(A[0 * 4 + r] * B[c * 4 + 0])
+            t1 = _f(A[1 * 4 + r] * B[c * 4 + 1])
+            t2 = _f(A[2 * 4 + r] * B[c * 4 + 2])
+            t3 = _f(A[3 * 4 + r] * B[c * 4 + 3])
+            R[c * 4 + r] = _f(_f(_f(t0 + t1) + t2) + t3)
+    return R
+
+
+def _dot3(a, b):
+    return _f(_f(_f(a[0] * b[0]) + _f(a[1] * b[1])) + _f(a[2] * b[2]))
+
+
+def _cross3(a, b):
+    return [_f(_f(a[1] * b[2]) - _f(b[1] * a[2])), _f(_f(a[2] * b[0]) - _f(b[2] * a[0])),
+            _f(_f(a[0] * b[1]) - _f(b[0] * a[1]))]
+
+
+def quat_normalize(q):
+    """glm::normalize of (x, y, z, w): dot = (w w + x x) + (y y + z z)."""
+    x, y, z, w = (_f(v) for v in q)
+    d = _f(_f(_f(w * w) + _f(x * x)) + _f(_f(y * y) + _f(z * z)))
+    ln = _f(np.sqrt(d))
+    if not ln > 0:
+        return [_f(0), _f(0), _f(0), _f(1)]
+    inv = _f(_f(1) / ln)
+    return [_f(x * inv), _f(y * inv), _f(z * inv), _f(w * inv)]
+
+
+def decompose(M):
+    """glm::decompose of a float32 4x4 (column-major list): (scale, (x, y, z, w), translation) or None."""
+    eps = _f(1.1920928955078125e-7)
+    L = [_f(v) for v in M]
+    if abs(L[15]) < eps:
+        return None
+    w33 = L[15]
+    L = [_f(v / w33) for v in L[:15]] + [_f(w33 / w33)]
+    rows = [L[0:3], L[4:7], L[8:11]]
+    if abs(_dot3(rows[0], _cross3(rows[1], rows[2]))) < eps:
+        return None
+    t = [L[12], L[13], L[14]]
+    length = lambda v: _f(np.sqrt(_dot3(v, v)))
+    unit = lambda v: [_f(c / length(v)) for c in v]
+    comb = lambda a, b, s: [_f(a[i] + _f(b[i] * s)) for i in range(3)]
+    scale = [length(rows[0])]
+    rows[0] = unit(rows[0])
+    sz = _dot3(rows[0], rows[1])
+    rows[1] = comb(rows[1], rows[0], _f(-sz))
+    scale.append(length(rows[1]))
+    rows[1] = unit(rows[1])
+    sy = _dot3(rows[0], rows[2])
+    rows[2] = comb(rows[2], rows[0], _f(-sy))
+    sx = _dot3(rows[1], rows[2])
+    rows[2] = comb(rows[2], rows[1], _f(-sx))
+    scale.append(length(rows[2]))
+    rows[2] = unit(rows[2])
+    if _dot3(rows[0], _cross3(rows[1], rows[2])) < 0:
+        scale = [_f(s * _f(-1)) for s in scale]
+        rows = [[_f(c * _f(-1)) for c in r] for r in rows]
+    q = [_f(0)] * 4
+    trace = _f(_f(rows[0][0] + rows[1][1]) + rows[2][2])
+    if trace > 0:
+        root = _f(np.sqrt(_f(trace + _f(1))))
+        q[3] = _f(_f(0.5) * root)
+        root = _f(_f(0.5) / root)
+        q[0] = _f(root * _f(rows[1][2] - rows[2][1]))
+        q[1] = _f(root * _f(rows[2][0] - rows[0][2]))
+        q[2] = _f(root * _f(rows[0][1] - rows[1][0]))
+    else:
+        nxt = [1, 2, 0]
+        i = 0
+        if rows[1][1] > rows[0][0]:
+            i = 1
+        if rows[2][2] > rows[i][i]:
+            i = 2
+        j = nxt[i]
+        k = nxt[j]
+        root = _f(np.sqrt(_f(_f(_f(rows[i][i] - rows[j][j]) - rows[k][k]) + _f(1))))
+        q[i] = _f(_f(0.5) * root)
+        root = _f(_f(0.5) / root)
+        q[j] = _f(root * _f(rows[i][j] + rows[j][i]))
+        q[k] = _f(root * _f(rows[i][k] + rows[k][i]))
+        q[3] = _f(root * _f(rows[j][k] - rows[k][j]))
+    return scale, q, t
+
+
+def nodes(path: str):
+    """(trs13 (n, 13) float32, parent (n,) uint32 with ~0 for roots, mesh (n,), collides (n,))."""
+    doc, _ = read_document(path)
+    nodes_ = doc.get("nodes", [])
+    n = len(nodes_)
+    trs = np.zeros((n, 13), np.float32)
+    parent = np.full(n, 0xFFFFFFFF, np.uint32)
+    mesh = np.full(n, -1, np.int32)
+    coll = np.zeros(n, bool)
+    meshes = doc.get("meshes", [])
+
+    def mesh_flags(m):
+        prims = meshes[m]["primitives"]
+        skinned = any("JOINTS_0" in p["attributes"] or "WEIGHTS_0" in p["attributes"] for p in prims)
+        morphed = any(len(p.get("targets", [])) > 0 and len(p["targets"][0]) > 0 for p in prims)
+        return skinned, morphed
+
+    F = [_f(1), _f(0), _f(0), _f(0), _f(0), _f(-1), _f(0), _f(0), _f(0), _f(0), _f(-1), _f(0),
+         _f(0), _f(0), _f(0), _f(1)]
+    for i, nd in enumerate(nodes_):
+        scale, rot, tr = [_f(1)] * 3, [_f(0), _f(0), _f(0), _f(1)], [_f(0)] * 3
+        if len(nd.get("matrix", [])) == 16:
+            A = [_f(v) for v in nd["matrix"]]
+            out = decompose(_mat_mul(_mat_mul(F, A), F))
+            if out is not None:
+                scale, rot, tr = out
+        else:
+            if "scale" in nd:
+                scale = [_f(v) for v in nd["scale"]]
+            # tinygltf holds these as doubles: the negations act on doubles (an integer 0 gives -0.0)
+            if "rotation" in nd:
+                x, y, z, w = (float(v) for v in nd["rotation"])
+                rot = [_f(x), _f(-y), _f(-z), _f(w)]
+            if "translation" in nd:
+                x, y, z = (float(v) for v in nd["translation"])
+                tr = [_f(x), _f(-y), _f(-z)]
+        rot = quat_normalize(rot)
+        trs[i, 0:3], trs[i, 3:7], trs[i, 7:10] = scale, rot, tr
+        if "mesh" in nd:
+            mesh[i] = nd["mesh"]
+            skinned, morphed = mesh_flags(nd["mesh"])
+            coll[i] = not skinned and not morphed
+        for ch in nd.get("children", []):
+            parent[ch] = i
+    return trs, parent, mesh, coll

@@ -5,6 +5,7 @@
 #include "oracle.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -1715,7 +1716,81 @@ struct OrcResult {
     Counters total;
 };
 
+// ---------------------------------------------------------------------------
+// Node transforms (NodeDataCompEntity::GetGlobalMatrix, inMyRoom_vulkan/src/
+// ECS/GeneralCompEntities/NodeDataCompEntity.cpp:63-72), glm's builders on the
+// identity written out as glm writes them: translate sets column 3 to
+// ((I0 x + I1 y) + I2 z) + I3, scale sets column i to Ii v[i], toMat4 is
+// mat3_cast's nine terms (glm gtc/quaternion).
+// ---------------------------------------------------------------------------
+static M4 ident4() {
+    M4 I;
+    for (int i = 0; i < 16; ++i) I.m[i] = (i % 5 == 0) ? 1.f : 0.f;
+    return I;
+}
+static M4 glm_translate_identity(float x, float y, float z) {
+    const M4 I = ident4();
+    M4 R = I;
+    for (int r = 0; r < 4; ++r) R.m[12 + r] = ((I.m[0 + r] * x + I.m[4 + r] * y) + I.m[8 + r] * z) + I.m[12 + r];
+    return R;
+}
+static M4 glm_scale_identity(float x, float y, float z) {
+    const M4 I = ident4();
+    M4 R = I;
+    const float v[3] = {x, y, z};
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 4; ++r) R.m[c * 4 + r] = I.m[c * 4 + r] * v[c];
+    return R;
+}
+static M4 glm_to_mat4(float x, float y, float z, float w) {
+    const float qxx = x * x, qyy = y * y, qzz = z * z, qxz = x * z, qxy = x * y, qyz = y * z;
+    const float qwx = w * x, qwy = w * y, qwz = w * z;
+    M4 R = ident4();
+    R.m[0] = 1.f - 2.f * (qyy + qzz);
+    R.m[1] = 2.f * (qxy + qwz);
+    R.m[2] = 2.f * (qxz - qwy);
+    R.m[4] = 2.f * (qxy - qwz);
+    R.m[5] = 1.f - 2.f * (qxx + qzz);
+    R.m[6] = 2.f * (qyz + qwx);
+    R.m[8] = 2.f * (qxz + qwy);
+    R.m[9] = 2.f * (qyz - qwx);
+    R.m[10] = 1.f - 2.f * (qxx + qyy);
+    return R;
+}
+// trs13: scale[3], rotation (x, y, z, w), translation[3], global translation[3]
+static M4 get_global_matrix(const float* n, const M4& parent) {
+    const M4 S = glm_scale_identity(n[0], n[1], n[2]);
+    const M4 R = glm_to_mat4(n[3], n[4], n[5], n[6]);
+    const M4 T = glm_translate_identity(n[7], n[8], n[9]);
+    const M4 gT = glm_translate_identity(n[10], n[11], n[12]);
+    return mat_mul(mat_mul(mat_mul(mat_mul(gT, parent), T), R), S);
+}
+
 extern "C" {
+
+// Global matrices of n nodes (parents computed first, recursively);
+// parent ~0u = root.  Returns 0, or -1 on a cycle.
+int orc_node_globals(const float* trs13, const uint32_t* parent, int64_t n, float* out16) {
+    std::vector<int> state(static_cast<size_t>(n), 0);  // 0 todo, 1 on the stack, 2 done
+    std::vector<M4> G(static_cast<size_t>(n));
+    std::function<bool(int64_t)> visit = [&](int64_t i) -> bool {
+        if (state[size_t(i)] == 2) return true;
+        if (state[size_t(i)] == 1) return false;
+        state[size_t(i)] = 1;
+        M4 P = ident4();
+        if (parent[i] != 0xFFFFFFFFu) {
+            if (!visit(int64_t(parent[i]))) return false;
+            P = G[parent[i]];
+        }
+        G[size_t(i)] = get_global_matrix(trs13 + 13 * i, P);
+        state[size_t(i)] = 2;
+        return true;
+    };
+    for (int64_t i = 0; i < n; ++i)
+        if (!visit(i)) return -1;
+    for (int64_t i = 0; i < n; ++i) std::memcpy(out16 + 16 * i, G[size_t(i)].m, 16 * sizeof(float));
+    return 0;
+}
 
 void orc_eig3(const double* A9, double* V9, double* d3) {
     double A[3][3], V[3][3];
