@@ -784,10 +784,24 @@ int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32
     HCK(ctx, hipMemcpyAsync(d_pos.p, pos.data(), pos.size() * sizeof(float), hipMemcpyHostToDevice, st));
     HCK(ctx, hipMemcpyAsync(d_idx.p, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     HCK(ctx, hipMemcpyAsync(d_spans.p, spans.data(), spans.size() * sizeof(PrimSpan), hipMemcpyHostToDevice, st));
+    HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_assemble(d_spans.as<PrimSpan>(), uint32_t(spans.size()), d_idx.as<uint32_t>(), d_pos.as<float>(),
                              uint32_t(T), d_soup.as<float>(), d_vid3.as<uint32_t>(), st));
+    HCK(ctx, hipEventRecord(ctx->ev[1], st));
     BuildResult r = build_soups(ctx, d_soup.as<float>(), counts.data(), uint32_t(reg.size()), true);
     if (!r.err.empty()) return fail(ctx, CD_E_HIP, "tree build: " + r.err);
+    HCK(ctx, hipEventRecord(ctx->ev[2], st));
+    HCK(ctx, hipEventSynchronize(ctx->ev[2]));
+    {  // assembly in broad_ms, the tree build in traverse_ms (device time between events)
+        float t_asm = 0.f, t_build = 0.f;
+        HCK(ctx, hipEventElapsedTime(&t_asm, ctx->ev[0], ctx->ev[1]));
+        HCK(ctx, hipEventElapsedTime(&t_build, ctx->ev[1], ctx->ev[2]));
+        ctx->timings = cd_timings{};
+        ctx->timings.broad_ms = t_asm;
+        ctx->timings.traverse_ms = t_build;
+        ctx->timings.total_ms = t_asm + t_build;
+        ctx->timings.launches = 1;
+    }
     const uint32_t first = uint32_t(ctx->meshes.size());
     ctx->meshes.insert(ctx->meshes.end(), r.meshes.begin(), r.meshes.end());
     for (uint32_t m : reg) {  // plain soups (vertex ids 3 t + k) until their attributes are in
