@@ -3,10 +3,10 @@
 //  inMyRoom_vulkan/src/CollisionDetection/ShootUncollideRays.cpp:14-179;
 //  Ray::IntersectOBBtree, Ray.cpp:13-236; CollisionDetection.cpp:70-99, 143-150).
 //
-//  k_shoot    one lane per uncollide ray.  Pass 1 shoots the ray at the other
-//             entity's tree (closest hit); it succeeds when that hit is a
-//             back face reached before the ray re-enters its own tree (the
-//             Hermann pass, :129-158).  On success the ray is reflected (origin
+//  k_shoot    two lanes per uncollide ray.  Pass 1 shoots the ray at the
+//             other entity's tree (closest hit, even lane); it succeeds when
+//             that hit is a back face reached before the ray re-enters its own
+//             tree (odd lane; the Hermann pass, :129-158).  On success the ray is reflected (origin
 //             moved by the response, direction minus the hit's normal,
 //             :104-110) and pass 2 shoots it the other way.  Each pass's force
 //             response and push-out vector are kept per ray.
@@ -231,17 +231,37 @@ struct MeshView {
     const uint32_t* perm;
 };
 
-// HermannPass (ShootUncollideRays.cpp:129-158).
+// HermannPass (ShootUncollideRays.cpp:129-158), on two lanes: the even lane
+// shoots the ray at the other entity's tree (p2), the odd lane shoots the
+// ray moved 4 epsilon along itself at the entity's own tree (p3), which does
+// not depend on p2; both lanes then hold both hits and decide the same way.
+// The reference skips p3 when p2 fails; computing it anyway changes nothing.
+__device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
+    Hit r;
+    r.hit = __shfl(int(h.hit), src, 64) != 0;
+    r.back = __shfl(int(h.back), src, 64) != 0;
+    r.dist = __shfl(h.dist, src, 64);
+    r.bx = __shfl(h.bx, src, 64);
+    r.by = __shfl(h.by, src, 64);
+    r.tri = uint32_t(__shfl(int(h.tri), src, 64));
+    return r;
+}
+
 __device__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
                         const M3& Bn, f3 o, f3 d, RayStack& S) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
-    const MeshRec mb = V.meshes[meshB];
-    const Hit p2 = ray_tree(mb, V.nodes, V.tris, Bmat, o, d, S);
-    if (!(p2.hit && p2.back)) return r;
+    const bool own = (threadIdx.x & 1u) != 0;
     const f3 o2 = move_eps(o, d, 4.f);
+    M4 C;  // one call with lane-selected arguments, so the two traversals run as one instruction stream
+#pragma unroll
+    for (int e = 0; e < 16; ++e) C.m[e] = own ? Amat.m[e] : Bmat.m[e];
+    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S);
+    const int even = int(threadIdx.x & ~1u);
+    const Hit p2 = shfl_hit(h, even), p3 = shfl_hit(h, even + 1);
+    if (!(p2.hit && p2.back)) return r;
     const float eps = vlength(sub(o2, o));
-    const Hit p3 = ray_tree(V.meshes[meshA], V.nodes, V.tris, Amat, o2, d, S);
     if (p2.dist <= p3.dist + eps) {
+        const MeshRec mb = V.meshes[meshB];
         r.ok = true;
         r.response = scale(d, p2.dist);
         const uint2 attr = V.attrs[meshB];
@@ -287,8 +307,8 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
     __shared__ uint32_t s_link[RAY_STACK * 64], s_cnt[RAY_STACK * 64];
     __shared__ float s_t0[RAY_STACK * 64];
     RayStack S = {s_link, s_cnt, s_t0, int(threadIdx.x)};
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrays) return;
+    const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
+    if (r >= nrays) return;  // both lanes of a ray leave together
     const int side = r >= base1 ? 1 : 0;
     const uint32_t local = r - (side ? base1 : 0u);
     const uint32_t* begin = pbegin + size_t(nq) * side;
@@ -328,7 +348,7 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
         d = neg(p.normal);
     }
     s.pair = lo;
-    out[r] = s;
+    if ((threadIdx.x & 1u) == 0) out[r] = s;
 }
 
 __device__ __forceinline__ float smoother_step(float a, float b, float x) {  // :171-175
@@ -429,8 +449,9 @@ hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, cons
                         hipStream_t st) {
     if (!nrays) return hipSuccess;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
-    hipLaunchKernelGGL(k_shoot, dim3((nrays + 63) / 64), dim3(64), 0, st, rays, nrays, base1, pbegin, nq, qs, V,
-                       shots);
+    const uint64_t lanes = 2ull * nrays;
+    hipLaunchKernelGGL(k_shoot, dim3(uint32_t((lanes + 63) / 64)), dim3(64), 0, st, rays, nrays, base1, pbegin, nq,
+                       qs, V, shots);
     return hipGetLastError();
 }
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
