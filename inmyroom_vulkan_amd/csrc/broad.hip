@@ -7,10 +7,13 @@
 //                 one workgroup; ranks within a wavefront from ballots on
 //                 the digit bits, across wavefronts from an LDS table
 //                 (up to RS_SMALL keys)
+//  k_rank_sort    the small sorts (up to RANK_MAX keys): every key ranked
+//                 against all others in LDS, same stable order
 //  k_rs_*         the same sort over many workgroups for longer arrays
 //                 (contacts): 8-bit digits, per-tile digit counts, one
 //                 scan of the count table, stable scatter
-//  k_sap_sweep    one lane per entity in U order: every later entity whose
+//  k_sap_sweep    one lane per entity in U order, later entities streamed
+//                 through LDS a tile at a time: every later entity whose
 //                 U.min <= this U.max, kept when V and W overlap and either
 //                 entity wants callbacks (SweepAndPrune.cpp:50-85); the pair
 //                 is oriented (earlier, later) as the reference orients it
@@ -111,6 +114,34 @@ __global__ __launch_bounds__(1024) void k_radix_sort(uint32_t* keys, uint32_t* v
     }
 }
 
+// Small sorts (the broad phase's keys and pairs of a scene): one workgroup
+// ranks every key against all others in LDS, rank = (keys below) + (equal
+// keys before it), the stable order an LSD radix sort over the same bits
+// gives, with no passes and no scans.
+constexpr uint32_t RANK_MAX = 2048;
+
+__global__ __launch_bounds__(1024) void k_rank_sort(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t mask) {
+    __shared__ uint32_t s_k[RANK_MAX], s_key[RANK_MAX], s_v[RANK_MAX];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t k = keys[i];
+        s_key[i] = k;
+        s_k[i] = k & mask;
+        s_v[i] = vals[i];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t k = s_k[i];
+        uint32_t r = 0;
+#pragma unroll 8
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t kj = s_k[j];
+            r += (kj < k || (kj == k && j < i)) ? 1u : 0u;
+        }
+        keys[r] = s_key[i];
+        vals[r] = s_v[i];
+    }
+}
+
 // Multi-workgroup pass of the LSD sort: tiles of RS_TILE keys; the count
 // table is digit-major ([digit][tile]) so that one exclusive scan gives
 // every tile the start of each of its digit runs in tile order (stable).
@@ -188,23 +219,66 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
     }
 }
 
-__global__ void k_sap_sweep(const uint32_t* __restrict__ order, const float* __restrict__ iv,
-                            const cd_entity* __restrict__ ents, uint32_t n, uint2* __restrict__ pairs, uint32_t cap,
-                            unsigned int* __restrict__ count) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t e = order[p];
-    const float* a = iv + size_t(e) * 6;
-    const bool cbe = ents[e].should_callback != 0;
-    for (uint32_t q = p + 1; q < n; ++q) {
-        const uint32_t f = order[q];
-        const float* b = iv + size_t(f) * 6;
-        if (a[1] < b[0]) break;  // retired: this U.max is below every later U.min (SweepAndPrune.cpp:58-59)
-        if (!(cbe || ents[f].should_callback)) continue;
-        if ((a[3] >= b[2]) && (b[3] >= a[2]) && (a[5] >= b[4]) && (b[5] >= a[4])) {
-            const unsigned int s = atomicAdd(count, 1u);
-            if (s < cap) pairs[s] = make_uint2(e, f);
+constexpr uint32_t SAP_TILE = 256;
+
+__global__ __launch_bounds__(256) void k_sap_sweep(const uint32_t* __restrict__ order, const float* __restrict__ iv,
+                                                   const cd_entity* __restrict__ ents, uint32_t n,
+                                                   uint2* __restrict__ pairs, uint32_t cap,
+                                                   unsigned int* __restrict__ count) {
+    // the later entities stream through LDS one tile of the U order at a
+    // time, until every lane of the workgroup has met a U.min above its U.max
+    __shared__ float s_iv[SAP_TILE * 6];
+    __shared__ uint32_t s_id[SAP_TILE];
+    __shared__ uint32_t s_cb[SAP_TILE];
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t first = blockIdx.x * SAP_TILE;
+    const uint32_t p = first + tid;
+    uint32_t e = 0;
+    float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool cbe = false, done = p >= n;
+    if (!done) {
+        e = order[p];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[k] = iv[size_t(e) * 6 + k];
+        cbe = ents[e].should_callback != 0;
+    }
+    uint32_t q = p + 1;
+    for (uint32_t t0 = first; t0 < n; t0 += SAP_TILE) {
+        const uint32_t j = t0 + tid;
+        if (j < n) {
+            const uint32_t f = order[j];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s_iv[tid * 6 + k] = iv[size_t(f) * 6 + k];
+            s_id[tid] = f;
+            s_cb[tid] = ents[f].should_callback != 0 ? 1u : 0u;
         }
+        __syncthreads();
+        const uint32_t tend = min(t0 + SAP_TILE, n);
+        if (!done) {
+            for (; q < tend; ++q) {
+                const float* b = s_iv + (q - t0) * 6;
+                if (a[1] < b[0]) {  // retired: this U.max is below every later U.min (SweepAndPrune.cpp:58-59)
+                    done = true;
+                    break;
+                }
+                const bool hit = (cbe || s_cb[q - t0] != 0u) && (a[3] >= b[2]) && (b[3] >= a[2]) && (a[5] >= b[4]) &&
+                                 (b[5] >= a[4]);
+                const unsigned long long m = __ballot(hit);  // one atomic per wavefront
+                if (m) {
+                    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+                    unsigned int base = 0;
+                    if (int(lane) == leader) base = atomicAdd(count, unsigned(__popcll(m)));
+                    base = __shfl(base, leader, 64);
+                    if (hit) {
+                        const unsigned int sl = base + unsigned(__popcll(m & below));
+                        if (sl < cap) pairs[sl] = make_uint2(e, s_id[q - t0]);
+                    }
+                }
+            }
+            if (q >= n) done = true;
+        }
+        if (__syncthreads_or(done ? 0 : 1) == 0) break;  // also keeps the tile until every lane is through it
     }
 }
 
@@ -273,6 +347,11 @@ uint32_t key_bits_below(uint64_t bound) {
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                              uint32_t key_bits, uint32_t* table, hipStream_t st) {
     if (n < 2) return hipSuccess;
+    if (n <= RANK_MAX) {
+        const uint32_t mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << ((key_bits + 3) / 4 * 4)) - 1u);
+        hipLaunchKernelGGL(k_rank_sort, dim3(1), dim3(1024), 0, st, keys, vals, n, mask);
+        return hipGetLastError();
+    }
     if (n <= RS_SMALL || !table) {
         uint32_t passes = (key_bits + 3) / 4;
         passes += passes & 1u;  // even, so the result lands back in keys/vals
@@ -298,7 +377,8 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
 }
 hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
                             uint32_t cap, unsigned int* count, hipStream_t st) {
-    hipLaunchKernelGGL(k_sap_sweep, g256(n), dim3(256), 0, st, order, iv, ents, n, pairs, cap, count);
+    hipLaunchKernelGGL(k_sap_sweep, dim3((n + SAP_TILE - 1) / SAP_TILE), dim3(SAP_TILE), 0, st, order, iv, ents, n, pairs,
+                       cap, count);
     return hipGetLastError();
 }
 hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st) {
