@@ -239,6 +239,7 @@ inline bool accessor(Ctx& c, int64_t ai, uint32_t want_comps, const std::vector<
     }
     if (bi < 0 || uint64_t(bi) >= c.buffers.size()) { c.err = where + ": no such buffer"; return false; }
     if (cnt < 0 || aoff < 0 || boff < 0 || blen < 0) { c.err = where + ": negative size or offset"; return false; }
+    if (cnt > (int64_t(1) << 40)) { c.err = where + ": count too large"; return false; }
     const uint64_t need = uint64_t(aoff) + uint64_t(cnt) * esize;
     if (need > uint64_t(blen)) { c.err = where + ": runs past its buffer view"; return false; }
     if (uint64_t(boff) + uint64_t(blen) > c.buffers[size_t(bi)].size()) {

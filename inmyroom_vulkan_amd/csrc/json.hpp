@@ -27,7 +27,10 @@ struct Value {
             if (kv.first == key) return &kv.second;
         return nullptr;
     }
-    bool is_int() const { return kind == Number && num == double(int64_t(num)); }
+    // an integer that int64_t holds exactly (|x| < 2^53 keeps the cast defined)
+    bool is_int() const {
+        return kind == Number && num > -9007199254740992.0 && num < 9007199254740992.0 && num == double(int64_t(num));
+    }
 };
 
 class Parser {

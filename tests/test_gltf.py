@@ -238,6 +238,8 @@ def test_parser_rejects(tmp_path):
                          "POSITION")
     _expect_format_error(edited("vec2", lambda d: d["accessors"][0].update(type="VEC2")), "expected type VEC3")
     _expect_format_error(edited("past", lambda d: d["accessors"][0].update(count=4)), "runs past")
+    _expect_format_error(edited("huge", lambda d: d["accessors"][0].update(count=2 ** 45)), "count too large")
+    _expect_format_error(edited("nonint", lambda d: d["accessors"][0].update(count=1e300)), "not an integer")
     bad = tmp_path / "bad.gltf"
     bad.write_text('{"meshes": [1, 2,]}')
     _expect_format_error(str(bad), "JSON")
