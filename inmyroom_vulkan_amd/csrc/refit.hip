@@ -241,58 +241,71 @@ __global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ item
                                                   const float* __restrict__ tris, unsigned int* __restrict__ slack_bits,
                                                   unsigned int* __restrict__ radius_bits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t it = list[i];
-    const uint2 item = items[it];
-    MeshRec& m = meshes[item.x];
-    const uint2 rg = ranges[it];
-    const float* P = item_tris(m, tris, rg.x);
-    const uint32_t npts = 3 * rg.y;
-    bool multi = false;
-    for (uint32_t k = 1; k < npts && !multi; ++k)
-        multi = !(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2]);
-    Box box;
-    if (!multi) {
-        box = empty_box(P, npts);
-    } else {
-        const double N = double(npts);
-        double sx = 0, sy = 0, sz = 0;
-        for (uint32_t k = 0; k < npts; ++k) {
-            sx += double(P[3 * k]);
-            sy += double(P[3 * k + 1]);
-            sz += double(P[3 * k + 2]);
-        }
-        const double mx = sx / N, my = sy / N, mz = sz / N;
-        double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
-        for (uint32_t k = 0; k < npts; ++k) {
-            const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
-            cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
-            cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
-        }
-        double ax[9];
-        axes_of(cxx / N, cyy / N, czz / N, cxy / N, cxz / N, cyz / N, ax);
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t k = 0; k < npts; ++k) {
-            const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
-            for (int q = 0; q < 3; ++q) {
-                const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
-                lo[q] = fmin(lo[q], pr);
-                hi[q] = fmax(hi[q], pr);
-            }
-        }
-        box = box_from_extents(ax, lo, hi);
-    }
-    store_box(m, nodes, item.y, box);
-    float b12[12];
-    box_to12(box, b12);
-    const SlackFrame f = slack_frame(b12);
+    // every lane stays to the end: the slack bound is reduced over the
+    // wavefront before it goes to the per-mesh maximum (one atomic per
+    // wavefront and mesh instead of one per box)
+    const bool valid = i < n;
+    const uint32_t it = valid ? list[i] : 0u;
+    const uint2 item = valid ? items[it] : make_uint2(0u, 0u);
     double worst = 0.0, rad = 0.0;
-    for (uint32_t k = 0; k < npts; ++k) {
-        const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
-        worst = fmax(worst, excess(f, p));
-        rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+    if (valid) {
+        MeshRec& m = meshes[item.x];
+        const uint2 rg = ranges[it];
+        const float* P = item_tris(m, tris, rg.x);
+        const uint32_t npts = 3 * rg.y;
+        bool multi = false;
+        for (uint32_t k = 1; k < npts && !multi; ++k)
+            multi = !(P[3 * k] == P[0] && P[3 * k + 1] == P[1] && P[3 * k + 2] == P[2]);
+        Box box;
+        if (!multi) {
+            box = empty_box(P, npts);
+        } else {
+            const double N = double(npts);
+            double sx = 0, sy = 0, sz = 0;
+            for (uint32_t k = 0; k < npts; ++k) {
+                sx += double(P[3 * k]);
+                sy += double(P[3 * k + 1]);
+                sz += double(P[3 * k + 2]);
+            }
+            const double mx = sx / N, my = sy / N, mz = sz / N;
+            double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
+            for (uint32_t k = 0; k < npts; ++k) {
+                const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+                cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
+                cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
+            }
+            double ax[9];
+            axes_of(cxx / N, cyy / N, czz / N, cxy / N, cxz / N, cyz / N, ax);
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t k = 0; k < npts; ++k) {
+                const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+                for (int q = 0; q < 3; ++q) {
+                    const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
+                    lo[q] = fmin(lo[q], pr);
+                    hi[q] = fmax(hi[q], pr);
+                }
+            }
+            box = box_from_extents(ax, lo, hi);
+        }
+        store_box(m, nodes, item.y, box);
+        float b12[12];
+        box_to12(box, b12);
+        const SlackFrame f = slack_frame(b12);
+        for (uint32_t k = 0; k < npts; ++k) {
+            const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+            worst = fmax(worst, excess(f, p));
+            rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+        }
     }
-    post_slack(slack_bits, radius_bits, item.x, item.y == ROOT_CODE, worst, rad);
+    const uint32_t mesh0 = __shfl(item.x, 0, 64);
+    const bool one_mesh = __ballot(valid && item.x != mesh0) == 0ull;
+    const bool root = valid && item.y == ROOT_CODE;
+    if (one_mesh && __ballot(root) == 0ull) {  // the usual case: one mesh, no root box in this wavefront
+        worst = wave_max_d(worst);
+        if ((threadIdx.x & 63u) == 0) post_slack(slack_bits, radius_bits, mesh0, false, worst, 0.0);
+    } else if (valid) {
+        post_slack(slack_bits, radius_bits, item.x, root, worst, rad);
+    }
 }
 
 // ---- large boxes: chunk passes ----
