@@ -453,10 +453,23 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
     const double u[3] = {nd.box[3], nd.box[4], nd.box[5]};
     const double v[3] = {nd.box[6], nd.box[7], nd.box[8]};
     const double w[3] = {nd.box[9], nd.box[10], nd.box[11]};
-    auto det3 = [](const double* a, const double* b, const double* c) {
-        return a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+    // a point's coordinates in the side vectors: its offset dotted with the
+    // dual basis (v x w, w x u, u x v) / det(u, v, w)
+    auto cross = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
     };
-    const double D = det3(u, v, w);
+    double du[3], dv[3], dw[3];
+    cross(v, w, du);
+    cross(w, u, dv);
+    cross(u, v, dw);
+    const double D = u[0] * du[0] + u[1] * du[1] + u[2] * du[2];
+    for (int k = 0; k < 3; ++k) {
+        du[k] = du[k] / D;
+        dv[k] = dv[k] / D;
+        dw[k] = dw[k] / D;
+    }
     const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     const double lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const double lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
@@ -469,7 +482,9 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
         const double r[3] = {p[0] - nd.box[0], p[1] - nd.box[1], p[2] - nd.box[2]};
         double ex;
         if (fabs(D) > 0.0) {
-            const double al = det3(r, v, w) / D, be = det3(u, r, w) / D, ga = det3(u, v, r) / D;
+            const double al = r[0] * du[0] + r[1] * du[1] + r[2] * du[2];
+            const double be = r[0] * dv[0] + r[1] * dv[1] + r[2] * dv[2];
+            const double ga = r[0] * dw[0] + r[1] * dw[1] + r[2] * dw[2];
             ex = fmax(0.0, fabs(al) - 1.0) * lu + fmax(0.0, fabs(be) - 1.0) * lv + fmax(0.0, fabs(ga) - 1.0) * lw;
         } else {
             ex = INFINITY;

@@ -184,30 +184,50 @@ __device__ inline void store_box(MeshRec& m, NodeRec* nodes, uint32_t code, cons
 }
 // How far a point lies outside a box (distance along each side beyond the
 // box), in FP64 from the stored float box.
+// A box's frame for the containment slack: a point's coordinates in the
+// box's side vectors are its offset dotted with the dual basis (v x w, w x u,
+// u x v) / det(u, v, w), computed once per box.
 struct SlackFrame {
-    double c[3], u[3], v[3], w[3], D, lu, lv, lw;
+    double c[3], du[3], dv[3], dw[3], lu, lv, lw;
+    bool flat;
 };
-__device__ inline double det3(const double* a, const double* b, const double* cc) {
-    return a[0] * (b[1] * cc[2] - b[2] * cc[1]) - a[1] * (b[0] * cc[2] - b[2] * cc[0]) + a[2] * (b[0] * cc[1] - b[1] * cc[0]);
+__device__ inline void cross_d(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
 }
 __device__ inline SlackFrame slack_frame(const float* bx) {
     SlackFrame f;
+    double u[3], v[3], w[3];
     for (int k = 0; k < 3; ++k) {
         f.c[k] = bx[k];
-        f.u[k] = bx[3 + k];
-        f.v[k] = bx[6 + k];
-        f.w[k] = bx[9 + k];
+        u[k] = bx[3 + k];
+        v[k] = bx[6 + k];
+        w[k] = bx[9 + k];
     }
-    f.D = det3(f.u, f.v, f.w);
-    f.lu = sqrt(f.u[0] * f.u[0] + f.u[1] * f.u[1] + f.u[2] * f.u[2]);
-    f.lv = sqrt(f.v[0] * f.v[0] + f.v[1] * f.v[1] + f.v[2] * f.v[2]);
-    f.lw = sqrt(f.w[0] * f.w[0] + f.w[1] * f.w[1] + f.w[2] * f.w[2]);
+    double vw[3], wu[3], uv[3];
+    cross_d(v, w, vw);
+    cross_d(w, u, wu);
+    cross_d(u, v, uv);
+    const double D = u[0] * vw[0] + u[1] * vw[1] + u[2] * vw[2];
+    f.flat = !(fabs(D) > 0.0);
+    for (int k = 0; k < 3; ++k) {
+        f.du[k] = vw[k] / D;
+        f.dv[k] = wu[k] / D;
+        f.dw[k] = uv[k] / D;
+    }
+    f.lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    f.lv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    f.lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
     return f;
 }
+// How far p lies outside the box, measured along its side vectors.
 __device__ inline double excess(const SlackFrame& f, const double* p) {
+    if (f.flat) return INFINITY;
     const double r[3] = {p[0] - f.c[0], p[1] - f.c[1], p[2] - f.c[2]};
-    if (!(fabs(f.D) > 0.0)) return INFINITY;
-    const double al = det3(r, f.v, f.w) / f.D, be = det3(f.u, r, f.w) / f.D, ga = det3(f.u, f.v, r) / f.D;
+    const double al = r[0] * f.du[0] + r[1] * f.du[1] + r[2] * f.du[2];
+    const double be = r[0] * f.dv[0] + r[1] * f.dv[1] + r[2] * f.dv[2];
+    const double ga = r[0] * f.dw[0] + r[1] * f.dw[1] + r[2] * f.dw[2];
     return fmax(0.0, fabs(al) - 1.0) * f.lu + fmax(0.0, fabs(be) - 1.0) * f.lv + fmax(0.0, fabs(ga) - 1.0) * f.lw;
 }
 __device__ inline void post_slack(unsigned int* slack_bits, unsigned int* radius_bits, uint32_t mesh, bool root,
