@@ -490,13 +490,14 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
             ex = INFINITY;
         }
         worst = fmax(worst, ex);
-        rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+        if (nd.level == 0) rad = fmax(rad, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);  // squared; roots only
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         worst = fmax(worst, __shfl_xor(worst, off, 64));
         rad = fmax(rad, __shfl_xor(rad, off, 64));
     }
+    rad = sqrt(rad);  // sqrt is monotone: the root of the largest square is the largest root
     if (lane == 0) {
         // round up to float so the bound stays an upper bound
         float wf = float(worst);

@@ -1516,20 +1516,20 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
     }
     HCK(ctx, ctx->d_refit_scratch.ensure((size_t(ctx->refit_nitems) + 1) * 24 * sizeof(double)));
     const uint32_t nm = uint32_t(ctx->meshes.size());
-    HCK(ctx, ctx->d_refit_bits.ensure(2 * size_t(nm) * sizeof(unsigned int)));
+    HCK(ctx, ctx->d_refit_bits.ensure((SLACK_REPS + 1) * size_t(nm) * sizeof(unsigned int)));
     unsigned int* bits = ctx->d_refit_bits.as<unsigned int>();
     HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_skin(dsk, n, dvs, ctx->d_bind.as<float>(), ctx->d_joints.as<ushort4>(), ctx->d_weights.as<float4>(),
                          ctx->d_bones.as<float>(), vtot, ctx->d_skinpos.as<float>(), st));
     HCK(ctx, launch_gather_tris(dsk, n, dts, ctx->d_vidx.as<uint32_t>(), ctx->d_skinpos.as<float>(), ctx->d_tris.as<float>(),
                                 ttot, st));
-    HCK(ctx, launch_fill_u32(bits, 2 * nm, 0u, st));
+    HCK(ctx, launch_fill_u32(bits, (SLACK_REPS + 1) * nm, 0u, st));
     const uint2* items = ctx->d_refit_items.as<uint2>();
     const uint32_t* ids = reinterpret_cast<const uint32_t*>(ctx->d_refit_items.as<char>() + ctx->refit_nitems * sizeof(uint2));
     (void)items;
     HCK(ctx, launch_refit(ctx->refit_plan, ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
                           ctx->d_tris.as<float>(), ctx->d_refit_scratch.as<double>(),
-                          ctx->d_refit_partial.as<double>(), bits, bits + nm, ids, n, st));
+                          ctx->d_refit_partial.as<double>(), bits, bits + size_t(SLACK_REPS) * nm, ids, n, st));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
     // keep the host copy of the mesh table in step (root boxes, slack, radius): one copy of the id span
     const uint32_t lo_id = *std::min_element(mesh_ids, mesh_ids + n), hi_id = *std::max_element(mesh_ids, mesh_ids + n);

@@ -187,6 +187,9 @@ struct PrimSpan {
 static_assert(sizeof(PrimSpan) == 32, "primitive span layout");
 static_assert(sizeof(cd_node) == 64, "node layout");
 
+// Copies of each mesh's slack maximum in the refit (see post_slack).
+constexpr uint32_t SLACK_REPS = 64;
+
 // Device-side counters of one run (zeroed by a fill kernel, never by a
 // memset, so a run can be captured into a HIP graph).
 struct RunCounters {

@@ -230,13 +230,17 @@ __device__ inline double excess(const SlackFrame& f, const double* p) {
     const double ga = r[0] * f.dw[0] + r[1] * f.dw[1] + r[2] * f.dw[2];
     return fmax(0.0, fabs(al) - 1.0) * f.lu + fmax(0.0, fabs(be) - 1.0) * f.lv + fmax(0.0, fabs(ga) - 1.0) * f.lw;
 }
+// Per-mesh maxima: the slack bound goes to one of SLACK_REPS copies per mesh
+// (chosen by the caller, e.g. its chunk or wavefront index), so the
+// thousands of boxes of a mesh do not all queue on one address;
+// k_store_slack takes the largest copy.
 __device__ inline void post_slack(unsigned int* slack_bits, unsigned int* radius_bits, uint32_t mesh, bool root,
-                                  double worst, double rad) {
+                                  double worst, double rad, uint32_t rep) {
     float wf = float(worst);
     if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
     float rf = float(rad);
     if (double(rf) < rad) rf = nextafterf(rf, INFINITY);
-    atomicMax(&slack_bits[mesh], __float_as_uint(wf));
+    atomicMax(&slack_bits[size_t(mesh) * SLACK_REPS + (rep % SLACK_REPS)], __float_as_uint(wf));
     if (root) atomicMax(&radius_bits[mesh], __float_as_uint(rf));
 }
 __device__ inline const float* box_ptr(const MeshRec& m, const NodeRec* nodes, uint32_t code) {
@@ -311,20 +315,22 @@ __global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ item
         float b12[12];
         box_to12(box, b12);
         const SlackFrame f = slack_frame(b12);
+        double rad2 = 0.0;
         for (uint32_t k = 0; k < npts; ++k) {
             const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
             worst = fmax(worst, excess(f, p));
-            rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+            rad2 = fmax(rad2, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
         }
+        rad = sqrt(rad2);  // sqrt is monotone: the root of the largest square is the largest root
     }
     const uint32_t mesh0 = __shfl(item.x, 0, 64);
     const bool one_mesh = __ballot(valid && item.x != mesh0) == 0ull;
     const bool root = valid && item.y == ROOT_CODE;
     if (one_mesh && __ballot(root) == 0ull) {  // the usual case: one mesh, no root box in this wavefront
         worst = wave_max_d(worst);
-        if ((threadIdx.x & 63u) == 0) post_slack(slack_bits, radius_bits, mesh0, false, worst, 0.0);
+        if ((threadIdx.x & 63u) == 0) post_slack(slack_bits, radius_bits, mesh0, false, worst, 0.0, i >> 6);
     } else if (valid) {
-        post_slack(slack_bits, radius_bits, item.x, root, worst, rad);
+        post_slack(slack_bits, radius_bits, item.x, root, worst, rad, i >> 6);
     }
 }
 
@@ -403,15 +409,16 @@ __global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ item
             }
     } else {
         const SlackFrame f = slack_frame(box_ptr(m, nodes, item.y));
-        double worst = 0.0, rad = 0.0;
+        const bool root = item.y == ROOT_CODE;  // only the root box's radius is kept
+        double worst = 0.0, rad2 = 0.0;
         for (uint32_t k = lane; k < npts; k += 64) {
             const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
             worst = fmax(worst, excess(f, p));
-            rad = fmax(rad, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+            if (root) rad2 = fmax(rad2, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
         }
         worst = wave_max_d(worst);
-        rad = wave_max_d(rad);
-        if (lane == 0) post_slack(slack_bits, radius_bits, item.x, item.y == ROOT_CODE, worst, rad);
+        rad2 = wave_max_d(rad2);  // sqrt is monotone: the root of the largest square is the largest root
+        if (lane == 0) post_slack(slack_bits, radius_bits, item.x, root, worst, sqrt(rad2), ci);
     }
 }
 
@@ -469,7 +476,9 @@ __global__ void k_store_slack(MeshRec* meshes, const uint32_t* ids, uint32_t n, 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     MeshRec& m = meshes[ids[i]];
-    m.slack = __uint_as_float(slack_bits[ids[i]]);
+    unsigned int sb = 0;
+    for (uint32_t r = 0; r < SLACK_REPS; ++r) sb = max(sb, slack_bits[size_t(ids[i]) * SLACK_REPS + r]);
+    m.slack = __uint_as_float(sb);
     m.radius = __uint_as_float(radius_bits[ids[i]]);
 }
 
