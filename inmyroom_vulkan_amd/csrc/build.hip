@@ -504,7 +504,8 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
         if (double(wf) < worst) wf = nextafterf(wf, INFINITY);
         float rf = float(rad);
         if (double(rf) < rad) rf = nextafterf(rf, INFINITY);
-        atomicMax(&mesh_slack_bits[nd.mesh], __float_as_uint(wf));
+        // one of SLACK_REPS copies per mesh, so a mesh's many nodes do not queue on one address
+        atomicMax(&mesh_slack_bits[size_t(nd.mesh) * SLACK_REPS + (ni % SLACK_REPS)], __float_as_uint(wf));
         if (nd.level == 0) atomicMax(&mesh_radius_bits[nd.mesh], __float_as_uint(rf));
     }
 }
@@ -539,7 +540,8 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     std::vector<BNode> roots(nmesh);
     std::vector<std::pair<uint32_t, uint32_t>> levels;
     std::vector<uint32_t> host_ids;
-    std::vector<uint32_t> node_base(nmesh), tri_base(nmesh), depth(nmesh), slack(nmesh), radius(nmesh);
+    std::vector<uint32_t> node_base(nmesh), tri_base(nmesh), depth(nmesh), slack(size_t(nmesh) * SLACK_REPS),
+        radius(nmesh);
     uint64_t total_internal = 0;
     uint32_t first = 0, count = nmesh, cur = 0;
     const size_t posb = std::max<uint64_t>(T, 1) * 9 * sizeof(float), idb = std::max<uint64_t>(T, 1) * sizeof(uint32_t);
@@ -552,7 +554,7 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     CK(id_fin.ensure(idb));
     CK(nodes.ensure(node_cap * sizeof(BNode)));
     CK(mesh_depth.ensure(nmesh * sizeof(uint32_t)));
-    CK(slack_bits.ensure(nmesh * sizeof(uint32_t)));
+    CK(slack_bits.ensure(size_t(nmesh) * SLACK_REPS * sizeof(uint32_t)));
     CK(radius_bits.ensure(nmesh * sizeof(uint32_t)));
     CK(next_count.ensure(sizeof(uint32_t)));
     CK(base_nodes.ensure(nmesh * sizeof(uint32_t)));
@@ -577,7 +579,7 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     }
     CK(hipMemcpyAsync(nodes.p, roots.data(), nmesh * sizeof(BNode), hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(mesh_depth.p, 0, nmesh * sizeof(uint32_t), st));
-    CK(hipMemsetAsync(slack_bits.p, 0, nmesh * sizeof(uint32_t), st));
+    CK(hipMemsetAsync(slack_bits.p, 0, size_t(nmesh) * SLACK_REPS * sizeof(uint32_t), st));
     CK(hipMemsetAsync(radius_bits.p, 0, nmesh * sizeof(uint32_t), st));
 
     while (count > 0) {
@@ -665,7 +667,8 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
         hipLaunchKernelGGL(k_slack, dim3(total), dim3(64), 0, st, nodes.as<BNode>(), total, pos_fin.as<float>(),
                            slack_bits.as<unsigned int>(), radius_bits.as<unsigned int>());
         CK(hipGetLastError());
-        CK(hipMemcpyAsync(slack.data(), slack_bits.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(slack.data(), slack_bits.p, size_t(nmesh) * SLACK_REPS * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
         CK(hipMemcpyAsync(radius.data(), radius_bits.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         res.meshes.resize(nmesh);
@@ -678,7 +681,9 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
             std::memcpy(r.root, roots[m].box, sizeof r.root);
             r.root_surf = roots[m].surf;
             r.depth = depth[m];
-            std::memcpy(&r.slack, &slack[m], sizeof(float));
+            uint32_t sb = 0;  // the largest of the mesh's copies (positive floats order as their bits)
+            for (uint32_t k = 0; k < SLACK_REPS; ++k) sb = std::max(sb, slack[size_t(m) * SLACK_REPS + k]);
+            std::memcpy(&r.slack, &sb, sizeof(float));
             std::memcpy(&r.radius, &radius[m], sizeof(float));
         }
         ctx->nodes_used += total_internal;
