@@ -41,6 +41,12 @@ def config2(args):
 
     sc = scenes.Scene(n_entities=512, seed=2)
     gpu = Collider(leaf_max=4)
+    tuning = {}
+    for kv in filter(None, getattr(args, "tuning", "").split(",")):
+        k, v = kv.split("=")
+        tuning[k.strip()] = int(v)
+    if tuning:
+        gpu.set_tuning(**tuning)
     ids = gpu.register_soups(sc.soups)
     frames = []
     for _ in range(100):
@@ -166,7 +172,11 @@ def config5(args):
     q = np.zeros(64, QUERY_DTYPE)
     q["mesh_a"], q["mesh_b"], q["rel"] = ids, terrain, scenes.identity16()
     frames = [db.bone_matrices() for _ in range(args.warmup + args.steps)]
-    gpu.set_tuning(use_graph=0)
+    tuning = {"use_graph": 0}
+    for kv in filter(None, getattr(args, "tuning", "").split(",")):
+        k, v = kv.split("=")
+        tuning[k.strip()] = int(v)
+    gpu.set_tuning(**tuning)
     refit_ms, query_ms, wall, ncon = [], [], [], 0
     for k, bones in enumerate(frames):
         t = time.perf_counter()

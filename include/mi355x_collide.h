@@ -159,8 +159,8 @@ typedef struct cd_timings {   /* device time of the last run, from HIP events */
  * the reference has no GPU).  Chosen per workload by the autotuner
  * (inmyroom_vulkan_amd/autotune.py); zero fields mean "default".            */
 typedef struct cd_tuning {
-    uint32_t split_levels;   /* breadth-first levels before the persistent kernel (work-split granularity) */
-    uint32_t stack_entries;  /* LDS stack entries per wavefront: 512, 1024 or 2048 */
+    uint32_t split_levels;   /* breadth-first levels before the persistent kernel (work-split granularity; 0: 16) */
+    uint32_t stack_entries;  /* LDS stack entries per wavefront: 512, 1024 or 2048 (0: 512) */
     uint32_t pop_width;      /* node pairs a wavefront pops per step, 1..64 */
     uint32_t waves_per_cu;   /* persistent wavefronts per compute unit */
     uint32_t use_graph;      /* 1: capture the batched pipeline in a HIP graph and replay it */
@@ -173,7 +173,7 @@ typedef struct cd_tuning {
                                    shared out by a second launch */
     uint32_t spin_polls;     /* polls an idle wavefront waits for a donation before it leaves (0: 4096) */
     uint32_t split_tris;     /* a split level keeps splitting only node pairs over this many triangles
-                                (0: all of them); the others go to the persistent kernel */
+                                (0: 512; 1: all of them); the others go to the persistent kernel */
     uint32_t handoff_pct;    /* work_sharing 3: the budget, percent of the mean (0: 100) */
     uint32_t resplit_levels; /* work_sharing 3: breadth-first levels over what was handed on (0: 4) */
 } cd_tuning;

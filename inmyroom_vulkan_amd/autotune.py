@@ -24,7 +24,7 @@ TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
 SPACE = {
     # joint keys: each value sets both fields
-    "split_levels/split_tris": [(6, 0), (8, 0), (10, 0), (12, 0), (16, 512), (24, 2048), (32, 8192)],
+    "split_levels/split_tris": [(6, 1), (8, 1), (10, 1), (12, 1), (16, 512), (24, 2048), (32, 8192)],
     "stack_entries": [512, 1024],
     "pop_width": [64],
     "waves_per_cu": [2, 4, 8],
@@ -109,7 +109,7 @@ def main():
     q["mesh_a"], q["mesh_b"], q["rel"] = ids[qa], ids[qb], rel
     space = None
     if args.quick:
-        space = {"split_levels/split_tris": [(10, 0), (24, 2048)], "stack_entries": [512], "pop_width": [64],
+        space = {"split_levels/split_tris": [(10, 1), (24, 2048)], "stack_entries": [512], "pop_width": [64],
                  "waves_per_cu": [4], "work_sharing": [0, 1]}
     res = tune(gpu, q, space=space, log=lambda s: print(s, flush=True))
     save(args.nq, res, args.out)

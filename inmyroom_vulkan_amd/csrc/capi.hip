@@ -163,10 +163,14 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
 }
 
 cd_tuning effective_tuning(const cd_ctx* ctx) {
+    // defaults: the values the autotuner picks for the 4,096-query batch, which
+    // also beat the earlier defaults (8 levels, every pair split, 1,024-entry
+    // stacks) on scene frames and on the deformable batch
     cd_tuning t = ctx->tuning;
-    if (t.split_levels == 0) t.split_levels = 8;
+    if (t.split_levels == 0) t.split_levels = 16;
     if (t.split_levels > 64) t.split_levels = 64;
-    if (t.stack_entries != 512 && t.stack_entries != 2048) t.stack_entries = 1024;
+    if (t.split_tris == 0) t.split_tris = 512;
+    if (t.stack_entries != 1024 && t.stack_entries != 2048) t.stack_entries = 512;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
     if (t.waves_per_cu == 0) t.waves_per_cu = 8;
     if (t.spin_polls == 0) t.spin_polls = 4096;
