@@ -41,12 +41,11 @@ def config2(args):
 
     sc = scenes.Scene(n_entities=512, seed=2)
     gpu = Collider(leaf_max=4)
-    tuning = {}
+    tuning = {"use_graph": 1}  # batched frames replay one captured graph whatever their pair count
     for kv in filter(None, getattr(args, "tuning", "").split(",")):
         k, v = kv.split("=")
         tuning[k.strip()] = int(v)
-    if tuning:
-        gpu.set_tuning(**tuning)
+    gpu.set_tuning(**tuning)
     ids = gpu.register_soups(sc.soups)
     frames = []
     for _ in range(100):

@@ -72,6 +72,9 @@ public:
         cd_config cfg{};
         cfg.leaf_max = leaf_max;
         check(cd_create(&ctx_, &cfg), "cd_create");
+        cd_tuning t{};
+        t.use_graph = 1;  // the batched pipeline is captured once and replayed every frame
+        check(cd_set_tuning(ctx_, &t), "cd_set_tuning");
     }
     ~CollisionDetection() {
         if (ctx_) cd_destroy(ctx_);

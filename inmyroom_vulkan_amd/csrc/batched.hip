@@ -223,10 +223,10 @@ __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-__global__ void k_setup(const QueryRec* __restrict__ qs, uint32_t nq, const MeshRec* __restrict__ meshes,
-                        QueryPre* __restrict__ out) {
+__global__ void k_setup(const QueryRec* __restrict__ qs, const unsigned int* __restrict__ nq_p,
+                        const MeshRec* __restrict__ meshes, QueryPre* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
+    if (i >= *nq_p) return;
     const QueryRec q = qs[i];
     const MeshRec& A = meshes[q.mesh_a];
     const MeshRec& B = meshes[q.mesh_b];
@@ -260,9 +260,10 @@ __global__ void k_setup(const QueryRec* __restrict__ qs, uint32_t nq, const Mesh
     out[i] = p;
 }
 
-__global__ void k_seed(uint32_t nq, const QueryPre* __restrict__ qp, uint3* __restrict__ frontier,
-                       unsigned int* __restrict__ count) {
+__global__ void k_seed(const unsigned int* __restrict__ nq_p, const QueryPre* __restrict__ qp,
+                       uint3* __restrict__ frontier, unsigned int* __restrict__ count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nq = *nq_p;
     if (i < nq) frontier[i] = make_uint3(i, ROOT_FLAG | qp[i].mesh_a, ROOT_FLAG | qp[i].mesh_b);
     if (i == 0) *count = nq;
 }
@@ -1123,13 +1124,17 @@ __global__ void k_keep_budget(const RunCounters* __restrict__ ctr, unsigned long
 }
 
 // ---- host launchers -------------------------------------------------------
-hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st) {
-    if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3((nq + 127) / 128), dim3(128), 0, st, qs, nq, meshes, out);
+// nq_cap sizes the grids; the kernels read the query count from nq_p (<= nq_cap),
+// so one captured graph serves every count up to its capacity
+hipError_t launch_setup(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
+                        QueryPre* out, hipStream_t st) {
+    if (!nq_cap) return hipSuccess;
+    hipLaunchKernelGGL(k_setup, dim3((nq_cap + 127) / 128), dim3(128), 0, st, qs, nq_p, meshes, out);
     return hipGetLastError();
 }
-hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigned int* count, hipStream_t st) {
-    hipLaunchKernelGGL(k_seed, dim3((std::max(nq, 1u) + 255) / 256), dim3(256), 0, st, nq, qp, frontier, count);
+hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre* qp, uint3* frontier,
+                       unsigned int* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed, dim3((std::max(nq_cap, 1u) + 255) / 256), dim3(256), 0, st, nq_p, qp, frontier, count);
     return hipGetLastError();
 }
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,

@@ -222,6 +222,7 @@ uint64_t frontier_cap(const cd_ctx* ctx, uint32_t nq, uint32_t L) {
     return std::max<uint64_t>(64, std::min<uint64_t>(worst, std::max<uint64_t>(ctx->frontier_cap, 1u << 20)));
 }
 
+// nq: the capacity the launch geometry is built for; the count itself is in d_nq
 int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     hipStream_t st = ctx->stream;
     const uint32_t L = t.split_levels;
@@ -240,8 +241,9 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     if (t.work_sharing == 1)  // ready words of the donation queue start clear
         HCK(ctx, launch_fill_u32(reinterpret_cast<unsigned int*>(dq), 4 * DQ_CAP, 0u, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
-    HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dm, qp, st));
-    HCK(ctx, launch_seed(nq, qp, F[0], &small[0], st));
+    const unsigned int* dnq = ctx->d_nq.as<unsigned int>();
+    HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dnq, dm, qp, st));
+    HCK(ctx, launch_seed(nq, dnq, qp, F[0], &small[0], st));
     uint3* T = ctx->d_work[W_TASKS].as<uint3>();
     const uint32_t cap32 = uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull));
     for (uint32_t l = 0; l < L; ++l) {  // the last level sends every split to the task list
@@ -300,7 +302,16 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
     if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
     if (ctx->defer_cap == 0) ctx->defer_cap = 1u << 16;
-    HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(std::max(nq, 1u)) * sizeof(QueryPre)));
+    // the launch geometry (and a captured graph) is built for a power-of-two
+    // capacity; the kernels read the count itself from d_nq, so frames whose
+    // pair counts differ replay one graph
+    uint32_t nq_cap = 64;
+    while (nq_cap < nq) nq_cap *= 2;
+    HCK(ctx, ctx->d_nq.ensure(sizeof(unsigned int)));
+    HCK(ctx, ctx->h_nq.ensure(sizeof(unsigned int)));
+    *static_cast<unsigned int*>(ctx->h_nq.p) = nq;
+    HCK(ctx, hipMemcpyAsync(ctx->d_nq.p, ctx->h_nq.p, sizeof(unsigned int), hipMemcpyHostToDevice, st));
+    HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(nq_cap) * sizeof(QueryPre)));
     HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
     if (t.work_sharing == 1 || t.work_sharing == 2)
@@ -311,7 +322,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     }
     RunCounters h{};
     for (int attempt = 0; attempt < 8; ++attempt) {
-        const uint64_t fcap = frontier_cap(ctx, nq, L);
+        const uint64_t fcap = frontier_cap(ctx, nq_cap, L);
         HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
         HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
@@ -323,13 +334,14 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
         HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
-        std::vector<uint64_t> key = {nq, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
+        std::vector<uint64_t> key = {nq_cap, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
                                      t.split_tris, t.handoff_pct, t.resplit_levels, ctx->lp_cap, ctx->contact_cap,
                                      ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
                                 &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
                                 &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER], &ctx->d_budget,
+                                &ctx->d_nq,
                                 &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2], &ctx->d_work[W_HITS]})
             key.push_back(reinterpret_cast<uint64_t>(b->p));
         if (t.use_graph) {
@@ -341,7 +353,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
                 }
                 hipGraph_t g = nullptr;
                 HCK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_batched(ctx, nq, t, false);
+                int rc = enqueue_batched(ctx, nq_cap, t, false);
                 hipError_t ec = hipStreamEndCapture(st, &g);
                 if (rc) {
                     if (g) (void)hipGraphDestroy(g);
@@ -357,7 +369,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
             HCK(ctx, hipGraphLaunch(ctx->graph, st));
             HCK(ctx, hipEventRecord(ctx->ev[5], st));
         } else {
-            int rc = enqueue_batched(ctx, nq, t, true);
+            int rc = enqueue_batched(ctx, nq_cap, t, true);
             if (rc) return rc;
         }
         int rc = read_counters(ctx, h);
@@ -585,6 +597,8 @@ int cd_destroy(cd_ctx* ctx) {
         b->release();
     for (auto& b : ctx->d_work) b.release();
     ctx->d_budget.release();
+    ctx->d_nq.release();
+    ctx->h_nq.release();
     ctx->h_stage.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto& e : ctx->ev)

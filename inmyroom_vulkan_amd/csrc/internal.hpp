@@ -241,6 +241,8 @@ struct cd_ctx {
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
     cdk::HostBuf h_stage;  // pinned staging of results copied back to the host
     cdk::DevBuf d_budget;  // node-pair tests of the last batched run (budgeted traversal pass)
+    cdk::DevBuf d_nq;      // query count of the batched run, read by its kernels (graph replays keep it live)
+    cdk::HostBuf h_nq;     // pinned staging of that count
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;

@@ -27,8 +27,10 @@ hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max
                                RunCounters* ctr, hipStream_t st);
 
 // batched.hip
-hipError_t launch_setup(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, QueryPre* out, hipStream_t st);
-hipError_t launch_seed(uint32_t nq, const QueryPre* qp, uint3* frontier, unsigned int* count, hipStream_t st);
+hipError_t launch_setup(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
+                        QueryPre* out, hipStream_t st);
+hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre* qp, uint3* frontier,
+                       unsigned int* count, hipStream_t st);
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,

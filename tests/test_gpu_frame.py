@@ -73,6 +73,40 @@ def test_frame_batched_matches_exact_oracle(gpu, orc):
     assert got == want and len(want) > 0
 
 
+def test_frame_batched_graph_replays(orc):
+    """Frames through one captured graph: the pair count changes from frame
+    to frame (the kernels read it from the device), every replay against the
+    exact oracle, with eager frames in between."""
+    from inmyroom_vulkan_amd import Collider
+
+    sc = _scene(200, 24, cube=18.0)
+    case("frame batched n=200 through graph replays")
+    with Collider(leaf_max=4) as c:
+        c.set_tuning(use_graph=1)
+        ids = c.register_soups(sc.soups)
+        trees = [orc.tree(s, 4) for s in sc.soups]
+        counts = set()
+        for frame in range(6):
+            if frame == 3:
+                c.set_tuning(use_graph=0)  # an eager frame between replays
+            if frame == 4:
+                c.set_tuning(use_graph=1)
+            ents = sc.entities(ids, ENTITY_DTYPE)
+            pairs, cons = c.frame(ents, MODE_BATCHED)
+            res = orc.frame(trees, sc.mesh_of, ents["current"], sc.should_callback, mode=1)
+            want = {(int(res.pairs[q["query"]][0]), int(res.pairs[q["query"]][1]), int(q["tri_a"]), int(q["tri_b"]))
+                    for q in res.contacts}
+            got = set()
+            for p in pairs:
+                for q in cons[p["contact_begin"]: p["contact_begin"] + p["contact_count"]]:
+                    got.add((int(p["first"]), int(p["second"]), int(q["tri_a"]), int(q["tri_b"])))
+            assert got == want and len(want) > 0, frame
+            counts.add(c.counters()["broad_pairs"])
+            for _ in range(3):
+                sc.step()
+        assert len(counts) > 1  # the replays saw different pair counts
+
+
 def test_frame_many_contacts_sorted(gpu, orc):
     """The 512-entity scene: tens of thousands of contacts go through the
     multi-workgroup radix sort; they come back grouped by pair in (first,
