@@ -191,12 +191,36 @@ def config5(args):
             ncon += n
     ms = float(np.mean(wall))
     log(f"config 5: {ms:.3f} ms/frame wall; refit {np.mean(refit_ms):.3f} ms, queries {np.mean(query_ms):.3f} ms")
+    cpu = None
+    if not args.no_cpu:
+        # the reference has no refit: a deformable mesh's tree would be rebuilt from
+        # its skinned triangles; per mesh: CPU skinning, the oracle's tree build and
+        # its exact query against the (prebuilt) terrain tree
+        from oracle import Oracle
+
+        orc = Oracle()
+        terrain_tree = orc.tree(db.tP.astype(np.float32)[db.tF].reshape(-1, 9), 4)
+        bones = frames[-1].reshape(64, -1)
+        rel = scenes.identity16()[None]
+        t = time.perf_counter()
+        done = 0
+        for k in range(64):
+            Pk = scenes.DeformableBatch.skin_reference(db.P, db.joints, db.weights, bones[k]).astype(np.float32)
+            tree = orc.tree(Pk[db.F].reshape(-1, 9), 4)
+            orc.batch([tree, terrain_tree], np.zeros(1, np.uint32), np.ones(1, np.uint32), rel, mode=1)
+            done += 1
+            if time.perf_counter() - t > args.cpu_seconds:
+                break
+        per_frame = (time.perf_counter() - t) / done * 64
+        cpu = {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{done} of the 64 meshes of one frame (CPU skinning, tree rebuild, exact query against "
+                         "the terrain), extrapolated to 64, single thread"}
     return _line("deformable batch frames/s (skin + refit + collide vs terrain)", 1e3 / ms, "frames/s", ms, args,
                  {"workload": "64 skinned tori x 20,000 triangles (64 bones, 4 weights/vertex) vs 198,450-triangle "
                               "fBm terrain", "parallelism": "single GPU"},
                  dtype="f32 (f64 predicate fallback)",
                  extra={"refit_ms": round(float(np.mean(refit_ms)), 4), "query_ms": round(float(np.mean(query_ms)), 4),
-                        "contacts_per_frame": ncon / max(1, len(wall))})
+                        "contacts_per_frame": ncon / max(1, len(wall)), "cpu_baseline": cpu})
 
 
 def config6(args):
