@@ -92,24 +92,45 @@ __global__ void k_cand(const cd_contact* __restrict__ c, const uint32_t* __restr
             tw += gw;
         }
     };
-    for (uint32_t j = i; j < n; ++j) {
-        const cd_contact& cj = at(c, order, j);
-        if (cj.query != ci.query || own_tri(cj, side) != tri) break;
-        const uint32_t other = side ? cj.tri_a : cj.tri_b;
-        const uint32_t lf = leafof[size_t(other_base) + other];
-        if (j > i && lf != leaf) {
-            flush();
-            gwp = mk(0.f, 0.f, 0.f);
-            gw = 0.f;
-            gmask = 7u;
+    // The run is walked in order (the sums are order-dependent); its contacts
+    // and their leaves are fetched RUN_FETCH at a time so that their loads
+    // overlap instead of forming one round trip per contact.
+    constexpr uint32_t RUN_FETCH = 4;
+    bool more = true;
+    for (uint32_t j0 = i; more && j0 < n; j0 += RUN_FETCH) {
+        const uint32_t m = min(RUN_FETCH, n - j0);
+        cd_contact cj[RUN_FETCH];
+        uint32_t lf[RUN_FETCH];
+#pragma unroll
+        for (uint32_t u = 0; u < RUN_FETCH; ++u)
+            if (u < m) cj[u] = at(c, order, j0 + u);
+#pragma unroll
+        for (uint32_t u = 0; u < RUN_FETCH; ++u) {
+            const bool in_run = u < m && cj[u].query == ci.query && own_tri(cj[u], side) == tri;
+            lf[u] = in_run ? leafof[size_t(other_base) + (side ? cj[u].tri_a : cj[u].tri_b)] : 0u;
         }
-        leaf = lf;
-        const f3 s = mk(cj.s[0], cj.s[1], cj.s[2]), t = mk(cj.t[0], cj.t[1], cj.t[2]);
-        const float weight = vlength(sub(s, t));                 // :93
-        const f3 avg = divs(scale(add(s, t), weight), 2.f);      // :94
-        gwp = add(gwp, avg);
-        gw += weight;
-        gmask &= ~((cj.flags >> shift) & 7u);                   // :102-112
+#pragma unroll
+        for (uint32_t u = 0; u < RUN_FETCH; ++u) {
+            if (!more) break;
+            if (u >= m || cj[u].query != ci.query || own_tri(cj[u], side) != tri) {
+                more = false;
+                break;
+            }
+            const uint32_t j = j0 + u;
+            if (j > i && lf[u] != leaf) {
+                flush();
+                gwp = mk(0.f, 0.f, 0.f);
+                gw = 0.f;
+                gmask = 7u;
+            }
+            leaf = lf[u];
+            const f3 s = mk(cj[u].s[0], cj[u].s[1], cj[u].s[2]), t = mk(cj[u].t[0], cj[u].t[1], cj[u].t[2]);
+            const float weight = vlength(sub(s, t));                 // :93
+            const f3 avg = divs(scale(add(s, t), weight), 2.f);      // :94
+            gwp = add(gwp, avg);
+            gw += weight;
+            gmask &= ~((cj[u].flags >> shift) & 7u);                // :102-112
+        }
     }
     flush();
     if (have) {
