@@ -108,22 +108,37 @@ struct Hit {
     uint32_t tri;  // mesh-local tree-order slot
 };
 
+// A leaf's triangles are fetched LEAF_FETCH at a time (their loads overlap),
+// then tested in order.
+constexpr uint32_t LEAF_FETCH = 4;
 __device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, const M4& C, f3 dir, uint32_t off,
                           uint32_t cnt, Hit& best) {
-    for (uint32_t i = off; i != off + cnt; ++i) {
-        const float* t9 = tris + (size_t(m.tri_base) + i) * 9;
-        const f3 P0 = mul4(C, mk(t9[0], t9[1], t9[2]), 1.f);
-        const f3 P1 = mul4(C, mk(t9[3], t9[4], t9[5]), 1.f);
-        const f3 P2 = mul4(C, mk(t9[6], t9[7], t9[8]), 1.f);
-        float bx = 0.f, by = 0.f, dist = __builtin_huge_valf();
-        bool back = false;
-        if (ray_triangle(dir, P0, P1, P2, bx, by, dist, back) && dist > 0.f && dist < best.dist) {
-            best.hit = true;
-            best.back = back;
-            best.dist = dist;
-            best.tri = i;
-            best.bx = bx;
-            best.by = by;
+    for (uint32_t i0 = off; i0 < off + cnt; i0 += LEAF_FETCH) {
+        const uint32_t k = min(LEAF_FETCH, off + cnt - i0);
+        float t9[LEAF_FETCH][9];
+#pragma unroll
+        for (uint32_t u = 0; u < LEAF_FETCH; ++u)
+            if (u < k) {
+                const float* src = tris + (size_t(m.tri_base) + i0 + u) * 9;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) t9[u][e] = src[e];
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < LEAF_FETCH; ++u) {
+            if (u >= k) break;
+            const f3 P0 = mul4(C, mk(t9[u][0], t9[u][1], t9[u][2]), 1.f);
+            const f3 P1 = mul4(C, mk(t9[u][3], t9[u][4], t9[u][5]), 1.f);
+            const f3 P2 = mul4(C, mk(t9[u][6], t9[u][7], t9[u][8]), 1.f);
+            float bx = 0.f, by = 0.f, dist = __builtin_huge_valf();
+            bool back = false;
+            if (ray_triangle(dir, P0, P1, P2, bx, by, dist, back) && dist > 0.f && dist < best.dist) {
+                best.hit = true;
+                best.back = back;
+                best.dist = dist;
+                best.tri = i0 + u;
+                best.bx = bx;
+                best.by = by;
+            }
         }
     }
 }
