@@ -479,8 +479,11 @@ hipError_t grow_keep(DevBuf& b, size_t need, hipStream_t st) {
 
 // Sweep and prune on the GPU.  Leaves the oriented pairs, sorted by
 // (first, second), in *hp and the matching queries (relative matrices) in
-// d_queries.
-int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out, std::vector<uint2>* hp) {
+// d_queries.  With wait false the pairs are copied into h_pairs without
+// waiting for them; finish_broad() takes them, and the broad phase's time,
+// after a later synchronisation of the stream.
+int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out, std::vector<uint2>* hp,
+                bool wait = true) {
     if (!ents) return fail(ctx, CD_E_INVALID, "null entity array");
     if (n >= 0xFFFFu) return fail(ctx, CD_E_INVALID, "more entities than the 16-bit Entity type allows");
     for (uint32_t i = 0; i < n; ++i) {
@@ -531,14 +534,30 @@ int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out
                                      ctx->d_work[W_SPAIRS].as<uint2>(), st));
     }
     HCK(ctx, hipEventRecord(ctx->ev[7], st));
+    ctx->timings = cd_timings{};
+    *np_out = np;
+    if (!wait) {
+        HCK(ctx, ctx->h_pairs.ensure(size_t(np) * sizeof(uint2)));
+        if (np)
+            HCK(ctx, hipMemcpyAsync(ctx->h_pairs.p, ctx->d_work[W_SPAIRS].p, size_t(np) * sizeof(uint2),
+                                    hipMemcpyDeviceToHost, st));
+        return CD_OK;
+    }
     hp->assign(np, make_uint2(0, 0));
     if (np) HCK(ctx, hipMemcpyAsync(hp->data(), ctx->d_work[W_SPAIRS].p, size_t(np) * sizeof(uint2), hipMemcpyDeviceToHost, st));
     HCK(ctx, hipStreamSynchronize(st));
     float ms = 0.f;
     HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
-    ctx->timings = cd_timings{};
     ctx->timings.broad_ms = ms;
-    *np_out = np;
+    return CD_OK;
+}
+
+// The pairs and time of a broad_phase(..., wait = false).
+int finish_broad(cd_ctx* ctx, uint32_t np, std::vector<uint2>* hp, float* ms) {
+    HCK(ctx, hipEventSynchronize(ctx->ev[7]));
+    hp->resize(np);
+    if (np) std::memcpy(hp->data(), ctx->h_pairs.p, size_t(np) * sizeof(uint2));
+    HCK(ctx, hipEventElapsedTime(ms, ctx->ev[6], ctx->ev[7]));
     return CD_OK;
 }
 
@@ -600,6 +619,7 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->d_nq.release();
     ctx->h_nq.release();
     ctx->h_stage.release();
+    ctx->h_pairs.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1103,11 +1123,18 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
     std::vector<uint2> hp;
     ctx->last_run = 0;
     ctx->rays_ready = false;
-    int rc = broad_phase(ctx, entities, n, &np, &hp);
+    // The plain mode checks the pairs' tree depths first; the batched mode
+    // takes the pairs after its run, whose count read-back waits for them.
+    const bool defer = mode != CD_MODE_PLAIN;
+    int rc = broad_phase(ctx, entities, n, &np, &hp, !defer);
     if (rc) return rc;
     ctx->frame_mesh.resize(n);
     for (uint32_t i = 0; i < n; ++i) ctx->frame_mesh[i] = entities[i].mesh_id;
-    const float broad_ms = ctx->timings.broad_ms;
+    float broad_ms = ctx->timings.broad_ms;
+    if (np == 0 && defer) {
+        rc = finish_broad(ctx, np, &hp, &broad_ms);
+        if (rc) return rc;
+    }
     if (np == 0) {
         ctx->counters = cd_counters{};
         ctx->timings = cd_timings{};
@@ -1128,6 +1155,10 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
     uint64_t nc = 0;
     rc = run_queries(ctx, np, mode, &nc);
     if (rc) return rc;
+    if (defer) {
+        rc = finish_broad(ctx, np, &hp, &broad_ms);
+        if (rc) return rc;
+    }
     ctx->counters.broad_pairs = np;
     ctx->timings.broad_ms = broad_ms;
     ctx->timings.total_ms += broad_ms;

@@ -243,6 +243,7 @@ struct cd_ctx {
     cdk::DevBuf d_budget;  // node-pair tests of the last batched run (budgeted traversal pass)
     cdk::DevBuf d_nq;      // query count of the batched run, read by its kernels (graph replays keep it live)
     cdk::HostBuf h_nq;     // pinned staging of that count
+    cdk::HostBuf h_pairs;  // pinned staging of a frame's broad-phase pairs
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
     uint64_t staged_contacts = 0;
