@@ -52,7 +52,9 @@ def config2(args):
         frames.append(sc.entities(ids, ENTITY_DTYPE))
         sc.step()
     out = {}
-    for name, mode in (("plain", MODE_PLAIN), ("batched", MODE_BATCHED)):
+    # --plain-steps 0 leaves the plain mode out (profiles of the batched frame)
+    modes = (("plain", MODE_PLAIN),) * (args.plain_steps > 0) + (("batched", MODE_BATCHED),)
+    for name, mode in modes:
         for f in frames[: args.warmup]:
             gpu.frame(f, mode)
         t = time.perf_counter()

@@ -8,7 +8,8 @@
 //                 the digit bits, across wavefronts from an LDS table
 //                 (up to RS_SMALL keys)
 //  k_rank_sort    the small sorts (up to RANK_MAX keys): every key ranked
-//                 against all others in LDS, same stable order
+//                 against all others in LDS, same stable order, 64 keys
+//                 per workgroup
 //  k_rs_*         the same sort over many workgroups for longer arrays
 //                 (contacts): 8-bit digits, per-tile digit counts, one
 //                 scan of the count table, stable scatter
@@ -16,7 +17,9 @@
 //                 through LDS a tile at a time: every later entity whose
 //                 U.min <= this U.max, kept when V and W overlap and either
 //                 entity wants callbacks (SweepAndPrune.cpp:50-85); the pair
-//                 is oriented (earlier, later) as the reference orients it
+//                 is oriented (earlier, later) as the reference orients it;
+//                 pairs are staged per wavefront in LDS, one atomic per
+//                 flush (a returning atomic per step serialised the sweep)
 //  k_pair_queries relative matrix inverse(first) * second per pair
 //                 (OBBtreesCollision.cpp:15) with glm's formulas
 //
@@ -114,31 +117,55 @@ __global__ __launch_bounds__(1024) void k_radix_sort(uint32_t* keys, uint32_t* v
     }
 }
 
-// Small sorts (the broad phase's keys and pairs of a scene): one workgroup
-// ranks every key against all others in LDS, rank = (keys below) + (equal
-// keys before it), the stable order an LSD radix sort over the same bits
-// gives, with no passes and no scans.
+// Small sorts (the broad phase's keys and pairs of a scene): every key is
+// ranked against all others in LDS, rank = (keys below) + (equal keys before
+// it), the stable order an LSD radix sort over the same bits gives, with no
+// passes and no scans.  A workgroup ranks RANK_KEYS keys, one wavefront per
+// quarter of the others (the keys it compares against are read by every
+// lane at once), and scatters them into the second buffers; k_copy_back
+// returns them.  One workgroup ranking everything took ~40 us for 512 keys.
 constexpr uint32_t RANK_MAX = 2048;
+constexpr uint32_t RANK_KEYS = 64;
+constexpr uint32_t RANK_SPLIT = 4;
 
-__global__ __launch_bounds__(1024) void k_rank_sort(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t mask) {
-    __shared__ uint32_t s_k[RANK_MAX], s_key[RANK_MAX], s_v[RANK_MAX];
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t k = keys[i];
-        s_key[i] = k;
-        s_k[i] = k & mask;
-        s_v[i] = vals[i];
-    }
+__global__ __launch_bounds__(RANK_KEYS * RANK_SPLIT) void k_rank_sort(const uint32_t* __restrict__ keys,
+                                                                      const uint32_t* __restrict__ vals, uint32_t n,
+                                                                      uint32_t mask, uint32_t* __restrict__ keys_out,
+                                                                      uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t s_k[RANK_MAX];
+    __shared__ uint32_t s_part[RANK_SPLIT][RANK_KEYS];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_k[i] = keys[i] & mask;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t kl = threadIdx.x % RANK_KEYS, part = threadIdx.x / RANK_KEYS;
+    const uint32_t i = blockIdx.x * RANK_KEYS + kl;
+    const uint32_t chunk = (n + RANK_SPLIT - 1) / RANK_SPLIT;
+    const uint32_t j0 = min(n, part * chunk), j1 = min(n, j0 + chunk);
+    uint32_t r = 0;
+    if (i < n) {
         const uint32_t k = s_k[i];
-        uint32_t r = 0;
 #pragma unroll 8
-        for (uint32_t j = 0; j < n; ++j) {
+        for (uint32_t j = j0; j < j1; ++j) {
             const uint32_t kj = s_k[j];
             r += (kj < k || (kj == k && j < i)) ? 1u : 0u;
         }
-        keys[r] = s_key[i];
-        vals[r] = s_v[i];
+    }
+    s_part[part][kl] = r;
+    __syncthreads();
+    if (part == 0 && i < n) {
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t s = 0; s < RANK_SPLIT; ++s) rank += s_part[s][kl];
+        keys_out[rank] = keys[i];
+        vals_out[rank] = vals[i];
+    }
+}
+
+__global__ void k_copy_back(const uint32_t* __restrict__ ka, const uint32_t* __restrict__ va, uint32_t n,
+                            uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kb[i] = ka[i];
+        vb[i] = va[i];
     }
 }
 
@@ -220,21 +247,38 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__
 }
 
 constexpr uint32_t SAP_TILE = 256;
+constexpr uint32_t SAP_STAGE = 256;  // pairs a wavefront stages in LDS before one atomic reserves their slots
+
+// Writes a wavefront's staged pairs.  Every lane of the wavefront calls it.
+__device__ __forceinline__ void sap_flush(const uint2* stage, uint32_t n, uint2* __restrict__ pairs, uint32_t cap,
+                                          unsigned int* __restrict__ count, uint32_t lane) {
+    __threadfence_block();  // the wavefront's LDS writes are complete before other lanes read them
+    unsigned int base = 0;
+    if (lane == 0) base = atomicAdd(count, n);
+    base = __shfl(base, 0, 64);
+    for (uint32_t k = lane; k < n; k += 64)
+        if (base + k < cap) pairs[base + k] = stage[k];
+    __threadfence_block();  // read before the next pairs overwrite the stage
+}
 
 __global__ __launch_bounds__(256) void k_sap_sweep(const uint32_t* __restrict__ order, const float* __restrict__ iv,
                                                    const cd_entity* __restrict__ ents, uint32_t n,
                                                    uint2* __restrict__ pairs, uint32_t cap,
                                                    unsigned int* __restrict__ count) {
     // the later entities stream through LDS one tile of the U order at a
-    // time, until every lane of the workgroup has met a U.min above its U.max
+    // time, until every lane of the workgroup has met a U.min above its U.max;
+    // the pairs found are staged per wavefront (their order is free: the
+    // caller sorts them)
     __shared__ float s_iv[SAP_TILE * 6];
     __shared__ uint32_t s_id[SAP_TILE];
     __shared__ uint32_t s_cb[SAP_TILE];
+    __shared__ uint2 s_stage[SAP_TILE / 64][SAP_STAGE];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
+    uint2* stage = s_stage[tid >> 6];
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t first = blockIdx.x * SAP_TILE;
     const uint32_t p = first + tid;
-    uint32_t e = 0;
+    uint32_t e = 0, staged = 0;  // staged: the same in every lane of a wavefront
     float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool cbe = false, done = p >= n;
     if (!done) {
@@ -255,31 +299,38 @@ __global__ __launch_bounds__(256) void k_sap_sweep(const uint32_t* __restrict__ 
         }
         __syncthreads();
         const uint32_t tend = min(t0 + SAP_TILE, n);
-        if (!done) {
-            for (; q < tend; ++q) {
+        // every lane of the wavefront runs the loop while one of them is live
+        for (;;) {
+            const bool live = !done && q < tend;
+            if (__ballot(live) == 0ull) break;
+            bool hit = false;
+            uint32_t other = 0;
+            if (live) {
                 const float* b = s_iv + (q - t0) * 6;
                 if (a[1] < b[0]) {  // retired: this U.max is below every later U.min (SweepAndPrune.cpp:58-59)
                     done = true;
-                    break;
-                }
-                const bool hit = (cbe || s_cb[q - t0] != 0u) && (a[3] >= b[2]) && (b[3] >= a[2]) && (a[5] >= b[4]) &&
-                                 (b[5] >= a[4]);
-                const unsigned long long m = __ballot(hit);  // one atomic per wavefront
-                if (m) {
-                    const int leader = __ffsll(static_cast<long long>(m)) - 1;
-                    unsigned int base = 0;
-                    if (int(lane) == leader) base = atomicAdd(count, unsigned(__popcll(m)));
-                    base = __shfl(base, leader, 64);
-                    if (hit) {
-                        const unsigned int sl = base + unsigned(__popcll(m & below));
-                        if (sl < cap) pairs[sl] = make_uint2(e, s_id[q - t0]);
-                    }
+                } else {
+                    hit = (cbe || s_cb[q - t0] != 0u) && (a[3] >= b[2]) && (b[3] >= a[2]) && (a[5] >= b[4]) &&
+                          (b[5] >= a[4]);
+                    other = s_id[q - t0];
+                    ++q;
                 }
             }
-            if (q >= n) done = true;
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                const uint32_t nh = uint32_t(__popcll(m));
+                if (staged + nh > SAP_STAGE) {
+                    sap_flush(stage, staged, pairs, cap, count, lane);
+                    staged = 0;
+                }
+                if (hit) stage[staged + uint32_t(__popcll(m & below))] = make_uint2(e, other);
+                staged += nh;
+            }
         }
+        if (q >= n) done = true;
         if (__syncthreads_or(done ? 0 : 1) == 0) break;  // also keeps the tile until every lane is through it
     }
+    if (staged) sap_flush(stage, staged, pairs, cap, count, lane);
 }
 
 __global__ void k_pair_keys(const uint2* __restrict__ pairs, uint32_t n, uint32_t* __restrict__ keys,
@@ -349,7 +400,9 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
     if (n < 2) return hipSuccess;
     if (n <= RANK_MAX) {
         const uint32_t mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << ((key_bits + 3) / 4 * 4)) - 1u);
-        hipLaunchKernelGGL(k_rank_sort, dim3(1), dim3(1024), 0, st, keys, vals, n, mask);
+        hipLaunchKernelGGL(k_rank_sort, dim3((n + RANK_KEYS - 1) / RANK_KEYS), dim3(RANK_KEYS * RANK_SPLIT), 0, st, keys,
+                           vals, n, mask, keys_tmp, vals_tmp);
+        hipLaunchKernelGGL(k_copy_back, g256(n), dim3(256), 0, st, keys_tmp, vals_tmp, n, keys, vals);
         return hipGetLastError();
     }
     if (n <= RS_SMALL || !table) {
