@@ -184,26 +184,59 @@ __global__ __launch_bounds__(256) void k_rs_count(const uint32_t* __restrict__ k
     table[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the m-entry count table in place, one workgroup.
+// Exclusive scan of the m-entry count table in place, one workgroup: each
+// thread sums a run of the table (up to RS_SCAN_REGS entries loaded at once
+// and kept in registers), the run sums are scanned within each wavefront by
+// shuffles and across the 16 wavefronts through LDS.
+constexpr uint32_t RS_SCAN_REGS = 16;
+
 __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ table, uint32_t m) {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t per = (m + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per, hi = min(lo + per, m);
+    const uint32_t lo = min(m, tid * per), hi = min(lo + per, m);
+    uint32_t v[RS_SCAN_REGS];
     uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += table[i];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the partial sums
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    if (per <= RS_SCAN_REGS) {
+#pragma unroll
+        for (uint32_t k = 0; k < RS_SCAN_REGS; ++k) v[k] = lo + k < hi ? table[lo + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < RS_SCAN_REGS; ++k) sum += v[k];
+    } else {
+        for (uint32_t i = lo; i < hi; ++i) sum += table[i];
     }
-    uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t c = table[i];
-        table[i] = run;
-        run += c;
+    uint32_t x = sum;  // inclusive scan of the run sums within the wavefront
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+        for (uint32_t off = 1; off < 16; off <<= 1) {
+            const uint32_t y = __shfl_up(w, off, 64);
+            if (lane >= off) w += y;
+        }
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    uint32_t run = x - sum + (wave ? wsum[wave - 1] : 0u);
+    if (per <= RS_SCAN_REGS) {
+#pragma unroll
+        for (uint32_t k = 0; k < RS_SCAN_REGS; ++k)
+            if (lo + k < hi) {
+                table[lo + k] = run;
+                run += v[k];
+            }
+    } else {
+        for (uint32_t i = lo; i < hi; ++i) {
+            const uint32_t c = table[i];
+            table[i] = run;
+            run += c;
+        }
     }
 }
 
