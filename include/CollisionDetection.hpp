@@ -21,6 +21,7 @@
 // itself never throws).
 #pragma once
 
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <iterator>
@@ -111,38 +112,40 @@ public:
     void ExecuteCollisionDetection() {  // :38-67, phases 1-3
         results_.clear();
         if (entries_.size() < 2) return;
+        // The output buffers keep the largest sizes seen (they never shrink):
+        // a frame that fits them takes one call; a larger one is called again.
         uint32_t np = 0;
         uint64_t nc = 0;
-        int rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, nullptr, 0, &np, nullptr, 0, &nc);
+        int rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, pairs_.data(), uint32_t(pairs_.size()),
+                          &np, contacts_.data(), contacts_.size(), &nc);
         if (rc == CD_E_CAPACITY) {
-            pairs_.resize(np);
-            contacts_.resize(nc);
-            rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, pairs_.data(), np, &np,
-                          contacts_.data(), nc, &nc);
+            pairs_.resize(std::max<size_t>(pairs_.size(), np));
+            contacts_.resize(std::max<size_t>(contacts_.size(), nc));
+            rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, pairs_.data(), uint32_t(pairs_.size()),
+                          &np, contacts_.data(), contacts_.size(), &nc);
         }
         check(rc, "cd_frame");
-        pairs_.resize(np);
-        contacts_.resize(nc);
         // phase 3 (CreateUncollideRays), on the contacts the frame left on the device
         uint32_t nrp = 0;
         uint64_t nr = 0;
-        rc = cd_uncollide_rays(ctx_, nullptr, 0, &nrp, nullptr, 0, &nr);
+        rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), uint32_t(ray_pairs_.size()), &nrp, rays_.data(), rays_.size(), &nr);
         if (rc == CD_E_CAPACITY) {
-            ray_pairs_.resize(nrp);
-            rays_.resize(nr);
-            rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), nrp, &nrp, rays_.data(), nr, &nr);
+            ray_pairs_.resize(std::max<size_t>(ray_pairs_.size(), nrp));
+            rays_.resize(std::max<size_t>(rays_.size(), nr));
+            rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), uint32_t(ray_pairs_.size()), &nrp, rays_.data(), rays_.size(),
+                                   &nr);
         }
         check(rc, "cd_uncollide_rays");
-        ray_pairs_.resize(nrp);
         size_t k = 0;  // both lists follow the frame's pair order
-        for (const cd_pair_rays& rp : ray_pairs_) {
-            while (k < pairs_.size() && (pairs_[k].first != rp.first || pairs_[k].second != rp.second)) ++k;
+        for (uint32_t i = 0; i < nrp; ++i) {
+            const cd_pair_rays& rp = ray_pairs_[i];
+            while (k < np && (pairs_[k].first != rp.first || pairs_[k].second != rp.second)) ++k;
             CollidingPair r;
             r.firstIndex = rp.first;
             r.secondIndex = rp.second;
             r.first = entries_[rp.first].entity;
             r.second = entries_[rp.second].entity;
-            if (k < pairs_.size())
+            if (k < np)
                 r.contacts.assign(contacts_.begin() + pairs_[k].contact_begin,
                                   contacts_.begin() + pairs_[k].contact_begin + pairs_[k].contact_count);
             r.rays_from_first_to_second.assign(rays_.begin() + rp.begin_first, rays_.begin() + rp.begin_first + rp.n_first);

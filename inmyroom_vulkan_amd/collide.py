@@ -257,6 +257,7 @@ class Collider:
             raise CollideError(rc, "cd_create failed (no HIP device?)")
         self.h = h
         self.leaf_max = leaf_max
+        self._caps = {}  # output capacities of the last calls: a frame like the last needs no second call
 
     def close(self):
         if getattr(self, "h", None):
@@ -412,7 +413,7 @@ class Collider:
 
     def frame(self, entities: np.ndarray, mode: int = MODE_PLAIN):
         ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
-        pcap, ccap = 1 << 12, 1 << 16
+        pcap, ccap = self._caps.get("frame", (1 << 12, 1 << 16))
         while True:
             pairs = np.zeros(pcap, PAIR_DTYPE)
             cons = np.zeros(ccap, CONTACT_DTYPE)
@@ -424,11 +425,12 @@ class Collider:
                 ccap = max(ccap, int(ncons.value))
                 continue
             self._check(rc, "cd_frame")
+            self._caps["frame"] = (pcap, ccap)
             return pairs[: npairs.value], cons[: ncons.value]
 
     def uncollide_rays(self):
         """Rays of the last run's contacts: (pairs [PAIR_RAYS_DTYPE], rays [RAY_DTYPE])."""
-        pcap, rcap = 1 << 10, 1 << 14
+        pcap, rcap = self._caps.get("rays", (1 << 10, 1 << 14))
         while True:
             pairs = np.zeros(pcap, PAIR_RAYS_DTYPE)
             rays = np.zeros(rcap, RAY_DTYPE)
@@ -440,6 +442,7 @@ class Collider:
                 rcap = max(rcap, int(nrays.value))
                 continue
             self._check(rc, "cd_uncollide_rays")
+            self._caps["rays"] = (pcap, rcap)
             return pairs[: npairs.value], rays[: nrays.value]
 
     def shoot_rays(self, start_rads=None, finish_rads=None, bias=None):

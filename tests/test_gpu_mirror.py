@@ -51,6 +51,16 @@ int main() {
         for (const auto& c : kv.second)
             std::printf("callback %d family %d other %d %.6f %.6f %.6f\n", kv.first, c.familyEntity,
                         c.collideWithEntity, c.deltaVector[0], c.deltaVector[1], c.deltaVector[2]);
+    // output buffers kept across frames: a frame without pairs, then the first frame again
+    for (int again = 0; again < 2; ++again) {
+        cd.Reset();
+        cd.AddCollisionDetectionEntry(entry(5, 0.f, 0.f, 0.f, 0.f));
+        cd.AddCollisionDetectionEntry(again ? entry(7, 1.5f, 2.5f, 0.25f, 0.35f) : entry(7, 9.f, 10.f, 0.25f, 0.35f));
+        cd.ExecuteCollisionDetection();
+        for (const auto& r : cd.GetCollidingPairs())
+            std::printf("again%d %d %d shot %d delta %.6f %.6f %.6f\n", again, r.first, r.second, int(r.shot),
+                        r.delta[0], r.delta[1], r.delta[2]);
+    }
     return 0;
 }
 '''
@@ -106,6 +116,9 @@ def test_mirror_frame_to_callbacks(tmp_path, orc):
     assert (c5[3], c5[5]) == ("5", "7") and (c7[3], c7[5]) == ("7", "5")
     assert all(abs(float(v)) == 0.0 for v in c5[6:9])  # the static cube takes no share
     assert [float(v) for v in c7[6:9]] == delta  # the moving cube takes all of it
+    assert not [l for l in lines if l.startswith("again0")]  # no pairs
+    again = [l.split() for l in lines if l.startswith("again1")]
+    assert len(again) == 1 and again[0][1:] == pairs[0][1:]  # the first frame again, through the kept buffers
 
 
 GLTF_PROGRAM = r'''
