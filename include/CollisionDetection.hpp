@@ -112,15 +112,16 @@ public:
     void ExecuteCollisionDetection() {  // :38-67, phases 1-3
         results_.clear();
         if (entries_.size() < 2) return;
-        // The output buffers keep the largest sizes seen (they never shrink):
-        // a frame that fits them takes one call; a larger one is called again.
+        // The output buffers keep the largest sizes seen, with a quarter more
+        // room (they never shrink): a frame that fits them takes one call; a
+        // larger one is called again.
         uint32_t np = 0;
         uint64_t nc = 0;
         int rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, pairs_.data(), uint32_t(pairs_.size()),
                           &np, contacts_.data(), contacts_.size(), &nc);
         if (rc == CD_E_CAPACITY) {
-            pairs_.resize(std::max<size_t>(pairs_.size(), np));
-            contacts_.resize(std::max<size_t>(contacts_.size(), nc));
+            pairs_.resize(std::max<size_t>(pairs_.size(), np + np / 4));
+            contacts_.resize(std::max<size_t>(contacts_.size(), nc + nc / 4));
             rc = cd_frame(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, pairs_.data(), uint32_t(pairs_.size()),
                           &np, contacts_.data(), contacts_.size(), &nc);
         }
@@ -130,8 +131,8 @@ public:
         uint64_t nr = 0;
         rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), uint32_t(ray_pairs_.size()), &nrp, rays_.data(), rays_.size(), &nr);
         if (rc == CD_E_CAPACITY) {
-            ray_pairs_.resize(std::max<size_t>(ray_pairs_.size(), nrp));
-            rays_.resize(std::max<size_t>(rays_.size(), nr));
+            ray_pairs_.resize(std::max<size_t>(ray_pairs_.size(), nrp + nrp / 4));
+            rays_.resize(std::max<size_t>(rays_.size(), nr + nr / 4));
             rc = cd_uncollide_rays(ctx_, ray_pairs_.data(), uint32_t(ray_pairs_.size()), &nrp, rays_.data(), rays_.size(),
                                    &nr);
         }

@@ -245,6 +245,12 @@ class GltfDocument:
         return out
 
 
+def _grown(n: int) -> int:
+    """A capacity for n records with a quarter more room, so that the next
+    slightly larger frame does not call again."""
+    return int(n) + int(n) // 4
+
+
 class Collider:
     """One library context (cd_ctx) on one GPU."""
 
@@ -421,8 +427,8 @@ class Collider:
             rc = self.lib.cd_frame(self.h, _ptr(ents), len(ents), mode, _ptr(pairs), pcap, C.byref(npairs),
                                    _ptr(cons), ccap, C.byref(ncons))
             if rc == CD_E_CAPACITY:
-                pcap = max(pcap, int(npairs.value))
-                ccap = max(ccap, int(ncons.value))
+                pcap = max(pcap, _grown(npairs.value))
+                ccap = max(ccap, _grown(ncons.value))
                 continue
             self._check(rc, "cd_frame")
             self._caps["frame"] = (pcap, ccap)
@@ -438,8 +444,8 @@ class Collider:
             rc = self.lib.cd_uncollide_rays(self.h, _ptr(pairs), pcap, C.byref(npairs), _ptr(rays), rcap,
                                             C.byref(nrays))
             if rc == CD_E_CAPACITY:
-                pcap = max(pcap, int(npairs.value))
-                rcap = max(rcap, int(nrays.value))
+                pcap = max(pcap, _grown(npairs.value))
+                rcap = max(rcap, _grown(nrays.value))
                 continue
             self._check(rc, "cd_uncollide_rays")
             self._caps["rays"] = (pcap, rcap)
