@@ -27,6 +27,29 @@ def test_broad_phase_matches_oracle(gpu, orc):
         sc.step()
 
 
+def test_broad_phase_crowded_matches_oracle(gpu, orc):
+    """240 unit boxes in a 2 m cube: an entity overlaps up to ~200 later ones
+    along U, so the sweep tests them in several chunks of 64."""
+    case("broad phase, 240 boxes in a 2 m cube")
+    box = scenes.box_soup()
+    mid = gpu.register_soups([box])[0]
+    tree = orc.tree(box, 4)
+    rng = np.random.default_rng(5)
+    n = 240
+    ents = np.zeros(n, ENTITY_DTYPE)
+    ents["current"] = ents["previous"] = scenes.pose_matrices(rng.uniform(-1, 1, (n, 3)), scenes.random_quaternions(rng, n))
+    ents["mesh_id"] = mid
+    ents["entity"] = np.arange(n)
+    ents["should_callback"] = 1
+    ents["should_callback"][::5] = 0
+    got = [tuple(p) for p in gpu.broad_phase(ents).tolist()]
+    res = orc.frame([tree], np.zeros(n, np.uint32), ents["current"], ents["should_callback"], mode=0)
+    want = [tuple(p) for p in res.pairs.tolist()]
+    assert got == want
+    later = np.bincount(np.array(want)[:, 0], minlength=n)
+    assert later.max() > 128  # some entity's pairs span three chunks or more
+
+
 def test_frame_plain_matches_oracle(gpu, orc):
     sc = _scene(200, 22, cube=18.0)
     case("frame plain n=200")
