@@ -1351,9 +1351,13 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     if (npairs == 0) return CD_OK;
     if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
     for (uint32_t q = 0; q < nq; ++q)  // the closest-hit stack holds a tree's depth + 2 entries
-        for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y})
-            if (ctx->meshes[ctx->frame_mesh[e]].depth + 2 > uint32_t(RAY_STACK_ENTRIES))
+        for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y}) {
+            const MeshRec& m = ctx->meshes[ctx->frame_mesh[e]];
+            if (m.depth + 2 > uint32_t(RAY_STACK_ENTRIES))
                 return fail(ctx, CD_E_STACK, "tree too deep for the ray traversal stack");
+            if (m.ntri >= (1u << RAY_LINK_BITS) || ctx->leaf_max > 64)
+                return fail(ctx, CD_E_CAPACITY, "mesh too large for the ray traversal stack (2^25 triangles)");
+        }
     // radians and cosines as the reference computes them (glm::radians, std::cos on float)
     const float deg = 0.01745329251994329576923690768489f;
     const float a0 = params ? params->smoothstep_start_rads : 40.f * deg;

@@ -84,6 +84,7 @@ struct RayShot {
     uint32_t pair;  // query (entity pair) of the ray
 };
 constexpr int RAY_STACK_ENTRIES = 48;  // closest-hit traversal stack per lane (LDS): a tree's depth + 2
+constexpr uint32_t RAY_LINK_BITS = 25;  // a stack entry packs a node or triangle index below 2^25 with a leaf count
 
 // A refit plan: small boxes (one thread each) and large boxes split into
 // chunks of triangles (device arrays, cached per mesh list).

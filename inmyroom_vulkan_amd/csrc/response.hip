@@ -149,14 +149,15 @@ __device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, cons
 // cnt triangles from `link`; t0 its entry distance.  A child whose exit
 // distance is negative is never pushed (that check does not depend on the
 // best hit, so making it at push time is the reference's check).
+// link and cnt share one word (link < 2^RAY_LINK_BITS, cnt <= 64; the host
+// checks the meshes): 8 bytes an entry, so six wavefronts fit a CU's LDS
+// where 12-byte entries allowed four.
 struct RayStack {
-    uint32_t* link;
-    uint32_t* cnt;
+    uint32_t* lc;
     float* t0;
     int lane;
     __device__ __forceinline__ void put(int i, uint32_t l, uint32_t c, float t) {
-        link[i * 64 + lane] = l;
-        cnt[i * 64 + lane] = c;
+        lc[i * 64 + lane] = l | (c << RAY_LINK_BITS);
         t0[i * 64 + lane] = t;
     }
 };
@@ -207,7 +208,8 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
         bool found = false;
         while (sp > 0) {
             --sp;
-            const uint32_t el = S.link[sp * 64 + S.lane], ec = S.cnt[sp * 64 + S.lane];
+            const uint32_t ew = S.lc[sp * 64 + S.lane];
+            const uint32_t el = ew & ((1u << RAY_LINK_BITS) - 1u), ec = ew >> RAY_LINK_BITS;
             const float et = S.t0[sp * 64 + S.lane];
             if (!(et < best.dist)) continue;
             if (ec == 0u) {
@@ -319,9 +321,9 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
                                               RayShot* __restrict__ out) {
-    __shared__ uint32_t s_link[RAY_STACK * 64], s_cnt[RAY_STACK * 64];
+    __shared__ uint32_t s_lc[RAY_STACK * 64];
     __shared__ float s_t0[RAY_STACK * 64];
-    RayStack S = {s_link, s_cnt, s_t0, int(threadIdx.x)};
+    RayStack S = {s_lc, s_t0, int(threadIdx.x)};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
     if (r >= nrays) return;  // both lanes of a ray leave together
     const int side = r >= base1 ? 1 : 0;
