@@ -369,7 +369,10 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
  * (a fork of glm that is not available) is taken as the negative-determinant
  * branch of glm 0.9.9's test.  The callbacks' fan-out to entity ancestors
  * (:101-121) needs the engine's entity tree and is left to the caller.
- * Params NULL: the reference's values.  CD_E_CAPACITY writes the count back. */
+ * Params NULL: the reference's values.  CD_E_CAPACITY writes the count back;
+ * it is also returned, with no count, for a pair whose mesh has 2^25
+ * triangles or more (the traversal stack packs an index and a leaf count
+ * into one word), and CD_E_STACK for a tree deeper than 46 levels. */
 int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* out, uint32_t cap,
                   uint32_t* n_out);
 
