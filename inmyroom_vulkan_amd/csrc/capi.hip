@@ -1350,11 +1350,13 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     *n_out = npairs;
     if (npairs == 0) return CD_OK;
     if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
-    for (uint32_t q = 0; q < nq; ++q)  // the closest-hit stack holds a tree's depth + 2 entries
+    uint32_t stack_entries = 2;  // the closest-hit stack holds a tree's depth + 2 entries
+    for (uint32_t q = 0; q < nq; ++q)
         for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y}) {
             const MeshRec& m = ctx->meshes[ctx->frame_mesh[e]];
             if (m.depth + 2 > uint32_t(RAY_STACK_ENTRIES))
                 return fail(ctx, CD_E_STACK, "tree too deep for the ray traversal stack");
+            stack_entries = std::max(stack_entries, m.depth + 2);
             if (m.ntri >= (1u << RAY_LINK_BITS) || ctx->leaf_max > 64)
                 return fail(ctx, CD_E_CAPACITY, "mesh too large for the ray traversal stack (2^25 triangles)");
         }
@@ -1387,7 +1389,8 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
                           ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
                           ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
-                          ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(), st));
+                          ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
+                          stack_entries, st));
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
                             ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
                             ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,

@@ -156,6 +156,7 @@ struct RayStack {
     uint32_t* lc;
     float* t0;
     int lane;
+    int cap;  // entries per lane
     __device__ __forceinline__ void put(int i, uint32_t l, uint32_t c, float t) {
         lc[i * 64 + lane] = l | (c << RAY_LINK_BITS);
         t0[i * 64 + lane] = t;
@@ -184,7 +185,7 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
         float l0, l1, q0, q1;
         const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1) && l1 >= 0.f;
         const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1) && q1 >= 0.f;
-        if (sp + 2 > RAY_STACK) {  // deeper than the host allows (it checks tree depths first)
+        if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
             best.hit = false;
             break;
         }
@@ -320,10 +321,11 @@ __device__ __forceinline__ f3 get3(const float* o) { return mk(o[0], o[1], o[2])
 __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, uint32_t nrays, uint32_t base1,
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
-                                              RayShot* __restrict__ out) {
-    __shared__ uint32_t s_lc[RAY_STACK * 64];
-    __shared__ float s_t0[RAY_STACK * 64];
-    RayStack S = {s_lc, s_t0, int(threadIdx.x)};
+                                              RayShot* __restrict__ out, int stack_entries) {
+    // the stacks are sized by the launch: the deepest tree of the call + 2
+    // entries per lane, so shallow trees leave LDS for more wavefronts
+    extern __shared__ uint32_t s_stack[];
+    RayStack S = {s_stack, reinterpret_cast<float*>(s_stack + stack_entries * 64), int(threadIdx.x), stack_entries};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
     if (r >= nrays) return;  // both lanes of a ray leave together
     const int side = r >= base1 ? 1 : 0;
@@ -463,12 +465,14 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        hipStream_t st) {
+                        uint32_t stack_entries, hipStream_t st) {
     if (!nrays) return hipSuccess;
+    if (stack_entries < 2 || stack_entries > uint32_t(RAY_STACK)) return hipErrorInvalidValue;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
     const uint64_t lanes = 2ull * nrays;
-    hipLaunchKernelGGL(k_shoot, dim3(uint32_t((lanes + 63) / 64)), dim3(64), 0, st, rays, nrays, base1, pbegin, nq,
-                       qs, V, shots);
+    const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
+    hipLaunchKernelGGL(k_shoot, dim3(uint32_t((lanes + 63) / 64)), dim3(64), lds, st, rays, nrays, base1, pbegin, nq,
+                       qs, V, shots, int(stack_entries));
     return hipGetLastError();
 }
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
