@@ -383,17 +383,12 @@ hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* 
 }
 uint64_t radix_table_words(uint32_t n) { return uint64_t(256) * ((uint64_t(n) + RS_TILE - 1) / RS_TILE); }
 
-uint32_t key_bits_below(uint64_t bound) {
-    uint32_t b = 1;
-    while (b < 32 && (uint64_t(1) << b) < bound) ++b;
-    return b;
-}
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                              uint32_t key_bits, uint32_t* table, hipStream_t st) {
     if (n < 2) return hipSuccess;
     if (n <= RANK_MAX) {
-        const uint32_t mask = key_bits >= 32 ? 0xFFFFFFFFu : ((1u << ((key_bits + 3) / 4 * 4)) - 1u);
+        const uint32_t mask = rank_sort_mask(key_bits);
         hipLaunchKernelGGL(k_rank_sort, dim3((n + RANK_KEYS - 1) / RANK_KEYS), dim3(RANK_KEYS * RANK_SPLIT), 0, st, keys,
                            vals, n, mask, keys_tmp, vals_tmp);
         hipLaunchKernelGGL(k_copy_back, g256(n), dim3(256), 0, st, keys_tmp, vals_tmp, n, keys, vals);

@@ -1386,11 +1386,12 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     HCK(ctx, hipMemcpyAsync(ctx->d_work[W_PIDS].p, up.data(), up.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     const uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();
     const uint32_t* pbegin = ctx->d_work[W_PIDS].as<uint32_t>() + 2 * size_t(nq);
+    HCK(ctx, launch_zero(dev_counters(ctx), st));
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
                           ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
                           ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
                           ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
-                          stack_entries, st));
+                          stack_entries, &dev_counters(ctx)->err, st));
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
                             ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
                             ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,
@@ -1398,7 +1399,10 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     std::vector<cd_pair_response> all(nq);
     HCK(ctx, hipMemcpyAsync(all.data(), ctx->d_work[W_RESP].p, size_t(nq) * sizeof(cd_pair_response),
                             hipMemcpyDeviceToHost, st));
-    HCK(ctx, hipStreamSynchronize(st));
+    RunCounters h{};
+    int rc = read_counters(ctx, h);  // synchronises the stream
+    if (rc) return rc;
+    if (h.err) return fail(ctx, CD_E_STACK, "ray traversal stack overflow");
     uint32_t k = 0;
     for (uint32_t q = 0; q < nq; ++q)
         if (ctx->ray_counts[q] + ctx->ray_counts[size_t(nq) + q]) out[k++] = all[q];
@@ -1410,6 +1414,14 @@ int cd_broad_phase(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t*
     (void)hipSetDevice(ctx->device);
     *n_pairs = 0;
     if (n < 2) return CD_OK;
+    // the broad phase overwrites the staged queries (relative matrices) and
+    // the entity copy the last run's rays and response read: that run is gone
+    ctx->last_run = 0;
+    ctx->rays_ready = false;
+    ctx->contacts_sorted = false;
+    ctx->staged_nq = 0;
+    ctx->staged_contacts = 0;
+    ctx->staged_ids.clear();
     uint32_t np = 0;
     std::vector<uint2> hp;
     int rc = broad_phase(ctx, entities, n, &np, &hp);

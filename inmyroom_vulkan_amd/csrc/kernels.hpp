@@ -2,6 +2,7 @@
 #pragma once
 
 #include "internal.hpp"
+#include "sortbits.hpp"
 
 namespace cdk {
 
@@ -64,8 +65,6 @@ hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                              uint32_t key_bits, uint32_t* table, hipStream_t st);
 uint64_t radix_table_words(uint32_t n);
-// Bits that hold every value below `bound` (at least 1).
-uint32_t key_bits_below(uint64_t bound);
 hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
                             uint32_t cap, unsigned int* count, hipStream_t st);
 hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
@@ -84,11 +83,12 @@ hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mi
 
 // response.hip
 // pbegin: per side (nq each) the first ray of every pair, non-decreasing;
-// stack_entries: the deepest tree the rays meet + 2, at most RAY_STACK_ENTRIES
+// stack_entries: the deepest tree the rays meet + 2, at most RAY_STACK_ENTRIES;
+// *err gets bit 0 if a traversal stack would have overflowed
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, hipStream_t st);
+                        uint32_t stack_entries, unsigned long long* err, hipStream_t st);
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
                           float cos_finish, float bias, cd_pair_response* out, hipStream_t st);

@@ -164,8 +164,11 @@ struct RayStack {
 };
 
 // Closest hit of the ray (o, dir) on mesh m moved by M (Ray.cpp:135-236).
+// A stack that would overflow (the host sizes it from the tree depths, so
+// this is unreachable unless that sizing is wrong) sets bit 0 of *err and
+// ends the traversal; the host then fails the call with CD_E_STACK.
 __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
-                        f3 o, f3 dir, RayStack& S) {
+                        f3 o, f3 dir, RayStack& S, unsigned long long* err) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
@@ -186,6 +189,7 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
         const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1) && l1 >= 0.f;
         const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1) && q1 >= 0.f;
         if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
+            atomicOr(err, 1ull);
             best.hit = false;
             break;
         }
@@ -266,14 +270,14 @@ __device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
 }
 
 __device__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
-                        const M3& Bn, f3 o, f3 d, RayStack& S) {
+                        const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool own = (threadIdx.x & 1u) != 0;
     const f3 o2 = move_eps(o, d, 4.f);
     M4 C;  // one call with lane-selected arguments, so the two traversals run as one instruction stream
 #pragma unroll
     for (int e = 0; e < 16; ++e) C.m[e] = own ? Amat.m[e] : Bmat.m[e];
-    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S);
+    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err);
     const int even = int(threadIdx.x & ~1u);
     const Hit p2 = shfl_hit(h, even), p3 = shfl_hit(h, even + 1);
     if (!(p2.hit && p2.back)) return r;
@@ -321,7 +325,8 @@ __device__ __forceinline__ f3 get3(const float* o) { return mk(o[0], o[1], o[2])
 __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, uint32_t nrays, uint32_t base1,
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
-                                              RayShot* __restrict__ out, int stack_entries) {
+                                              RayShot* __restrict__ out, int stack_entries,
+                                              unsigned long long* err) {
     // the stacks are sized by the launch: the deepest tree of the call + 2
     // entries per lane, so shallow trees leave LDS for more wavefronts
     extern __shared__ uint32_t s_stack[];
@@ -356,8 +361,8 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         const bool f2s = (pass == 0) == (side == 0);
-        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S)
-                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S);
+        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S, err)
+                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S, err);
         if (!p.ok) break;
         const f3 F = force_response(p);
         put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
@@ -465,14 +470,14 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, hipStream_t st) {
+                        uint32_t stack_entries, unsigned long long* err, hipStream_t st) {
     if (!nrays) return hipSuccess;
     if (stack_entries < 2 || stack_entries > uint32_t(RAY_STACK)) return hipErrorInvalidValue;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
     const uint64_t lanes = 2ull * nrays;
     const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
     hipLaunchKernelGGL(k_shoot, dim3(uint32_t((lanes + 63) / 64)), dim3(64), lds, st, rays, nrays, base1, pbegin, nq,
-                       qs, V, shots, int(stack_entries));
+                       qs, V, shots, int(stack_entries), err);
     return hipGetLastError();
 }
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,

@@ -169,3 +169,78 @@ def test_frame_small_inputs(gpu):
     pairs, cons = gpu.frame(e, MODE_PLAIN)
     assert len(pairs) == 0 and len(cons) == 0
     assert len(gpu.broad_phase(e[:0])) == 0
+
+
+def _sparse_boxes(n, side, seed, mid):
+    rng = np.random.default_rng(seed)
+    ents = np.zeros(n, ENTITY_DTYPE)
+    ents["current"] = ents["previous"] = scenes.pose_matrices(rng.uniform(0, side, (n, 3)),
+                                                              scenes.random_quaternions(rng, n))
+    ents["mesh_id"] = mid
+    ents["entity"] = np.arange(n)
+    ents["should_callback"] = 1
+    return ents
+
+
+def test_broad_phase_over_4096_entities_keeps_pair_order(gpu, orc):
+    """4,500 unit boxes in a 100 m cube.  The pair sort's key (first << 16 |
+    second) needs 29 bits above 4,096 entities; with at most 2,048 pairs it
+    runs as the one-pass rank sort, whose digit mask must then cover all 32
+    bits.  Pairs come out in (first, second) order, equal to the oracle's,
+    and the same on a second run."""
+    case("broad phase, 4,500 sparse boxes (29-bit pair keys)")
+    box = scenes.box_soup()
+    mid = gpu.register_soups([box])[0]
+    tree = orc.tree(box, 4)
+    ents = _sparse_boxes(4500, 100.0, 31, mid)
+    got = [tuple(p) for p in gpu.broad_phase(ents).tolist()]
+    res = orc.frame([tree], np.zeros(len(ents), np.uint32), ents["current"], ents["should_callback"], mode=0)
+    want = [tuple(p) for p in res.pairs.tolist()]
+    assert 16 < len(want) <= 2048, len(want)
+    assert max(a for a, _ in want) >= 4096  # keys above 2^28 take part
+    assert got == want
+    assert [tuple(p) for p in gpu.broad_phase(ents).tolist()] == got
+
+
+def test_frame_over_4096_entities_matches_oracle(gpu, orc):
+    """The same sparse scene as one plain frame: pairs in (first, second)
+    order, contacts bit-exact against the oracle."""
+    case("frame plain, 4,500 sparse boxes")
+    box = scenes.box_soup()
+    mid = gpu.register_soups([box])[0]
+    tree = orc.tree(box, 4)
+    ents = _sparse_boxes(4500, 100.0, 32, mid)
+    pairs, cons = gpu.frame(ents, MODE_PLAIN)
+    res = orc.frame([tree], np.zeros(len(ents), np.uint32), ents["current"], ents["should_callback"], mode=0)
+    keys = [(int(p["first"]), int(p["second"])) for p in pairs]
+    assert keys == sorted(keys) and len(keys) > 0
+    want = {}
+    for c in res.contacts:
+        a, b = res.pairs[c["query"]]
+        want[(int(a), int(b), int(c["tri_a"]), int(c["tri_b"]))] = c
+    got = {}
+    for p in pairs:
+        for c in cons[p["contact_begin"]: p["contact_begin"] + p["contact_count"]]:
+            got[(int(p["first"]), int(p["second"]), int(c["tri_a"]), int(c["tri_b"]))] = c
+    assert set(got) == set(want)
+    for k, c in want.items():
+        assert got[k]["s"].tobytes() == c["s"].tobytes() and got[k]["t"].tobytes() == c["t"].tobytes()
+
+
+def test_broad_phase_ends_the_last_run(gpu):
+    """cd_broad_phase overwrites the relative matrices and entity copy the
+    last frame's rays and response would read: after it, both refuse."""
+    from inmyroom_vulkan_amd import CollideError
+
+    case("rays and response refused after a standalone broad phase")
+    sc = _scene(120, 25, cube=12.0)
+    ids = gpu.register_soups(sc.soups)
+    ents = sc.entities(ids, ENTITY_DTYPE)
+    pairs, cons = gpu.frame(ents, MODE_PLAIN)
+    assert len(cons) > 0
+    gpu.uncollide_rays()  # valid straight after the frame
+    sc.step()
+    gpu.broad_phase(sc.entities(ids, ENTITY_DTYPE))
+    for call in (gpu.uncollide_rays, gpu.shoot_rays):
+        with pytest.raises(CollideError):
+            call()
