@@ -317,6 +317,21 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
 int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode,
                     int32_t* flags, float* seg6);
 
+/* No counterpart in the reference.  cd_tritri_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): a9, b9, flags and seg6 (may be NULL) are
+ * DEVICE pointers into memory of this runtime the caller owns, and the call
+ * makes no allocation.  The exact (batched) mode's rerun lists and the run's
+ * counters live in a fixed device array of the library, one per GPU, of
+ * about 8 MiB (CD_NOALLOC_MAX_PAIRS pairs), which that GPU holds until the
+ * process ends; n above CD_NOALLOC_MAX_PAIRS (either mode) is rejected with
+ * CD_E_CAPACITY, and buffers shorter than n pairs with CD_E_INVALID.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * contexts or two threads at once) race.  Synchronous, like the rest.     */
+#define CD_NOALLOC_MAX_PAIRS (1u << 20)
+int cd_tritri_batch_noalloc(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode,
+                            int32_t* flags, float* seg6);
+
 /* Mesh-pair queries: OBBtreesCollision::ExecuteOBBtreesCollision
  * (src/CollisionDetection/OBBtreesCollision.cpp:7-18) followed by the
  * triangle loop of CreateUncollideRays::ExecuteCreateUncollideRays

@@ -34,8 +34,9 @@ EXPORTS = (
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
     "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays", "cd_gltf_open", "cd_gltf_close",
     "cd_gltf_mesh_count", "cd_gltf_mesh_info_get", "cd_gltf_primitive_get", "cd_register_gltf",
-    "cd_node_global_matrices", "cd_gltf_node_count", "cd_gltf_nodes",
+    "cd_node_global_matrices", "cd_gltf_node_count", "cd_gltf_nodes", "cd_tritri_batch_noalloc",
 )
+CD_NOALLOC_MAX_PAIRS = 1 << 20
 
 CONTACT_DTYPE = np.dtype([
     ("query", "<u4"), ("tri_a", "<u4"), ("tri_b", "<u4"), ("flags", "<u4"),
@@ -136,6 +137,7 @@ def load_library() -> C.CDLL:
     lib.cd_mesh_info_get.argtypes = [vp, u32, P(cd_mesh_info)]
     lib.cd_export_tree.argtypes = [vp, u32, vp, vp, vp, vp]
     lib.cd_tritri_batch.argtypes = [vp, vp, vp, u64, u32, vp, vp]
+    lib.cd_tritri_batch_noalloc.argtypes = [vp, vp, vp, u64, u32, vp, vp]
     lib.cd_query_batch.argtypes = [vp, vp, u32, u32, vp, u64, P(u64)]
     lib.cd_query_leafpairs.argtypes = [vp, vp, u32, vp, u64, P(u64)]
     lib.cd_frame.argtypes = [vp, vp, u32, u32, vp, u32, P(u32), vp, u64, P(u64)]

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -561,6 +562,51 @@ int finish_broad(cd_ctx* ctx, uint32_t np, std::vector<uint2>* hp, float* ms) {
     return CD_OK;
 }
 
+// ---- scratch of the allocation-free triangle-pair stream ----------------------
+// cd_tritri_batch_noalloc keeps its run counters (first 256 bytes) and the
+// exact mode's two rerun lists (64 + 2 n words) here.  Each GPU holds its own
+// copy of the array at its own address, resolved once per device at context
+// creation; in host code `scratch` names no device address, so only that
+// resolved address is ever handed to kernels or copies.
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_COUNTER_BYTES + (64 + 2 * size_t(CD_NOALLOC_MAX_PAIRS)) * sizeof(uint32_t);
+static_assert(sizeof(RunCounters) <= NOALLOC_COUNTER_BYTES, "counters fit their slot");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, which must be current.
+hipError_t resolve_scratch(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// p points into one device allocation of this runtime with at least `bytes`
+// bytes from p on.  A pointer that is not (host memory) fails the lookup,
+// whose error would otherwise stay as the thread's last error and be
+// reported by the next launch's hipGetLastError: it is cleared here.
+bool device_span(const void* p, size_t bytes) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess || !base) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+    return q >= b && q - b <= size && bytes <= size - (q - b);
+}
+
 }  // namespace
 
 extern "C" {
@@ -599,6 +645,13 @@ int cd_create(cd_ctx** out, const cd_config* cfg) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cu = prop.multiProcessorCount;
+    {  // the allocation-free entry point's scratch on this device (only that entry point needs it)
+        const hipError_t e = resolve_scratch(ctx->device, &ctx->scratch);
+        if (e != hipSuccess) {
+            ctx->scratch = nullptr;
+            ctx->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        }
+    }
     *out = ctx;
     return CD_OK;
 }
@@ -1041,6 +1094,54 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
     RunCounters h{};
     int rc = read_counters(ctx, h);
     if (rc) return rc;
+    ctx->counters = cd_counters{};
+    ctx->counters.tri_tests = n;
+    ctx->counters.fp64_fallbacks = h.fp64;
+    ctx->counters.exact_fallbacks = h.exact;
+    ctx->timings = cd_timings{};
+    ctx->timings.total_ms = ms;
+    ctx->timings.narrow_ms = ms;
+    ctx->timings.launches = mode == CD_MODE_PLAIN ? 1 : 4;
+    return CD_OK;
+}
+
+int cd_tritri_batch_noalloc(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode,
+                            int32_t* flags, float* seg6) {
+    if (!ctx) return CD_E_INVALID;
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (n > CD_NOALLOC_MAX_PAIRS)
+        return fail(ctx, CD_E_CAPACITY, "more than " + std::to_string(CD_NOALLOC_MAX_PAIRS) +
+                                            " pairs in one allocation-free call");
+    if (!ctx->scratch) return fail(ctx, CD_E_HIP, "allocation-free scratch unavailable: " + ctx->scratch_err);
+    (void)hipSetDevice(ctx->device);
+    if (n) {
+        if (!a9 || !b9 || !flags) return fail(ctx, CD_E_INVALID, "null argument");
+        if (!device_span(a9, n * 9 * sizeof(float)) || !device_span(b9, n * 9 * sizeof(float)))
+            return fail(ctx, CD_E_INVALID, "a9 / b9 must be device memory holding n x 9 floats");
+        if (!device_span(flags, n * sizeof(int32_t)) || (seg6 && !device_span(seg6, n * 6 * sizeof(float))))
+            return fail(ctx, CD_E_INVALID, "flags / seg6 must be device memory holding n results");
+    }
+    hipStream_t st = ctx->stream;
+    RunCounters* ctr = static_cast<RunCounters*>(ctx->scratch);
+    uint32_t* lists = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(ctx->scratch) + NOALLOC_COUNTER_BYTES);
+    // the array keeps the last call's contents: the counters start at zero,
+    // and the exact launcher zeroes its list lengths itself (fill kernels)
+    HCK(ctx, launch_zero(ctr, st));
+    HCK(ctx, hipEventRecord(ctx->ev[0], st));
+    if (mode == CD_MODE_PLAIN)
+        HCK(ctx, launch_tritri_plain(a9, b9, n, flags, seg6, st));
+    else
+        HCK(ctx, launch_tritri_exact(a9, b9, n, flags, seg6, lists, uint32_t(ctx->num_cu) * 8, ctr, st));
+    HCK(ctx, hipEventRecord(ctx->ev[1], st));
+    RunCounters h{};
+    unsigned long long lens[2] = {0, 0};
+    HCK(ctx, hipMemcpyAsync(&h, ctr, sizeof h, hipMemcpyDeviceToHost, st));
+    if (mode == CD_MODE_BATCHED && n) HCK(ctx, hipMemcpyAsync(lens, lists, sizeof lens, hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipStreamSynchronize(st));
+    if (std::max(lens[0], lens[1]) > n)  // a pass defers each pair at most once: unreachable
+        return fail(ctx, CD_E_HIP, "exact stream: rerun list overflow");
+    float ms = 0.f;
+    HCK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
     ctx->counters = cd_counters{};
     ctx->counters.tri_tests = n;
     ctx->counters.fp64_fallbacks = h.fp64;

@@ -266,6 +266,8 @@ struct cd_ctx {
     uint32_t refit_nitems = 0;
     cdk::RefitPlan refit_plan{};
 
+    void* scratch = nullptr;               // this device's allocation-free scratch array (device address)
+    std::string scratch_err;               // why it could not be resolved
     hipGraphExec_t graph = nullptr;        // captured batched pipeline
     std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured
     hipEvent_t ev[8] = {};
