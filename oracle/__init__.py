@@ -57,6 +57,7 @@ class Oracle:
         vp = C.c_void_p
         lib.orc_eig3.argtypes = [_D, _D, _D]
         lib.orc_box_from_points.argtypes = [_F, C.c_int64, _F]
+        lib.orc_sort3_desc.argtypes = [_F, C.POINTER(C.c_int32)]
         lib.orc_sat.argtypes = [_F, _F]
         lib.orc_sat.restype = C.c_int
         lib.orc_surface.argtypes = [_F]

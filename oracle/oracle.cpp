@@ -1804,6 +1804,16 @@ void orc_box_from_points(const float* pts, int64_t n, float* box12) {
     std::memcpy(box12, &b, sizeof b);
 }
 
+// The split's axis order (OBBtree.cpp:45-51): three half-lengths sorted
+// descending; order3 receives the original positions (0 = u, 1 = v, 2 = w)
+// in sorted order.
+void orc_sort3_desc(const float* len3, int32_t* order3) {
+    AxisCand c[3];
+    for (int k = 0; k < 3; ++k) c[k] = {len3[k], {float(k), 0.f, 0.f}};
+    sort3_desc(c);
+    for (int k = 0; k < 3; ++k) order3[k] = int32_t(c[k].dir.x);
+}
+
 int orc_sat(const float* l12, const float* r12) {
     Box l, r;
     std::memcpy(&l, l12, sizeof l);

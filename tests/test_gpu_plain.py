@@ -67,6 +67,26 @@ def test_tree_build_matches_oracle(gpu, gpu_l1, orc, L):
         assert np.array_equal(got["perm"], want["perm"]), name
 
 
+def test_tree_build_hand_derived_splits(gpu_l1, orc):
+    """The split cases derived by hand (tie to the left, an axis that leaves a
+    side empty, index halves, the larger-surface swap): the GPU build gives
+    the hand-derived permutation and links, and the oracle's bits."""
+    from inmyroom_vulkan_amd import Collider
+    from test_oracle_build_pins import CASES, _mesh
+
+    with Collider(leaf_max=2) as gpu_l2:
+        ctxs = {1: gpu_l1, 2: gpu_l2}
+        for name, (rows, leaf, perm, links) in sorted(CASES.items()):
+            case(f"hand-derived split case {name} L={leaf}")
+            tri = _mesh(rows)
+            got = ctxs[leaf].export_tree(ctxs[leaf].register_soups([tri])[0])
+            assert got["perm"].tolist() == perm, name
+            assert got["links"].tolist() == links, name
+            want = orc.tree(tri, leaf).export()
+            assert got["root"].tobytes() == want["root"].tobytes(), name
+            assert got["obbs"].tobytes() == want["obbs"].tobytes(), name
+
+
 @pytest.mark.parametrize("L", [4, 1])
 def test_plain_queries_config1(gpu, gpu_l1, orc, L):
     ctx = gpu if L == 4 else gpu_l1
