@@ -58,6 +58,7 @@ class Oracle:
         lib.orc_eig3.argtypes = [_D, _D, _D]
         lib.orc_box_from_points.argtypes = [_F, C.c_int64, _F]
         lib.orc_sort3_desc.argtypes = [_F, C.POINTER(C.c_int32)]
+        lib.orc_segment_f64.argtypes = [_F, _F, C.c_int64, _F]
         lib.orc_sat.argtypes = [_F, _F]
         lib.orc_sat.restype = C.c_int
         lib.orc_surface.argtypes = [_F]
@@ -129,6 +130,15 @@ class Oracle:
         pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
         out = np.zeros(12, np.float32)
         self.lib.orc_box_from_points(pts.reshape(-1), len(pts), out)
+        return out
+
+    def segment_f64(self, a9, b9):
+        """float64 intersection segments (n, 6) of intersecting, non-coplanar pairs."""
+        a9 = np.ascontiguousarray(a9, np.float32).reshape(-1, 9)
+        b9 = np.ascontiguousarray(b9, np.float32).reshape(-1, 9)
+        out = np.zeros((len(a9), 6), np.float32)
+        if len(a9):
+            self.lib.orc_segment_f64(a9.reshape(-1), b9.reshape(-1), len(a9), out.reshape(-1))
         return out
 
     def sat(self, a12, b12) -> bool:

@@ -1655,6 +1655,77 @@ static void exact_rec(const Tree& A, const Tree& B, const Trav& f, const Trav& s
     }
 }
 
+// The intersection segment of two intersecting, non-coplanar triangles in
+// float64 from their float corners (the checker of the exact mode's contact
+// segments, which have no reference counterpart): each triangle's boundary
+// meets the other's plane in one or two points of the common line (a corner
+// on the plane, or an edge whose ends lie strictly on opposite sides); the
+// segment is the overlap of the two intervals those points span along the
+// line.  s gets the point at the low end, t the point at the high end.
+// Which side of the other plane a corner lies on is its exact orientation
+// sign; a float64 distance of the wrong sign (a corner within rounding of the
+// plane) is replaced by a tiny one of the exact sign, so the crossing point
+// falls on that corner.  NaN when no corner or edge meets the other plane
+// (the pair does not intersect).
+static void segment_f64(const float PA[3][3], const float PB[3][3], float s[3], float t[3]) {
+    auto sub = [](const double* a, const double* b, double* o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; };
+    auto crs = [](const double* a, const double* b, double* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dt = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    double a[3][3], b[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { a[i][j] = PA[i][j]; b[i][j] = PB[i][j]; }
+    double e1[3], e2[3], n1[3], n2[3], D[3];
+    sub(a[1], a[0], e1); sub(a[2], a[0], e2); crs(e1, e2, n1);
+    sub(b[1], b[0], e1); sub(b[2], b[0], e2); crs(e1, e2, n2);
+    crs(n1, n2, D);
+    // [lo point, hi point] of one triangle X against the plane (n, p0)
+    // sides[k]: exact side of X's corner k of the other triangle's plane
+    auto span = [&](const double X[3][3], const double* n, const double* p0, const int* sides, double lo[3],
+                    double hi[3], double& tlo, double& thi) {
+        double d[3], w[3];
+        for (int k = 0; k < 3; ++k) {
+            sub(X[k], p0, w);
+            d[k] = dt(n, w);
+            if (sides[k] == 0) d[k] = 0.0;
+            else if ((d[k] > 0.0) != (sides[k] > 0)) d[k] = sides[k] * 1e-300;
+        }
+        tlo = std::numeric_limits<double>::infinity();
+        thi = -tlo;
+        auto take = [&](const double* q) {
+            const double tq = dt(D, q);
+            if (tq < tlo) { tlo = tq; std::memcpy(lo, q, 3 * sizeof(double)); }
+            if (tq > thi) { thi = tq; std::memcpy(hi, q, 3 * sizeof(double)); }
+        };
+        for (int k = 0; k < 3; ++k) {
+            if (d[k] == 0.0) take(X[k]);
+            const int j = (k + 1) % 3;
+            if ((d[k] < 0.0 && d[j] > 0.0) || (d[k] > 0.0 && d[j] < 0.0)) {
+                const double u = d[k] / (d[k] - d[j]);
+                double q[3];
+                for (int c = 0; c < 3; ++c) q[c] = X[k][c] + (X[j][c] - X[k][c]) * u;
+                take(q);
+            }
+        }
+    };
+    int sa[3], sb[3];
+    for (int k = 0; k < 3; ++k) {
+        sa[k] = orient_exact(PB[1], PB[2], PB[0], PA[k]);  // sign of n2 . (a_k - b_0)
+        sb[k] = orient_exact(PA[1], PA[2], PA[0], PB[k]);  // sign of n1 . (b_k - a_0)
+    }
+    double alo[3], ahi[3], blo[3], bhi[3], tal, tah, tbl, tbh;
+    span(a, n2, b[0], sa, alo, ahi, tal, tah);
+    span(b, n1, a[0], sb, blo, bhi, tbl, tbh);
+    if (!(tal <= tah) || !(tbl <= tbh)) {
+        for (int c = 0; c < 3; ++c) s[c] = t[c] = std::numeric_limits<float>::quiet_NaN();
+        return;
+    }
+    const double* lo = tal >= tbl ? alo : blo;
+    const double* hi = tah <= tbh ? ahi : bhi;
+    for (int c = 0; c < 3; ++c) { s[c] = float(lo[c]); t[c] = float(hi[c]); }
+}
+
 void query_exact(const Tree& A, const Tree& B, const M4& M, uint32_t query, std::vector<Contact>& out, Counters& k) {
     // rounding of B's float transform, measured on B's own vertices
     double dev = 0.0;
@@ -1693,6 +1764,21 @@ void query_exact(const Tree& A, const Tree& B, const M4& M, uint32_t query, std:
                     c.query = query;
                     c.triA = ta;
                     c.triB = tb;
+                    // the contact filter's planes and vertex-outside flags, as the
+                    // reference forms them for a colliding pair (CreateUncollideRays.cpp:88-112)
+                    V3 VA[3], VB[3];
+                    for (int q = 0; q < 3; ++q) {
+                        VA[q] = {PA[q][0], PA[q][1], PA[q][2]};
+                        VB[q] = {PB[q][0], PB[q][1], PB[q][2]};
+                    }
+                    const PlaneF pa = plane_of(VA), pb = plane_of(VB);
+                    for (int q = 0; q < 3; ++q) {
+                        if (outside(pb, VA[q])) c.flags |= 1u << q;
+                        if (outside(pa, VB[q])) c.flags |= 1u << (3 + q);
+                    }
+                    c.na[0] = pa.n.x; c.na[1] = pa.n.y; c.na[2] = pa.n.z;
+                    c.nb[0] = pb.n.x; c.nb[1] = pb.n.y; c.nb[2] = pb.n.z;
+                    segment_f64(PA, PB, c.s, c.t);
                     out.push_back(c);
                     ++k.contacts;
                 }
@@ -1802,6 +1888,17 @@ void orc_eig3(const double* A9, double* V9, double* d3) {
 void orc_box_from_points(const float* pts, int64_t n, float* box12) {
     Box b = box_from_points(reinterpret_cast<const V3*>(pts), size_t(n));
     std::memcpy(box12, &b, sizeof b);
+}
+
+// float64 intersection segments of n intersecting, non-coplanar triangle
+// pairs (9 floats each), 6 floats per pair: the low end, then the high end.
+void orc_segment_f64(const float* a9, const float* b9, int64_t n, float* out6) {
+    for (int64_t i = 0; i < n; ++i) {
+        float PA[3][3], PB[3][3];
+        std::memcpy(PA, a9 + 9 * i, sizeof PA);
+        std::memcpy(PB, b9 + 9 * i, sizeof PB);
+        segment_f64(PA, PB, out6 + 6 * i, out6 + 6 * i + 3);
+    }
 }
 
 // The split's axis order (OBBtree.cpp:45-51): three half-lengths sorted

@@ -201,6 +201,63 @@ def test_tritri_exact_vs_bruteforce(orc):
     del rng
 
 
+def _segment_fraction(a9, b9):
+    """Exact intersection segment of two triangles by clipping the planes'
+    common line against both triangles (Cyrus-Beck), in Fractions: the line
+    P(t) = P0 + t D, D = n1 x n2, P0 on both planes with D . P0 = 0; each
+    edge keeps the side of its triangle's third vertex."""
+    F = Fraction
+    A = [[F(float(x)) for x in a9[3 * k:3 * k + 3]] for k in range(3)]
+    B = [[F(float(x)) for x in b9[3 * k:3 * k + 3]] for k in range(3)]
+    sub = lambda p, q: [p[i] - q[i] for i in range(3)]
+    dot = lambda p, q: p[0] * q[0] + p[1] * q[1] + p[2] * q[2]
+    crs = lambda p, q: [p[1] * q[2] - p[2] * q[1], p[2] * q[0] - p[0] * q[2], p[0] * q[1] - p[1] * q[0]]
+    n1, n2 = crs(sub(A[1], A[0]), sub(A[2], A[0])), crs(sub(B[1], B[0]), sub(B[2], B[0]))
+    D = crs(n1, n2)
+    rows, rhs = [n1, n2, D], [dot(n1, A[0]), dot(n2, B[0]), F(0)]
+    det3 = lambda m: dot(m[0], crs(m[1], m[2]))
+    den = det3(rows)
+    cols = [[rows[r][c] for r in range(3)] for c in range(3)]
+    P0 = []
+    for c in range(3):
+        mc = [list(col) for col in cols]
+        mc[c] = rhs
+        P0.append(det3([[mc[cc][r] for cc in range(3)] for r in range(3)]) / den)
+    lo, hi = None, None
+    for T, n in ((A, n1), (B, n2)):
+        for i in range(3):
+            j, k = (i + 1) % 3, (i + 2) % 3
+            m = crs(n, sub(T[j], T[i]))
+            if dot(m, sub(T[k], T[i])) < 0:
+                m = [-x for x in m]
+            a, b = dot(m, D), -dot(m, sub(P0, T[i]))  # need a t >= b
+            if a > 0:
+                lo = b / a if lo is None else max(lo, b / a)
+            elif a < 0:
+                hi = b / a if hi is None else min(hi, b / a)
+    pt = lambda t: [float(P0[i] + t * D[i]) for i in range(3)]
+    return np.array(pt(lo) + pt(hi))
+
+
+def test_segment_f64_closed_form_and_exact_clip(orc):
+    """The exact mode's contact segments are checked against segment_f64:
+    a hand-computed crossing, then exact clipping on random intersecting pairs."""
+    a = np.array([-1, -1, 0, 2, -1, 0, -1, 2, 0], np.float32)   # z = 0, hypotenuse x + y = 1
+    b = np.array([0, -5, -1, 0, 5, -1, 0, 0, 1], np.float32)    # x = 0, crosses z = 0 at y = +-2.5
+    seg = orc.segment_f64(a[None], b[None])[0]
+    assert seg.tolist() in ([0, -1, 0, 0, 1, 0], [0, 1, 0, 0, -1, 0])
+    a9, b9, _ = scenes.config4(3000, seed=19, span=1.0)
+    ex = orc.tritri_exact(a9, b9)
+    hit = np.flatnonzero(ex == 1)[:150]
+    assert len(hit) >= 100
+    got = orc.segment_f64(a9[hit], b9[hit])
+    for i, g in zip(hit, got):
+        want = _segment_fraction(a9[i], b9[i])
+        scale = max(1.0, np.abs(want).max())
+        err = min(np.abs(g - want).max(), np.abs(g - np.r_[want[3:], want[:3]]).max())
+        assert err <= 1e-6 * scale, (i, g, want)
+
+
 def test_exact_vs_moller_agree_away_from_degeneracy(orc):
     a9, b9, kind = scenes.config4(20000, seed=31)
     f, _ = orc.tritri(a9, b9)
