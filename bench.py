@@ -20,7 +20,11 @@ divided by that kernel's average duration measured with HIP events on the
 library's stream.
 
 CPU baseline: the oracle (CPU restatement of the reference path, one
-thread, as the reference runs) on a bounded sample of the same queries.
+thread, as the reference runs) over the whole batch, built on this host with
+the reference's release flags (-O3 -ffast-math -march=native); the strict
+parity build is timed beside it (1 warm-up, median of 5 each).  The strict
+build also checks the plain path's pair set on every query, and the exact
+oracle the batched path's.
 """
 from __future__ import annotations
 
@@ -96,32 +100,6 @@ def measured_traffic(kind: str, args):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(soups, qa, qb, rel, budget_s: float, leaf_max: int):
-    """Oracle, one thread, on an evenly spaced sample of the queries."""
-    from oracle import Oracle
-
-    orc = Oracle()
-    nq = len(qa)
-    order = np.argsort(np.arange(nq) % 16, kind="stable")  # every 16th query first, then the next offset, ...
-    trees = {}
-    done, elapsed, tt = 0, 0.0, 0
-    sample = []
-    for i in order:
-        for m in (int(qa[i]), int(qb[i])):
-            if m not in trees:
-                trees[m] = orc.tree(soups[m], leaf_max)
-        t0 = time.perf_counter()
-        r = orc.batch([trees[int(qa[i])], trees[int(qb[i])]], np.array([0], np.uint32), np.array([1], np.uint32),
-                      rel[i:i + 1], mode=0)
-        elapsed += time.perf_counter() - t0
-        tt += r.total_tritests
-        done += 1
-        sample.append(int(i))
-        if elapsed >= budget_s or done == nq:
-            break
-    return {"queries": done, "seconds": elapsed, "tritests": tt, "sample": sample, "trees": trees, "orc": orc}
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,15 +108,13 @@ def main() -> int:
     ap.add_argument("--nq", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--exact-sample", type=int, default=48, help="queries checked against the exact oracle")
     ap.add_argument("--copy-probe", type=int, default=0, help="bytes of a calibration copy run first (PMC runs)")
     ap.add_argument("--tuning", default="", help="override tuning fields, e.g. work_sharing=1,split_levels=8")
-    ap.add_argument("--config", type=int, default=3, choices=(2, 3, 4, 5, 6),
-                    help="3: the headline batch; 2, 4, 5, 6: scene frames, triangle-pair stream, deformable batch, "
-                         "glTF ingest")
+    ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3, 4, 5, 6),
+                    help="3: the headline batch; 1, 2, 4, 5, 6: two icospheres over 1,000 poses, scene frames, "
+                         "triangle-pair stream, deformable batch, glTF ingest")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported (the batch shards by query "
@@ -270,27 +246,41 @@ def main() -> int:
         "tuning": gpu.tuning(),
     }
 
-    # ---- exact-oracle check on a sample and the CPU baseline ----
+    # ---- CPU baseline and the exact-oracle check, both over the whole batch ----
     if not args.no_cpu:
-        log(f"CPU baseline: oracle, 1 thread, up to {args.cpu_seconds:.0f}s of queries")
-        cb = cpu_baseline(soups, qa, qb, rel, args.cpu_seconds, 4)
-        cpu_qps = cb["queries"] / cb["seconds"]
-        result["cpu_baseline"] = {
-            "value": round(cpu_qps, 2), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{cb['queries']} of the {args.nq} queries (every 16th first), reference semantics, "
-                      f"{cb['seconds']:.1f}s single-threaded; tri-tri tests/s "
-                      f"{cb['tritests'] / cb['seconds']:.0f}",
-        }
-        orc, trees = cb["orc"], cb["trees"]
-        sample = cb["sample"][: args.exact_sample]
-        mism = 0
-        for i in sample:
-            r = orc.batch([trees[int(qa[i])], trees[int(qb[i])]], np.array([0], np.uint32),
-                          np.array([1], np.uint32), rel[i:i + 1], mode=1)
-            want = {(int(i), int(c["tri_a"]), int(c["tri_b"])) for c in r.contacts}
-            got = {k for k in batched_keys if k[0] == i}
-            mism += len(want ^ got)
-        result["pairset_mismatches_vs_exact_oracle"] = {"queries_checked": len(sample), "mismatches": mism}
+        from bench_configs import CpuOracles, cpu_line, timed_median
+
+        orcs = CpuOracles()
+        t0 = time.time()
+        strict_trees = orcs.trees("strict", soups)
+        native_trees = orcs.trees("native", soups)
+        log(f"oracle trees (both builds, 16 threads): {time.time() - t0:.1f}s")
+        qa32, qb32 = qa.astype(np.uint32), qb.astype(np.uint32)
+        runs = {}
+
+        def run(which, trees):
+            runs[which] = getattr(orcs, which).batch(trees, qa32, qb32, rel, mode=0, nthreads=1)
+
+        native_s = timed_median(lambda: run("native", native_trees))
+        strict_s = timed_median(lambda: run("strict", strict_trees))
+        tt = runs["strict"].total_tritests
+        log(f"CPU baseline: {native_s:.3f} s/batch release build, {strict_s:.3f} s strict, one thread")
+        result["cpu_baseline"] = cpu_line(
+            args.nq / native_s, "queries/s",
+            f"the whole batch of {args.nq} queries (reference semantics: traversal and triangle tests, trees "
+            f"built beforehand); tri-tri tests/s {tt / native_s:.0f} (strict build {tt / strict_s:.0f})",
+            native_s, strict_s, orcs, extra={"strict_value": round(args.nq / strict_s, 2)})
+        # the strict build is the reference's arithmetic: the plain GPU path must match it on every query
+        ref_keys = set(zip(runs["strict"].contacts["query"].tolist(), runs["strict"].contacts["tri_a"].tolist(),
+                           runs["strict"].contacts["tri_b"].tolist()))
+        result["plain_pairset_mismatches_vs_reference_semantics"] = {"queries_checked": args.nq,
+                                                                     "mismatches": len(ref_keys ^ plain_keys)}
+        t0 = time.time()
+        ex = orcs.strict.batch(strict_trees, qa32, qb32, rel, mode=1, nthreads=16)
+        want = set(zip(ex.contacts["query"].tolist(), ex.contacts["tri_a"].tolist(), ex.contacts["tri_b"].tolist()))
+        log(f"exact oracle over the whole batch: {time.time() - t0:.1f}s")
+        result["pairset_mismatches_vs_exact_oracle"] = {"queries_checked": args.nq,
+                                                        "mismatches": len(want ^ batched_keys)}
     print(json.dumps(result), flush=True)
     gpu.close()
     return 0

@@ -44,16 +44,28 @@ def build(ref: bool | None = None, fast: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 
+def build_native(out: str) -> str:
+    """Compile the oracle with the reference's release flags (-O3 -ffast-math
+    -march=native, inMyRoom_vulkan/CMakeLists.txt:30) for this host into
+    `out`; the CPU baseline of the benchmark, built where it is measured."""
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={os.path.abspath(out)}"], check=True)
+    return os.path.abspath(out)
+
+
+NATIVE_FLAGS = "-std=c++17 -O3 -ffast-math -march=native"
+STRICT_FLAGS = "-std=c++17 -O2 -ffp-contract=off -fno-fast-math"
+
+
 def _load(name: str) -> C.CDLL:
-    path = os.path.join(HERE, name)
+    path = name if os.path.isabs(name) else os.path.join(HERE, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is not built; run oracle.build() or `make -C oracle`")
     return C.CDLL(path)
 
 
 class Oracle:
-    def __init__(self, fast: bool = False):
-        lib = self.lib = _load("liboracle_fast.so" if fast else "liboracle.so")
+    def __init__(self, fast: bool = False, path: str | None = None):
+        lib = self.lib = _load(path or ("liboracle_fast.so" if fast else "liboracle.so"))
         vp = C.c_void_p
         lib.orc_eig3.argtypes = [_D, _D, _D]
         lib.orc_box_from_points.argtypes = [_F, C.c_int64, _F]
