@@ -3,18 +3,20 @@
 //
 // Same member names, call order and argument meaning as
 // inMyRoom_vulkan/include/CollisionDetection/CollisionDetection.h:11-19
-// (Reset, AddCollisionDetectionEntry, ExecuteCollisionDetection), with the
-// entry type of inMyRoom_vulkan/include/ECS/ECStypes.h:149-156 restated on
-// plain floats (the OBBtree pointer becomes a registered mesh id).
+// (CollisionDetection(ECSwrapper*), Reset, AddCollisionDetectionEntry,
+// ExecuteCollisionDetection).  BasicCollisionDetection takes the engine's own
+// CollisionDetectionEntry and CollisionCallbackData (ECStypes.h:149-163) as
+// template arguments, so the engine's callers compile unchanged: the entry's
+// `const OBBtree* OBBtree_ptr` (set at ModelCollisionCompEntity.cpp:73) is
+// mapped to the GPU mesh by TreeRegistry, and the ECS wrapper's entities
+// handler and components receive the callbacks as the reference hands them
+// over.  mi355x::CollisionDetection instantiates it on plain-float
+// restatements of those types for hosts without the engine.
 // ExecuteCollisionDetection runs src/CollisionDetection/CollisionDetection.cpp:38-141
 // on the GPU: phases 1-3 (broad, mid, narrow), the uncollide rays of
 // CreateUncollideRays.cpp:60-200, the ray shooting of ShootUncollideRays.cpp
 // with the movement split (:69-99), then the callback fan-out to entity
-// ancestors (:101-121) and MakeCallbacks (:124-134).  The two engine
-// services those last steps use are hooks: the entity's ancestor list
-// (EntitiesHandler::GetEntityAncestors) and the components' CollisionCallback
-// (ComponentBaseClass.h:25), which receives the callbacks as the reference
-// hands them over.
+// ancestors (:101-121) and MakeCallbacks (:124-134).
 //
 // Error behaviour: the reference has no error returns; this mirror throws
 // std::runtime_error with the library's message when a call fails (the C ABI
@@ -23,10 +25,12 @@
 
 #include <algorithm>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <iterator>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -37,25 +41,83 @@ namespace mi355x {
 
 typedef uint16_t Entity;  // ECStypes.h:18
 
+// With this header alone, an entry's `const OBBtree*` points to a handle of a
+// registered GPU tree (MeshesOfNodes::MeshInfo::boundBoxTree below).  In the
+// engine the pointer is the engine's own OBBtree (MeshesOfNodes.h:28), which
+// TreeRegistry maps to its mesh id.
+class OBBtree {
+public:
+    OBBtree() = default;
+    explicit OBBtree(uint32_t mesh_id) : meshId(mesh_id) {}
+    uint32_t meshId = CD_NO_MESH;
+};
+
+// Column-major 4x4 with glm's access pattern: m[column][row].
+struct Mat4 {
+    float m[16];
+    float* operator[](int column) { return m + 4 * column; }
+    const float* operator[](int column) const { return m + 4 * column; }
+};
+struct Vec3 {
+    float v[3] = {0.f, 0.f, 0.f};
+    float& operator[](int k) { return v[k]; }
+    const float& operator[](int k) const { return v[k]; }
+};
+
 struct CollisionDetectionEntry {  // ECStypes.h:149-156
-    float currentGlobalMatrix[16];
-    float previousGlobalMatrix[16];
-    uint32_t meshId;  // replaces const OBBtree* OBBtree_ptr
-    bool shouldCallback;
-    Entity entity;
+    Mat4 currentGlobalMatrix;
+    Mat4 previousGlobalMatrix;
+    const OBBtree* OBBtree_ptr = nullptr;
+    bool shouldCallback = false;
+    Entity entity = 0;
 };
 
 struct CollisionCallbackData {  // ECStypes.h:158-163
     Entity familyEntity;
     Entity collideWithEntity;
-    float deltaVector[3] = {0.f, 0.f, 0.f};
+    Vec3 deltaVector;
 };
-typedef std::vector<std::pair<Entity, std::vector<CollisionCallbackData>>> CallbackEntityDataPairs;
 
-struct CollidingPair {
-    Entity first, second;          // oriented as SweepAndPrune.cpp:63 orients them
-    uint32_t firstIndex, secondIndex;  // positions in the frame's entry list
-    std::vector<cd_contact> contacts;  // in the first entity's model space
+// Engine tree object -> GPU mesh id.  The engine keeps its meshes in a
+// std::vector grown by emplace_back while models are added
+// (MeshesOfNodes.cpp:62, MeshesOfNodes.h:62), so a tree's address moves until
+// the last model is in; MeshesOfNodes::GetMeshInfo asserts that the meshes
+// were flashed (MeshesOfNodes.h:48), after which the vector never grows.
+// Register each mesh by its index at load, then, once the list is final,
+// bind the addresses: Bind(&GetMeshInfo(i).boundBoxTree, i).
+class TreeRegistry {
+public:
+    void SetMeshIdOfIndex(size_t mesh_index, uint32_t mesh_id) {
+        if (idOfIndex_.size() <= mesh_index) idOfIndex_.resize(mesh_index + 1, CD_NO_MESH);
+        idOfIndex_[mesh_index] = mesh_id;
+    }
+    void Bind(const void* tree, size_t mesh_index) {
+        if (mesh_index >= idOfIndex_.size()) throw std::runtime_error("TreeRegistry::Bind: mesh index never registered");
+        idOfTree_[tree] = idOfIndex_[mesh_index];
+    }
+    void Clear() { idOfTree_.clear(); }
+    uint32_t MeshId(const OBBtree* t) const {  // the handle carries its id
+        if (!t) throw std::runtime_error("collision entry without a tree");
+        return t->meshId;
+    }
+    template <class Tree> uint32_t MeshId(const Tree* t) const {
+        auto it = idOfTree_.find(static_cast<const void*>(t));
+        if (it == idOfTree_.end())
+            throw std::runtime_error("collision entry names a tree that was never bound (bind after the last model)");
+        return it->second;
+    }
+
+private:
+    std::vector<uint32_t> idOfIndex_;
+    std::unordered_map<const void*, uint32_t> idOfTree_;
+};
+
+// One colliding entity pair of the last ExecuteCollisionDetection.
+template <class EntityT, class CallbackData>
+struct BasicCollidingPair {
+    EntityT first, second;                 // oriented as SweepAndPrune.cpp:63 orients them
+    uint32_t firstIndex, secondIndex;      // positions in the frame's entry list
+    std::vector<cd_contact> contacts;      // in the first entity's model space
     // CDentriesUncollideRays (include/CollisionDetection/CreateUncollideRays.h:6-14);
     // every ray in the first entity's model space
     std::vector<cd_ray> rays_from_first_to_second, rays_from_second_to_first;
@@ -64,12 +126,23 @@ struct CollidingPair {
     // the response (CollisionDetection.cpp:70-99): zero when neither entity moved
     bool shot = false;
     float delta[3] = {0.f, 0.f, 0.f};
-    CollisionCallbackData first_callback, second_callback;
+    CallbackData first_callback{}, second_callback{};
 };
 
-class CollisionDetection {
+// CollisionDetection (CollisionDetection.h:11-19) over the engine's own entry
+// and callback types: an Entry has currentGlobalMatrix / previousGlobalMatrix
+// (m[column][row] access, glm::mat4 or Mat4), OBBtree_ptr, shouldCallback and
+// entity; a CallbackData has familyEntity, collideWithEntity and deltaVector
+// (v[k] access, glm::vec3 or Vec3).  In the engine:
+//   using CollisionDetection = mi355x::BasicCollisionDetection<CollisionDetectionEntry, CollisionCallbackData>;
+template <class Entry = CollisionDetectionEntry, class CallbackData = CollisionCallbackData>
+class BasicCollisionDetection {
 public:
-    explicit CollisionDetection(uint32_t leaf_max = 4, uint32_t mode = CD_MODE_PLAIN) : mode_(mode) {
+    using EntityT = typename std::decay<decltype(std::declval<Entry&>().entity)>::type;
+    using CallbackEntityDataPairs = std::vector<std::pair<EntityT, std::vector<CallbackData>>>;
+    using CollidingPair = BasicCollidingPair<EntityT, CallbackData>;
+
+    explicit BasicCollisionDetection(uint32_t leaf_max = 4, uint32_t mode = CD_MODE_PLAIN) : mode_(mode) {
         cd_config cfg{};
         cfg.leaf_max = leaf_max;
         check(cd_create(&ctx_, &cfg), "cd_create");
@@ -77,17 +150,41 @@ public:
         t.use_graph = 1;  // the batched pipeline is captured once and replayed every frame
         check(cd_set_tuning(ctx_, &t), "cd_set_tuning");
     }
-    ~CollisionDetection() {
+
+    // The reference's constructor (CollisionDetection.h:14): the engine's ECS
+    // gives the ancestors of an entity (CollisionDetection.cpp:106-107) and the
+    // components whose CollisionCallback receives the callbacks (:124-134).
+    template <class ECSwrapperT, class = decltype(std::declval<ECSwrapperT&>().GetEntitiesHandler())>
+    explicit BasicCollisionDetection(ECSwrapperT* ecs, uint32_t mode = CD_MODE_PLAIN, uint32_t leaf_max = 4)
+        : BasicCollisionDetection(leaf_max, mode) {
+        ancestors_ = [ecs](EntityT e) {
+            const auto a = ecs->GetEntitiesHandler()->GetEntityAncestors(e);
+            return std::vector<EntityT>(a.begin(), a.end());
+        };
+        sink_ = [ecs](const CallbackEntityDataPairs& v) {
+            for (const auto& this_compID_component_ptr_pair : ecs->GetComponentIDtoComponentBaseClassMap())
+                if (this_compID_component_ptr_pair.second != nullptr)
+                    this_compID_component_ptr_pair.second->CollisionCallback(v);
+        };
+    }
+    ~BasicCollisionDetection() {
         if (ctx_) cd_destroy(ctx_);
     }
-    CollisionDetection(const CollisionDetection&) = delete;
-    CollisionDetection& operator=(const CollisionDetection&) = delete;
+    BasicCollisionDetection(const BasicCollisionDetection&) = delete;
+    BasicCollisionDetection& operator=(const BasicCollisionDetection&) = delete;
 
     // Mesh ingest: an indexed triangle list (glTF "triangles" mode); the OBB
     // tree is built on the GPU (OBBtree.cpp:321-358).
     uint32_t RegisterMesh(const float* positions, uint32_t nvert, const uint32_t* indices, uint32_t ntri) {
         uint32_t id = 0;
         check(cd_register_mesh(ctx_, positions, nvert, indices, ntri, &id), "cd_register_mesh");
+        return id;
+    }
+    // A triangle soup (9 floats per triangle), as GetOBBtreeAndReset hands
+    // its triangles to OBBtree (PrimitivesOfMeshes.cpp:840-863).
+    uint32_t RegisterTriangles(const float* tri9, uint32_t ntri) {
+        uint32_t id = 0;
+        check(cd_register_soups(ctx_, tri9, &ntri, 1, &id), "cd_register_soups");
         return id;
     }
 
@@ -97,19 +194,26 @@ public:
         check(cd_set_mesh_normals(ctx_, mesh, normals, nvert), "cd_set_mesh_normals");
     }
 
+    // Engine trees: see TreeRegistry.
+    TreeRegistry& Trees() { return trees_; }
+
     void Reset() { entries_.clear(); }  // CollisionDetection.cpp:28-31
 
-    void AddCollisionDetectionEntry(const CollisionDetectionEntry in_collisionDetectionEntry) {  // :33-36
+    void AddCollisionDetectionEntry(const Entry in_collisionDetectionEntry) {  // :33-36
+        const Entry& src = in_collisionDetectionEntry;
         cd_entity e{};
-        std::memcpy(e.current, in_collisionDetectionEntry.currentGlobalMatrix, sizeof e.current);
-        std::memcpy(e.previous, in_collisionDetectionEntry.previousGlobalMatrix, sizeof e.previous);
-        e.mesh_id = in_collisionDetectionEntry.meshId;
-        e.entity = in_collisionDetectionEntry.entity;
-        e.should_callback = in_collisionDetectionEntry.shouldCallback ? 1 : 0;
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) {
+                e.current[4 * c + r] = float(src.currentGlobalMatrix[c][r]);
+                e.previous[4 * c + r] = float(src.previousGlobalMatrix[c][r]);
+            }
+        e.mesh_id = trees_.MeshId(src.OBBtree_ptr);
+        e.entity = uint16_t(src.entity);
+        e.should_callback = src.shouldCallback ? 1 : 0;
         entries_.push_back(e);
     }
 
-    void ExecuteCollisionDetection() {  // :38-67, phases 1-3
+    void ExecuteCollisionDetection() {  // :38-141
         results_.clear();
         if (entries_.size() < 2) return;
         // The output buffers keep the largest sizes seen, with a quarter more
@@ -144,8 +248,8 @@ public:
             CollidingPair r;
             r.firstIndex = rp.first;
             r.secondIndex = rp.second;
-            r.first = entries_[rp.first].entity;
-            r.second = entries_[rp.second].entity;
+            r.first = EntityT(entries_[rp.first].entity);
+            r.second = EntityT(entries_[rp.second].entity);
             if (k < np)
                 r.contacts.assign(contacts_.begin() + pairs_[k].contact_begin,
                                   contacts_.begin() + pairs_[k].contact_begin + pairs_[k].contact_count);
@@ -164,41 +268,44 @@ public:
         std::vector<cd_pair_response> resp(results_.size());
         uint32_t nresp = 0;
         check(cd_shoot_rays(ctx_, nullptr, resp.data(), uint32_t(resp.size()), &nresp), "cd_shoot_rays");
-        std::unordered_map<Entity, std::vector<CollisionCallbackData>> callbacks_to_be_made;
+        std::unordered_map<EntityT, std::vector<CallbackData>> callbacks_to_be_made;
         for (size_t i = 0; i < results_.size() && i < nresp; ++i) {
             CollidingPair& r = results_[i];
             r.shot = resp[i].shot != 0;
             std::memcpy(r.delta, resp[i].delta, sizeof r.delta);
             r.first_callback.familyEntity = r.first;
             r.first_callback.collideWithEntity = r.second;
-            std::memcpy(r.first_callback.deltaVector, resp[i].delta_first, sizeof r.first_callback.deltaVector);
             r.second_callback.familyEntity = r.second;
             r.second_callback.collideWithEntity = r.first;
-            std::memcpy(r.second_callback.deltaVector, resp[i].delta_second, sizeof r.second_callback.deltaVector);
+            for (int c = 0; c < 3; ++c) {
+                r.first_callback.deltaVector[c] = resp[i].delta_first[c];
+                r.second_callback.deltaVector[c] = resp[i].delta_second[c];
+            }
             // the callback goes to every ancestor the two entities do not share (:101-121)
-            const std::vector<Entity> fa = Ancestors(r.first), sa = Ancestors(r.second);
-            for (size_t k = 0; k != fa.size(); ++k)
-                if (k >= sa.size() || fa[k] != sa[k]) callbacks_to_be_made[fa[k]].emplace_back(r.first_callback);
-            for (size_t k = 0; k != sa.size(); ++k)
-                if (k >= fa.size() || fa[k] != sa[k]) callbacks_to_be_made[sa[k]].emplace_back(r.second_callback);
+            const std::vector<EntityT> fa = Ancestors(r.first), sa = Ancestors(r.second);
+            for (size_t a = 0; a != fa.size(); ++a)
+                if (a >= sa.size() || fa[a] != sa[a]) callbacks_to_be_made[fa[a]].emplace_back(r.first_callback);
+            for (size_t a = 0; a != sa.size(); ++a)
+                if (a >= fa.size() || fa[a] != sa[a]) callbacks_to_be_made[sa[a]].emplace_back(r.second_callback);
         }
         MakeCallbacks(std::move(callbacks_to_be_made));
     }
 
-    // Engine hooks for the last steps (CollisionDetection.cpp:101-134).
-    // Without an ancestor provider an entity is its own only ancestor.
-    void SetAncestorsProvider(std::function<std::vector<Entity>(Entity)> f) { ancestors_ = std::move(f); }
+    // Engine hooks for the last steps (CollisionDetection.cpp:101-134), for a
+    // host without the engine's ECS.  Without an ancestor provider an entity
+    // is its own only ancestor.
+    void SetAncestorsProvider(std::function<std::vector<EntityT>(EntityT)> f) { ancestors_ = std::move(f); }
     void SetCallbackSink(std::function<void(const CallbackEntityDataPairs&)> f) { sink_ = std::move(f); }
 
     const std::vector<CollidingPair>& GetCollidingPairs() const { return results_; }
     cd_ctx* Context() const { return ctx_; }
 
 private:
-    std::vector<Entity> Ancestors(Entity e) const { return ancestors_ ? ancestors_(e) : std::vector<Entity>{e}; }
+    std::vector<EntityT> Ancestors(EntityT e) const { return ancestors_ ? ancestors_(e) : std::vector<EntityT>{e}; }
 
     // CollisionDetection::MakeCallbacks (:124-134): the map becomes a vector
     // of (entity, callbacks) pairs handed to the components.
-    void MakeCallbacks(std::unordered_map<Entity, std::vector<CollisionCallbackData>>&& callbacks_to_be_made) const {
+    void MakeCallbacks(std::unordered_map<EntityT, std::vector<CallbackData>>&& callbacks_to_be_made) const {
         CallbackEntityDataPairs v(std::make_move_iterator(callbacks_to_be_made.begin()),
                                   std::make_move_iterator(callbacks_to_be_made.end()));
         if (sink_) sink_(v);
@@ -210,28 +317,34 @@ private:
 
     cd_ctx* ctx_ = nullptr;
     uint32_t mode_;
+    TreeRegistry trees_;
     std::vector<cd_entity> entries_;
     std::vector<cd_pair> pairs_;
     std::vector<cd_contact> contacts_;
     std::vector<cd_pair_rays> ray_pairs_;
     std::vector<cd_ray> rays_;
     std::vector<CollidingPair> results_;
-    std::function<std::vector<Entity>(Entity)> ancestors_;
+    std::function<std::vector<EntityT>(EntityT)> ancestors_;
     std::function<void(const CallbackEntityDataPairs&)> sink_;
 };
+
+using CollisionDetection = BasicCollisionDetection<>;
+using CallbackEntityDataPairs = CollisionDetection::CallbackEntityDataPairs;
+using CollidingPair = CollisionDetection::CollidingPair;
 
 // The collision part of MeshesOfNodes (include/Graphics/Meshes/MeshesOfNodes.h:
 // 28, 48-51): AddMeshesOfModel gives every mesh of a glTF model its tree
 // (src/Graphics/Meshes/MeshesOfNodes.cpp:29-64), GetMeshIndexOffsetOfModel
 // (:66-73) and GetMeshInfo(i).boundBoxTree name it.  A model is named by its
-// path where the reference keys it by its tinygltf::Model; boundBoxTree is
-// the mesh id entries carry (CD_NO_MESH for a mesh with no collision
-// triangles: skinned or morphed only, which the reference never attaches a
-// collision component to, src/GameImporter.cpp:320-321).
+// path where the reference keys it by its tinygltf::Model; boundBoxTree is an
+// OBBtree handle whose meshId is the GPU tree (CD_NO_MESH for a mesh with no
+// collision triangles: skinned or morphed only, which the reference never
+// attaches a collision component to, src/GameImporter.cpp:320-321), so
+// `entry.OBBtree_ptr = &GetMeshInfo(i).boundBoxTree` works as in the engine.
 class MeshesOfNodes {
 public:
     struct MeshInfo {
-        uint32_t boundBoxTree = CD_NO_MESH;
+        OBBtree boundBoxTree;  // a handle: boundBoxTree.meshId is the GPU tree
     };
 
     explicit MeshesOfNodes(CollisionDetection& collisionDetection) : cd_(collisionDetection) {}
@@ -248,7 +361,7 @@ public:
         cd_gltf_close(doc);
         if (rc != CD_OK) throw std::runtime_error(std::string("cd_register_gltf: ") + cd_last_error(cd_.Context()));
         modelToMeshIndexOffset_.emplace(path, meshes_.size());
-        for (uint32_t id : ids) meshes_.push_back(MeshInfo{id});
+        for (uint32_t id : ids) meshes_.push_back(MeshInfo{OBBtree(id)});
     }
 
     size_t GetMeshIndexOffsetOfModel(const std::string& path) const {
@@ -257,12 +370,14 @@ public:
         return it->second;
     }
 
+    // the address stays valid while models are added (a deque never moves its
+    // elements when it grows at the end), so entries may take it at any time
     const MeshInfo& GetMeshInfo(size_t this_mesh_index) const { return meshes_.at(this_mesh_index); }
     size_t size() const { return meshes_.size(); }
 
 private:
     CollisionDetection& cd_;
-    std::vector<MeshInfo> meshes_;
+    std::deque<MeshInfo> meshes_;
     std::unordered_map<std::string, size_t> modelToMeshIndexOffset_;
 };
 

@@ -21,7 +21,7 @@ int main() {
     const float P[8 * 3] = {-1,-1,-1, 1,-1,-1, 1,1,-1, -1,1,-1, -1,-1,1, 1,-1,1, 1,1,1, -1,1,1};
     const uint32_t I[12 * 3] = {0,2,1, 0,3,2, 4,5,6, 4,6,7, 0,1,5, 0,5,4, 2,3,7, 2,7,6, 1,2,6, 1,6,5, 0,4,7, 0,7,3};
     mi355x::CollisionDetection cd(4, CD_MODE_PLAIN);
-    const uint32_t mesh = cd.RegisterMesh(P, 8, I, 12);
+    const mi355x::OBBtree tree(cd.RegisterMesh(P, 8, I, 12));
     // entities 5 and 7 share ancestor 2: callbacks go to 5 and 7 only
     cd.SetAncestorsProvider([](mi355x::Entity e) { return std::vector<mi355x::Entity>{2, e}; });
     std::map<int, std::vector<mi355x::CollisionCallbackData>> got;
@@ -30,12 +30,12 @@ int main() {
     });
     auto entry = [&](mi355x::Entity ent, float x, float prev_x, float y, float z) {
         mi355x::CollisionDetectionEntry e{};
-        for (int i = 0; i < 16; ++i) e.currentGlobalMatrix[i] = e.previousGlobalMatrix[i] = (i % 5 == 0) ? 1.f : 0.f;
-        e.currentGlobalMatrix[12] = x;
-        e.currentGlobalMatrix[13] = e.previousGlobalMatrix[13] = y;
-        e.currentGlobalMatrix[14] = e.previousGlobalMatrix[14] = z;
-        e.previousGlobalMatrix[12] = prev_x;
-        e.meshId = mesh;
+        for (int i = 0; i < 16; ++i) e.currentGlobalMatrix.m[i] = e.previousGlobalMatrix.m[i] = (i % 5 == 0) ? 1.f : 0.f;
+        e.currentGlobalMatrix[3][0] = x;
+        e.currentGlobalMatrix[3][1] = e.previousGlobalMatrix[3][1] = y;
+        e.currentGlobalMatrix[3][2] = e.previousGlobalMatrix[3][2] = z;
+        e.previousGlobalMatrix[3][0] = prev_x;
+        e.OBBtree_ptr = &tree;
         e.shouldCallback = true;
         e.entity = ent;
         return e;
@@ -128,19 +128,22 @@ int main(int argc, char** argv) {
     mi355x::CollisionDetection cd(4, CD_MODE_PLAIN);
     mi355x::MeshesOfNodes meshes(cd);
     meshes.AddMeshesOfModel(argv[1]);  // Apple: one mesh
+    // taken before the next model is added: the mirror's meshes never move
+    const mi355x::OBBtree* apple_tree = &meshes.GetMeshInfo(0).boundBoxTree;
     meshes.AddMeshesOfModel(argv[2]);  // Table_room: 46 meshes, 4 skinned
     const size_t apple = meshes.GetMeshIndexOffsetOfModel(argv[1]);
     const size_t room = meshes.GetMeshIndexOffsetOfModel(argv[2]);
     std::printf("meshes %zu apple %zu room %zu\n", meshes.size(), apple, room);
     int none = 0;
-    for (size_t i = room; i < meshes.size(); ++i) none += meshes.GetMeshInfo(i).boundBoxTree == CD_NO_MESH;
+    for (size_t i = room; i < meshes.size(); ++i) none += meshes.GetMeshInfo(i).boundBoxTree.meshId == CD_NO_MESH;
     std::printf("without trees %d\n", none);
     auto entry = [&](mi355x::Entity ent, float x) {
         mi355x::CollisionDetectionEntry e{};
-        for (int i = 0; i < 16; ++i) e.currentGlobalMatrix[i] = e.previousGlobalMatrix[i] = (i % 5 == 0) ? 1.f : 0.f;
-        e.currentGlobalMatrix[12] = x;
-        e.previousGlobalMatrix[12] = x + 0.5f;
-        e.meshId = meshes.GetMeshInfo(apple).boundBoxTree;
+        for (int i = 0; i < 16; ++i) e.currentGlobalMatrix.m[i] = e.previousGlobalMatrix.m[i] = (i % 5 == 0) ? 1.f : 0.f;
+        e.currentGlobalMatrix[3][0] = x;
+        e.previousGlobalMatrix[3][0] = x + 0.5f;
+        e.OBBtree_ptr = &meshes.GetMeshInfo(apple).boundBoxTree;
+        if (e.OBBtree_ptr != apple_tree) std::printf("moved\n");
         e.shouldCallback = true;
         e.entity = ent;
         return e;
@@ -174,3 +177,4 @@ def test_mirror_meshes_of_nodes_gltf(tmp_path):
     assert lines[1] == "without trees 4"
     pairs = [l.split() for l in lines if l.startswith("pair")]
     assert [p[1:3] for p in pairs] == [["1", "2"]] and pairs[0][4] == "1"
+    assert "moved" not in lines
