@@ -173,9 +173,11 @@ typedef struct cd_tuning {
                                    shared out by a second launch */
     uint32_t spin_polls;     /* polls an idle wavefront waits for a donation before it leaves (0: 4096) */
     uint32_t split_tris;     /* a split level keeps splitting only node pairs over this many triangles
-                                (0: 512; 1: all of them); the others go to the persistent kernel */
+                                (0: 256; 1: all of them); the others go to the persistent kernel */
     uint32_t handoff_pct;    /* work_sharing 3: the budget, percent of the mean (0: 100) */
     uint32_t resplit_levels; /* work_sharing 3: breadth-first levels over what was handed on (0: 4) */
+    uint32_t pipeline;       /* batches of node pairs a persistent wavefront keeps in flight, 1 or 2 (0: 2): with 2
+                                it issues the next batch's loads before it tests the current one (work_sharing 0) */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

@@ -3,8 +3,9 @@
 Searches split levels and the size above which a node pair keeps being
 split (work-split granularity), LDS stack entries per
 wavefront (stack depth), node pairs popped per step (lanes per step of the
-wavefront traversal), persistent wavefronts per CU and work sharing (idle
-wavefronts taking node pairs that busy ones donate), timing each candidate
+wavefront traversal), persistent wavefronts per CU, whether a wavefront
+keeps a second batch of node pairs in flight (pipeline) and work sharing
+(idle wavefronts taking node pairs that busy ones donate), timing each candidate
 with the library's own HIP events on staged queries, and caches the winner
 per (device architecture, workload size class) in tuning_gfx950.json.
 
@@ -24,11 +25,13 @@ TABLE = os.path.join(HERE, "tuning_gfx950.json")
 
 SPACE = {
     # joint keys: each value sets both fields
-    "split_levels/split_tris": [(6, 1), (8, 1), (10, 1), (12, 1), (16, 512), (24, 2048), (32, 8192)],
+    "split_levels/split_tris": [(8, 512), (12, 256), (14, 192), (16, 192), (16, 256), (16, 512), (18, 256),
+                                (20, 256), (24, 2048)],
     "stack_entries": [512, 1024],
-    "pop_width": [64],
-    "waves_per_cu": [2, 4, 8],
-    "work_sharing": [0],  # 1 and 2 are correct but have not won on any workload measured
+    "pop_width": [32, 64],
+    "waves_per_cu": [4, 5, 6, 8, 10],
+    "pipeline": [1, 2],
+    "work_sharing": [0],  # 1, 2 and 3 are correct but have not won on any workload measured
 }
 
 

@@ -142,8 +142,18 @@ struct Code {  // resolved traversal code
     uint32_t link, cnt;  // link: mesh-local child node index, or triangle offset of a leaf
     uint32_t tris;       // triangles under the box
 };
-__device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
-                                        uint32_t code) {
+// A traversal code's record words as loaded, before they are decoded: the
+// loads of a node pair are issued in one phase (load_code) and used in
+// another (finish_code), so a wavefront can have one batch's loads in flight
+// while it tests the previous batch.
+struct RawCode {
+    float bx[12];
+    float surf;
+    uint32_t a, b, left, all;
+    uint32_t code;
+};
+__device__ __forceinline__ void load_code(const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
+                                          uint32_t code, RawCode& r) {
     // Both layouts are read through selected addresses, so every lane runs
     // the same load instructions whatever kind of code it holds.
     const bool root = (code & ROOT_FLAG) != 0u;
@@ -156,10 +166,22 @@ __device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, cons
     const uint32_t* w2 = root ? &mr->nnodes : &nr->cnt;
     const uint32_t* w3 = root ? &mr->ntri : &nr->pad[2];  // left child's triangles
     const uint32_t* w4 = root ? &mr->ntri : &nr->pad[1];  // the node's triangles
-    Code r;
-    r.box = box_from12(bx);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r.bx[k] = bx[k];
     r.surf = *sf;
-    const uint32_t a = *w1, b = *w2;
+    r.a = *w1;
+    r.b = *w2;
+    r.left = *w3;
+    r.all = *w4;
+    r.code = code;
+}
+__device__ __forceinline__ Code finish_code(const RawCode& w) {
+    const bool root = (w.code & ROOT_FLAG) != 0u;
+    const uint32_t side = w.code & 1u;
+    Code r;
+    r.box = box_from12(w.bx);
+    r.surf = w.surf;
+    const uint32_t a = w.a, b = w.b;
     // a root is a leaf when its tree has no internal node; it then holds all
     // ntri triangles from offset 0.  Both readings are computed and one is
     // selected: a branch here would make the lanes wait for these loads
@@ -170,8 +192,7 @@ __device__ __forceinline__ Code resolve(const MeshRec* __restrict__ meshes, cons
     r.link = a & ~rm;
     r.cnt = (cnt_root & rm) | (cnt_node & ~rm);
     r.leaf = r.cnt != 0u;
-    const uint32_t left = *w3, all = *w4;
-    r.tris = root ? a : (side ? all - left : left);
+    r.tris = root ? a : (side ? w.all - w.left : w.left);
     return r;
 }
 
@@ -186,38 +207,64 @@ __device__ __forceinline__ void close_chunk(uint4* __restrict__ lp_out, uint64_t
         if (i < lp_cap) lp_out[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// One node pair: conservative test, then leaf pair or the two child pairs
-// (first child on top), with the reference's descent rule.
-__device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
-                                         const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
-                                         uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf,
-                                         uint32_t& cost) {
+// The loads of one node pair: the query's record and both codes' records.
+struct PairLoad {
+    float m[16];
+    float sigma;
+    uint32_t base_a, base_b, tbase_a, tbase_b;
+    RawCode A, B;
+};
+__device__ __forceinline__ void load_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
+                                          const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
+                                          PairLoad& P) {
     const QueryPre& Q = qp[qi];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) P.m[k] = Q.m[k];
+    P.sigma = Q.sigma;
+    P.base_a = Q.node_base_a;
+    P.base_b = Q.node_base_b;
+    P.tbase_a = Q.tri_base_a;
+    P.tbase_b = Q.tri_base_b;
+    load_code(meshes, nodes, ca, P.A);
+    load_code(meshes, nodes, cb, P.B);
+}
+
+// One node pair from its loads: conservative test, then leaf pair or the two
+// child pairs (first child on top), with the reference's descent rule.
+__device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint32_t& n0a, uint32_t& n0b,
+                                           uint32_t& n1a, uint32_t& n1b, uint4& leaf, uint32_t& cost) {
     M4 M;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) M.m[k] = Q.m[k];
-    const float sigma = Q.sigma;
-    const uint32_t base_a = Q.node_base_a, base_b = Q.node_base_b;
-    const uint32_t tbase_a = Q.tri_base_a, tbase_b = Q.tri_base_b;
-    const Code A = resolve(meshes, nodes, ca);
-    const Code B0 = resolve(meshes, nodes, cb);
+    for (int k = 0; k < 16; ++k) M.m[k] = P.m[k];
+    const Code A = finish_code(P.A);
+    const Code B0 = finish_code(P.B);
+    const uint32_t ca = P.A.code, cb = P.B.code;
     const Box sb = xform_box(M, B0.box);
-    const bool hit = sat_conservative(A.box, sb, sigma);
+    const bool hit = sat_conservative(A.box, sb, P.sigma);
     // Outputs are formed whatever the test says (selects, no branches), so
     // the compiler keeps every record load ahead of the test instead of
     // sinking some of them behind it into a second round trip.
     const bool both_leaf = A.leaf && B0.leaf;
     // leaf pair with arena-global triangle offsets: the narrow phase can fetch
     // triangles without first reading the query's record
-    leaf = make_uint4(qi, tbase_a + A.link, tbase_b + B0.link, A.cnt | (B0.cnt << 16));
+    leaf = make_uint4(qi, P.tbase_a + A.link, P.tbase_b + B0.link, A.cnt | (B0.cnt << 16));
     const bool split_a = !A.leaf && (B0.leaf || A.surf >= surface(sb));
-    const uint32_t ka = (base_a + A.link) << 1, kb = (base_b + B0.link) << 1;
+    const uint32_t ka = (P.base_a + A.link) << 1, kb = (P.base_b + B0.link) << 1;
     n0a = split_a ? ka : ca;
     n1a = split_a ? (ka | 1u) : ca;
     n0b = split_a ? cb : kb;
     n1b = split_a ? cb : (kb | 1u);
     cost = A.tris + B0.tris;  // size of the pair's subtrees: how much work splitting it further would share out
     return hit ? (both_leaf ? PAIR_LEAF : PAIR_SPLIT) : PAIR_NONE;
+}
+
+__device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
+                                         const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
+                                         uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf,
+                                         uint32_t& cost) {
+    PairLoad P;
+    load_pair(qp, meshes, nodes, qi, ca, cb, P);
+    return eval_loaded(P, qi, n0a, n0b, n1a, n1b, leaf, cost);
 }
 
 }  // namespace
@@ -706,6 +753,166 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
     }
 }
 
+// Pipelined persistent traversal (no work sharing).  A wavefront keeps two
+// batches of node pairs in flight: it pops the next batch and issues its
+// record loads before it tests the current one, so the loads of one batch
+// overlap the other's arithmetic instead of following it.  The next batch
+// is popped before the current one's children are pushed, which changes
+// only the order of the depth-first walk: every node pair is still tested
+// exactly once and the leaf pairs are the same set.  The loop body is
+// written twice with the two register sets' roles swapped, so no copy makes
+// the compiler wait for a batch's loads before they are needed.
+template <int CAP>
+__global__ __launch_bounds__(64) void k_dfs_pipe(const uint3* __restrict__ tasks,
+                                                 const unsigned int* __restrict__ ntasks_p, uint32_t task_cap,
+                                                 unsigned int* __restrict__ head, const QueryPre* __restrict__ qp,
+                                                 const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
+                                                 uint4* __restrict__ lp_out, uint64_t lp_cap,
+                                                 uint3* __restrict__ spill_base, uint32_t spill_cap,
+                                                 uint32_t pop_width, RunCounters* __restrict__ ctr) {
+    __shared__ uint32_t sq[CAP], sa[CAP], sb[CAP];
+    const int lane = threadIdx.x;
+    uint3* spill = spill_base + size_t(blockIdx.x) * spill_cap;
+    const uint32_t ntasks = min(*ntasks_p, task_cap);
+    // a fair share per grab, not a full stack (see k_dfs); smaller grabs late
+    // in the run cost more than they balance: every refill's returning atomic
+    // waits for the batch loads in flight
+    const uint32_t grab = max(1u, min(64u, ntasks / (2u * gridDim.x)));
+    uint32_t size = 0, nspill = 0;
+    bool tasks_done = false, overflow = false;
+    unsigned long long nsat = 0, nleaf = 0;
+    uint64_t lp_next = 0, lp_end = 0;
+    const unsigned long long below = lanes_below_b();
+
+    // Refill a low stack: the spilled bottom block first, else new tasks.
+    auto refill = [&]() {
+        if (size < 64 && nspill > 0) {
+            const uint32_t k = min(uint32_t(CAP / 2), nspill);
+            uint32_t tq = 0, ta = 0, tb = 0;
+            if (uint32_t(lane) < size) { tq = sq[lane]; ta = sa[lane]; tb = sb[lane]; }
+            __syncthreads();
+            if (uint32_t(lane) < size) { sq[k + lane] = tq; sa[k + lane] = ta; sb[k + lane] = tb; }
+            for (uint32_t i = lane; i < k; i += 64) {
+                const uint3 e = spill[nspill - k + i];
+                sq[i] = e.x; sa[i] = e.y; sb[i] = e.z;
+            }
+            nspill -= k;
+            size += k;
+            __syncthreads();
+        }
+        if (size < 64 && nspill == 0 && !tasks_done) {
+            const uint32_t want = min(64 - size, grab);
+            uint32_t t0 = 0;
+            if (lane == 0) t0 = atomicAdd(head, want);
+            t0 = __shfl(t0, 0, 64);
+            const uint32_t got = t0 < ntasks ? min(want, ntasks - t0) : 0;
+            if (got < want) tasks_done = true;
+            if (uint32_t(lane) < got) {
+                const uint32_t k = t0 + lane;  // the pseudo-random task order of k_dfs
+                const uint32_t idx = ntasks < 2147483647u ? uint32_t((uint64_t(k) * 2147483647ull) % ntasks) : k;
+                const uint3 e = tasks[idx];
+                sq[size + lane] = e.x; sa[size + lane] = e.y; sb[size + lane] = e.z;
+            }
+            size += got;
+            __syncthreads();
+        }
+    };
+    // Pop up to pop_width node pairs and issue their loads.  Every lane
+    // loads, a lane without an entry the root pair of query 0 (a valid
+    // record: the kernel runs only for at least one query), and every lane
+    // tests: with no lane-dependent branch around them, the compiler sees
+    // that each batch's loads are consumed by the next test and needs no
+    // early wait for them before issuing the following batch's.
+    auto pop = [&](PairLoad& L, uint32_t& q, bool& h) {
+        const uint32_t p = min(size, pop_width);
+        h = uint32_t(lane) < p;
+        const uint32_t idx = h ? size - p + lane : 0u;
+        const uint32_t eq = sq[idx], ea = sa[idx], eb = sb[idx];
+        q = h ? eq : 0u;
+        const uint32_t ca = h ? ea : ROOT_FLAG, cb = h ? eb : ROOT_FLAG;
+        size -= p;
+        __syncthreads();
+        load_pair(qp, meshes, nodes, q, ca, cb, L);
+        // keep the previous batch's arithmetic below these loads: hoisted
+        // above them, it would wait for its own loads before they are issued
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // Test a loaded batch: leaf pairs out, children onto the stack.
+    auto process = [&](const PairLoad& L, uint32_t q, bool h) {
+        uint32_t n0a = 0, n0b = 0, n1a = 0, n1b = 0, cost = 0;
+        uint4 leaf = make_uint4(0, 0, 0, 0);
+        const int k_all = eval_loaded(L, q, n0a, n0b, n1a, n1b, leaf, cost);
+        const int kind = h ? k_all : int(PAIR_NONE);
+        nsat += h ? 1u : 0u;
+        const unsigned long long mL = __ballot(kind == PAIR_LEAF);
+        if (mL) {
+            const uint32_t nl = uint32_t(__popcll(mL));
+            if (lp_next + nl > lp_end) {
+                close_chunk(lp_out, lp_cap, lp_next, lp_end, lane);
+                unsigned long long c0 = 0;
+                if (lane == 0) c0 = atomicAdd(&ctr->lpslots, static_cast<unsigned long long>(LP_CHUNK));
+                lp_next = __shfl(c0, 0, 64);
+                lp_end = lp_next + LP_CHUNK;
+            }
+            const uint64_t sl = lp_next + uint64_t(__popcll(mL & below));
+            if (kind == PAIR_LEAF && sl < lp_cap) lp_out[sl] = leaf;
+            lp_next += nl;
+            nleaf += nl;
+        }
+        const unsigned long long mS = __ballot(kind == PAIR_SPLIT);
+        const uint32_t ns = uint32_t(__popcll(mS));
+        if (size + 2 * ns > uint32_t(CAP)) {  // spill the bottom half of the stack to this wave's HBM area
+            const uint32_t k = CAP / 2;
+            if (nspill + k <= spill_cap) {
+                for (uint32_t i = lane; i < k; i += 64) spill[nspill + i] = make_uint3(sq[i], sa[i], sb[i]);
+                __syncthreads();
+                for (uint32_t i = 0; i + k < size; i += 64) {
+                    const uint32_t j = i + lane;
+                    uint32_t tq = 0, ta = 0, tb = 0;
+                    if (j + k < size) { tq = sq[j + k]; ta = sa[j + k]; tb = sb[j + k]; }
+                    __syncthreads();
+                    if (j + k < size) { sq[j] = tq; sa[j] = ta; sb[j] = tb; }
+                    __syncthreads();
+                }
+                nspill += k;
+                size -= k;
+            } else {
+                overflow = true;  // reported, never silently dropped
+                return;
+            }
+        }
+        if (kind == PAIR_SPLIT) {
+            const uint32_t pos = size + 2u * uint32_t(__popcll(mS & below));
+            sq[pos] = q; sa[pos] = n1a; sb[pos] = n1b;
+            sq[pos + 1] = q; sa[pos + 1] = n0a; sb[pos + 1] = n0b;
+        }
+        size += 2 * ns;
+        __syncthreads();
+    };
+    PairLoad L0, L1;
+    uint32_t q0 = 0, q1 = 0;
+    bool h0 = false, h1 = false;
+    refill();
+    pop(L0, q0, h0);
+    for (;;) {
+        refill();
+        pop(L1, q1, h1);
+        process(L0, q0, h0);
+        if (overflow || (size == 0 && nspill == 0 && tasks_done && __ballot(h1) == 0ull)) break;
+        refill();
+        pop(L0, q0, h0);
+        process(L1, q1, h1);
+        if (overflow || (size == 0 && nspill == 0 && tasks_done && __ballot(h0) == 0ull)) break;
+    }
+    close_chunk(lp_out, lp_cap, lp_next, lp_end, lane);
+    nsat = wave_sum(nsat);
+    if (lane == 0) {
+        atomicAdd(&ctr->sat, nsat);
+        if (nleaf) atomicAdd(&ctr->leafpairs, nleaf);
+        if (overflow) atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 4u);
+    }
+}
+
 // Hits of the batched narrow phase are recorded as {query, global A slot,
 // global B slot, exact sides}; k_contacts turns record i into contact i.
 // Building the contact record inside the test kernels would set their
@@ -1151,7 +1358,17 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
                       uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
                       uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
-                      hipStream_t st) {
+                      hipStream_t st, bool pipelined) {
+    if (pipelined && share == 0u && tasks && !tasks4 && !task_lo) {  // the pipelined kernel: plain task list, no sharing
+#define CD_PIPE_ARGS tasks, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, pop_width, ctr
+        switch (cap_entries) {
+            case 512: hipLaunchKernelGGL(k_dfs_pipe<512>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
+            case 2048: hipLaunchKernelGGL(k_dfs_pipe<2048>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
+            default: hipLaunchKernelGGL(k_dfs_pipe<1024>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
+        }
+#undef CD_PIPE_ARGS
+        return hipGetLastError();
+    }
 #define CD_DFS_ARGS tasks, tasks4, task_lo, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, \
                     pop_width, dq, dq_ctl, dq_cap, share, spin_polls, budget, budget_pct, dump, ndump, dump_cap, ctr
     switch (cap_entries) {

@@ -164,19 +164,20 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
 }
 
 cd_tuning effective_tuning(const cd_ctx* ctx) {
-    // defaults: the values the autotuner picks for the 4,096-query batch, which
-    // also beat the earlier defaults (8 levels, every pair split, 1,024-entry
-    // stacks) on scene frames and on the deformable batch
+    // defaults: the values the tuning sweep picks for the 4,096-query batch
+    // with the pipelined traversal (16 levels, pairs over 256 triangles split
+    // further, 512-entry stacks, 6 wavefronts per CU)
     cd_tuning t = ctx->tuning;
     if (t.split_levels == 0) t.split_levels = 16;
     if (t.split_levels > 64) t.split_levels = 64;
-    if (t.split_tris == 0) t.split_tris = 512;
+    if (t.split_tris == 0) t.split_tris = 256;
     if (t.stack_entries != 1024 && t.stack_entries != 2048) t.stack_entries = 512;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
-    if (t.waves_per_cu == 0) t.waves_per_cu = 8;
+    if (t.waves_per_cu == 0) t.waves_per_cu = 6;
     if (t.spin_polls == 0) t.spin_polls = 4096;
     if (t.handoff_pct == 0) t.handoff_pct = 100;
     if (t.resplit_levels == 0) t.resplit_levels = 4;
+    if (t.pipeline == 0 || t.pipeline > 2) t.pipeline = 2;
     return t;
 }
 
@@ -247,10 +248,12 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     HCK(ctx, launch_seed(nq, dnq, qp, F[0], &small[0], st));
     uint3* T = ctx->d_work[W_TASKS].as<uint3>();
     const uint32_t cap32 = uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull));
+    // level l holds at most nq * 2^l pairs: no more workgroups than that needs
+    auto level_bound = [&](uint32_t l) {
+        return l >= 32 ? uint64_t(cap32) : std::min<uint64_t>(cap32, uint64_t(std::max(nq, 1u)) << l);
+    };
     for (uint32_t l = 0; l < L; ++l) {  // the last level sends every split to the task list
-        // level l holds at most nq * 2^l pairs: no more workgroups than that needs
-        const uint64_t bound = l >= 32 ? uint64_t(cap32) : std::min<uint64_t>(cap32, uint64_t(std::max(nq, 1u)) << l);
-        const uint32_t lgrid = uint32_t(std::min<uint64_t>(grid, (bound + 255) / 256));
+        const uint32_t lgrid = uint32_t(std::min<uint64_t>(grid, (level_bound(l) + 255) / 256));
         HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], cap32, qp, dm, dn,
                                     ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT],
                                     l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, lgrid, st));
@@ -264,7 +267,8 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     HCK(ctx, launch_dfs(int(t.stack_entries), L ? T : F[0], nullptr, nullptr, &small[L ? TASK_COUNT : 0], cap32,
                         &small[HEAD0], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
                         ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP,
-                        share, t.spin_polls, budget, t.handoff_pct, B3[0], &small[DUMP_COUNT], dcap, ctr, st));
+                        share, t.spin_polls, budget, t.handoff_pct, B3[0], &small[DUMP_COUNT], dcap, ctr, st,
+                        t.pipeline == 2));
     if (share == 1 || share == 2)  // donations nobody claimed, [claimed, reserved), or the stacks handed off
         HCK(ctx, launch_dfs(int(t.stack_entries), nullptr, dq, &small[DQ_CTL + 1], &small[DQ_CTL], DQ_CAP,
                             &small[HEAD1], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
@@ -336,7 +340,8 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
         std::vector<uint64_t> key = {nq_cap, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
-                                     t.split_tris, t.handoff_pct, t.resplit_levels, ctx->lp_cap, ctx->contact_cap,
+                                     t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline,
+                                     ctx->lp_cap, ctx->contact_cap,
                                      ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
@@ -436,6 +441,13 @@ int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
     return rc;
 }
 int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (nq == 0) {  // nothing to launch (the kernels assume at least one query record)
+        ctx->counters = cd_counters{};
+        ctx->timings = cd_timings{};
+        *n_contacts = 0;
+        return CD_OK;
+    }
     if (mode == CD_MODE_PLAIN) return run_plain(ctx, nq, n_contacts);
     if (mode == CD_MODE_BATCHED) return run_batched(ctx, nq, n_contacts);
     return fail(ctx, CD_E_INVALID, "unknown mode");
@@ -1724,7 +1736,7 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
     if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
     if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 3 ||
-        t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT)
+        t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT || t->pipeline > 2)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;

@@ -42,7 +42,7 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
                       uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
                       uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
-                      hipStream_t st);
+                      hipStream_t st, bool pipelined = false);
 // Keeps this run's node-pair test count for the next run's budgeted pass.
 hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget, hipStream_t st);
 // defer: 2 * defer_cap entries, the float64 and the exact rerun lists;

@@ -126,6 +126,10 @@ def test_batched_self_contact_deferred_tests(gpu, orc):
     # deep, selective splitting: only pairs over split_tris triangles go on to the next level
     dict(split_levels=24, split_tris=64, stack_entries=512, pop_width=64, waves_per_cu=4),
     dict(split_levels=40, split_tris=1, stack_entries=512, pop_width=64, waves_per_cu=4),
+    # one batch of node pairs in flight per wavefront, against the two-batch default
+    dict(split_levels=16, split_tris=256, stack_entries=512, pop_width=64, waves_per_cu=6, pipeline=1),
+    dict(split_levels=0, stack_entries=2048, pop_width=16, waves_per_cu=2, pipeline=2),
+    dict(split_levels=6, stack_entries=1024, pop_width=3, waves_per_cu=10, pipeline=2),
 ])
 def test_batched_tuning_variants_agree(gpu, tuning):
     soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
