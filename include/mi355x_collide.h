@@ -178,6 +178,8 @@ typedef struct cd_tuning {
     uint32_t resplit_levels; /* work_sharing 3: breadth-first levels over what was handed on (0: 4) */
     uint32_t pipeline;       /* batches of node pairs a persistent wavefront keeps in flight, 1 or 2 (0: 2): with 2
                                 it issues the next batch's loads before it tests the current one (work_sharing 0) */
+    uint32_t split_depth;    /* split levels one launch runs, 1 or 2 (0: 2): with 2 a thread tests its pair and
+                                then both of that pair's children */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

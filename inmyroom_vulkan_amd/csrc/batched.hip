@@ -66,6 +66,43 @@ constexpr float U32 = 5.9604644775390625e-08f;  // 2^-24
 
 __device__ __forceinline__ float norm1(f3 v) { return (fabsf(v.x) + fabsf(v.y)) + fabsf(v.z); }
 
+// Fused-multiply-add forms of the products the conservative test, the box
+// transform and the descent rule of this path use.  A fused operation rounds
+// once where a product and a sum round twice, so each of these stays within
+// the bound its unfused form has (gamma_2 for a cross-product component,
+// gamma_3 for a three-term dot product or matrix row): the rounding margins
+// below and in the query set-up hold unchanged.  The plain path keeps the
+// reference's unfused operation order; this one only has to be conservative.
+__device__ __forceinline__ float dotf(f3 a, f3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ f3 crossf(f3 a, f3 b) {
+    return mk(fmaf(a.y, b.z, -(b.y * a.z)), fmaf(a.z, b.x, -(b.z * a.x)), fmaf(a.x, b.y, -(b.x * a.y)));
+}
+__device__ __forceinline__ f3 mul4f(const M4& M, f3 p) {  // M (p, 0)
+    return mk(fmaf(M.m[8], p.z, fmaf(M.m[4], p.y, M.m[0] * p.x)), fmaf(M.m[9], p.z, fmaf(M.m[5], p.y, M.m[1] * p.x)),
+              fmaf(M.m[10], p.z, fmaf(M.m[6], p.y, M.m[2] * p.x)));
+}
+__device__ __forceinline__ f3 xform_pointf(const M4& M, f3 p) {  // M (p, 1): every term rounded at most three times
+    return mk(fmaf(M.m[8], p.z, fmaf(M.m[4], p.y, fmaf(M.m[0], p.x, M.m[12]))),
+              fmaf(M.m[9], p.z, fmaf(M.m[5], p.y, fmaf(M.m[1], p.x, M.m[13]))),
+              fmaf(M.m[10], p.z, fmaf(M.m[6], p.y, fmaf(M.m[2], p.x, M.m[14]))));
+}
+__device__ __forceinline__ Box xform_boxf(const M4& M, const Box& b) {
+    Box r;
+    r.c = xform_pointf(M, b.c);
+    r.u = mul4f(M, b.u);
+    r.v = mul4f(M, b.v);
+    r.w = mul4f(M, b.w);
+    return r;
+}
+// The descent rule's measure of a box: any consistent choice of which side to
+// split keeps the traversal conservative, so the hardware square root (within
+// one ulp) stands in for the correctly rounded one.
+__device__ __forceinline__ float surfacef(const Box& b) {
+    const f3 uv = crossf(b.u, b.v), uw = crossf(b.u, b.w), vw = crossf(b.v, b.w);
+    return 2.f * ((__builtin_amdgcn_sqrtf(dotf(uv, uv)) + __builtin_amdgcn_sqrtf(dotf(uw, uw))) +
+                  __builtin_amdgcn_sqrtf(dotf(vw, vw)));
+}
+
 // Conservative separating-axis test of two boxes (centre + three half-axis
 // vectors each) on the 15 axes of Paralgram.cpp:17-173.  On an axis L the
 // boxes are disjoint when |L.T| > sum_k |L.a_k| + sum_l |L.b_l|, T = r.c - l.c,
@@ -96,33 +133,33 @@ __device__ inline bool sat_conservative(const Box& l, const Box& r, float sigma)
     const float S = norm1(T) + ((na[0] + na[1]) + na[2]) + ((nb[0] + nb[1]) + nb[2]);
     const float K = 16.f * U32 * S + sigma;
     // face normals in the reference's order: v x w, u x w, u x v
-    const f3 NA[3] = {cross(a[1], a[2]), cross(a[0], a[2]), cross(a[0], a[1])};
-    const f3 NB[3] = {cross(b[1], b[2]), cross(b[0], b[2]), cross(b[0], b[1])};
+    const f3 NA[3] = {crossf(a[1], a[2]), crossf(a[0], a[2]), crossf(a[0], a[1])};
+    const f3 NB[3] = {crossf(b[1], b[2]), crossf(b[0], b[2]), crossf(b[0], b[1])};
     const float fa[3] = {na[1] * na[2], na[0] * na[2], na[0] * na[1]};
     const float fb[3] = {nb[1] * nb[2], nb[0] * nb[2], nb[0] * nb[1]};
-    const float detA = fabsf(dot(NA[0], a[0])), detB = fabsf(dot(NB[0], b[0]));
+    const float detA = fabsf(dotf(NA[0], a[0])), detB = fabsf(dotf(NB[0], b[0]));
     float D1[3][3], D2[3][3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            D1[m][j] = fabsf(dot(NA[m], b[j]));
-            D2[m][j] = fabsf(dot(a[m], NB[j]));
+            D1[m][j] = fabsf(dotf(NA[m], b[j]));
+            D2[m][j] = fabsf(dotf(a[m], NB[j]));
         }
     bool sep = false;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        sep |= fabsf(dot(NA[m], T)) - (detA + ((D1[m][0] + D1[m][1]) + D1[m][2])) > fa[m] * K;
-        sep |= fabsf(dot(NB[m], T)) - (((D2[0][m] + D2[1][m]) + D2[2][m]) + detB) > fb[m] * K;
+        sep |= fabsf(dotf(NA[m], T)) - (detA + ((D1[m][0] + D1[m][1]) + D1[m][2])) > fa[m] * K;
+        sep |= fabsf(dotf(NB[m], T)) - (((D2[0][m] + D2[1][m]) + D2[2][m]) + detB) > fb[m] * K;
     }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const f3 X = cross(b[j], T);
+        const f3 X = crossf(b[j], T);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const float rA = D1[i == 0 ? 1 : 0][j] + D1[i == 2 ? 1 : 2][j];
             const float rB = D2[i][j == 0 ? 1 : 0] + D2[i][j == 2 ? 1 : 2];
-            sep |= fabsf(dot(a[i], X)) - (rA + rB) > (na[i] * nb[j]) * K;
+            sep |= fabsf(dotf(a[i], X)) - (rA + rB) > (na[i] * nb[j]) * K;
         }
     }
     return !sep;
@@ -239,7 +276,7 @@ __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint3
     const Code A = finish_code(P.A);
     const Code B0 = finish_code(P.B);
     const uint32_t ca = P.A.code, cb = P.B.code;
-    const Box sb = xform_box(M, B0.box);
+    const Box sb = xform_boxf(M, B0.box);
     const bool hit = sat_conservative(A.box, sb, P.sigma);
     // Outputs are formed whatever the test says (selects, no branches), so
     // the compiler keeps every record load ahead of the test instead of
@@ -248,7 +285,7 @@ __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint3
     // leaf pair with arena-global triangle offsets: the narrow phase can fetch
     // triangles without first reading the query's record
     leaf = make_uint4(qi, P.tbase_a + A.link, P.tbase_b + B0.link, A.cnt | (B0.cnt << 16));
-    const bool split_a = !A.leaf && (B0.leaf || A.surf >= surface(sb));
+    const bool split_a = !A.leaf && (B0.leaf || A.surf >= surfacef(sb));
     const uint32_t ka = (P.base_a + A.link) << 1, kb = (P.base_b + B0.link) << 1;
     n0a = split_a ? ka : ca;
     n1a = split_a ? (ka | 1u) : ca;
@@ -388,6 +425,125 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
                 dst[slot + 1] = make_uint3(qi, n1a, n1b);
             } else {
                 atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
+            }
+        }
+        __syncthreads();  // s_n and s_base are rewritten by the next iteration
+    }
+    block_add(nsat, &ctr->sat);
+    block_add(nleaf, &ctr->leafpairs);
+}
+
+// Two split levels per launch.  A thread tests its pair and, when the pair
+// splits with more than split_tris triangles under it, tests both children
+// itself (their four records loaded together) instead of leaving them to the
+// next launch; a child that splits sends its two children on to the next
+// launch when it holds more than split_tris2 triangles, else to the task
+// list.  The node pairs tested, and what becomes of each, are those of two
+// k_split_level launches with split_tris and split_tris2; only the order of
+// the outputs differs.  Half the launches, and half the count loads, entry
+// loads and slot reservations the levels wait on in turn.
+__global__ __launch_bounds__(256) void k_split_level2(const uint3* __restrict__ in, const unsigned int* __restrict__ nin,
+                                                      uint3* __restrict__ out, unsigned int* __restrict__ nout,
+                                                      uint32_t out_cap, const QueryPre* __restrict__ qp,
+                                                      const MeshRec* __restrict__ meshes,
+                                                      const NodeRec* __restrict__ nodes, uint4* __restrict__ lp_out,
+                                                      uint64_t lp_cap, uint3* __restrict__ tasks,
+                                                      unsigned int* __restrict__ ntasks, uint32_t split_tris,
+                                                      uint32_t split_tris2, RunCounters* __restrict__ ctr) {
+    const uint32_t n = min(*nin, out_cap);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    __shared__ unsigned int s_n[4][3];  // per wavefront: leaf pairs, next-launch entries, tasks
+    __shared__ unsigned long long s_base[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long below = lanes_below_b();
+    unsigned long long nsat = 0, nleaf = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t qi = 0, n0a = 0, n0b = 0, n1a = 0, n1b = 0, cost = 0;
+        int kind = PAIR_NONE;
+        uint4 leaf0 = make_uint4(0, 0, 0, 0);
+        PairLoad P;
+        if (i < n) {
+            const uint3 e = in[i];
+            qi = e.x;
+            ++nsat;
+            load_pair(qp, meshes, nodes, qi, e.y, e.z, P);
+            kind = eval_loaded(P, qi, n0a, n0b, n1a, n1b, leaf0, cost);
+        }
+        const bool deeper = kind == PAIR_SPLIT && cost > split_tris;
+        // Two output groups of two entries each (g0, g1) and two leaf slots:
+        // a pair that splits without going deeper fills g0 with its children
+        // for the task list; one that goes deeper fills group c (and leaf
+        // slot c) from its child c.
+        uint32_t g[2][4] = {{n0a, n0b, n1a, n1b}, {0, 0, 0, 0}};
+        int dest[2] = {kind == PAIR_SPLIT && !deeper ? 2 : 0, 0};  // 0 none, 1 next launch, 2 task list
+        uint4 lf[2] = {leaf0, make_uint4(0, 0, 0, 0)};
+        bool has_leaf[2] = {kind == PAIR_LEAF, false};
+        if (deeper) {
+            PairLoad C[2];
+            const uint32_t codes[2][2] = {{n0a, n0b}, {n1a, n1b}};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                C[c] = P;  // the query's record: the same for the children
+                load_code(meshes, nodes, codes[c][0], C[c].A);
+                load_code(meshes, nodes, codes[c][1], C[c].B);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                uint32_t ccost = 0;
+                const int ck = eval_loaded(C[c], qi, g[c][0], g[c][1], g[c][2], g[c][3], lf[c], ccost);
+                has_leaf[c] = ck == PAIR_LEAF;
+                dest[c] = ck == PAIR_SPLIT ? (ccost > split_tris2 ? 1 : 2) : 0;
+            }
+            nsat += 2;
+        }
+        const unsigned long long mL0 = __ballot(has_leaf[0]), mL1 = __ballot(has_leaf[1]);
+        const unsigned long long mN0 = __ballot(dest[0] == 1), mN1 = __ballot(dest[1] == 1);
+        const unsigned long long mT0 = __ballot(dest[0] == 2), mT1 = __ballot(dest[1] == 2);
+        nleaf += uint32_t(has_leaf[0]) + uint32_t(has_leaf[1]);
+        if (lane == 0) {
+            s_n[wave][0] = unsigned(__popcll(mL0) + __popcll(mL1));
+            s_n[wave][1] = 2u * unsigned(__popcll(mN0) + __popcll(mN1));
+            s_n[wave][2] = 2u * unsigned(__popcll(mT0) + __popcll(mT1));
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int c = int(threadIdx.x);
+            const unsigned int tot = (s_n[0][c] + s_n[1][c]) + (s_n[2][c] + s_n[3][c]);
+            unsigned long long b0 = 0;
+            if (tot) {
+                if (c == 0) b0 = atomicAdd(&ctr->lpslots, static_cast<unsigned long long>(tot));
+                else b0 = atomicAdd(c == 1 ? nout : ntasks, tot);
+            }
+            s_base[c] = b0;
+        }
+        __syncthreads();
+        unsigned long long first[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            unsigned long long f = s_base[c];
+            for (int w = 0; w < wave; ++w) f += s_n[w][c];
+            first[c] = f;
+        }
+        unsigned long long sl = first[0] + unsigned(__popcll(mL0 & below) + __popcll(mL1 & below));
+        unsigned long long sn = first[1] + 2u * unsigned(__popcll(mN0 & below) + __popcll(mN1 & below));
+        unsigned long long st = first[2] + 2u * unsigned(__popcll(mT0 & below) + __popcll(mT1 & below));
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (has_leaf[c]) {
+                if (sl < lp_cap) lp_out[sl] = lf[c];
+                ++sl;
+            }
+            if (dest[c] != 0) {
+                unsigned long long& slot = dest[c] == 1 ? sn : st;
+                uint3* dst = dest[c] == 1 ? out : tasks;
+                if (slot + 1 < out_cap) {
+                    dst[slot] = make_uint3(qi, g[c][0], g[c][1]);
+                    dst[slot + 1] = make_uint3(qi, g[c][2], g[c][3]);
+                } else {
+                    atomicOr(reinterpret_cast<unsigned int*>(&ctr->err), 2u);
+                }
+                slot += 2;
             }
         }
         __syncthreads();  // s_n and s_base are rewritten by the next iteration
@@ -1350,6 +1506,14 @@ hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* o
                               RunCounters* ctr, uint32_t grid, hipStream_t st) {
     hipLaunchKernelGGL(k_split_level, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
                        lp_cap, tasks, ntasks, split_tris, ctr);
+    return hipGetLastError();
+}
+hipError_t launch_split_level2(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout,
+                               uint32_t out_cap, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes,
+                               uint4* lp, uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
+                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_split_level2, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
+                       lp_cap, tasks, ntasks, split_tris, split_tris2, ctr);
     return hipGetLastError();
 }
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,

@@ -178,6 +178,7 @@ cd_tuning effective_tuning(const cd_ctx* ctx) {
     if (t.handoff_pct == 0) t.handoff_pct = 100;
     if (t.resplit_levels == 0) t.resplit_levels = 4;
     if (t.pipeline == 0 || t.pipeline > 2) t.pipeline = 2;
+    if (t.split_depth == 0 || t.split_depth > 2) t.split_depth = 2;
     return t;
 }
 
@@ -252,11 +253,22 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
     auto level_bound = [&](uint32_t l) {
         return l >= 32 ? uint64_t(cap32) : std::min<uint64_t>(cap32, uint64_t(std::max(nq, 1u)) << l);
     };
-    for (uint32_t l = 0; l < L; ++l) {  // the last level sends every split to the task list
+    // launch k takes frontier k from F[k & 1] (count small[k]); the last level sends every split to the task list
+    uint32_t k = 0;
+    for (uint32_t l = 0; l < L; ++k) {
         const uint32_t lgrid = uint32_t(std::min<uint64_t>(grid, (level_bound(l) + 255) / 256));
-        HCK(ctx, launch_split_level(F[l & 1], &small[l], F[(l + 1) & 1], &small[l + 1], cap32, qp, dm, dn,
-                                    ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT],
-                                    l + 1 == L ? 0xFFFFFFFFu : t.split_tris, ctr, lgrid, st));
+        const auto rule = [&](uint32_t level) { return level + 1 == L ? 0xFFFFFFFFu : t.split_tris; };
+        if (t.split_depth == 2 && l + 1 < L) {
+            HCK(ctx, launch_split_level2(F[k & 1], &small[k], F[(k + 1) & 1], &small[k + 1], cap32, qp, dm, dn,
+                                         ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT], rule(l),
+                                         rule(l + 1), ctr, lgrid, st));
+            l += 2;
+        } else {
+            HCK(ctx, launch_split_level(F[k & 1], &small[k], F[(k + 1) & 1], &small[k + 1], cap32, qp, dm, dn,
+                                        ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT], rule(l), ctr,
+                                        lgrid, st));
+            l += 1;
+        }
     }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[1], st));
     const uint32_t share = t.work_sharing;
@@ -340,7 +352,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
         std::vector<uint64_t> key = {nq_cap, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
-                                     t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline,
+                                     t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline, t.split_depth,
                                      ctx->lp_cap, ctx->contact_cap,
                                      ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
@@ -1736,7 +1748,8 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
     if (t->stack_entries && t->stack_entries != 512 && t->stack_entries != 1024 && t->stack_entries != 2048)
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
     if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 3 ||
-        t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT || t->pipeline > 2)
+        t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT || t->pipeline > 2 ||
+        t->split_depth > 2)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;

@@ -36,6 +36,12 @@ hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* o
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
                               RunCounters* ctr, uint32_t grid, hipStream_t st);
+// Two split levels in one launch: split_tris rules the first, split_tris2 the
+// second (0xFFFFFFFF: the second is the last split level).
+hipError_t launch_split_level2(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout,
+                               uint32_t out_cap, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes,
+                               uint4* lp, uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
+                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st);
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,
                       const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
                       const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,

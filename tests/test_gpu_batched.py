@@ -130,6 +130,10 @@ def test_batched_self_contact_deferred_tests(gpu, orc):
     dict(split_levels=16, split_tris=256, stack_entries=512, pop_width=64, waves_per_cu=6, pipeline=1),
     dict(split_levels=0, stack_entries=2048, pop_width=16, waves_per_cu=2, pipeline=2),
     dict(split_levels=6, stack_entries=1024, pop_width=3, waves_per_cu=10, pipeline=2),
+    # split levels one per launch, and two per launch with an odd count (the last level alone)
+    dict(split_levels=16, split_tris=256, stack_entries=512, waves_per_cu=6, split_depth=1),
+    dict(split_levels=15, split_tris=64, stack_entries=512, waves_per_cu=6, split_depth=2),
+    dict(split_levels=1, stack_entries=512, waves_per_cu=6, split_depth=2),
 ])
 def test_batched_tuning_variants_agree(gpu, tuning):
     soups, qa, qb, rel, _ = scenes.config3(64, seed=5, freq_cap=24)
