@@ -179,57 +179,42 @@ struct Code {  // resolved traversal code
     uint32_t link, cnt;  // link: mesh-local child node index, or triangle offset of a leaf
     uint32_t tris;       // triangles under the box
 };
-// A traversal code's record words as loaded, before they are decoded: the
-// loads of a node pair are issued in one phase (load_code) and used in
-// another (finish_code), so a wavefront can have one batch's loads in flight
-// while it tests the previous batch.
+// A traversal code's child record as loaded, before it is decoded: the loads
+// of a node pair are issued in one phase (load_code) and used in another
+// (finish_code), so a wavefront can have one batch's loads in flight while it
+// tests the previous batch.  A root's record sits in its mesh record and has
+// the layout of every other child record, so the only lane-dependent choice is
+// the record's address; the four 16-byte loads are the same for every lane.
 struct RawCode {
-    float bx[12];
-    float surf;
-    uint32_t a, b, left, all;
+    uint4 w[4];  // box (w[0..2]); surf, link, tris, toff (w[3])
     uint32_t code;
 };
 __device__ __forceinline__ void load_code(const MeshRec* __restrict__ meshes, const NodeRec* __restrict__ nodes,
                                           uint32_t code, RawCode& r) {
-    // Both layouts are read through selected addresses, so every lane runs
-    // the same load instructions whatever kind of code it holds.
     const bool root = (code & ROOT_FLAG) != 0u;
-    const uint32_t side = code & 1u;
     const MeshRec* mr = meshes + (root ? (code & ~ROOT_FLAG) : 0u);
     const NodeRec* nr = nodes + (root ? 0u : (code >> 1));
-    const float* bx = root ? mr->root : nr->box[side];
-    const float* sf = root ? &mr->root_surf : &nr->surf[side];
-    const uint32_t* w1 = root ? &mr->ntri : &nr->link[side];
-    const uint32_t* w2 = root ? &mr->nnodes : &nr->cnt;
-    const uint32_t* w3 = root ? &mr->ntri : &nr->pad[2];  // left child's triangles
-    const uint32_t* w4 = root ? &mr->ntri : &nr->pad[1];  // the node's triangles
+    const uint4* p = reinterpret_cast<const uint4*>(root ? &mr->root : &nr->c[code & 1u]);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) r.bx[k] = bx[k];
-    r.surf = *sf;
-    r.a = *w1;
-    r.b = *w2;
-    r.left = *w3;
-    r.all = *w4;
+    for (int k = 0; k < 4; ++k) r.w[k] = p[k];
     r.code = code;
 }
 __device__ __forceinline__ Code finish_code(const RawCode& w) {
-    const bool root = (w.code & ROOT_FLAG) != 0u;
-    const uint32_t side = w.code & 1u;
+    float bx[12];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bx[4 * k] = __uint_as_float(w.w[k].x);
+        bx[4 * k + 1] = __uint_as_float(w.w[k].y);
+        bx[4 * k + 2] = __uint_as_float(w.w[k].z);
+        bx[4 * k + 3] = __uint_as_float(w.w[k].w);
+    }
     Code r;
-    r.box = box_from12(w.bx);
-    r.surf = w.surf;
-    const uint32_t a = w.a, b = w.b;
-    // a root is a leaf when its tree has no internal node; it then holds all
-    // ntri triangles from offset 0.  Both readings are computed and one is
-    // selected: a branch here would make the lanes wait for these loads
-    // before the next record's loads are issued.
-    const uint32_t cnt_root = a & (0u - uint32_t(b == 0u));
-    const uint32_t cnt_node = (b >> (16 * side)) & 0xFFFFu;
-    const uint32_t rm = 0u - uint32_t(root);
-    r.link = a & ~rm;
-    r.cnt = (cnt_root & rm) | (cnt_node & ~rm);
-    r.leaf = r.cnt != 0u;
-    r.tris = root ? a : (side ? w.all - w.left : w.left);
+    r.box = box_from12(bx);
+    r.surf = __uint_as_float(w.w[3].x);
+    r.link = w.w[3].y;
+    r.leaf = (w.w[3].z & LEAF_BIT) != 0u;
+    r.tris = w.w[3].z & ~LEAF_BIT;
+    r.cnt = r.leaf ? r.tris : 0u;
     return r;
 }
 

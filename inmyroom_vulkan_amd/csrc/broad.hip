@@ -43,7 +43,7 @@ __global__ void k_sap_project(const cd_entity* __restrict__ ents, uint32_t n, co
     if (i >= n) return;
     M4 M;
     for (int k = 0; k < 16; ++k) M.m[k] = ents[i].current[k];
-    const Box P = xform_box(M, box_from12(meshes[ents[i].mesh_id].root));
+    const Box P = xform_box(M, box_from12(meshes[ents[i].mesh_id].root.box));
     const f3 axes[3] = {ax.u, ax.v, ax.w};
     for (int a = 0; a < 3; ++a) {
         float lo, hi;

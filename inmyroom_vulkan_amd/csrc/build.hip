@@ -408,22 +408,16 @@ __global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t*
     const BNode& nd = nodes[i];
     if (nd.flags & 1u) return;
     NodeRec rec;
-    uint32_t cnt = 0;
     for (int s = 0; s < 2; ++s) {
         const BNode& k = nodes[nd.kid[s]];
-        for (int t = 0; t < 12; ++t) rec.box[s][t] = k.box[t];
-        rec.surf[s] = k.surf;
-        if (k.flags & 1u) {
-            rec.link[s] = k.toff;
-            cnt |= (k.tcount & 0xFFFFu) << (16 * s);
-        } else {
-            rec.link[s] = k.pre;
-        }
+        ChildRec& c = rec.c[s];
+        for (int t = 0; t < 12; ++t) c.box[t] = k.box[t];
+        c.surf = k.surf;
+        const bool leaf = (k.flags & 1u) != 0u;
+        c.link = leaf ? k.toff : k.pre;
+        c.tris = k.tcount | (leaf ? LEAF_BIT : 0u);
+        c.toff = k.toff;
     }
-    rec.cnt = cnt;
-    rec.pad[0] = nd.toff;    // this node's subtree: triangles [toff, toff + tcount) in tree order
-    rec.pad[1] = nd.tcount;
-    rec.pad[2] = nodes[nd.kid[0]].tcount;  // of which the left child holds the first pad[2]
     out[node_base[nd.mesh] + nd.pre] = rec;
 }
 
@@ -678,8 +672,12 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
             r.tri_base = tri_base[m];
             r.ntri = counts[m];
             r.nnodes = roots[m].icount;
-            std::memcpy(r.root, roots[m].box, sizeof r.root);
-            r.root_surf = roots[m].surf;
+            r.root = ChildRec{};
+            std::memcpy(r.root.box, roots[m].box, sizeof r.root.box);
+            r.root.surf = roots[m].surf;
+            r.root.link = 0;  // node 0, or triangle 0 of a one-leaf tree
+            r.root.tris = counts[m] | (roots[m].icount == 0 ? LEAF_BIT : 0u);
+            r.root.toff = 0;
             r.depth = depth[m];
             uint32_t sb = 0;  // the largest of the mesh's copies (positive floats order as their bits)
             for (uint32_t k = 0; k < SLACK_REPS; ++k) sb = std::max(sb, slack[size_t(m) * SLACK_REPS + k]);

@@ -1038,7 +1038,7 @@ int cd_mesh_info_get(cd_ctx* ctx, uint32_t mesh, cd_mesh_info* out) {
     out->nnodes = m.nnodes;
     out->depth = m.depth;
     out->leaf_max = ctx->leaf_max;
-    std::memcpy(out->root, m.root, sizeof out->root);
+    std::memcpy(out->root, m.root.box, sizeof out->root);
     out->slack = m.slack;
     out->radius = m.radius;
     return CD_OK;
@@ -1048,7 +1048,7 @@ int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, 
     if (!ctx) return CD_E_INVALID;
     if (mesh >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "unknown mesh");
     const MeshRec& m = ctx->meshes[mesh];
-    if (root12) std::memcpy(root12, m.root, sizeof m.root);
+    if (root12) std::memcpy(root12, m.root.box, sizeof m.root.box);
     if (m.nnodes && (node_obbs || node_links)) {
         std::vector<NodeRec> recs(m.nnodes);
         HCK(ctx, hipMemcpyAsync(recs.data(), ctx->d_nodes.as<NodeRec>() + m.node_base, m.nnodes * sizeof(NodeRec),
@@ -1056,14 +1056,14 @@ int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, 
         HCK(ctx, hipStreamSynchronize(ctx->stream));
         for (uint32_t i = 0; i < m.nnodes; ++i) {
             if (node_obbs) {
-                std::memcpy(node_obbs + size_t(i) * 24, recs[i].box[0], 12 * sizeof(float));
-                std::memcpy(node_obbs + size_t(i) * 24 + 12, recs[i].box[1], 12 * sizeof(float));
+                std::memcpy(node_obbs + size_t(i) * 24, recs[i].c[0].box, 12 * sizeof(float));
+                std::memcpy(node_obbs + size_t(i) * 24 + 12, recs[i].c[1].box, 12 * sizeof(float));
             }
             if (node_links) {
-                node_links[size_t(i) * 4 + 0] = recs[i].link[0];
-                node_links[size_t(i) * 4 + 1] = recs[i].link[1];
-                node_links[size_t(i) * 4 + 2] = recs[i].cnt & 0xFFFFu;
-                node_links[size_t(i) * 4 + 3] = recs[i].cnt >> 16;
+                node_links[size_t(i) * 4 + 0] = recs[i].c[0].link;
+                node_links[size_t(i) * 4 + 1] = recs[i].c[1].link;
+                node_links[size_t(i) * 4 + 2] = child_cnt(recs[i].c[0]);
+                node_links[size_t(i) * 4 + 3] = child_cnt(recs[i].c[1]);
             }
         }
     }

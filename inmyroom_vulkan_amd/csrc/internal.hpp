@@ -12,31 +12,46 @@
 
 namespace cdk {
 
-// One internal OBB-tree node, 128 B (one L2 line).  Same content as the
-// reference's OBBtreeNode (include/Geometry/OBBtree.h:61-75) plus the surface
-// of each child box, which the traversal's descent rule compares
-// (OBBtree.cpp:426) and which is a pure function of the stored box.
+// One child box of an internal OBB-tree node as the traversals read it: 64 B,
+// four 16-byte loads from one half of a 128-byte line.  A node's two child
+// records hold the content of the reference's OBBtreeNode
+// (include/Geometry/OBBtree.h:61-75) plus each child box's surface, which the
+// traversal's descent rule compares (OBBtree.cpp:426) and which is a pure
+// function of the stored box.
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+struct alignas(64) ChildRec {
+    float box[12];   // centre, u, v, w
+    float surf;      // surface(box)
+    uint32_t link;   // internal child: mesh-local node index; leaf: mesh-local triangle offset (= toff)
+    uint32_t tris;   // triangles under the child, LEAF_BIT set for a leaf
+    uint32_t toff;   // the child's triangles: [toff, toff + tris) in tree order, mesh-local
+};
+static_assert(sizeof(ChildRec) == 64, "child record must stay 64 bytes");
+__host__ __device__ inline bool child_leaf(const ChildRec& c) { return (c.tris & LEAF_BIT) != 0u; }
+__host__ __device__ inline uint32_t child_tris(const ChildRec& c) { return c.tris & ~LEAF_BIT; }
+// triangles of a leaf child, 0 for an internal one
+__host__ __device__ inline uint32_t child_cnt(const ChildRec& c) { return child_leaf(c) ? child_tris(c) : 0u; }
+
+// One internal OBB-tree node: its left and right child, 128 B (one L2 line).
 struct alignas(128) NodeRec {
-    float box[2][12];   // left, right child boxes: centre, u, v, w
-    uint32_t link[2];   // child node index (count 0) or mesh-local triangle offset
-    uint32_t cnt;       // left count | right count << 16 (0 = internal child)
-    float surf[2];      // surface(box[0]), surface(box[1])
-    uint32_t pad[3];    // [0], [1]: this node's own triangle range (offset, count); [2]: the left child's count
+    ChildRec c[2];
 };
 static_assert(sizeof(NodeRec) == 128, "node record must stay one 128-byte line");
 
-// Mesh table entry (device and host).
-struct MeshRec {
+// Mesh table entry (device and host).  The root box is kept as a child
+// record (link 0: the tree's node 0, or triangle offset 0 when the whole tree
+// is one leaf), so a traversal reads a root exactly as it reads any child.
+struct alignas(64) MeshRec {
+    ChildRec root;        // root OBB (OBBtree.h:129)
     uint32_t node_base;   // first NodeRec of this mesh in the node arena
     uint32_t tri_base;    // first triangle of this mesh in the triangle arena (tree order)
     uint32_t ntri, nnodes;
-    float root[12];       // root OBB (OBBtree.h:129)
-    float root_surf;
     uint32_t depth;       // internal levels below the root
     float slack;          // max distance a vertex lies outside a box meant to hold it
     float radius;         // max |vertex|
+    uint32_t pad[9];
 };
-static_assert(sizeof(MeshRec) == 80, "mesh record layout");
+static_assert(sizeof(MeshRec) == 128, "mesh record layout");
 
 // Device copy of cd_query.
 struct QueryRec {

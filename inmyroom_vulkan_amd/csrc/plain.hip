@@ -60,33 +60,21 @@ struct TreeView {
     const NodeRec* nodes;  // this mesh's nodes (arena + node_base)
 };
 
-__device__ __forceinline__ void code_box(const TreeView& t, uint32_t code, Box& b, float& surf) {
-    if (code == ROOT_CODE) {
-        b = box_from12(t.m->root);
-        surf = t.m->root_surf;
-    } else {
-        const NodeRec& n = t.nodes[code >> 1];
-        b = box_from12(n.box[code & 1]);
-        surf = n.surf[code & 1];
-    }
+__device__ __forceinline__ const ChildRec& code_rec(const TreeView& t, uint32_t code) {
+    return code == ROOT_CODE ? t.m->root : t.nodes[code >> 1].c[code & 1];
 }
-// leaf?  If so, triangle offset/count; else the child node index.
+__device__ __forceinline__ void code_box(const TreeView& t, uint32_t code, Box& b, float& surf) {
+    const ChildRec& c = code_rec(t, code);
+    b = box_from12(c.box);
+    surf = c.surf;
+}
+// leaf?  If so, triangle offset/count; else the child node index (0 for the
+// root, the tree's node 0).
 __device__ __forceinline__ bool code_leaf(const TreeView& t, uint32_t code, uint32_t& link, uint32_t& cnt) {
-    if (code == ROOT_CODE) {
-        if (t.m->nnodes == 0) {
-            link = 0;
-            cnt = t.m->ntri;
-            return true;
-        }
-        link = 0;
-        cnt = 0;
-        return false;
-    }
-    const NodeRec& n = t.nodes[code >> 1];
-    const uint32_t side = code & 1;
-    link = n.link[side];
-    cnt = (n.cnt >> (16 * side)) & 0xFFFFu;
-    return cnt != 0;
+    const ChildRec& c = code_rec(t, code);
+    link = c.link;
+    cnt = child_cnt(c);
+    return child_leaf(c);
 }
 
 __global__ __launch_bounds__(64) void k_traverse_plain(const QueryRec* __restrict__ qs, uint32_t nq,

@@ -86,17 +86,9 @@ __device__ __forceinline__ void item_range(const MeshRec& m, const NodeRec* node
         cnt = m.ntri;
         return;
     }
-    const NodeRec& n = nodes[m.node_base + (code >> 1)];
-    const uint32_t side = code & 1;
-    const uint32_t c = (n.cnt >> (16 * side)) & 0xFFFFu;
-    if (c) {
-        off = n.link[side];
-        cnt = c;
-    } else {
-        const NodeRec& k = nodes[m.node_base + n.link[side]];
-        off = k.pad[0];
-        cnt = k.pad[1];
-    }
+    const ChildRec& c = nodes[m.node_base + (code >> 1)].c[code & 1];
+    off = c.toff;
+    cnt = child_tris(c);
 }
 
 // Wavefront reductions (butterfly: min and max are the same on every lane;
@@ -173,14 +165,9 @@ __device__ inline void axes_of(double xx, double yy, double zz, double xy, doubl
 }
 __device__ inline void store_box(MeshRec& m, NodeRec* nodes, uint32_t code, const Box& box) {
     const float sf = surface(box);
-    if (code == ROOT_CODE) {
-        box_to12(box, m.root);
-        m.root_surf = sf;
-    } else {
-        NodeRec& n = nodes[m.node_base + (code >> 1)];
-        box_to12(box, n.box[code & 1]);
-        n.surf[code & 1] = sf;
-    }
+    ChildRec& c = code == ROOT_CODE ? m.root : nodes[m.node_base + (code >> 1)].c[code & 1];
+    box_to12(box, c.box);
+    c.surf = sf;
 }
 // How far a point lies outside a box (distance along each side beyond the
 // box), in FP64 from the stored float box.
@@ -244,7 +231,7 @@ __device__ inline void post_slack(unsigned int* slack_bits, unsigned int* radius
     if (root) atomicMax(&radius_bits[mesh], __float_as_uint(rf));
 }
 __device__ inline const float* box_ptr(const MeshRec& m, const NodeRec* nodes, uint32_t code) {
-    return code == ROOT_CODE ? m.root : nodes[m.node_base + (code >> 1)].box[code & 1];
+    return code == ROOT_CODE ? m.root.box : nodes[m.node_base + (code >> 1)].c[code & 1].box;
 }
 
 // Triangle range of every item (tree order, mesh-local).

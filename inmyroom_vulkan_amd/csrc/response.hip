@@ -175,7 +175,7 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
     C.m[15] = C.m[15] - 0.f;
     Hit best = {false, false, __builtin_huge_valf(), 0.f, 0.f, 0xFFFFFFFFu};
     float r0, r1;
-    if (!(ray_box(dir, xform_box(C, box_from12(m.root)), r0, r1) && r1 >= 0.f)) return best;
+    if (!(ray_box(dir, xform_box(C, box_from12(m.root.box)), r0, r1) && r1 >= 0.f)) return best;
     if (m.nnodes == 0) {  // the root is a leaf
         leaf_hits(tris, m, C, dir, 0, m.ntri, best);
         return best;
@@ -186,28 +186,28 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
     for (;;) {
         const NodeRec& n = nb[node];
         float l0, l1, q0, q1;
-        const bool lh = ray_box(dir, xform_box(C, box_from12(n.box[0])), l0, l1) && l1 >= 0.f;
-        const bool rh = ray_box(dir, xform_box(C, box_from12(n.box[1])), q0, q1) && q1 >= 0.f;
+        const bool lh = ray_box(dir, xform_box(C, box_from12(n.c[0].box)), l0, l1) && l1 >= 0.f;
+        const bool rh = ray_box(dir, xform_box(C, box_from12(n.c[1].box)), q0, q1) && q1 >= 0.f;
         if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
             atomicOr(err, 1ull);
             best.hit = false;
             break;
         }
-        const uint32_t cl = n.cnt & 0xFFFFu, cr = n.cnt >> 16;
+        const uint32_t cl = child_cnt(n.c[0]), cr = child_cnt(n.c[1]);
         // the farther below the nearer (Ray.cpp:175-199); order set by the entry distances alone
         const bool left_first = !(lh && rh) || l0 < q0;
         if (lh && rh) {
             if (left_first) {
-                S.put(sp++, n.link[1], cr, q0);
-                S.put(sp++, n.link[0], cl, l0);
+                S.put(sp++, n.c[1].link, cr, q0);
+                S.put(sp++, n.c[0].link, cl, l0);
             } else {
-                S.put(sp++, n.link[0], cl, l0);
-                S.put(sp++, n.link[1], cr, q0);
+                S.put(sp++, n.c[0].link, cl, l0);
+                S.put(sp++, n.c[1].link, cr, q0);
             }
         } else if (lh) {
-            S.put(sp++, n.link[0], cl, l0);
+            S.put(sp++, n.c[0].link, cl, l0);
         } else if (rh) {
-            S.put(sp++, n.link[1], cr, q0);
+            S.put(sp++, n.c[1].link, cr, q0);
         }
         // next internal node to enter; leaves are tested as they are popped
         bool found = false;
