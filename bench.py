@@ -83,6 +83,15 @@ def copy_probe(nbytes: int, reps: int = 3) -> None:
     log(f"copy probe: {reps} x {nbytes} bytes device to device")
 
 
+def record_mismatches(got, want) -> int:
+    """Contact records present in one set and not the other, comparing every
+    field's bytes (query, triangle slots, flags, segment, normals)."""
+    def keyed(c):
+        c = np.sort(c, order=["query", "tri_a", "tri_b"])
+        return {bytes(r) for r in c.view(np.uint8).reshape(len(c), -1)} if len(c) else set()
+    return len(keyed(got) ^ keyed(want))
+
+
 PMC_SUMMARY = "profiles/traversal_narrow_4096q_pmc.md"
 
 
@@ -121,7 +130,7 @@ def main() -> int:
               f"across GPUs by design, but only the one-GPU figure is measured here).", file=sys.stderr)
         return 2
 
-    from inmyroom_vulkan_amd import MODE_BATCHED, MODE_PLAIN, QUERY_DTYPE, Collider, autotune
+    from inmyroom_vulkan_amd import MODE_BATCHED, MODE_PLAIN, MODE_REFERENCE, QUERY_DTYPE, Collider, autotune
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -195,6 +204,27 @@ def main() -> int:
                            batched_contacts["tri_b"].tolist()))
     diff_plain = len(batched_keys ^ plain_keys)
 
+    # ---- reference semantics on the batched machinery (same tuning, graph replays) ----
+    ref_eager = []
+    gpu.set_tuning(**dict(tuning, use_graph=0))
+    for _ in range(3):
+        gpu.run_staged(MODE_REFERENCE)
+        ref_eager.append(gpu.timings())
+    ref_counters = gpu.counters()
+    ref_contacts = gpu.fetch()
+    gpu.set_tuning(**tuning)
+    for _ in range(args.warmup):
+        gpu.run_staged(MODE_REFERENCE)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.run_staged(MODE_REFERENCE)
+    ref_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    ref_mismatch = record_mismatches(ref_contacts, plain_contacts)
+    ref_count_diff = {k: int(ref_counters[k]) - int(plain_counters[k]) for k in ("sat_tests", "leaf_pairs",
+                                                                                 "tri_tests", "contacts")}
+    log(f"reference mode: {ref_ms:.3f} ms/step, {ref_mismatch} contact records differ from the plain path's, "
+        f"count differences {ref_count_diff}")
+
     # ---- roofline of the dominant kernel ----
     trav_b, narrow_b = algorithmic_bytes(plain_counters, lp, args.nq)
     med = lambda k: float(np.median([e[k] for e in eager]))
@@ -231,6 +261,12 @@ def main() -> int:
         "plain_queries_per_s": round(plain_qps, 1),
         "plain_tritests_per_s": round(plain_counters["tri_tests"] / (np.median(plain_ms) * 1e-3), 1),
         "speedup_batched_vs_plain": round(value / plain_qps, 2),
+        "reference_mode": {"queries_per_s": round(args.nq / (ref_ms * 1e-3), 1), "ms_per_step": round(ref_ms, 4),
+                           "speedup_vs_plain": round(args.nq / (ref_ms * 1e-3) / plain_qps, 2),
+                           "kernel_ms": {"traversal": round(float(np.median([e["traverse_ms"] for e in ref_eager])), 4),
+                                         "narrow": round(float(np.median([e["narrow_ms"] for e in ref_eager])), 4)},
+                           "contact_records_differing_from_plain": ref_mismatch,
+                           "count_differences_from_plain": ref_count_diff},
         "pairset_diff_batched_vs_plain": diff_plain,
         "reference_counts": {k: int(plain_counters[k]) for k in ("sat_tests", "leaf_pairs", "tri_tests", "contacts")},
         "batched_counts": {k: int(batched_counters[k]) for k in ("sat_tests", "leaf_pairs", "tri_tests", "contacts",

@@ -213,7 +213,7 @@ def config1(args):
 
 
 def config2(args):
-    from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, Collider, scenes
+    from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, MODE_REFERENCE, Collider, scenes
 
     sc = scenes.Scene(n_entities=512, seed=2)
     gpu = Collider(leaf_max=4)
@@ -229,7 +229,8 @@ def config2(args):
         sc.step()
     out = {}
     # --plain-steps 0 leaves the plain mode out (profiles of the batched frame)
-    modes = (("plain", MODE_PLAIN),) * (args.plain_steps > 0) + (("batched", MODE_BATCHED),)
+    modes = (("plain", MODE_PLAIN),) * (args.plain_steps > 0) + (("reference", MODE_REFERENCE),
+                                                                 ("batched", MODE_BATCHED))
     for name, mode in modes:
         for f in frames[: args.warmup]:
             gpu.frame(f, mode)
@@ -261,9 +262,9 @@ def config2(args):
     # every frame's pairs and contacts in both modes (for the mismatch counts),
     # and the batched traversal's device time from eager runs (events around
     # the kernels) against the reference's node-pair tests from the plain frames
-    keys = {"plain": [], "batched": []}
+    keys = {"plain": [], "reference": [], "batched": []}
     trav_bytes = trav_ms = 0.0
-    for name, mode in (("plain", MODE_PLAIN), ("batched", MODE_BATCHED)):
+    for name, mode in (("plain", MODE_PLAIN), ("reference", MODE_REFERENCE), ("batched", MODE_BATCHED)):
         gpu.set_tuning(**dict(tuning, use_graph=0))
         for f in frames:
             p, c = gpu.frame(f, mode)
@@ -276,7 +277,7 @@ def config2(args):
             if name == "plain":
                 cnt = gpu.counters()
                 trav_bytes += 64.0 * cnt["broad_pairs"] + 96.0 * cnt["sat_tests"]
-            else:
+            elif name == "batched":
                 trav_ms += gpu.timings()["traverse_ms"]
     gpu.set_tuning(**tuning)
     ach = trav_bytes / (trav_ms * 1e-3) / 1e9
@@ -316,6 +317,7 @@ def config2(args):
         log(f"config 2: oracle over the 100 frames, both semantics: {time.time() - t:.1f}s")
         mism = {"frames_checked": len(frames),
                 "plain_vs_reference_semantics": sum(len(a ^ b) for a, b in zip(keys["plain"], ref)),
+                "reference_mode_vs_reference_semantics": sum(len(a ^ b) for a, b in zip(keys["reference"], ref)),
                 "batched_vs_exact_oracle": sum(len(a ^ b) for a, b in zip(keys["batched"], exact)),
                 "plain_vs_exact_oracle": sum(len(a ^ b) for a, b in zip(keys["plain"], exact))}
     ms = out["batched"]["ms_per_frame"]

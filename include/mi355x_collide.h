@@ -36,9 +36,14 @@ extern "C" {
 /* Plain: reference semantics bit for bit (Möller test with its 1e-6 snap,
  * float SAT), one lane per query.  Batched: persistent load-balanced
  * traversal, exact predicates (float filter, double fallback, exact
- * expansion as last resort), conservative SAT.                              */
+ * expansion as last resort), conservative SAT.  Reference: the plain mode's
+ * tests on the batched mode's machinery (split levels, persistent
+ * traversal, dense narrow phase): the plain mode's contacts and counts, bit
+ * for bit, in another order, at the batched mode's speed.  The triangle-pair
+ * stream (cd_tritri_batch) runs the plain test for it.                      */
 #define CD_MODE_PLAIN 0u
 #define CD_MODE_BATCHED 1u
+#define CD_MODE_REFERENCE 2u
 
 /* ---- records --------------------------------------------------------------- */
 typedef struct cd_ctx cd_ctx;

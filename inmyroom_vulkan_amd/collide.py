@@ -25,6 +25,7 @@ CD_E_FORMAT = -6
 CD_NO_MESH = 0xFFFFFFFF
 MODE_PLAIN = 0
 MODE_BATCHED = 1
+MODE_REFERENCE = 2  # the plain mode's tests and results on the batched mode's machinery
 
 # Symbols the header declares (checked by the CPU tests).
 EXPORTS = (

@@ -251,8 +251,13 @@ __device__ __forceinline__ void load_pair(const QueryPre* __restrict__ qp, const
     load_code(meshes, nodes, cb, P.B);
 }
 
-// One node pair from its loads: conservative test, then leaf pair or the two
+// One node pair from its loads: the box test, then leaf pair or the two
 // child pairs (first child on top), with the reference's descent rule.
+// REF: the reference's float test, box transform and surface (the plain
+// path's, operation for operation), so the node pairs visited and the leaf
+// pairs found are the reference's; else the conservative test of the
+// exact-predicate path.
+template <bool REF>
 __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint32_t& n0a, uint32_t& n0b,
                                            uint32_t& n1a, uint32_t& n1b, uint4& leaf, uint32_t& cost) {
     M4 M;
@@ -261,8 +266,15 @@ __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint3
     const Code A = finish_code(P.A);
     const Code B0 = finish_code(P.B);
     const uint32_t ca = P.A.code, cb = P.B.code;
-    const Box sb = xform_boxf(M, B0.box);
-    const bool hit = sat_conservative(A.box, sb, P.sigma);
+    Box sb;
+    bool hit;
+    if constexpr (REF) {
+        sb = xform_box(M, B0.box);
+        hit = sat(A.box, sb);
+    } else {
+        sb = xform_boxf(M, B0.box);
+        hit = sat_conservative(A.box, sb, P.sigma);
+    }
     // Outputs are formed whatever the test says (selects, no branches), so
     // the compiler keeps every record load ahead of the test instead of
     // sinking some of them behind it into a second round trip.
@@ -270,7 +282,7 @@ __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint3
     // leaf pair with arena-global triangle offsets: the narrow phase can fetch
     // triangles without first reading the query's record
     leaf = make_uint4(qi, P.tbase_a + A.link, P.tbase_b + B0.link, A.cnt | (B0.cnt << 16));
-    const bool split_a = !A.leaf && (B0.leaf || A.surf >= surfacef(sb));
+    const bool split_a = !A.leaf && (B0.leaf || A.surf >= (REF ? surface(sb) : surfacef(sb)));
     const uint32_t ka = (P.base_a + A.link) << 1, kb = (P.base_b + B0.link) << 1;
     n0a = split_a ? ka : ca;
     n1a = split_a ? (ka | 1u) : ca;
@@ -280,13 +292,14 @@ __device__ __forceinline__ int eval_loaded(const PairLoad& P, uint32_t qi, uint3
     return hit ? (both_leaf ? PAIR_LEAF : PAIR_SPLIT) : PAIR_NONE;
 }
 
+template <bool REF>
 __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const MeshRec* __restrict__ meshes,
                                          const NodeRec* __restrict__ nodes, uint32_t qi, uint32_t ca, uint32_t cb,
                                          uint32_t& n0a, uint32_t& n0b, uint32_t& n1a, uint32_t& n1b, uint4& leaf,
                                          uint32_t& cost) {
     PairLoad P;
     load_pair(qp, meshes, nodes, qi, ca, cb, P);
-    return eval_loaded(P, qi, n0a, n0b, n1a, n1b, leaf, cost);
+    return eval_loaded<REF>(P, qi, n0a, n0b, n1a, n1b, leaf, cost);
 }
 
 }  // namespace
@@ -337,6 +350,7 @@ __global__ void k_seed(const unsigned int* __restrict__ nq_p, const QueryPre* __
     if (i == 0) *count = nq;
 }
 
+template <bool REF>
 __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ in, const unsigned int* __restrict__ nin,
                                                      uint3* __restrict__ out, unsigned int* __restrict__ nout,
                                                      uint32_t out_cap, const QueryPre* __restrict__ qp,
@@ -366,7 +380,7 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
             const uint3 e = in[i];
             qi = e.x;
             ++nsat;
-            kind = eval_pair(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf, cost);
+            kind = eval_pair<REF>(qp, meshes, nodes, qi, e.y, e.z, n0a, n0b, n1a, n1b, leaf, cost);
         }
         const bool go_on = cost > split_tris;
         const bool is_leaf = kind == PAIR_LEAF;
@@ -427,6 +441,7 @@ __global__ __launch_bounds__(256) void k_split_level(const uint3* __restrict__ i
 // k_split_level launches with split_tris and split_tris2; only the order of
 // the outputs differs.  Half the launches, and half the count loads, entry
 // loads and slot reservations the levels wait on in turn.
+template <bool REF>
 __global__ __launch_bounds__(256) void k_split_level2(const uint3* __restrict__ in, const unsigned int* __restrict__ nin,
                                                       uint3* __restrict__ out, unsigned int* __restrict__ nout,
                                                       uint32_t out_cap, const QueryPre* __restrict__ qp,
@@ -453,7 +468,7 @@ __global__ __launch_bounds__(256) void k_split_level2(const uint3* __restrict__ 
             qi = e.x;
             ++nsat;
             load_pair(qp, meshes, nodes, qi, e.y, e.z, P);
-            kind = eval_loaded(P, qi, n0a, n0b, n1a, n1b, leaf0, cost);
+            kind = eval_loaded<REF>(P, qi, n0a, n0b, n1a, n1b, leaf0, cost);
         }
         const bool deeper = kind == PAIR_SPLIT && cost > split_tris;
         // Two output groups of two entries each (g0, g1) and two leaf slots:
@@ -476,7 +491,7 @@ __global__ __launch_bounds__(256) void k_split_level2(const uint3* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 uint32_t ccost = 0;
-                const int ck = eval_loaded(C[c], qi, g[c][0], g[c][1], g[c][2], g[c][3], lf[c], ccost);
+                const int ck = eval_loaded<REF>(C[c], qi, g[c][0], g[c][1], g[c][2], g[c][3], lf[c], ccost);
                 has_leaf[c] = ck == PAIR_LEAF;
                 dest[c] = ck == PAIR_SPLIT ? (ccost > split_tris2 ? 1 : 2) : 0;
             }
@@ -566,7 +581,7 @@ __device__ __forceinline__ unsigned int coherent_read(unsigned int* p) {
     return atomicAdd(p, zero);
 }
 
-template <int CAP>
+template <int CAP, bool REF>
 __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, const uint4* __restrict__ tasks4,
                                             const unsigned int* __restrict__ task_lo_p,
                                             const unsigned int* __restrict__ ntasks_p, uint32_t task_cap,
@@ -836,7 +851,7 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
         if (have) {
             ++nsat;
             uint32_t cost = 0;
-            kind = eval_pair(qp, meshes, nodes, qi, ca, cb, n0a, n0b, n1a, n1b, leaf, cost);
+            kind = eval_pair<REF>(qp, meshes, nodes, qi, ca, cb, n0a, n0b, n1a, n1b, leaf, cost);
         }
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
         if (mL) {
@@ -903,7 +918,7 @@ __global__ __launch_bounds__(64) void k_dfs(const uint3* __restrict__ tasks, con
 // exactly once and the leaf pairs are the same set.  The loop body is
 // written twice with the two register sets' roles swapped, so no copy makes
 // the compiler wait for a batch's loads before they are needed.
-template <int CAP>
+template <int CAP, bool REF>
 __global__ __launch_bounds__(64) void k_dfs_pipe(const uint3* __restrict__ tasks,
                                                  const unsigned int* __restrict__ ntasks_p, uint32_t task_cap,
                                                  unsigned int* __restrict__ head, const QueryPre* __restrict__ qp,
@@ -982,7 +997,7 @@ __global__ __launch_bounds__(64) void k_dfs_pipe(const uint3* __restrict__ tasks
     auto process = [&](const PairLoad& L, uint32_t q, bool h) {
         uint32_t n0a = 0, n0b = 0, n1a = 0, n1b = 0, cost = 0;
         uint4 leaf = make_uint4(0, 0, 0, 0);
-        const int k_all = eval_loaded(L, q, n0a, n0b, n1a, n1b, leaf, cost);
+        const int k_all = eval_loaded<REF>(L, q, n0a, n0b, n1a, n1b, leaf, cost);
         const int kind = h ? k_all : int(PAIR_NONE);
         nsat += h ? 1u : 0u;
         const unsigned long long mL = __ballot(kind == PAIR_LEAF);
@@ -1260,6 +1275,117 @@ __global__ __launch_bounds__(256) void k_narrow_exact(const uint4* __restrict__ 
     block_add(ntt, &ctr->tritests);
 }
 
+// Narrow phase with the reference's semantics over the leaf pairs of the
+// batched traversal (CreateUncollideRays.cpp:74-115).  The leaf pairs of a
+// wavefront's 64 are expanded densely to triangle tests over its lanes (as
+// k_narrow_exact does).  Each lane runs the plane stage of the reference's
+// Möller test (tri_tri_planes, the very code tri_tri_isect starts with) on the
+// B triangle moved by the query's matrix in the reference's operation order;
+// the few pairs it leaves open queue in LDS and are run through the whole
+// test 64 at a time (ref_batch), which recomputes the plane stage with the
+// same result and then writes a contact record (mesh-local triangle slots,
+// planes, outside flags, segment) for each non-coplanar hit.  The tests, and
+// so the contacts and counts, are the plain path's; only their order differs.
+__device__ __forceinline__ void ref_batch(uint4* q, uint32_t& n, const QueryPre* __restrict__ qp,
+                                          const float* __restrict__ tris, cd_contact* __restrict__ out, uint64_t cap,
+                                          RunCounters* __restrict__ ctr, unsigned long long& nub, int lane,
+                                          unsigned long long below) {
+    const uint32_t m = min(n, 64u);
+    const bool have = uint32_t(lane) < m;
+    uint4 e = make_uint4(0, 0, 0, 0);
+    if (have) e = q[n - m + uint32_t(lane)];
+    n -= m;
+    __threadfence_block();  // read before the slots are written again
+    bool hit = false;
+    cd_contact c;
+    if (have) {
+        const QueryPre& Q = qp[e.x];
+        f3 A0, A1, A2, B0, B1, B2;
+        narrow_load(Q, tris, e.y, e.z, A0, A1, A2, B0, B1, B2);
+        f3 sp, tp;
+        const uint32_t res = tri_tri_isect(A0, A1, A2, B0, B1, B2, sp, tp);
+        nub += (res & TT_UB) ? 1u : 0u;
+        hit = (res & TT_HIT) && !(res & TT_COPLANAR);
+        if (hit) {
+            const PlaneF pa = plane_of(A0, A1, A2), pb = plane_of(B0, B1, B2);
+            c.query = e.x;
+            c.tri_a = e.y - Q.tri_base_a;
+            c.tri_b = e.z - Q.tri_base_b;
+            c.flags = outside_flags(pa, pb, A0, A1, A2, B0, B1, B2);
+            c.s[0] = sp.x; c.s[1] = sp.y; c.s[2] = sp.z;
+            c.t[0] = tp.x; c.t[1] = tp.y; c.t[2] = tp.z;
+            c.na[0] = pa.n.x; c.na[1] = pa.n.y; c.na[2] = pa.n.z;
+            c.nb[0] = pb.n.x; c.nb[1] = pb.n.y; c.nb[2] = pb.n.z;
+        }
+    }
+    const unsigned long long mh = __ballot(hit);
+    if (mh) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&ctr->contacts, static_cast<unsigned long long>(__popcll(mh)));
+        const unsigned long long slot = __shfl(base, 0, 64) + static_cast<unsigned long long>(__popcll(mh & below));
+        if (hit && slot < cap) out[slot] = c;
+    }
+}
+__global__ __launch_bounds__(256) void k_narrow_ref(const uint4* __restrict__ lps,
+                                                    const unsigned long long* __restrict__ nlp_p, uint64_t lp_cap,
+                                                    const QueryPre* __restrict__ qp, const float* __restrict__ tris,
+                                                    cd_contact* __restrict__ out, uint64_t cap,
+                                                    RunCounters* __restrict__ ctr) {
+    __shared__ uint4 s_q[4][NQCAP];
+    const uint64_t nlp = min(*nlp_p, lp_cap);
+    const int lane = threadIdx.x & 63;
+    uint4* q = s_q[threadIdx.x >> 6];
+    uint32_t nqd = 0;
+    const unsigned long long below = lanes_below_b();
+    const uint64_t wave = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+    unsigned long long ntt = 0, nub = 0;
+    for (uint64_t chunk = wave; chunk * 64 < nlp; chunk += nwaves) {
+        const uint64_t li = chunk * 64 + lane;
+        uint4 lp = make_uint4(0, 0, 0, 0);
+        if (li < nlp) lp = lps[li];
+        const uint32_t cnt = (lp.w & 0xFFFFu) * (lp.w >> 16);
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const uint32_t excl = incl - cnt;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            int k = 0;  // the leaf pair holding test t: the last with excl <= t
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = k + step;
+                const uint32_t oc = __shfl(excl, cand & 63, 64);
+                if (cand < 64 && oc <= t) k = cand;
+            }
+            const uint32_t kq = __shfl(lp.x, k, 64), ka = __shfl(lp.y, k, 64), kb = __shfl(lp.z, k, 64);
+            const uint32_t kw = __shfl(lp.w, k, 64), ko = __shfl(excl, k, 64);
+            bool open = false;
+            uint32_t ga = 0, gb = 0;
+            if (t < total) {
+                ++ntt;
+                const uint32_t kcB = kw >> 16;
+                const uint32_t r = t - ko;
+                ga = ka + r / kcB;  // arena-global triangle slots
+                gb = kb + r % kcB;
+                f3 A0, A1, A2, B0, B1, B2;
+                narrow_load(qp[kq], tris, ga, gb, A0, A1, A2, B0, B1, B2);
+                TTPlanes pl;
+                open = tri_tri_planes(A0, A1, A2, B0, B1, B2, pl);
+            }
+            queue_push(open, make_uint4(kq, ga, gb, 0u), q, nqd, below);
+            if (nqd >= 64) ref_batch(q, nqd, qp, tris, out, cap, ctr, nub, lane, below);
+        }
+    }
+    while (nqd > 0) ref_batch(q, nqd, qp, tris, out, cap, ctr, nub, lane, below);
+    block_add(ntt, &ctr->tritests);
+    block_add(nub, &ctr->ub);
+}
+
 // Rerun of deferred triangle tests at precision level L (1: float64, whose
 // undecided tests go on to `next`; 2: exact).  Wavefront-stride over the
 // list, whose length is read on the device.
@@ -1488,17 +1614,25 @@ hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
-                              RunCounters* ctr, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_split_level, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
-                       lp_cap, tasks, ntasks, split_tris, ctr);
+                              RunCounters* ctr, uint32_t grid, hipStream_t st, bool ref) {
+    if (ref)
+        hipLaunchKernelGGL(k_split_level<true>, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes,
+                           nodes, lp, lp_cap, tasks, ntasks, split_tris, ctr);
+    else
+        hipLaunchKernelGGL(k_split_level<false>, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes,
+                           nodes, lp, lp_cap, tasks, ntasks, split_tris, ctr);
     return hipGetLastError();
 }
 hipError_t launch_split_level2(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout,
                                uint32_t out_cap, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes,
                                uint4* lp, uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
-                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_split_level2, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes, nodes, lp,
-                       lp_cap, tasks, ntasks, split_tris, split_tris2, ctr);
+                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st, bool ref) {
+    if (ref)
+        hipLaunchKernelGGL(k_split_level2<true>, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes,
+                           nodes, lp, lp_cap, tasks, ntasks, split_tris, split_tris2, ctr);
+    else
+        hipLaunchKernelGGL(k_split_level2<false>, dim3(grid), dim3(256), 0, st, in, nin, out, nout, out_cap, qp, meshes,
+                           nodes, lp, lp_cap, tasks, ntasks, split_tris, split_tris2, ctr);
     return hipGetLastError();
 }
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,
@@ -1507,24 +1641,28 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
                       uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
                       uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
-                      hipStream_t st, bool pipelined) {
+                      hipStream_t st, bool pipelined, bool ref) {
     if (pipelined && share == 0u && tasks && !tasks4 && !task_lo) {  // the pipelined kernel: plain task list, no sharing
 #define CD_PIPE_ARGS tasks, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, pop_width, ctr
+#define CD_PIPE(C, R) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_dfs_pipe<C, R>), dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS)
         switch (cap_entries) {
-            case 512: hipLaunchKernelGGL(k_dfs_pipe<512>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
-            case 2048: hipLaunchKernelGGL(k_dfs_pipe<2048>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
-            default: hipLaunchKernelGGL(k_dfs_pipe<1024>, dim3(nwaves), dim3(64), 0, st, CD_PIPE_ARGS); break;
+            case 512: if (ref) CD_PIPE(512, true); else CD_PIPE(512, false); break;
+            case 2048: if (ref) CD_PIPE(2048, true); else CD_PIPE(2048, false); break;
+            default: if (ref) CD_PIPE(1024, true); else CD_PIPE(1024, false); break;
         }
+#undef CD_PIPE
 #undef CD_PIPE_ARGS
         return hipGetLastError();
     }
 #define CD_DFS_ARGS tasks, tasks4, task_lo, ntasks, task_cap, head, qp, meshes, nodes, lp, lp_cap, spill, spill_cap, \
                     pop_width, dq, dq_ctl, dq_cap, share, spin_polls, budget, budget_pct, dump, ndump, dump_cap, ctr
+#define CD_DFS(C, R) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_dfs<C, R>), dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS)
     switch (cap_entries) {
-        case 512: hipLaunchKernelGGL(k_dfs<512>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
-        case 2048: hipLaunchKernelGGL(k_dfs<2048>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
-        default: hipLaunchKernelGGL(k_dfs<1024>, dim3(nwaves), dim3(64), 0, st, CD_DFS_ARGS); break;
+        case 512: if (ref) CD_DFS(512, true); else CD_DFS(512, false); break;
+        case 2048: if (ref) CD_DFS(2048, true); else CD_DFS(2048, false); break;
+        default: if (ref) CD_DFS(1024, true); else CD_DFS(1024, false); break;
     }
+#undef CD_DFS
 #undef CD_DFS_ARGS
     return hipGetLastError();
 }
@@ -1543,6 +1681,12 @@ hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, 
     hipLaunchKernelGGL(k_narrow_rerun<2>, dim3(rerun_grid), dim3(256), 0, st, defer + defer_cap, &ctr->deferred2,
                        defer_cap, qp, tris, hits, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
     hipLaunchKernelGGL(k_contacts, dim3(grid), dim3(256), 0, st, hits, &ctr->contacts, cap, qp, tris, out);
+    return hipGetLastError();
+}
+hipError_t launch_narrow_ref(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
+                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_narrow_ref, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, ctr);
     return hipGetLastError();
 }
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,

@@ -226,7 +226,7 @@ uint64_t frontier_cap(const cd_ctx* ctx, uint32_t nq, uint32_t L) {
 }
 
 // nq: the capacity the launch geometry is built for; the count itself is in d_nq
-int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
+int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bool ref) {
     hipStream_t st = ctx->stream;
     const uint32_t L = t.split_levels;
     const uint64_t fcap = frontier_cap(ctx, nq, L);
@@ -261,12 +261,12 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
         if (t.split_depth == 2 && l + 1 < L) {
             HCK(ctx, launch_split_level2(F[k & 1], &small[k], F[(k + 1) & 1], &small[k + 1], cap32, qp, dm, dn,
                                          ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT], rule(l),
-                                         rule(l + 1), ctr, lgrid, st));
+                                         rule(l + 1), ctr, lgrid, st, ref));
             l += 2;
         } else {
             HCK(ctx, launch_split_level(F[k & 1], &small[k], F[(k + 1) & 1], &small[k + 1], cap32, qp, dm, dn,
                                         ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, T, &small[TASK_COUNT], rule(l), ctr,
-                                        lgrid, st));
+                                        lgrid, st, ref));
             l += 1;
         }
     }
@@ -280,12 +280,12 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
                         &small[HEAD0], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
                         ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP,
                         share, t.spin_polls, budget, t.handoff_pct, B3[0], &small[DUMP_COUNT], dcap, ctr, st,
-                        t.pipeline == 2));
+                        t.pipeline == 2, ref));
     if (share == 1 || share == 2)  // donations nobody claimed, [claimed, reserved), or the stacks handed off
         HCK(ctx, launch_dfs(int(t.stack_entries), nullptr, dq, &small[DQ_CTL + 1], &small[DQ_CTL], DQ_CAP,
                             &small[HEAD1], qp, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap,
                             ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP, t.pop_width, nwaves, dq, &small[DQ_CTL],
-                            DQ_CAP, 0u, t.spin_polls, nullptr, 0u, nullptr, nullptr, 0u, ctr, st));
+                            DQ_CAP, 0u, t.spin_polls, nullptr, 0u, nullptr, nullptr, 0u, ctr, st, false, ref));
     if (share == 3) {
         // what the budgeted launch handed on is split again breadth first and
         // shared out by a second launch over the new task list
@@ -294,24 +294,30 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed) {
             HCK(ctx, launch_split_level(B3[r & 1], &small[r ? RS_BASE + r : DUMP_COUNT], B3[(r + 1) & 1],
                                         &small[RS_BASE + r + 1], dcap, qp, dm, dn, ctx->d_leafpairs.as<uint4>(),
                                         ctx->lp_cap, B3[2], &small[TASK2_COUNT],
-                                        r + 1 == R ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st));
+                                        r + 1 == R ? 0xFFFFFFFFu : t.split_tris, ctr, grid, st, ref));
         HCK(ctx, launch_dfs(int(t.stack_entries), R ? B3[2] : B3[0], nullptr, nullptr,
                             &small[R ? TASK2_COUNT : DUMP_COUNT], dcap, &small[HEAD2], qp, dm, dn,
                             ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, ctx->d_work[W_SPILL].as<uint3>(), SPILL_CAP,
                             t.pop_width, nwaves, dq, &small[DQ_CTL], DQ_CAP, 0u, t.spin_polls, nullptr, 0u, nullptr,
-                            nullptr, 0u, ctr, st));
+                            nullptr, 0u, ctr, st, false, ref));
         HCK(ctx, launch_keep_budget(ctr, budget, st));
     }
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
-    HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
-                                 ctx->d_work[W_HITS].as<uint4>(), ctx->d_contacts.as<cd_contact>(), ctx->contact_cap,
-                                 ctx->d_work[W_DEFER].as<uint4>(), ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu),
-                                 st));
+    if (ref)
+        HCK(ctx, launch_narrow_ref(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
+                                   ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
+    else
+        HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp,
+                                     ctx->d_tris.as<float>(), ctx->d_work[W_HITS].as<uint4>(),
+                                     ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctx->d_work[W_DEFER].as<uint4>(),
+                                     ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu), st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
     return CD_OK;
 }
 
-int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
+// ref: reference semantics (the plain path's tests and results) on the
+// batched machinery; else the exact-predicate batched path.
+int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
     hipStream_t st = ctx->stream;
     const cd_tuning t = effective_tuning(ctx);
     const uint32_t L = t.split_levels;
@@ -353,7 +359,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
         std::vector<uint64_t> key = {nq_cap, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
                                      t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline, t.split_depth,
-                                     ctx->lp_cap, ctx->contact_cap,
+                                     uint64_t(ref), ctx->lp_cap, ctx->contact_cap,
                                      ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
         for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
                                 &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
@@ -371,7 +377,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
                 }
                 hipGraph_t g = nullptr;
                 HCK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_batched(ctx, nq_cap, t, false);
+                int rc = enqueue_batched(ctx, nq_cap, t, false, ref);
                 hipError_t ec = hipStreamEndCapture(st, &g);
                 if (rc) {
                     if (g) (void)hipGraphDestroy(g);
@@ -387,7 +393,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
             HCK(ctx, hipGraphLaunch(ctx->graph, st));
             HCK(ctx, hipEventRecord(ctx->ev[5], st));
         } else {
-            int rc = enqueue_batched(ctx, nq_cap, t, true);
+            int rc = enqueue_batched(ctx, nq_cap, t, true, ref);
             if (rc) return rc;
         }
         int rc = read_counters(ctx, h);
@@ -437,6 +443,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         ctx->counters.contacts = h.contacts;
         ctx->counters.fp64_fallbacks = h.fp64;
         ctx->counters.exact_fallbacks = h.exact;
+        ctx->counters.ub_cases = h.ub;
         *n_contacts = h.contacts;
         return CD_OK;
     }
@@ -453,7 +460,8 @@ int run_queries(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
     return rc;
 }
 int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) {
-    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED && mode != CD_MODE_REFERENCE)
+        return fail(ctx, CD_E_INVALID, "unknown mode");
     if (nq == 0) {  // nothing to launch (the kernels assume at least one query record)
         ctx->counters = cd_counters{};
         ctx->timings = cd_timings{};
@@ -461,7 +469,8 @@ int run_queries_(cd_ctx* ctx, uint32_t nq, uint32_t mode, uint64_t* n_contacts) 
         return CD_OK;
     }
     if (mode == CD_MODE_PLAIN) return run_plain(ctx, nq, n_contacts);
-    if (mode == CD_MODE_BATCHED) return run_batched(ctx, nq, n_contacts);
+    if (mode == CD_MODE_BATCHED) return run_batched(ctx, nq, n_contacts, false);
+    if (mode == CD_MODE_REFERENCE) return run_batched(ctx, nq, n_contacts, true);
     return fail(ctx, CD_E_INVALID, "unknown mode");
 }
 
@@ -1078,7 +1087,9 @@ int cd_export_tree(cd_ctx* ctx, uint32_t mesh, float* root12, float* node_obbs, 
 int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode, int32_t* flags, float* seg6) {
     if (!ctx) return CD_E_INVALID;
     if (n && (!a9 || !b9 || !flags)) return fail(ctx, CD_E_INVALID, "null argument");
-    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED && mode != CD_MODE_REFERENCE)
+        return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode == CD_MODE_REFERENCE) mode = CD_MODE_PLAIN;  // one pair per test: the reference test is the plain one
     hipStream_t st = ctx->stream;
     HCK(ctx, ctx->d_tt_a.ensure(n * 9 * sizeof(float) + 4));
     HCK(ctx, ctx->d_tt_b.ensure(n * 9 * sizeof(float) + 4));
@@ -1132,7 +1143,9 @@ int cd_tritri_batch(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, u
 int cd_tritri_batch_noalloc(cd_ctx* ctx, const float* a9, const float* b9, uint64_t n, uint32_t mode,
                             int32_t* flags, float* seg6) {
     if (!ctx) return CD_E_INVALID;
-    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED) return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode != CD_MODE_PLAIN && mode != CD_MODE_BATCHED && mode != CD_MODE_REFERENCE)
+        return fail(ctx, CD_E_INVALID, "unknown mode");
+    if (mode == CD_MODE_REFERENCE) mode = CD_MODE_PLAIN;  // one pair per test: the reference test is the plain one
     if (n > CD_NOALLOC_MAX_PAIRS)
         return fail(ctx, CD_E_CAPACITY, "more than " + std::to_string(CD_NOALLOC_MAX_PAIRS) +
                                             " pairs in one allocation-free call");

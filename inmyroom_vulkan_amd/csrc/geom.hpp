@@ -327,24 +327,43 @@ CD_HD float snap(float x) { return ((double)fabsf(x) < 0.000001) ? 0.0f : x; }
 // Result bits of the triangle test.
 enum : uint32_t { TT_HIT = 1u, TT_COPLANAR = 2u, TT_UB = 4u };
 
+// The plane stage of the test (:870-904): each triangle's snapped signed
+// distances to the other's plane.  False at the test's first two exits (one
+// triangle wholly on one side of the other's plane); those pairs do not
+// intersect, and the rest go on to the interval stage.
+struct TTPlanes {
+    f3 N1, N2;
+    float du0, du1, du2, du0du1, du0du2;
+    float dv0, dv1, dv2, dv0dv1, dv0dv2;
+};
+CD_HD bool tri_tri_planes(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, TTPlanes& p) {
+    p.N1 = cross(sub(V1, V0), sub(V2, V0));
+    float d1 = -dot(p.N1, V0);
+    p.du0 = snap(dot(p.N1, U0) + d1);
+    p.du1 = snap(dot(p.N1, U1) + d1);
+    p.du2 = snap(dot(p.N1, U2) + d1);
+    p.du0du1 = p.du0 * p.du1;
+    p.du0du2 = p.du0 * p.du2;
+    if (p.du0du1 > 0.0f && p.du0du2 > 0.0f) return false;
+    p.N2 = cross(sub(U1, U0), sub(U2, U0));
+    float d2 = -dot(p.N2, U0);
+    p.dv0 = snap(dot(p.N2, V0) + d2);
+    p.dv1 = snap(dot(p.N2, V1) + d2);
+    p.dv2 = snap(dot(p.N2, V2) + d2);
+    p.dv0dv1 = p.dv0 * p.dv1;
+    p.dv0dv2 = p.dv0 * p.dv2;
+    return !(p.dv0dv1 > 0.0f && p.dv0dv2 > 0.0f);
+}
+
 // :866-1003.  The case the reference leaves undefined (U's distances all
 // snap to zero while V's do not, :960-964) is defined as coplanar and
 // decided by the 2D coplanar test; TT_UB flags it.
 CD_HD uint32_t tri_tri_isect(f3 V0, f3 V1, f3 V2, f3 U0, f3 U1, f3 U2, f3& s, f3& t) {
-    f3 N1 = cross(sub(V1, V0), sub(V2, V0));
-    float d1 = -dot(N1, V0);
-    float du0 = snap(dot(N1, U0) + d1);
-    float du1 = snap(dot(N1, U1) + d1);
-    float du2 = snap(dot(N1, U2) + d1);
-    float du0du1 = du0 * du1, du0du2 = du0 * du2;
-    if (du0du1 > 0.0f && du0du2 > 0.0f) return 0u;
-    f3 N2 = cross(sub(U1, U0), sub(U2, U0));
-    float d2 = -dot(N2, U0);
-    float dv0 = snap(dot(N2, V0) + d2);
-    float dv1 = snap(dot(N2, V1) + d2);
-    float dv2 = snap(dot(N2, V2) + d2);
-    float dv0dv1 = dv0 * dv1, dv0dv2 = dv0 * dv2;
-    if (dv0dv1 > 0.0f && dv0dv2 > 0.0f) return 0u;
+    TTPlanes p;
+    if (!tri_tri_planes(V0, V1, V2, U0, U1, U2, p)) return 0u;
+    const f3 N1 = p.N1, N2 = p.N2;
+    const float du0 = p.du0, du1 = p.du1, du2 = p.du2, du0du1 = p.du0du1, du0du2 = p.du0du2;
+    const float dv0 = p.dv0, dv1 = p.dv1, dv2 = p.dv2, dv0dv1 = p.dv0dv1, dv0dv2 = p.dv0dv2;
     f3 D = cross(N1, N2);
     float mx = fabsf(D.x);
     int index = 0;

@@ -32,23 +32,25 @@ hipError_t launch_setup(const QueryRec* qs, uint32_t nq_cap, const unsigned int*
                         QueryPre* out, hipStream_t st);
 hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre* qp, uint3* frontier,
                        unsigned int* count, hipStream_t st);
+// ref (these three launchers): the reference's box test and descent measure
+// instead of the conservative ones (reference-semantics mode).
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
                               const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes, uint4* lp,
                               uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
-                              RunCounters* ctr, uint32_t grid, hipStream_t st);
+                              RunCounters* ctr, uint32_t grid, hipStream_t st, bool ref = false);
 // Two split levels in one launch: split_tris rules the first, split_tris2 the
 // second (0xFFFFFFFF: the second is the last split level).
 hipError_t launch_split_level2(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout,
                                uint32_t out_cap, const QueryPre* qp, const MeshRec* meshes, const NodeRec* nodes,
                                uint4* lp, uint64_t lp_cap, uint3* tasks, unsigned int* ntasks, uint32_t split_tris,
-                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st);
+                               uint32_t split_tris2, RunCounters* ctr, uint32_t grid, hipStream_t st, bool ref = false);
 hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, const unsigned int* task_lo,
                       const unsigned int* ntasks, uint32_t task_cap, unsigned int* head, const QueryPre* qp,
                       const MeshRec* meshes, const NodeRec* nodes, uint4* lp, uint64_t lp_cap, uint3* spill,
                       uint32_t spill_cap, uint32_t pop_width, uint32_t nwaves, uint4* dq, unsigned int* dq_ctl,
                       uint32_t dq_cap, uint32_t share, uint32_t spin_polls, const unsigned long long* budget,
                       uint32_t budget_pct, uint3* dump, unsigned int* ndump, uint32_t dump_cap, RunCounters* ctr,
-                      hipStream_t st, bool pipelined = false);
+                      hipStream_t st, bool pipelined = false, bool ref = false);
 // Keeps this run's node-pair test count for the next run's budgeted pass.
 hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget, hipStream_t st);
 // defer: 2 * defer_cap entries, the float64 and the exact rerun lists;
@@ -57,6 +59,11 @@ hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, 
                                const float* tris, uint4* hits, cd_contact* out, uint64_t cap, uint4* defer,
                                uint64_t defer_cap, RunCounters* ctr, uint32_t grid, uint32_t rerun_grid,
                                hipStream_t st);
+// The reference's narrow phase (plain path's test and contact record) over
+// the batched traversal's leaf pairs, contacts straight to `out`.
+hipError_t launch_narrow_ref(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
+                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                             hipStream_t st);
 // lists: 64 + 2n words of device scratch (two rerun lists of n entries)
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
                                uint32_t* lists, uint32_t grid, RunCounters* ctr, hipStream_t st);

@@ -71,6 +71,20 @@ def test_full_batch_plain_bit_exact(c3, orc):
     assert max(len(s) for s in c3.soups) > 60_000  # the largest meshes take part
 
 
+def test_full_batch_reference_mode_bit_exact(c3, orc):
+    """Reference semantics on the batched machinery, all 4,096 queries: the
+    oracle's reference-semantics contacts and counts, bit for bit."""
+    from inmyroom_vulkan_amd import MODE_REFERENCE
+
+    case("full batch, reference mode vs the oracle (reference semantics), 4,096 queries")
+    got = sort_contacts(c3.gpu.query_batch(c3.q, MODE_REFERENCE))
+    cnt = c3.gpu.counters()
+    res = orc.batch(c3.trees, c3.qa, c3.qb, c3.rel, mode=0, nthreads=THREADS)
+    assert_plain_bit_exact(got, sort_contacts(res.contacts))
+    assert (cnt["sat_tests"], cnt["leaf_pairs"], cnt["tri_tests"]) == (res.total_sat, res.total_leafpairs,
+                                                                        res.total_tritests)
+
+
 def test_full_batch_batched_vs_exact_oracle(c3, orc):
     case("full batch, batched path vs the exact oracle, 4,096 queries")
     got = sort_contacts(c3.gpu.query_batch(c3.q, MODE_BATCHED))
