@@ -142,7 +142,7 @@ public:
     using CallbackEntityDataPairs = std::vector<std::pair<EntityT, std::vector<CallbackData>>>;
     using CollidingPair = BasicCollidingPair<EntityT, CallbackData>;
 
-    explicit BasicCollisionDetection(uint32_t leaf_max = 4, uint32_t mode = CD_MODE_PLAIN) : mode_(mode) {
+    explicit BasicCollisionDetection(uint32_t leaf_max = 4, uint32_t mode = CD_MODE_REFERENCE) : mode_(mode) {
         cd_config cfg{};
         cfg.leaf_max = leaf_max;
         check(cd_create(&ctx_, &cfg), "cd_create");
@@ -155,7 +155,7 @@ public:
     // gives the ancestors of an entity (CollisionDetection.cpp:106-107) and the
     // components whose CollisionCallback receives the callbacks (:124-134).
     template <class ECSwrapperT, class = decltype(std::declval<ECSwrapperT&>().GetEntitiesHandler())>
-    explicit BasicCollisionDetection(ECSwrapperT* ecs, uint32_t mode = CD_MODE_PLAIN, uint32_t leaf_max = 4)
+    explicit BasicCollisionDetection(ECSwrapperT* ecs, uint32_t mode = CD_MODE_REFERENCE, uint32_t leaf_max = 4)
         : BasicCollisionDetection(leaf_max, mode) {
         ancestors_ = [ecs](EntityT e) {
             const auto a = ecs->GetEntitiesHandler()->GetEntityAncestors(e);

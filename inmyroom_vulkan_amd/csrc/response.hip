@@ -67,6 +67,39 @@ __device__ __forceinline__ bool ray_triangle(f3 dir, f3 v0, f3 v1, f3 v2, float&
     return true;
 }
 
+// The ray in a tree's own frame.  C maps the mesh's coordinates to the
+// ray's frame (origin at the ray's origin), so the mesh-frame ray
+// x(t) = o + t d with o = -A^-1 c, d = A^-1 dir (A, c: C's linear part and
+// translation) is mapped onto t dir: a slab crossing has the same parameter t
+// in both frames.  The boxes are then tested where they are stored, with no
+// transform and no normalised normal per test.
+struct LocalRay {
+    f3 o, d;
+    float on, dn;   // |o|_1, |d|_1
+    float kappa;    // relative rounding margin of the box tests, grown with A's condition number
+    bool ok;        // A invertible: else every box is entered
+};
+__device__ __forceinline__ float norm1f(f3 v) { return (fabsf(v.x) + fabsf(v.y)) + fabsf(v.z); }
+__device__ __forceinline__ LocalRay local_ray(const M4& C, f3 dir) {
+    const f3 a0 = mk(C.m[0], C.m[1], C.m[2]), a1 = mk(C.m[4], C.m[5], C.m[6]), a2 = mk(C.m[8], C.m[9], C.m[10]);
+    const f3 r0 = cross(a1, a2), r1 = cross(a2, a0), r2 = cross(a0, a1);  // rows of det(A) A^-1
+    const float det = dot(a0, r0);
+    const float inv = 1.f / det;
+    const f3 c = mk(C.m[12], C.m[13], C.m[14]);
+    LocalRay L;
+    L.o = mk(-dot(r0, c) * inv, -dot(r1, c) * inv, -dot(r2, c) * inv);
+    L.d = mk(dot(r0, dir) * inv, dot(r1, dir) * inv, dot(r2, dir) * inv);
+    L.on = norm1f(L.o);
+    L.dn = norm1f(L.d);
+    // |A|_F |A^-1|_F is 3 for a rotation times a uniform scale
+    const float fa = sqrtf((dot(a0, a0) + dot(a1, a1)) + dot(a2, a2));
+    const float fi = sqrtf((dot(r0, r0) + dot(r1, r1)) + dot(r2, r2)) * fabsf(inv);
+    const float cond = fa * fi * (1.f / 3.f);
+    L.kappa = 0x1p-16f * (cond > 1.f ? cond : 1.f);
+    L.ok = det != 0.f && __builtin_isfinite(inv) && __builtin_isfinite(L.kappa) && L.kappa < 0x1p-4f &&
+           __builtin_isfinite(L.on) && __builtin_isfinite(L.dn);
+    return L;
+}
 // Ray::IntersectParalgram from the origin (Ray.cpp:36-133): the three slabs
 // of the box, normals v x w, w x u, u x v.
 __device__ __forceinline__ bool ray_box(f3 dir, const Box& b, float& tmin, float& tmax) {
@@ -102,6 +135,69 @@ __device__ __forceinline__ bool ray_box(f3 dir, const Box& b, float& tmin, float
     return true;
 }
 
+// The mesh-frame test of a stored box (centre c, half-sides u, v, w): the
+// three slabs of Ray::IntersectParalgram (Ray.cpp:36-133), normals v x w,
+// w x u, u x v, each slab's half-width the box's projection radius on its
+// normal.  Every slab and crossing parameter is computed once and bounded by
+// its rounding (kappa, and the relative error of the crossing's
+// denominator), giving a widened and a shrunk interval.  BOX_OUT: even the
+// widened interval misses, so the exact ray misses and so does the
+// reference's float test, whose rounding is far inside these bounds.
+// BOX_IN: even the shrunk interval meets the ray ahead of its origin.
+// BOX_NEAR: the decision is within rounding (a box the ray grazes, one ending
+// at the ray's origin, a flat box, a slab the ray runs along); the caller
+// then runs the reference's own test.  tmin/tmax: the unwidened parameters,
+// for the order of the children and the pruning against the best hit.
+enum : int { BOX_OUT = 0, BOX_IN = 1, BOX_NEAR = 2 };
+__device__ __forceinline__ int box_local(const LocalRay& R, const float* bx, float& tmin, float& tmax) {
+    tmin = -__builtin_huge_valf();
+    tmax = __builtin_huge_valf();
+    if (!R.ok) return BOX_NEAR;
+    const f3 c = mk(bx[0], bx[1], bx[2]);
+    const f3 u = mk(bx[3], bx[4], bx[5]), v = mk(bx[6], bx[7], bx[8]), w = mk(bx[9], bx[10], bx[11]);
+    const f3 q = sub(R.o, c);
+    const float reach = ((norm1f(q) + R.on) + norm1f(c)) + ((norm1f(u) + norm1f(v)) + norm1f(w));
+    const f3 nrm[3] = {cross(v, w), cross(w, u), cross(u, v)};
+    float lo_w = -__builtin_huge_valf(), hi_w = __builtin_huge_valf();
+    float lo_s = -__builtin_huge_valf(), hi_s = __builtin_huge_valf();
+    bool decided = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 n = nrm[k];
+        const float nn = norm1f(n);
+        const float h = (fabsf(dot(n, u)) + fabsf(dot(n, v))) + fabsf(dot(n, w));
+        const float e = R.kappa * nn * reach;  // bound on the rounding of h and num
+        const float num = dot(n, q);
+        const float den = dot(n, R.d);
+        const float dl = R.kappa * nn * R.dn;  // bound on den's rounding
+        if (fabsf(den) > 4.f * dl && fabsf(den) > 0x1p-100f) {
+            const float r = __builtin_amdgcn_rcpf(den);  // within an ulp, far inside rho
+            const float rho = 2.f * dl * fabsf(r) + R.kappa;  // relative error of a crossing parameter
+            const float sg = r < 0.f ? -1.f : 1.f;  // the crossing through -h is the entry when den > 0
+            const float t1 = (-h - num) * r, t2 = (h - num) * r;
+            tmin = max_std(sg > 0.f ? t1 : t2, tmin);
+            tmax = min_std(sg > 0.f ? t2 : t1, tmax);
+            const float w1 = (-(h + e) - num) * r, w2 = ((h + e) - num) * r;
+            const float a = sg > 0.f ? w1 : w2, b = sg > 0.f ? w2 : w1;
+            lo_w = max_std(a - rho * fabsf(a), lo_w);
+            hi_w = min_std(b + rho * fabsf(b), hi_w);
+            if (h > e) {
+                const float s1 = (-(h - e) - num) * r, s2 = ((h - e) - num) * r;
+                const float as = sg > 0.f ? s1 : s2, bs = sg > 0.f ? s2 : s1;
+                lo_s = max_std(as + rho * fabsf(as), lo_s);
+                hi_s = min_std(bs - rho * fabsf(bs), hi_s);
+            } else {
+                decided = false;  // a slab thinner than its rounding
+            }
+        } else if (!(fabsf(num) + e < h)) {
+            decided = false;  // runs along the slab, not clearly inside it
+        }
+    }
+    if (!(lo_w <= hi_w && hi_w >= 0.f)) return BOX_OUT;
+    if (decided && lo_s <= hi_s && hi_s >= 0.f) return BOX_IN;
+    return BOX_NEAR;
+}
+
 struct Hit {
     bool hit, back;
     float dist, bx, by;
@@ -111,7 +207,7 @@ struct Hit {
 // A leaf's triangles are fetched LEAF_FETCH at a time (their loads overlap),
 // then tested in order.
 constexpr uint32_t LEAF_FETCH = 4;
-__device__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, const M4& C, f3 dir, uint32_t off,
+__device__ __forceinline__ void leaf_hits(const float* __restrict__ tris, const MeshRec& m, const M4& C, f3 dir, uint32_t off,
                           uint32_t cnt, Hit& best) {
     for (uint32_t i0 = off; i0 < off + cnt; i0 += LEAF_FETCH) {
         const uint32_t k = min(LEAF_FETCH, off + cnt - i0);
@@ -167,15 +263,22 @@ struct RayStack {
 // A stack that would overflow (the host sizes it from the tree depths, so
 // this is unreachable unless that sizing is wrong) sets bit 0 of *err and
 // ends the traversal; the host then fails the call with CD_E_STACK.
-__device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
+__device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
                         f3 o, f3 dir, RayStack& S, unsigned long long* err) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
     C.m[15] = C.m[15] - 0.f;
     Hit best = {false, false, __builtin_huge_valf(), 0.f, 0.f, 0xFFFFFFFFu};
+    const LocalRay R = local_ray(C, dir);
+    // the mesh-frame test, or the reference's own where that one is within rounding
+    auto enter = [&](const float* bx, float& t0, float& t1) -> bool {
+        const int k = box_local(R, bx, t0, t1);
+        if (k != BOX_NEAR) return k == BOX_IN;
+        return ray_box(dir, xform_box(C, box_from12(bx)), t0, t1) && t1 >= 0.f;
+    };
     float r0, r1;
-    if (!(ray_box(dir, xform_box(C, box_from12(m.root.box)), r0, r1) && r1 >= 0.f)) return best;
+    if (!enter(m.root.box, r0, r1)) return best;
     if (m.nnodes == 0) {  // the root is a leaf
         leaf_hits(tris, m, C, dir, 0, m.ntri, best);
         return best;
@@ -186,8 +289,8 @@ __device__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, con
     for (;;) {
         const NodeRec& n = nb[node];
         float l0, l1, q0, q1;
-        const bool lh = ray_box(dir, xform_box(C, box_from12(n.c[0].box)), l0, l1) && l1 >= 0.f;
-        const bool rh = ray_box(dir, xform_box(C, box_from12(n.c[1].box)), q0, q1) && q1 >= 0.f;
+        const bool lh = enter(n.c[0].box, l0, l1);
+        const bool rh = enter(n.c[1].box, q0, q1);
         if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
             atomicOr(err, 1ull);
             best.hit = false;
@@ -269,7 +372,7 @@ __device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
     return r;
 }
 
-__device__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
+__device__ __forceinline__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
                         const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool own = (threadIdx.x & 1u) != 0;
