@@ -256,7 +256,7 @@ inline bool vec3_attr(Ctx& c, int64_t ai, std::vector<float>& out, uint64_t& cou
     int64_t ct;
     if (!accessor(c, ai, 3, {5126}, p, count, ct, what)) return false;
     out.resize(size_t(count) * 3);
-    std::memcpy(out.data(), p, size_t(count) * 12);
+    if (count) std::memcpy(out.data(), p, size_t(count) * 12);  // an empty vector's data() may be null
     for (uint64_t v = 0; v < count; ++v) {  // (x, y, z) -> (x, -y, -z)
         out[size_t(v) * 3 + 1] = -out[size_t(v) * 3 + 1];
         out[size_t(v) * 3 + 2] = -out[size_t(v) * 3 + 2];
@@ -314,7 +314,7 @@ inline bool primitive(Ctx& c, const json::Value& pv, Primitive& pr, const std::s
                 pr.idx[size_t(i)] = v;
             }
         } else {
-            std::memcpy(pr.idx.data(), p, size_t(n) * 4);
+            if (n) std::memcpy(pr.idx.data(), p, size_t(n) * 4);  // an empty vector's data() may be null
         }
         for (uint32_t v : pr.idx)
             if (v >= pr.nvert) { c.err = where + ": index " + std::to_string(v) + " out of range"; return false; }
