@@ -90,16 +90,26 @@ int register_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint3
     return rc ? rc : upload_attrs(ctx);
 }
 
+// Also sets ctx->plain_need: the plain walk of a mesh pair holds at most its
+// trees' depths + 2 node pairs on its stack.
 int check_queries(cd_ctx* ctx, const QueryRec* q, uint32_t nq) {
     const uint32_t nm = uint32_t(ctx->meshes.size());
+    uint32_t need = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         if (q[i].mesh_a >= nm || q[i].mesh_b >= nm) return fail(ctx, CD_E_INVALID, "query names an unknown mesh");
         if (!all_finite(q[i].rel, 16)) return fail(ctx, CD_E_INVALID, "non-finite relative matrix");
-        if (ctx->meshes[q[i].mesh_a].depth + ctx->meshes[q[i].mesh_b].depth + 2 > uint32_t(PLAIN_STACK))
-            return fail(ctx, CD_E_STACK, "trees too deep for the plain traversal stack");
+        need = std::max(need, ctx->meshes[q[i].mesh_a].depth + ctx->meshes[q[i].mesh_b].depth + 2);
     }
+    ctx->plain_need = need;
     return CD_OK;
 }
+
+// Work buffers (d_work slots): batched path, then broad phase / frame.
+enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
+       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
+       W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
+       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS, W_GSTACK };
+static_assert(W_GSTACK < 40, "work slots");
 
 RunCounters* dev_counters(cd_ctx* ctx) { return ctx->d_counters.as<RunCounters>(); }
 
@@ -123,7 +133,13 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
         HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
         HCK(ctx, launch_zero(dev_counters(ctx), st));
         HCK(ctx, hipEventRecord(ctx->ev[0], st));
-        HCK(ctx, launch_traverse_plain(dq, nq, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, dev_counters(ctx), st));
+        uint2* gstack = nullptr;  // trees deeper than the LDS stack holds: the stacks in HBM
+        if (ctx->plain_need > uint32_t(PLAIN_STACK)) {
+            HCK(ctx, ctx->d_work[W_GSTACK].ensure(size_t((nq + 63) / 64) * ctx->plain_need * 64 * sizeof(uint2)));
+            gstack = ctx->d_work[W_GSTACK].as<uint2>();
+        }
+        HCK(ctx, launch_traverse_plain(dq, nq, dm, dn, ctx->d_leafpairs.as<uint4>(), ctx->lp_cap, dev_counters(ctx), st,
+                                       ctx->plain_need, gstack));
         HCK(ctx, hipEventRecord(ctx->ev[1], st));
         int rc = read_counters(ctx, h);
         if (rc) return rc;
@@ -195,12 +211,6 @@ constexpr uint32_t DUMP_COUNT = 80, RS_BASE = 81, TASK2_COUNT = 100, HEAD2 = 101
 constexpr uint32_t SMALL_WORDS = 104;
 constexpr uint32_t MAX_RESPLIT = 16;
 
-// Work buffers (d_work slots): batched path, then broad phase / frame.
-enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
-       W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
-       W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS };
-static_assert(W_PIDS < 40, "work slots");
 
 // Count table of the multi-workgroup radix sort for n keys.
 hipError_t sort_table(cd_ctx* ctx, uint32_t n, uint32_t** table) {
@@ -1284,11 +1294,13 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
         ctx->last_run = 2;
         return CD_OK;
     }
-    if (mode == CD_MODE_PLAIN)
+    if (mode == CD_MODE_PLAIN) {  // the stack the plain walk of the deepest pair needs
+        uint32_t need = 0;
         for (uint32_t i = 0; i < np; ++i)
-            if (ctx->meshes[entities[hp[i].x].mesh_id].depth + ctx->meshes[entities[hp[i].y].mesh_id].depth + 2 >
-                uint32_t(PLAIN_STACK))
-                return fail(ctx, CD_E_STACK, "trees too deep for the plain traversal stack");
+            need = std::max(need, ctx->meshes[entities[hp[i].x].mesh_id].depth +
+                                      ctx->meshes[entities[hp[i].y].mesh_id].depth + 2);
+        ctx->plain_need = need;
+    }
     ctx->staged_nq = np;
     uint64_t nc = 0;
     rc = run_queries(ctx, np, mode, &nc);
@@ -1492,8 +1504,6 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     for (uint32_t q = 0; q < nq; ++q)
         for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y}) {
             const MeshRec& m = ctx->meshes[ctx->frame_mesh[e]];
-            if (m.depth + 2 > uint32_t(RAY_STACK_ENTRIES))
-                return fail(ctx, CD_E_STACK, "tree too deep for the ray traversal stack");
             stack_entries = std::max(stack_entries, m.depth + 2);
             if (m.ntri >= (1u << RAY_LINK_BITS) || ctx->leaf_max > 64)
                 return fail(ctx, CD_E_CAPACITY, "mesh too large for the ray traversal stack (2^25 triangles)");
@@ -1524,12 +1534,18 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     HCK(ctx, hipMemcpyAsync(ctx->d_work[W_PIDS].p, up.data(), up.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     const uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();
     const uint32_t* pbegin = ctx->d_work[W_PIDS].as<uint32_t>() + 2 * size_t(nq);
+    uint32_t* gstack = nullptr;  // trees deeper than the LDS stack holds: the stacks in HBM
+    if (stack_entries > uint32_t(RAY_STACK_ENTRIES)) {
+        const size_t blocks = (2 * ctx->ray_total + 63) / 64;
+        HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * stack_entries * 64 * sizeof(uint32_t)));
+        gstack = ctx->d_work[W_GSTACK].as<uint32_t>();
+    }
     HCK(ctx, launch_zero(dev_counters(ctx), st));
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
                           ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
                           ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
                           ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
-                          stack_entries, &dev_counters(ctx)->err, st));
+                          stack_entries, &dev_counters(ctx)->err, st, gstack));
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
                             ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
                             ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,

@@ -262,6 +262,7 @@ struct cd_ctx {
     cdk::HostBuf h_pairs;  // pinned staging of a frame's broad-phase pairs
     cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
     uint32_t staged_nq = 0;
+    uint32_t plain_need = 0;  // stack entries the staged queries' plain walk needs (their trees' depths + 2)
     uint64_t staged_contacts = 0;
     uint64_t lp_cap = 0, contact_cap = 0, frontier_cap = 0;
     uint64_t defer_cap = 0;  // entries of each rerun list of the batched narrow phase

@@ -21,8 +21,11 @@ hipError_t launch_assemble(const PrimSpan* prims, uint32_t nprim, const uint32_t
 hipError_t launch_zero(RunCounters* c, hipStream_t st);
 hipError_t launch_tritri_plain(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
                                hipStream_t st);
+// need: stack entries the deepest query needs (its trees' depths + 2); above
+// PLAIN_STACK the stacks live in gstack, ceil(nq / 64) * need * 64 entries
 hipError_t launch_traverse_plain(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, const NodeRec* nodes,
-                                 uint4* lp, uint64_t lp_cap, RunCounters* ctr, hipStream_t st);
+                                 uint4* lp, uint64_t lp_cap, RunCounters* ctr, hipStream_t st, uint32_t need = 0,
+                                 uint2* gstack = nullptr);
 hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max, const QueryRec* qs,
                                const MeshRec* meshes, const float* tris, cd_contact* out, uint64_t cap,
                                RunCounters* ctr, hipStream_t st);
@@ -96,12 +99,13 @@ hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mi
 
 // response.hip
 // pbegin: per side (nq each) the first ray of every pair, non-decreasing;
-// stack_entries: the deepest tree the rays meet + 2, at most RAY_STACK_ENTRIES;
-// *err gets bit 0 if a traversal stack would have overflowed
+// stack_entries: the deepest tree the rays meet + 2; above RAY_STACK_ENTRIES
+// the stacks live in gstack, ceil(2 nrays / 64) * 2 * stack_entries * 64
+// words; *err gets bit 0 if a traversal stack would have overflowed
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, unsigned long long* err, hipStream_t st);
+                        uint32_t stack_entries, unsigned long long* err, hipStream_t st, uint32_t* gstack = nullptr);
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
                           float cos_finish, float bias, cd_pair_response* out, hipStream_t st);

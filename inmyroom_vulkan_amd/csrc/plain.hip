@@ -77,11 +77,18 @@ __device__ __forceinline__ bool code_leaf(const TreeView& t, uint32_t code, uint
     return child_leaf(c);
 }
 
+// DEEP: the lanes' stacks live in HBM, `cap` entries each (entry e of lane l
+// of block b at gstack[(b cap + e) 64 + l]), for trees deeper than the LDS
+// stack holds; the walk is the same.
+template <bool DEEP>
 __global__ __launch_bounds__(64) void k_traverse_plain(const QueryRec* __restrict__ qs, uint32_t nq,
                                                        const MeshRec* __restrict__ meshes,
                                                        const NodeRec* __restrict__ nodes, uint4* __restrict__ lp_out,
-                                                       uint64_t lp_cap, RunCounters* __restrict__ ctr) {
-    __shared__ uint2 stack[PLAIN_STACK][64];
+                                                       uint64_t lp_cap, RunCounters* __restrict__ ctr,
+                                                       uint2* __restrict__ gstack, uint32_t cap) {
+    __shared__ uint2 lstack[DEEP ? 1 : PLAIN_STACK][64];
+    uint2* stack = DEEP ? gstack + size_t(blockIdx.x) * cap * 64 : &lstack[0][0];
+    const int scap = DEEP ? int(cap) : PLAIN_STACK;
     const int lane = threadIdx.x;
     const uint32_t qi = blockIdx.x * 64 + lane;
     if (qi >= nq) return;
@@ -93,11 +100,11 @@ __global__ __launch_bounds__(64) void k_traverse_plain(const QueryRec* __restric
     const TreeView B{&meshes[q.mesh_b], nodes + meshes[q.mesh_b].node_base};
 
     int sp = 0;
-    stack[sp++][lane] = make_uint2(ROOT_CODE, ROOT_CODE);
+    stack[sp++ * 64 + lane] = make_uint2(ROOT_CODE, ROOT_CODE);
     unsigned long long nsat = 0, nlp = 0;
     bool overflow = false;
     while (sp > 0) {
-        const uint2 top = stack[--sp][lane];
+        const uint2 top = stack[--sp * 64 + lane];
         const uint32_t ca = top.x, cb = top.y;
         Box fa, sb0;
         float sfa, unused;
@@ -115,18 +122,18 @@ __global__ __launch_bounds__(64) void k_traverse_plain(const QueryRec* __restric
             ++nlp;
             continue;
         }
-        if (sp + 2 > PLAIN_STACK) {
+        if (sp + 2 > scap) {
             overflow = true;
             break;
         }
         // descend the first tree when it is the larger (OBBtree.cpp:426-447)
         const bool split_a = !la && (lb || sfa >= surface(sb));
         if (split_a) {
-            stack[sp++][lane] = make_uint2((la_link << 1) | 1u, cb);
-            stack[sp++][lane] = make_uint2((la_link << 1) | 0u, cb);
+            stack[sp++ * 64 + lane] = make_uint2((la_link << 1) | 1u, cb);
+            stack[sp++ * 64 + lane] = make_uint2((la_link << 1) | 0u, cb);
         } else {
-            stack[sp++][lane] = make_uint2(ca, (lb_link << 1) | 1u);
-            stack[sp++][lane] = make_uint2(ca, (lb_link << 1) | 0u);
+            stack[sp++ * 64 + lane] = make_uint2(ca, (lb_link << 1) | 1u);
+            stack[sp++ * 64 + lane] = make_uint2(ca, (lb_link << 1) | 0u);
         }
     }
     atomicAdd(&ctr->sat, nsat);
@@ -226,9 +233,17 @@ hipError_t launch_tritri_plain(const float* a9, const float* b9, uint64_t n, int
     return hipGetLastError();
 }
 hipError_t launch_traverse_plain(const QueryRec* qs, uint32_t nq, const MeshRec* meshes, const NodeRec* nodes,
-                                 uint4* lp, uint64_t lp_cap, RunCounters* ctr, hipStream_t st) {
+                                 uint4* lp, uint64_t lp_cap, RunCounters* ctr, hipStream_t st, uint32_t need,
+                                 uint2* gstack) {
     if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_traverse_plain, dim3((nq + 63) / 64), dim3(64), 0, st, qs, nq, meshes, nodes, lp, lp_cap, ctr);
+    if (need <= uint32_t(PLAIN_STACK))
+        hipLaunchKernelGGL(k_traverse_plain<false>, dim3((nq + 63) / 64), dim3(64), 0, st, qs, nq, meshes, nodes, lp,
+                           lp_cap, ctr, (uint2*)nullptr, 0u);
+    else if (!gstack)
+        return hipErrorInvalidValue;
+    else
+        hipLaunchKernelGGL(k_traverse_plain<true>, dim3((nq + 63) / 64), dim3(64), 0, st, qs, nq, meshes, nodes, lp,
+                           lp_cap, ctr, gstack, need);
     return hipGetLastError();
 }
 hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max, const QueryRec* qs,

@@ -425,15 +425,20 @@ __device__ __forceinline__ f3 get3(const float* o) { return mk(o[0], o[1], o[2])
 // second-to-first pass adds and pushes +response).
 // pbegin: per side, the first ray of every pair (non-decreasing; a pair
 // without rays has the next pair's begin)
+// DEEP: the stacks live in HBM, a block's 2 stack_entries 64 words at
+// gstack + 2 b stack_entries 64 (link words, then entry distances), for
+// trees deeper than the LDS stack holds; the walk is the same.
+template <bool DEEP>
 __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, uint32_t nrays, uint32_t base1,
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
                                               RayShot* __restrict__ out, int stack_entries,
-                                              unsigned long long* err) {
+                                              unsigned long long* err, uint32_t* __restrict__ gstack) {
     // the stacks are sized by the launch: the deepest tree of the call + 2
     // entries per lane, so shallow trees leave LDS for more wavefronts
     extern __shared__ uint32_t s_stack[];
-    RayStack S = {s_stack, reinterpret_cast<float*>(s_stack + stack_entries * 64), int(threadIdx.x), stack_entries};
+    uint32_t* base = DEEP ? gstack + size_t(blockIdx.x) * 2 * size_t(stack_entries) * 64 : s_stack;
+    RayStack S = {base, reinterpret_cast<float*>(base + stack_entries * 64), int(threadIdx.x), stack_entries};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
     if (r >= nrays) return;  // both lanes of a ray leave together
     const int side = r >= base1 ? 1 : 0;
@@ -573,14 +578,21 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, unsigned long long* err, hipStream_t st) {
+                        uint32_t stack_entries, unsigned long long* err, hipStream_t st, uint32_t* gstack) {
     if (!nrays) return hipSuccess;
-    if (stack_entries < 2 || stack_entries > uint32_t(RAY_STACK)) return hipErrorInvalidValue;
+    if (stack_entries < 2) return hipErrorInvalidValue;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
     const uint64_t lanes = 2ull * nrays;
-    const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
-    hipLaunchKernelGGL(k_shoot, dim3(uint32_t((lanes + 63) / 64)), dim3(64), lds, st, rays, nrays, base1, pbegin, nq,
-                       qs, V, shots, int(stack_entries), err);
+    const dim3 grid(uint32_t((lanes + 63) / 64));
+    if (stack_entries <= uint32_t(RAY_STACK)) {
+        const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
+        hipLaunchKernelGGL(k_shoot<false>, grid, dim3(64), lds, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
+                           int(stack_entries), err, (uint32_t*)nullptr);
+    } else {
+        if (!gstack) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_shoot<true>, grid, dim3(64), 0, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
+                           int(stack_entries), err, gstack);
+    }
     return hipGetLastError();
 }
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
