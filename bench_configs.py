@@ -249,15 +249,31 @@ def config2(args):
         # the whole ExecuteCollisionDetection: frame, uncollide rays, ray shooting and the split
         t = time.perf_counter()
         nrays = nshot = 0
+        shoot_ms = boxes = tritests = 0.0
         for f in frames:
             gpu.frame(f, mode)
             _, rays = gpu.uncollide_rays()
             resp = gpu.shoot_rays()
             nrays += len(rays)
             nshot += int(resp["shot"].sum())
+            shoot_ms += gpu.timings()["traverse_ms"]
+            c = gpu.counters()
+            boxes += c["sat_tests"]
+            tritests += c["tri_tests"]
         full = (time.perf_counter() - t) * 1e3 / len(frames)
+        # k_shoot's roofline: 64 B per box tested (one child record), 36 B per
+        # ray-triangle test, 24 B per ray read and 56 B per ray result written
+        shoot_bytes = 64.0 * boxes + 36.0 * tritests + (24.0 + 56.0) * nrays
+        shoot_gbs = shoot_bytes / (shoot_ms * 1e-3) / 1e9
         out[name].update({"full_ms_per_frame": round(full, 4), "rays_per_frame": nrays / len(frames),
-                          "pairs_shot_per_frame": nshot / len(frames)})
+                          "pairs_shot_per_frame": nshot / len(frames),
+                          "shoot": {"kernel_ms_per_frame": round(shoot_ms / len(frames), 4),
+                                    "boxes_per_frame": boxes / len(frames),
+                                    "ray_triangle_tests_per_frame": tritests / len(frames),
+                                    "roofline": {"bound": "hbm", "kernel": "k_shoot", "achieved": round(shoot_gbs, 2),
+                                                 "peak": 8000.0, "unit": "GB/s",
+                                                 "frac": round(shoot_gbs / 8000.0, 5), "traffic": None,
+                                                 "algorithmic_bytes_per_frame": int(shoot_bytes / len(frames))}}})
         log(f"config 2 {name}: {full:.3f} ms/frame with rays and response")
     # every frame's pairs and contacts in both modes (for the mismatch counts),
     # and the batched traversal's device time from eager runs (events around

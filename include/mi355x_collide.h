@@ -396,7 +396,11 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
  * Params NULL: the reference's values.  CD_E_CAPACITY writes the count back;
  * it is also returned, with no count, for a pair whose mesh has 2^25
  * triangles or more (the traversal stack packs an index and a leaf count
- * into one word), and CD_E_STACK for a tree deeper than 46 levels. */
+ * into one word).  Trees deeper than 46 levels keep the traversal stacks in
+ * device memory instead of LDS.  Afterwards cd_get_counters reports the
+ * boxes tested (sat_tests) and ray-triangle tests (tri_tests) of the call,
+ * and cd_get_timings the shooting kernel (traverse_ms) and, with the
+ * response, the whole call's device time (total_ms). */
 int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* out, uint32_t cap,
                   uint32_t* n_out);
 

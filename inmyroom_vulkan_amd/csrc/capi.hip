@@ -1541,11 +1541,13 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
         gstack = ctx->d_work[W_GSTACK].as<uint32_t>();
     }
     HCK(ctx, launch_zero(dev_counters(ctx), st));
+    HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
                           ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
                           ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
                           ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
-                          stack_entries, &dev_counters(ctx)->err, st, gstack));
+                          stack_entries, dev_counters(ctx), st, gstack));
+    HCK(ctx, hipEventRecord(ctx->ev[1], st));
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
                             ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
                             ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,
@@ -1554,9 +1556,23 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     HCK(ctx, hipMemcpyAsync(all.data(), ctx->d_work[W_RESP].p, size_t(nq) * sizeof(cd_pair_response),
                             hipMemcpyDeviceToHost, st));
     RunCounters h{};
+    HCK(ctx, hipEventRecord(ctx->ev[2], st));
     int rc = read_counters(ctx, h);  // synchronises the stream
     if (rc) return rc;
     if (h.err) return fail(ctx, CD_E_STACK, "ray traversal stack overflow");
+    // the shot's work and time: boxes tested (sat_tests), ray-triangle tests
+    // (tri_tests), the shooting kernel (traverse_ms) and with the response
+    // (total_ms)
+    float t_shoot = 0.f, t_all = 0.f;
+    HCK(ctx, hipEventElapsedTime(&t_shoot, ctx->ev[0], ctx->ev[1]));
+    HCK(ctx, hipEventElapsedTime(&t_all, ctx->ev[0], ctx->ev[2]));
+    ctx->counters = cd_counters{};
+    ctx->counters.sat_tests = h.sat;
+    ctx->counters.tri_tests = h.tritests;
+    ctx->timings = cd_timings{};
+    ctx->timings.traverse_ms = t_shoot;
+    ctx->timings.total_ms = t_all;
+    ctx->timings.launches = 2;
     uint32_t k = 0;
     for (uint32_t q = 0; q < nq; ++q)
         if (ctx->ray_counts[q] + ctx->ray_counts[size_t(nq) + q]) out[k++] = all[q];

@@ -101,11 +101,12 @@ hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mi
 // pbegin: per side (nq each) the first ray of every pair, non-decreasing;
 // stack_entries: the deepest tree the rays meet + 2; above RAY_STACK_ENTRIES
 // the stacks live in gstack, ceil(2 nrays / 64) * 2 * stack_entries * 64
-// words; *err gets bit 0 if a traversal stack would have overflowed
+// words; ctr->err gets bit 0 if a traversal stack would have overflowed,
+// ctr->sat the boxes tested and ctr->tritests the ray-triangle tests
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, unsigned long long* err, hipStream_t st, uint32_t* gstack = nullptr);
+                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack = nullptr);
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
                           float cos_finish, float bias, cd_pair_response* out, hipStream_t st);

@@ -264,7 +264,7 @@ struct RayStack {
 // this is unreachable unless that sizing is wrong) sets bit 0 of *err and
 // ends the traversal; the host then fails the call with CD_E_STACK.
 __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
-                        f3 o, f3 dir, RayStack& S, unsigned long long* err) {
+                        f3 o, f3 dir, RayStack& S, unsigned long long* err, uint32_t& nbox, uint32_t& ntri) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
@@ -278,42 +278,60 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
         return ray_box(dir, xform_box(C, box_from12(bx)), t0, t1) && t1 >= 0.f;
     };
     float r0, r1;
+    ++nbox;
     if (!enter(m.root.box, r0, r1)) return best;
     if (m.nnodes == 0) {  // the root is a leaf
+        ntri += m.ntri;
         leaf_hits(tris, m, C, dir, 0, m.ntri, best);
         return best;
     }
     const NodeRec* nb = nodes + m.node_base;
     int sp = 0;
+    // One step enters at most one internal node and tests at most one leaf,
+    // then pops the next entry: the lanes of a wavefront, each at its own
+    // place in its own walk, all make progress every step, where a lane
+    // working through a run of leaves would otherwise hold the others.  Each
+    // lane's sequence of pops, tests and best-hit updates is the walk's own.
+    bool at_node = true;
     uint32_t node = 0;  // the root, entered without a distance check
+    uint32_t leaf_link = 0, leaf_cnt = 0;
     for (;;) {
-        const NodeRec& n = nb[node];
-        float l0, l1, q0, q1;
-        const bool lh = enter(n.c[0].box, l0, l1);
-        const bool rh = enter(n.c[1].box, q0, q1);
-        if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
-            atomicOr(err, 1ull);
-            best.hit = false;
-            break;
-        }
-        const uint32_t cl = child_cnt(n.c[0]), cr = child_cnt(n.c[1]);
-        // the farther below the nearer (Ray.cpp:175-199); order set by the entry distances alone
-        const bool left_first = !(lh && rh) || l0 < q0;
-        if (lh && rh) {
-            if (left_first) {
-                S.put(sp++, n.c[1].link, cr, q0);
+        if (at_node) {
+            const NodeRec& n = nb[node];
+            float l0, l1, q0, q1;
+            const bool lh = enter(n.c[0].box, l0, l1);
+            const bool rh = enter(n.c[1].box, q0, q1);
+            nbox += 2;
+            if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
+                atomicOr(err, 1ull);
+                best.hit = false;
+                break;
+            }
+            const uint32_t cl = child_cnt(n.c[0]), cr = child_cnt(n.c[1]);
+            // the farther below the nearer (Ray.cpp:175-199); order set by the entry distances alone
+            const bool left_first = !(lh && rh) || l0 < q0;
+            if (lh && rh) {
+                if (left_first) {
+                    S.put(sp++, n.c[1].link, cr, q0);
+                    S.put(sp++, n.c[0].link, cl, l0);
+                } else {
+                    S.put(sp++, n.c[0].link, cl, l0);
+                    S.put(sp++, n.c[1].link, cr, q0);
+                }
+            } else if (lh) {
                 S.put(sp++, n.c[0].link, cl, l0);
-            } else {
-                S.put(sp++, n.c[0].link, cl, l0);
+            } else if (rh) {
                 S.put(sp++, n.c[1].link, cr, q0);
             }
-        } else if (lh) {
-            S.put(sp++, n.c[0].link, cl, l0);
-        } else if (rh) {
-            S.put(sp++, n.c[1].link, cr, q0);
+            at_node = false;
         }
-        // next internal node to enter; leaves are tested as they are popped
-        bool found = false;
+        if (leaf_cnt) {
+            ntri += leaf_cnt;
+            leaf_hits(tris, m, C, dir, leaf_link, leaf_cnt, best);
+            leaf_cnt = 0;
+        }
+        // the next entry the best hit so far does not prune
+        bool more = false;
         while (sp > 0) {
             --sp;
             const uint32_t ew = S.lc[sp * 64 + S.lane];
@@ -322,12 +340,15 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
             if (!(et < best.dist)) continue;
             if (ec == 0u) {
                 node = el;
-                found = true;
-                break;
+                at_node = true;
+            } else {
+                leaf_link = el;
+                leaf_cnt = ec;
             }
-            leaf_hits(tris, m, C, dir, el, ec, best);
+            more = true;
+            break;
         }
-        if (!found) break;
+        if (!more) break;
     }
     return best;
 }
@@ -373,14 +394,15 @@ __device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
 }
 
 __device__ __forceinline__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
-                        const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err) {
+                        const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err, uint32_t& nbox,
+                        uint32_t& ntri) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool own = (threadIdx.x & 1u) != 0;
     const f3 o2 = move_eps(o, d, 4.f);
     M4 C;  // one call with lane-selected arguments, so the two traversals run as one instruction stream
 #pragma unroll
     for (int e = 0; e < 16; ++e) C.m[e] = own ? Amat.m[e] : Bmat.m[e];
-    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err);
+    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err, nbox, ntri);
     const int even = int(threadIdx.x & ~1u);
     const Hit p2 = shfl_hit(h, even), p3 = shfl_hit(h, even + 1);
     if (!(p2.hit && p2.back)) return r;
@@ -410,6 +432,10 @@ __device__ __forceinline__ f3 force_response(const Pass& p) {
 }
 
 __device__ __forceinline__ f3 neg(f3 v) { return mk(-v.x, -v.y, -v.z); }  // glm's unary minus
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 __device__ __forceinline__ void put3(float* o, f3 v) {
     o[0] = v.x;
     o[1] = v.y;
@@ -433,54 +459,63 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
                                               const uint32_t* __restrict__ pbegin, uint32_t nq,
                                               const QueryRec* __restrict__ qs, MeshView V,
                                               RayShot* __restrict__ out, int stack_entries,
-                                              unsigned long long* err, uint32_t* __restrict__ gstack) {
+                                              RunCounters* __restrict__ ctr, uint32_t* __restrict__ gstack) {
+    unsigned long long* err = &ctr->err;
     // the stacks are sized by the launch: the deepest tree of the call + 2
     // entries per lane, so shallow trees leave LDS for more wavefronts
     extern __shared__ uint32_t s_stack[];
     uint32_t* base = DEEP ? gstack + size_t(blockIdx.x) * 2 * size_t(stack_entries) * 64 : s_stack;
     RayStack S = {base, reinterpret_cast<float*>(base + stack_entries * 64), int(threadIdx.x), stack_entries};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
-    if (r >= nrays) return;  // both lanes of a ray leave together
-    const int side = r >= base1 ? 1 : 0;
-    const uint32_t local = r - (side ? base1 : 0u);
-    const uint32_t* begin = pbegin + size_t(nq) * side;
-    // the pair: the last one whose rays begin at or before this ray (of equal
-    // begins only the last can hold rays)
-    uint32_t lo = 0, hi = nq;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (begin[mid] <= local) lo = mid;
-        else hi = mid;
-    }
-    const QueryRec q = qs[lo];
-    M4 M, I;
+    uint32_t nbox = 0, ntri = 0;  // boxes tested, ray-triangle tests: the work counted in ctr->sat, ctr->tritests
+    if (r < nrays) {  // both lanes of a ray take part or neither
+        const int side = r >= base1 ? 1 : 0;
+        const uint32_t local = r - (side ? base1 : 0u);
+        const uint32_t* begin = pbegin + size_t(nq) * side;
+        // the pair: the last one whose rays begin at or before this ray (of equal
+        // begins only the last can hold rays)
+        uint32_t lo = 0, hi = nq;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (begin[mid] <= local) lo = mid;
+            else hi = mid;
+        }
+        const QueryRec q = qs[lo];
+        M4 M, I;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        M.m[e] = q.rel[e];
-        I.m[e] = (e % 5 == 0) ? 1.f : 0.f;
-    }
-    const M3 Mn = adjoint_transpose(q.rel);
-    M3 I3;
+        for (int e = 0; e < 16; ++e) {
+            M.m[e] = q.rel[e];
+            I.m[e] = (e % 5 == 0) ? 1.f : 0.f;
+        }
+        const M3 Mn = adjoint_transpose(q.rel);
+        M3 I3;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) I3.m[e] = (e % 4 == 0) ? 1.f : 0.f;
-    RayShot s = {};
-    f3 o = get3(rays[r].origin), d = get3(rays[r].direction);
-    // side 0: first-to-second, then the reflected ray second-to-first; side 1 the other way round
+        for (int e = 0; e < 9; ++e) I3.m[e] = (e % 4 == 0) ? 1.f : 0.f;
+        RayShot s = {};
+        f3 o = get3(rays[r].origin), d = get3(rays[r].direction);
+        // side 0: first-to-second, then the reflected ray second-to-first; side 1 the other way round
 #pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool f2s = (pass == 0) == (side == 0);
-        const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S, err)
-                           : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S, err);
-        if (!p.ok) break;
-        const f3 F = force_response(p);
-        put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
-        put3(s.push[pass], f2s ? neg(p.response) : p.response);
-        s.ok |= 1u << pass;
-        o = add(o, p.response);  // ReflectHermannResult (:104-110)
-        d = neg(p.normal);
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool f2s = (pass == 0) == (side == 0);
+            const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S, err, nbox, ntri)
+                               : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S, err, nbox, ntri);
+            if (!p.ok) break;
+            const f3 F = force_response(p);
+            put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
+            put3(s.push[pass], f2s ? neg(p.response) : p.response);
+            s.ok |= 1u << pass;
+            o = add(o, p.response);  // ReflectHermannResult (:104-110)
+            d = neg(p.normal);
+        }
+        s.pair = lo;
+        if ((threadIdx.x & 1u) == 0) out[r] = s;
     }
-    s.pair = lo;
-    if ((threadIdx.x & 1u) == 0) out[r] = s;
+    // per wavefront, one atomic each
+    const unsigned long long bsum = wave_sum_u64(nbox), tsum = wave_sum_u64(ntri);
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&ctr->sat, bsum);
+        atomicAdd(&ctr->tritests, tsum);
+    }
 }
 
 __device__ __forceinline__ float smoother_step(float a, float b, float x) {  // :171-175
@@ -578,7 +613,7 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, unsigned long long* err, hipStream_t st, uint32_t* gstack) {
+                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack) {
     if (!nrays) return hipSuccess;
     if (stack_entries < 2) return hipErrorInvalidValue;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
@@ -587,11 +622,11 @@ hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, cons
     if (stack_entries <= uint32_t(RAY_STACK)) {
         const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
         hipLaunchKernelGGL(k_shoot<false>, grid, dim3(64), lds, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
-                           int(stack_entries), err, (uint32_t*)nullptr);
+                           int(stack_entries), ctr, (uint32_t*)nullptr);
     } else {
         if (!gstack) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_shoot<true>, grid, dim3(64), 0, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
-                           int(stack_entries), err, gstack);
+                           int(stack_entries), ctr, gstack);
     }
     return hipGetLastError();
 }
