@@ -275,6 +275,15 @@ def config2(args):
                                                  "frac": round(shoot_gbs / 8000.0, 5), "traffic": None,
                                                  "algorithmic_bytes_per_frame": int(shoot_bytes / len(frames))}}})
         log(f"config 2 {name}: {full:.3f} ms/frame with rays and response")
+        # the same step as one call (cd_step): contacts and rays stay on the device
+        for f in frames[: args.warmup]:
+            gpu.step(f, mode)
+        t = time.perf_counter()
+        for f in frames:
+            gpu.step(f, mode)
+        one = (time.perf_counter() - t) * 1e3 / len(frames)
+        out[name]["step_ms_per_frame"] = round(one, 4)
+        log(f"config 2 {name}: {one:.3f} ms/frame as one cd_step")
     # every frame's pairs and contacts in both modes (for the mismatch counts),
     # and the batched traversal's device time from eager runs (events around
     # the kernels) against the reference's node-pair tests from the plain frames

@@ -216,6 +216,27 @@ public:
     void ExecuteCollisionDetection() {  // :38-141
         results_.clear();
         if (entries_.size() < 2) return;
+        if (!keep_details_) {
+            // one call for the whole step; contacts and rays stay on the device
+            uint32_t nresp = 0;
+            int rc = cd_step(ctx_, entries_.data(), uint32_t(entries_.size()), mode_, nullptr, resp_.data(),
+                             uint32_t(resp_.size()), &nresp);
+            if (rc == CD_E_CAPACITY) {  // the frame and its rays are done: shoot into a larger buffer
+                resp_.resize(std::max<size_t>(resp_.size(), nresp + nresp / 4));
+                rc = cd_shoot_rays(ctx_, nullptr, resp_.data(), uint32_t(resp_.size()), &nresp);
+            }
+            check(rc, "cd_step");
+            for (uint32_t i = 0; i < nresp; ++i) {
+                CollidingPair r;
+                r.firstIndex = resp_[i].first;
+                r.secondIndex = resp_[i].second;
+                r.first = EntityT(entries_[resp_[i].first].entity);
+                r.second = EntityT(entries_[resp_[i].second].entity);
+                results_.push_back(std::move(r));
+            }
+            Respond(resp_.data(), nresp);
+            return;
+        }
         // The output buffers keep the largest sizes seen, with a quarter more
         // room (they never shrink): a frame that fits them takes one call; a
         // larger one is called again.
@@ -265,9 +286,32 @@ public:
             return;
         }
         // ray shooting and the movement split (:69-99), one record per pair with rays
-        std::vector<cd_pair_response> resp(results_.size());
+        resp_.resize(std::max(resp_.size(), results_.size()));
         uint32_t nresp = 0;
-        check(cd_shoot_rays(ctx_, nullptr, resp.data(), uint32_t(resp.size()), &nresp), "cd_shoot_rays");
+        check(cd_shoot_rays(ctx_, nullptr, resp_.data(), uint32_t(resp_.size()), &nresp), "cd_shoot_rays");
+        Respond(resp_.data(), nresp);
+    }
+
+    // Keep every pair's contacts and rays in GetCollidingPairs (three calls
+    // per step, the contacts and rays copied to the host); off by default,
+    // when the step is one cd_step and a pair holds its entities, shot flag
+    // and deltas.
+    void SetKeepDetails(bool keep) { keep_details_ = keep; }
+
+    // Engine hooks for the last steps (CollisionDetection.cpp:101-134), for a
+    // host without the engine's ECS.  Without an ancestor provider an entity
+    // is its own only ancestor.
+    void SetAncestorsProvider(std::function<std::vector<EntityT>(EntityT)> f) { ancestors_ = std::move(f); }
+    void SetCallbackSink(std::function<void(const CallbackEntityDataPairs&)> f) { sink_ = std::move(f); }
+
+    const std::vector<CollidingPair>& GetCollidingPairs() const { return results_; }
+    cd_ctx* Context() const { return ctx_; }
+
+private:
+    // The responses of results_'s pairs, in order: shot flags, deltas, the
+    // two callback records of each pair sent to every ancestor the two
+    // entities do not share (:101-121), then MakeCallbacks.
+    void Respond(const cd_pair_response* resp, uint32_t nresp) {
         std::unordered_map<EntityT, std::vector<CallbackData>> callbacks_to_be_made;
         for (size_t i = 0; i < results_.size() && i < nresp; ++i) {
             CollidingPair& r = results_[i];
@@ -291,16 +335,6 @@ public:
         MakeCallbacks(std::move(callbacks_to_be_made));
     }
 
-    // Engine hooks for the last steps (CollisionDetection.cpp:101-134), for a
-    // host without the engine's ECS.  Without an ancestor provider an entity
-    // is its own only ancestor.
-    void SetAncestorsProvider(std::function<std::vector<EntityT>(EntityT)> f) { ancestors_ = std::move(f); }
-    void SetCallbackSink(std::function<void(const CallbackEntityDataPairs&)> f) { sink_ = std::move(f); }
-
-    const std::vector<CollidingPair>& GetCollidingPairs() const { return results_; }
-    cd_ctx* Context() const { return ctx_; }
-
-private:
     std::vector<EntityT> Ancestors(EntityT e) const { return ancestors_ ? ancestors_(e) : std::vector<EntityT>{e}; }
 
     // CollisionDetection::MakeCallbacks (:124-134): the map becomes a vector
@@ -323,6 +357,8 @@ private:
     std::vector<cd_contact> contacts_;
     std::vector<cd_pair_rays> ray_pairs_;
     std::vector<cd_ray> rays_;
+    std::vector<cd_pair_response> resp_ = std::vector<cd_pair_response>(64);
+    bool keep_details_ = false;
     std::vector<CollidingPair> results_;
     std::function<std::vector<EntityT>(EntityT)> ancestors_;
     std::function<void(const CallbackEntityDataPairs&)> sink_;

@@ -404,6 +404,19 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
 int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* out, uint32_t cap,
                   uint32_t* n_out);
 
+/* One whole step of CollisionDetection::ExecuteCollisionDetection
+ * (src/CollisionDetection/CollisionDetection.cpp:38-99): cd_frame,
+ * cd_uncollide_rays and cd_shoot_rays in one call, with the contacts and the
+ * rays kept on the device; only the per-pair responses (the records
+ * cd_shoot_rays gives) come back.  For hosts that need the deltas and not
+ * the contacts or rays themselves (the engine's callbacks).  CD_E_CAPACITY
+ * writes the count back with the frame and its rays done: cd_shoot_rays
+ * with a larger buffer finishes the step.  Afterwards the context is as
+ * after the three calls (cd_uncollide_rays and cd_shoot_rays may be called
+ * again on the same frame). */
+int cd_step(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, const cd_shoot_params* params,
+            cd_pair_response* out, uint32_t cap, uint32_t* n_out);
+
 /* Vertex normals of a mesh for its rays (TriangleNormal::CreateTriangleNormalList,
  * src/Geometry/Triangle.cpp:214-234): nvert x 3 floats indexed like the
  * vertices given to cd_register_mesh (for a soup: 3 t + k).  A mesh without

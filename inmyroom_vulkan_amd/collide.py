@@ -35,7 +35,7 @@ EXPORTS = (
     "cd_set_tuning", "cd_get_tuning", "cd_export_triangles", "cd_register_skinned", "cd_skin_refit",
     "cd_uncollide_rays", "cd_set_mesh_normals", "cd_shoot_rays", "cd_gltf_open", "cd_gltf_close",
     "cd_gltf_mesh_count", "cd_gltf_mesh_info_get", "cd_gltf_primitive_get", "cd_register_gltf",
-    "cd_node_global_matrices", "cd_gltf_node_count", "cd_gltf_nodes", "cd_tritri_batch_noalloc",
+    "cd_node_global_matrices", "cd_gltf_node_count", "cd_gltf_nodes", "cd_tritri_batch_noalloc", "cd_step",
 )
 CD_NOALLOC_MAX_PAIRS = 1 << 20
 
@@ -148,6 +148,7 @@ def load_library() -> C.CDLL:
     lib.cd_fetch_contacts.argtypes = [vp, vp, u64, P(u64)]
     lib.cd_uncollide_rays.argtypes = [vp, vp, u32, P(u32), vp, u64, P(u64)]
     lib.cd_shoot_rays.argtypes = [vp, P(cd_shoot_params), vp, u32, P(u32)]
+    lib.cd_step.argtypes = [vp, vp, u32, u32, P(cd_shoot_params), vp, u32, P(u32)]
     lib.cd_set_mesh_normals.argtypes = [vp, u32, vp, u32]
     lib.cd_get_counters.argtypes = [vp, P(cd_counters)]
     lib.cd_get_timings.argtypes = [vp, P(cd_timings)]
@@ -267,6 +268,7 @@ class Collider:
         self.h = h
         self.leaf_max = leaf_max
         self._caps = {}  # output capacities of the last calls: a frame like the last needs no second call
+        self._bufs = {}  # output buffers kept across calls
 
     def close(self):
         if getattr(self, "h", None):
@@ -473,6 +475,30 @@ class Collider:
                 continue
             self._check(rc, "cd_shoot_rays")
             return out[: n.value]
+
+    def step(self, entities: np.ndarray, mode: int = MODE_REFERENCE, start_rads=None, finish_rads=None, bias=None):
+        """One whole scene step (cd_step): frame, uncollide rays and response
+        in one call, contacts and rays kept on the device; the responses of
+        the pairs with rays (PAIR_RESPONSE_DTYPE), as shoot_rays() gives them."""
+        ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
+        p = None
+        if start_rads is not None or finish_rads is not None or bias is not None:
+            p = C.byref(cd_shoot_params(math.radians(40.0) if start_rads is None else start_rads,
+                                        math.radians(65.0) if finish_rads is None else finish_rads,
+                                        1.01 if bias is None else bias))
+        cap = self._caps.get("step", 1 << 10)
+        out = self._bufs.get("step")
+        if out is None or len(out) < cap:
+            out = self._bufs["step"] = np.zeros(cap, PAIR_RESPONSE_DTYPE)
+        n = C.c_uint32()
+        rc = self.lib.cd_step(self.h, _ptr(ents), len(ents), mode, p, _ptr(out), cap, C.byref(n))
+        if rc == CD_E_CAPACITY:  # the frame and its rays are done: a larger buffer finishes the step
+            cap = _grown(int(n.value))
+            out = self._bufs["step"] = np.zeros(cap, PAIR_RESPONSE_DTYPE)
+            rc = self.lib.cd_shoot_rays(self.h, p, _ptr(out), cap, C.byref(n))
+            self._caps["step"] = cap
+        self._check(rc, "cd_step")
+        return out[: n.value].copy()
 
     def set_mesh_normals(self, mesh: int, normals: np.ndarray) -> None:
         n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)

@@ -1257,8 +1257,13 @@ int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_ou
     return CD_OK;
 }
 
-int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
-             uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+}  // extern "C"
+
+namespace {
+// cd_frame; to_host false (cd_step): the sorted contacts stay on the device,
+// pairs and contacts are not returned.
+int frame_impl(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
+               uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts, bool to_host) {
     if (!ctx || !n_pairs || !n_contacts) return CD_E_INVALID;
     (void)hipSetDevice(ctx->device);
     *n_pairs = 0;
@@ -1328,14 +1333,19 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
         HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), uint32_t(nc), ctx->d_work[W_CSORT].as<cd_contact>(),
                                       bits, ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
                                       ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), table, st));
-        HCK(ctx, ctx->h_stage.ensure(nc * sizeof(cd_contact)));
-        HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost, st));
-        HCK(ctx, hipStreamSynchronize(st));
-        hc = static_cast<const cd_contact*>(ctx->h_stage.p);
+        if (to_host) {
+            HCK(ctx, ctx->h_stage.ensure(nc * sizeof(cd_contact)));
+            HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact),
+                                    hipMemcpyDeviceToHost, st));
+            HCK(ctx, hipStreamSynchronize(st));
+            hc = static_cast<const cd_contact*>(ctx->h_stage.p);
+        }
     }
     ctx->contacts_sorted = true;
     ctx->last_run = 2;
     ctx->staged_ids = hp;
+    *n_contacts = nc;
+    if (!to_host) return CD_OK;
     // pairs that produced contacts (CollisionDetection.cpp:52, :63 keep only those)
     std::vector<cd_pair> out_pairs;
     for (uint64_t i = 0; i < nc;) {
@@ -1355,8 +1365,10 @@ int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, 
 }
 
 // ---- rays (CreateUncollideRays.cpp:60-200) -----------------------------------
-int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
-                      uint64_t ray_cap, uint64_t* n_rays) {
+// cd_uncollide_rays; to_host false (cd_step): the rays stay on the device for
+// the shooting, and neither the pairs nor the rays are returned.
+int rays_impl(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
+              uint64_t ray_cap, uint64_t* n_rays, bool to_host) {
     if (!ctx || !n_pairs || !n_rays) return CD_E_INVALID;
     *n_pairs = 0;
     *n_rays = 0;
@@ -1451,7 +1463,7 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
     HCK(ctx, hipStreamSynchronize(st));
     // pairs with rays (CollisionDetection.cpp:63); rays of the first sides, then of the second sides
     std::vector<cd_pair_rays> out;
-    for (uint32_t q = 0; q < nq; ++q) {
+    for (uint32_t q = 0; q < nq && to_host; ++q) {
         const uint32_t c0 = hq[q], c1 = hq[2 * size_t(nq) + q];
         if (c0 == 0 && c1 == 0) continue;
         cd_pair_rays r{};
@@ -1479,11 +1491,39 @@ int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint3
     ctx->rays_ready = true;
     *n_pairs = uint32_t(out.size());
     *n_rays = total;
+    if (!to_host) return CD_OK;
     if (out.size() > pair_cap || total > ray_cap || (total && (!pairs || !rays)))
         return fail(ctx, CD_E_CAPACITY, "pair or ray buffer too small");
     if (!out.empty()) std::memcpy(pairs, out.data(), out.size() * sizeof(cd_pair_rays));
     if (total) HCK(ctx, hipMemcpy(rays, ctx->d_work[W_RAYS].p, total * sizeof(cd_ray), hipMemcpyDeviceToHost));
     return CD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
+             uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+    return frame_impl(ctx, entities, n, mode, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts, true);
+}
+
+int cd_uncollide_rays(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
+                      uint64_t ray_cap, uint64_t* n_rays) {
+    return rays_impl(ctx, pairs, pair_cap, n_pairs, rays, ray_cap, n_rays, true);
+}
+
+int cd_step(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, const cd_shoot_params* params,
+            cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
+    if (!ctx || !n_out) return CD_E_INVALID;
+    *n_out = 0;
+    uint32_t np = 0;
+    uint64_t nc = 0, nr = 0;
+    int rc = frame_impl(ctx, entities, n, mode, nullptr, 0, &np, nullptr, 0, &nc, false);
+    if (rc) return rc;
+    if (n < 2 || ctx->staged_nq == 0) return CD_OK;  // no pairs: nothing to shoot
+    rc = rays_impl(ctx, nullptr, 0, &np, nullptr, 0, &nr, false);
+    if (rc) return rc;
+    return cd_shoot_rays(ctx, params, out, cap, n_out);
 }
 
 // ---- response (ShootUncollideRays.cpp:14-179, CollisionDetection.cpp:70-99) ----

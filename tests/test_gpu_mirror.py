@@ -21,6 +21,7 @@ int main() {
     const float P[8 * 3] = {-1,-1,-1, 1,-1,-1, 1,1,-1, -1,1,-1, -1,-1,1, 1,-1,1, 1,1,1, -1,1,1};
     const uint32_t I[12 * 3] = {0,2,1, 0,3,2, 4,5,6, 4,6,7, 0,1,5, 0,5,4, 2,3,7, 2,7,6, 1,2,6, 1,6,5, 0,4,7, 0,7,3};
     mi355x::CollisionDetection cd(4, MODE);
+    cd.SetKeepDetails(KEEP);
     const mi355x::OBBtree tree(cd.RegisterMesh(P, 8, I, 12));
     // entities 5 and 7 share ancestor 2: callbacks go to 5 and 7 only
     cd.SetAncestorsProvider([](mi355x::Entity e) { return std::vector<mi355x::Entity>{2, e}; });
@@ -92,16 +93,18 @@ def _oracle_delta(orc):
     return d[0, 0]
 
 
-@pytest.mark.parametrize("mode", ["CD_MODE_PLAIN", "CD_MODE_REFERENCE"])
-def test_mirror_frame_to_callbacks(tmp_path, orc, mode):
-    case(f"C++ mirror, two cubes to callbacks, {mode}")
+@pytest.mark.parametrize("mode,keep", [("CD_MODE_PLAIN", 1), ("CD_MODE_REFERENCE", 0), ("CD_MODE_REFERENCE", 1)])
+def test_mirror_frame_to_callbacks(tmp_path, orc, mode, keep):
+    """keep 0: one cd_step per ExecuteCollisionDetection; 1: the three calls
+    with every pair's contacts and rays kept."""
+    case(f"C++ mirror, two cubes to callbacks, {mode}, details {keep}")
     from inmyroom_vulkan_amd import collide
     src = tmp_path / "mirror.cpp"
     src.write_text(PROGRAM)
     exe = tmp_path / "mirror"
     libdir = os.path.dirname(collide.LIB_PATH)
-    subprocess.run(["g++", "-std=c++17", f"-DMODE={mode}", "-I", os.path.join(ROOT, "include"), str(src), "-L",
-                    libdir, "-lmi355x_collide", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    subprocess.run(["g++", "-std=c++17", f"-DMODE={mode}", f"-DKEEP={keep}", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-L", libdir, "-lmi355x_collide", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout
     lines = out.split("\n")
     pairs = [l.split() for l in lines if l.startswith("pair")]

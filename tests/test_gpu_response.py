@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from conftest import case
-from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, CollideError, Collider, scenes
+from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_BATCHED, MODE_PLAIN, MODE_REFERENCE, CollideError, Collider, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -127,3 +127,28 @@ def test_shoot_needs_frame_and_rays(gpu):
         assert len(r) >= 0 and (len(r) == 0 or r["shot"].sum() == 0)  # nothing moved
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("mode", [MODE_PLAIN, MODE_REFERENCE, MODE_BATCHED])
+def test_step_equals_frame_rays_and_shoot(gpu, mode):
+    """cd_step (contacts and rays kept on the device) gives the records the
+    three calls give, bit for bit, frame after frame."""
+    sc = scenes.Scene(n_entities=96, seed=2, cube=12.0)
+    case(f"cd_step vs cd_frame + cd_uncollide_rays + cd_shoot_rays, mode {mode}")
+    ids = gpu.register_soups(sc.soups)
+    shot = 0
+    for frame in range(3):
+        ents = sc.entities(ids, ENTITY_DTYPE)
+        gpu.frame(ents, mode)
+        gpu.uncollide_rays()
+        want = gpu.shoot_rays()
+        got = gpu.step(ents, mode)
+        assert got.tobytes() == want.tobytes(), frame
+        shot += int(got["shot"].sum())
+        sc.step()
+    assert shot > 0
+    e = sc.entities(ids, ENTITY_DTYPE)[:1]
+    assert len(gpu.step(e, mode)) == 0  # one entity: no pairs
+    far = sc.entities(ids, ENTITY_DTYPE)[:2]
+    far["current"][1][12] += 1e4
+    assert len(gpu.step(far, mode)) == 0  # two entities far apart
