@@ -263,13 +263,24 @@ struct RayStack {
 // A stack that would overflow (the host sizes it from the tree depths, so
 // this is unreachable unless that sizing is wrong) sets bit 0 of *err and
 // ends the traversal; the host then fails the call with CD_E_STACK.
+// sbest: per lane of the wavefront, the best hit distance so far of its walk,
+// and when the walk is over its hit distance if that is a back face, else -1.
+// bounded (the own-tree pass of a Hermann pair): the walk only decides
+// whether p2.dist <= p3.dist + eps holds, with p2 the partner lane's closest
+// hit, so an entry whose entry distance lies beyond the partner's best so far
+// (never below its final one) plus eps, widened by BOUND_SLACK for rounding,
+// is skipped, and the walk stops once the partner ends without a back-face
+// hit (the pass fails whatever this walk finds).
+constexpr float BOUND_SLACK = 1.f + 0x1p-12f;
 __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
-                        f3 o, f3 dir, RayStack& S, unsigned long long* err, uint32_t& nbox, uint32_t& ntri) {
+                        f3 o, f3 dir, RayStack& S, unsigned long long* err, uint32_t& nbox, uint32_t& ntri,
+                        volatile float* sbest, bool bounded, float eps) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
     C.m[15] = C.m[15] - 0.f;
     Hit best = {false, false, __builtin_huge_valf(), 0.f, 0.f, 0xFFFFFFFFu};
+    sbest[S.lane] = __builtin_huge_valf();
     const LocalRay R = local_ray(C, dir);
     // the mesh-frame test, or the reference's own where that one is within rounding
     auto enter = [&](const float* bx, float& t0, float& t1) -> bool {
@@ -279,11 +290,15 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
     };
     float r0, r1;
     ++nbox;
-    if (!enter(m.root.box, r0, r1)) return best;
+    auto finish = [&]() {  // the walk's end, as the partner lane reads it
+        sbest[S.lane] = (best.hit && best.back) ? best.dist : -1.f;
+        return best;
+    };
+    if (!enter(m.root.box, r0, r1)) return finish();
     if (m.nnodes == 0) {  // the root is a leaf
         ntri += m.ntri;
         leaf_hits(tris, m, C, dir, 0, m.ntri, best);
-        return best;
+        return finish();
     }
     const NodeRec* nb = nodes + m.node_base;
     int sp = 0;
@@ -329,15 +344,21 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
             ntri += leaf_cnt;
             leaf_hits(tris, m, C, dir, leaf_link, leaf_cnt, best);
             leaf_cnt = 0;
+            sbest[S.lane] = best.dist;
         }
         // the next entry the best hit so far does not prune
         bool more = false;
-        while (sp > 0) {
+        bool stop = false;
+        const float pb = bounded ? sbest[S.lane ^ 1] : __builtin_huge_valf();
+        if (pb < 0.f) stop = true;  // the partner ended without a back-face hit
+        const float bound = (pb + eps) * BOUND_SLACK;
+        while (sp > 0 && !stop) {
             --sp;
             const uint32_t ew = S.lc[sp * 64 + S.lane];
             const uint32_t el = ew & ((1u << RAY_LINK_BITS) - 1u), ec = ew >> RAY_LINK_BITS;
             const float et = S.t0[sp * 64 + S.lane];
             if (!(et < best.dist)) continue;
+            if (et > bound) continue;  // no hit in it can make the pass fail
             if (ec == 0u) {
                 node = el;
                 at_node = true;
@@ -350,7 +371,7 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
         }
         if (!more) break;
     }
-    return best;
+    return finish();
 }
 
 // Ray::MoveOriginEpsilonTowardsDirection (Ray.cpp:13-21).
@@ -395,18 +416,19 @@ __device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
 
 __device__ __forceinline__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
                         const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err, uint32_t& nbox,
-                        uint32_t& ntri) {
+                        uint32_t& ntri, volatile float* sbest) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool own = (threadIdx.x & 1u) != 0;
     const f3 o2 = move_eps(o, d, 4.f);
     M4 C;  // one call with lane-selected arguments, so the two traversals run as one instruction stream
 #pragma unroll
     for (int e = 0; e < 16; ++e) C.m[e] = own ? Amat.m[e] : Bmat.m[e];
-    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err, nbox, ntri);
+    const float eps = vlength(sub(o2, o));
+    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err, nbox, ntri,
+                           sbest, own, eps);
     const int even = int(threadIdx.x & ~1u);
     const Hit p2 = shfl_hit(h, even), p3 = shfl_hit(h, even + 1);
     if (!(p2.hit && p2.back)) return r;
-    const float eps = vlength(sub(o2, o));
     if (p2.dist <= p3.dist + eps) {
         const MeshRec mb = V.meshes[meshB];
         r.ok = true;
@@ -464,6 +486,7 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
     // the stacks are sized by the launch: the deepest tree of the call + 2
     // entries per lane, so shallow trees leave LDS for more wavefronts
     extern __shared__ uint32_t s_stack[];
+    __shared__ float s_best[64];  // the lanes' best hits, for the own-tree walks' bound
     uint32_t* base = DEEP ? gstack + size_t(blockIdx.x) * 2 * size_t(stack_entries) * 64 : s_stack;
     RayStack S = {base, reinterpret_cast<float*>(base + stack_entries * 64), int(threadIdx.x), stack_entries};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
@@ -497,8 +520,8 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
 #pragma unroll 1
         for (int pass = 0; pass < 2; ++pass) {
             const bool f2s = (pass == 0) == (side == 0);
-            const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S, err, nbox, ntri)
-                               : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S, err, nbox, ntri);
+            const Pass p = f2s ? hermann(V, q.mesh_a, I, q.mesh_b, M, Mn, o, d, S, err, nbox, ntri, s_best)
+                               : hermann(V, q.mesh_b, M, q.mesh_a, I, I3, o, d, S, err, nbox, ntri, s_best);
             if (!p.ok) break;
             const f3 F = force_response(p);
             put3(s.force[pass], f2s ? neg(F) : F);  // sum - F == sum + (-F) exactly
