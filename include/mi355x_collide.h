@@ -185,6 +185,8 @@ typedef struct cd_tuning {
                                 it issues the next batch's loads before it tests the current one (work_sharing 0) */
     uint32_t split_depth;    /* split levels one launch runs, 1 or 2 (0: 2): with 2 a thread tests its pair and
                                 then both of that pair's children */
+    uint32_t step_graph;     /* cd_step: 0 or 1 replays the whole step as one captured HIP graph; 2 enqueues the
+                                same kernels one by one (no graph) */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */

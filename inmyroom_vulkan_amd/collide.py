@@ -96,7 +96,8 @@ class cd_tuning(C.Structure):
     _fields_ = [("split_levels", C.c_uint32), ("stack_entries", C.c_uint32), ("pop_width", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("work_sharing", C.c_uint32),
                 ("spin_polls", C.c_uint32), ("split_tris", C.c_uint32), ("handoff_pct", C.c_uint32),
-                ("resplit_levels", C.c_uint32), ("pipeline", C.c_uint32), ("split_depth", C.c_uint32)]
+                ("resplit_levels", C.c_uint32), ("pipeline", C.c_uint32), ("split_depth", C.c_uint32),
+                ("step_graph", C.c_uint32)]
 
 
 class cd_gltf_mesh_info(C.Structure):
@@ -523,11 +524,13 @@ class Collider:
         return {f: getattr(c, f) for f, _ in cd_counters._fields_}
 
     def set_tuning(self, split_levels=0, stack_entries=0, pop_width=0, waves_per_cu=0, use_graph=0, work_sharing=0,
-                   spin_polls=0, split_tris=0, handoff_pct=0, resplit_levels=0, pipeline=0, split_depth=0):
+                   spin_polls=0, split_tris=0, handoff_pct=0, resplit_levels=0, pipeline=0, split_depth=0,
+                   step_graph=0):
         t = cd_tuning(split_levels=split_levels, stack_entries=stack_entries, pop_width=pop_width,
                       waves_per_cu=waves_per_cu, use_graph=use_graph, work_sharing=work_sharing,
                       spin_polls=spin_polls, split_tris=split_tris, handoff_pct=handoff_pct,
-                      resplit_levels=resplit_levels, pipeline=pipeline, split_depth=split_depth)
+                      resplit_levels=resplit_levels, pipeline=pipeline, split_depth=split_depth,
+                      step_graph=step_graph)
         self._check(self.lib.cd_set_tuning(self.h, C.byref(t)), "cd_set_tuning")
 
     def tuning(self) -> dict:

@@ -55,10 +55,15 @@ __global__ void k_sap_project(const cd_entity* __restrict__ ents, uint32_t n, co
     vals[i] = i;
 }
 
+// n_dev (here and in the sort and pair kernels below): when given, the count
+// is min(*n_dev, n) and n is only the capacity the launch was shaped for, so
+// that a captured graph serves every count up to it.
 __global__ __launch_bounds__(1024) void k_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
-                                                     uint32_t* vals_tmp, uint32_t n, uint32_t passes) {
+                                                     uint32_t* vals_tmp, uint32_t n, const uint32_t* __restrict__ n_dev,
+                                                     uint32_t passes) {
     __shared__ uint32_t s_cnt[16][16];  // [wavefront][digit]
     __shared__ uint32_t s_base[16];
+    n = count_of(n, n_dev);
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t *sk = keys, *sv = vals, *dk = keys_tmp, *dv = vals_tmp;
@@ -128,10 +133,12 @@ constexpr uint32_t RANK_SPLIT = 4;
 
 __global__ __launch_bounds__(RANK_KEYS * RANK_SPLIT) void k_rank_sort(const uint32_t* __restrict__ keys,
                                                                       const uint32_t* __restrict__ vals, uint32_t n,
-                                                                      uint32_t mask, uint32_t* __restrict__ keys_out,
+                                                                      const uint32_t* __restrict__ n_dev, uint32_t mask,
+                                                                      uint32_t* __restrict__ keys_out,
                                                                       uint32_t* __restrict__ vals_out) {
     __shared__ uint32_t s_k[RANK_MAX];
     __shared__ uint32_t s_part[RANK_SPLIT][RANK_KEYS];
+    n = count_of(n, n_dev);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_k[i] = keys[i] & mask;
     __syncthreads();
     const uint32_t kl = threadIdx.x % RANK_KEYS, part = threadIdx.x / RANK_KEYS;
@@ -159,9 +166,9 @@ __global__ __launch_bounds__(RANK_KEYS * RANK_SPLIT) void k_rank_sort(const uint
 }
 
 __global__ void k_copy_back(const uint32_t* __restrict__ ka, const uint32_t* __restrict__ va, uint32_t n,
-                            uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
+                            const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    if (i < count_of(n, n_dev)) {
         kb[i] = ka[i];
         vb[i] = va[i];
     }
@@ -173,9 +180,11 @@ __global__ void k_copy_back(const uint32_t* __restrict__ ka, const uint32_t* __r
 constexpr uint32_t RS_TILE = 1024;  // keys per workgroup per pass: 4 rounds of 256
 constexpr uint32_t RS_SMALL = 4096;
 
-__global__ __launch_bounds__(256) void k_rs_count(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+__global__ __launch_bounds__(256) void k_rs_count(const uint32_t* __restrict__ keys, uint32_t n,
+                                                  const uint32_t* __restrict__ n_dev, uint32_t shift,
                                                   uint32_t* __restrict__ table) {
     __shared__ uint32_t h[256];
+    n = count_of(n, n_dev);
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t lo = blockIdx.x * RS_TILE, hi = min(lo + RS_TILE, n);
@@ -241,10 +250,12 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ table, 
 }
 
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* __restrict__ sk, const uint32_t* __restrict__ sv,
-                                                    uint32_t n, uint32_t shift, const uint32_t* __restrict__ table,
-                                                    uint32_t* __restrict__ dk, uint32_t* __restrict__ dv) {
+                                                    uint32_t n, const uint32_t* __restrict__ n_dev, uint32_t shift,
+                                                    const uint32_t* __restrict__ table, uint32_t* __restrict__ dk,
+                                                    uint32_t* __restrict__ dv) {
     __shared__ uint32_t start[256];    // next output slot of each digit for this tile
     __shared__ uint32_t wcnt[4][256];  // keys of each digit per wavefront in the current round
+    n = count_of(n, n_dev);
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     start[tid] = table[size_t(tid) * gridDim.x + blockIdx.x];
@@ -327,19 +338,19 @@ __global__ __launch_bounds__(64 * SAP_WAVES) void k_sap_sweep(const uint32_t* __
     }
 }
 
-__global__ void k_pair_keys(const uint2* __restrict__ pairs, uint32_t n, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals) {
+__global__ void k_pair_keys(const uint2* __restrict__ pairs, uint32_t n, const uint32_t* __restrict__ n_dev,
+                            uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     keys[i] = (pairs[i].x << 16) | pairs[i].y;
     vals[i] = i;
 }
 
 __global__ void k_pair_queries(const uint2* __restrict__ pairs, const uint32_t* __restrict__ order, uint32_t n,
-                               const cd_entity* __restrict__ ents, QueryRec* __restrict__ qs,
-                               uint2* __restrict__ sorted_pairs) {
+                               const uint32_t* __restrict__ n_dev, const cd_entity* __restrict__ ents,
+                               QueryRec* __restrict__ qs, uint2* __restrict__ sorted_pairs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     const uint2 pr = pairs[order[i]];
     M4 A, B;
     for (int k = 0; k < 16; ++k) {
@@ -356,10 +367,10 @@ __global__ void k_pair_queries(const uint2* __restrict__ pairs, const uint32_t* 
 }
 
 // Contact grouping: keys of one field of the contact records.
-__global__ void k_contact_keys(const cd_contact* __restrict__ c, uint32_t n, int field, const uint32_t* __restrict__ order,
-                               uint32_t* __restrict__ keys) {
+__global__ void k_contact_keys(const cd_contact* __restrict__ c, uint32_t n, const uint32_t* __restrict__ n_dev,
+                               int field, const uint32_t* __restrict__ order, uint32_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= count_of(n, n_dev)) return;
     const cd_contact& r = c[order[i]];
     keys[i] = field == 0 ? r.query : (field == 1 ? r.tri_a : r.tri_b);
 }
@@ -368,9 +379,9 @@ __global__ void k_iota(uint32_t* v, uint32_t n) {
     if (i < n) v[i] = i;
 }
 __global__ void k_gather_contacts(const cd_contact* __restrict__ in, const uint32_t* __restrict__ order, uint32_t n,
-                                  cd_contact* __restrict__ out) {
+                                  const uint32_t* __restrict__ n_dev, cd_contact* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[order[i]];
+    if (i < count_of(n, n_dev)) out[i] = in[order[i]];
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -385,28 +396,28 @@ uint64_t radix_table_words(uint32_t n) { return uint64_t(256) * ((uint64_t(n) + 
 
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-                             uint32_t key_bits, uint32_t* table, hipStream_t st) {
+                             uint32_t key_bits, uint32_t* table, hipStream_t st, const uint32_t* n_dev) {
     if (n < 2) return hipSuccess;
     if (n <= RANK_MAX) {
         const uint32_t mask = rank_sort_mask(key_bits);
         hipLaunchKernelGGL(k_rank_sort, dim3((n + RANK_KEYS - 1) / RANK_KEYS), dim3(RANK_KEYS * RANK_SPLIT), 0, st, keys,
-                           vals, n, mask, keys_tmp, vals_tmp);
-        hipLaunchKernelGGL(k_copy_back, g256(n), dim3(256), 0, st, keys_tmp, vals_tmp, n, keys, vals);
+                           vals, n, n_dev, mask, keys_tmp, vals_tmp);
+        hipLaunchKernelGGL(k_copy_back, g256(n), dim3(256), 0, st, keys_tmp, vals_tmp, n, n_dev, keys, vals);
         return hipGetLastError();
     }
     if (n <= RS_SMALL || !table) {
         uint32_t passes = (key_bits + 3) / 4;
         passes += passes & 1u;  // even, so the result lands back in keys/vals
-        hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(1024), 0, st, keys, vals, keys_tmp, vals_tmp, n, passes);
+        hipLaunchKernelGGL(k_radix_sort, dim3(1), dim3(1024), 0, st, keys, vals, keys_tmp, vals_tmp, n, n_dev, passes);
         return hipGetLastError();
     }
     const uint32_t tiles = uint32_t((uint64_t(n) + RS_TILE - 1) / RS_TILE);
     const uint32_t m = 256 * tiles;
     uint32_t *sk = keys, *sv = vals, *dk = keys_tmp, *dv = vals_tmp;
     for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(256), 0, st, sk, n, shift, table);
+        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(256), 0, st, sk, n, n_dev, shift, table);
         hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, table, m);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(256), 0, st, sk, sv, n, shift, table, dk, dv);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(256), 0, st, sk, sv, n, n_dev, shift, table, dk, dv);
         std::swap(sk, dk);
         std::swap(sv, dv);
     }
@@ -423,32 +434,33 @@ hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_ent
                        pairs, cap, count);
     return hipGetLastError();
 }
-hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st) {
-    hipLaunchKernelGGL(k_pair_keys, g256(n), dim3(256), 0, st, pairs, n, keys, vals);
+hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st,
+                            const uint32_t* n_dev) {
+    hipLaunchKernelGGL(k_pair_keys, g256(n), dim3(256), 0, st, pairs, n, n_dev, keys, vals);
     return hipGetLastError();
 }
 hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32_t n, const cd_entity* ents,
-                               QueryRec* qs, uint2* sorted_pairs, hipStream_t st) {
-    hipLaunchKernelGGL(k_pair_queries, g256(n), dim3(256), 0, st, pairs, order, n, ents, qs, sorted_pairs);
+                               QueryRec* qs, uint2* sorted_pairs, hipStream_t st, const uint32_t* n_dev) {
+    hipLaunchKernelGGL(k_pair_queries, g256(n), dim3(256), 0, st, pairs, order, n, n_dev, ents, qs, sorted_pairs);
     return hipGetLastError();
 }
 hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo,
                             const uint32_t field_bits[3], uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
-                            uint32_t* table, hipStream_t st) {
+                            uint32_t* table, hipStream_t st, const uint32_t* n_dev) {
     hipLaunchKernelGGL(k_iota, g256(n), dim3(256), 0, st, v0, n);
     for (int field : {f_lo, f_mid, f_hi}) {  // stable LSD: least significant field first
-        hipLaunchKernelGGL(k_contact_keys, g256(n), dim3(256), 0, st, in, n, field, v0, k0);
-        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, field_bits[field], table, st);
+        hipLaunchKernelGGL(k_contact_keys, g256(n), dim3(256), 0, st, in, n, n_dev, field, v0, k0);
+        hipError_t e = launch_radix_sort(k0, v0, k1, v1, n, field_bits[field], table, st, n_dev);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
 hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, const uint32_t field_bits[3],
                                 uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* table,
-                                hipStream_t st) {
-    hipError_t e = launch_sort_perm(in, n, 0, 1, 2, field_bits, k0, v0, k1, v1, table, st);
+                                hipStream_t st, const uint32_t* n_dev) {
+    hipError_t e = launch_sort_perm(in, n, 0, 1, 2, field_bits, k0, v0, k1, v1, table, st, n_dev);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gather_contacts, g256(n), dim3(256), 0, st, in, v0, n, out);
+    hipLaunchKernelGGL(k_gather_contacts, g256(n), dim3(256), 0, st, in, v0, n, n_dev, out);
     return hipGetLastError();
 }
 

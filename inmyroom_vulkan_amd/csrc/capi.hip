@@ -108,8 +108,9 @@ int check_queries(cd_ctx* ctx, const QueryRec* q, uint32_t nq) {
 enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
-       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS, W_GSTACK };
-static_assert(W_GSTACK < 40, "work slots");
+       W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS, W_GSTACK,
+       W_SCTR, W_STEPC, W_STEPOUT };
+static_assert(W_STEPOUT < sizeof(cd_ctx::d_work) / sizeof(DevBuf), "work slots");
 
 RunCounters* dev_counters(cd_ctx* ctx) { return ctx->d_counters.as<RunCounters>(); }
 
@@ -325,25 +326,15 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bo
     return CD_OK;
 }
 
-// ref: reference semantics (the plain path's tests and results) on the
-// batched machinery; else the exact-predicate batched path.
-int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
-    hipStream_t st = ctx->stream;
-    const cd_tuning t = effective_tuning(ctx);
-    const uint32_t L = t.split_levels;
+// Buffers of a batched run for nq_cap queries (nq: the count, for the first
+// sizing of the leaf-pair list); *fcap the frontier capacity they hold.
+int ensure_batched(cd_ctx* ctx, uint32_t nq, uint32_t nq_cap, const cd_tuning& t, uint64_t* fcap) {
     const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
     if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
     if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
     if (ctx->defer_cap == 0) ctx->defer_cap = 1u << 16;
-    // the launch geometry (and a captured graph) is built for a power-of-two
-    // capacity; the kernels read the count itself from d_nq, so frames whose
-    // pair counts differ replay one graph
-    uint32_t nq_cap = 64;
-    while (nq_cap < nq) nq_cap *= 2;
     HCK(ctx, ctx->d_nq.ensure(sizeof(unsigned int)));
     HCK(ctx, ctx->h_nq.ensure(sizeof(unsigned int)));
-    *static_cast<unsigned int*>(ctx->h_nq.p) = nq;
-    HCK(ctx, hipMemcpyAsync(ctx->d_nq.p, ctx->h_nq.p, sizeof(unsigned int), hipMemcpyHostToDevice, st));
     HCK(ctx, ctx->d_work[W_QPRE].ensure(size_t(nq_cap) * sizeof(QueryPre)));
     HCK(ctx, ctx->d_work[W_SMALL].ensure(SMALL_WORDS * sizeof(unsigned int)));
     HCK(ctx, ctx->d_work[W_SPILL].ensure(size_t(nwaves) * SPILL_CAP * sizeof(uint3)));
@@ -351,84 +342,140 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
         HCK(ctx, ctx->d_work[W_DUMP0].ensure(size_t(DQ_CAP) * sizeof(uint4)));
     if (!ctx->d_budget.p) {  // node-pair tests of the last run, kept across runs (none yet)
         HCK(ctx, ctx->d_budget.ensure(sizeof(unsigned long long)));
-        HCK(ctx, hipMemsetAsync(ctx->d_budget.p, 0, sizeof(unsigned long long), st));
+        HCK(ctx, hipMemsetAsync(ctx->d_budget.p, 0, sizeof(unsigned long long), ctx->stream));
     }
+    *fcap = frontier_cap(ctx, nq_cap, t.split_levels);
+    HCK(ctx, ctx->d_work[W_FRONT0].ensure(*fcap * sizeof(uint3)));
+    HCK(ctx, ctx->d_work[W_FRONT1].ensure(*fcap * sizeof(uint3)));
+    HCK(ctx, ctx->d_work[W_TASKS].ensure(*fcap * sizeof(uint3)));
+    if (t.work_sharing == 3) {
+        if (ctx->dump_cap == 0) ctx->dump_cap = 1u << 20;
+        for (int w : {W_BUD0, W_BUD1, W_BUD2}) HCK(ctx, ctx->d_work[w].ensure(ctx->dump_cap * sizeof(uint3)));
+    }
+    HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
+    HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
+    HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
+    HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
+    return CD_OK;
+}
+
+// What a captured batched run depends on: sizes, tuning and the buffers.
+void batched_key(const cd_ctx* ctx, uint32_t nq_cap, const cd_tuning& t, bool ref, uint64_t fcap,
+                 std::vector<uint64_t>& key) {
+    const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
+    const uint64_t k[] = {nq_cap, t.split_levels, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
+                          t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline, t.split_depth, uint64_t(ref),
+                          ctx->lp_cap, ctx->contact_cap, ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
+    key.insert(key.end(), std::begin(k), std::end(k));
+    for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
+                            &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
+                            &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
+                            &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER], &ctx->d_budget,
+                            &ctx->d_nq, &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2],
+                            &ctx->d_work[W_HITS]})
+        key.push_back(reinterpret_cast<uint64_t>(b->p));
+}
+
+// A batched run's counters: grows what ran out of room (*retry: run it
+// again) or fails on an error the run cannot recover from.
+int check_batched(cd_ctx* ctx, const RunCounters& h, uint64_t fcap, const cd_tuning& t, bool* retry) {
+    *retry = false;
+    if (h.err & 2u) {  // a split level or the hand-off ran out of room: repeat with twice as much
+        ctx->frontier_cap = 2 * fcap;
+        if (t.work_sharing == 3) ctx->dump_cap = 2 * std::max<uint64_t>(ctx->dump_cap, 1u << 20);
+        *retry = true;
+    }
+    if (h.err & 8ull) return fail(ctx, CD_E_HIP, "batched traversal: a donated node pair was never published");
+    if (h.err & ~2ull) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
+    if (h.lpslots > ctx->lp_cap) {
+        ctx->lp_cap = h.lpslots + h.lpslots / 4;
+        *retry = true;
+    }
+    if (h.contacts > ctx->contact_cap) {
+        ctx->contact_cap = h.contacts + h.contacts / 4;
+        *retry = true;
+    }
+    if (h.deferred > ctx->defer_cap || h.deferred2 > ctx->defer_cap) {  // rerun lists ran out of room
+        const uint64_t need = std::max(h.deferred, h.deferred2);
+        ctx->defer_cap = need + need / 4;
+        *retry = true;
+    }
+    return CD_OK;
+}
+
+// The run's work counters as the API reports them.
+void set_run_counters(cd_ctx* ctx, const RunCounters& h) {
+    ctx->counters = cd_counters{};
+    ctx->counters.sat_tests = h.sat;
+    ctx->counters.leaf_pairs = h.leafpairs;
+    ctx->counters.tri_tests = h.tritests;
+    ctx->counters.contacts = h.contacts;
+    ctx->counters.fp64_fallbacks = h.fp64;
+    ctx->counters.exact_fallbacks = h.exact;
+    ctx->counters.ub_cases = h.ub;
+}
+
+// Capture what enq enqueues on the stream as a graph in *exec (replacing
+// the one there).
+template <class F>
+int capture_graph(cd_ctx* ctx, hipGraphExec_t* exec, F&& enq) {
+    if (*exec) {
+        (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+    }
+    hipStream_t st = ctx->stream;
+    hipGraph_t g = nullptr;
+    HCK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = enq();
+    hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
+    hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
+    return CD_OK;
+}
+
+// ref: reference semantics (the plain path's tests and results) on the
+// batched machinery; else the exact-predicate batched path.
+int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
+    hipStream_t st = ctx->stream;
+    const cd_tuning t = effective_tuning(ctx);
+    // the launch geometry (and a captured graph) is built for a power-of-two
+    // capacity; the kernels read the count itself from d_nq, so frames whose
+    // pair counts differ replay one graph
+    uint32_t nq_cap = 64;
+    while (nq_cap < nq) nq_cap *= 2;
     RunCounters h{};
     for (int attempt = 0; attempt < 8; ++attempt) {
-        const uint64_t fcap = frontier_cap(ctx, nq_cap, L);
-        HCK(ctx, ctx->d_work[W_FRONT0].ensure(fcap * sizeof(uint3)));
-        HCK(ctx, ctx->d_work[W_FRONT1].ensure(fcap * sizeof(uint3)));
-        HCK(ctx, ctx->d_work[W_TASKS].ensure(fcap * sizeof(uint3)));
-        if (t.work_sharing == 3) {
-            if (ctx->dump_cap == 0) ctx->dump_cap = 1u << 20;
-            for (int w : {W_BUD0, W_BUD1, W_BUD2}) HCK(ctx, ctx->d_work[w].ensure(ctx->dump_cap * sizeof(uint3)));
-        }
-        HCK(ctx, ctx->d_leafpairs.ensure(ctx->lp_cap * sizeof(uint4)));
-        HCK(ctx, ctx->d_contacts.ensure(ctx->contact_cap * sizeof(cd_contact)));
-        HCK(ctx, ctx->d_work[W_HITS].ensure(ctx->contact_cap * sizeof(uint4)));
-        HCK(ctx, ctx->d_work[W_DEFER].ensure(2 * ctx->defer_cap * sizeof(uint4)));
-        std::vector<uint64_t> key = {nq_cap, L, t.stack_entries, t.pop_width, nwaves, t.work_sharing, t.spin_polls,
-                                     t.split_tris, t.handoff_pct, t.resplit_levels, t.pipeline, t.split_depth,
-                                     uint64_t(ref), ctx->lp_cap, ctx->contact_cap,
-                                     ctx->defer_cap, ctx->dump_cap, uint64_t(ctx->leaf_max), fcap};
-        for (const DevBuf* b : {&ctx->d_queries, &ctx->d_meshes, &ctx->d_nodes, &ctx->d_tris, &ctx->d_leafpairs,
-                                &ctx->d_contacts, &ctx->d_counters, &ctx->d_work[W_QPRE], &ctx->d_work[W_FRONT0],
-                                &ctx->d_work[W_FRONT1], &ctx->d_work[W_SMALL], &ctx->d_work[W_SPILL],
-                                &ctx->d_work[W_DUMP0], &ctx->d_work[W_TASKS], &ctx->d_work[W_DEFER], &ctx->d_budget,
-                                &ctx->d_nq,
-                                &ctx->d_work[W_BUD0], &ctx->d_work[W_BUD1], &ctx->d_work[W_BUD2], &ctx->d_work[W_HITS]})
-            key.push_back(reinterpret_cast<uint64_t>(b->p));
+        uint64_t fcap = 0;
+        int rc = ensure_batched(ctx, nq, nq_cap, t, &fcap);
+        if (rc) return rc;
+        *static_cast<unsigned int*>(ctx->h_nq.p) = nq;
+        HCK(ctx, hipMemcpyAsync(ctx->d_nq.p, ctx->h_nq.p, sizeof(unsigned int), hipMemcpyHostToDevice, st));
         if (t.use_graph) {
-            bool same = ctx->graph && key == ctx->graph_key;
-            if (!same) {
-                if (ctx->graph) {
-                    (void)hipGraphExecDestroy(ctx->graph);
-                    ctx->graph = nullptr;
-                }
-                hipGraph_t g = nullptr;
-                HCK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                int rc = enqueue_batched(ctx, nq_cap, t, false, ref);
-                hipError_t ec = hipStreamEndCapture(st, &g);
-                if (rc) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc;
-                }
-                if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
-                hipError_t ei = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
+            std::vector<uint64_t> key;
+            batched_key(ctx, nq_cap, t, ref, fcap, key);
+            if (!(ctx->graph && key == ctx->graph_key)) {
+                rc = capture_graph(ctx, &ctx->graph, [&] { return enqueue_batched(ctx, nq_cap, t, false, ref); });
+                if (rc) return rc;
                 ctx->graph_key = key;
             }
             HCK(ctx, hipEventRecord(ctx->ev[4], st));
             HCK(ctx, hipGraphLaunch(ctx->graph, st));
             HCK(ctx, hipEventRecord(ctx->ev[5], st));
         } else {
-            int rc = enqueue_batched(ctx, nq_cap, t, true, ref);
+            rc = enqueue_batched(ctx, nq_cap, t, true, ref);
             if (rc) return rc;
         }
-        int rc = read_counters(ctx, h);
+        rc = read_counters(ctx, h);
         if (rc) return rc;
         bool retry = false;
-        if (h.err & 2u) {  // a split level or the hand-off ran out of room: repeat with twice as much
-            ctx->frontier_cap = 2 * fcap;
-            if (t.work_sharing == 3) ctx->dump_cap = 2 * std::max<uint64_t>(ctx->dump_cap, 1u << 20);
-            retry = true;
-        }
-        if (h.err & 8ull) return fail(ctx, CD_E_HIP, "batched traversal: a donated node pair was never published");
-        if (h.err & ~2ull) return fail(ctx, CD_E_STACK, "batched traversal stack overflow (err " + std::to_string(h.err) + ")");
-        if (h.lpslots > ctx->lp_cap) {
-            ctx->lp_cap = h.lpslots + h.lpslots / 4;
-            retry = true;
-        }
-        if (h.contacts > ctx->contact_cap) {
-            ctx->contact_cap = h.contacts + h.contacts / 4;
-            retry = true;
-        }
-        if (h.deferred > ctx->defer_cap || h.deferred2 > ctx->defer_cap) {  // rerun lists ran out of room
-            const uint64_t need = std::max(h.deferred, h.deferred2);
-            ctx->defer_cap = need + need / 4;
-            retry = true;
-        }
+        rc = check_batched(ctx, h, fcap, t, &retry);
+        if (rc) return rc;
         if (retry) continue;
         ctx->timings = cd_timings{};
         if (t.use_graph) {
@@ -446,14 +493,7 @@ int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
             ctx->timings.total_ms = t_split + t_dfs + t_narrow;
         }
         ctx->timings.launches = 1;
-        ctx->counters = cd_counters{};
-        ctx->counters.sat_tests = h.sat;
-        ctx->counters.leaf_pairs = h.leafpairs;
-        ctx->counters.tri_tests = h.tritests;
-        ctx->counters.contacts = h.contacts;
-        ctx->counters.fp64_fallbacks = h.fp64;
-        ctx->counters.exact_fallbacks = h.exact;
-        ctx->counters.ub_cases = h.ub;
+        set_run_counters(ctx, h);
         *n_contacts = h.contacts;
         return CD_OK;
     }
@@ -526,14 +566,20 @@ hipError_t grow_keep(DevBuf& b, size_t need, hipStream_t st) {
 // d_queries.  With wait false the pairs are copied into h_pairs without
 // waiting for them; finish_broad() takes them, and the broad phase's time,
 // after a later synchronisation of the stream.
-int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out, std::vector<uint2>* hp,
-                bool wait = true) {
+int check_entities(cd_ctx* ctx, const cd_entity* ents, uint32_t n) {
     if (!ents) return fail(ctx, CD_E_INVALID, "null entity array");
     if (n >= 0xFFFFu) return fail(ctx, CD_E_INVALID, "more entities than the 16-bit Entity type allows");
     for (uint32_t i = 0; i < n; ++i) {
         if (ents[i].mesh_id >= ctx->meshes.size()) return fail(ctx, CD_E_INVALID, "entity names an unknown mesh");
         if (!all_finite(ents[i].current, 16)) return fail(ctx, CD_E_INVALID, "non-finite entity matrix");
     }
+    return CD_OK;
+}
+
+int broad_phase(cd_ctx* ctx, const cd_entity* ents, uint32_t n, uint32_t* np_out, std::vector<uint2>* hp,
+                bool wait = true) {
+    int rc = check_entities(ctx, ents, n);
+    if (rc) return rc;
     hipStream_t st = ctx->stream;
     HCK(ctx, ctx->d_work[W_ENTS].ensure(size_t(n) * sizeof(cd_entity)));
     HCK(ctx, ctx->d_work[W_IV].ensure(size_t(n) * 6 * sizeof(float)));
@@ -716,7 +762,9 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_nq.release();
     ctx->h_stage.release();
     ctx->h_pairs.release();
+    ctx->h_step.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1365,8 +1413,95 @@ int frame_impl(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode
 }
 
 // ---- rays (CreateUncollideRays.cpp:60-200) -----------------------------------
-// cd_uncollide_rays; to_host false (cd_step): the rays stay on the device for
-// the shooting, and neither the pairs nor the rays are returned.
+// Slots of the ray generation's vertex hash set for nc contacts: more than
+// the vertices offered (3 per contact), so that every probe ends.
+uint64_t ray_hash_slots(uint32_t nc) {
+    uint64_t H = 64;
+    while (H < 6ull * nc) H <<= 1;
+    return H;
+}
+
+// Buffers of the ray generation for nc contacts (sorted in W_CSORT) and the
+// pairs' counts, begins and averages at stride spacing per side.  W_RAYS
+// holds 6 nc rays: every contact starts at most one candidate per side and a
+// candidate gives at most three rays.
+int prepare_rays(cd_ctx* ctx, uint32_t nc, uint32_t stride) {
+    HCK(ctx, ctx->d_work[W_CSORT].ensure(size_t(nc) * sizeof(cd_contact)));
+    for (int w : {W_K0, W_V0, W_K1, W_V1, W_RSTART, W_RPOS, W_RCNT, W_RPOS2})
+        HCK(ctx, ctx->d_work[w].ensure((size_t(nc) + 1) * sizeof(uint32_t)));
+    for (int w : {W_RCAND, W_RDENSE}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(RayCand)));
+    HCK(ctx, ctx->d_work[W_RSUMS].ensure((size_t(nc) / 1024 + 2) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_RQ].ensure(size_t(stride) * 4 * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_RAVG].ensure(size_t(stride) * 6 * sizeof(float)));
+    HCK(ctx, ctx->d_work[W_RTOT].ensure(64 * sizeof(uint32_t)));
+    const uint64_t H = ray_hash_slots(nc);
+    HCK(ctx, ctx->d_work[W_RHKEY].ensure(H * sizeof(unsigned long long)));
+    HCK(ctx, ctx->d_work[W_RHVAL].ensure(H * sizeof(unsigned long long)));
+    HCK(ctx, ctx->d_work[W_RAYS].ensure((6 * size_t(nc) + 1) * sizeof(cd_ray)));
+    uint32_t* table = nullptr;
+    HCK(ctx, sort_table(ctx, nc, &table));
+    return CD_OK;
+}
+
+// The ray generation on the stream, with no wait: the second side's order of
+// the sorted contacts, then per side the candidates, the claimed vertices,
+// the rays (the second side's after the first's) and the averages.  nc and
+// nq are the counts, or with nc_dev / nq_dev the capacities the launches are
+// shaped for; tot gets the totals DC_CAND0 .. DC_RAYS1.
+int enqueue_rays(cd_ctx* ctx, uint32_t nc, uint32_t nq, uint32_t stride, const uint32_t* nc_dev,
+                 const uint32_t* nq_dev, uint32_t* tot) {
+    hipStream_t st = ctx->stream;
+    const uint64_t H = ray_hash_slots(nc);
+    const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
+    uint32_t bits[3];
+    contact_field_bits(ctx, nq, bits);
+    // second side's order: (query, tri_b, tri_a)
+    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
+                              ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
+                              ctx->d_work[W_V1].as<uint32_t>(), ctx->d_work[W_SORTT].as<uint32_t>(), st, nc_dev));
+    const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
+    const QueryRec* dq = ctx->d_queries.as<QueryRec>();
+    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();  // per side: count[stride], begin[stride]
+    float* avg = ctx->d_work[W_RAVG].as<float>();      // per side: stride x 3
+    cd_ray* rays = ctx->d_work[W_RAYS].as<cd_ray>();
+    HCK(ctx, launch_fill_u32(qw, 4 * stride, 0u, st));
+    HCK(ctx, launch_fill_u32(tot, 4, 0u, st));
+    for (int side = 0; side < 2; ++side) {
+        uint32_t* t2 = tot + 2 * side;
+        const uint32_t* base = side ? &tot[DC_RAYS0] : nullptr;  // the second side's rays follow the first's
+        HCK(ctx, launch_cand(sorted, side ? orderB : nullptr, nc, side, dq, dm, ctx->d_leafof.as<uint32_t>(),
+                             ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(), st, nc_dev));
+        HCK(ctx, launch_scan(ctx->d_work[W_RSTART].as<uint32_t>(), nc, ctx->d_work[W_RPOS].as<uint32_t>(),
+                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[0], st, nc_dev));
+        HCK(ctx, launch_compact(ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(),
+                                ctx->d_work[W_RPOS].as<uint32_t>(), nc, ctx->d_work[W_RDENSE].as<RayCand>(), st, nc_dev));
+        auto* hkey = ctx->d_work[W_RHKEY].as<unsigned long long>();
+        auto* hval = ctx->d_work[W_RHVAL].as<unsigned long long>();
+        HCK(ctx, launch_fill_u64(hkey, H, ~0ull, st));
+        HCK(ctx, launch_fill_u64(hval, H, ~0ull, st));
+        HCK(ctx, launch_fill_u32(ctx->d_work[W_RCNT].as<uint32_t>(), nc + 1, 0u, st));
+        const RayCand* cand = ctx->d_work[W_RDENSE].as<RayCand>();
+        HCK(ctx, launch_vert_claim(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                                   ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, st));
+        uint32_t* qcount = qw + 2 * size_t(stride) * side;
+        uint32_t* qbegin = qcount + stride;
+        HCK(ctx, launch_ray_count(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                                  ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, ctx->d_work[W_RCNT].as<uint32_t>(),
+                                  qcount, st));
+        HCK(ctx, launch_scan(ctx->d_work[W_RCNT].as<uint32_t>(), nc, ctx->d_work[W_RPOS2].as<uint32_t>(),
+                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[1], st, nc_dev));
+        HCK(ctx, launch_ray_emit(cand, &t2[0], nc, side, dq, dm, ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(),
+                                 ctx->d_vid.as<uint32_t>(), ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), hkey, hval,
+                                 H - 1, ctx->d_work[W_RPOS2].as<uint32_t>(), 0, rays, qbegin, st, base));
+        HCK(ctx, launch_ray_avg(rays, 0, qbegin, qcount, nq, side, dq, avg + size_t(stride) * 3 * side, st, base,
+                                nq_dev));
+    }
+    return CD_OK;
+}
+
+// cd_uncollide_rays; to_host false: the rays stay on the device for the
+// shooting, and neither the pairs nor the rays are returned.
 int rays_impl(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_pairs, cd_ray* rays,
               uint64_t ray_cap, uint64_t* n_rays, bool to_host) {
     if (!ctx || !n_pairs || !n_rays) return CD_E_INVALID;
@@ -1380,87 +1515,36 @@ int rays_impl(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_p
     ctx->rays_ready = false;
     if (nc64 == 0 || nq == 0) {
         ctx->ray_total = 0;
+        ctx->ray_stride = nq;
         ctx->ray_counts.assign(2 * size_t(nq), 0u);
         ctx->rays_ready = true;
         return CD_OK;
     }
     if (nc64 > 0x3FFFFFFFull) return fail(ctx, CD_E_CAPACITY, "too many contacts for ray generation");
     const uint32_t nc = uint32_t(nc64);
-    for (int w : {W_CSORT}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(cd_contact)));
-    for (int w : {W_K0, W_V0, W_K1, W_V1, W_RSTART, W_RPOS, W_RCNT, W_RPOS2})
-        HCK(ctx, ctx->d_work[w].ensure((size_t(nc) + 1) * sizeof(uint32_t)));
-    for (int w : {W_RCAND, W_RDENSE}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(RayCand)));
-    HCK(ctx, ctx->d_work[W_RSUMS].ensure((size_t(nc) / 1024 + 2) * sizeof(uint32_t)));
-    HCK(ctx, ctx->d_work[W_RQ].ensure(size_t(nq) * 4 * sizeof(uint32_t)));
-    HCK(ctx, ctx->d_work[W_RAVG].ensure(size_t(nq) * 6 * sizeof(float)));
-    HCK(ctx, ctx->d_work[W_RTOT].ensure(64 * sizeof(uint32_t)));
-    uint64_t H = 64;
-    while (H < 6ull * nc) H <<= 1;  // more slots than vertices offered: every probe ends
-    HCK(ctx, ctx->d_work[W_RHKEY].ensure(H * sizeof(unsigned long long)));
-    HCK(ctx, ctx->d_work[W_RHVAL].ensure(H * sizeof(unsigned long long)));
-    const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
-    uint32_t* table = nullptr;
-    HCK(ctx, sort_table(ctx, nc, &table));
-    uint32_t bits[3];
-    contact_field_bits(ctx, nq, bits);
+    int rc = prepare_rays(ctx, nc, nq);
+    if (rc) return rc;
     if (!ctx->contacts_sorted) {
+        uint32_t bits[3];
+        contact_field_bits(ctx, nq, bits);
         HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), nc, ctx->d_work[W_CSORT].as<cd_contact>(), bits,
                                       ctx->d_work[W_K0].as<uint32_t>(), ctx->d_work[W_V0].as<uint32_t>(),
-                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(), table, st));
+                                      ctx->d_work[W_K1].as<uint32_t>(), ctx->d_work[W_V1].as<uint32_t>(),
+                                      ctx->d_work[W_SORTT].as<uint32_t>(), st));
         ctx->contacts_sorted = true;
     }
-    // second side's order: (query, tri_b, tri_a)
-    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
-                              ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
-                              ctx->d_work[W_V1].as<uint32_t>(), table, st));
-    const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
-    const QueryRec* dq = ctx->d_queries.as<QueryRec>();
-    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
-    uint32_t* tot = ctx->d_work[W_RTOT].as<uint32_t>();   // [0] candidates, [1] rays, per side at +2 * side
-    uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();      // per side: count[nq], begin[nq]
-    float* avg = ctx->d_work[W_RAVG].as<float>();          // per side: nq x 3
-    HCK(ctx, launch_fill_u32(qw, 4 * nq, 0u, st));
-    HCK(ctx, launch_fill_u32(tot, 64, 0u, st));
-    uint32_t side_rays[2] = {0, 0};
-    for (int side = 0; side < 2; ++side) {
-        uint32_t* t2 = tot + 2 * side;
-        HCK(ctx, launch_cand(sorted, side ? orderB : nullptr, nc, side, dq, dm, ctx->d_leafof.as<uint32_t>(),
-                             ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(), st));
-        HCK(ctx, launch_scan(ctx->d_work[W_RSTART].as<uint32_t>(), nc, ctx->d_work[W_RPOS].as<uint32_t>(),
-                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[0], st));
-        HCK(ctx, launch_compact(ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(),
-                                ctx->d_work[W_RPOS].as<uint32_t>(), nc, ctx->d_work[W_RDENSE].as<RayCand>(), st));
-        auto* hkey = ctx->d_work[W_RHKEY].as<unsigned long long>();
-        auto* hval = ctx->d_work[W_RHVAL].as<unsigned long long>();
-        HCK(ctx, launch_fill_u64(hkey, H, ~0ull, st));
-        HCK(ctx, launch_fill_u64(hval, H, ~0ull, st));
-        HCK(ctx, launch_fill_u32(ctx->d_work[W_RCNT].as<uint32_t>(), nc + 1, 0u, st));
-        const RayCand* cand = ctx->d_work[W_RDENSE].as<RayCand>();
-        HCK(ctx, launch_vert_claim(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
-                                   ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, st));
-        uint32_t* qcount = qw + 2 * size_t(nq) * side;
-        uint32_t* qbegin = qcount + nq;
-        HCK(ctx, launch_ray_count(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
-                                  ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, ctx->d_work[W_RCNT].as<uint32_t>(),
-                                  qcount, st));
-        HCK(ctx, launch_scan(ctx->d_work[W_RCNT].as<uint32_t>(), nc, ctx->d_work[W_RPOS2].as<uint32_t>(),
-                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[1], st));
-        HCK(ctx, hipMemcpyAsync(&side_rays[side], &t2[1], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HCK(ctx, hipStreamSynchronize(st));
-        const uint64_t base = side ? side_rays[0] : 0;
-        HCK(ctx, grow_keep(ctx->d_work[W_RAYS], (base + side_rays[side] + 1) * sizeof(cd_ray), st));
-        HCK(ctx, launch_ray_emit(cand, &t2[0], nc, side, dq, dm, ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(),
-                                 ctx->d_vid.as<uint32_t>(), ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), hkey, hval,
-                                 H - 1, ctx->d_work[W_RPOS2].as<uint32_t>(), base, ctx->d_work[W_RAYS].as<cd_ray>(), qbegin,
-                                 st));
-        HCK(ctx, launch_ray_avg(ctx->d_work[W_RAYS].as<cd_ray>(), base, qbegin, qcount, nq, side, dq,
-                                avg + size_t(nq) * 3 * side, st));
-    }
+    uint32_t* tot = ctx->d_work[W_RTOT].as<uint32_t>();
+    rc = enqueue_rays(ctx, nc, nq, nq, nullptr, nullptr, tot);
+    if (rc) return rc;
+    // one wait, for the totals, the pairs' counts and begins and the averages
+    uint32_t t4[4];
     std::vector<uint32_t> hq(size_t(nq) * 4);
     std::vector<float> ha(size_t(nq) * 6);
-    HCK(ctx, hipMemcpyAsync(hq.data(), qw, hq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HCK(ctx, hipMemcpyAsync(ha.data(), avg, ha.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipMemcpyAsync(t4, tot, sizeof t4, hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipMemcpyAsync(hq.data(), ctx->d_work[W_RQ].p, hq.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HCK(ctx, hipMemcpyAsync(ha.data(), ctx->d_work[W_RAVG].p, ha.size() * sizeof(float), hipMemcpyDeviceToHost, st));
     HCK(ctx, hipStreamSynchronize(st));
+    const uint32_t side_rays[2] = {t4[DC_RAYS0], t4[DC_RAYS1]};
     // pairs with rays (CollisionDetection.cpp:63); rays of the first sides, then of the second sides
     std::vector<cd_pair_rays> out;
     for (uint32_t q = 0; q < nq && to_host; ++q) {
@@ -1483,6 +1567,7 @@ int rays_impl(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_p
     const uint64_t total = uint64_t(side_rays[0]) + side_rays[1];
     ctx->ray_base1 = side_rays[0];
     ctx->ray_total = total;
+    ctx->ray_stride = nq;
     ctx->ray_counts.resize(2 * size_t(nq));
     for (uint32_t q = 0; q < nq; ++q) {
         ctx->ray_counts[q] = hq[q];
@@ -1497,6 +1582,283 @@ int rays_impl(cd_ctx* ctx, cd_pair_rays* pairs, uint32_t pair_cap, uint32_t* n_p
     if (!out.empty()) std::memcpy(pairs, out.data(), out.size() * sizeof(cd_pair_rays));
     if (total) HCK(ctx, hipMemcpy(rays, ctx->d_work[W_RAYS].p, total * sizeof(cd_ray), hipMemcpyDeviceToHost));
     return CD_OK;
+}
+
+// The response's parameters: radians and cosines as the reference computes
+// them (glm::radians, std::cos on float), defaults of
+// CollisionDetection.cpp:70-72.
+void shoot_params(const cd_shoot_params* params, float* cos_start, float* cos_finish, float* bias) {
+    const float deg = 0.01745329251994329576923690768489f;
+    const float a0 = params ? params->smoothstep_start_rads : 40.f * deg;
+    const float a1 = params ? params->smoothstep_finish_rads : 65.f * deg;
+    *bias = params ? params->distance_bias : 1.01f;
+    *cos_start = std::cos(a0);
+    *cos_finish = std::cos(a1);
+}
+
+// ---- the one-call scene step on the device ------------------------------------
+// cd_step with every count kept on the device: the broad phase, the batched
+// traversal and narrow phase, the contact order, the rays, the shots and the
+// responses are enqueued at once (and replayed as one graph), shaped for
+// capacities of pairs, contacts and shot rays.  The host reads one block of
+// results after the step and runs it again, with larger capacities, when a
+// count exceeded one.
+uint32_t pow2_at_least(uint64_t v) {
+    uint64_t p = 64;
+    while (p < v) p <<= 1;
+    return uint32_t(std::min<uint64_t>(p, 1ull << 31));
+}
+
+struct StepArgs {
+    uint32_t n;                // entities
+    uint32_t np, nc, rays;     // capacities: pairs, contacts, rays shot
+    cd_tuning t;
+    bool ref;
+    uint32_t stack_entries;    // the ray traversal's stack: the entities' deepest tree + 2
+    float cos_start, cos_finish, bias;
+};
+
+// Word offsets of the result block (W_STEPOUT on the device, h_step on the
+// host): the run's counters, the shots' counters, the count block, per pair
+// the rays of each side, the pairs, the responses.
+struct StepLayout {
+    uint32_t fctr, sctr, dc, qc0, qc1, pairs, resp, words;
+    explicit StepLayout(uint32_t np) {
+        const uint32_t rc = uint32_t(sizeof(RunCounters) / 4);
+        fctr = 0;
+        sctr = rc;
+        dc = 2 * rc;
+        qc0 = dc + DC_WORDS;
+        qc1 = qc0 + np;
+        pairs = qc1 + np;
+        resp = pairs + 2 * np;
+        words = resp + np * uint32_t(sizeof(cd_pair_response) / 4);
+    }
+};
+static_assert(sizeof(RunCounters) % 4 == 0 && sizeof(cd_pair_response) % 4 == 0, "results copied as words");
+
+int ensure_step(cd_ctx* ctx, const StepArgs& a, uint64_t* fcap) {
+    const uint32_t n = a.n, P = a.np, C = a.nc, R = a.rays;
+    HCK(ctx, ctx->d_work[W_ENTS].ensure(size_t(n) * sizeof(cd_entity)));
+    HCK(ctx, ctx->d_work[W_IV].ensure(size_t(n) * 6 * sizeof(float)));
+    for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure(size_t(std::max(n, P)) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_PCOUNT].ensure(sizeof(unsigned int)));
+    HCK(ctx, ctx->d_work[W_PAIRS].ensure(size_t(P) * sizeof(uint2)));
+    HCK(ctx, ctx->d_work[W_SPAIRS].ensure(size_t(P) * sizeof(uint2)));
+    HCK(ctx, ctx->d_queries.ensure(size_t(P) * sizeof(QueryRec)));
+    if (ctx->contact_cap < C) ctx->contact_cap = C;
+    int rc = ensure_batched(ctx, P, P, a.t, fcap);
+    if (rc) return rc;
+    rc = prepare_rays(ctx, C, P);
+    if (rc) return rc;
+    uint32_t* table = nullptr;
+    HCK(ctx, sort_table(ctx, std::max(std::max(n, P), C), &table));
+    HCK(ctx, ctx->d_work[W_PIDS].ensure(2 * size_t(P) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_SHOTS].ensure(size_t(R) * sizeof(RayShot)));
+    HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(P) * sizeof(cd_pair_response)));
+    HCK(ctx, ctx->d_work[W_SCTR].ensure(sizeof(RunCounters)));
+    HCK(ctx, ctx->d_work[W_STEPC].ensure(DC_WORDS * sizeof(uint32_t)));
+    if (a.stack_entries > uint32_t(RAY_STACK_ENTRIES)) {  // trees deeper than the LDS stack holds
+        const size_t blocks = (2 * size_t(R) + 63) / 64;
+        HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * a.stack_entries * 64 * sizeof(uint32_t)));
+    }
+    const StepLayout L(P);
+    HCK(ctx, ctx->d_work[W_STEPOUT].ensure(size_t(L.words) * sizeof(uint32_t)));
+    HCK(ctx, ctx->h_step.ensure(size_t(L.words) * sizeof(uint32_t)));
+    return CD_OK;
+}
+
+// The whole step on the stream, no host wait inside (the entities already in
+// W_ENTS); its last command copies the result block to h_step.
+int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
+    hipStream_t st = ctx->stream;
+    const uint32_t n = a.n, P = a.np, C = a.nc, R = a.rays;
+    const cd_entity* de = ctx->d_work[W_ENTS].as<cd_entity>();
+    uint32_t* dc = ctx->d_work[W_STEPC].as<uint32_t>();
+    uint32_t* k0 = ctx->d_work[W_K0].as<uint32_t>();
+    uint32_t* v0 = ctx->d_work[W_V0].as<uint32_t>();
+    uint32_t* k1 = ctx->d_work[W_K1].as<uint32_t>();
+    uint32_t* v1 = ctx->d_work[W_V1].as<uint32_t>();
+    uint32_t* table = ctx->d_work[W_SORTT].as<uint32_t>();
+    unsigned int* cnt = ctx->d_work[W_PCOUNT].as<unsigned int>();
+    uint2* pairs = ctx->d_work[W_PAIRS].as<uint2>();
+    uint2* spairs = ctx->d_work[W_SPAIRS].as<uint2>();
+    QueryRec* dq = ctx->d_queries.as<QueryRec>();
+    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    HCK(ctx, launch_fill_u32(dc, DC_WORDS, 0u, st));
+    // broad phase: sweep and prune, the pairs in (first, second) order, their relative matrices
+    HCK(ctx, launch_sap_project(de, n, dm, sap_axes(), ctx->d_work[W_IV].as<float>(), k0, v0, st));
+    HCK(ctx, launch_radix_sort(k0, v0, k1, v1, n, 32, table, st));
+    HCK(ctx, launch_fill_u32(cnt, 1, 0u, st));
+    HCK(ctx, launch_sap_sweep(v0, ctx->d_work[W_IV].as<float>(), de, n, pairs, P, cnt, st));
+    HCK(ctx, launch_pair_keys(pairs, P, k0, v0, st, cnt));
+    HCK(ctx, launch_radix_sort(k0, v0, k1, v1, P, key_bits_below(uint64_t(n) << 16), table, st, cnt));
+    HCK(ctx, launch_pair_queries(pairs, v0, P, de, dq, spairs, st, cnt));
+    HCK(ctx, launch_step_pairs(cnt, P, dc, ctx->d_nq.as<unsigned int>(), st));
+    // the pairs' contacts, ordered by (pair, tri_a, tri_b)
+    int rc = enqueue_batched(ctx, P, a.t, false, a.ref);
+    if (rc) return rc;
+    HCK(ctx, launch_step_contacts(dev_counters(ctx), std::min<uint64_t>(C, ctx->contact_cap), dc, st));
+    uint32_t bits[3];
+    contact_field_bits(ctx, P, bits);
+    HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), C, ctx->d_work[W_CSORT].as<cd_contact>(), bits, k0,
+                                  v0, k1, v1, table, st, &dc[DC_NC]));
+    // rays, shots, responses
+    rc = enqueue_rays(ctx, C, P, P, &dc[DC_NC], &dc[DC_NQ], dc);
+    if (rc) return rc;
+    uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();
+    uint32_t* pbegin = ctx->d_work[W_PIDS].as<uint32_t>();
+    RunCounters* sctr = ctx->d_work[W_SCTR].as<RunCounters>();
+    uint32_t* gstack = a.stack_entries > uint32_t(RAY_STACK_ENTRIES) ? ctx->d_work[W_GSTACK].as<uint32_t>() : nullptr;
+    HCK(ctx, launch_pair_begins(qw, P, dc, pbegin, st));
+    HCK(ctx, launch_zero(sctr, st));
+    HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), R, 0, pbegin, P, dq, dm, ctx->d_nodes.as<NodeRec>(),
+                          ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                          ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
+                          a.stack_entries, sctr, st, gstack, P, dc));
+    HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), 0, qw, P, ctx->d_work[W_RAVG].as<float>(), dq, spairs, de,
+                            a.cos_start, a.cos_finish, a.bias, ctx->d_work[W_RESP].as<cd_pair_response>(), st, P, dc,
+                            R));
+    // the results in one block, one copy to the host
+    const StepLayout L(P);
+    PackSpans sp{};
+    auto span = [&](const void* src, uint32_t words, uint32_t at) {
+        sp.src[sp.n] = static_cast<const uint32_t*>(src);
+        sp.words[sp.n] = words;
+        sp.dst[sp.n] = at;
+        ++sp.n;
+    };
+    span(ctx->d_counters.p, L.sctr - L.fctr, L.fctr);
+    span(sctr, L.dc - L.sctr, L.sctr);
+    span(dc, DC_WORDS, L.dc);
+    span(qw, P, L.qc0);
+    span(qw + 2 * size_t(P), P, L.qc1);
+    span(spairs, 2 * P, L.pairs);
+    span(ctx->d_work[W_RESP].p, L.words - L.resp, L.resp);
+    HCK(ctx, launch_pack(sp, ctx->d_work[W_STEPOUT].as<uint32_t>(), st));
+    HCK(ctx, hipMemcpyAsync(ctx->h_step.p, ctx->d_work[W_STEPOUT].p, size_t(L.words) * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+    return CD_OK;
+}
+
+// What a captured step depends on: sizes, parameters and every buffer.
+void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<uint64_t>& key) {
+    uint32_t f[3];
+    std::memcpy(&f[0], &a.cos_start, 4);
+    std::memcpy(&f[1], &a.cos_finish, 4);
+    std::memcpy(&f[2], &a.bias, 4);
+    const uint64_t k[] = {a.n, a.np, a.nc, a.rays, a.stack_entries, f[0], f[1], f[2], ctx->meshes.size()};
+    key.assign(std::begin(k), std::end(k));
+    batched_key(ctx, a.np, a.t, a.ref, fcap, key);
+    for (const DevBuf& b : ctx->d_work) key.push_back(reinterpret_cast<uint64_t>(b.p));
+    for (const DevBuf* b : {&ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm, &ctx->d_perm})
+        key.push_back(reinterpret_cast<uint64_t>(b->p));
+    key.push_back(reinterpret_cast<uint64_t>(ctx->h_step.p));
+}
+
+int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const cd_shoot_params* params,
+                uint32_t stack_entries, cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
+    hipStream_t st = ctx->stream;
+    StepArgs a{};
+    a.n = n;
+    a.t = effective_tuning(ctx);
+    a.ref = ref;
+    a.stack_entries = stack_entries;
+    shoot_params(params, &a.cos_start, &a.cos_finish, &a.bias);
+    const bool graph = ctx->tuning.step_graph != 2;
+    // first capacities; a step that outgrows one runs again with it larger,
+    // and later steps start from there
+    if (!ctx->step_np_cap) ctx->step_np_cap = pow2_at_least(n);
+    if (!ctx->step_nc_cap) ctx->step_nc_cap = 4096;
+    if (!ctx->step_ray_cap) ctx->step_ray_cap = 4096;
+    ctx->last_run = 0;
+    ctx->rays_ready = false;
+    ctx->contacts_sorted = false;
+    ctx->frame_mesh.resize(n);
+    for (uint32_t i = 0; i < n; ++i) ctx->frame_mesh[i] = ents[i].mesh_id;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        a.np = ctx->step_np_cap;
+        a.nc = ctx->step_nc_cap;
+        a.rays = ctx->step_ray_cap;
+        uint64_t fcap = 0;
+        int rc = ensure_step(ctx, a, &fcap);
+        if (rc) return rc;
+        HCK(ctx, hipMemcpyAsync(ctx->d_work[W_ENTS].p, ents, size_t(n) * sizeof(cd_entity), hipMemcpyHostToDevice, st));
+        HCK(ctx, hipEventRecord(ctx->ev[4], st));
+        if (graph) {
+            std::vector<uint64_t> key;
+            step_key(ctx, a, fcap, key);
+            if (!(ctx->step_graph && key == ctx->step_key)) {
+                ctx->step_key.clear();
+                rc = capture_graph(ctx, &ctx->step_graph, [&] { return enqueue_step(ctx, a); });
+                if (rc) return rc;
+                ctx->step_key = key;
+            }
+            HCK(ctx, hipGraphLaunch(ctx->step_graph, st));
+        } else {
+            rc = enqueue_step(ctx, a);
+            if (rc) return rc;
+        }
+        HCK(ctx, hipEventRecord(ctx->ev[5], st));
+        HCK(ctx, hipEventSynchronize(ctx->ev[5]));
+        const StepLayout L(a.np);
+        const uint32_t* w = static_cast<const uint32_t*>(ctx->h_step.p);
+        RunCounters fh{}, sh{};
+        std::memcpy(&fh, w + L.fctr, sizeof fh);
+        std::memcpy(&sh, w + L.sctr, sizeof sh);
+        const uint32_t* dcw = w + L.dc;
+        bool retry = false;
+        rc = check_batched(ctx, fh, fcap, a.t, &retry);
+        if (rc) return rc;
+        if (dcw[DC_SWEEP] > a.np) {
+            ctx->step_np_cap = pow2_at_least(uint64_t(dcw[DC_SWEEP]) + dcw[DC_SWEEP] / 4);
+            retry = true;
+        }
+        if (fh.contacts > a.nc) {
+            ctx->step_nc_cap = pow2_at_least(fh.contacts + fh.contacts / 4);
+            retry = true;
+        }
+        const uint64_t nrays = uint64_t(dcw[DC_RAYS0]) + dcw[DC_RAYS1];
+        if (nrays > a.rays) {
+            ctx->step_ray_cap = pow2_at_least(nrays + nrays / 4);
+            retry = true;
+        }
+        if (retry) continue;
+        if (sh.err) return fail(ctx, CD_E_STACK, "ray traversal stack overflow");
+        // what the separate calls would leave: the pairs, sorted contacts and rays of the step
+        const uint32_t nq = dcw[DC_NQ];
+        const uint2* hp = reinterpret_cast<const uint2*>(w + L.pairs);
+        ctx->staged_nq = nq;
+        ctx->staged_ids.assign(hp, hp + nq);
+        ctx->staged_contacts = fh.contacts;
+        ctx->contacts_sorted = true;
+        ctx->last_run = 2;
+        ctx->ray_base1 = dcw[DC_RAYS0];
+        ctx->ray_total = nrays;
+        ctx->ray_stride = a.np;
+        ctx->ray_counts.resize(2 * size_t(nq));
+        uint32_t npairs = 0;
+        for (uint32_t q = 0; q < nq; ++q) {
+            ctx->ray_counts[q] = w[L.qc0 + q];
+            ctx->ray_counts[size_t(nq) + q] = w[L.qc1 + q];
+            npairs += (w[L.qc0 + q] + w[L.qc1 + q]) ? 1u : 0u;
+        }
+        ctx->rays_ready = true;
+        set_run_counters(ctx, fh);
+        ctx->counters.broad_pairs = nq;
+        ctx->timings = cd_timings{};
+        HCK(ctx, hipEventElapsedTime(&ctx->timings.total_ms, ctx->ev[4], ctx->ev[5]));
+        ctx->timings.launches = 1;
+        *n_out = npairs;
+        if (npairs == 0) return CD_OK;
+        if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
+        const cd_pair_response* resp = reinterpret_cast<const cd_pair_response*>(w + L.resp);
+        uint32_t k = 0;
+        for (uint32_t q = 0; q < nq; ++q)
+            if (w[L.qc0 + q] + w[L.qc1 + q]) out[k++] = resp[q];
+        return CD_OK;
+    }
+    return fail(ctx, CD_E_NOMEM, "step sizing did not converge");
 }
 }  // namespace
 
@@ -1516,6 +1878,22 @@ int cd_step(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, c
             cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
     if (!ctx || !n_out) return CD_E_INVALID;
     *n_out = 0;
+    if ((mode == CD_MODE_BATCHED || mode == CD_MODE_REFERENCE) && n >= 2) {
+        (void)hipSetDevice(ctx->device);
+        int rc = check_entities(ctx, entities, n);
+        if (rc) return rc;
+        // the ray traversal's stack: the deepest tree an entity has + 2; the
+        // separate calls take a mesh beyond the ray traversal's limits
+        uint32_t stack_entries = 2;
+        bool fits = true;
+        for (uint32_t i = 0; i < n; ++i) {
+            const MeshRec& m = ctx->meshes[entities[i].mesh_id];
+            stack_entries = std::max(stack_entries, m.depth + 2);
+            fits = fits && m.ntri < (1u << RAY_LINK_BITS);
+        }
+        if (fits && ctx->leaf_max <= 64)
+            return step_device(ctx, entities, n, mode == CD_MODE_REFERENCE, params, stack_entries, out, cap, n_out);
+    }
     uint32_t np = 0;
     uint64_t nc = 0, nr = 0;
     int rc = frame_impl(ctx, entities, n, mode, nullptr, 0, &np, nullptr, 0, &nc, false);
@@ -1548,12 +1926,8 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
             if (m.ntri >= (1u << RAY_LINK_BITS) || ctx->leaf_max > 64)
                 return fail(ctx, CD_E_CAPACITY, "mesh too large for the ray traversal stack (2^25 triangles)");
         }
-    // radians and cosines as the reference computes them (glm::radians, std::cos on float)
-    const float deg = 0.01745329251994329576923690768489f;
-    const float a0 = params ? params->smoothstep_start_rads : 40.f * deg;
-    const float a1 = params ? params->smoothstep_finish_rads : 65.f * deg;
-    const float bias = params ? params->distance_bias : 1.01f;
-    const float cos_start = std::cos(a0), cos_finish = std::cos(a1);
+    float cos_start = 0.f, cos_finish = 0.f, bias = 0.f;
+    shoot_params(params, &cos_start, &cos_finish, &bias);
     HCK(ctx, ctx->d_work[W_SHOTS].ensure(std::max<uint64_t>(ctx->ray_total, 1) * sizeof(RayShot)));
     HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(nq) * sizeof(cd_pair_response)));
     // pair ids, then per side the first ray of every pair (an exclusive scan
@@ -1591,7 +1965,7 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), ctx->ray_base1, qw, nq, ctx->d_work[W_RAVG].as<float>(),
                             ctx->d_queries.as<QueryRec>(), ctx->d_work[W_PIDS].as<uint2>(),
                             ctx->d_work[W_ENTS].as<cd_entity>(), cos_start, cos_finish, bias,
-                            ctx->d_work[W_RESP].as<cd_pair_response>(), st));
+                            ctx->d_work[W_RESP].as<cd_pair_response>(), st, ctx->ray_stride));
     std::vector<cd_pair_response> all(nq);
     HCK(ctx, hipMemcpyAsync(all.data(), ctx->d_work[W_RESP].p, size_t(nq) * sizeof(cd_pair_response),
                             hipMemcpyDeviceToHost, st));
@@ -1834,7 +2208,7 @@ int cd_set_tuning(cd_ctx* ctx, const cd_tuning* t) {
         return fail(ctx, CD_E_INVALID, "stack_entries must be 512, 1024 or 2048");
     if (t->pop_width > 64 || t->split_levels > 64 || t->waves_per_cu > 64 || t->work_sharing > 3 ||
         t->handoff_pct > 10000 || t->resplit_levels > MAX_RESPLIT || t->pipeline > 2 ||
-        t->split_depth > 2)
+        t->split_depth > 2 || t->step_graph > 2)
         return fail(ctx, CD_E_INVALID, "tuning out of range");
     ctx->tuning = *t;
     return CD_OK;
@@ -1844,6 +2218,7 @@ int cd_get_tuning(cd_ctx* ctx, cd_tuning* t) {
     if (!ctx || !t) return CD_E_INVALID;
     *t = effective_tuning(ctx);
     t->use_graph = ctx->tuning.use_graph;
+    t->step_graph = ctx->tuning.step_graph == 2 ? 2 : 1;
     return CD_OK;
 }
 

@@ -101,6 +101,18 @@ struct RayShot {
 constexpr int RAY_STACK_ENTRIES = 48;  // closest-hit traversal stack per lane (LDS): a tree's depth + 2
 constexpr uint32_t RAY_LINK_BITS = 25;  // a stack entry packs a node or triangle index below 2^25 with a leaf count
 
+// Counts of the one-call scene step, kept on the device so that no stage
+// waits for the host (words of its count block): the ray generation's
+// candidates and rays of each side (the first four words are its totals),
+// the pairs the sweep found (also above its capacity) and the ones kept, and
+// the contacts the later stages take.
+constexpr uint32_t DC_CAND0 = 0, DC_RAYS0 = 1, DC_CAND1 = 2, DC_RAYS1 = 3;
+constexpr uint32_t DC_SWEEP = 4, DC_NQ = 5, DC_NC = 6, DC_WORDS = 16;
+
+// A count read on the device: min(*n_dev, n), n being the capacity a launch
+// (or a captured graph) was shaped for; without n_dev the count is n.
+__device__ __forceinline__ uint32_t count_of(uint32_t n, const uint32_t* n_dev) { return n_dev ? min(*n_dev, n) : n; }
+
 // A refit plan: small boxes (one thread each) and large boxes split into
 // chunks of triangles (device arrays, cached per mesh list).
 struct RefitPlan {
@@ -252,6 +264,7 @@ struct cd_ctx {
     bool rays_ready = false;
     uint32_t ray_base1 = 0;                // first ray of the second sides
     uint64_t ray_total = 0;
+    uint32_t ray_stride = 0;               // per-side spacing of the ray counts, begins and averages on the device
     std::vector<uint32_t> ray_counts;      // per query: rays of the first side, then (at +nq) of the second
 
     cdk::DevBuf d_queries, d_leafpairs, d_contacts, d_counters, d_tt_a, d_tt_b, d_tt_flags, d_tt_seg, d_tt_list;
@@ -260,7 +273,7 @@ struct cd_ctx {
     cdk::DevBuf d_nq;      // query count of the batched run, read by its kernels (graph replays keep it live)
     cdk::HostBuf h_nq;     // pinned staging of that count
     cdk::HostBuf h_pairs;  // pinned staging of a frame's broad-phase pairs
-    cdk::DevBuf d_work[40];                 // scratch of the batched path, the broad phase and the rays
+    cdk::DevBuf d_work[48];                 // scratch of the batched path, the broad phase, the rays and the step
     uint32_t staged_nq = 0;
     uint32_t plain_need = 0;  // stack entries the staged queries' plain walk needs (their trees' depths + 2)
     uint64_t staged_contacts = 0;
@@ -286,5 +299,11 @@ struct cd_ctx {
     std::string scratch_err;               // why it could not be resolved
     hipGraphExec_t graph = nullptr;        // captured batched pipeline
     std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured
+    // the one-call scene step: capacities (grown when a step exceeds one),
+    // its captured graph and key, and the pinned block its results land in
+    uint32_t step_np_cap = 0, step_nc_cap = 0, step_ray_cap = 0;
+    hipGraphExec_t step_graph = nullptr;
+    std::vector<uint64_t> step_key;
+    cdk::HostBuf h_step;
     hipEvent_t ev[8] = {};
 };

@@ -78,45 +78,59 @@ hipError_t launch_sap_project(const cd_entity* ents, uint32_t n, const MeshRec* 
 // Stable sort of (key, value) pairs by keys below 2^key_bits; the result is
 // in keys/vals.  Longer arrays run over many workgroups and need `table`,
 // radix_table_words(n) words of device scratch (without it they run in one).
+// n_dev (here, in the pair kernels and in the contact sorts): a device count,
+// min(*n_dev, n) used, n then being the capacity the launches are shaped for.
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-                             uint32_t key_bits, uint32_t* table, hipStream_t st);
+                             uint32_t key_bits, uint32_t* table, hipStream_t st, const uint32_t* n_dev = nullptr);
 uint64_t radix_table_words(uint32_t n);
 hipError_t launch_sap_sweep(const uint32_t* order, const float* iv, const cd_entity* ents, uint32_t n, uint2* pairs,
                             uint32_t cap, unsigned int* count, hipStream_t st);
-hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st);
+hipError_t launch_pair_keys(const uint2* pairs, uint32_t n, uint32_t* keys, uint32_t* vals, hipStream_t st,
+                            const uint32_t* n_dev = nullptr);
 hipError_t launch_pair_queries(const uint2* pairs, const uint32_t* order, uint32_t n, const cd_entity* ents,
-                               QueryRec* qs, uint2* sorted_pairs, hipStream_t st);
+                               QueryRec* qs, uint2* sorted_pairs, hipStream_t st, const uint32_t* n_dev = nullptr);
 // Contacts ordered by (query, tri_a, tri_b) into out.  field_bits: key bits
 // of query, tri_a and tri_b (values below 2^bits).
 hipError_t launch_sort_contacts(const cd_contact* in, uint32_t n, cd_contact* out, const uint32_t field_bits[3],
                                 uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* table,
-                                hipStream_t st);
+                                hipStream_t st, const uint32_t* n_dev = nullptr);
 // Stable order of contacts by three fields (0 query, 1 tri_a, 2 tri_b), most
 // significant first; the permutation ends in v0.
 hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mid, int f_lo,
                             const uint32_t field_bits[3], uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
-                            uint32_t* table, hipStream_t st);
+                            uint32_t* table, hipStream_t st, const uint32_t* n_dev = nullptr);
 
 // response.hip
 // pbegin: per side (nq each) the first ray of every pair, non-decreasing;
 // stack_entries: the deepest tree the rays meet + 2; above RAY_STACK_ENTRIES
 // the stacks live in gstack, ceil(2 nrays / 64) * 2 * stack_entries * 64
 // words; ctr->err gets bit 0 if a traversal stack would have overflowed,
-// ctr->sat the boxes tested and ctr->tritests the ray-triangle tests
+// ctr->sat the boxes tested and ctr->tritests the ray-triangle tests.
+// stride: the per-side spacing of pbegin (0: nq); dc: the step's count block
+// (DC_*), from which the ray and pair counts are read when given, nrays and
+// nq being the capacities.
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack = nullptr);
+                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack = nullptr,
+                        uint32_t stride = 0, const uint32_t* dc = nullptr);
+// qw / avg per side at stride spacing (0: nq), dc as for launch_shoot, nshots
+// the shots written.
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
-                          float cos_finish, float bias, cd_pair_response* out, hipStream_t st);
+                          float cos_finish, float bias, cd_pair_response* out, hipStream_t st, uint32_t stride = 0,
+                          const uint32_t* dc = nullptr, uint64_t nshots = ~0ull);
 
 // rays.hip
+// n_dev / nq_dev: device counts as in the sorts (n, nq the capacities);
+// base_dev: the first ray's index read on the device instead of ray_base.
 hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
-                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st);
-hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st);
+                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st,
+                       const uint32_t* n_dev = nullptr);
+hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st,
+                       const uint32_t* n_dev = nullptr);
 hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
-                          hipStream_t st);
+                          hipStream_t st, const uint32_t* n_dev = nullptr);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 hipError_t launch_vert_claim(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
                              const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
@@ -130,9 +144,29 @@ hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t 
                            const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
                            const uint32_t* vid, const float* nrm, const uint32_t* perm,
                            const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
-                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st);
+                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st,
+                           const uint32_t* base_dev = nullptr);
 hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
-                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st);
+                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st,
+                          const uint32_t* base_dev = nullptr, const uint32_t* nq_dev = nullptr);
+
+// step.hip (the one-call scene step; dc is its count block, DC_* words)
+// Word spans gathered into one block: span s, words[s] words from src[s],
+// lands at dst[s] words into the block.
+struct PackSpans {
+    static constexpr uint32_t MAX = 8;
+    const uint32_t* src[MAX];
+    uint32_t words[MAX], dst[MAX];
+    uint32_t n;
+};
+// dc[DC_SWEEP] the pairs the sweep found, dc[DC_NQ] and *d_nq those of them kept
+hipError_t launch_step_pairs(const unsigned int* sweep, uint32_t cap, uint32_t* dc, unsigned int* d_nq, hipStream_t st);
+// dc[DC_NC] the contacts of the run, up to cap
+hipError_t launch_step_contacts(const RunCounters* ctr, uint64_t cap, uint32_t* dc, hipStream_t st);
+// pbegin[stride side + q]: the first ray of pair q on each side, from the
+// counts at qw + 2 stride side, for the dc[DC_NQ] pairs
+hipError_t launch_pair_begins(const uint32_t* qw, uint32_t stride, const uint32_t* dc, uint32_t* pbegin, hipStream_t st);
+hipError_t launch_pack(const PackSpans& sp, uint32_t* dst, hipStream_t st);
 
 // refit.hip
 hipError_t launch_tree_vidx(const uint32_t* perm, uint32_t ntri, const uint32_t* indices, uint32_t vbase,

@@ -54,10 +54,12 @@ struct Corner {  // one corner of a candidate triangle, in the space its rays us
 
 }  // namespace
 
-__global__ void k_cand(const cd_contact* __restrict__ c, const uint32_t* __restrict__ order, uint32_t n, int side,
-                       const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
-                       const uint32_t* __restrict__ leafof, RayCand* __restrict__ out, uint32_t* __restrict__ start) {
+__global__ void k_cand(const cd_contact* __restrict__ c, const uint32_t* __restrict__ order, uint32_t n,
+                       const uint32_t* __restrict__ n_dev, int side, const QueryRec* __restrict__ qs,
+                       const MeshRec* __restrict__ meshes, const uint32_t* __restrict__ leafof,
+                       RayCand* __restrict__ out, uint32_t* __restrict__ start) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    n = count_of(n, n_dev);
     if (i >= n) return;
     const cd_contact& ci = at(c, order, i);
     const uint32_t tri = own_tri(ci, side);
@@ -170,9 +172,10 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
     return base + incl - v;
 }
 __global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t n,
-                                                   uint32_t* __restrict__ sums) {
+                                                   const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_w[4];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    n = count_of(n, n_dev);
     uint32_t v = 0;
     for (uint32_t k = 0; k < 4; ++k) {
         const uint32_t i = b * SCAN_BLOCK + tid * 4 + k;
@@ -196,9 +199,11 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* __restrict__ sums, u
     if (threadIdx.x == 0) *total = carry;
 }
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                    const uint32_t* __restrict__ n_dev,
                                                     const uint32_t* __restrict__ sums, uint32_t* __restrict__ out) {
     __shared__ uint32_t s_w[4];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    n = count_of(n, n_dev);
     uint32_t v[4], s = 0;
     for (uint32_t k = 0; k < 4; ++k) {
         const uint32_t i = b * SCAN_BLOCK + tid * 4 + k;
@@ -215,9 +220,10 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 }
 
 __global__ void k_compact(const RayCand* __restrict__ in, const uint32_t* __restrict__ start,
-                          const uint32_t* __restrict__ pos, uint32_t n, RayCand* __restrict__ out) {
+                          const uint32_t* __restrict__ pos, uint32_t n, const uint32_t* __restrict__ n_dev,
+                          RayCand* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && start[i]) out[pos[i]] = in[i];
+    if (i < count_of(n, n_dev) && start[i]) out[pos[i]] = in[i];
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
@@ -292,11 +298,12 @@ __global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __r
                            const uint32_t* __restrict__ vid, const float* __restrict__ nrm,
                            const uint32_t* __restrict__ perm, const unsigned long long* __restrict__ hkey,
                            const unsigned long long* __restrict__ hval, uint64_t hmask,
-                           const uint32_t* __restrict__ pos, uint64_t ray_base, cd_ray* __restrict__ rays,
-                           uint32_t* __restrict__ qbegin) {
+                           const uint32_t* __restrict__ pos, uint64_t ray_base, const uint32_t* __restrict__ base_dev,
+                           cd_ray* __restrict__ rays, uint32_t* __restrict__ qbegin) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nc = *ncand_p;
     if (i >= nc) return;
+    if (base_dev) ray_base = *base_dev;
     const RayCand r = cand[i];
     if (i == 0 || cand[i - 1].query != r.query) qbegin[r.query] = pos[i];  // candidates are in query order
     const QueryRec& q = qs[r.query];
@@ -351,12 +358,15 @@ __global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __r
 // tree, so the sum is the same on every run (the reference sums in its
 // unordered_map's order, which no GPU reproduces).
 __global__ __launch_bounds__(64) void k_ray_avg(const cd_ray* __restrict__ rays, uint64_t ray_base,
+                                                const uint32_t* __restrict__ base_dev,
                                                 const uint32_t* __restrict__ qbegin,
-                                                const uint32_t* __restrict__ qcount, uint32_t nq, int side,
+                                                const uint32_t* __restrict__ qcount, uint32_t nq,
+                                                const uint32_t* __restrict__ nq_dev, int side,
                                                 const QueryRec* __restrict__ qs, float* __restrict__ avg) {
     const uint32_t qi = blockIdx.x;
     const int lane = threadIdx.x;
-    if (qi >= nq) return;
+    if (qi >= count_of(nq, nq_dev)) return;
+    if (base_dev) ray_base = *base_dev;
     const uint32_t n = qcount[qi];
     f3 acc = mk(0.f, 0.f, 0.f);
     if (n) {
@@ -382,20 +392,22 @@ __global__ __launch_bounds__(64) void k_ray_avg(const cd_ray* __restrict__ rays,
 static inline dim3 gr(uint64_t n) { return dim3(uint32_t((std::max<uint64_t>(n, 1) + 255) / 256)); }
 
 hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
-                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st) {
-    hipLaunchKernelGGL(k_cand, gr(n), dim3(256), 0, st, c, order, n, side, qs, meshes, leafof, out, start);
+                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st,
+                       const uint32_t* n_dev) {
+    hipLaunchKernelGGL(k_cand, gr(n), dim3(256), 0, st, c, order, n, n_dev, side, qs, meshes, leafof, out, start);
     return hipGetLastError();
 }
-hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st) {
+hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st,
+                       const uint32_t* n_dev) {
     const uint32_t nb = std::max(1u, (n + SCAN_BLOCK - 1) / SCAN_BLOCK);
-    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, in, n, n_dev, sums);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, sums, nb, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, in, n, sums, out);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, in, n, n_dev, sums, out);
     return hipGetLastError();
 }
 hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(k_compact, gr(n), dim3(256), 0, st, in, start, pos, n, out);
+                          hipStream_t st, const uint32_t* n_dev) {
+    hipLaunchKernelGGL(k_compact, gr(n), dim3(256), 0, st, in, start, pos, n, n_dev, out);
     return hipGetLastError();
 }
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st) {
@@ -422,15 +434,18 @@ hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t 
                            const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
                            const uint32_t* vid, const float* nrm, const uint32_t* perm,
                            const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
-                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st) {
+                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st,
+                           const uint32_t* base_dev) {
     hipLaunchKernelGGL(k_ray_emit, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, tris, attrs, vid,
-                       nrm, perm, hkey, hval, hmask, pos, ray_base, rays, qbegin);
+                       nrm, perm, hkey, hval, hmask, pos, ray_base, base_dev, rays, qbegin);
     return hipGetLastError();
 }
 hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
-                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st) {
+                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st,
+                          const uint32_t* base_dev, const uint32_t* nq_dev) {
     if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_ray_avg, dim3(nq), dim3(64), 0, st, rays, ray_base, qbegin, qcount, nq, side, qs, avg);
+    hipLaunchKernelGGL(k_ray_avg, dim3(nq), dim3(64), 0, st, rays, ray_base, base_dev, qbegin, qcount, nq, nq_dev, side,
+                       qs, avg);
     return hipGetLastError();
 }
 

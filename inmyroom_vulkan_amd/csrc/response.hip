@@ -471,14 +471,17 @@ __device__ __forceinline__ f3 get3(const float* o) { return mk(o[0], o[1], o[2])
 // force response and the push-out vector, as the reference accumulates them
 // (a first-to-second pass subtracts its force and pushes -response, a
 // second-to-first pass adds and pushes +response).
-// pbegin: per side, the first ray of every pair (non-decreasing; a pair
-// without rays has the next pair's begin)
+// pbegin: per side (at + stride), the first ray of every pair
+// (non-decreasing; a pair without rays has the next pair's begin)
+// dc: the step's count block, when the ray and pair counts are read on the
+// device (nrays, nq then the capacities of the launch)
 // DEEP: the stacks live in HBM, a block's 2 stack_entries 64 words at
 // gstack + 2 b stack_entries 64 (link words, then entry distances), for
 // trees deeper than the LDS stack holds; the walk is the same.
 template <bool DEEP>
 __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, uint32_t nrays, uint32_t base1,
-                                              const uint32_t* __restrict__ pbegin, uint32_t nq,
+                                              const uint32_t* __restrict__ pbegin, uint32_t nq, uint32_t stride,
+                                              const uint32_t* __restrict__ dc,
                                               const QueryRec* __restrict__ qs, MeshView V,
                                               RayShot* __restrict__ out, int stack_entries,
                                               RunCounters* __restrict__ ctr, uint32_t* __restrict__ gstack) {
@@ -491,10 +494,15 @@ __global__ __launch_bounds__(64) void k_shoot(const cd_ray* __restrict__ rays, u
     RayStack S = {base, reinterpret_cast<float*>(base + stack_entries * 64), int(threadIdx.x), stack_entries};
     const uint32_t r = (blockIdx.x * blockDim.x + threadIdx.x) >> 1;  // two lanes per ray
     uint32_t nbox = 0, ntri = 0;  // boxes tested, ray-triangle tests: the work counted in ctr->sat, ctr->tritests
+    if (dc) {
+        base1 = dc[DC_RAYS0];
+        nrays = min(nrays, dc[DC_RAYS0] + dc[DC_RAYS1]);
+        nq = min(nq, dc[DC_NQ]);
+    }
     if (r < nrays) {  // both lanes of a ray take part or neither
         const int side = r >= base1 ? 1 : 0;
         const uint32_t local = r - (side ? base1 : 0u);
-        const uint32_t* begin = pbegin + size_t(nq) * side;
+        const uint32_t* begin = pbegin + size_t(stride) * side;
         // the pair: the last one whose rays begin at or before this ray (of equal
         // begins only the last can hold rays)
         uint32_t lo = 0, hi = nq;
@@ -552,22 +560,32 @@ __device__ __forceinline__ bool same_matrix(const float* a, const float* b) {
     return true;
 }
 
+__device__ __forceinline__ f3 shfl_down3(f3 v, int off) {
+    return mk(__shfl_down(v.x, off, 64), __shfl_down(v.y, off, 64), __shfl_down(v.z, off, 64));
+}
 // One wavefront per pair.  The force responses are summed lane-strided
 // (each shot's first pass, then its second) and then over a fixed shuffle
 // tree: a sum that is the same on every run.  The reference sums them in its
 // ray order, which follows an unordered_map walk no GPU reproduces.  The
 // maximum of FindResponse does not depend on the order.
-__device__ __forceinline__ f3 shfl_down3(f3 v, int off) {
-    return mk(__shfl_down(v.x, off, 64), __shfl_down(v.y, off, 64), __shfl_down(v.z, off, 64));
-}
+// qw: per side (at + 2 stride) the pairs' ray counts, then (at + stride)
+// their first rays; avg: per side (at + 3 stride) the average origins; dc as
+// for k_shoot; nshots: the shots there are (a pair whose rays would lie
+// beyond them, which only a step over its ray capacity can give, is left
+// unshot for the step to run again).
 __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shots, uint32_t base1,
-                                                const uint32_t* __restrict__ qw, uint32_t nq,
+                                                const uint32_t* __restrict__ qw, uint32_t nq, uint32_t stride,
+                                                const uint32_t* __restrict__ dc, uint64_t nshots,
                                                 const float* __restrict__ avg, const QueryRec* __restrict__ qs,
                                                 const uint2* __restrict__ pair_ids, const cd_entity* __restrict__ ents,
                                                 float cos_start, float cos_finish, float bias,
                                                 cd_pair_response* __restrict__ out) {
     const uint32_t q = blockIdx.x;
     const int lane = threadIdx.x;
+    if (dc) {
+        base1 = dc[DC_RAYS0];
+        nq = min(nq, dc[DC_NQ]);
+    }
     if (q >= nq) return;
     const uint2 ids = pair_ids[q];
     cd_pair_response res = {};
@@ -577,9 +595,10 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
     const float* c2 = ents[ids.y].current;
     const float* p1 = ents[ids.x].previous;
     const float* p2 = ents[ids.y].previous;
-    const uint32_t n0 = qw[q], b0 = qw[nq + q];
-    const uint32_t n1 = qw[2 * size_t(nq) + q], b1 = base1 + qw[3 * size_t(nq) + q];
-    if ((n0 == 0 && n1 == 0) || (same_matrix(c1, p1) && same_matrix(c2, p2))) {  // :63, :80-81
+    const uint32_t n0 = qw[q], b0 = qw[stride + q];
+    const uint32_t n1 = qw[2 * size_t(stride) + q], b1 = base1 + qw[3 * size_t(stride) + q];
+    const bool beyond = (n0 && uint64_t(b0) + n0 > nshots) || (n1 && uint64_t(b1) + n1 > nshots);
+    if ((n0 == 0 && n1 == 0) || beyond || (same_matrix(c1, p1) && same_matrix(c2, p2))) {  // :63, :80-81
         if (lane == 0) out[q] = res;
         return;
     }
@@ -623,7 +642,7 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
         C2.m[e] = c2[e];
         P2.m[e] = p2[e];
     }
-    const f3 a1 = get3(avg + 3 * size_t(q)), a2 = get3(avg + 3 * (size_t(nq) + q));
+    const f3 a1 = get3(avg + 3 * size_t(q)), a2 = get3(avg + 3 * (size_t(stride) + q));
     const float m1 = vlength(sub(mul4(C1, a1, 1.f), mul4(P1, a1, 1.f)));
     const float m2 = vlength(sub(mul4(C2, a2, 1.f), mul4(P2, a2, 1.f)));
     const float total = m1 + m2;
@@ -636,29 +655,33 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
                         const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
-                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack) {
+                        uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack, uint32_t stride,
+                        const uint32_t* dc) {
     if (!nrays) return hipSuccess;
     if (stack_entries < 2) return hipErrorInvalidValue;
+    if (!stride) stride = nq;
     const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
     const uint64_t lanes = 2ull * nrays;
     const dim3 grid(uint32_t((lanes + 63) / 64));
     if (stack_entries <= uint32_t(RAY_STACK)) {
         const size_t lds = size_t(stack_entries) * 64 * (sizeof(uint32_t) + sizeof(float));
-        hipLaunchKernelGGL(k_shoot<false>, grid, dim3(64), lds, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
-                           int(stack_entries), ctr, (uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_shoot<false>, grid, dim3(64), lds, st, rays, nrays, base1, pbegin, nq, stride, dc, qs, V,
+                           shots, int(stack_entries), ctr, (uint32_t*)nullptr);
     } else {
         if (!gstack) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_shoot<true>, grid, dim3(64), 0, st, rays, nrays, base1, pbegin, nq, qs, V, shots,
-                           int(stack_entries), ctr, gstack);
+        hipLaunchKernelGGL(k_shoot<true>, grid, dim3(64), 0, st, rays, nrays, base1, pbegin, nq, stride, dc, qs, V,
+                           shots, int(stack_entries), ctr, gstack);
     }
     return hipGetLastError();
 }
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,
                           const QueryRec* qs, const uint2* pair_ids, const cd_entity* ents, float cos_start,
-                          float cos_finish, float bias, cd_pair_response* out, hipStream_t st) {
+                          float cos_finish, float bias, cd_pair_response* out, hipStream_t st, uint32_t stride,
+                          const uint32_t* dc, uint64_t nshots) {
     if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_respond, dim3(nq), dim3(64), 0, st, shots, base1, qw, nq, avg, qs, pair_ids, ents, cos_start,
-                       cos_finish, bias, out);
+    if (!stride) stride = nq;
+    hipLaunchKernelGGL(k_respond, dim3(nq), dim3(64), 0, st, shots, base1, qw, nq, stride, dc, nshots, avg, qs, pair_ids,
+                       ents, cos_start, cos_finish, bias, out);
     return hipGetLastError();
 }
 

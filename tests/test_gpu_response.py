@@ -129,26 +129,40 @@ def test_shoot_needs_frame_and_rays(gpu):
         c.close()
 
 
-@pytest.mark.parametrize("mode", [MODE_PLAIN, MODE_REFERENCE, MODE_BATCHED])
-def test_step_equals_frame_rays_and_shoot(gpu, mode):
-    """cd_step (contacts and rays kept on the device) gives the records the
-    three calls give, bit for bit, frame after frame."""
+@pytest.mark.parametrize("mode,step_graph", [(MODE_PLAIN, 0), (MODE_REFERENCE, 0), (MODE_BATCHED, 0),
+                                             (MODE_REFERENCE, 2), (MODE_BATCHED, 2)])
+def test_step_equals_frame_rays_and_shoot(mode, step_graph):
+    """cd_step (contacts and rays kept on the device; for the batched modes
+    one graph per step, or with step_graph 2 the same kernels launched one by
+    one) gives the records the three calls give, bit for bit, frame after
+    frame, and leaves the state the three calls leave: rays and shots taken
+    after it give the same records again.  Its first step outgrows the
+    initial pair, contact and ray capacities of the on-device step and runs
+    again with larger ones."""
     sc = scenes.Scene(n_entities=96, seed=2, cube=12.0)
-    case(f"cd_step vs cd_frame + cd_uncollide_rays + cd_shoot_rays, mode {mode}")
+    case(f"cd_step vs cd_frame + cd_uncollide_rays + cd_shoot_rays, mode {mode}, step_graph {step_graph}")
+    gpu = Collider(leaf_max=4)  # a fresh context: its step starts at the initial capacities
     ids = gpu.register_soups(sc.soups)
-    shot = 0
-    for frame in range(3):
-        ents = sc.entities(ids, ENTITY_DTYPE)
-        gpu.frame(ents, mode)
-        gpu.uncollide_rays()
-        want = gpu.shoot_rays()
-        got = gpu.step(ents, mode)
-        assert got.tobytes() == want.tobytes(), frame
-        shot += int(got["shot"].sum())
-        sc.step()
-    assert shot > 0
-    e = sc.entities(ids, ENTITY_DTYPE)[:1]
-    assert len(gpu.step(e, mode)) == 0  # one entity: no pairs
-    far = sc.entities(ids, ENTITY_DTYPE)[:2]
-    far["current"][1][12] += 1e4
-    assert len(gpu.step(far, mode)) == 0  # two entities far apart
+    gpu.set_tuning(step_graph=step_graph)
+    try:
+        shot = 0
+        for frame in range(3):
+            ents = sc.entities(ids, ENTITY_DTYPE)
+            gpu.frame(ents, mode)
+            pr, rays = gpu.uncollide_rays()
+            want = gpu.shoot_rays()
+            got = gpu.step(ents, mode)
+            assert got.tobytes() == want.tobytes(), frame
+            assert gpu.shoot_rays().tobytes() == want.tobytes(), frame
+            pr2, rays2 = gpu.uncollide_rays()
+            assert pr2.tobytes() == pr.tobytes() and rays2.tobytes() == rays.tobytes(), frame
+            shot += int(got["shot"].sum())
+            sc.step()
+        assert shot > 0
+        e = sc.entities(ids, ENTITY_DTYPE)[:1]
+        assert len(gpu.step(e, mode)) == 0  # one entity: no pairs
+        far = sc.entities(ids, ENTITY_DTYPE)[:2]
+        far["current"][1][12] += 1e4
+        assert len(gpu.step(far, mode)) == 0  # two entities far apart
+    finally:
+        gpu.close()
