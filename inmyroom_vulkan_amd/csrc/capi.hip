@@ -109,8 +109,8 @@ enum { W_QPRE = 0, W_FRONT0, W_FRONT1, W_SMALL, W_SPILL,
        W_ENTS, W_IV, W_K0, W_V0, W_K1, W_V1, W_PAIRS, W_SPAIRS, W_PCOUNT, W_CSORT, W_DUMP0, W_DEFER,
        W_RSTART, W_RPOS, W_RSUMS, W_RCAND, W_RDENSE, W_RCNT, W_RPOS2, W_RHKEY, W_RHVAL, W_RQ, W_RAVG, W_RTOT,
        W_RAYS, W_TASKS, W_SORTT, W_BUD0, W_BUD1, W_BUD2, W_HITS, W_SHOTS, W_RESP, W_PIDS, W_GSTACK,
-       W_SCTR, W_STEPC, W_STEPOUT };
-static_assert(W_STEPOUT < sizeof(cd_ctx::d_work) / sizeof(DevBuf), "work slots");
+       W_SCTR, W_STEPC, W_STEPOUT, W_SEGQ, W_CSEG };
+static_assert(W_CSEG < sizeof(cd_ctx::d_work) / sizeof(DevBuf), "work slots");
 
 RunCounters* dev_counters(cd_ctx* ctx) { return ctx->d_counters.as<RunCounters>(); }
 
@@ -1447,18 +1447,20 @@ int prepare_rays(cd_ctx* ctx, uint32_t nc, uint32_t stride) {
 // the sorted contacts, then per side the candidates, the claimed vertices,
 // the rays (the second side's after the first's) and the averages.  nc and
 // nq are the counts, or with nc_dev / nq_dev the capacities the launches are
-// shaped for; tot gets the totals DC_CAND0 .. DC_RAYS1.
+// shaped for; tot gets the totals DC_CAND0 .. DC_RAYS1.  order_b: the
+// second side's order is already in W_V0.
 int enqueue_rays(cd_ctx* ctx, uint32_t nc, uint32_t nq, uint32_t stride, const uint32_t* nc_dev,
-                 const uint32_t* nq_dev, uint32_t* tot) {
+                 const uint32_t* nq_dev, uint32_t* tot, bool order_b = false) {
     hipStream_t st = ctx->stream;
     const uint64_t H = ray_hash_slots(nc);
     const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
     uint32_t bits[3];
     contact_field_bits(ctx, nq, bits);
     // second side's order: (query, tri_b, tri_a)
-    HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
-                              ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
-                              ctx->d_work[W_V1].as<uint32_t>(), ctx->d_work[W_SORTT].as<uint32_t>(), st, nc_dev));
+    if (!order_b)
+        HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
+                                  ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
+                                  ctx->d_work[W_V1].as<uint32_t>(), ctx->d_work[W_SORTT].as<uint32_t>(), st, nc_dev));
     const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
     const QueryRec* dq = ctx->d_queries.as<QueryRec>();
     const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
@@ -1614,6 +1616,7 @@ struct StepArgs {
     uint32_t np, nc, rays;     // capacities: pairs, contacts, rays shot
     cd_tuning t;
     bool ref;
+    bool radix;                // the contact order by the radix sorts (a pair beyond the per-pair order)
     uint32_t stack_entries;    // the ray traversal's stack: the entities' deepest tree + 2
     float cos_start, cos_finish, bias;
 };
@@ -1658,6 +1661,8 @@ int ensure_step(cd_ctx* ctx, const StepArgs& a, uint64_t* fcap) {
     HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(P) * sizeof(cd_pair_response)));
     HCK(ctx, ctx->d_work[W_SCTR].ensure(sizeof(RunCounters)));
     HCK(ctx, ctx->d_work[W_STEPC].ensure(DC_WORDS * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_SEGQ].ensure(3 * size_t(P) * sizeof(uint32_t)));
+    HCK(ctx, ctx->d_work[W_CSEG].ensure(size_t(C) * sizeof(cd_contact)));
     if (a.stack_entries > uint32_t(RAY_STACK_ENTRIES)) {  // trees deeper than the LDS stack holds
         const size_t blocks = (2 * size_t(R) + 63) / 64;
         HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * a.stack_entries * 64 * sizeof(uint32_t)));
@@ -1695,16 +1700,24 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
     HCK(ctx, launch_radix_sort(k0, v0, k1, v1, P, key_bits_below(uint64_t(n) << 16), table, st, cnt));
     HCK(ctx, launch_pair_queries(pairs, v0, P, de, dq, spairs, st, cnt));
     HCK(ctx, launch_step_pairs(cnt, P, dc, ctx->d_nq.as<unsigned int>(), st));
-    // the pairs' contacts, ordered by (pair, tri_a, tri_b)
+    // the pairs' contacts, ordered by (pair, tri_a, tri_b) and by (pair, tri_b, tri_a)
     int rc = enqueue_batched(ctx, P, a.t, false, a.ref);
     if (rc) return rc;
-    HCK(ctx, launch_step_contacts(dev_counters(ctx), std::min<uint64_t>(C, ctx->contact_cap), dc, st));
-    uint32_t bits[3];
-    contact_field_bits(ctx, P, bits);
-    HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), C, ctx->d_work[W_CSORT].as<cd_contact>(), bits, k0,
-                                  v0, k1, v1, table, st, &dc[DC_NC]));
+    const uint64_t ccap = std::min<uint64_t>(C, ctx->contact_cap);
+    if (a.radix) {
+        HCK(ctx, launch_step_contacts(dev_counters(ctx), ccap, dc, st));
+        uint32_t bits[3];
+        contact_field_bits(ctx, P, bits);
+        HCK(ctx, launch_sort_contacts(ctx->d_contacts.as<cd_contact>(), C, ctx->d_work[W_CSORT].as<cd_contact>(), bits,
+                                      k0, v0, k1, v1, table, st, &dc[DC_NC]));
+    } else {
+        uint32_t* qw3 = ctx->d_work[W_SEGQ].as<uint32_t>();
+        HCK(ctx, launch_step_contacts(dev_counters(ctx), ccap, dc, st, qw3, P));
+        HCK(ctx, launch_seg_order(ctx->d_contacts.as<cd_contact>(), C, P, dc, qw3, ctx->d_work[W_CSEG].as<cd_contact>(),
+                                  ctx->d_work[W_CSORT].as<cd_contact>(), v0, st));
+    }
     // rays, shots, responses
-    rc = enqueue_rays(ctx, C, P, P, &dc[DC_NC], &dc[DC_NQ], dc);
+    rc = enqueue_rays(ctx, C, P, P, &dc[DC_NC], &dc[DC_NQ], dc, !a.radix);
     if (rc) return rc;
     uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();
     uint32_t* pbegin = ctx->d_work[W_PIDS].as<uint32_t>();
@@ -1747,7 +1760,8 @@ void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<u
     std::memcpy(&f[0], &a.cos_start, 4);
     std::memcpy(&f[1], &a.cos_finish, 4);
     std::memcpy(&f[2], &a.bias, 4);
-    const uint64_t k[] = {a.n, a.np, a.nc, a.rays, a.stack_entries, f[0], f[1], f[2], ctx->meshes.size()};
+    const uint64_t k[] = {a.n, a.np, a.nc, a.rays, a.stack_entries, f[0], f[1], f[2], ctx->meshes.size(),
+                          uint64_t(a.radix)};
     key.assign(std::begin(k), std::end(k));
     batched_key(ctx, a.np, a.t, a.ref, fcap, key);
     for (const DevBuf& b : ctx->d_work) key.push_back(reinterpret_cast<uint64_t>(b.p));
@@ -1780,6 +1794,7 @@ int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const 
         a.np = ctx->step_np_cap;
         a.nc = ctx->step_nc_cap;
         a.rays = ctx->step_ray_cap;
+        a.radix = ctx->step_radix;
         uint64_t fcap = 0;
         int rc = ensure_step(ctx, a, &fcap);
         if (rc) return rc;
@@ -1816,6 +1831,10 @@ int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const 
         }
         if (fh.contacts > a.nc) {
             ctx->step_nc_cap = pow2_at_least(fh.contacts + fh.contacts / 4);
+            retry = true;
+        }
+        if (dcw[DC_BIGSEG]) {
+            ctx->step_radix = true;
             retry = true;
         }
         const uint64_t nrays = uint64_t(dcw[DC_RAYS0]) + dcw[DC_RAYS1];

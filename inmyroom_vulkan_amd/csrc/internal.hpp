@@ -105,9 +105,10 @@ constexpr uint32_t RAY_LINK_BITS = 25;  // a stack entry packs a node or triangl
 // waits for the host (words of its count block): the ray generation's
 // candidates and rays of each side (the first four words are its totals),
 // the pairs the sweep found (also above its capacity) and the ones kept, and
-// the contacts the later stages take.
+// the contacts the later stages take, and a flag set when a pair has more
+// contacts than the step's contact order handles.
 constexpr uint32_t DC_CAND0 = 0, DC_RAYS0 = 1, DC_CAND1 = 2, DC_RAYS1 = 3;
-constexpr uint32_t DC_SWEEP = 4, DC_NQ = 5, DC_NC = 6, DC_WORDS = 16;
+constexpr uint32_t DC_SWEEP = 4, DC_NQ = 5, DC_NC = 6, DC_BIGSEG = 7, DC_WORDS = 16;
 
 // A count read on the device: min(*n_dev, n), n being the capacity a launch
 // (or a captured graph) was shaped for; without n_dev the count is n.
@@ -302,6 +303,7 @@ struct cd_ctx {
     // the one-call scene step: capacities (grown when a step exceeds one),
     // its captured graph and key, and the pinned block its results land in
     uint32_t step_np_cap = 0, step_nc_cap = 0, step_ray_cap = 0;
+    bool step_radix = false;  // a pair had more contacts than the per-pair order ranks: the radix sorts
     hipGraphExec_t step_graph = nullptr;
     std::vector<uint64_t> step_key;
     cdk::HostBuf h_step;

@@ -161,8 +161,18 @@ struct PackSpans {
 };
 // dc[DC_SWEEP] the pairs the sweep found, dc[DC_NQ] and *d_nq those of them kept
 hipError_t launch_step_pairs(const unsigned int* sweep, uint32_t cap, uint32_t* dc, unsigned int* d_nq, hipStream_t st);
-// dc[DC_NC] the contacts of the run, up to cap
-hipError_t launch_step_contacts(const RunCounters* ctr, uint64_t cap, uint32_t* dc, hipStream_t st);
+// dc[DC_NC] the contacts of the run, up to cap; qcnt (np words), when given,
+// cleared for launch_seg_order
+hipError_t launch_step_contacts(const RunCounters* ctr, uint64_t cap, uint32_t* dc, hipStream_t st,
+                                uint32_t* qcnt = nullptr, uint32_t np = 0);
+// The dc[DC_NC] contacts at c, of dc[DC_NQ] pairs, ordered by (pair, tri_a,
+// tri_b) into sorted, and order_b their permutation by (pair, tri_b, tri_a);
+// capacities nc_cap contacts and np_cap pairs; qw3 3 np_cap words (cleared
+// counts first), seg nc_cap contacts of scratch.  dc[DC_BIGSEG] is set when
+// a pair has more contacts than one workgroup ranks, and the order is then
+// incomplete.
+hipError_t launch_seg_order(const cd_contact* c, uint32_t nc_cap, uint32_t np_cap, uint32_t* dc, uint32_t* qw3,
+                            cd_contact* seg, cd_contact* sorted, uint32_t* order_b, hipStream_t st);
 // pbegin[stride side + q]: the first ray of pair q on each side, from the
 // counts at qw + 2 stride side, for the dc[DC_NQ] pairs
 hipError_t launch_pair_begins(const uint32_t* qw, uint32_t stride, const uint32_t* dc, uint32_t* pbegin, hipStream_t st);

@@ -166,3 +166,34 @@ def test_step_equals_frame_rays_and_shoot(mode, step_graph):
         assert len(gpu.step(far, mode)) == 0  # two entities far apart
     finally:
         gpu.close()
+
+
+@pytest.mark.parametrize("step_graph", [0, 2])
+def test_step_pair_beyond_per_pair_order(step_graph):
+    """An icosphere inside a copy of itself with seeded radial noise: the two
+    surfaces cross all over, one pair with more contacts than the step's
+    per-pair contact order ranks (2,048).  The step runs again with the radix
+    sorts and still gives the three calls' records, bit for bit."""
+    case(f"cd_step, one pair beyond the per-pair contact order, step_graph {step_graph}")
+    P, F = scenes.icosphere(12)
+    rng = np.random.default_rng(7)
+    Q = P * (1.0 + rng.uniform(-0.02, 0.02, (len(P), 1)))
+    gpu = Collider(leaf_max=4)
+    try:
+        gpu.set_tuning(step_graph=step_graph)
+        a = gpu.register_mesh(P.astype(np.float32), F)
+        b = gpu.register_mesh(Q.astype(np.float32), F)
+        e = np.zeros(2, ENTITY_DTYPE)
+        e["current"] = [scenes.identity16(), scenes.identity16()]
+        e["previous"] = [_tmat(-0.2), scenes.identity16()]
+        e["mesh_id"] = [a, b]
+        e["should_callback"] = 1
+        for mode in (MODE_REFERENCE, MODE_BATCHED):
+            pairs, _ = gpu.frame(e, mode)
+            assert len(pairs) == 1 and int(pairs[0]["contact_count"]) > 2048, pairs
+            gpu.uncollide_rays()
+            want = gpu.shoot_rays()
+            got = gpu.step(e, mode)
+            assert len(got) == 1 and got.tobytes() == want.tobytes(), mode
+    finally:
+        gpu.close()
