@@ -1420,23 +1420,27 @@ uint64_t ray_hash_slots(uint32_t nc) {
     while (H < 6ull * nc) H <<= 1;
     return H;
 }
+// Block sums of the ray generation's scans per side (one per 1,024 contacts).
+uint32_t ray_sums(uint32_t nc) { return std::max(1u, (nc + 1023) / 1024); }
 
 // Buffers of the ray generation for nc contacts (sorted in W_CSORT) and the
-// pairs' counts, begins and averages at stride spacing per side.  W_RAYS
-// holds 6 nc rays: every contact starts at most one candidate per side and a
-// candidate gives at most three rays.
+// pairs' counts, begins and averages at stride spacing per side; the
+// per-candidate buffers and hash sets once per side.  W_RAYS holds 6 nc
+// rays: every contact starts at most one candidate per side and a candidate
+// gives at most three rays.
 int prepare_rays(cd_ctx* ctx, uint32_t nc, uint32_t stride) {
     HCK(ctx, ctx->d_work[W_CSORT].ensure(size_t(nc) * sizeof(cd_contact)));
-    for (int w : {W_K0, W_V0, W_K1, W_V1, W_RSTART, W_RPOS, W_RCNT, W_RPOS2})
-        HCK(ctx, ctx->d_work[w].ensure((size_t(nc) + 1) * sizeof(uint32_t)));
-    for (int w : {W_RCAND, W_RDENSE}) HCK(ctx, ctx->d_work[w].ensure(size_t(nc) * sizeof(RayCand)));
-    HCK(ctx, ctx->d_work[W_RSUMS].ensure((size_t(nc) / 1024 + 2) * sizeof(uint32_t)));
+    for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure((size_t(nc) + 1) * sizeof(uint32_t)));
+    for (int w : {W_RSTART, W_RPOS, W_RCNT, W_RPOS2})
+        HCK(ctx, ctx->d_work[w].ensure(2 * (size_t(nc) + 1) * sizeof(uint32_t)));
+    for (int w : {W_RCAND, W_RDENSE}) HCK(ctx, ctx->d_work[w].ensure(2 * size_t(nc) * sizeof(RayCand)));
+    HCK(ctx, ctx->d_work[W_RSUMS].ensure(2 * ray_sums(nc) * sizeof(uint32_t)));
     HCK(ctx, ctx->d_work[W_RQ].ensure(size_t(stride) * 4 * sizeof(uint32_t)));
     HCK(ctx, ctx->d_work[W_RAVG].ensure(size_t(stride) * 6 * sizeof(float)));
     HCK(ctx, ctx->d_work[W_RTOT].ensure(64 * sizeof(uint32_t)));
     const uint64_t H = ray_hash_slots(nc);
-    HCK(ctx, ctx->d_work[W_RHKEY].ensure(H * sizeof(unsigned long long)));
-    HCK(ctx, ctx->d_work[W_RHVAL].ensure(H * sizeof(unsigned long long)));
+    HCK(ctx, ctx->d_work[W_RHKEY].ensure(2 * H * sizeof(unsigned long long)));
+    HCK(ctx, ctx->d_work[W_RHVAL].ensure(2 * H * sizeof(unsigned long long)));
     HCK(ctx, ctx->d_work[W_RAYS].ensure((6 * size_t(nc) + 1) * sizeof(cd_ray)));
     uint32_t* table = nullptr;
     HCK(ctx, sort_table(ctx, nc, &table));
@@ -1444,61 +1448,48 @@ int prepare_rays(cd_ctx* ctx, uint32_t nc, uint32_t stride) {
 }
 
 // The ray generation on the stream, with no wait: the second side's order of
-// the sorted contacts, then per side the candidates, the claimed vertices,
-// the rays (the second side's after the first's) and the averages.  nc and
-// nq are the counts, or with nc_dev / nq_dev the capacities the launches are
-// shaped for; tot gets the totals DC_CAND0 .. DC_RAYS1.  order_b: the
-// second side's order is already in W_V0.
+// the sorted contacts (unless order_b: already in W_V0), then both sides'
+// candidates, claimed vertices, rays (the second side's after the first's)
+// and averages.  nc and nq are the counts, or with nc_dev / nq_dev the
+// capacities the launches are shaped for; tot gets the totals DC_CAND0 ..
+// DC_RAYS1.
 int enqueue_rays(cd_ctx* ctx, uint32_t nc, uint32_t nq, uint32_t stride, const uint32_t* nc_dev,
                  const uint32_t* nq_dev, uint32_t* tot, bool order_b = false) {
     hipStream_t st = ctx->stream;
-    const uint64_t H = ray_hash_slots(nc);
     const cd_contact* sorted = ctx->d_work[W_CSORT].as<cd_contact>();
-    uint32_t bits[3];
-    contact_field_bits(ctx, nq, bits);
-    // second side's order: (query, tri_b, tri_a)
-    if (!order_b)
+    if (!order_b) {  // second side's order: (query, tri_b, tri_a)
+        uint32_t bits[3];
+        contact_field_bits(ctx, nq, bits);
         HCK(ctx, launch_sort_perm(sorted, nc, 0, 2, 1, bits, ctx->d_work[W_K0].as<uint32_t>(),
                                   ctx->d_work[W_V0].as<uint32_t>(), ctx->d_work[W_K1].as<uint32_t>(),
                                   ctx->d_work[W_V1].as<uint32_t>(), ctx->d_work[W_SORTT].as<uint32_t>(), st, nc_dev));
-    const uint32_t* orderB = ctx->d_work[W_V0].as<uint32_t>();
-    const QueryRec* dq = ctx->d_queries.as<QueryRec>();
-    const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
-    uint32_t* qw = ctx->d_work[W_RQ].as<uint32_t>();  // per side: count[stride], begin[stride]
-    float* avg = ctx->d_work[W_RAVG].as<float>();      // per side: stride x 3
-    cd_ray* rays = ctx->d_work[W_RAYS].as<cd_ray>();
-    HCK(ctx, launch_fill_u32(qw, 4 * stride, 0u, st));
-    HCK(ctx, launch_fill_u32(tot, 4, 0u, st));
-    for (int side = 0; side < 2; ++side) {
-        uint32_t* t2 = tot + 2 * side;
-        const uint32_t* base = side ? &tot[DC_RAYS0] : nullptr;  // the second side's rays follow the first's
-        HCK(ctx, launch_cand(sorted, side ? orderB : nullptr, nc, side, dq, dm, ctx->d_leafof.as<uint32_t>(),
-                             ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(), st, nc_dev));
-        HCK(ctx, launch_scan(ctx->d_work[W_RSTART].as<uint32_t>(), nc, ctx->d_work[W_RPOS].as<uint32_t>(),
-                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[0], st, nc_dev));
-        HCK(ctx, launch_compact(ctx->d_work[W_RCAND].as<RayCand>(), ctx->d_work[W_RSTART].as<uint32_t>(),
-                                ctx->d_work[W_RPOS].as<uint32_t>(), nc, ctx->d_work[W_RDENSE].as<RayCand>(), st, nc_dev));
-        auto* hkey = ctx->d_work[W_RHKEY].as<unsigned long long>();
-        auto* hval = ctx->d_work[W_RHVAL].as<unsigned long long>();
-        HCK(ctx, launch_fill_u64(hkey, H, ~0ull, st));
-        HCK(ctx, launch_fill_u64(hval, H, ~0ull, st));
-        HCK(ctx, launch_fill_u32(ctx->d_work[W_RCNT].as<uint32_t>(), nc + 1, 0u, st));
-        const RayCand* cand = ctx->d_work[W_RDENSE].as<RayCand>();
-        HCK(ctx, launch_vert_claim(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
-                                   ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, st));
-        uint32_t* qcount = qw + 2 * size_t(stride) * side;
-        uint32_t* qbegin = qcount + stride;
-        HCK(ctx, launch_ray_count(cand, &t2[0], nc, side, dq, dm, ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
-                                  ctx->d_perm.as<uint32_t>(), hkey, hval, H - 1, ctx->d_work[W_RCNT].as<uint32_t>(),
-                                  qcount, st));
-        HCK(ctx, launch_scan(ctx->d_work[W_RCNT].as<uint32_t>(), nc, ctx->d_work[W_RPOS2].as<uint32_t>(),
-                             ctx->d_work[W_RSUMS].as<uint32_t>(), &t2[1], st, nc_dev));
-        HCK(ctx, launch_ray_emit(cand, &t2[0], nc, side, dq, dm, ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(),
-                                 ctx->d_vid.as<uint32_t>(), ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), hkey, hval,
-                                 H - 1, ctx->d_work[W_RPOS2].as<uint32_t>(), 0, rays, qbegin, st, base));
-        HCK(ctx, launch_ray_avg(rays, 0, qbegin, qcount, nq, side, dq, avg + size_t(stride) * 3 * side, st, base,
-                                nq_dev));
     }
+    RaySides b{};
+    b.sorted = sorted;
+    b.order_b = ctx->d_work[W_V0].as<uint32_t>();
+    b.n = nc;
+    b.n_dev = nc_dev;
+    b.nq = nq;
+    b.nq_dev = nq_dev;
+    b.qstride = stride;
+    b.cand = ctx->d_work[W_RCAND].as<RayCand>();
+    b.dense = ctx->d_work[W_RDENSE].as<RayCand>();
+    b.start = ctx->d_work[W_RSTART].as<uint32_t>();
+    b.pos = ctx->d_work[W_RPOS].as<uint32_t>();
+    b.cnt = ctx->d_work[W_RCNT].as<uint32_t>();
+    b.pos2 = ctx->d_work[W_RPOS2].as<uint32_t>();
+    b.sums = ctx->d_work[W_RSUMS].as<uint32_t>();
+    b.nsums = ray_sums(nc);
+    b.hkey = ctx->d_work[W_RHKEY].as<unsigned long long>();
+    b.hval = ctx->d_work[W_RHVAL].as<unsigned long long>();
+    b.H = ray_hash_slots(nc);
+    b.qw = ctx->d_work[W_RQ].as<uint32_t>();
+    b.avg = ctx->d_work[W_RAVG].as<float>();
+    b.tot = tot;
+    b.rays = ctx->d_work[W_RAYS].as<cd_ray>();
+    HCK(ctx, launch_rays(b, ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_leafof.as<uint32_t>(),
+                         ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                         ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), st));
     return CD_OK;
 }
 

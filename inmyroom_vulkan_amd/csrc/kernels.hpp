@@ -122,33 +122,38 @@ hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* 
                           const uint32_t* dc = nullptr, uint64_t nshots = ~0ull);
 
 // rays.hip
-// n_dev / nq_dev: device counts as in the sorts (n, nq the capacities);
-// base_dev: the first ray's index read on the device instead of ray_base.
-hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
-                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st,
-                       const uint32_t* n_dev = nullptr);
-hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st,
-                       const uint32_t* n_dev = nullptr);
-hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
-                          hipStream_t st, const uint32_t* n_dev = nullptr);
-hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
-hipError_t launch_vert_claim(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                             const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
-                             const uint32_t* perm, unsigned long long* hkey, unsigned long long* hval, uint64_t hmask,
-                             hipStream_t st);
-hipError_t launch_ray_count(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                            const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
-                            const uint32_t* perm, const unsigned long long* hkey, const unsigned long long* hval,
-                            uint64_t hmask, uint32_t* cnt, uint32_t* qcount, hipStream_t st);
-hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                           const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
-                           const uint32_t* vid, const float* nrm, const uint32_t* perm,
-                           const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
-                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st,
-                           const uint32_t* base_dev = nullptr);
-hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
-                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st,
-                          const uint32_t* base_dev = nullptr, const uint32_t* nq_dev = nullptr);
+// The ray generation's buffers, both sides: side s's part of a per-side
+// buffer starts s times its size in.  n, nq: contacts and pairs (the
+// capacities the launches are shaped for when n_dev / nq_dev give the counts
+// on the device).
+struct RaySides {
+    const cd_contact* sorted;   // contacts by (pair, tri_a, tri_b)
+    const uint32_t* order_b;    // their order by (pair, tri_b, tri_a)
+    uint32_t n;
+    const uint32_t* n_dev;
+    uint32_t nq;
+    const uint32_t* nq_dev;
+    uint32_t qstride;           // per-side spacing of qw (counts, then begins) and avg (x 3)
+    RayCand* cand;              // n per side
+    RayCand* dense;             // n per side
+    uint32_t* start;            // n + 1 per side (likewise pos, cnt, pos2)
+    uint32_t* pos;
+    uint32_t* cnt;
+    uint32_t* pos2;
+    uint32_t* sums;             // nsums per side: one per 1,024 contacts
+    uint32_t nsums;
+    unsigned long long* hkey;   // H per side (a power of two, at least 6 n)
+    unsigned long long* hval;
+    uint64_t H;
+    uint32_t* qw;               // per side: the pairs' ray counts, then their first rays
+    float* avg;                 // per side: the pairs' average ray origins
+    uint32_t* tot;              // DC_CAND0, DC_RAYS0, DC_CAND1, DC_RAYS1
+    cd_ray* rays;               // the first side's rays, then the second side's: 6 n at most
+};
+// The whole ray generation (twelve launches, both sides in each).
+hipError_t launch_rays(const RaySides& b, const QueryRec* qs, const MeshRec* meshes, const uint32_t* leafof,
+                       const float* tris, const uint2* attrs, const uint32_t* vid, const float* nrm,
+                       const uint32_t* perm, hipStream_t st);
 
 // step.hip (the one-call scene step; dc is its count block, DC_* words)
 // Word spans gathered into one block: span s, words[s] words from src[s],

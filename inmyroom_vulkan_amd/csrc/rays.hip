@@ -3,7 +3,9 @@
 //  inMyRoom_vulkan/src/CollisionDetection/CreateUncollideRays.cpp:60-200).
 //
 // Input: the contacts of a run sorted by (query, tri_a, tri_b), and a
-// permutation of them sorted by (query, tri_b, tri_a).  For each side:
+// permutation of them sorted by (query, tri_b, tri_a).  Every launch runs
+// both sides (grid y: the side, each side with its own part of the buffers,
+// RaySides); for each side:
 //  k_cand        one thread per run of contacts with the same (query,
 //                triangle): weight |s - t|, weighted midpoint and the
 //                vertex-not-outside mask, summed per leaf of the other tree
@@ -26,6 +28,9 @@
 //                first's space (:135, :146, :160)
 //  k_ray_avg     average origin per query and side (:179-197), one
 //                wavefront per query
+// k_cand also clears what the later kernels accumulate into (the pairs' ray
+// counts, the hash sets, the per-candidate ray counts), so the chain is
+// twelve launches with no fills.
 #include "rayattr.hpp"
 
 namespace cdk {
@@ -54,12 +59,27 @@ struct Corner {  // one corner of a candidate triangle, in the space its rays us
 
 }  // namespace
 
-__global__ void k_cand(const cd_contact* __restrict__ c, const uint32_t* __restrict__ order, uint32_t n,
-                       const uint32_t* __restrict__ n_dev, int side, const QueryRec* __restrict__ qs,
-                       const MeshRec* __restrict__ meshes, const uint32_t* __restrict__ leafof,
-                       RayCand* __restrict__ out, uint32_t* __restrict__ start) {
+__global__ void k_cand(RaySides b, const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+                       const uint32_t* __restrict__ leafof) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    n = count_of(n, n_dev);
+    const int side = int(blockIdx.y);
+    const cd_contact* __restrict__ c = b.sorted;
+    const uint32_t* __restrict__ order = side ? b.order_b : nullptr;
+    RayCand* __restrict__ out = b.cand + size_t(b.n) * side;
+    uint32_t* __restrict__ start = b.start + (size_t(b.n) + 1) * side;
+    {  // clears: the hash sets and the pairs' counts (by side 0's threads), this side's ray counts
+        const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+        if (side == 0) {
+            for (uint64_t k = i; k < 2 * b.H; k += stride) {
+                b.hkey[k] = ~0ull;
+                b.hval[k] = ~0ull;
+            }
+            for (uint64_t k = i; k < 4 * uint64_t(b.qstride); k += stride) b.qw[k] = 0u;
+        }
+        uint32_t* cnt = b.cnt + (size_t(b.n) + 1) * side;
+        for (uint64_t k = i; k <= b.n; k += stride) cnt[k] = 0u;
+    }
+    const uint32_t n = count_of(b.n, b.n_dev);
     if (i >= n) return;
     const cd_contact& ci = at(c, order, i);
     const uint32_t tri = own_tri(ci, side);
@@ -171,10 +191,15 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
     total = all;
     return base + incl - v;
 }
+// Grid y: the side, whose input and output start y (n + 1) words in and
+// whose block sums y nb words in (nb: the blocks per side); total per side
+// at total + 2 y.
 __global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ in, uint32_t n,
                                                    const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_w[4];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    in += (size_t(n) + 1) * blockIdx.y;
+    sums += size_t(gridDim.x) * blockIdx.y;
     n = count_of(n, n_dev);
     uint32_t v = 0;
     for (uint32_t k = 0; k < 4; ++k) {
@@ -187,6 +212,8 @@ __global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ 
 }
 __global__ __launch_bounds__(256) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ total) {
     __shared__ uint32_t s_w[4];
+    sums += size_t(nb) * blockIdx.y;
+    total += 2 * blockIdx.y;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += 256) {
         const uint32_t i = base + threadIdx.x;
@@ -203,6 +230,9 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ sums, uint32_t* __restrict__ out) {
     __shared__ uint32_t s_w[4];
     const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    in += (size_t(n) + 1) * blockIdx.y;
+    out += (size_t(n) + 1) * blockIdx.y;
+    sums += size_t(gridDim.x) * blockIdx.y;
     n = count_of(n, n_dev);
     uint32_t v[4], s = 0;
     for (uint32_t k = 0; k < 4; ++k) {
@@ -219,16 +249,11 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     }
 }
 
-__global__ void k_compact(const RayCand* __restrict__ in, const uint32_t* __restrict__ start,
-                          const uint32_t* __restrict__ pos, uint32_t n, const uint32_t* __restrict__ n_dev,
-                          RayCand* __restrict__ out) {
+__global__ void k_compact(RaySides b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count_of(n, n_dev) && start[i]) out[pos[i]] = in[i];
-}
-
-__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    const uint32_t side = blockIdx.y;
+    const size_t cs = size_t(b.n) * side, us = (size_t(b.n) + 1) * side;
+    if (i < count_of(b.n, b.n_dev) && b.start[us + i]) b.dense[cs + b.pos[us + i]] = b.cand[cs + i];
 }
 
 // Hash set of claimed vertices: key (query, side, vertex id), value the
@@ -237,13 +262,16 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
 __device__ __forceinline__ uint64_t vkey(uint32_t query, int side, uint32_t v) {
     return (uint64_t(query) << 33) | (uint64_t(side) << 32) | uint64_t(v);
 }
-__global__ void k_vert_claim(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
-                             const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+__global__ void k_vert_claim(RaySides b, const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
                              const uint2* __restrict__ attrs, const uint32_t* __restrict__ vid,
-                             const uint32_t* __restrict__ perm, unsigned long long* __restrict__ hkey,
-                             unsigned long long* __restrict__ hval, uint64_t hmask) {
+                             const uint32_t* __restrict__ perm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *ncand_p) return;
+    const int side = int(blockIdx.y);
+    if (i >= b.tot[2 * side]) return;
+    const RayCand* __restrict__ cand = b.dense + size_t(b.n) * side;
+    unsigned long long* __restrict__ hkey = b.hkey + b.H * side;
+    unsigned long long* __restrict__ hval = b.hval + b.H * side;
+    const uint64_t hmask = b.H - 1;
     const RayCand r = cand[i];
     if (r.mask == 0u) return;
     const uint32_t mesh = side ? qs[r.query].mesh_b : qs[r.query].mesh_a;
@@ -268,15 +296,18 @@ __device__ __forceinline__ bool claimed(const unsigned long long* hkey, const un
     return hval[h] == ((static_cast<unsigned long long>(tri) << 2) | static_cast<unsigned long long>(k));
 }
 
-__global__ void k_ray_count(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
-                            const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+__global__ void k_ray_count(RaySides b, const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
                             const uint2* __restrict__ attrs, const uint32_t* __restrict__ vid,
-                            const uint32_t* __restrict__ perm, const unsigned long long* __restrict__ hkey,
-                            const unsigned long long* __restrict__ hval, uint64_t hmask, uint32_t* __restrict__ cnt,
-                            uint32_t* __restrict__ qcount) {
+                            const uint32_t* __restrict__ perm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nc = *ncand_p;
-    if (i >= nc) return;
+    const int side = int(blockIdx.y);
+    if (i >= b.tot[2 * side]) return;
+    const RayCand* __restrict__ cand = b.dense + size_t(b.n) * side;
+    const unsigned long long* __restrict__ hkey = b.hkey + b.H * side;
+    const unsigned long long* __restrict__ hval = b.hval + b.H * side;
+    const uint64_t hmask = b.H - 1;
+    uint32_t* __restrict__ cnt = b.cnt + (size_t(b.n) + 1) * side;
+    uint32_t* __restrict__ qcount = b.qw + 2 * size_t(b.qstride) * side;
     const RayCand r = cand[i];
     uint32_t k_rays = 1u;  // fallback ray (:155-167)
     if (r.mask != 0u) {
@@ -292,18 +323,21 @@ __global__ void k_ray_count(const RayCand* __restrict__ cand, const uint32_t* __
     atomicAdd(&qcount[r.query], k_rays);
 }
 
-__global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __restrict__ ncand_p, int side,
-                           const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
+__global__ void k_ray_emit(RaySides b, const QueryRec* __restrict__ qs, const MeshRec* __restrict__ meshes,
                            const float* __restrict__ tris, const uint2* __restrict__ attrs,
                            const uint32_t* __restrict__ vid, const float* __restrict__ nrm,
-                           const uint32_t* __restrict__ perm, const unsigned long long* __restrict__ hkey,
-                           const unsigned long long* __restrict__ hval, uint64_t hmask,
-                           const uint32_t* __restrict__ pos, uint64_t ray_base, const uint32_t* __restrict__ base_dev,
-                           cd_ray* __restrict__ rays, uint32_t* __restrict__ qbegin) {
+                           const uint32_t* __restrict__ perm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nc = *ncand_p;
-    if (i >= nc) return;
-    if (base_dev) ray_base = *base_dev;
+    const int side = int(blockIdx.y);
+    if (i >= b.tot[2 * side]) return;
+    const RayCand* __restrict__ cand = b.dense + size_t(b.n) * side;
+    const unsigned long long* __restrict__ hkey = b.hkey + b.H * side;
+    const unsigned long long* __restrict__ hval = b.hval + b.H * side;
+    const uint64_t hmask = b.H - 1;
+    const uint32_t* __restrict__ pos = b.pos2 + (size_t(b.n) + 1) * side;
+    const uint64_t ray_base = side ? b.tot[DC_RAYS0] : 0u;  // the second side's rays follow the first's
+    cd_ray* __restrict__ rays = b.rays;
+    uint32_t* __restrict__ qbegin = b.qw + 2 * size_t(b.qstride) * side + b.qstride;
     const RayCand r = cand[i];
     if (i == 0 || cand[i - 1].query != r.query) qbegin[r.query] = pos[i];  // candidates are in query order
     const QueryRec& q = qs[r.query];
@@ -357,16 +391,16 @@ __global__ void k_ray_emit(const RayCand* __restrict__ cand, const uint32_t* __r
 // One wavefront per query: lane-strided partial sums, then a fixed shuffle
 // tree, so the sum is the same on every run (the reference sums in its
 // unordered_map's order, which no GPU reproduces).
-__global__ __launch_bounds__(64) void k_ray_avg(const cd_ray* __restrict__ rays, uint64_t ray_base,
-                                                const uint32_t* __restrict__ base_dev,
-                                                const uint32_t* __restrict__ qbegin,
-                                                const uint32_t* __restrict__ qcount, uint32_t nq,
-                                                const uint32_t* __restrict__ nq_dev, int side,
-                                                const QueryRec* __restrict__ qs, float* __restrict__ avg) {
+__global__ __launch_bounds__(64) void k_ray_avg(RaySides b, const QueryRec* __restrict__ qs) {
     const uint32_t qi = blockIdx.x;
     const int lane = threadIdx.x;
-    if (qi >= count_of(nq, nq_dev)) return;
-    if (base_dev) ray_base = *base_dev;
+    const int side = int(blockIdx.y);
+    if (qi >= count_of(b.nq, b.nq_dev)) return;
+    const cd_ray* __restrict__ rays = b.rays;
+    const uint64_t ray_base = side ? b.tot[DC_RAYS0] : 0u;
+    const uint32_t* __restrict__ qcount = b.qw + 2 * size_t(b.qstride) * side;
+    const uint32_t* __restrict__ qbegin = qcount + b.qstride;
+    float* __restrict__ avg = b.avg + 3 * size_t(b.qstride) * side;
     const uint32_t n = qcount[qi];
     f3 acc = mk(0.f, 0.f, 0.f);
     if (n) {
@@ -389,63 +423,29 @@ __global__ __launch_bounds__(64) void k_ray_avg(const cd_ray* __restrict__ rays,
 }
 
 // ---- launchers ---------------------------------------------------------------
-static inline dim3 gr(uint64_t n) { return dim3(uint32_t((std::max<uint64_t>(n, 1) + 255) / 256)); }
+static inline dim3 gr2(uint64_t n) { return dim3(uint32_t((std::max<uint64_t>(n, 1) + 255) / 256), 2); }
 
-hipError_t launch_cand(const cd_contact* c, const uint32_t* order, uint32_t n, int side, const QueryRec* qs,
-                       const MeshRec* meshes, const uint32_t* leafof, RayCand* out, uint32_t* start, hipStream_t st,
-                       const uint32_t* n_dev) {
-    hipLaunchKernelGGL(k_cand, gr(n), dim3(256), 0, st, c, order, n, n_dev, side, qs, meshes, leafof, out, start);
-    return hipGetLastError();
-}
-hipError_t launch_scan(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* sums, uint32_t* total, hipStream_t st,
-                       const uint32_t* n_dev) {
-    const uint32_t nb = std::max(1u, (n + SCAN_BLOCK - 1) / SCAN_BLOCK);
-    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, in, n, n_dev, sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, sums, nb, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, in, n, n_dev, sums, out);
-    return hipGetLastError();
-}
-hipError_t launch_compact(const RayCand* in, const uint32_t* start, const uint32_t* pos, uint32_t n, RayCand* out,
-                          hipStream_t st, const uint32_t* n_dev) {
-    hipLaunchKernelGGL(k_compact, gr(n), dim3(256), 0, st, in, start, pos, n, n_dev, out);
-    return hipGetLastError();
-}
-hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_u64, gr(n), dim3(256), 0, st, p, n, v);
-    return hipGetLastError();
-}
-hipError_t launch_vert_claim(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                             const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
-                             const uint32_t* perm, unsigned long long* hkey, unsigned long long* hval, uint64_t hmask,
-                             hipStream_t st) {
-    hipLaunchKernelGGL(k_vert_claim, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, attrs, vid, perm,
-                       hkey, hval, hmask);
-    return hipGetLastError();
-}
-hipError_t launch_ray_count(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                            const QueryRec* qs, const MeshRec* meshes, const uint2* attrs, const uint32_t* vid,
-                            const uint32_t* perm, const unsigned long long* hkey, const unsigned long long* hval,
-                            uint64_t hmask, uint32_t* cnt, uint32_t* qcount, hipStream_t st) {
-    hipLaunchKernelGGL(k_ray_count, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, attrs, vid, perm,
-                       hkey, hval, hmask, cnt, qcount);
-    return hipGetLastError();
-}
-hipError_t launch_ray_emit(const RayCand* cand, const uint32_t* ncand, uint32_t ncand_max, int side,
-                           const QueryRec* qs, const MeshRec* meshes, const float* tris, const uint2* attrs,
-                           const uint32_t* vid, const float* nrm, const uint32_t* perm,
-                           const unsigned long long* hkey, const unsigned long long* hval, uint64_t hmask,
-                           const uint32_t* pos, uint64_t ray_base, cd_ray* rays, uint32_t* qbegin, hipStream_t st,
-                           const uint32_t* base_dev) {
-    hipLaunchKernelGGL(k_ray_emit, gr(ncand_max), dim3(256), 0, st, cand, ncand, side, qs, meshes, tris, attrs, vid,
-                       nrm, perm, hkey, hval, hmask, pos, ray_base, base_dev, rays, qbegin);
-    return hipGetLastError();
-}
-hipError_t launch_ray_avg(const cd_ray* rays, uint64_t ray_base, const uint32_t* qbegin, const uint32_t* qcount,
-                          uint32_t nq, int side, const QueryRec* qs, float* avg, hipStream_t st,
-                          const uint32_t* base_dev, const uint32_t* nq_dev) {
-    if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_ray_avg, dim3(nq), dim3(64), 0, st, rays, ray_base, base_dev, qbegin, qcount, nq, nq_dev, side,
-                       qs, avg);
+hipError_t launch_rays(const RaySides& b, const QueryRec* qs, const MeshRec* meshes, const uint32_t* leafof,
+                       const float* tris, const uint2* attrs, const uint32_t* vid, const float* nrm,
+                       const uint32_t* perm, hipStream_t st) {
+    if (b.H & (b.H - 1)) return hipErrorInvalidValue;
+    const uint32_t nb = std::max(1u, (b.n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    if (b.nsums < nb) return hipErrorInvalidValue;
+    const dim3 g = gr2(b.n), gs(nb, 2);
+    hipLaunchKernelGGL(k_cand, g, dim3(256), 0, st, b, qs, meshes, leafof);
+    // candidates: dense list at the exclusive scan of the run starts
+    hipLaunchKernelGGL(k_scan_sums, gs, dim3(256), 0, st, b.start, b.n, b.n_dev, b.sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, 2), dim3(256), 0, st, b.sums, nb, b.tot + DC_CAND0);
+    hipLaunchKernelGGL(k_scan_apply, gs, dim3(256), 0, st, b.start, b.n, b.n_dev, b.sums, b.pos);
+    hipLaunchKernelGGL(k_compact, g, dim3(256), 0, st, b);
+    hipLaunchKernelGGL(k_vert_claim, g, dim3(256), 0, st, b, qs, meshes, attrs, vid, perm);
+    hipLaunchKernelGGL(k_ray_count, g, dim3(256), 0, st, b, qs, meshes, attrs, vid, perm);
+    // rays: each candidate's at the exclusive scan of the ray counts
+    hipLaunchKernelGGL(k_scan_sums, gs, dim3(256), 0, st, b.cnt, b.n, b.n_dev, b.sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, 2), dim3(256), 0, st, b.sums, nb, b.tot + DC_RAYS0);
+    hipLaunchKernelGGL(k_scan_apply, gs, dim3(256), 0, st, b.cnt, b.n, b.n_dev, b.sums, b.pos2);
+    hipLaunchKernelGGL(k_ray_emit, g, dim3(256), 0, st, b, qs, meshes, tris, attrs, vid, nrm, perm);
+    if (b.nq) hipLaunchKernelGGL(k_ray_avg, dim3(b.nq, 2), dim3(64), 0, st, b, qs);
     return hipGetLastError();
 }
 
