@@ -259,6 +259,16 @@ struct RayStack {
     }
 };
 
+// The lanes' published best hits (LDS, one word per lane): relaxed atomic
+// accesses, so that a lane reads its partner's latest value each time and the
+// accesses stay LDS operations.
+__device__ __forceinline__ void publish(float* sbest, int lane, float v) {
+    __hip_atomic_store(&sbest[lane], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ float partner(float* sbest, int lane) {
+    return __hip_atomic_load(&sbest[lane ^ 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // Closest hit of the ray (o, dir) on mesh m moved by M (Ray.cpp:135-236).
 // A stack that would overflow (the host sizes it from the tree depths, so
 // this is unreachable unless that sizing is wrong) sets bit 0 of *err and
@@ -274,13 +284,13 @@ struct RayStack {
 constexpr float BOUND_SLACK = 1.f + 0x1p-12f;
 __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
                         f3 o, f3 dir, RayStack& S, unsigned long long* err, uint32_t& nbox, uint32_t& ntri,
-                        volatile float* sbest, bool bounded, float eps) {
+                        float* sbest, bool bounded, float eps) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
     C.m[15] = C.m[15] - 0.f;
     Hit best = {false, false, __builtin_huge_valf(), 0.f, 0.f, 0xFFFFFFFFu};
-    sbest[S.lane] = __builtin_huge_valf();
+    publish(sbest, S.lane, __builtin_huge_valf());
     const LocalRay R = local_ray(C, dir);
     // the mesh-frame test, or the reference's own where that one is within rounding
     auto enter = [&](const float* bx, float& t0, float& t1) -> bool {
@@ -291,7 +301,7 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
     float r0, r1;
     ++nbox;
     auto finish = [&]() {  // the walk's end, as the partner lane reads it
-        sbest[S.lane] = (best.hit && best.back) ? best.dist : -1.f;
+        publish(sbest, S.lane, (best.hit && best.back) ? best.dist : -1.f);
         return best;
     };
     if (!enter(m.root.box, r0, r1)) return finish();
@@ -344,12 +354,12 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
             ntri += leaf_cnt;
             leaf_hits(tris, m, C, dir, leaf_link, leaf_cnt, best);
             leaf_cnt = 0;
-            sbest[S.lane] = best.dist;
+            publish(sbest, S.lane, best.dist);
         }
         // the next entry the best hit so far does not prune
         bool more = false;
         bool stop = false;
-        const float pb = bounded ? sbest[S.lane ^ 1] : __builtin_huge_valf();
+        const float pb = bounded ? partner(sbest, S.lane) : __builtin_huge_valf();
         if (pb < 0.f) stop = true;  // the partner ended without a back-face hit
         const float bound = (pb + eps) * BOUND_SLACK;
         while (sp > 0 && !stop) {
@@ -416,7 +426,7 @@ __device__ __forceinline__ Hit shfl_hit(const Hit& h, int src) {
 
 __device__ __forceinline__ Pass hermann(const MeshView& V, uint32_t meshA, const M4& Amat, uint32_t meshB, const M4& Bmat,
                         const M3& Bn, f3 o, f3 d, RayStack& S, unsigned long long* err, uint32_t& nbox,
-                        uint32_t& ntri, volatile float* sbest) {
+                        uint32_t& ntri, float* sbest) {
     Pass r = {false, mk(0.f, 0.f, 0.f), mk(0.f, 0.f, 0.f)};
     const bool own = (threadIdx.x & 1u) != 0;
     const f3 o2 = move_eps(o, d, 4.f);
