@@ -53,7 +53,7 @@ __global__ __launch_bounds__(1024) void k_step_contacts(const RunCounters* __res
 // The count and the scatter add to a pair's counter once per wavefront and
 // pair (a run's contacts of one pair lie together), not once per contact.
 constexpr uint32_t SEG_MAX = 2048;
-constexpr uint32_t SEG_THREADS = 256;
+constexpr uint32_t SEG_THREADS = 1024;  // one compare-exchange per thread and stage at SEG_MAX
 
 // Slot of each lane's contact among its pair's (base + rank among the
 // wavefront's lanes of that pair); valid lanes only.  One atomic add per
