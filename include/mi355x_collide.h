@@ -415,7 +415,14 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
  * writes the count back with the frame and its rays done: cd_shoot_rays
  * with a larger buffer finishes the step.  Afterwards the context is as
  * after the three calls (cd_uncollide_rays and cd_shoot_rays may be called
- * again on the same frame). */
+ * again on the same frame).  In the batched and reference modes every count
+ * stays on the device and the whole step is one launch of a captured HIP
+ * graph (cd_tuning.step_graph) with one wait: its buffers are sized for
+ * capacities of pairs, contacts and rays that grow when a step exceeds them
+ * (that step runs again), so a context's first step may run more than once.
+ * cd_get_counters then reports the run's counters (as cd_frame does) and
+ * cd_get_timings the step's device time (total_ms); the plain mode runs the
+ * three calls. */
 int cd_step(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, const cd_shoot_params* params,
             cd_pair_response* out, uint32_t cap, uint32_t* n_out);
 
