@@ -197,3 +197,27 @@ def test_step_pair_beyond_per_pair_order(step_graph):
             assert len(got) == 1 and got.tobytes() == want.tobytes(), mode
     finally:
         gpu.close()
+
+
+@pytest.mark.parametrize("mode", [MODE_REFERENCE, MODE_BATCHED])
+def test_step_crowded_scene_grows_its_capacities(mode):
+    """40 entities packed into a 4 m cube: far more pairs than the step's
+    first pair capacity (the entity count), and many contacts and rays.  The
+    step runs again with larger capacities and gives the three calls'
+    records, bit for bit; the next frame replays the grown step."""
+    sc = scenes.Scene(n_entities=40, seed=5, cube=4.0, torus_grid=(48, 12))
+    case(f"cd_step, crowded scene, mode {mode}")
+    gpu = Collider(leaf_max=4)
+    try:
+        ids = gpu.register_soups(sc.soups)
+        for frame in range(2):
+            ents = sc.entities(ids, ENTITY_DTYPE)
+            pairs, _ = gpu.frame(ents, mode)
+            gpu.uncollide_rays()
+            want = gpu.shoot_rays()
+            got = gpu.step(ents, mode)
+            assert got.tobytes() == want.tobytes(), frame
+            assert gpu.counters()["broad_pairs"] > 64, frame
+            sc.step()
+    finally:
+        gpu.close()
