@@ -290,7 +290,7 @@ def config2(args):
     keys = {"plain": [], "reference": [], "batched": []}
     trav_bytes = trav_ms = 0.0
     for name, mode in (("plain", MODE_PLAIN), ("reference", MODE_REFERENCE), ("batched", MODE_BATCHED)):
-        gpu.set_tuning(**dict(tuning, use_graph=0))
+        gpu.set_tuning(**dict(tuning, use_graph=0, step_graph=2))  # launched one by one: kernel events
         for f in frames:
             p, c = gpu.frame(f, mode)
             k = set()

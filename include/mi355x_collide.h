@@ -185,8 +185,9 @@ typedef struct cd_tuning {
                                 it issues the next batch's loads before it tests the current one (work_sharing 0) */
     uint32_t split_depth;    /* split levels one launch runs, 1 or 2 (0: 2): with 2 a thread tests its pair and
                                 then both of that pair's children */
-    uint32_t step_graph;     /* cd_step: 0 or 1 replays the whole step as one captured HIP graph; 2 enqueues the
-                                same kernels one by one (no graph) */
+    uint32_t step_graph;     /* cd_step and cd_frame (batched and reference modes): 0 or 1 replays the whole
+                                step (the frame) as one captured HIP graph; 2 enqueues the same kernels one by
+                                one, with events around the traversal and the narrow phase (cd_get_timings) */
 } cd_tuning;
 
 /* ---- context --------------------------------------------------------------- */
@@ -360,7 +361,12 @@ int cd_query_leafpairs(cd_ctx* ctx, const cd_query* q, uint32_t nq, uint32_t* ou
 /* Phases 1-3 of CollisionDetection::ExecuteCollisionDetection
  * (src/CollisionDetection/CollisionDetection.cpp:38-67): sweep-and-prune on
  * the fixed U/V/W frame, relative matrix per pair, traversal, triangle tests.
- * Pairs without contacts are dropped, as :52 and :63 drop them.            */
+ * Pairs without contacts are dropped, as :52 and :63 drop them.  In the
+ * batched and reference modes the frame is one launch of a captured HIP
+ * graph (cd_step's first part, with its capacities) and one wait, then the
+ * contacts are copied back; cd_get_timings reports its device time
+ * (total_ms), and with cd_tuning.step_graph 2 the traversal (traverse_ms)
+ * and the narrow phase (narrow_ms) too.                                    */
 int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs,
              uint32_t pair_cap, uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap,
              uint64_t* n_contacts);

@@ -765,6 +765,7 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_step.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
+    if (ctx->frame_graph) (void)hipGraphExecDestroy(ctx->frame_graph);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1308,6 +1309,29 @@ int cd_fetch_contacts(cd_ctx* ctx, cd_contact* out, uint64_t cap, uint64_t* n_ou
 }  // extern "C"
 
 namespace {
+// cd_frame's results from the nc sorted contacts hc (in host memory) and the
+// frame's pairs (ctx->staged_ids): the pairs that produced contacts
+// (CollisionDetection.cpp:52, :63 keep only those) with their ranges.
+int frame_output(cd_ctx* ctx, const cd_contact* hc, uint64_t nc, cd_pair* pairs, uint32_t pair_cap,
+                 uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+    const std::vector<uint2>& hp = ctx->staged_ids;
+    std::vector<cd_pair> out_pairs;
+    for (uint64_t i = 0; i < nc;) {
+        uint64_t j = i;
+        while (j < nc && hc[j].query == hc[i].query) ++j;
+        const uint2 pr = hp[hc[i].query];
+        out_pairs.push_back(cd_pair{pr.x, pr.y, uint32_t(i), uint32_t(j - i)});
+        i = j;
+    }
+    *n_pairs = uint32_t(out_pairs.size());
+    *n_contacts = nc;
+    if (out_pairs.size() > pair_cap || nc > contact_cap || (nc && (!pairs || !contacts)))
+        return fail(ctx, CD_E_CAPACITY, "pair or contact buffer too small");
+    if (!out_pairs.empty()) std::memcpy(pairs, out_pairs.data(), out_pairs.size() * sizeof(cd_pair));
+    if (nc) std::memcpy(contacts, hc, nc * sizeof(cd_contact));
+    return CD_OK;
+}
+
 // cd_frame; to_host false (cd_step): the sorted contacts stay on the device,
 // pairs and contacts are not returned.
 int frame_impl(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
@@ -1394,22 +1418,7 @@ int frame_impl(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode
     ctx->staged_ids = hp;
     *n_contacts = nc;
     if (!to_host) return CD_OK;
-    // pairs that produced contacts (CollisionDetection.cpp:52, :63 keep only those)
-    std::vector<cd_pair> out_pairs;
-    for (uint64_t i = 0; i < nc;) {
-        uint64_t j = i;
-        while (j < nc && hc[j].query == hc[i].query) ++j;
-        const uint2 pr = hp[hc[i].query];
-        out_pairs.push_back(cd_pair{pr.x, pr.y, uint32_t(i), uint32_t(j - i)});
-        i = j;
-    }
-    *n_pairs = uint32_t(out_pairs.size());
-    *n_contacts = nc;
-    if (out_pairs.size() > pair_cap || nc > contact_cap || (nc && (!pairs || !contacts)))
-        return fail(ctx, CD_E_CAPACITY, "pair or contact buffer too small");
-    if (!out_pairs.empty()) std::memcpy(pairs, out_pairs.data(), out_pairs.size() * sizeof(cd_pair));
-    if (nc) std::memcpy(contacts, hc, nc * sizeof(cd_contact));
-    return CD_OK;
+    return frame_output(ctx, hc, nc, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts);
 }
 
 // ---- rays (CreateUncollideRays.cpp:60-200) -----------------------------------
@@ -1589,13 +1598,13 @@ void shoot_params(const cd_shoot_params* params, float* cos_start, float* cos_fi
     *cos_finish = std::cos(a1);
 }
 
-// ---- the one-call scene step on the device ------------------------------------
-// cd_step with every count kept on the device: the broad phase, the batched
-// traversal and narrow phase, the contact order, the rays, the shots and the
-// responses are enqueued at once (and replayed as one graph), shaped for
-// capacities of pairs, contacts and shot rays.  The host reads one block of
-// results after the step and runs it again, with larger capacities, when a
-// count exceeded one.
+// ---- the frame and the scene step on the device -----------------------------
+// cd_step (and cd_frame) with every count kept on the device: the broad
+// phase, the batched traversal and narrow phase, the contact order, and for
+// the step the rays, the shots and the responses are enqueued at once (and
+// replayed as one graph), shaped for capacities of pairs, contacts and shot
+// rays.  The host reads one block of results after it and runs it again,
+// with larger capacities, when a count exceeded one.
 uint32_t pow2_at_least(uint64_t v) {
     uint64_t p = 64;
     while (p < v) p <<= 1;
@@ -1604,9 +1613,11 @@ uint32_t pow2_at_least(uint64_t v) {
 
 struct StepArgs {
     uint32_t n;                // entities
+    bool shoot;                // the whole step; else the frame (broad, narrow, contact order)
     uint32_t np, nc, rays;     // capacities: pairs, contacts, rays shot
     cd_tuning t;
     bool ref;
+    bool timed;                // launched one by one: events around the traversal and the narrow phase
     bool radix;                // the contact order by the radix sorts (a pair beyond the per-pair order)
     uint32_t stack_entries;    // the ray traversal's stack: the entities' deepest tree + 2
     float cos_start, cos_finish, bias;
@@ -1614,19 +1625,21 @@ struct StepArgs {
 
 // Word offsets of the result block (W_STEPOUT on the device, h_step on the
 // host): the run's counters, the shots' counters, the count block, per pair
-// the rays of each side, the pairs, the responses.
+// the rays of each side, the pairs, the responses (a frame's block: the
+// counters, the count block and the pairs).
 struct StepLayout {
     uint32_t fctr, sctr, dc, qc0, qc1, pairs, resp, words;
-    explicit StepLayout(uint32_t np) {
+    StepLayout(uint32_t np, bool shoot) {
         const uint32_t rc = uint32_t(sizeof(RunCounters) / 4);
+        const uint32_t q = shoot ? np : 0u;
         fctr = 0;
         sctr = rc;
         dc = 2 * rc;
         qc0 = dc + DC_WORDS;
-        qc1 = qc0 + np;
-        pairs = qc1 + np;
+        qc1 = qc0 + q;
+        pairs = qc1 + q;
         resp = pairs + 2 * np;
-        words = resp + np * uint32_t(sizeof(cd_pair_response) / 4);
+        words = resp + q * uint32_t(sizeof(cd_pair_response) / 4);
     }
 };
 static_assert(sizeof(RunCounters) % 4 == 0 && sizeof(cd_pair_response) % 4 == 0, "results copied as words");
@@ -1643,29 +1656,37 @@ int ensure_step(cd_ctx* ctx, const StepArgs& a, uint64_t* fcap) {
     if (ctx->contact_cap < C) ctx->contact_cap = C;
     int rc = ensure_batched(ctx, P, P, a.t, fcap);
     if (rc) return rc;
-    rc = prepare_rays(ctx, C, P);
-    if (rc) return rc;
+    if (a.shoot) {
+        rc = prepare_rays(ctx, C, P);
+        if (rc) return rc;
+    } else {  // the contact order's buffers (the radix sorts' too)
+        HCK(ctx, ctx->d_work[W_CSORT].ensure(size_t(C) * sizeof(cd_contact)));
+        for (int w : {W_K0, W_V0, W_K1, W_V1}) HCK(ctx, ctx->d_work[w].ensure((size_t(C) + 1) * sizeof(uint32_t)));
+    }
     uint32_t* table = nullptr;
     HCK(ctx, sort_table(ctx, std::max(std::max(n, P), C), &table));
-    HCK(ctx, ctx->d_work[W_PIDS].ensure(2 * size_t(P) * sizeof(uint32_t)));
-    HCK(ctx, ctx->d_work[W_SHOTS].ensure(size_t(R) * sizeof(RayShot)));
-    HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(P) * sizeof(cd_pair_response)));
     HCK(ctx, ctx->d_work[W_SCTR].ensure(sizeof(RunCounters)));
     HCK(ctx, ctx->d_work[W_STEPC].ensure(DC_WORDS * sizeof(uint32_t)));
     HCK(ctx, ctx->d_work[W_SEGQ].ensure(3 * size_t(P) * sizeof(uint32_t)));
     HCK(ctx, ctx->d_work[W_CSEG].ensure(size_t(C) * sizeof(cd_contact)));
-    if (a.stack_entries > uint32_t(RAY_STACK_ENTRIES)) {  // trees deeper than the LDS stack holds
-        const size_t blocks = (2 * size_t(R) + 63) / 64;
-        HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * a.stack_entries * 64 * sizeof(uint32_t)));
+    if (a.shoot) {
+        HCK(ctx, ctx->d_work[W_PIDS].ensure(2 * size_t(P) * sizeof(uint32_t)));
+        HCK(ctx, ctx->d_work[W_SHOTS].ensure(size_t(R) * sizeof(RayShot)));
+        HCK(ctx, ctx->d_work[W_RESP].ensure(size_t(P) * sizeof(cd_pair_response)));
+        if (a.stack_entries > uint32_t(RAY_STACK_ENTRIES)) {  // trees deeper than the LDS stack holds
+            const size_t blocks = (2 * size_t(R) + 63) / 64;
+            HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * a.stack_entries * 64 * sizeof(uint32_t)));
+        }
     }
-    const StepLayout L(P);
+    const StepLayout L(P, a.shoot);
     HCK(ctx, ctx->d_work[W_STEPOUT].ensure(size_t(L.words) * sizeof(uint32_t)));
     HCK(ctx, ctx->h_step.ensure(size_t(L.words) * sizeof(uint32_t)));
     return CD_OK;
 }
 
-// The whole step on the stream, no host wait inside (the entities already in
-// W_ENTS); its last command copies the result block to h_step.
+// The whole step (or the frame) on the stream, no host wait inside (the
+// entities already in W_ENTS); its last command copies the result block to
+// h_step.
 int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
     hipStream_t st = ctx->stream;
     const uint32_t n = a.n, P = a.np, C = a.nc, R = a.rays;
@@ -1692,7 +1713,7 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
     HCK(ctx, launch_pair_queries(pairs, v0, P, de, dq, spairs, st, cnt));
     HCK(ctx, launch_step_pairs(cnt, P, dc, ctx->d_nq.as<unsigned int>(), st));
     // the pairs' contacts, ordered by (pair, tri_a, tri_b) and by (pair, tri_b, tri_a)
-    int rc = enqueue_batched(ctx, P, a.t, false, a.ref);
+    int rc = enqueue_batched(ctx, P, a.t, a.timed, a.ref);
     if (rc) return rc;
     const uint64_t ccap = std::min<uint64_t>(C, ctx->contact_cap);
     if (a.radix) {
@@ -1706,6 +1727,23 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
         HCK(ctx, launch_step_contacts(dev_counters(ctx), ccap, dc, st, qw3, P));
         HCK(ctx, launch_seg_order(ctx->d_contacts.as<cd_contact>(), C, P, dc, qw3, ctx->d_work[W_CSEG].as<cd_contact>(),
                                   ctx->d_work[W_CSORT].as<cd_contact>(), v0, st));
+    }
+    const StepLayout L(P, a.shoot);
+    PackSpans sp{};
+    auto span = [&](const void* src, uint32_t words, uint32_t at) {
+        sp.src[sp.n] = static_cast<const uint32_t*>(src);
+        sp.words[sp.n] = words;
+        sp.dst[sp.n] = at;
+        ++sp.n;
+    };
+    if (!a.shoot) {  // the frame: its counters, counts and pairs
+        span(ctx->d_counters.p, L.sctr - L.fctr, L.fctr);
+        span(dc, DC_WORDS, L.dc);
+        span(spairs, 2 * P, L.pairs);
+        HCK(ctx, launch_pack(sp, ctx->d_work[W_STEPOUT].as<uint32_t>(), st));
+        HCK(ctx, hipMemcpyAsync(ctx->h_step.p, ctx->d_work[W_STEPOUT].p, size_t(L.words) * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, st));
+        return CD_OK;
     }
     // rays, shots, responses
     rc = enqueue_rays(ctx, C, P, P, &dc[DC_NC], &dc[DC_NQ], dc, !a.radix);
@@ -1724,14 +1762,6 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
                             a.cos_start, a.cos_finish, a.bias, ctx->d_work[W_RESP].as<cd_pair_response>(), st, P, dc,
                             R));
     // the results in one block, one copy to the host
-    const StepLayout L(P);
-    PackSpans sp{};
-    auto span = [&](const void* src, uint32_t words, uint32_t at) {
-        sp.src[sp.n] = static_cast<const uint32_t*>(src);
-        sp.words[sp.n] = words;
-        sp.dst[sp.n] = at;
-        ++sp.n;
-    };
     span(ctx->d_counters.p, L.sctr - L.fctr, L.fctr);
     span(sctr, L.dc - L.sctr, L.sctr);
     span(dc, DC_WORDS, L.dc);
@@ -1752,7 +1782,7 @@ void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<u
     std::memcpy(&f[1], &a.cos_finish, 4);
     std::memcpy(&f[2], &a.bias, 4);
     const uint64_t k[] = {a.n, a.np, a.nc, a.rays, a.stack_entries, f[0], f[1], f[2], ctx->meshes.size(),
-                          uint64_t(a.radix)};
+                          uint64_t(a.radix), uint64_t(a.shoot)};
     key.assign(std::begin(k), std::end(k));
     batched_key(ctx, a.np, a.t, a.ref, fcap, key);
     for (const DevBuf& b : ctx->d_work) key.push_back(reinterpret_cast<uint64_t>(b.p));
@@ -1761,18 +1791,26 @@ void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<u
     key.push_back(reinterpret_cast<uint64_t>(ctx->h_step.p));
 }
 
-int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const cd_shoot_params* params,
-                uint32_t stack_entries, cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
+// The frame (shoot false) or the whole step on the device, run until its
+// counts fit the capacities.  On success the result block is in h_step (its
+// layout *L) and the context holds what the separate calls leave: the pairs
+// and sorted contacts of the frame, and after a step its rays.
+int device_run(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, bool shoot, const cd_shoot_params* params,
+               uint32_t stack_entries, StepLayout* L) {
     hipStream_t st = ctx->stream;
     StepArgs a{};
     a.n = n;
+    a.shoot = shoot;
     a.t = effective_tuning(ctx);
     a.ref = ref;
     a.stack_entries = stack_entries;
     shoot_params(params, &a.cos_start, &a.cos_finish, &a.bias);
     const bool graph = ctx->tuning.step_graph != 2;
-    // first capacities; a step that outgrows one runs again with it larger,
-    // and later steps start from there
+    a.timed = !graph;
+    hipGraphExec_t* exec = shoot ? &ctx->step_graph : &ctx->frame_graph;
+    std::vector<uint64_t>& exec_key = shoot ? ctx->step_key : ctx->frame_key;
+    // first capacities; a run that outgrows one runs again with it larger,
+    // and later runs start from there
     if (!ctx->step_np_cap) ctx->step_np_cap = pow2_at_least(n);
     if (!ctx->step_nc_cap) ctx->step_nc_cap = 4096;
     if (!ctx->step_ray_cap) ctx->step_ray_cap = 4096;
@@ -1794,25 +1832,25 @@ int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const 
         if (graph) {
             std::vector<uint64_t> key;
             step_key(ctx, a, fcap, key);
-            if (!(ctx->step_graph && key == ctx->step_key)) {
-                ctx->step_key.clear();
-                rc = capture_graph(ctx, &ctx->step_graph, [&] { return enqueue_step(ctx, a); });
+            if (!(*exec && key == exec_key)) {
+                exec_key.clear();
+                rc = capture_graph(ctx, exec, [&] { return enqueue_step(ctx, a); });
                 if (rc) return rc;
-                ctx->step_key = key;
+                exec_key = key;
             }
-            HCK(ctx, hipGraphLaunch(ctx->step_graph, st));
+            HCK(ctx, hipGraphLaunch(*exec, st));
         } else {
             rc = enqueue_step(ctx, a);
             if (rc) return rc;
         }
         HCK(ctx, hipEventRecord(ctx->ev[5], st));
         HCK(ctx, hipEventSynchronize(ctx->ev[5]));
-        const StepLayout L(a.np);
+        *L = StepLayout(a.np, shoot);
         const uint32_t* w = static_cast<const uint32_t*>(ctx->h_step.p);
         RunCounters fh{}, sh{};
-        std::memcpy(&fh, w + L.fctr, sizeof fh);
-        std::memcpy(&sh, w + L.sctr, sizeof sh);
-        const uint32_t* dcw = w + L.dc;
+        std::memcpy(&fh, w + L->fctr, sizeof fh);
+        if (shoot) std::memcpy(&sh, w + L->sctr, sizeof sh);
+        const uint32_t* dcw = w + L->dc;
         bool retry = false;
         rc = check_batched(ctx, fh, fcap, a.t, &retry);
         if (rc) return rc;
@@ -1828,47 +1866,87 @@ int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const 
             ctx->step_radix = true;
             retry = true;
         }
-        const uint64_t nrays = uint64_t(dcw[DC_RAYS0]) + dcw[DC_RAYS1];
+        const uint64_t nrays = shoot ? uint64_t(dcw[DC_RAYS0]) + dcw[DC_RAYS1] : 0;
         if (nrays > a.rays) {
             ctx->step_ray_cap = pow2_at_least(nrays + nrays / 4);
             retry = true;
         }
         if (retry) continue;
         if (sh.err) return fail(ctx, CD_E_STACK, "ray traversal stack overflow");
-        // what the separate calls would leave: the pairs, sorted contacts and rays of the step
+        // what the separate calls would leave: the pairs, sorted contacts (and rays) of the frame
         const uint32_t nq = dcw[DC_NQ];
-        const uint2* hp = reinterpret_cast<const uint2*>(w + L.pairs);
+        const uint2* hp = reinterpret_cast<const uint2*>(w + L->pairs);
         ctx->staged_nq = nq;
         ctx->staged_ids.assign(hp, hp + nq);
         ctx->staged_contacts = fh.contacts;
         ctx->contacts_sorted = true;
         ctx->last_run = 2;
-        ctx->ray_base1 = dcw[DC_RAYS0];
-        ctx->ray_total = nrays;
-        ctx->ray_stride = a.np;
-        ctx->ray_counts.resize(2 * size_t(nq));
-        uint32_t npairs = 0;
-        for (uint32_t q = 0; q < nq; ++q) {
-            ctx->ray_counts[q] = w[L.qc0 + q];
-            ctx->ray_counts[size_t(nq) + q] = w[L.qc1 + q];
-            npairs += (w[L.qc0 + q] + w[L.qc1 + q]) ? 1u : 0u;
+        if (shoot) {
+            ctx->ray_base1 = dcw[DC_RAYS0];
+            ctx->ray_total = nrays;
+            ctx->ray_stride = a.np;
+            ctx->ray_counts.resize(2 * size_t(nq));
+            for (uint32_t q = 0; q < nq; ++q) {
+                ctx->ray_counts[q] = w[L->qc0 + q];
+                ctx->ray_counts[size_t(nq) + q] = w[L->qc1 + q];
+            }
+            ctx->rays_ready = true;
         }
-        ctx->rays_ready = true;
         set_run_counters(ctx, fh);
         ctx->counters.broad_pairs = nq;
         ctx->timings = cd_timings{};
         HCK(ctx, hipEventElapsedTime(&ctx->timings.total_ms, ctx->ev[4], ctx->ev[5]));
+        if (a.timed) {  // the batched run's kernels, as run_batched reports them
+            float t_split = 0.f, t_dfs = 0.f, t_narrow = 0.f;
+            HCK(ctx, hipEventElapsedTime(&t_split, ctx->ev[0], ctx->ev[1]));
+            HCK(ctx, hipEventElapsedTime(&t_dfs, ctx->ev[1], ctx->ev[2]));
+            HCK(ctx, hipEventElapsedTime(&t_narrow, ctx->ev[2], ctx->ev[3]));
+            ctx->timings.traverse_ms = t_split + t_dfs;
+            ctx->timings.narrow_ms = t_narrow;
+        }
         ctx->timings.launches = 1;
-        *n_out = npairs;
-        if (npairs == 0) return CD_OK;
-        if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
-        const cd_pair_response* resp = reinterpret_cast<const cd_pair_response*>(w + L.resp);
-        uint32_t k = 0;
-        for (uint32_t q = 0; q < nq; ++q)
-            if (w[L.qc0 + q] + w[L.qc1 + q]) out[k++] = resp[q];
         return CD_OK;
     }
     return fail(ctx, CD_E_NOMEM, "step sizing did not converge");
+}
+
+int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const cd_shoot_params* params,
+                uint32_t stack_entries, cd_pair_response* out, uint32_t cap, uint32_t* n_out) {
+    StepLayout L(0, true);
+    int rc = device_run(ctx, ents, n, ref, true, params, stack_entries, &L);
+    if (rc) return rc;
+    const uint32_t* w = static_cast<const uint32_t*>(ctx->h_step.p);
+    const uint32_t nq = ctx->staged_nq;
+    uint32_t npairs = 0;
+    for (uint32_t q = 0; q < nq; ++q) npairs += (w[L.qc0 + q] + w[L.qc1 + q]) ? 1u : 0u;
+    *n_out = npairs;
+    if (npairs == 0) return CD_OK;
+    if (npairs > cap || !out) return fail(ctx, CD_E_CAPACITY, "response buffer too small");
+    const cd_pair_response* resp = reinterpret_cast<const cd_pair_response*>(w + L.resp);
+    uint32_t k = 0;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (w[L.qc0 + q] + w[L.qc1 + q]) out[k++] = resp[q];
+    return CD_OK;
+}
+
+// cd_frame on the device: one run, then the sorted contacts copied back and
+// grouped by pair on the host.
+int frame_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, cd_pair* pairs, uint32_t pair_cap,
+                 uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+    StepLayout L(0, false);
+    int rc = device_run(ctx, ents, n, ref, false, nullptr, 2, &L);
+    if (rc) return rc;
+    const uint64_t nc = ctx->staged_contacts;
+    const cd_contact* hc = nullptr;
+    if (nc) {
+        hipStream_t st = ctx->stream;
+        HCK(ctx, ctx->h_stage.ensure(nc * sizeof(cd_contact)));
+        HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost,
+                                st));
+        HCK(ctx, hipStreamSynchronize(st));
+        hc = static_cast<const cd_contact*>(ctx->h_stage.p);
+    }
+    return frame_output(ctx, hc, nc, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts);
 }
 }  // namespace
 
@@ -1876,6 +1954,15 @@ extern "C" {
 
 int cd_frame(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode, cd_pair* pairs, uint32_t pair_cap,
              uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
+    if (ctx && n_pairs && n_contacts && (mode == CD_MODE_BATCHED || mode == CD_MODE_REFERENCE) && n >= 2) {
+        (void)hipSetDevice(ctx->device);
+        *n_pairs = 0;
+        *n_contacts = 0;
+        int rc = check_entities(ctx, entities, n);
+        if (rc) return rc;
+        return frame_device(ctx, entities, n, mode == CD_MODE_REFERENCE, pairs, pair_cap, n_pairs, contacts,
+                            contact_cap, n_contacts);
+    }
     return frame_impl(ctx, entities, n, mode, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts, true);
 }
 

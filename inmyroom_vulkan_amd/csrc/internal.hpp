@@ -300,12 +300,13 @@ struct cd_ctx {
     std::string scratch_err;               // why it could not be resolved
     hipGraphExec_t graph = nullptr;        // captured batched pipeline
     std::vector<uint64_t> graph_key;       // sizes, tuning and every device pointer the graph captured
-    // the one-call scene step: capacities (grown when a step exceeds one),
-    // its captured graph and key, and the pinned block its results land in
+    // the frame and the scene step on the device: capacities (grown when a
+    // run exceeds one), their captured graphs and keys, and the pinned block
+    // their results land in
     uint32_t step_np_cap = 0, step_nc_cap = 0, step_ray_cap = 0;
     bool step_radix = false;  // a pair had more contacts than the per-pair order ranks: the radix sorts
-    hipGraphExec_t step_graph = nullptr;
-    std::vector<uint64_t> step_key;
+    hipGraphExec_t step_graph = nullptr, frame_graph = nullptr;  // the whole step; cd_frame's part of it
+    std::vector<uint64_t> step_key, frame_key;
     cdk::HostBuf h_step;
     hipEvent_t ev[8] = {};
 };
