@@ -763,6 +763,7 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_stage.release();
     ctx->h_pairs.release();
     ctx->h_step.release();
+    ctx->h_ents.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
     if (ctx->frame_graph) (void)hipGraphExecDestroy(ctx->frame_graph);
@@ -1681,6 +1682,8 @@ int ensure_step(cd_ctx* ctx, const StepArgs& a, uint64_t* fcap) {
     const StepLayout L(P, a.shoot);
     HCK(ctx, ctx->d_work[W_STEPOUT].ensure(size_t(L.words) * sizeof(uint32_t)));
     HCK(ctx, ctx->h_step.ensure(size_t(L.words) * sizeof(uint32_t)));
+    HCK(ctx, ctx->h_ents.ensure(size_t(n) * sizeof(cd_entity)));
+    if (!a.shoot) HCK(ctx, ctx->h_stage.ensure(size_t(C) * sizeof(cd_contact)));
     return CD_OK;
 }
 
@@ -1702,6 +1705,9 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
     uint2* spairs = ctx->d_work[W_SPAIRS].as<uint2>();
     QueryRec* dq = ctx->d_queries.as<QueryRec>();
     const MeshRec* dm = ctx->d_meshes.as<MeshRec>();
+    // the entity records, from pinned staging the host fills before each launch
+    HCK(ctx, hipMemcpyAsync(ctx->d_work[W_ENTS].p, ctx->h_ents.p, size_t(n) * sizeof(cd_entity), hipMemcpyHostToDevice,
+                            st));
     HCK(ctx, launch_fill_u32(dc, DC_WORDS, 0u, st));
     // broad phase: sweep and prune, the pairs in (first, second) order, their relative matrices
     HCK(ctx, launch_sap_project(de, n, dm, sap_axes(), ctx->d_work[W_IV].as<float>(), k0, v0, st));
@@ -1736,12 +1742,14 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
         sp.dst[sp.n] = at;
         ++sp.n;
     };
-    if (!a.shoot) {  // the frame: its counters, counts and pairs
+    if (!a.shoot) {  // the frame: its counters, counts and pairs, and the sorted contacts (the capacity)
         span(ctx->d_counters.p, L.sctr - L.fctr, L.fctr);
         span(dc, DC_WORDS, L.dc);
         span(spairs, 2 * P, L.pairs);
         HCK(ctx, launch_pack(sp, ctx->d_work[W_STEPOUT].as<uint32_t>(), st));
         HCK(ctx, hipMemcpyAsync(ctx->h_step.p, ctx->d_work[W_STEPOUT].p, size_t(L.words) * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, st));
+        HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, size_t(C) * sizeof(cd_contact),
                                 hipMemcpyDeviceToHost, st));
         return CD_OK;
     }
@@ -1788,7 +1796,7 @@ void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<u
     for (const DevBuf& b : ctx->d_work) key.push_back(reinterpret_cast<uint64_t>(b.p));
     for (const DevBuf* b : {&ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm, &ctx->d_perm})
         key.push_back(reinterpret_cast<uint64_t>(b->p));
-    key.push_back(reinterpret_cast<uint64_t>(ctx->h_step.p));
+    for (const HostBuf* b : {&ctx->h_step, &ctx->h_ents, &ctx->h_stage}) key.push_back(reinterpret_cast<uint64_t>(b->p));
 }
 
 // The frame (shoot false) or the whole step on the device, run until its
@@ -1827,7 +1835,7 @@ int device_run(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, bool sh
         uint64_t fcap = 0;
         int rc = ensure_step(ctx, a, &fcap);
         if (rc) return rc;
-        HCK(ctx, hipMemcpyAsync(ctx->d_work[W_ENTS].p, ents, size_t(n) * sizeof(cd_entity), hipMemcpyHostToDevice, st));
+        std::memcpy(ctx->h_ents.p, ents, size_t(n) * sizeof(cd_entity));
         HCK(ctx, hipEventRecord(ctx->ev[4], st));
         if (graph) {
             std::vector<uint64_t> key;
@@ -1929,23 +1937,16 @@ int step_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, const 
     return CD_OK;
 }
 
-// cd_frame on the device: one run, then the sorted contacts copied back and
-// grouped by pair on the host.
+// cd_frame on the device: one run, whose last copy brings the sorted contacts
+// back too, then grouped by pair on the host.
 int frame_device(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, cd_pair* pairs, uint32_t pair_cap,
                  uint32_t* n_pairs, cd_contact* contacts, uint64_t contact_cap, uint64_t* n_contacts) {
     StepLayout L(0, false);
     int rc = device_run(ctx, ents, n, ref, false, nullptr, 2, &L);
     if (rc) return rc;
+    // the run copied its sorted contacts (up to the capacity, which they fit) to h_stage
     const uint64_t nc = ctx->staged_contacts;
-    const cd_contact* hc = nullptr;
-    if (nc) {
-        hipStream_t st = ctx->stream;
-        HCK(ctx, ctx->h_stage.ensure(nc * sizeof(cd_contact)));
-        HCK(ctx, hipMemcpyAsync(ctx->h_stage.p, ctx->d_work[W_CSORT].p, nc * sizeof(cd_contact), hipMemcpyDeviceToHost,
-                                st));
-        HCK(ctx, hipStreamSynchronize(st));
-        hc = static_cast<const cd_contact*>(ctx->h_stage.p);
-    }
+    const cd_contact* hc = static_cast<const cd_contact*>(ctx->h_stage.p);
     return frame_output(ctx, hc, nc, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts);
 }
 }  // namespace

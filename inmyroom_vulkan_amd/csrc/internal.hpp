@@ -307,6 +307,6 @@ struct cd_ctx {
     bool step_radix = false;  // a pair had more contacts than the per-pair order ranks: the radix sorts
     hipGraphExec_t step_graph = nullptr, frame_graph = nullptr;  // the whole step; cd_frame's part of it
     std::vector<uint64_t> step_key, frame_key;
-    cdk::HostBuf h_step;
+    cdk::HostBuf h_step, h_ents;  // the run's result block; its entity records, staged for the graph's upload
     hipEvent_t ev[8] = {};
 };
