@@ -383,7 +383,7 @@ class Collider:
         queries = np.ascontiguousarray(queries, QUERY_DTYPE)
         cap = cap if cap is not None else 1 << 16
         while True:
-            out = np.zeros(cap, CONTACT_DTYPE)
+            out = np.empty(cap, CONTACT_DTYPE)  # the call fills the prefix that is returned
             n = C.c_uint64()
             rc = self.lib.cd_query_batch(self.h, _ptr(queries), len(queries), mode, _ptr(out), cap, C.byref(n))
             if rc == CD_E_CAPACITY:
@@ -427,8 +427,8 @@ class Collider:
         ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
         pcap, ccap = self._caps.get("frame", (1 << 12, 1 << 16))
         while True:
-            pairs = np.zeros(pcap, PAIR_DTYPE)
-            cons = np.zeros(ccap, CONTACT_DTYPE)
+            pairs = np.empty(pcap, PAIR_DTYPE)  # the call fills the prefix that is returned
+            cons = np.empty(ccap, CONTACT_DTYPE)
             npairs, ncons = C.c_uint32(), C.c_uint64()
             rc = self.lib.cd_frame(self.h, _ptr(ents), len(ents), mode, _ptr(pairs), pcap, C.byref(npairs),
                                    _ptr(cons), ccap, C.byref(ncons))
@@ -444,8 +444,8 @@ class Collider:
         """Rays of the last run's contacts: (pairs [PAIR_RAYS_DTYPE], rays [RAY_DTYPE])."""
         pcap, rcap = self._caps.get("rays", (1 << 10, 1 << 14))
         while True:
-            pairs = np.zeros(pcap, PAIR_RAYS_DTYPE)
-            rays = np.zeros(rcap, RAY_DTYPE)
+            pairs = np.empty(pcap, PAIR_RAYS_DTYPE)  # the call fills the prefix that is returned
+            rays = np.empty(rcap, RAY_DTYPE)
             npairs, nrays = C.c_uint32(), C.c_uint64()
             rc = self.lib.cd_uncollide_rays(self.h, _ptr(pairs), pcap, C.byref(npairs), _ptr(rays), rcap,
                                             C.byref(nrays))
