@@ -180,17 +180,20 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
     return fail(ctx, CD_E_NOMEM, "output sizing did not converge");
 }
 
-cd_tuning effective_tuning(const cd_ctx* ctx) {
-    // defaults: the values the tuning sweep picks for the 4,096-query batch
-    // with the pipelined traversal (16 levels, pairs over 256 triangles split
-    // further, 512-entry stacks, 6 wavefronts per CU)
+// nq_cap: the query capacity of the run (0: unknown, the large-batch defaults).
+cd_tuning effective_tuning(const cd_ctx* ctx, uint32_t nq_cap = 0) {
+    // defaults: the values the tuning sweeps pick with the pipelined traversal:
+    // for the 4,096-query batch 16 levels, pairs over 256 triangles split
+    // further, 512-entry stacks, 6 wavefronts per CU; for runs of up to 2,048
+    // queries (a 512-entity scene's ~1,100 pairs) 12 levels and 4 wavefronts
     cd_tuning t = ctx->tuning;
-    if (t.split_levels == 0) t.split_levels = 16;
+    const bool small = nq_cap != 0 && nq_cap <= 2048;
+    if (t.split_levels == 0) t.split_levels = small ? 12 : 16;
     if (t.split_levels > 64) t.split_levels = 64;
     if (t.split_tris == 0) t.split_tris = 256;
     if (t.stack_entries != 1024 && t.stack_entries != 2048) t.stack_entries = 512;
     if (t.pop_width == 0 || t.pop_width > 64) t.pop_width = 64;
-    if (t.waves_per_cu == 0) t.waves_per_cu = 6;
+    if (t.waves_per_cu == 0) t.waves_per_cu = small ? 4 : 6;
     if (t.spin_polls == 0) t.spin_polls = 4096;
     if (t.handoff_pct == 0) t.handoff_pct = 100;
     if (t.resplit_levels == 0) t.resplit_levels = 4;
@@ -443,12 +446,12 @@ int capture_graph(cd_ctx* ctx, hipGraphExec_t* exec, F&& enq) {
 // batched machinery; else the exact-predicate batched path.
 int run_batched(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts, bool ref) {
     hipStream_t st = ctx->stream;
-    const cd_tuning t = effective_tuning(ctx);
     // the launch geometry (and a captured graph) is built for a power-of-two
     // capacity; the kernels read the count itself from d_nq, so frames whose
     // pair counts differ replay one graph
     uint32_t nq_cap = 64;
     while (nq_cap < nq) nq_cap *= 2;
+    const cd_tuning t = effective_tuning(ctx, nq_cap);
     RunCounters h{};
     for (int attempt = 0; attempt < 8; ++attempt) {
         uint64_t fcap = 0;
@@ -1809,7 +1812,6 @@ int device_run(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, bool sh
     StepArgs a{};
     a.n = n;
     a.shoot = shoot;
-    a.t = effective_tuning(ctx);
     a.ref = ref;
     a.stack_entries = stack_entries;
     shoot_params(params, &a.cos_start, &a.cos_finish, &a.bias);
@@ -1832,6 +1834,7 @@ int device_run(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, bool sh
         a.nc = ctx->step_nc_cap;
         a.rays = ctx->step_ray_cap;
         a.radix = ctx->step_radix;
+        a.t = effective_tuning(ctx, a.np);
         uint64_t fcap = 0;
         int rc = ensure_step(ctx, a, &fcap);
         if (rc) return rc;
