@@ -767,6 +767,8 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_pairs.release();
     ctx->h_step.release();
     ctx->h_ents.release();
+    ctx->d_rayslab.release();
+    ctx->d_rootslab.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
     if (ctx->frame_graph) (void)hipGraphExecDestroy(ctx->frame_graph);
@@ -1425,6 +1427,25 @@ int frame_impl(cd_ctx* ctx, const cd_entity* entities, uint32_t n, uint32_t mode
     return frame_output(ctx, hc, nc, pairs, pair_cap, n_pairs, contacts, contact_cap, n_contacts);
 }
 
+// The slab records of the ray shooting's box tests for the given meshes,
+// recomputed for a mesh whose boxes changed since (a build or a refit).
+int ensure_slabs(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n) {
+    hipStream_t st = ctx->stream;
+    ctx->slab_ok.resize(ctx->meshes.size(), 0);
+    HCK(ctx, grow_keep(ctx->d_rayslab, (2 * ctx->nodes_used + 2) * sizeof(RaySlab), st));
+    HCK(ctx, grow_keep(ctx->d_rootslab, (ctx->meshes.size() + 1) * sizeof(RaySlab), st));
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t m = mesh_ids[i];
+        if (ctx->slab_ok[m]) continue;
+        const MeshRec& r = ctx->meshes[m];
+        HCK(ctx, launch_ray_slabs(ctx->d_nodes.as<NodeRec>() + r.node_base, r.nnodes,
+                                  ctx->d_rayslab.as<RaySlab>() + 2 * size_t(r.node_base), ctx->d_meshes.as<MeshRec>(),
+                                  m, ctx->d_rootslab.as<RaySlab>() + m, st));
+        ctx->slab_ok[m] = 1;
+    }
+    return CD_OK;
+}
+
 // ---- rays (CreateUncollideRays.cpp:60-200) -----------------------------------
 // Slots of the ray generation's vertex hash set for nc contacts: more than
 // the vertices offered (3 per contact), so that every probe ends.
@@ -1766,7 +1787,7 @@ int enqueue_step(cd_ctx* ctx, const StepArgs& a) {
     HCK(ctx, launch_pair_begins(qw, P, dc, pbegin, st));
     HCK(ctx, launch_zero(sctr, st));
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), R, 0, pbegin, P, dq, dm, ctx->d_nodes.as<NodeRec>(),
-                          ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                          ctx->d_rayslab.as<RaySlab>(), ctx->d_rootslab.as<RaySlab>(), ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
                           ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
                           a.stack_entries, sctr, st, gstack, P, dc));
     HCK(ctx, launch_respond(ctx->d_work[W_SHOTS].as<RayShot>(), 0, qw, P, ctx->d_work[W_RAVG].as<float>(), dq, spairs, de,
@@ -1800,6 +1821,7 @@ void step_key(const cd_ctx* ctx, const StepArgs& a, uint64_t fcap, std::vector<u
     for (const DevBuf* b : {&ctx->d_leafof, &ctx->d_attrs, &ctx->d_vid, &ctx->d_nrm, &ctx->d_perm})
         key.push_back(reinterpret_cast<uint64_t>(b->p));
     for (const HostBuf* b : {&ctx->h_step, &ctx->h_ents, &ctx->h_stage}) key.push_back(reinterpret_cast<uint64_t>(b->p));
+    for (const DevBuf* b : {&ctx->d_rayslab, &ctx->d_rootslab}) key.push_back(reinterpret_cast<uint64_t>(b->p));
 }
 
 // The frame (shoot false) or the whole step on the device, run until its
@@ -1829,6 +1851,10 @@ int device_run(cd_ctx* ctx, const cd_entity* ents, uint32_t n, bool ref, bool sh
     ctx->contacts_sorted = false;
     ctx->frame_mesh.resize(n);
     for (uint32_t i = 0; i < n; ++i) ctx->frame_mesh[i] = ents[i].mesh_id;
+    if (shoot) {
+        int rc = ensure_slabs(ctx, ctx->frame_mesh.data(), n);
+        if (rc) return rc;
+    }
     for (int attempt = 0; attempt < 8; ++attempt) {
         a.np = ctx->step_np_cap;
         a.nc = ctx->step_nc_cap;
@@ -2055,11 +2081,19 @@ int cd_shoot_rays(cd_ctx* ctx, const cd_shoot_params* params, cd_pair_response* 
         HCK(ctx, ctx->d_work[W_GSTACK].ensure(blocks * 2 * stack_entries * 64 * sizeof(uint32_t)));
         gstack = ctx->d_work[W_GSTACK].as<uint32_t>();
     }
+    {
+        std::vector<uint32_t> used;
+        used.reserve(2 * size_t(nq));
+        for (uint32_t q = 0; q < nq; ++q)
+            for (uint32_t e : {ctx->staged_ids[q].x, ctx->staged_ids[q].y}) used.push_back(ctx->frame_mesh[e]);
+        int rc = ensure_slabs(ctx, used.data(), uint32_t(used.size()));
+        if (rc) return rc;
+    }
     HCK(ctx, launch_zero(dev_counters(ctx), st));
     HCK(ctx, hipEventRecord(ctx->ev[0], st));
     HCK(ctx, launch_shoot(ctx->d_work[W_RAYS].as<cd_ray>(), uint32_t(ctx->ray_total), ctx->ray_base1, pbegin, nq,
                           ctx->d_queries.as<QueryRec>(), ctx->d_meshes.as<MeshRec>(), ctx->d_nodes.as<NodeRec>(),
-                          ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
+                          ctx->d_rayslab.as<RaySlab>(), ctx->d_rootslab.as<RaySlab>(), ctx->d_tris.as<float>(), ctx->d_attrs.as<uint2>(), ctx->d_vid.as<uint32_t>(),
                           ctx->d_nrm.as<float>(), ctx->d_perm.as<uint32_t>(), ctx->d_work[W_SHOTS].as<RayShot>(),
                           stack_entries, dev_counters(ctx), st, gstack));
     HCK(ctx, hipEventRecord(ctx->ev[1], st));
@@ -2184,6 +2218,8 @@ int cd_skin_refit(cd_ctx* ctx, const uint32_t* mesh_ids, uint32_t n, const float
     if (!ctx) return CD_E_INVALID;
     if (n == 0) return CD_OK;
     if (!mesh_ids || !bone_matrices) return fail(ctx, CD_E_INVALID, "null argument");
+    for (uint32_t i = 0; i < n; ++i)  // their boxes change: the ray shooting's slab records too
+        if (mesh_ids[i] < ctx->slab_ok.size()) ctx->slab_ok[mesh_ids[i]] = 0;
     (void)hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     std::vector<SkinRec> recs(n);

@@ -98,6 +98,19 @@ struct RayShot {
     uint32_t ok;    // bit p: pass p succeeded
     uint32_t pair;  // query (entity pair) of the ray
 };
+// The box-only terms of the ray shooting's box test for one stored box
+// (centre c, half-sides u, v, w): the three slab normals v x w, w x u, u x v,
+// the box's projection radius on each, and the 1-norm sums of c and of
+// u, v, w that bound the test's rounding.  Computed once per box (after each
+// build or refit of its mesh) with the very operations the test would use.
+struct alignas(64) RaySlab {
+    float n[9];
+    float h[3];
+    float cr, sr;
+    float pad[2];
+};
+static_assert(sizeof(RaySlab) == 64, "slab record layout");
+
 constexpr int RAY_STACK_ENTRIES = 48;  // closest-hit traversal stack per lane (LDS): a tree's depth + 2
 constexpr uint32_t RAY_LINK_BITS = 25;  // a stack entry packs a node or triangle index below 2^25 with a leaf count
 
@@ -303,6 +316,11 @@ struct cd_ctx {
     // the frame and the scene step on the device: capacities (grown when a
     // run exceeds one), their captured graphs and keys, and the pinned block
     // their results land in
+    // slab records of the ray shooting's box tests: two per internal node
+    // (node arena order), one per mesh root; valid for a mesh until it is
+    // refit
+    cdk::DevBuf d_rayslab, d_rootslab;
+    std::vector<uint8_t> slab_ok;
     uint32_t step_np_cap = 0, step_nc_cap = 0, step_ray_cap = 0;
     bool step_radix = false;  // a pair had more contacts than the per-pair order ranks: the radix sorts
     hipGraphExec_t step_graph = nullptr, frame_graph = nullptr;  // the whole step; cd_frame's part of it

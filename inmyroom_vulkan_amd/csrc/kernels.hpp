@@ -109,11 +109,18 @@ hipError_t launch_sort_perm(const cd_contact* in, uint32_t n, int f_hi, int f_mi
 // stride: the per-side spacing of pbegin (0: nq); dc: the step's count block
 // (DC_*), from which the ray and pair counts are read when given, nrays and
 // nq being the capacities.
+// slabs / roots: the slab records of every node's children and every mesh's
+// root (launch_ray_slabs), current for the meshes the rays meet.
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
-                        const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
+                        const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const RaySlab* slabs,
+                        const RaySlab* roots, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
                         uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack = nullptr,
                         uint32_t stride = 0, const uint32_t* dc = nullptr);
+// Slab records of count internal nodes' children (out: 2 per node) and of
+// mesh m's root (root_out).
+hipError_t launch_ray_slabs(const NodeRec* nodes, uint32_t count, RaySlab* out, const MeshRec* meshes, uint32_t m,
+                            RaySlab* root_out, hipStream_t st);
 // qw / avg per side at stride spacing (0: nq), dc as for launch_shoot, nshots
 // the shots written.
 hipError_t launch_respond(const RayShot* shots, uint32_t base1, const uint32_t* qw, uint32_t nq, const float* avg,

@@ -148,16 +148,35 @@ __device__ __forceinline__ bool ray_box(f3 dir, const Box& b, float& tmin, float
 // at the ray's origin, a flat box, a slab the ray runs along); the caller
 // then runs the reference's own test.  tmin/tmax: the unwidened parameters,
 // for the order of the children and the pruning against the best hit.
+// The box-only terms (RaySlab), computed once per stored box.
+__device__ __forceinline__ RaySlab make_slab(const float* bx) {
+    const f3 c = mk(bx[0], bx[1], bx[2]);
+    const f3 u = mk(bx[3], bx[4], bx[5]), v = mk(bx[6], bx[7], bx[8]), w = mk(bx[9], bx[10], bx[11]);
+    const f3 nrm[3] = {cross(v, w), cross(w, u), cross(u, v)};
+    RaySlab s;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f3 n = nrm[k];
+        s.n[3 * k] = n.x;
+        s.n[3 * k + 1] = n.y;
+        s.n[3 * k + 2] = n.z;
+        s.h[k] = (fabsf(dot(n, u)) + fabsf(dot(n, v))) + fabsf(dot(n, w));
+    }
+    s.cr = norm1f(c);
+    s.sr = (norm1f(u) + norm1f(v)) + norm1f(w);
+    s.pad[0] = s.pad[1] = 0.f;
+    return s;
+}
 enum : int { BOX_OUT = 0, BOX_IN = 1, BOX_NEAR = 2 };
-__device__ __forceinline__ int box_local(const LocalRay& R, const float* bx, float& tmin, float& tmax) {
+__device__ __forceinline__ int box_local(const LocalRay& R, const float* bx, const RaySlab& sl, float& tmin,
+                                         float& tmax) {
     tmin = -__builtin_huge_valf();
     tmax = __builtin_huge_valf();
     if (!R.ok) return BOX_NEAR;
     const f3 c = mk(bx[0], bx[1], bx[2]);
-    const f3 u = mk(bx[3], bx[4], bx[5]), v = mk(bx[6], bx[7], bx[8]), w = mk(bx[9], bx[10], bx[11]);
     const f3 q = sub(R.o, c);
-    const float reach = ((norm1f(q) + R.on) + norm1f(c)) + ((norm1f(u) + norm1f(v)) + norm1f(w));
-    const f3 nrm[3] = {cross(v, w), cross(w, u), cross(u, v)};
+    const float reach = ((norm1f(q) + R.on) + sl.cr) + sl.sr;
+    const f3 nrm[3] = {mk(sl.n[0], sl.n[1], sl.n[2]), mk(sl.n[3], sl.n[4], sl.n[5]), mk(sl.n[6], sl.n[7], sl.n[8])};
     float lo_w = -__builtin_huge_valf(), hi_w = __builtin_huge_valf();
     float lo_s = -__builtin_huge_valf(), hi_s = __builtin_huge_valf();
     bool decided = true;
@@ -165,7 +184,7 @@ __device__ __forceinline__ int box_local(const LocalRay& R, const float* bx, flo
     for (int k = 0; k < 3; ++k) {
         const f3 n = nrm[k];
         const float nn = norm1f(n);
-        const float h = (fabsf(dot(n, u)) + fabsf(dot(n, v))) + fabsf(dot(n, w));
+        const float h = sl.h[k];
         const float e = R.kappa * nn * reach;  // bound on the rounding of h and num
         const float num = dot(n, q);
         const float den = dot(n, R.d);
@@ -282,9 +301,13 @@ __device__ __forceinline__ float partner(float* sbest, int lane) {
 // is skipped, and the walk stops once the partner ends without a back-face
 // hit (the pass fails whatever this walk finds).
 constexpr float BOUND_SLACK = 1.f + 0x1p-12f;
-__device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes, const float* __restrict__ tris, M4 C,
-                        f3 o, f3 dir, RayStack& S, unsigned long long* err, uint32_t& nbox, uint32_t& ntri,
-                        float* sbest, bool bounded, float eps) {
+// nslab: the slab records of this mesh's nodes' children (2 per node),
+// rslab: its root's.
+__device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restrict__ nodes,
+                                        const RaySlab* __restrict__ nslab, const RaySlab& rslab,
+                                        const float* __restrict__ tris, M4 C, f3 o, f3 dir, RayStack& S,
+                                        unsigned long long* err, uint32_t& nbox, uint32_t& ntri, float* sbest,
+                                        bool bounded, float eps) {
     C.m[12] = C.m[12] - o.x;  // centred on the ray origin (Ray.cpp:150-154)
     C.m[13] = C.m[13] - o.y;
     C.m[14] = C.m[14] - o.z;
@@ -293,8 +316,8 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
     publish(sbest, S.lane, __builtin_huge_valf());
     const LocalRay R = local_ray(C, dir);
     // the mesh-frame test, or the reference's own where that one is within rounding
-    auto enter = [&](const float* bx, float& t0, float& t1) -> bool {
-        const int k = box_local(R, bx, t0, t1);
+    auto enter = [&](const float* bx, const RaySlab& sl, float& t0, float& t1) -> bool {
+        const int k = box_local(R, bx, sl, t0, t1);
         if (k != BOX_NEAR) return k == BOX_IN;
         return ray_box(dir, xform_box(C, box_from12(bx)), t0, t1) && t1 >= 0.f;
     };
@@ -304,7 +327,7 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
         publish(sbest, S.lane, (best.hit && best.back) ? best.dist : -1.f);
         return best;
     };
-    if (!enter(m.root.box, r0, r1)) return finish();
+    if (!enter(m.root.box, rslab, r0, r1)) return finish();
     if (m.nnodes == 0) {  // the root is a leaf
         ntri += m.ntri;
         leaf_hits(tris, m, C, dir, 0, m.ntri, best);
@@ -324,8 +347,9 @@ __device__ __forceinline__ Hit ray_tree(const MeshRec& m, const NodeRec* __restr
         if (at_node) {
             const NodeRec& n = nb[node];
             float l0, l1, q0, q1;
-            const bool lh = enter(n.c[0].box, l0, l1);
-            const bool rh = enter(n.c[1].box, q0, q1);
+            const RaySlab* ns = nslab + 2 * size_t(node);
+            const bool lh = enter(n.c[0].box, ns[0], l0, l1);
+            const bool rh = enter(n.c[1].box, ns[1], q0, q1);
             nbox += 2;
             if (sp + 2 > S.cap) {  // deeper than the host allows (it sizes the stack from the tree depths)
                 atomicOr(err, 1ull);
@@ -401,6 +425,8 @@ struct Pass {
 struct MeshView {
     const MeshRec* meshes;
     const NodeRec* nodes;
+    const RaySlab* slabs;  // 2 per node, node arena order
+    const RaySlab* roots;  // per mesh
     const float* tris;
     const uint2* attrs;
     const uint32_t* vid;
@@ -434,8 +460,10 @@ __device__ __forceinline__ Pass hermann(const MeshView& V, uint32_t meshA, const
 #pragma unroll
     for (int e = 0; e < 16; ++e) C.m[e] = own ? Amat.m[e] : Bmat.m[e];
     const float eps = vlength(sub(o2, o));
-    const Hit h = ray_tree(V.meshes[own ? meshA : meshB], V.nodes, V.tris, C, own ? o2 : o, d, S, err, nbox, ntri,
-                           sbest, own, eps);
+    const uint32_t mesh = own ? meshA : meshB;
+    const MeshRec& mw = V.meshes[mesh];
+    const Hit h = ray_tree(mw, V.nodes, V.slabs + 2 * size_t(mw.node_base), V.roots[mesh], V.tris, C, own ? o2 : o, d, S,
+                           err, nbox, ntri, sbest, own, eps);
     const int even = int(threadIdx.x & ~1u);
     const Hit p2 = shfl_hit(h, even), p3 = shfl_hit(h, even + 1);
     if (!(p2.hit && p2.back)) return r;
@@ -662,15 +690,29 @@ __global__ __launch_bounds__(64) void k_respond(const RayShot* __restrict__ shot
     out[q] = res;
 }
 
+// Slab records: one thread per child of count nodes, and one for mesh m's root.
+__global__ void k_ray_slabs(const NodeRec* __restrict__ nodes, uint32_t count, RaySlab* __restrict__ out,
+                            const MeshRec* __restrict__ meshes, uint32_t m, RaySlab* __restrict__ root_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * count) out[i] = make_slab(nodes[i >> 1].c[i & 1u].box);
+    if (i == 0) *root_out = make_slab(meshes[m].root.box);
+}
+hipError_t launch_ray_slabs(const NodeRec* nodes, uint32_t count, RaySlab* out, const MeshRec* meshes, uint32_t m,
+                            RaySlab* root_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ray_slabs, dim3((2 * count + 255) / 256 + 1), dim3(256), 0, st, nodes, count, out, meshes, m,
+                       root_out);
+    return hipGetLastError();
+}
 hipError_t launch_shoot(const cd_ray* rays, uint32_t nrays, uint32_t base1, const uint32_t* pbegin, uint32_t nq,
-                        const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const float* tris,
+                        const QueryRec* qs, const MeshRec* meshes, const NodeRec* nodes, const RaySlab* slabs,
+                        const RaySlab* roots, const float* tris,
                         const uint2* attrs, const uint32_t* vid, const float* nrm, const uint32_t* perm, RayShot* shots,
                         uint32_t stack_entries, RunCounters* ctr, hipStream_t st, uint32_t* gstack, uint32_t stride,
                         const uint32_t* dc) {
     if (!nrays) return hipSuccess;
     if (stack_entries < 2) return hipErrorInvalidValue;
     if (!stride) stride = nq;
-    const MeshView V = {meshes, nodes, tris, attrs, vid, nrm, perm};
+    const MeshView V = {meshes, nodes, slabs, roots, tris, attrs, vid, nrm, perm};
     const uint64_t lanes = 2ull * nrays;
     const dim3 grid(uint32_t((lanes + 63) / 64));
     if (stack_entries <= uint32_t(RAY_STACK)) {

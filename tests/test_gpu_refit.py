@@ -115,3 +115,36 @@ def test_deformable_vs_terrain_exact_pairs(gpu, orc):
         assert mine == want, k
         total += len(want)
     assert total > 0
+
+
+def test_rays_after_refit_see_the_new_boxes():
+    """Ray shooting keeps per-box terms of its box test (computed once per
+    mesh); a refit must renew them.  One context refits twice and shoots
+    after each; a second context refits once, to the second pose: their
+    responses after that pose agree bit for bit."""
+    from inmyroom_vulkan_amd import ENTITY_DTYPE, MODE_REFERENCE, Collider
+    db = _batch(n=1, nu=48, nv=24)
+    case("rays after a refit use the refit boxes")
+    bones_a, bones_b = db.bone_matrices(), db.bone_matrices()
+    shift = np.eye(4, dtype=np.float32)
+    shift[3, :3] = (0.35, 0.1, 0.0)
+    moved = np.eye(4, dtype=np.float32)
+    moved[3, :3] = (0.3, 0.1, 0.0)
+    e = np.zeros(2, ENTITY_DTYPE)
+    e["current"] = [scenes.identity16(), shift.reshape(-1)]
+    e["previous"] = [scenes.identity16(), moved.reshape(-1)]
+    e["should_callback"] = 1
+    out = []
+    for bones_seq in ((bones_a, bones_b), (bones_b,)):
+        gpu = Collider(leaf_max=4)
+        try:
+            ids = _register(gpu, db)
+            e["mesh_id"] = ids[0]
+            for bones in bones_seq:
+                gpu.skin_refit(ids, bones)
+                got = gpu.step(e, MODE_REFERENCE)
+            out.append(got.copy())
+        finally:
+            gpu.close()
+    assert len(out[0]) > 0 and int(out[0]["shot"].sum()) > 0
+    assert out[0].tobytes() == out[1].tobytes()
