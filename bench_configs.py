@@ -261,9 +261,10 @@ def config2(args):
             boxes += c["sat_tests"]
             tritests += c["tri_tests"]
         full = (time.perf_counter() - t) * 1e3 / len(frames)
-        # k_shoot's roofline: 64 B per box tested (one child record), 36 B per
-        # ray-triangle test, 24 B per ray read and 56 B per ray result written
-        shoot_bytes = 64.0 * boxes + 36.0 * tritests + (24.0 + 56.0) * nrays
+        # k_shoot's roofline: 128 B per box tested (its child record and its
+        # slab record), 36 B per ray-triangle test, 24 B per ray read and 56 B
+        # per ray result written
+        shoot_bytes = 128.0 * boxes + 36.0 * tritests + (24.0 + 56.0) * nrays
         shoot_gbs = shoot_bytes / (shoot_ms * 1e-3) / 1e9
         out[name].update({"full_ms_per_frame": round(full, 4), "rays_per_frame": nrays / len(frames),
                           "pairs_shot_per_frame": nshot / len(frames),
