@@ -184,10 +184,13 @@ int run_plain(cd_ctx* ctx, uint32_t nq, uint64_t* n_contacts) {
 cd_tuning effective_tuning(const cd_ctx* ctx, uint32_t nq_cap = 0) {
     // defaults: the values the tuning sweeps pick with the pipelined traversal:
     // for the 4,096-query batch 16 levels, pairs over 256 triangles split
-    // further, 512-entry stacks, 6 wavefronts per CU; for runs of up to 2,048
-    // queries (a 512-entity scene's ~1,100 pairs) 12 levels and 4 wavefronts
+    // further, 512-entry stacks, 6 wavefronts per CU; for runs of 512 to
+    // 2,048 queries (a 512-entity scene's ~1,100 pairs, 1,000 poses) 12
+    // levels and 4 wavefronts.  Fewer queries keep 16 levels: their trees
+    // must be split further to give the persistent kernel enough tasks (64
+    // skinned meshes against a terrain: 0.32 ms against 0.40 with 12).
     cd_tuning t = ctx->tuning;
-    const bool small = nq_cap != 0 && nq_cap <= 2048;
+    const bool small = nq_cap >= 512 && nq_cap <= 2048;
     if (t.split_levels == 0) t.split_levels = small ? 12 : 16;
     if (t.split_levels > 64) t.split_levels = 64;
     if (t.split_tris == 0) t.split_tris = 256;
