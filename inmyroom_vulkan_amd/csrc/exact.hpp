@@ -216,15 +216,30 @@ constexpr int DEFERRED = -1;
 //   tt_part2  T2's vertices against T1's plane: 0, 2 (coplanar) or MORE;
 //   tt_part3  the canonical form and its two final orientations: 0 or 1.
 // Each may also return DEFERRED at a level below 2.
+// At level 0 the three orientations of a part are evaluated before one
+// combined check: with a check after each, the compiler moves the loads of
+// the later vertices behind it, and a test waits for memory three times.
 constexpr int TT_MORE = 3;
 template <int L>
+__device__ __forceinline__ bool any_undecided(int a, int b, int c) {
+    return L < 2 && ((a == UNDECIDED) | (b == UNDECIDED) | (c == UNDECIDED));
+}
+template <int L>
 __device__ __forceinline__ int tt_part1(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
-    const int dp1 = orient_at<L>(p2, q2, r2, p1, st);
-    CD_DECIDE(dp1);
-    const int dq1 = orient_at<L>(p2, q2, r2, q1, st);
-    CD_DECIDE(dq1);
-    const int dr1 = orient_at<L>(p2, q2, r2, r1, st);
-    CD_DECIDE(dr1);
+    int dp1, dq1, dr1;
+    if (L == 0) {
+        dp1 = orient_at<L>(p2, q2, r2, p1, st);
+        dq1 = orient_at<L>(p2, q2, r2, q1, st);
+        dr1 = orient_at<L>(p2, q2, r2, r1, st);
+        if (any_undecided<L>(dp1, dq1, dr1)) return DEFERRED;
+    } else {
+        dp1 = orient_at<L>(p2, q2, r2, p1, st);
+        CD_DECIDE(dp1);
+        dq1 = orient_at<L>(p2, q2, r2, q1, st);
+        CD_DECIDE(dq1);
+        dr1 = orient_at<L>(p2, q2, r2, r1, st);
+        CD_DECIDE(dr1);
+    }
     dsides = (uint32_t(dp1 + 1) << 6) | (uint32_t(dq1 + 1) << 8) | (uint32_t(dr1 + 1) << 10);
     if (dp1 * dq1 > 0 && dp1 * dr1 > 0) return 0;
     return TT_MORE;
@@ -232,12 +247,20 @@ __device__ __forceinline__ int tt_part1(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2
 template <int L>
 __device__ __forceinline__ int tt_part2(f3 p1, f3 q1, f3 r1, f3 p2, f3 q2, f3 r2, uint32_t& dsides, PredStats& st) {
     const int dp1 = side_of(dsides, 3), dq1 = side_of(dsides, 4), dr1 = side_of(dsides, 5);
-    const int dp2 = orient_at<L>(q1, r1, p1, p2, st);
-    CD_DECIDE(dp2);
-    const int dq2 = orient_at<L>(q1, r1, p1, q2, st);
-    CD_DECIDE(dq2);
-    const int dr2 = orient_at<L>(q1, r1, p1, r2, st);
-    CD_DECIDE(dr2);
+    int dp2, dq2, dr2;
+    if (L == 0) {
+        dp2 = orient_at<L>(q1, r1, p1, p2, st);
+        dq2 = orient_at<L>(q1, r1, p1, q2, st);
+        dr2 = orient_at<L>(q1, r1, p1, r2, st);
+        if (any_undecided<L>(dp2, dq2, dr2)) return DEFERRED;
+    } else {
+        dp2 = orient_at<L>(q1, r1, p1, p2, st);
+        CD_DECIDE(dp2);
+        dq2 = orient_at<L>(q1, r1, p1, q2, st);
+        CD_DECIDE(dq2);
+        dr2 = orient_at<L>(q1, r1, p1, r2, st);
+        CD_DECIDE(dr2);
+    }
     dsides |= uint32_t(dp2 + 1) | (uint32_t(dq2 + 1) << 2) | (uint32_t(dr2 + 1) << 4);
     if (dp2 * dq2 > 0 && dp2 * dr2 > 0) return 0;
     if ((dp1 == 0 && dq1 == 0 && dr1 == 0) || (dp2 == 0 && dq2 == 0 && dr2 == 0)) return 2;
