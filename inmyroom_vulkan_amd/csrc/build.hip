@@ -4,8 +4,10 @@
 // inMyRoom_vulkan/src/Geometry/OBBtree.cpp:8-108 (recursive split) and
 // :179-233, :321-394 (DFS flatten).
 //
-// Level-synchronous: every node of one depth, across every mesh of the
-// batch, is one 64-lane workgroup of a single launch.  Within a node:
+// Level-synchronous: the nodes of one depth, across every mesh of the
+// batch, are one launch, one 64-lane workgroup per node at a time (levels
+// are launched in batches; each launch reads its node range from the
+// previous one).  Within a node:
 //   * the distinct-point check is a wave ballot;
 //   * the mean and the six covariance sums are SEQUENTIAL double chains in
 //     point order (the reference's std::accumulate / std::inner_product),
@@ -45,7 +47,6 @@ struct LevelArgs {
     uint32_t* id_out;
     float* pos_fin;
     uint32_t* id_fin;
-    unsigned int* next_count;
     unsigned int* mesh_depth;
     uint32_t leaf_max;
 };
@@ -62,6 +63,13 @@ static_assert(CHUNK * 3 % 64 == 0 && CHUNK % 64 == 0, "chunk split over a wave")
 // current chunk into its K terms per point (term(p, c, out) for the points
 // lane, lane + 64, ...), and the first K lanes add their term of each point
 // in order while the fetch is in flight.  The block is one wave.
+// k_level's workgroup is one wavefront: its lanes run in lockstep and see
+// each other's LDS accesses in program order, so a step between the phases
+// of a node only has to keep the compiler from moving LDS accesses across
+// it.  (__syncthreads would also wait for the global loads in flight, among
+// them the prefetch of the next chunk of points.)
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
 template <int K, class Term>
 __device__ __forceinline__ double stream_chain(const float* __restrict__ P, uint32_t npts, Term term, float* s_raw,
                                                double* s_val) {
@@ -81,7 +89,7 @@ __device__ __forceinline__ double stream_chain(const float* __restrict__ P, uint
     };
     fetch(0);
     store();
-    __syncthreads();
+    wave_sync();
     double acc = 0.0;
     for (uint32_t base = 0; base < npts; base += CHUNK) {
         const uint32_t next = base + CHUNK;
@@ -92,18 +100,38 @@ __device__ __forceinline__ double stream_chain(const float* __restrict__ P, uint
             const uint32_t k = uint32_t(lane) + 64u * j;
             if (k < m) term(s_raw + 3 * k, s_val + K * k);
         }
-        __syncthreads();
+        wave_sync();
         if (lane < K) {
+            // 16 terms are read from LDS while the previous 16 are added,
+            // so the chain waits on its additions only
             const double* q = s_val + lane;
-#pragma unroll 16
-            for (uint32_t k = 0; k < m; ++k) acc = acc + q[K * k];
+            uint32_t k = 0;
+            if (m >= 16) {
+                double cur[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) cur[i] = q[K * i];
+                for (k = 16; k + 16 <= m; k += 16) {
+                    double nxt[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) nxt[i] = q[K * (k + uint32_t(i))];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) acc = acc + cur[i];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) cur[i] = nxt[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc = acc + cur[i];
+            }
+            for (; k < m; ++k) acc = acc + q[K * k];
         }
-        __syncthreads();
+        wave_sync();
         if (next < npts) store();
-        __syncthreads();
+        wave_sync();
     }
     return acc;
-}__device__ inline void wave_min_last(double& v, int& i) {
+}
+
+__device__ inline void wave_min_last(double& v, int& i) {
     // smaller value wins; equal values (== , so -0 and +0 too): larger index wins
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -155,8 +183,8 @@ __device__ inline void sort3_desc(AxisCand c[3]) {
 }
 
 // Triangle::GetMinMaxProjectionToAxis (Triangle.cpp:113-138) then the
-// left/right rule of OBBtree.cpp:66-75.
-__device__ inline bool goes_left(const float* T, f3 axis, float cproj) {
+// left/right rule of OBBtree.cpp:66-75, on a triangle held in registers.
+__device__ inline bool goes_left(const float (&T)[9], f3 axis, float cproj) {
     float lo = INFINITY, hi = -INFINITY;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -168,22 +196,42 @@ __device__ inline bool goes_left(const float* T, f3 axis, float cproj) {
     return mid <= cproj;
 }
 
-__global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint32_t next_first) {
-    __shared__ float s_raw[CHUNK * 3];
-    __shared__ double s_val[CHUNK * 6];
-    __shared__ double s_mean[3];
-    __shared__ double s_cov[6];
-    __shared__ double s_ax[9];
-    __shared__ float s_box[12];
+// The node scans below take GROUP wave-wide chunks per step and issue the
+// loads of all of them before using any: a node's scans are latency bound
+// (one wave per node), and a chunk at a time waits on memory once per chunk.
+constexpr int GROUP = 8;
 
+__device__ __forceinline__ void load_tris(const float* __restrict__ P, uint32_t c0, uint32_t n, int lane,
+                                          float (&T)[GROUP][9]) {
+#pragma unroll
+    for (int j = 0; j < GROUP; ++j) {
+        const uint32_t i = c0 + 64u * j + uint32_t(lane);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) T[j][t] = i < n ? P[size_t(i) * 9 + t] : 0.f;
+    }
+}
+
+struct LevelShared {
+    float raw[CHUNK * 3];
+    double val[CHUNK * 6];
+    double mean[3];
+    double cov[6];
+    double ax[9];
+    float box[12];
+};
+
+// One node of a level: its box (OBB::CreateOBBfromPoints, OBB.cpp:33-89),
+// then either its triangles parked as a leaf or its split into two
+// children appended to the next level (OBBtreeSplitBuildNode::
+// SplitOBBandCreateChildren, OBBtree.cpp:43-108).
+__device__ __forceinline__ void level_node(const LevelArgs& a, uint32_t ni, uint32_t next_first,
+                                           unsigned int* next_count, LevelShared& S) {
     const int lane = threadIdx.x;
-    const uint32_t ni = first + blockIdx.x;
     BNode* nd = &a.nodes[ni];
     const uint32_t beg = nd->beg, end = nd->end;
     const uint32_t n = end - beg, npts = 3 * n;
     const float* P = a.pos_in + size_t(beg) * 9;
 
-    // ---- OBB::CreateOBBfromPoints (OBB.cpp:33-89) ----
     // more than one distinct point?  (the first wave-wide chunk that shows one ends the scan)
     bool multi = false;
     if (npts > 1) {
@@ -210,11 +258,11 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
                 o[1] = double(p[1]);
                 o[2] = double(p[2]);
             },
-            s_raw, s_val);
-        if (lane < 3) s_mean[lane] = acc / N;
-        __syncthreads();
+            S.raw, S.val);
+        if (lane < 3) S.mean[lane] = acc / N;
+        wave_sync();
         // covariance order xx, yy, zz, xy, xz, yz (OBB.cpp:56-73)
-        const double mx = s_mean[0], my = s_mean[1], mz = s_mean[2];
+        const double mx = S.mean[0], my = S.mean[1], mz = S.mean[2];
         acc = stream_chain<6>(
             P, npts,
             [mx, my, mz](const float* p, double* o) {
@@ -226,23 +274,24 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
                 o[4] = dx * dz;
                 o[5] = dy * dz;
             },
-            s_raw, s_val);
-        if (lane < 6) s_cov[lane] = acc / N;
-        __syncthreads();
+            S.raw, S.val);
+        if (lane < 6) S.cov[lane] = acc / N;
+        wave_sync();
         if (lane == 0) {
-            const double xx = s_cov[0], yy = s_cov[1], zz = s_cov[2], xy = s_cov[3], xz = s_cov[4], yz = s_cov[5];
+            const double xx = S.cov[0], yy = S.cov[1], zz = S.cov[2], xy = S.cov[3], xz = S.cov[4], yz = S.cov[5];
             double A[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
             double V[3][3], d[3];
             eig3(A, V, d);
             // OBB.cpp:80-87: the box axes are the ROWS of eig3's V
             for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c) s_ax[r * 3 + c] = V[r][c];
+                for (int c = 0; c < 3; ++c) S.ax[r * 3 + c] = V[r][c];
         }
-        __syncthreads();
+        wave_sync();
         double ax[9];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) ax[t] = s_ax[t];
-        // OBB::CreateAABBfromPoints (OBB.cpp:105-166)
+        for (int t = 0; t < 9; ++t) ax[t] = S.ax[t];
+        // OBB::CreateAABBfromPoints (OBB.cpp:105-166); each lane visits its
+        // points in increasing order, as the tie rules below assume
         double lo[3], hi[3];
         int loi[3], hii[3];
 #pragma unroll
@@ -252,14 +301,28 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
             loi[q] = -1;
             hii[q] = -1;
         }
-#pragma unroll 4
-        for (uint32_t k = lane; k < npts; k += 64) {
-            const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+        for (uint32_t k0 = 0; k0 < npts; k0 += 64u * GROUP) {
+            float px[GROUP], py[GROUP], pz[GROUP];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
-                if (!(lo[q] < pr)) { lo[q] = pr; loi[q] = int(k); }
-                if (hi[q] < pr) { hi[q] = pr; hii[q] = int(k); }
+            for (int j = 0; j < GROUP; ++j) {
+                const uint32_t k = k0 + 64u * j + uint32_t(lane);
+                const bool in = k < npts;
+                px[j] = in ? P[3 * size_t(k)] : 0.f;
+                py[j] = in ? P[3 * size_t(k) + 1] : 0.f;
+                pz[j] = in ? P[3 * size_t(k) + 2] : 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < GROUP; ++j) {
+                const uint32_t k = k0 + 64u * j + uint32_t(lane);
+                if (k < npts) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        double pr = (ax[3 * q] * double(px[j]) + ax[3 * q + 1] * double(py[j])) +
+                                    ax[3 * q + 2] * double(pz[j]);
+                        if (!(lo[q] < pr)) { lo[q] = pr; loi[q] = int(k); }
+                        if (hi[q] < pr) { hi[q] = pr; hii[q] = int(k); }
+                    }
+                }
             }
         }
 #pragma unroll
@@ -281,14 +344,14 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
                 side[q][1] = float(half * ax[3 * q + 1]);
                 side[q][2] = float(half * ax[3 * q + 2]);
             }
-            s_box[0] = float(cx);
-            s_box[1] = float(cy);
-            s_box[2] = float(cz);
+            S.box[0] = float(cx);
+            S.box[1] = float(cy);
+            S.box[2] = float(cz);
             for (int q = 0; q < 3; ++q)
-                for (int c = 0; c < 3; ++c) s_box[3 + 3 * q + c] = side[q][c];
+                for (int c = 0; c < 3; ++c) S.box[3 + 3 * q + c] = side[q][c];
         }
-        __syncthreads();
-        box = box_from12(s_box);
+        wave_sync();
+        box = box_from12(S.box);
     }
     if (lane == 0) {
         box_to12(box, nd->box);
@@ -307,50 +370,74 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
         return;
     }
 
-    // ---- OBBtreeSplitBuildNode::SplitOBBandCreateChildren (OBBtree.cpp:43-108) ----
+    // ---- split (OBBtree.cpp:43-108): the longest axis whose centre
+    // projection leaves both sides non-empty, else index halves; the three
+    // candidates are counted in one scan ----
     AxisCand cand[3] = {{vlength(box.u), normalize(box.u)}, {vlength(box.v), normalize(box.v)},
                         {vlength(box.w), normalize(box.w)}};
     sort3_desc(cand);
-    bool ok = false;
+    float cp[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) cp[ch] = dot(box.c, cand[ch].dir);
+    uint32_t cnt[3] = {0, 0, 0};
+    for (uint32_t c0 = 0; c0 < n; c0 += 64u * GROUP) {
+        float T[GROUP][9];
+        load_tris(P, c0, n, lane, T);
+#pragma unroll
+        for (int j = 0; j < GROUP; ++j) {
+            const bool valid = c0 + 64u * j + uint32_t(lane) < n;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch)
+                cnt[ch] += uint32_t(__popcll(__ballot(valid && goes_left(T[j], cand[ch].dir, cp[ch]))));
+        }
+    }
+    int pick = -1;
+#pragma unroll
+    for (int ch = 2; ch >= 0; --ch)
+        if (cnt[ch] != 0 && cnt[ch] != n) pick = ch;
+    const bool ok = pick >= 0;
     f3 axis = cand[0].dir;
     float cproj = 0.f;
-    uint32_t nl = 0;
-    for (int ch = 0; ch < 3 && !ok; ++ch) {
-        axis = cand[ch].dir;
-        cproj = dot(box.c, axis);
-        uint32_t cnt = 0;
-        for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-            const uint32_t i = c0 + lane;
-            const bool left = i < n && goes_left(P + size_t(i) * 9, axis, cproj);
-            cnt += uint32_t(__popcll(__ballot(left)));
+    uint32_t nl = n / 2;  // index halves (OBBtree.cpp:81-93)
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        if (ch == pick) {
+            axis = cand[ch].dir;
+            cproj = cp[ch];
+            nl = cnt[ch];
         }
-        nl = cnt;
-        ok = nl != 0 && nl != n;
-    }
-    if (!ok) nl = n / 2;  // index halves (OBBtree.cpp:81-93)
     const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     uint32_t runL = 0, runR = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
-        const uint32_t i = c0 + lane;
-        const bool valid = i < n;
-        bool left = false;
-        if (valid) left = ok ? goes_left(P + size_t(i) * 9, axis, cproj) : (i < n / 2);
-        const unsigned long long mL = __ballot(valid && left);
-        const unsigned long long mR = __ballot(valid && !left);
-        if (valid) {
-            const uint32_t dst = left ? beg + runL + uint32_t(__popcll(mL & below))
-                                      : beg + nl + runR + uint32_t(__popcll(mR & below));
-            const float* src = P + size_t(i) * 9;
-            float* o = a.pos_out + size_t(dst) * 9;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64u * GROUP) {
+        float T[GROUP][9];
+        load_tris(P, c0, n, lane, T);
+        uint32_t idv[GROUP];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) o[t] = src[t];
-            a.id_out[dst] = a.id_in[beg + i];
+        for (int j = 0; j < GROUP; ++j) {
+            const uint32_t i = c0 + 64u * j + uint32_t(lane);
+            idv[j] = i < n ? a.id_in[beg + i] : 0u;
         }
-        runL += uint32_t(__popcll(mL));
-        runR += uint32_t(__popcll(mR));
+#pragma unroll
+        for (int j = 0; j < GROUP; ++j) {
+            const uint32_t i = c0 + 64u * j + uint32_t(lane);
+            const bool valid = i < n;
+            const bool left = valid && (ok ? goes_left(T[j], axis, cproj) : (i < n / 2));
+            const unsigned long long mL = __ballot(left);
+            const unsigned long long mR = __ballot(valid && !left);
+            if (valid) {
+                const uint32_t dst = left ? beg + runL + uint32_t(__popcll(mL & below))
+                                          : beg + nl + runR + uint32_t(__popcll(mR & below));
+                float* o = a.pos_out + size_t(dst) * 9;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) o[t] = T[j][t];
+                a.id_out[dst] = idv[j];
+            }
+            runL += uint32_t(__popcll(mL));
+            runR += uint32_t(__popcll(mR));
+        }
     }
     if (lane == 0) {
-        const uint32_t slot = next_first + atomicAdd(a.next_count, 2u);
+        const uint32_t slot = next_first + atomicAdd(next_count, 2u);
         BNode* k0 = &a.nodes[slot];
         BNode* k1 = &a.nodes[slot + 1];
         k0->mesh = nd->mesh; k0->beg = beg;      k0->end = beg + nl; k0->level = nd->level + 1; k0->flags = 0;
@@ -359,6 +446,22 @@ __global__ __launch_bounds__(64) void k_level(LevelArgs a, uint32_t first, uint3
         nd->kid[1] = slot + 1;
         nd->flags = 0;
         atomicMax(&a.mesh_depth[nd->mesh], nd->level + 1);
+    }
+}
+
+// One level of the build: lv[0] = {first node, node count} of this level
+// (written by the previous level's launch or by the host); the launch
+// writes lv[1] = {first node, count} of the next, its count grown by two
+// per split.  A fixed grid strides over the level's nodes, so levels can be
+// launched back to back before their sizes are known on the host.
+__global__ __launch_bounds__(64) void k_level(LevelArgs a, uint2* lv) {
+    __shared__ LevelShared S;
+    const uint32_t first = lv[0].x, count = lv[0].y;
+    const uint32_t next_first = first + count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) lv[1].x = next_first;
+    for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
+        level_node(a, first + b, next_first, &lv[1].y, S);
+        wave_sync();  // the next node reuses the LDS
     }
 }
 
@@ -400,6 +503,47 @@ __global__ void k_preorder(BNode* nodes, uint32_t first, uint32_t count) {
     l.toff = nd.toff;
     r.pre = nd.pre + 1 + l.icount;
     r.toff = nd.toff + l.tcount;
+}
+
+// The three passes above in one workgroup, for builds of up to
+// FLATTEN_SMALL nodes: one launch instead of one per level and pass.
+// lv[l] = {first node, count} of level l.
+constexpr uint32_t FLATTEN_SMALL = 1u << 16;
+__global__ __launch_bounds__(1024) void k_flatten_small(BNode* nodes, uint32_t total, const uint2* lv, uint32_t nlev) {
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        BNode* nd = &nodes[i];
+        if (nd->flags & 1u) continue;
+        const uint32_t l = nd->kid[0], r = nd->kid[1];
+        if (nodes[r].surf > nodes[l].surf) {
+            nd->kid[0] = r;
+            nd->kid[1] = l;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = nlev; k-- > 0;) {
+        for (uint32_t i = threadIdx.x; i < lv[k].y; i += blockDim.x) {
+            BNode* nd = &nodes[lv[k].x + i];
+            if (nd->flags & 1u) continue;
+            const BNode& l = nodes[nd->kid[0]];
+            const BNode& r = nodes[nd->kid[1]];
+            nd->icount = 1 + l.icount + r.icount;
+            nd->tcount = l.tcount + r.tcount;
+        }
+        __syncthreads();
+    }
+    for (uint32_t k = 0; k < nlev; ++k) {
+        for (uint32_t i = threadIdx.x; i < lv[k].y; i += blockDim.x) {
+            const BNode& nd = nodes[lv[k].x + i];
+            if (nd.flags & 1u) continue;
+            BNode& l = nodes[nd.kid[0]];
+            BNode& r = nodes[nd.kid[1]];
+            l.pre = nd.pre + 1;
+            l.toff = nd.toff;
+            r.pre = nd.pre + 1 + l.icount;
+            r.toff = nd.toff + l.tcount;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_emit_nodes(const BNode* nodes, uint32_t count, const uint32_t* node_base, NodeRec* out) {
@@ -470,21 +614,35 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
     double worst = 0.0, rad = 0.0;
     const uint32_t npts = 3 * (nd.end - nd.beg);
     const float* P = pos_fin + size_t(nd.beg) * 9;
-#pragma unroll 4
-    for (uint32_t k = lane; k < npts; k += 64) {
-        const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
-        const double r[3] = {p[0] - nd.box[0], p[1] - nd.box[1], p[2] - nd.box[2]};
-        double ex;
-        if (fabs(D) > 0.0) {
-            const double al = r[0] * du[0] + r[1] * du[1] + r[2] * du[2];
-            const double be = r[0] * dv[0] + r[1] * dv[1] + r[2] * dv[2];
-            const double ga = r[0] * dw[0] + r[1] * dw[1] + r[2] * dw[2];
-            ex = fmax(0.0, fabs(al) - 1.0) * lu + fmax(0.0, fabs(be) - 1.0) * lv + fmax(0.0, fabs(ga) - 1.0) * lw;
-        } else {
-            ex = INFINITY;
+    const double c[3] = {nd.box[0], nd.box[1], nd.box[2]};
+    const bool root = nd.level == 0;
+    for (uint32_t k0 = 0; k0 < npts; k0 += 64u * GROUP) {
+        float px[GROUP], py[GROUP], pz[GROUP];
+#pragma unroll
+        for (int j = 0; j < GROUP; ++j) {
+            const uint32_t k = k0 + 64u * j + uint32_t(lane);
+            const bool in = k < npts;
+            px[j] = in ? P[3 * size_t(k)] : 0.f;
+            py[j] = in ? P[3 * size_t(k) + 1] : 0.f;
+            pz[j] = in ? P[3 * size_t(k) + 2] : 0.f;
         }
-        worst = fmax(worst, ex);
-        if (nd.level == 0) rad = fmax(rad, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);  // squared; roots only
+#pragma unroll
+        for (int j = 0; j < GROUP; ++j) {
+            if (k0 + 64u * j + uint32_t(lane) >= npts) continue;
+            const double p[3] = {px[j], py[j], pz[j]};
+            const double r[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+            double ex;
+            if (fabs(D) > 0.0) {
+                const double al = r[0] * du[0] + r[1] * du[1] + r[2] * du[2];
+                const double be = r[0] * dv[0] + r[1] * dv[1] + r[2] * dv[2];
+                const double ga = r[0] * dw[0] + r[1] * dw[1] + r[2] * dw[2];
+                ex = fmax(0.0, fabs(al) - 1.0) * lu + fmax(0.0, fabs(be) - 1.0) * lv + fmax(0.0, fabs(ga) - 1.0) * lw;
+            } else {
+                ex = INFINITY;
+            }
+            worst = fmax(worst, ex);
+            if (root) rad = fmax(rad, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);  // squared; roots only
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -515,6 +673,9 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t b) { return uint32_t((n + b - 1) / b); }
 
+constexpr uint32_t LEVEL_BATCH = 8;     // level launches per wait of the build
+constexpr uint32_t LEVEL_GRID = 65536;  // workgroups of a level launch at most (larger levels are strided)
+
 // Build every soup of the batch and append the trees to the context arenas.
 BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, uint32_t nmesh, bool tri9_on_device) {
     BuildResult res;
@@ -530,7 +691,12 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
         res.err = "batch too large (triangle or node count exceeds 32-bit indexing)";
         return res;
     }
-    DevBuf pos[2], ids[2], pos_fin, id_fin, nodes, mesh_depth, next_count, base_nodes, base_tris, slack_bits, radius_bits;
+    // work buffers, kept in the context between builds of up to BUILD_KEEP bytes
+    DevBuf* const W = ctx->build_scratch;
+    DevBuf* const pos = W;      // pos[0], pos[1]
+    DevBuf* const ids = W + 2;  // ids[0], ids[1]
+    DevBuf &pos_fin = W[4], &id_fin = W[5], &nodes = W[6], &mesh_depth = W[7], &level_tab = W[8], &base_nodes = W[9],
+           &base_tris = W[10], &slack_bits = W[11], &radius_bits = W[12], &level_list = W[13];
     std::vector<BNode> roots(nmesh);
     std::vector<std::pair<uint32_t, uint32_t>> levels;
     std::vector<uint32_t> host_ids;
@@ -550,7 +716,7 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     CK(mesh_depth.ensure(nmesh * sizeof(uint32_t)));
     CK(slack_bits.ensure(size_t(nmesh) * SLACK_REPS * sizeof(uint32_t)));
     CK(radius_bits.ensure(nmesh * sizeof(uint32_t)));
-    CK(next_count.ensure(sizeof(uint32_t)));
+    CK(level_tab.ensure((LEVEL_BATCH + 1) * sizeof(uint2)));
     CK(base_nodes.ensure(nmesh * sizeof(uint32_t)));
     CK(base_tris.ensure(nmesh * sizeof(uint32_t)));
 
@@ -576,43 +742,61 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
     CK(hipMemsetAsync(slack_bits.p, 0, size_t(nmesh) * SLACK_REPS * sizeof(uint32_t), st));
     CK(hipMemsetAsync(radius_bits.p, 0, nmesh * sizeof(uint32_t), st));
 
+    // Levels go LEVEL_BATCH launches at a time with one wait per batch: each
+    // launch takes its node range from the level table the previous one
+    // filled, over a grid sized by an upper bound (a level at most doubles).
     while (count > 0) {
-        LevelArgs a;
-        a.nodes = nodes.as<BNode>();
-        a.pos_in = pos[cur].as<float>();
-        a.id_in = ids[cur].as<uint32_t>();
-        a.pos_out = pos[cur ^ 1].as<float>();
-        a.id_out = ids[cur ^ 1].as<uint32_t>();
-        a.pos_fin = pos_fin.as<float>();
-        a.id_fin = id_fin.as<uint32_t>();
-        a.next_count = next_count.as<unsigned int>();
-        a.mesh_depth = mesh_depth.as<unsigned int>();
-        a.leaf_max = ctx->leaf_max;
-        CK(hipMemsetAsync(next_count.p, 0, sizeof(uint32_t), st));
-        const uint32_t next_first = first + count;
-        hipLaunchKernelGGL(k_level, dim3(count), dim3(64), 0, st, a, first, next_first);
-        CK(hipGetLastError());
-        uint32_t nxt = 0;
-        CK(hipMemcpyAsync(&nxt, next_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        std::vector<uint2> tab(LEVEL_BATCH + 1, make_uint2(0u, 0u));
+        tab[0] = make_uint2(first, count);
+        CK(hipMemcpyAsync(level_tab.p, tab.data(), tab.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+        uint64_t bound = count;
+        for (uint32_t j = 0; j < LEVEL_BATCH; ++j) {
+            LevelArgs a;
+            a.nodes = nodes.as<BNode>();
+            a.pos_in = pos[cur].as<float>();
+            a.id_in = ids[cur].as<uint32_t>();
+            a.pos_out = pos[cur ^ 1].as<float>();
+            a.id_out = ids[cur ^ 1].as<uint32_t>();
+            a.pos_fin = pos_fin.as<float>();
+            a.id_fin = id_fin.as<uint32_t>();
+            a.mesh_depth = mesh_depth.as<unsigned int>();
+            a.leaf_max = ctx->leaf_max;
+            const uint32_t grid = uint32_t(std::min<uint64_t>(bound, LEVEL_GRID));
+            hipLaunchKernelGGL(k_level, dim3(grid), dim3(64), 0, st, a, level_tab.as<uint2>() + j);
+            CK(hipGetLastError());
+            cur ^= 1;
+            bound = std::min<uint64_t>(2 * bound, node_cap);
+        }
+        CK(hipMemcpyAsync(tab.data(), level_tab.p, tab.size() * sizeof(uint2), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
-        levels.push_back({first, count});
-        first = next_first;
-        count = nxt;
-        cur ^= 1;
+        for (uint32_t j = 0; j < LEVEL_BATCH && tab[j].y > 0; ++j) levels.push_back({tab[j].x, tab[j].y});
+        first = tab[LEVEL_BATCH].x;
+        count = tab[LEVEL_BATCH].y;
     }
     {
         const uint32_t total = first;
-        hipLaunchKernelGGL(k_order_kids, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total);
-        CK(hipGetLastError());
-        for (size_t l = levels.size(); l-- > 0;) {
-            hipLaunchKernelGGL(k_counts, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st, nodes.as<BNode>(),
-                               levels[l].first, levels[l].second);
+        std::vector<uint2> lv;  // read by an asynchronous copy until the wait below
+        if (total <= FLATTEN_SMALL) {
+            lv.resize(levels.size());
+            for (size_t l = 0; l < levels.size(); ++l) lv[l] = make_uint2(levels[l].first, levels[l].second);
+            CK(level_list.ensure(std::max<size_t>(lv.size(), 1) * sizeof(uint2)));
+            CK(hipMemcpyAsync(level_list.p, lv.data(), lv.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_flatten_small, dim3(1), dim3(1024), 0, st, nodes.as<BNode>(), total,
+                               level_list.as<uint2>(), uint32_t(lv.size()));
             CK(hipGetLastError());
-        }
-        for (size_t l = 0; l < levels.size(); ++l) {
-            hipLaunchKernelGGL(k_preorder, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st, nodes.as<BNode>(),
-                               levels[l].first, levels[l].second);
+        } else {
+            hipLaunchKernelGGL(k_order_kids, dim3(blocks_for(total, 256)), dim3(256), 0, st, nodes.as<BNode>(), total);
             CK(hipGetLastError());
+            for (size_t l = levels.size(); l-- > 0;) {
+                hipLaunchKernelGGL(k_counts, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st,
+                                   nodes.as<BNode>(), levels[l].first, levels[l].second);
+                CK(hipGetLastError());
+            }
+            for (size_t l = 0; l < levels.size(); ++l) {
+                hipLaunchKernelGGL(k_preorder, dim3(blocks_for(levels[l].second, 256)), dim3(256), 0, st,
+                                   nodes.as<BNode>(), levels[l].first, levels[l].second);
+                CK(hipGetLastError());
+            }
         }
         CK(hipMemcpyAsync(roots.data(), nodes.p, nmesh * sizeof(BNode), hipMemcpyDeviceToHost, st));
         CK(hipMemcpyAsync(depth.data(), mesh_depth.p, nmesh * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -687,10 +871,12 @@ BuildResult build_soups(cd_ctx* ctx, const float* tri9, const uint32_t* counts, 
         ctx->nodes_used += total_internal;
         ctx->tris_used += T;
     }
-fail:
-    for (auto* b : {&pos[0], &pos[1], &ids[0], &ids[1], &pos_fin, &id_fin, &nodes, &mesh_depth, &next_count, &base_nodes,
-                    &base_tris, &slack_bits, &radius_bits})
-        b->release();
+fail: {
+        size_t kept = 0;
+        for (uint32_t k = 0; k < BUILD_SCRATCH; ++k) kept += W[k].bytes;
+        if (!res.err.empty() || kept > BUILD_KEEP)
+            for (uint32_t k = 0; k < BUILD_SCRATCH; ++k) W[k].release();
+    }
     return res;
 }
 

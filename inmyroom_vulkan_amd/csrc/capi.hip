@@ -772,6 +772,7 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_ents.release();
     ctx->d_rayslab.release();
     ctx->d_rootslab.release();
+    for (auto& b : ctx->build_scratch) b.release();
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
     if (ctx->frame_graph) (void)hipGraphExecDestroy(ctx->frame_graph);
@@ -1003,37 +1004,50 @@ int cd_register_gltf(cd_ctx* ctx, const cd_gltf* doc, uint32_t* mesh_ids, uint32
     int rc = upload_mesh_table(ctx);
     if (!rc) rc = upload_attrs(ctx);
     if (rc) return rc;
-    // vertex ids in tree order, then the vertex normals of meshes that have them
+    // vertex ids in tree order (one launch for the document), then the
+    // vertex normals of the meshes that have them (one upload)
     const size_t nvid = size_t(T) * 3;
     if (ctx->vid_used + nvid >= 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "vertex-id arena exceeds 32-bit indexing");
     HCK(ctx, grow_keep(ctx->d_vid, (ctx->vid_used + nvid + 1) * sizeof(uint32_t), st));
-    uint64_t tri_at = 0;
+    std::vector<uint32_t> prefix(reg.size() + 1, 0u);
+    size_t nnrm = 0;
     for (size_t i = 0; i < reg.size(); ++i) {
         const MeshRec& mr = ctx->meshes[first + i];
-        HCK(ctx, launch_tree_vidx(ctx->d_perm.as<uint32_t>() + mr.tri_base, mr.ntri, d_vid3.as<uint32_t>() + tri_at * 3,
-                                  0u, ctx->d_vid.as<uint32_t>() + ctx->vid_used, st));
+        if (mr.tri_base != ctx->meshes[first].tri_base + prefix[i])
+            return fail(ctx, CD_E_HIP, "tree build placed the document's meshes apart");
+        prefix[i + 1] = prefix[i] + mr.ntri;
+        if (meshes[reg[i]].with_normals) nnrm += size_t(meshes[reg[i]].nvert) * 3;
+    }
+    ScopedBuf d_prefix;
+    HCK(ctx, d_prefix.ensure(prefix.size() * sizeof(uint32_t)));
+    HCK(ctx, hipMemcpyAsync(d_prefix.p, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HCK(ctx, launch_tree_vidx_many(ctx->d_perm.as<uint32_t>() + ctx->meshes[first].tri_base, d_prefix.as<uint32_t>(),
+                                   uint32_t(reg.size()), T, d_vid3.as<uint32_t>(),
+                                   ctx->d_vid.as<uint32_t>() + ctx->vid_used, st));
+    if (ctx->nrm_used + nnrm >= 0xFFFFFFF0ull) return fail(ctx, CD_E_CAPACITY, "normal arena exceeds 32-bit indexing");
+    std::vector<float> nrm;
+    nrm.reserve(nnrm);
+    for (size_t i = 0; i < reg.size(); ++i) {
+        const MeshRec& mr = ctx->meshes[first + i];
         const auto& mesh = meshes[reg[i]];
         uint2 attr = make_uint2(uint32_t(ctx->vid_used), ~0u);
         ctx->vid_used += size_t(mr.ntri) * 3;
-        tri_at += mr.ntri;
         if (mesh.with_normals) {
-            const size_t n = size_t(mesh.nvert) * 3;
-            if (ctx->nrm_used + n >= 0xFFFFFFF0ull)
-                return fail(ctx, CD_E_CAPACITY, "normal arena exceeds 32-bit indexing");
-            HCK(ctx, grow_keep(ctx->d_nrm, (ctx->nrm_used + n + 1) * sizeof(float), st));
-            size_t at = ctx->nrm_used;
-            for (const auto& p : mesh.prims) {
-                if (p.excluded) continue;
-                HCK(ctx, hipMemcpyAsync(ctx->d_nrm.as<float>() + at, p.nrm.data(), p.nrm.size() * sizeof(float),
-                                        hipMemcpyHostToDevice, st));
-                at += p.nrm.size();
-            }
-            attr.y = uint32_t(ctx->nrm_used);
-            ctx->nrm_used += n;
+            attr.y = uint32_t(ctx->nrm_used + nrm.size());
+            const size_t at = nrm.size();
+            for (const auto& p : mesh.prims)
+                if (!p.excluded) nrm.insert(nrm.end(), p.nrm.begin(), p.nrm.end());
+            nrm.resize(at + size_t(mesh.nvert) * 3, 0.f);
         }
         ctx->attrs[first + i] = attr;
         ctx->mesh_nvert[first + i] = uint32_t(mesh.nvert);
         mesh_ids[reg[i]] = first + uint32_t(i);
+    }
+    if (!nrm.empty()) {
+        HCK(ctx, grow_keep(ctx->d_nrm, (ctx->nrm_used + nrm.size() + 1) * sizeof(float), st));
+        HCK(ctx, hipMemcpyAsync(ctx->d_nrm.as<float>() + ctx->nrm_used, nrm.data(), nrm.size() * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+        ctx->nrm_used += nrm.size();
     }
     HCK(ctx, hipStreamSynchronize(st));
     return upload_attrs(ctx);

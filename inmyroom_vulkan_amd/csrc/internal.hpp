@@ -245,6 +245,9 @@ struct RunCounters {
 };
 
 // Build of a batch of triangle soups into the arenas (build.hip).
+constexpr uint32_t BUILD_SCRATCH = 14;
+constexpr size_t BUILD_KEEP = size_t(256) << 20;
+
 struct BuildResult {
     std::vector<MeshRec> meshes;   // node_base / tri_base relative to the arenas
     std::string err;
@@ -321,6 +324,9 @@ struct cd_ctx {
     // refit
     cdk::DevBuf d_rayslab, d_rootslab;
     std::vector<uint8_t> slab_ok;
+    // the tree build's work buffers, kept between builds that need at most
+    // BUILD_KEEP bytes of them (a model load builds one document at a time)
+    cdk::DevBuf build_scratch[cdk::BUILD_SCRATCH];
     uint32_t step_np_cap = 0, step_nc_cap = 0, step_ray_cap = 0;
     bool step_radix = false;  // a pair had more contacts than the per-pair order ranks: the radix sorts
     hipGraphExec_t step_graph = nullptr, frame_graph = nullptr;  // the whole step; cd_frame's part of it

@@ -193,6 +193,8 @@ hipError_t launch_pack(const PackSpans& sp, uint32_t* dst, hipStream_t st);
 // refit.hip
 hipError_t launch_tree_vidx(const uint32_t* perm, uint32_t ntri, const uint32_t* indices, uint32_t vbase,
                             uint32_t* out, hipStream_t st);
+hipError_t launch_tree_vidx_many(const uint32_t* perm, const uint32_t* prefix, uint32_t nmesh, uint64_t ntri,
+                                 const uint32_t* indices, uint32_t* out, hipStream_t st);
 hipError_t launch_skin(const SkinRec* sk, uint32_t nsk, const uint32_t* vstart, const float* bind, const ushort4* joints,
                        const float4* weights, const float* bones, uint32_t total, float* out, hipStream_t st);
 hipError_t launch_gather_tris(const SkinRec* sk, uint32_t nsk, const uint32_t* tstart, const uint32_t* vidx,

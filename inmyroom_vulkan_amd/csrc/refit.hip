@@ -477,12 +477,34 @@ __global__ void k_tree_vidx(const uint32_t* __restrict__ perm, uint32_t ntri, co
     for (int k = 0; k < 3; ++k) out[size_t(s) * 3 + k] = vbase + indices[size_t(t) * 3 + k];
 }
 
+// k_tree_vidx for every mesh of one build at once: the meshes' slots are
+// back to back, prefix[m] is mesh m's first slot (prefix[nmesh] = all), and
+// perm and the per-mesh index lists are mesh-local.
+__global__ void k_tree_vidx_many(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ prefix,
+                                 uint32_t nmesh, const uint32_t* __restrict__ indices, uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= prefix[nmesh]) return;
+    uint32_t lo = 0, hi = nmesh;  // prefix[lo] <= s < prefix[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (prefix[mid] <= s) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t t = prefix[lo] + perm[s];
+    for (int k = 0; k < 3; ++k) out[size_t(s) * 3 + k] = indices[size_t(t) * 3 + k];
+}
+
 // ---- launchers ---------------------------------------------------------------
 static inline dim3 g256r(uint64_t n) { return dim3(uint32_t((std::max<uint64_t>(n, 1) + 255) / 256)); }
 
 hipError_t launch_tree_vidx(const uint32_t* perm, uint32_t ntri, const uint32_t* indices, uint32_t vbase,
                             uint32_t* out, hipStream_t st) {
     hipLaunchKernelGGL(k_tree_vidx, g256r(ntri), dim3(256), 0, st, perm, ntri, indices, vbase, out);
+    return hipGetLastError();
+}
+hipError_t launch_tree_vidx_many(const uint32_t* perm, const uint32_t* prefix, uint32_t nmesh, uint64_t ntri,
+                                 const uint32_t* indices, uint32_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_tree_vidx_many, g256r(ntri), dim3(256), 0, st, perm, prefix, nmesh, indices, out);
     return hipGetLastError();
 }
 hipError_t launch_skin(const SkinRec* sk, uint32_t nsk, const uint32_t* vstart, const float* bind, const ushort4* joints,

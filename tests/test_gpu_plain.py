@@ -87,6 +87,22 @@ def test_tree_build_hand_derived_splits(gpu_l1, orc):
             assert got["obbs"].tobytes() == want["obbs"].tobytes(), name
 
 
+def test_tree_build_levels_wider_than_one_launch(gpu_l1, orc):
+    """A mesh whose deepest levels hold more nodes than one level launch has
+    workgroups (65,536), so the launch strides over them, and deeper than one
+    batch of level launches: bit-identical to the oracle's tree."""
+    rng = np.random.default_rng(5)
+    tri = scenes.displaced_icosphere(128, 0.1, rng)
+    case(f"tree build, one mesh of {len(tri)} triangles, leaf size 1")
+    got = gpu_l1.export_tree(gpu_l1.register_soups([tri])[0])
+    want = orc.tree(tri, 1).export()
+    assert len(got["links"]) > 2 * 65536
+    assert got["root"].tobytes() == want["root"].tobytes()
+    assert got["obbs"].tobytes() == want["obbs"].tobytes()
+    assert np.array_equal(got["links"], want["links"])
+    assert np.array_equal(got["perm"], want["perm"])
+
+
 @pytest.mark.parametrize("L", [4, 1])
 def test_plain_queries_config1(gpu, gpu_l1, orc, L):
     ctx = gpu if L == 4 else gpu_l1
