@@ -199,15 +199,18 @@ __device__ inline bool goes_left(const float (&T)[9], f3 axis, float cproj) {
 // The node scans below take GROUP wave-wide chunks per step and issue the
 // loads of all of them before using any: a node's scans are latency bound
 // (one wave per node), and a chunk at a time waits on memory once per chunk.
+// Indices past the end are clamped to the last element rather than guarded:
+// a guarded load is moved behind the guard of its use, back to one wait per
+// chunk.
 constexpr int GROUP = 8;
 
 __device__ __forceinline__ void load_tris(const float* __restrict__ P, uint32_t c0, uint32_t n, int lane,
                                           float (&T)[GROUP][9]) {
 #pragma unroll
     for (int j = 0; j < GROUP; ++j) {
-        const uint32_t i = c0 + 64u * j + uint32_t(lane);
+        const size_t i = min(c0 + 64u * j + uint32_t(lane), n - 1);
 #pragma unroll
-        for (int t = 0; t < 9; ++t) T[j][t] = i < n ? P[size_t(i) * 9 + t] : 0.f;
+        for (int t = 0; t < 9; ++t) T[j][t] = P[i * 9 + t];
     }
 }
 
@@ -305,11 +308,10 @@ __device__ __forceinline__ void level_node(const LevelArgs& a, uint32_t ni, uint
             float px[GROUP], py[GROUP], pz[GROUP];
 #pragma unroll
             for (int j = 0; j < GROUP; ++j) {
-                const uint32_t k = k0 + 64u * j + uint32_t(lane);
-                const bool in = k < npts;
-                px[j] = in ? P[3 * size_t(k)] : 0.f;
-                py[j] = in ? P[3 * size_t(k) + 1] : 0.f;
-                pz[j] = in ? P[3 * size_t(k) + 2] : 0.f;
+                const size_t k = min(k0 + 64u * j + uint32_t(lane), npts - 1);
+                px[j] = P[3 * k];
+                py[j] = P[3 * k + 1];
+                pz[j] = P[3 * k + 2];
             }
 #pragma unroll
             for (int j = 0; j < GROUP; ++j) {
@@ -415,7 +417,7 @@ __device__ __forceinline__ void level_node(const LevelArgs& a, uint32_t ni, uint
 #pragma unroll
         for (int j = 0; j < GROUP; ++j) {
             const uint32_t i = c0 + 64u * j + uint32_t(lane);
-            idv[j] = i < n ? a.id_in[beg + i] : 0u;
+            idv[j] = a.id_in[beg + min(i, n - 1)];
         }
 #pragma unroll
         for (int j = 0; j < GROUP; ++j) {
@@ -620,11 +622,10 @@ __global__ __launch_bounds__(64) void k_slack(const BNode* nodes, uint32_t count
         float px[GROUP], py[GROUP], pz[GROUP];
 #pragma unroll
         for (int j = 0; j < GROUP; ++j) {
-            const uint32_t k = k0 + 64u * j + uint32_t(lane);
-            const bool in = k < npts;
-            px[j] = in ? P[3 * size_t(k)] : 0.f;
-            py[j] = in ? P[3 * size_t(k) + 1] : 0.f;
-            pz[j] = in ? P[3 * size_t(k) + 2] : 0.f;
+            const size_t k = min(k0 + 64u * j + uint32_t(lane), npts - 1);
+            px[j] = P[3 * k];
+            py[j] = P[3 * k + 1];
+            pz[j] = P[3 * k + 2];
         }
 #pragma unroll
         for (int j = 0; j < GROUP; ++j) {

@@ -245,6 +245,31 @@ __global__ void k_item_ranges(const uint2* __restrict__ items, uint32_t nitems, 
     ranges[it] = make_uint2(off, cnt);
 }
 
+// Visits points first, first + step, ... below npts in that order, with the
+// loads of RF_GROUP of them issued before any is used: the refit's loops
+// walk a box's points one per step, and one point at a time waits on memory
+// once per point.
+constexpr int RF_GROUP = 8;
+template <class F>
+__device__ __forceinline__ void for_points(const float* __restrict__ P, uint32_t first, uint32_t npts, uint32_t step,
+                                           F f) {
+    for (uint32_t k0 = first; k0 < npts; k0 += step * RF_GROUP) {
+        float x[RF_GROUP], y[RF_GROUP], z[RF_GROUP];
+#pragma unroll
+        for (int j = 0; j < RF_GROUP; ++j) {
+            // clamped rather than guarded: a guarded load is moved behind
+            // the guard of its use, back to one wait per point
+            const size_t k = min(k0 + step * uint32_t(j), npts - 1);
+            x[j] = P[3 * k];
+            y[j] = P[3 * k + 1];
+            z[j] = P[3 * k + 2];
+        }
+#pragma unroll
+        for (int j = 0; j < RF_GROUP; ++j)
+            if (k0 + step * uint32_t(j) < npts) f(x[j], y[j], z[j]);
+    }
+}
+
 // A small box, whole, in one thread.
 __global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ items, const uint32_t* __restrict__ list,
                                                   uint32_t n, const uint2* __restrict__ ranges,
@@ -273,29 +298,29 @@ __global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ item
         } else {
             const double N = double(npts);
             double sx = 0, sy = 0, sz = 0;
-            for (uint32_t k = 0; k < npts; ++k) {
-                sx += double(P[3 * k]);
-                sy += double(P[3 * k + 1]);
-                sz += double(P[3 * k + 2]);
-            }
+            for_points(P, 0, npts, 1, [&](float x, float y, float z) {
+                sx += double(x);
+                sy += double(y);
+                sz += double(z);
+            });
             const double mx = sx / N, my = sy / N, mz = sz / N;
             double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
-            for (uint32_t k = 0; k < npts; ++k) {
-                const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+            for_points(P, 0, npts, 1, [&](float x, float y, float z) {
+                const double dx = double(x) - mx, dy = double(y) - my, dz = double(z) - mz;
                 cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
                 cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
-            }
+            });
             double ax[9];
             axes_of(cxx / N, cyy / N, czz / N, cxy / N, cxz / N, cyz / N, ax);
             double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (uint32_t k = 0; k < npts; ++k) {
-                const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+            for_points(P, 0, npts, 1, [&](float x, float y, float z) {
+                const double px = x, py = y, pz = z;
                 for (int q = 0; q < 3; ++q) {
                     const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
                     lo[q] = fmin(lo[q], pr);
                     hi[q] = fmax(hi[q], pr);
                 }
-            }
+            });
             box = box_from_extents(ax, lo, hi);
         }
         store_box(m, nodes, item.y, box);
@@ -303,11 +328,11 @@ __global__ __launch_bounds__(256) void k_rf_small(const uint2* __restrict__ item
         box_to12(box, b12);
         const SlackFrame f = slack_frame(b12);
         double rad2 = 0.0;
-        for (uint32_t k = 0; k < npts; ++k) {
-            const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+        for_points(P, 0, npts, 1, [&](float x, float y, float z) {
+            const double p[3] = {x, y, z};
             worst = fmax(worst, excess(f, p));
             rad2 = fmax(rad2, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-        }
+        });
         rad = sqrt(rad2);  // sqrt is monotone: the root of the largest square is the largest root
     }
     const uint32_t mesh0 = __shfl(item.x, 0, 64);
@@ -346,12 +371,13 @@ __global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ item
         const float* P0 = item_tris(m, tris, ranges[ch.x].x);  // the box's first point
         bool differs = false;
         double sx = 0, sy = 0, sz = 0;
-        for (uint32_t k = lane; k < npts; k += 64) {
-            differs |= !(P[3 * k] == P0[0] && P[3 * k + 1] == P0[1] && P[3 * k + 2] == P0[2]);
-            sx += double(P[3 * k]);
-            sy += double(P[3 * k + 1]);
-            sz += double(P[3 * k + 2]);
-        }
+        const float x0 = P0[0], y0 = P0[1], z0 = P0[2];
+        for_points(P, uint32_t(lane), npts, 64, [&](float x, float y, float z) {
+            differs |= !(x == x0 && y == y0 && z == z0);
+            sx += double(x);
+            sy += double(y);
+            sz += double(z);
+        });
         const bool any = __ballot(differs) != 0ull;
         sx = wave_sum_d(sx); sy = wave_sum_d(sy); sz = wave_sum_d(sz);
         if (lane == 0) {
@@ -362,11 +388,11 @@ __global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ item
         if (s[RS_KIND] != 2.0) return;
         const double mx = s[RS_MEAN], my = s[RS_MEAN + 1], mz = s[RS_MEAN + 2];
         double cxx = 0, cyy = 0, czz = 0, cxy = 0, cxz = 0, cyz = 0;
-        for (uint32_t k = lane; k < npts; k += 64) {
-            const double dx = double(P[3 * k]) - mx, dy = double(P[3 * k + 1]) - my, dz = double(P[3 * k + 2]) - mz;
+        for_points(P, uint32_t(lane), npts, 64, [&](float x, float y, float z) {
+            const double dx = double(x) - mx, dy = double(y) - my, dz = double(z) - mz;
             cxx += dx * dx; cyy += dy * dy; czz += dz * dz;
             cxy += dx * dy; cxz += dx * dz; cyz += dy * dz;
-        }
+        });
         cxx = wave_sum_d(cxx); cyy = wave_sum_d(cyy); czz = wave_sum_d(czz);
         cxy = wave_sum_d(cxy); cxz = wave_sum_d(cxz); cyz = wave_sum_d(cyz);
         if (lane == 0) {
@@ -377,14 +403,14 @@ __global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ item
         double ax[9];
         for (int t = 0; t < 9; ++t) ax[t] = s[RS_AX + t];
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t k = lane; k < npts; k += 64) {
-            const double px = P[3 * k], py = P[3 * k + 1], pz = P[3 * k + 2];
+        for_points(P, uint32_t(lane), npts, 64, [&](float x, float y, float z) {
+            const double px = x, py = y, pz = z;
             for (int q = 0; q < 3; ++q) {
                 const double pr = (ax[3 * q] * px + ax[3 * q + 1] * py) + ax[3 * q + 2] * pz;
                 lo[q] = fmin(lo[q], pr);
                 hi[q] = fmax(hi[q], pr);
             }
-        }
+        });
         for (int q = 0; q < 3; ++q) {
             lo[q] = wave_min_d(lo[q]);
             hi[q] = wave_max_d(hi[q]);
@@ -398,11 +424,11 @@ __global__ __launch_bounds__(256) void k_rf_chunk(const uint2* __restrict__ item
         const SlackFrame f = slack_frame(box_ptr(m, nodes, item.y));
         const bool root = item.y == ROOT_CODE;  // only the root box's radius is kept
         double worst = 0.0, rad2 = 0.0;
-        for (uint32_t k = lane; k < npts; k += 64) {
-            const double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+        for_points(P, uint32_t(lane), npts, 64, [&](float x, float y, float z) {
+            const double p[3] = {x, y, z};
             worst = fmax(worst, excess(f, p));
             if (root) rad2 = fmax(rad2, p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-        }
+        });
         worst = wave_max_d(worst);
         rad2 = wave_max_d(rad2);  // sqrt is monotone: the root of the largest square is the largest root
         if (lane == 0) post_slack(slack_bits, radius_bits, item.x, root, worst, sqrt(rad2), ci);
