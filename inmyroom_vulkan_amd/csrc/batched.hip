@@ -1,8 +1,8 @@
 // Batched path: thousands of uneven mesh-pair queries in a few launches.
 //
-//  k_setup        per query: relative matrix and the slack that makes the
-//                 float32 separating-axis test conservative
-//  k_seed         one root node pair per query
+//  k_start        per query: relative matrix and the slack that makes the
+//                 float32 separating-axis test conservative, and one root
+//                 node pair in the first frontier; the run's counters cleared
 //  k_split_level  breadth-first expansion of node pairs for the first
 //                 levels (work splitting: a large query becomes many tasks)
 //  k_dfs          persistent wavefronts; each pops up to 64 node pairs at a
@@ -305,11 +305,9 @@ __device__ __forceinline__ int eval_pair(const QueryPre* __restrict__ qp, const 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-__global__ void k_setup(const QueryRec* __restrict__ qs, const unsigned int* __restrict__ nq_p,
-                        const MeshRec* __restrict__ meshes, QueryPre* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nq_p) return;
-    const QueryRec q = qs[i];
+// A query's record for the traversal: its matrix, the conservative test's
+// bound and the two meshes' arena offsets.
+__device__ __forceinline__ QueryPre query_pre(const QueryRec& q, const MeshRec* __restrict__ meshes) {
     const MeshRec& A = meshes[q.mesh_a];
     const MeshRec& B = meshes[q.mesh_b];
     QueryPre p;
@@ -339,15 +337,23 @@ __global__ void k_setup(const QueryRec* __restrict__ qs, const unsigned int* __r
     p.tri_base_a = A.tri_base;
     p.tri_base_b = B.tri_base;
     p.pad = 0;
-    out[i] = p;
+    return p;
 }
 
-__global__ void k_seed(const unsigned int* __restrict__ nq_p, const QueryPre* __restrict__ qp,
-                       uint3* __restrict__ frontier, unsigned int* __restrict__ count) {
+// A batched run's first launch: every query's record and its pair of roots
+// in the first frontier (counted in small[0]), the rest of the run's small
+// words and its counters cleared.
+__global__ void k_start(const QueryRec* __restrict__ qs, const unsigned int* __restrict__ nq_p,
+                        const MeshRec* __restrict__ meshes, QueryPre* __restrict__ qp, uint3* __restrict__ frontier,
+                        unsigned int* __restrict__ small, uint32_t nsmall, RunCounters* __restrict__ ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nq = *nq_p;
-    if (i < nq) frontier[i] = make_uint3(i, ROOT_FLAG | qp[i].mesh_a, ROOT_FLAG | qp[i].mesh_b);
-    if (i == 0) *count = nq;
+    if (i < nsmall) small[i] = i == 0 ? nq : 0u;
+    if (i == 0) *ctr = RunCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i >= nq) return;
+    const QueryPre p = query_pre(qs[i], meshes);
+    qp[i] = p;
+    frontier[i] = make_uint3(i, ROOT_FLAG | p.mesh_a, ROOT_FLAG | p.mesh_b);
 }
 
 template <bool REF>
@@ -1600,15 +1606,12 @@ __global__ void k_keep_budget(const RunCounters* __restrict__ ctr, unsigned long
 // ---- host launchers -------------------------------------------------------
 // nq_cap sizes the grids; the kernels read the query count from nq_p (<= nq_cap),
 // so one captured graph serves every count up to its capacity
-hipError_t launch_setup(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
-                        QueryPre* out, hipStream_t st) {
-    if (!nq_cap) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3((nq_cap + 127) / 128), dim3(128), 0, st, qs, nq_p, meshes, out);
-    return hipGetLastError();
-}
-hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre* qp, uint3* frontier,
-                       unsigned int* count, hipStream_t st) {
-    hipLaunchKernelGGL(k_seed, dim3((std::max(nq_cap, 1u) + 255) / 256), dim3(256), 0, st, nq_p, qp, frontier, count);
+hipError_t launch_start(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
+                        QueryPre* qp, uint3* frontier, unsigned int* small, uint32_t nsmall, RunCounters* ctr,
+                        hipStream_t st) {
+    const uint32_t n = std::max(std::max(nq_cap, nsmall), 1u);
+    hipLaunchKernelGGL(k_start, dim3((n + 127) / 128), dim3(128), 0, st, qs, nq_p, meshes, qp, frontier, small, nsmall,
+                       ctr);
     return hipGetLastError();
 }
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,

@@ -256,14 +256,11 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bo
     const NodeRec* dn = ctx->d_nodes.as<NodeRec>();
     RunCounters* ctr = dev_counters(ctx);
     const uint32_t grid = uint32_t(ctx->num_cu) * 8;
-    HCK(ctx, launch_zero(ctr, st));
-    HCK(ctx, launch_fill_u32(small, SMALL_WORDS, 0u, st));
     if (t.work_sharing == 1)  // ready words of the donation queue start clear
         HCK(ctx, launch_fill_u32(reinterpret_cast<unsigned int*>(dq), 4 * DQ_CAP, 0u, st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[0], st));
-    const unsigned int* dnq = ctx->d_nq.as<unsigned int>();
-    HCK(ctx, launch_setup(ctx->d_queries.as<QueryRec>(), nq, dnq, dm, qp, st));
-    HCK(ctx, launch_seed(nq, dnq, qp, F[0], &small[0], st));
+    HCK(ctx, launch_start(ctx->d_queries.as<QueryRec>(), nq, ctx->d_nq.as<unsigned int>(), dm, qp, F[0], small,
+                          SMALL_WORDS, ctr, st));
     uint3* T = ctx->d_work[W_TASKS].as<uint3>();
     const uint32_t cap32 = uint32_t(std::min<uint64_t>(fcap, 0xFFFFFFF0ull));
     // level l holds at most nq * 2^l pairs: no more workgroups than that needs

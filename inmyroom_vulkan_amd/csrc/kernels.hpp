@@ -31,10 +31,11 @@ hipError_t launch_narrow_plain(const uint4* lps, uint64_t nlp, uint32_t leaf_max
                                RunCounters* ctr, hipStream_t st);
 
 // batched.hip
-hipError_t launch_setup(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
-                        QueryPre* out, hipStream_t st);
-hipError_t launch_seed(uint32_t nq_cap, const unsigned int* nq_p, const QueryPre* qp, uint3* frontier,
-                       unsigned int* count, hipStream_t st);
+// The run's first launch: query records, the first frontier (count in
+// small[0]), small[1 .. nsmall) and the counters cleared.
+hipError_t launch_start(const QueryRec* qs, uint32_t nq_cap, const unsigned int* nq_p, const MeshRec* meshes,
+                        QueryPre* qp, uint3* frontier, unsigned int* small, uint32_t nsmall, RunCounters* ctr,
+                        hipStream_t st);
 // ref (these three launchers): the reference's box test and descent measure
 // instead of the conservative ones (reference-semantics mode).
 hipError_t launch_split_level(const uint3* in, const unsigned int* nin, uint3* out, unsigned int* nout, uint32_t out_cap,
