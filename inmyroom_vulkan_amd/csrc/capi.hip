@@ -114,9 +114,13 @@ static_assert(W_CSEG < sizeof(cd_ctx::d_work) / sizeof(DevBuf), "work slots");
 
 RunCounters* dev_counters(cd_ctx* ctx) { return ctx->d_counters.as<RunCounters>(); }
 
+// Through a pinned buffer: a copy into pageable memory is staged by the
+// runtime and takes longer to complete.
 int read_counters(cd_ctx* ctx, RunCounters& h) {
-    HCK(ctx, hipMemcpyAsync(&h, ctx->d_counters.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HCK(ctx, ctx->h_ctr.ensure(sizeof h));
+    HCK(ctx, hipMemcpyAsync(ctx->h_ctr.p, ctx->d_counters.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     HCK(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(&h, ctx->h_ctr.p, sizeof h);
     return CD_OK;
 }
 
@@ -767,6 +771,7 @@ int cd_destroy(cd_ctx* ctx) {
     ctx->h_pairs.release();
     ctx->h_step.release();
     ctx->h_ents.release();
+    ctx->h_ctr.release();
     ctx->d_rayslab.release();
     ctx->d_rootslab.release();
     for (auto& b : ctx->build_scratch) b.release();

@@ -332,5 +332,6 @@ struct cd_ctx {
     hipGraphExec_t step_graph = nullptr, frame_graph = nullptr;  // the whole step; cd_frame's part of it
     std::vector<uint64_t> step_key, frame_key;
     cdk::HostBuf h_step, h_ents;  // the run's result block; its entity records, staged for the graph's upload
+    cdk::HostBuf h_ctr;           // a run's counters, read back
     hipEvent_t ev[8] = {};
 };
