@@ -1673,23 +1673,45 @@ hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget
     hipLaunchKernelGGL(k_keep_budget, dim3(1), dim3(64), 0, st, ctr, budget);
     return hipGetLastError();
 }
+// Workgroups of 256 threads of `kernel` that one CU keeps resident (its
+// registers and LDS decide).  The narrow phase's grid-stride kernels are
+// launched with exactly that many per CU: with more, the extra ones wait
+// for a second round that starts as the first ones end, and the phase
+// lasts about a third longer.
+template <class K>
+static uint32_t resident_per_cu(K kernel) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    return uint32_t(nb);
+}
+struct NarrowShapes {
+    uint32_t exact, contacts, ref;
+};
+static const NarrowShapes& narrow_shapes() {
+    static const NarrowShapes s = {resident_per_cu(k_narrow_exact), resident_per_cu(k_contacts),
+                                   resident_per_cu(k_narrow_ref)};
+    return s;
+}
+void warm_launch_shapes() { (void)narrow_shapes(); }
+
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, uint4* hits, cd_contact* out, uint64_t cap, uint4* defer,
-                               uint64_t defer_cap, RunCounters* ctr, uint32_t grid, uint32_t rerun_grid,
-                               hipStream_t st) {
-    hipLaunchKernelGGL(k_narrow_exact, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, hits, cap, defer,
-                       defer_cap, ctr);
-    hipLaunchKernelGGL(k_narrow_rerun<1>, dim3(rerun_grid), dim3(256), 0, st, defer, &ctr->deferred, defer_cap, qp,
-                       tris, hits, cap, defer + defer_cap, &ctr->deferred2, ctr);
-    hipLaunchKernelGGL(k_narrow_rerun<2>, dim3(rerun_grid), dim3(256), 0, st, defer + defer_cap, &ctr->deferred2,
-                       defer_cap, qp, tris, hits, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
-    hipLaunchKernelGGL(k_contacts, dim3(grid), dim3(256), 0, st, hits, &ctr->contacts, cap, qp, tris, out);
+                               uint64_t defer_cap, RunCounters* ctr, uint32_t cus, hipStream_t st) {
+    const NarrowShapes& sh = narrow_shapes();
+    hipLaunchKernelGGL(k_narrow_exact, dim3(cus * sh.exact), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, hits, cap,
+                       defer, defer_cap, ctr);
+    hipLaunchKernelGGL(k_narrow_rerun<1>, dim3(cus), dim3(256), 0, st, defer, &ctr->deferred, defer_cap, qp, tris, hits,
+                       cap, defer + defer_cap, &ctr->deferred2, ctr);
+    hipLaunchKernelGGL(k_narrow_rerun<2>, dim3(cus), dim3(256), 0, st, defer + defer_cap, &ctr->deferred2, defer_cap,
+                       qp, tris, hits, cap, (uint4*)nullptr, (unsigned long long*)nullptr, ctr);
+    hipLaunchKernelGGL(k_contacts, dim3(cus * sh.contacts), dim3(256), 0, st, hits, &ctr->contacts, cap, qp, tris, out);
     return hipGetLastError();
 }
 hipError_t launch_narrow_ref(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
-                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t cus,
                              hipStream_t st) {
-    hipLaunchKernelGGL(k_narrow_ref, dim3(grid), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris, out, cap, ctr);
+    hipLaunchKernelGGL(k_narrow_ref, dim3(cus * narrow_shapes().ref), dim3(256), 0, st, lps, nlp, lp_cap, qp, tris,
+                       out, cap, ctr);
     return hipGetLastError();
 }
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,

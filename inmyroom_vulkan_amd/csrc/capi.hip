@@ -323,12 +323,12 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bo
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[2], st));
     if (ref)
         HCK(ctx, launch_narrow_ref(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp, ctx->d_tris.as<float>(),
-                                   ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, grid, st));
+                                   ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctr, uint32_t(ctx->num_cu), st));
     else
         HCK(ctx, launch_narrow_exact(ctx->d_leafpairs.as<uint4>(), &ctr->lpslots, ctx->lp_cap, qp,
                                      ctx->d_tris.as<float>(), ctx->d_work[W_HITS].as<uint4>(),
                                      ctx->d_contacts.as<cd_contact>(), ctx->contact_cap, ctx->d_work[W_DEFER].as<uint4>(),
-                                     ctx->defer_cap, ctr, grid, uint32_t(ctx->num_cu), st));
+                                     ctx->defer_cap, ctr, uint32_t(ctx->num_cu), st));
     if (timed) HCK(ctx, hipEventRecord(ctx->ev[3], st));
     return CD_OK;
 }
@@ -337,6 +337,7 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bo
 // sizing of the leaf-pair list); *fcap the frontier capacity they hold.
 int ensure_batched(cd_ctx* ctx, uint32_t nq, uint32_t nq_cap, const cd_tuning& t, uint64_t* fcap) {
     const uint32_t nwaves = uint32_t(ctx->num_cu) * t.waves_per_cu;
+    warm_launch_shapes();
     if (ctx->lp_cap == 0) ctx->lp_cap = std::max<uint64_t>(1u << 20, 64ull * nq);
     if (ctx->contact_cap == 0) ctx->contact_cap = 1u << 20;
     if (ctx->defer_cap == 0) ctx->defer_cap = 1u << 16;

@@ -58,16 +58,18 @@ hipError_t launch_dfs(int cap_entries, const uint3* tasks, const uint4* tasks4, 
 // Keeps this run's node-pair test count for the next run's budgeted pass.
 hipError_t launch_keep_budget(const RunCounters* ctr, unsigned long long* budget, hipStream_t st);
 // defer: 2 * defer_cap entries, the float64 and the exact rerun lists;
-// hits: cap hit records (query, A slot, B slot, sides), made into cap contacts
+// hits: cap hit records (query, A slot, B slot, sides), made into cap contacts;
+// cus: the device's compute units (the grids are sized from it)
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, uint4* hits, cd_contact* out, uint64_t cap, uint4* defer,
-                               uint64_t defer_cap, RunCounters* ctr, uint32_t grid, uint32_t rerun_grid,
-                               hipStream_t st);
+                               uint64_t defer_cap, RunCounters* ctr, uint32_t cus, hipStream_t st);
 // The reference's narrow phase (plain path's test and contact record) over
 // the batched traversal's leaf pairs, contacts straight to `out`.
 hipError_t launch_narrow_ref(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
-                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t grid,
+                             const float* tris, cd_contact* out, uint64_t cap, RunCounters* ctr, uint32_t cus,
                              hipStream_t st);
+// Queries the narrow launches' occupancy once, outside any graph capture.
+void warm_launch_shapes();
 // lists: 64 + 2n words of device scratch (two rerun lists of n entries)
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
                                uint32_t* lists, uint32_t grid, RunCounters* ctr, hipStream_t st);
