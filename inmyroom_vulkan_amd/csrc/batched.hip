@@ -1692,7 +1692,17 @@ static const NarrowShapes& narrow_shapes() {
                                    resident_per_cu(k_narrow_ref)};
     return s;
 }
-void warm_launch_shapes() { (void)narrow_shapes(); }
+// The same for the split levels (reference or conservative test, one or
+// two levels per launch).
+uint32_t split_blocks_per_cu(bool ref, bool two_levels) {
+    static const uint32_t b[4] = {resident_per_cu(k_split_level<false>), resident_per_cu(k_split_level<true>),
+                                  resident_per_cu(k_split_level2<false>), resident_per_cu(k_split_level2<true>)};
+    return b[(two_levels ? 2 : 0) + (ref ? 1 : 0)];
+}
+void warm_launch_shapes() {
+    (void)narrow_shapes();
+    (void)split_blocks_per_cu(false, false);
+}
 
 hipError_t launch_narrow_exact(const uint4* lps, const unsigned long long* nlp, uint64_t lp_cap, const QueryPre* qp,
                                const float* tris, uint4* hits, cd_contact* out, uint64_t cap, uint4* defer,

@@ -274,7 +274,8 @@ int enqueue_batched(cd_ctx* ctx, uint32_t nq, const cd_tuning& t, bool timed, bo
     // launch k takes frontier k from F[k & 1] (count small[k]); the last level sends every split to the task list
     uint32_t k = 0;
     for (uint32_t l = 0; l < L; ++k) {
-        const uint32_t lgrid = uint32_t(std::min<uint64_t>(grid, (level_bound(l) + 255) / 256));
+        const uint32_t sgrid = uint32_t(ctx->num_cu) * split_blocks_per_cu(ref, t.split_depth == 2 && l + 1 < L);
+        const uint32_t lgrid = uint32_t(std::min<uint64_t>(sgrid, (level_bound(l) + 255) / 256));
         const auto rule = [&](uint32_t level) { return level + 1 == L ? 0xFFFFFFFFu : t.split_tris; };
         if (t.split_depth == 2 && l + 1 < L) {
             HCK(ctx, launch_split_level2(F[k & 1], &small[k], F[(k + 1) & 1], &small[k + 1], cap32, qp, dm, dn,

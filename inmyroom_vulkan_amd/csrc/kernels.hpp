@@ -70,6 +70,8 @@ hipError_t launch_narrow_ref(const uint4* lps, const unsigned long long* nlp, ui
                              hipStream_t st);
 // Queries the narrow launches' occupancy once, outside any graph capture.
 void warm_launch_shapes();
+// Workgroups per CU a split-level launch keeps resident.
+uint32_t split_blocks_per_cu(bool ref, bool two_levels);
 // lists: 64 + 2n words of device scratch (two rerun lists of n entries)
 hipError_t launch_tritri_exact(const float* a9, const float* b9, uint64_t n, int32_t* flags, float* seg6,
                                uint32_t* lists, uint32_t grid, RunCounters* ctr, hipStream_t st);
